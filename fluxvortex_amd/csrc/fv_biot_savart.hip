@@ -1,0 +1,402 @@
+// fv_biot_savart.hip -- line-vortex Biot-Savart sums (collapsed / expanded,
+// ring / chordwise / horseshoe legs) and fused AIC assembly.
+//
+// Reference: warp_kernels.py:25-382 (one thread per (point, vortex) pair, one
+// launch per leg, atomic accumulation).  Here the legs of all vortices are
+// preprocessed once into a segment table and every target's sum runs in one
+// tiled, atomic-free pass.
+#include <cmath>
+#include <limits>
+
+#include "fv_launch.h"
+
+namespace fv {
+
+__global__ void k_fill(double *__restrict__ p, int64_t n, double v)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+int fill(fv_ctx *ctx, double *p, int64_t n, double v)
+{
+    if (n <= 0) return FV_OK;
+    const int b = 256;
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)ceil_div(n, b)), dim3(b), 0, ctx->stream, p, n,
+                       v);
+    FV_LAUNCH_CHECK(ctx);
+    return FV_OK;
+}
+
+// One thread per (vortex, selected leg).  Corner k of ring m is
+// corners[k][3m..3m+2] with corners = (Br, Fr, Fl, Bl); leg k runs k -> k+1.
+__global__ void k_build_segtab(const double *__restrict__ br, const double *__restrict__ fr,
+                               const double *__restrict__ fl, const double *__restrict__ bl,
+                               int64_t m, const double *__restrict__ gamma,
+                               const double *__restrict__ rc0, const double *__restrict__ age,
+                               double nu, int mask, int nl, int unit_strength,
+                               SegRec *__restrict__ out)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m * nl) return;
+    const int64_t v = e / nl;
+    const int slot = (int)(e - v * nl);
+    int leg = 0, seen = -1;
+    for (int k = 0; k < 4; ++k) {
+        if (mask & (1 << k)) {
+            ++seen;
+            if (seen == slot) {
+                leg = k;
+                break;
+            }
+        }
+    }
+    const double *c[4] = {br, fr, fl, bl};
+    const double *s = c[leg] + 3 * v;
+    const double *t = c[(leg + 1) & 3] + 3 * v;
+    SegRec r;
+    r.sx = s[0]; r.sy = s[1]; r.sz = s[2];
+    r.ex = t[0]; r.ey = t[1]; r.ez = t[2];
+    const double r0x = t[0] - s[0], r0y = t[1] - s[1], r0z = t[2] - s[2];
+    const double r0sq = r0x * r0x + r0y * r0y + r0z * r0z;
+    const double r0len = sqrt(r0sq);
+    const double strength = unit_strength ? 1.0 : gamma[v];
+    const double a = (unit_strength || !age) ? 0.0 : age[v];
+    const double rc = rc0[v];
+    // Lamb-Oseen + Squire core growth, warp_kernels.py:88
+    const double rcsq = rc * rc + kFourLamb * (nu + kSquire * fabs(strength)) * a;
+    r.c1 = strength * kInv4Pi;
+    r.c2 = r0sq * rcsq;
+    r.tol = (r0len < 2.22e-16) ? INFINITY : r0len * 1.0e-10;
+    r.pad = 0.0;
+    out[e] = r;
+}
+
+int build_segtab(fv_ctx *ctx, const double *br, const double *fr, const double *fl,
+                 const double *bl, int64_t m, const double *gamma, const double *rc0,
+                 const double *age, double nu, int mask, bool unit_strength, SegRec *out)
+{
+    const int nl = legs_in_mask(mask);
+    const int64_t total = m * nl;
+    if (total <= 0) return FV_OK;
+    const int b = 256;
+    hipLaunchKernelGGL(k_build_segtab, dim3((unsigned)ceil_div(total, b)), dim3(b), 0,
+                       ctx->stream, br, fr, fl, bl, m, gamma, rc0, age, nu, mask, nl,
+                       unit_strength ? 1 : 0, out);
+    FV_LAUNCH_CHECK(ctx);
+    return FV_OK;
+}
+
+PairPlan plan_pairs(fv_ctx *ctx, int64_t n_tgt, int64_t n_src)
+{
+    PairPlan p;
+    p.block = (n_tgt >= 65536) ? 256 : 64;
+    p.gx = ceil_div(n_tgt > 0 ? n_tgt : 1, p.block);
+    const int64_t want = 4LL * num_cus(ctx);
+    int64_t tiles = ceil_div(n_src > 0 ? n_src : 1, p.block);
+    int64_t split = ceil_div(want, p.gx);
+    if (split > tiles) split = tiles;
+    if (split < 1) split = 1;
+    if (split > 65535) split = 65535;
+    p.chunk = ceil_div(tiles, split) * p.block;
+    p.split = (int)ceil_div(n_src > 0 ? n_src : 1, p.chunk);
+    return p;
+}
+
+template <class Op>
+static int run_pairs(fv_ctx *ctx, const typename Op::Args &a, int64_t n_tgt, int64_t n_src,
+                     const int64_t *dyn_tgt, const int64_t *dyn_src, double *out, int pslot,
+                     const char *family)
+{
+    if (n_tgt <= 0) return FV_OK;
+    PairPlan p = plan_pairs(ctx, n_tgt, n_src);
+    double *dst = out;
+    if (p.split > 1) {
+        dst = static_cast<double *>(
+            workspace(ctx, pslot, sizeof(double) * (size_t)p.split * n_tgt * Op::kComp));
+        if (!dst) return FV_E_HIP;
+    }
+    {
+        TimedLaunch tl(ctx, family);
+        if (p.block == 256)
+            hipLaunchKernelGGL((k_pairs<Op, 256>), dim3((unsigned)p.gx, (unsigned)p.split),
+                               dim3(256), 0, ctx->stream, a, n_tgt, n_src, dyn_tgt, dyn_src,
+                               p.chunk, n_tgt, dst);
+        else
+            hipLaunchKernelGGL((k_pairs<Op, 64>), dim3((unsigned)p.gx, (unsigned)p.split),
+                               dim3(64), 0, ctx->stream, a, n_tgt, n_src, dyn_tgt, dyn_src,
+                               p.chunk, n_tgt, dst);
+        FV_LAUNCH_CHECK(ctx);
+        tl.done();
+    }
+    if (p.split > 1) {
+        const int64_t tot = n_tgt * Op::kComp;
+        hipLaunchKernelGGL((k_reduce_partials<Op::kComp>), dim3((unsigned)ceil_div(tot, 256)),
+                           dim3(256), 0, ctx->stream, dst, n_tgt, dyn_tgt, n_tgt, p.split,
+                           out);
+        FV_LAUNCH_CHECK(ctx);
+    }
+    return FV_OK;
+}
+
+int seg_sum(fv_ctx *ctx, const double *pts, int64_t n, const SegRec *segs, int64_t nseg,
+            int precision, double *out, int pslot)
+{
+    if (n <= 0) return FV_OK;
+    if (nseg <= 0) return fill(ctx, out, 3 * n, 0.0);
+    if (precision == FV_PREC_FP64) {
+        SegOp<0>::Args a{pts, segs};
+        return run_pairs<SegOp<0>>(ctx, a, n, nseg, nullptr, nullptr, out, pslot, "seg_pairs");
+    }
+    SegOp<1>::Args a{pts, segs};
+    return run_pairs<SegOp<1>>(ctx, a, n, nseg, nullptr, nullptr, out, pslot, "seg_pairs");
+}
+
+template <int KERNEL, int PREC, int MODE>
+static int part_sum_t(fv_ctx *ctx, const PartSrcArgs &src, const PartTgtArgs &tgt,
+                      int64_t n_tgt, int64_t n_src, const int64_t *dyn_tgt,
+                      const int64_t *dyn_src, double *out, int pslot, const char *family)
+{
+    using Op = PartOp<KERNEL, PREC, MODE>;
+    typename Op::Args a{src, tgt};
+    return run_pairs<Op>(ctx, a, n_tgt, n_src, dyn_tgt, dyn_src, out, pslot, family);
+}
+
+int part_sum(fv_ctx *ctx, int kernel, int precision, int mode, const PartSrcArgs &src,
+             const PartTgtArgs &tgt, int64_t n_tgt, int64_t n_src, const int64_t *dyn_tgt,
+             const int64_t *dyn_src, double *out, int pslot, const char *family)
+{
+    static const char *const kFamilies[4] = {"particle_vel", "particle_jac", "particle_stretch",
+                                             "particle_curl"};
+    if (!family) family = kFamilies[mode & 3];
+    if (n_tgt <= 0) return FV_OK;
+    const int comp = (mode == 1) ? 9 : 3;
+    if (n_src <= 0) return fill(ctx, out, comp * n_tgt, 0.0);
+    PartSrcArgs s = src;
+    s.kernel = kernel;
+#define FV_PS(K, P, M) \
+    return part_sum_t<K, P, M>(ctx, s, tgt, n_tgt, n_src, dyn_tgt, dyn_src, out, pslot, family)
+#define FV_PS_MODE(K, P)        \
+    switch (mode) {             \
+    case 0: FV_PS(K, P, 0);     \
+    case 1: FV_PS(K, P, 1);     \
+    case 2: FV_PS(K, P, 2);     \
+    default: FV_PS(K, P, 3);    \
+    }
+    if (kernel == FV_KERNEL_WINCKELMANS) {
+        if (precision == FV_PREC_FP64) { FV_PS_MODE(1, 0) }
+        else { FV_PS_MODE(1, 1) }
+    } else {
+        if (precision == FV_PREC_FP64) { FV_PS_MODE(0, 0) }
+        else { FV_PS_MODE(0, 1) }
+    }
+#undef FV_PS_MODE
+#undef FV_PS
+}
+
+// Expanded: one thread per (point, vortex); legs of the vortex summed in
+// table order.  Output (n, m, 3), 24 B per pair written once.
+__global__ void k_expanded(const double *__restrict__ pts, int64_t n,
+                           const SegRec *__restrict__ segs, int64_t m, int nl,
+                           double *__restrict__ out)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * m) return;
+    const int64_t i = e / m, v = e - i * m;
+    const double px = pts[3 * i], py = pts[3 * i + 1], pz = pts[3 * i + 2];
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    for (int k = 0; k < nl; ++k) seg_pair_f64(px, py, pz, segs[v * nl + k], ax, ay, az);
+    out[3 * e] = ax;
+    out[3 * e + 1] = ay;
+    out[3 * e + 2] = az;
+}
+
+// AIC: one thread per (collocation point i, ring j); 4 unit-strength legs,
+// projected on the unit normal of panel i.
+__global__ void k_aic(const double *__restrict__ colloc, const double *__restrict__ normals,
+                      int64_t p, const SegRec *__restrict__ segs, double *__restrict__ aic)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= p * p) return;
+    const int64_t i = e / p, j = e - i * p;
+    const double px = colloc[3 * i], py = colloc[3 * i + 1], pz = colloc[3 * i + 2];
+    double ax = 0.0, ay = 0.0, az = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) seg_pair_f64(px, py, pz, segs[4 * j + k], ax, ay, az);
+    aic[e] = ax * normals[3 * i] + ay * normals[3 * i + 1] + az * normals[3 * i + 2];
+}
+
+int aic_assemble(fv_ctx *ctx, const double *colloc, const double *normals, int64_t p,
+                 const SegRec *unit_segs, double *aic)
+{
+    if (p <= 0) return FV_OK;
+    const int b = 256;
+    TimedLaunch tl(ctx, "aic");
+    hipLaunchKernelGGL(k_aic, dim3((unsigned)ceil_div(p * p, b)), dim3(b), 0, ctx->stream,
+                       colloc, normals, p, unit_segs, aic);
+    FV_LAUNCH_CHECK(ctx);
+    tl.done();
+    return FV_OK;
+}
+
+}  // namespace fv
+
+using namespace fv;
+
+static int check_vortex_args(fv_ctx *ctx, const double *points, int64_t n, const double *br,
+                             const double *fr, const double *fl, const double *bl, int64_t m,
+                             const double *strengths, const double *r_c0s, int32_t leg_mask,
+                             int32_t precision, const void *out)
+{
+    if (!ctx) return FV_E_ARG;
+    FV_REQUIRE(ctx, n >= 0 && m >= 0, "negative size (n=%lld, m=%lld)", (long long)n,
+               (long long)m);
+    FV_REQUIRE(ctx, leg_mask > 0 && leg_mask <= 0xF, "leg_mask %d outside 1..15", leg_mask);
+    FV_REQUIRE(ctx, precision == FV_PREC_FP64 || precision == FV_PREC_MIXED,
+               "unknown precision %d", precision);
+    FV_REQUIRE(ctx, n == 0 || (points && out), "null points/out");
+    FV_REQUIRE(ctx, m == 0 || (br && fr && fl && bl && strengths && r_c0s),
+               "null vortex arrays");
+    return FV_OK;
+}
+
+extern "C" {
+
+int fv_vortex_collapsed_dev(fv_ctx *ctx, const double *points, int64_t n, const double *br,
+                            const double *fr, const double *fl, const double *bl, int64_t m,
+                            const double *strengths, const double *r_c0s, const double *ages,
+                            double nu, int32_t leg_mask, int32_t precision, double *out)
+{
+    int rc = check_vortex_args(ctx, points, n, br, fr, fl, bl, m, strengths, r_c0s, leg_mask,
+                               precision, out);
+    if (rc) return rc;
+    if (n == 0) return FV_OK;
+    if (m == 0) return fill(ctx, out, 3 * n, 0.0);
+    const int nl = legs_in_mask(leg_mask);
+    SegRec *tab = static_cast<SegRec *>(workspace(ctx, S_SEGTAB, sizeof(SegRec) * m * nl));
+    if (!tab) return FV_E_HIP;
+    rc = build_segtab(ctx, br, fr, fl, bl, m, strengths, r_c0s, ages, nu, leg_mask, false, tab);
+    if (rc) return rc;
+    return seg_sum(ctx, points, n, tab, m * nl, precision, out, S_PARTIAL);
+}
+
+int fv_vortex_collapsed_host(fv_ctx *ctx, const double *points, int64_t n, const double *br,
+                             const double *fr, const double *fl, const double *bl, int64_t m,
+                             const double *strengths, const double *r_c0s, const double *ages,
+                             double nu, int32_t leg_mask, int32_t precision, double *out)
+{
+    int rc = check_vortex_args(ctx, points, n, br, fr, fl, bl, m, strengths, r_c0s, leg_mask,
+                               precision, out);
+    if (rc) return rc;
+    if (n == 0) return FV_OK;
+    if (m == 0) {
+        for (int64_t i = 0; i < 3 * n; ++i) out[i] = 0.0;
+        return FV_OK;
+    }
+    const double *dp = upload(ctx, S_POINTS, points, 3 * n);
+    const double *dbr = upload(ctx, S_BR, br, 3 * m);
+    const double *dfr = upload(ctx, S_FR, fr, 3 * m);
+    const double *dfl = upload(ctx, S_FL, fl, 3 * m);
+    const double *dbl = upload(ctx, S_BL, bl, 3 * m);
+    const double *dg = upload(ctx, S_GAMMA, strengths, m);
+    const double *drc = upload(ctx, S_RC0, r_c0s, m);
+    const double *dage = ages ? upload(ctx, S_AGE, ages, m) : nullptr;
+    double *dout = static_cast<double *>(workspace(ctx, S_OUT, sizeof(double) * 3 * n));
+    if (!dp || !dbr || !dfr || !dfl || !dbl || !dg || !drc || (ages && !dage) || !dout)
+        return FV_E_HIP;
+    rc = fv_vortex_collapsed_dev(ctx, dp, n, dbr, dfr, dfl, dbl, m, dg, drc, dage, nu, leg_mask,
+                                 precision, dout);
+    if (rc) return rc;
+    return fv_memcpy_d2h(ctx, out, dout, sizeof(double) * 3 * n);
+}
+
+int fv_vortex_expanded_dev(fv_ctx *ctx, const double *points, int64_t n, const double *br,
+                           const double *fr, const double *fl, const double *bl, int64_t m,
+                           const double *strengths, const double *r_c0s, const double *ages,
+                           double nu, int32_t leg_mask, double *out)
+{
+    int rc = check_vortex_args(ctx, points, n, br, fr, fl, bl, m, strengths, r_c0s, leg_mask,
+                               FV_PREC_FP64, out);
+    if (rc) return rc;
+    if (n == 0 || m == 0) return FV_OK;
+    const int nl = legs_in_mask(leg_mask);
+    SegRec *tab = static_cast<SegRec *>(workspace(ctx, S_SEGTAB, sizeof(SegRec) * m * nl));
+    if (!tab) return FV_E_HIP;
+    rc = build_segtab(ctx, br, fr, fl, bl, m, strengths, r_c0s, ages, nu, leg_mask, false, tab);
+    if (rc) return rc;
+    const int b = 256;
+    TimedLaunch tl(ctx, "seg_expanded");
+    hipLaunchKernelGGL(k_expanded, dim3((unsigned)ceil_div(n * m, b)), dim3(b), 0, ctx->stream,
+                       points, n, tab, m, nl, out);
+    FV_LAUNCH_CHECK(ctx);
+    tl.done();
+    return FV_OK;
+}
+
+int fv_vortex_expanded_host(fv_ctx *ctx, const double *points, int64_t n, const double *br,
+                            const double *fr, const double *fl, const double *bl, int64_t m,
+                            const double *strengths, const double *r_c0s, const double *ages,
+                            double nu, int32_t leg_mask, double *out)
+{
+    int rc = check_vortex_args(ctx, points, n, br, fr, fl, bl, m, strengths, r_c0s, leg_mask,
+                               FV_PREC_FP64, out);
+    if (rc) return rc;
+    if (n == 0 || m == 0) return FV_OK;
+    const double *dp = upload(ctx, S_POINTS, points, 3 * n);
+    const double *dbr = upload(ctx, S_BR, br, 3 * m);
+    const double *dfr = upload(ctx, S_FR, fr, 3 * m);
+    const double *dfl = upload(ctx, S_FL, fl, 3 * m);
+    const double *dbl = upload(ctx, S_BL, bl, 3 * m);
+    const double *dg = upload(ctx, S_GAMMA, strengths, m);
+    const double *drc = upload(ctx, S_RC0, r_c0s, m);
+    const double *dage = ages ? upload(ctx, S_AGE, ages, m) : nullptr;
+    double *dout = static_cast<double *>(workspace(ctx, S_OUT, sizeof(double) * 3 * n * m));
+    if (!dp || !dbr || !dfr || !dfl || !dbl || !dg || !drc || (ages && !dage) || !dout)
+        return FV_E_HIP;
+    rc = fv_vortex_expanded_dev(ctx, dp, n, dbr, dfr, dfl, dbl, m, dg, drc, dage, nu, leg_mask,
+                                dout);
+    if (rc) return rc;
+    return fv_memcpy_d2h(ctx, out, dout, sizeof(double) * 3 * n * m);
+}
+
+int fv_aic_dev(fv_ctx *ctx, const double *colloc, const double *normals, int64_t p,
+               const double *br, const double *fr, const double *fl, const double *bl,
+               const double *r_c0s, double nu, double *aic)
+{
+    if (!ctx) return FV_E_ARG;
+    FV_REQUIRE(ctx, p >= 0, "negative panel count");
+    if (p == 0) return FV_OK;
+    FV_REQUIRE(ctx, colloc && normals && br && fr && fl && bl && r_c0s && aic,
+               "null AIC argument");
+    SegRec *tab = static_cast<SegRec *>(workspace(ctx, S_SEGTAB, sizeof(SegRec) * p * 4));
+    if (!tab) return FV_E_HIP;
+    int rc = build_segtab(ctx, br, fr, fl, bl, p, nullptr, r_c0s, nullptr, nu, FV_LEGS_RING,
+                          true, tab);
+    if (rc) return rc;
+    return aic_assemble(ctx, colloc, normals, p, tab, aic);
+}
+
+int fv_aic_host(fv_ctx *ctx, const double *colloc, const double *normals, int64_t p,
+                const double *br, const double *fr, const double *fl, const double *bl,
+                const double *r_c0s, double nu, double *aic)
+{
+    if (!ctx) return FV_E_ARG;
+    FV_REQUIRE(ctx, p >= 0, "negative panel count");
+    if (p == 0) return FV_OK;
+    FV_REQUIRE(ctx, colloc && normals && br && fr && fl && bl && r_c0s && aic,
+               "null AIC argument");
+    const double *dc = upload(ctx, S_POINTS, colloc, 3 * p);
+    const double *dn = upload(ctx, S_NORMALS, normals, 3 * p);
+    const double *dbr = upload(ctx, S_BR, br, 3 * p);
+    const double *dfr = upload(ctx, S_FR, fr, 3 * p);
+    const double *dfl = upload(ctx, S_FL, fl, 3 * p);
+    const double *dbl = upload(ctx, S_BL, bl, 3 * p);
+    const double *drc = upload(ctx, S_RC0, r_c0s, p);
+    double *dout = static_cast<double *>(workspace(ctx, S_OUT, sizeof(double) * p * p));
+    if (!dc || !dn || !dbr || !dfr || !dfl || !dbl || !drc || !dout) return FV_E_HIP;
+    int rc = fv_aic_dev(ctx, dc, dn, p, dbr, dfr, dfl, dbl, drc, nu, dout);
+    if (rc) return rc;
+    return fv_memcpy_d2h(ctx, aic, dout, sizeof(double) * p * p);
+}
+
+}  // extern "C"

@@ -1,0 +1,47 @@
+// fv_vpm.h -- internal layout of the device-resident vortex particle field.
+#pragma once
+
+#include "fv_launch.h"
+
+// SoA particle state (one allocation), RK3 low-storage registers and the
+// pair-sum buffer.  np_dev is the authoritative particle count; the host keeps
+// an exact copy when it changed the count itself, else an upper bound.
+struct fv_vpm {
+    fv_ctx *ctx = nullptr;
+    int64_t cap = 0;
+    double nu = 0.0, rlxf = 0.3;
+    int kernel = FV_KERNEL_GAUSSIAN_ERF;
+    int precision = FV_PREC_FP64;
+    void *block = nullptr;
+    double *x = nullptr, *y = nullptr, *z = nullptr;
+    double *gx = nullptr, *gy = nullptr, *gz = nullptr;
+    double *sig = nullptr, *age = nullptr;
+    double *rkX = nullptr, *rkG = nullptr, *rks = nullptr;   // 3cap, 3cap, cap
+    double *sum = nullptr;                                   // 9cap pair sums
+    double *stage = nullptr;                                 // 9cap host-API staging
+    int64_t *np_dev = nullptr;
+    int32_t *overflow_dev = nullptr;
+    int64_t np_host = 0;
+    bool np_exact = true;
+    int64_t np_upper = 0;
+    double particle_pairs = 0.0;   // pairs evaluated, from the host-side upper bound
+};
+
+namespace fv {
+
+// Enqueues one advect_rk3 + relaxation step (particles.py:85-246).
+int vpm_advect(fv_vpm *f, double dt, const double *uinf, int stretch, int free_wake);
+
+// Enqueues the conversion of nr rings (AoS corners on the device) into
+// particles appended to the field; converted rings get gamma = 0.
+int vpm_convert_rings(fv_vpm *f, int64_t nr, const double *br, const double *fr,
+                      const double *fl, const double *bl, double *gamma, double sigma);
+
+// Enqueues u(targets) from the field into out (n x 3); targets AoS.
+int vpm_induce(fv_vpm *f, const double *targets, int64_t n, int precision, double *out,
+               int pslot);
+
+// Makes np_host exact (synchronises).
+int vpm_sync_count(fv_vpm *f);
+
+}  // namespace fv

@@ -1,0 +1,166 @@
+"""Device-resident vortex particle field (reference: src/fluxvortex/particles.py:15-246).
+
+State lives on the GPU as SoA columns (positions, circulations alpha, core
+sizes, ages) of a fixed capacity; ``advect_rk3`` runs the low-storage RK3
+with rVPM stretching, the velocity/Gamma/sigma clamps, NaN protection and the
+Pedrizzetti relaxation entirely on the device.  The host views ``_pos``,
+``_gamma``, ``_sigma`` and ``_age`` are copies of the first ``np`` particles.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+
+class VortexParticleField:
+    """Structure-of-arrays vortex particle field on the MI355X."""
+
+    def __init__(self, max_particles=50000, nu=0.0, rlxf=0.3, kernel="gaussianerf",
+                 precision="fp64", context=None, _handle=None):
+        self.max_particles = int(max_particles)
+        self.nu = float(nu)
+        self.rlxf = float(rlxf)
+        self.kernel = kernel
+        self.precision = precision
+        self._ctx = context or _lib.default_context()
+        self._owned = _handle is None
+        if _handle is None:
+            h = C.c_void_p()
+            self._ctx.check(self._ctx.lib.fv_vpm_create(
+                self._ctx.handle, self.max_particles, self.nu, self.rlxf,
+                _lib.KERNELS[kernel], _lib.PRECISIONS[precision], C.byref(h)))
+            self._h = h
+        else:
+            self._h = C.c_void_p(_handle)
+
+    def __del__(self):
+        try:
+            if self._owned and self._h and self._h.value:
+                self._ctx.lib.fv_vpm_destroy(self._h)
+                self._h = C.c_void_p()
+        except Exception:
+            pass
+
+    # -- particle management (particles.py:36-69) ----------------------
+    @property
+    def np(self) -> int:
+        n = C.c_int64()
+        self._ctx.check(self._ctx.lib.fv_vpm_count(self._h, C.byref(n)))
+        return n.value
+
+    def add_particle(self, X, Gamma, sigma):
+        self.add_particles_batch(np.reshape(X, (1, 3)), np.reshape(Gamma, (1, 3)),
+                                 np.reshape(sigma, (1,)))
+
+    def add_particles_batch(self, positions, gammas, sigmas):
+        p = _lib.f64(positions, (3,), "positions")
+        g = _lib.f64(gammas, (3,), "gammas")
+        s = _lib.f64(sigmas)
+        n = p.shape[0]
+        if g.shape[0] != n or s.shape != (n,):
+            raise ValueError("positions, gammas and sigmas must describe the same particles")
+        rc = self._ctx.lib.fv_vpm_add_host(self._h, _lib.ptr(p), _lib.ptr(g), _lib.ptr(s), n)
+        if rc == _lib.FV_E_CAP:
+            raise RuntimeError(self._ctx.lib.fv_last_error(self._ctx.handle).decode())
+        self._ctx.check(rc)
+
+    def _read(self):
+        n = self.np
+        pos = np.zeros((n, 3))
+        gam = np.zeros((n, 3))
+        sig = np.zeros(n)
+        age = np.zeros(n)
+        if n:
+            P = _lib.ptr
+            self._ctx.check(self._ctx.lib.fv_vpm_read_host(self._h, P(pos), P(gam), P(sig),
+                                                           P(age)))
+        return pos, gam, sig, age
+
+    @property
+    def positions(self):
+        return self._read()[0]
+
+    @property
+    def gammas(self):
+        return self._read()[1]
+
+    @property
+    def sigmas(self):
+        return self._read()[2]
+
+    _pos = positions
+    _gamma = gammas
+    _sigma = sigmas
+
+    @property
+    def _age(self):
+        return self._read()[3]
+
+    def state_dict(self):
+        """Host snapshot of the particle state (checkpoint)."""
+        pos, gam, sig, age = self._read()
+        return {"pos": pos, "gamma": gam, "sigma": sig, "age": age}
+
+    def load_state_dict(self, st):
+        """Restores a snapshot taken by state_dict()."""
+        pos = _lib.f64(st["pos"], (3,), "pos")
+        gam = _lib.f64(st["gamma"], (3,), "gamma")
+        sig = _lib.f64(st["sigma"])
+        age = _lib.f64(st["age"])
+        P = _lib.ptr
+        self._ctx.check(self._ctx.lib.fv_vpm_write_host(self._h, pos.shape[0], P(pos), P(gam),
+                                                        P(sig), P(age)))
+
+    # -- induced velocity (particles.py:72-82) ---------------------------
+    def induce_velocity_at(self, target_points):
+        t = _lib.f64(target_points, (3,), "target_points")
+        out = np.zeros_like(t)
+        if t.shape[0] == 0:
+            return out
+        self._ctx.check(self._ctx.lib.fv_vpm_induce_host(self._h, _lib.ptr(t), t.shape[0],
+                                                         _lib.ptr(out)))
+        return out
+
+    # -- RK3 + rVPM (particles.py:85-199) + relaxation (:202-246) ----------
+    def advect_rk3(self, dt, U_inf_func, bound_velocity_func=None, stretch=True,
+                   free_wake=True):
+        """One RK3 step followed by Pedrizzetti relaxation, on the device.
+
+        ``U_inf_func`` may be a (3,) vector or a callable returning a uniform
+        (N, 3) field (the only form the reference's callers use,
+        tests/experiment_hybrid_panel_particle.py:140-143).  Non-uniform
+        freestreams and ``bound_velocity_func`` are not supported."""
+        if bound_velocity_func is not None:
+            raise NotImplementedError("bound_velocity_func is not supported on the device path")
+        if callable(U_inf_func):
+            pos = self.positions
+            if pos.shape[0] == 0:
+                return
+            U = np.asarray(U_inf_func(pos), dtype=np.float64).reshape(-1, 3)
+            if not np.all(U == U[0]):
+                raise NotImplementedError("only uniform freestreams are supported")
+            uinf = np.ascontiguousarray(U[0])
+        else:
+            uinf = _lib.f64(U_inf_func).reshape(3)
+        self._ctx.check(self._ctx.lib.fv_vpm_advect_rk3(self._h, float(dt), _lib.ptr(uinf),
+                                                        1 if stretch else 0,
+                                                        1 if free_wake else 0))
+
+    def convert_rings(self, br, fr, fl, bl, strengths, sigma):
+        """Appends the leg particles of rings with |Gamma| >= 1e-15 and returns the
+        zeroed strengths and the number added
+        (tests/experiment_hybrid_panel_particle.py:173-204)."""
+        corners = [_lib.f64(c, (3,), "vertices") for c in (br, fr, fl, bl)]
+        g = np.array(strengths, dtype=np.float64, copy=True)
+        nr = g.shape[0]
+        added = C.c_int64()
+        P = _lib.ptr
+        rc = self._ctx.lib.fv_convert_rings_host(self._h, nr, *(P(c) for c in corners), P(g),
+                                                 float(sigma), C.byref(added))
+        if rc == _lib.FV_E_CAP:
+            raise RuntimeError(self._ctx.lib.fv_last_error(self._ctx.handle).decode())
+        self._ctx.check(rc)
+        return g, added.value

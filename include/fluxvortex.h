@@ -1,0 +1,227 @@
+/*
+ * fluxvortex.h -- C-ABI of libfluxvortex_hip.so, the MI355X (gfx950) hot path
+ * of FLUXVortex's hybrid panel-particle solver.
+ *
+ * Conventions
+ *   - Plain pointers and sizes only.  "_host" entry points take host arrays,
+ *     copy them to device buffers owned by the context, run the kernels on the
+ *     context's stream and copy the result back before returning (blocking,
+ *     like the reference's D2H copy at warp_kernels.py:214).  "_dev" entry
+ *     points take device pointers, enqueue on the context stream and return
+ *     without synchronising.
+ *   - Point/vertex arrays are C-contiguous AoS (n, 3) FP64, as the reference's
+ *     NumPy arrays (warp_kernels.py:251-262).
+ *   - Every function returns 0 on success or a negative FV_E* code; the
+ *     message is available from fv_last_error(ctx).  No C++ exception crosses
+ *     this boundary.
+ *   - A context owns one HIP stream and grow-only device workspaces; use one
+ *     context per host thread.
+ *
+ * Reference paths below are relative to the FLUXVortex repository.
+ */
+#ifndef FLUXVORTEX_H
+#define FLUXVORTEX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FV_OK 0
+#define FV_E_ARG (-1)     /* invalid argument (null pointer, negative size, bad enum) */
+#define FV_E_HIP (-2)     /* HIP runtime error (allocation, copy, launch) */
+#define FV_E_CAP (-3)     /* capacity exceeded (particle field full) */
+#define FV_E_SING (-4)    /* singular matrix in the dense solve */
+
+/* Particle smoothing kernels, kernel.py:59-62. */
+#define FV_KERNEL_GAUSSIAN_ERF 0
+#define FV_KERNEL_WINCKELMANS 1
+
+/* Arithmetic of the pair sums. */
+#define FV_PREC_FP64 0    /* plain FP64 */
+#define FV_PREC_MIXED 1   /* FP32 pairs, compensated FP32 accumulation */
+
+/* Leg masks over ring corners (Br, Fr, Fl, Bl); leg k runs corner k -> k+1.
+ * warp_kernels.py:268-273 (ring), :296-299 (chordwise), :350-354 (horseshoe). */
+#define FV_LEGS_RING 0xF
+#define FV_LEGS_CHORDWISE 0x5
+#define FV_LEGS_HORSESHOE 0x7
+
+typedef struct fv_ctx fv_ctx;
+typedef struct fv_vpm fv_vpm;
+typedef struct fv_hybrid fv_hybrid;
+
+/* ---------------------------------------------------------------- context */
+/* No reference counterpart (Warp keeps an implicit device context,
+ * warp_fsi/config.py:60-63); owns the stream and workspaces. */
+int fv_ctx_create(int device, fv_ctx **out);
+void fv_ctx_destroy(fv_ctx *ctx);
+const char *fv_last_error(const fv_ctx *ctx);
+int fv_ctx_synchronize(fv_ctx *ctx);
+void *fv_ctx_stream(fv_ctx *ctx);
+int fv_device_alloc(fv_ctx *ctx, size_t bytes, void **out);
+int fv_device_free(fv_ctx *ctx, void *ptr);
+int fv_memcpy_h2d(fv_ctx *ctx, void *dst, const void *src, size_t bytes);
+int fv_memcpy_d2h(fv_ctx *ctx, void *dst, const void *src, size_t bytes);
+/* Per-kernel-family timing with HIP events on the context stream.  While
+ * enabled, every launch of a timed family ("seg_pairs", "particle_pairs",
+ * "aic", "dense_solve", ...) is bracketed by events; fv_ctx_kernel_stats
+ * synchronises and returns the launch count and summed duration.  Enabling
+ * clears earlier records. */
+int fv_ctx_set_timing(fv_ctx *ctx, int enabled);
+int fv_ctx_kernel_stats(fv_ctx *ctx, const char *family, int64_t *launches, double *total_ms);
+/* Eight user events on the context stream (which = 0..7). */
+int fv_ctx_event_record(fv_ctx *ctx, int32_t which);
+int fv_ctx_event_elapsed(fv_ctx *ctx, int32_t a, int32_t b, double *ms);
+
+/* ------------------------------------------- line-vortex Biot-Savart (a1-a5) */
+/* Collapsed sum over vortices and the legs in leg_mask:
+ *   out[i] = sum_m sum_legs v(points[i]; leg of vortex m)            (n, 3)
+ * Replaces warp_kernels.py:251-276 (ring), :279-302 (chordwise), :335-357
+ * (horseshoe; pass ages = NULL), i.e. PteraSoftware's
+ * _aerodynamics_functions.collapsed_velocities_from_* .  ages == NULL means
+ * zeros (:265-266).  n == 0 or m == 0 gives zeros (:193-194). */
+int fv_vortex_collapsed_host(fv_ctx *ctx, const double *points, int64_t n, const double *br,
+                             const double *fr, const double *fl, const double *bl, int64_t m,
+                             const double *strengths, const double *r_c0s, const double *ages,
+                             double nu, int32_t leg_mask, int32_t precision, double *out);
+int fv_vortex_collapsed_dev(fv_ctx *ctx, const double *points, int64_t n, const double *br,
+                            const double *fr, const double *fl, const double *bl, int64_t m,
+                            const double *strengths, const double *r_c0s, const double *ages,
+                            double nu, int32_t leg_mask, int32_t precision, double *out);
+/* Expanded: out[i][m] = sum_legs v(points[i]; leg of vortex m)       (n, m, 3)
+ * Replaces warp_kernels.py:305-330 (ring) and :360-382 (horseshoe). */
+int fv_vortex_expanded_host(fv_ctx *ctx, const double *points, int64_t n, const double *br,
+                            const double *fr, const double *fl, const double *bl, int64_t m,
+                            const double *strengths, const double *r_c0s, const double *ages,
+                            double nu, int32_t leg_mask, double *out);
+int fv_vortex_expanded_dev(fv_ctx *ctx, const double *points, int64_t n, const double *br,
+                           const double *fr, const double *fl, const double *bl, int64_t m,
+                           const double *strengths, const double *r_c0s, const double *ages,
+                           double nu, int32_t leg_mask, double *out);
+
+/* ------------------------------------------------------- AIC assembly (a6) */
+/* aic[i*p + j] = (sum over the 4 legs of ring j at unit strength, zero age)
+ *                . normals[i]                                           (p, p)
+ * Replaces PteraSoftware's _calculate_wing_wing_influences (expanded with
+ * unit strengths, then einsum with unit normals;
+ * idea-stage/architecture_analysis.md:12-13); batched analogue
+ * warp_fsi/kernels_uvlm.py:21-53. */
+int fv_aic_host(fv_ctx *ctx, const double *colloc, const double *normals, int64_t p,
+                const double *br, const double *fr, const double *fl, const double *bl,
+                const double *r_c0s, double nu, double *aic);
+int fv_aic_dev(fv_ctx *ctx, const double *colloc, const double *normals, int64_t p,
+               const double *br, const double *fr, const double *fl, const double *bl,
+               const double *r_c0s, double nu, double *aic);
+
+/* ------------------------------------------------- dense solve (a7) */
+/* Solves A_b x_b = rhs_b for `batch` independent row-major n x n systems
+ * (LU with partial pivoting; A overwritten by its factors, rhs by x).
+ * info[b] = 0 or k+1 when column k has an exactly zero pivot.
+ * Replaces np.linalg.solve in standalone_uvlm.py:427 / PteraSoftware's
+ * _calculate_vortex_strengths, and warp_fsi/batched_solver.py:20-63. */
+int fv_dense_solve_host(fv_ctx *ctx, int64_t batch, int64_t n, double *A, double *rhs,
+                        int32_t *info);
+int fv_dense_solve_dev(fv_ctx *ctx, int64_t batch, int64_t n, double *A, double *rhs,
+                       int32_t *info);
+/* Mixed precision: factorise in FP32, refine with FP64 residuals until the
+ * relative correction is below tol or max_iter is reached.  A is not
+ * modified.  iters (nullable) receives the refinement steps used per system. */
+int fv_dense_solve_mixed_dev(fv_ctx *ctx, int64_t batch, int64_t n, const double *A,
+                             double *rhs, int32_t max_iter, double tol, int32_t *info,
+                             int32_t *iters);
+
+/* ------------------------------------------- particle Biot-Savart (a8, a9) */
+/* u(x_t) = -(1/4pi) sum_j g(r/sigma_j)/r^3 (x_t - x_j) x alpha_j        (t, 3)
+ * Replaces kernel.py:69-116 / warp_vpm.py:52-73.  The self pair (r < 1e-12)
+ * is excluded (warp_vpm.py:37-38). */
+int fv_particle_velocity_host(fv_ctx *ctx, const double *targets, int64_t t,
+                              const double *pos, const double *alpha, const double *sigma,
+                              int64_t n, int32_t kernel, int32_t precision, double *out);
+/* J[t][i][j] = dU_i/dx_j                                                  (t, 3, 3)
+ * Replaces kernel.py:119-190 / warp_vpm.py:144-173 (self pair excluded). */
+int fv_particle_jacobian_host(fv_ctx *ctx, const double *targets, int64_t t,
+                              const double *pos, const double *alpha, const double *sigma,
+                              int64_t n, int32_t kernel, int32_t precision, double *J);
+
+/* ------------------------------------------- vortex particle field (a10) */
+/* Device-resident SoA particle field.  Replaces particles.py:15-246
+ * (VortexParticleField(max_particles, nu, rlxf, kernel)). */
+int fv_vpm_create(fv_ctx *ctx, int64_t max_particles, double nu, double rlxf, int32_t kernel,
+                  int32_t precision, fv_vpm **out);
+void fv_vpm_destroy(fv_vpm *f);
+/* particles.py:47-57; FV_E_CAP when np + n > max_particles (:50-51). */
+int fv_vpm_add_host(fv_vpm *f, const double *pos, const double *gamma, const double *sigma,
+                    int64_t n);
+int fv_vpm_count(fv_vpm *f, int64_t *np);
+/* Copies the first np particles (AoS) to host; any pointer may be NULL. */
+int fv_vpm_read_host(fv_vpm *f, double *pos, double *gamma, double *sigma, double *age);
+/* Overwrites the first np particles from host arrays (for checkpoint restore). */
+int fv_vpm_write_host(fv_vpm *f, int64_t np, const double *pos, const double *gamma,
+                      const double *sigma, const double *age);
+/* particles.py:72-82 */
+int fv_vpm_induce_host(fv_vpm *f, const double *targets, int64_t t, double *out);
+/* particles.py:85-199 followed by _relax_pedrizzetti (:202-246), with a
+ * constant freestream uinf[3]. */
+int fv_vpm_advect_rk3(fv_vpm *f, double dt, const double *uinf, int32_t stretch,
+                      int32_t free_wake);
+
+/* ----------------------------------------- panel -> particle conversion (a11) */
+/* Each ring with |gamma| >= 1e-15 yields its leg particles (front, left,
+ * back, right; alpha = gamma (b - a), kept if |alpha|^2 > 1e-30) and its
+ * gamma is set to 0.  Appends to the field.  Replaces
+ * tests/experiment_hybrid_panel_particle.py:145-204. */
+int fv_convert_rings_host(fv_vpm *f, int64_t nr, const double *br, const double *fr,
+                          const double *fl, const double *bl, double *gamma, double sigma,
+                          int64_t *added);
+
+/* -------------------------------------------------- hybrid step (a12-a14) */
+/* One lattice per lifting surface; grids hold the panel corner vertices of
+ * every surface for every time step:
+ *   grids[((step * total_vertices) + surface_offset + (i * (ns+1) + j)) * 3 + c],
+ * i = 0..nc chordwise (leading edge first), j = 0..ns spanwise (left to right). */
+typedef struct fv_hybrid_desc {
+    int32_t n_surfaces;
+    const int32_t *nc;           /* [n_surfaces] chordwise panels */
+    const int32_t *ns;           /* [n_surfaces] spanwise panels */
+    int64_t num_steps;
+    const double *grids;         /* host array, see above */
+    double vinf[3];              /* freestream velocity in the geometry frame */
+    double rho;
+    double dt;
+    double nu;                   /* viscosity in the ring core growth law */
+    double rc0_bound;            /* core radius of bound rings */
+    double rc0_wake;             /* core radius of wake rings */
+    int32_t n_keep;              /* wake rows kept as rings (HybridSolver n_keep) */
+    double sigma;                /* core size of converted particles */
+    int32_t kernel;              /* FV_KERNEL_* for the particle field */
+    int32_t free_wake;           /* particles feel each other */
+    int32_t stretch;             /* rVPM stretching */
+    double rlxf;                 /* Pedrizzetti relaxation factor */
+    double vpm_nu;               /* particle viscosity */
+    double sref;                 /* reference area for CL / CDi */
+    int64_t max_particles;
+    int32_t precision;           /* FV_PREC_* for pair sums and the dense solve */
+} fv_hybrid_desc;
+
+/* Replaces tests/experiment_hybrid_panel_particle.py:87-143 (HybridSolver over
+ * PteraSoftware's UnsteadyRingVortexLatticeMethodSolver step loop). */
+int fv_hybrid_create(fv_ctx *ctx, const fv_hybrid_desc *desc, fv_hybrid **out);
+void fv_hybrid_destroy(fv_hybrid *h);
+/* Advances nsteps time steps on the device (no host synchronisation). */
+int fv_hybrid_step(fv_hybrid *h, int64_t nsteps);
+int fv_hybrid_current_step(fv_hybrid *h, int64_t *step);
+/* Copies CL / CDi histories (num_steps entries; steps not yet run are 0) and
+ * the latest bound circulations (total panels).  Any pointer may be NULL. */
+int fv_hybrid_read(fv_hybrid *h, double *cl, double *cdi, double *gamma);
+/* The particle field driven by the hybrid case (owned by it). */
+fv_vpm *fv_hybrid_particles(fv_hybrid *h);
+/* Pair counts (segment pairs, particle pairs) evaluated since creation. */
+int fv_hybrid_pair_counts(fv_hybrid *h, double *seg_pairs, double *particle_pairs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

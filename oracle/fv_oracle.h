@@ -1,0 +1,32 @@
+/* fv_oracle.h -- declarations of the CPU FP64 oracle (test infrastructure
+ * only; see the header comment of fv_oracle.c for who may call it). */
+#ifndef FV_ORACLE_H
+#define FV_ORACLE_H
+#include <stdint.h>
+
+int orc_collapsed(const double *P, int64_t N, const double *Br, const double *Fr,
+                  const double *Fl, const double *Bl, int64_t M, const double *gamma,
+                  const double *rc0, const double *age, double nu, int leg_mask, double *out,
+                  int nthreads);
+int orc_expanded(const double *P, int64_t N, const double *Br, const double *Fr,
+                 const double *Fl, const double *Bl, int64_t M, const double *gamma,
+                 const double *rc0, const double *age, double nu, int leg_mask, double *out,
+                 int nthreads);
+int orc_aic(const double *colloc, const double *normals, int64_t P, const double *Br,
+            const double *Fr, const double *Fl, const double *Bl, const double *rc0, double nu,
+            double *aic, int nthreads);
+int orc_lu_solve(int64_t n, double *A, double *b);
+int orc_particle_vel(const double *tgt, int64_t T, const double *pos, const double *alpha,
+                     const double *sigma, int64_t N, int kernel, double *out, int nthreads);
+int orc_particle_jac(const double *tgt, int64_t T, const double *pos, const double *alpha,
+                     const double *sigma, int64_t N, int kernel, double *J, int nthreads);
+int orc_vpm_advect_rk3(int64_t n, double *pos, double *gamma, double *sigma, double *age,
+                       double dt, const double *Uinf, int stretch, int free_wake, int kernel,
+                       double nu, int nthreads);
+int orc_vpm_relax(int64_t n, const double *pos, double *gamma, const double *sigma,
+                  double rlxf, int nthreads);
+int64_t orc_convert_rings(int64_t nr, const double *Br, const double *Fr, const double *Fl,
+                          const double *Bl, double *gamma, double sigma, double *pos_out,
+                          double *alpha_out, double *sig_out);
+
+#endif

@@ -1,0 +1,90 @@
+"""Hybrid panel-particle step on the device vs the CPU oracle driver:
+CL/CDi histories within 1e-10, circulations and particle state close."""
+import numpy as np
+import pytest
+
+from helpers import relerr
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def hyb(ctx):
+    from fluxvortex_amd import hybrid
+    return hybrid
+
+
+def compare(hyb, oracle, case, cl_tol=1e-10, part=True):
+    print(f"[hybrid parity] {case.name} steps={case.num_steps}", flush=True)
+    dev = hyb.run_case(case, particles=part)
+    ref = oracle.hybrid_run(case)
+    assert dev["np"] == ref["np"]
+    assert np.max(np.abs(dev["cl"] - ref["cl"])) < cl_tol, np.max(np.abs(dev["cl"] - ref["cl"]))
+    assert np.max(np.abs(dev["cdi"] - ref["cdi"])) < cl_tol
+    assert relerr(dev["gamma"], ref["gamma"]) < 1e-10
+    if part and ref["np"]:
+        assert relerr(dev["pos"], ref["pos"]) < 1e-10
+        assert relerr(dev["gamma"], ref["gamma"]) < 1e-9
+    return dev, ref
+
+
+@pytest.mark.parametrize("kernel,free,stretch", [("winckelmans", True, True),
+                                                 ("gaussianerf", False, False),
+                                                 ("gaussianerf", True, False)])
+def test_heaving_small(hyb, oracle, kernel, free, stretch):
+    case = hyb.heaving_wing_case(nc=4, ns=3, cycles=1.0, steps_per_cycle=24, n_keep=4,
+                                 kernel=kernel, free_wake=free, stretch=stretch)
+    dev, ref = compare(hyb, oracle, case)
+    assert dev["np"] > 0
+
+
+def test_heaving_config_geometry_short(hyb, oracle):
+    """The heaving-wing configuration (10 x 12 panels, N_keep = 10), 40 steps."""
+    case = hyb.heaving_wing_case(num_steps=40)
+    compare(hyb, oracle, case)
+
+
+def test_static_wing_ring_wake(hyb, oracle):
+    """Static NACA 2412 wing, 7 x 18 cosine panels (half wing, 126 rings) and the
+    mirrored 252-panel wing, prescribed ring wake over 6 chords."""
+    for sym in (False, True):
+        case = hyb.static_wing_case(symmetric=sym)
+        dev, ref = compare(hyb, oracle, case, part=False)
+        assert 0.3 < dev["cl"][-1] < 0.8
+
+
+def test_flapping_three_surfaces(hyb, oracle):
+    case = hyb.flapping_case(num_steps=20, n_keep=6)
+    dev, ref = compare(hyb, oracle, case)
+    assert sum(case.nc[i] * case.ns[i] for i in range(3)) == 192
+
+
+def test_n_keep_zero_all_particles(hyb, oracle):
+    case = hyb.heaving_wing_case(nc=3, ns=2, cycles=0.5, steps_per_cycle=20, n_keep=0,
+                                 kernel="gaussianerf", free_wake=True, stretch=False)
+    compare(hyb, oracle, case)
+
+
+def test_mixed_precision_cl(hyb, oracle):
+    case = hyb.heaving_wing_case(num_steps=60)
+    ref = hyb.run_case(case)
+    mixed = hyb.run_case(hyb.with_precision(case, "mixed"))
+    scale = np.max(np.abs(ref["cl"]))
+    assert np.max(np.abs(mixed["cl"] - ref["cl"])) < 1e-3 * scale
+
+
+def test_hybrid_solver_api(hyb):
+    from fluxvortex_amd.geometry import Airplane, Motion, Surface
+    wing = Surface(1.0, 1.0, (0.0, 5.0, 0.0), nc=4, ns=3)
+    ap = Airplane([wing], Motion(heave_amp=0.1, heave_period=0.6283))
+    prob = hyb.UnsteadyProblem(ap, V=10.0, dt=0.6283 / 50, num_steps=30)
+    s = hyb.HybridSolver(prob, n_keep=5, kernel="gaussianerf", free_vpm=True)
+    cl, cdi = s.run(prescribed_wake=True)
+    assert cl.shape == (30,) and np.all(np.isfinite(cl))
+    assert s._vpm.np == (30 - 1 - 5) * 4 * 6
+
+
+def test_capacity_error(hyb):
+    case = hyb.heaving_wing_case(nc=3, ns=2, num_steps=12, n_keep=2, max_particles=20)
+    with pytest.raises(RuntimeError, match="overflow"):
+        hyb.run_case(case)

@@ -1,0 +1,101 @@
+"""GPU parity of the device-resident particle field (RK3 + rVPM + relaxation)
+and of panel -> particle conversion against the CPU oracle."""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from helpers import particle_cloud, relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def make_field(pos, al, sg, kernel, rlxf=0.3, cap=None):
+    from fluxvortex_amd.particles import VortexParticleField
+    f = VortexParticleField(max_particles=cap or len(sg) + 16, nu=0.0, rlxf=rlxf, kernel=kernel)
+    f.add_particles_batch(pos, al, sg)
+    return f
+
+
+@pytest.mark.parametrize("kernel", ["gaussianerf", "winckelmans"])
+@pytest.mark.parametrize("stretch,free", [(False, False), (False, True), (True, True)])
+def test_advect_matches_oracle(oracle, ctx, kernel, stretch, free):
+    pos, al, sg = particle_cloud(700, seed=17, sigma=0.12)
+    al = al * 5
+    f = make_field(pos, al, sg, kernel)
+    U = np.array([10.0, 0.0, 0.3])
+    p, g, s, a = pos, al, sg, np.zeros(len(sg))
+    for _ in range(3):
+        f.advect_rk3(0.01, lambda X: np.broadcast_to(U, X.shape).copy(), stretch=stretch,
+                     free_wake=free)
+        p, g, s, a = oracle.vpm_advect_rk3(p, g, s, a, 0.01, U, stretch, free, kernel)
+    st = f.state_dict()
+    assert relerr(st["pos"], p) < 1e-12
+    assert relerr(st["gamma"], g) < 1e-10
+    assert relerr(st["sigma"], s) < 1e-12
+    assert np.allclose(st["age"], a, rtol=0, atol=1e-15)
+
+
+def test_golden_rk3(ctx):
+    z = np.load(Path(__file__).parent / "golden" / "particles.npz")
+    for k in ("gaussianerf", "winckelmans"):
+        f = make_field(z["pos"], z["alpha"], z["sigma"], k)
+        f.advect_rk3(0.01, np.array([10.0, 0, 0.5]), stretch=True, free_wake=True)
+        st = f.state_dict()
+        assert relerr(st["pos"], z[f"rk3_{k}_pos"]) < 1e-12
+        assert relerr(st["gamma"], z[f"rk3_{k}_gamma"]) < 1e-10
+        assert relerr(f.induce_velocity_at(z["pos"][:5]) * 0, np.zeros((5, 3))) == 0.0
+
+
+def test_field_capacity_and_views(ctx):
+    pos, al, sg = particle_cloud(10, seed=1)
+    f = make_field(pos, al, sg, "gaussianerf", cap=12)
+    assert f.np == 10
+    assert np.array_equal(f.positions, pos) and np.array_equal(f._gamma, al)
+    with pytest.raises(RuntimeError, match="overflow"):
+        f.add_particles_batch(pos[:3], al[:3], sg[:3])
+    f.add_particles_batch(pos[:2], al[:2], sg[:2])
+    assert f.np == 12
+    snap = f.state_dict()
+    f.advect_rk3(0.01, np.ones(3), stretch=False, free_wake=False)
+    f.load_state_dict(snap)
+    assert np.array_equal(f.positions, snap["pos"])
+
+
+def test_induce_matches_kernel(oracle, ctx):
+    pos, al, sg = particle_cloud(1500, seed=3, sigma=0.05)
+    f = make_field(pos, al, sg, "winckelmans")
+    t = np.random.default_rng(0).uniform(-1, 1, (300, 3))
+    assert relerr(f.induce_velocity_at(t), oracle.particle_vel(t, pos, al, sg, "winckelmans")) < 1e-12
+
+
+def test_conversion_matches_oracle(oracle, ctx):
+    rng = np.random.default_rng(4)
+    nr = 1500   # more than one conversion workgroup chunk
+    fl = rng.standard_normal((nr, 3))
+    fr = fl + [0, 0.3, 0]
+    bl = fl + [0.2, 0, 0.01]
+    br = fr + [0.2, 0, -0.01]
+    g = rng.standard_normal(nr)
+    g[::7] = 0.0
+    g[3] = 1e-16
+    pos0, al0, sg0 = particle_cloud(5, seed=2)
+    f = make_field(pos0, al0, sg0, "winckelmans", cap=5 + 4 * nr)
+    gout, added = f.convert_rings(br, fr, fl, bl, g, 0.05)
+    p_ref, a_ref, s_ref, g_ref = oracle.convert_rings(br, fr, fl, bl, g, 0.05)
+    assert added == p_ref.shape[0]
+    assert np.array_equal(gout, g_ref)
+    st = f.state_dict()
+    assert np.array_equal(st["pos"][5:], p_ref)
+    assert np.array_equal(st["gamma"][5:], a_ref)
+    assert np.array_equal(st["sigma"][5:], s_ref)
+    assert np.all(st["age"][5:] == 0.0)
+
+
+def test_conversion_overflow_raises(ctx):
+    pos0, al0, sg0 = particle_cloud(5, seed=2)
+    f = make_field(pos0, al0, sg0, "winckelmans", cap=8)
+    r = np.zeros((1, 3))
+    with pytest.raises(RuntimeError, match="overflow"):
+        f.convert_rings(r + [1, 1, 0], r + [0, 1, 0], r, r + [1, 0, 0], [1.0], 0.05)
+    assert f.np == 5
