@@ -5,10 +5,14 @@ Workload: the heaving-wing hybrid case (rectangular NACA 0012 wing, chord
 1 m, half span 5 m, 10 x 12 panels, V = 10 m/s, h = 0.1 c sin(2 k V t / c) at
 k = 0.5, dt = period/50, N_keep = 10 ring rows, older rows converted to
 Winckelmans particles with sigma = 0.5 V dt, free-wake self-induction +
-rVPM stretching + Pedrizzetti relaxation).  The case is run for 3 cycles, the
-particle wake is then extended periodically downstream to the target count
-(synthetic far wake of the same shape), and K full hybrid steps are timed
-with HIP events on the library's stream.
+rVPM stretching + Pedrizzetti relaxation).  The case is run for 3 cycles with
+the particle wake free but unstretched (with stretching this configuration
+diverges within ~25 steps in the reference algorithm itself), the particle
+wake is then extended periodically downstream to the target count (synthetic
+far wake of the same shape), and K full hybrid steps (stretching on) are timed
+with HIP events on the library's stream.  Accuracy comparisons (mixed,
+mixed + learned closure) start from bitwise-identical states and use the
+unstretched wake, whose history is well conditioned.
 
     python bench.py [--steps K] [--warmup W] [--particles N] [--gpus 1]
 
@@ -41,11 +45,6 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def build_case(hybrid, steps_total, precision, max_particles):
-    return hybrid.heaving_wing_case(num_steps=steps_total, precision=precision,
-                                    max_particles=max_particles)
-
-
 def extend_wake(state, target, period_shift, period_age, block):
     """Appends downstream copies of the oldest `block` particles (one heave
     period of wake) shifted by k * V * T until `target` particles exist."""
@@ -53,15 +52,15 @@ def extend_wake(state, target, period_shift, period_age, block):
     n0 = pos.shape[0]
     if target <= n0:
         return {k: v[:target].copy() for k, v in state.items()}
-    base = slice(0, min(block, n0))
+    base = min(block, n0)
     need = target - n0
     reps = []
     k = 1
     while need > 0:
-        take = min(need, base.stop)
-        p = pos[base][:take].copy()
+        take = min(need, base)
+        p = pos[:take].copy()
         p[:, 0] += k * period_shift
-        reps.append((p, gam[base][:take], sig[base][:take], age[base][:take] + k * period_age))
+        reps.append((p, gam[:take], sig[:take], age[:take] + k * period_age))
         need -= take
         k += 1
     return {
@@ -72,59 +71,87 @@ def extend_wake(state, target, period_shift, period_age, block):
     }
 
 
-def measure(hybrid, lib, particles, steps, warmup, precision, prefill=150, timing_steps=2,
-            seed_state=None):
-    """Returns a dict with the timed-region and per-kernel measurements."""
-    total = prefill + warmup + steps + timing_steps + 2
-    cap = particles + 64 * total
-    case = build_case(hybrid, total, precision, cap)
+PREFILL = 150
+
+
+def prepare(hybrid, particles, steps_after):
+    """A heaving-wing run advanced 3 cycles with the particle wake free but
+    unstretched (the reference HybridSolver's own advect, which stays physical),
+    its particle wake then extended to `particles`.  Deterministic: two calls
+    give bitwise-identical states."""
+    total = PREFILL + steps_after + 2
+    case = hybrid.heaving_wing_case(num_steps=total, precision="fp64",
+                                    max_particles=particles + 64 * total)
     run = hybrid.HybridRun(case)
-    ctx = run._ctx
-    run.step(prefill)
-    st = run.particles.state_dict() if seed_state is None else seed_state
+    run.set_flags(True, False)
+    run.step(PREFILL)
     T = case.meta["period"]
-    per_period = 50 * 2 * 6 * 4          # rows per period x rings per row x 4
-    st = extend_wake(st, particles, case.meta["V"] * T, T, per_period)
+    st = extend_wake(run.particles.state_dict(), particles, case.meta["V"] * T, T,
+                     50 * 2 * 6 * 4)
     run.particles.load_state_dict(st)
-    run.step(warmup)
+    return run, case
+
+
+def timed(run, steps):
+    ctx = run._ctx
     ctx.synchronize()
     seg0, par0 = run.pair_counts()
     np0 = run.particles.np
-    t0 = time.perf_counter()
+    s0 = run.current_step
     ctx.event_record(0)
     run.step(steps)
     ctx.event_record(1)
     ms = ctx.event_elapsed_ms(0, 1)
-    wall = time.perf_counter() - t0
     seg1, par1 = run.pair_counts()
-    np1 = run.particles.np
     cl, cdi, _ = run.read()
-    s0 = prefill + warmup
-    cl_t = cl[s0:s0 + steps].copy()
-    # per-kernel timing pass (events around every launch of each family)
+    return {"ms": ms, "steps": steps, "np": [np0, run.particles.np], "seg_pairs": seg1 - seg0,
+            "particle_pairs": par1 - par0, "cl": cl[s0:s0 + steps].copy()}
+
+
+def kernel_families(run, steps=2):
+    """Per-launch HIP-event timing of each kernel family over `steps` steps."""
+    ctx = run._ctx
     ctx.set_timing(True)
     fam = {}
-    for _ in range(timing_steps):
+    for _ in range(steps):
+        marks = {f: ctx.kernel_stats(f) for f in FAMILIES}
         run.step(1)
         n = run.particles.np
-        for f in ("particle_vel", "particle_stretch", "particle_curl", "particle_induce",
-                  "seg_pairs", "aic", "dense_solve"):
+        for f in FAMILIES:
             k, t = ctx.kernel_stats(f)
             e = fam.setdefault(f, {"launches": 0, "ms": 0.0, "pairs": 0.0})
-            e["launches"] += k - e.get("_k", 0)
-            e["ms"] += t - e.get("_t", 0.0)
-            e["_k"], e["_t"] = k, t
-            if f.startswith("particle_") and f != "particle_induce":
-                e["pairs"] += (k - e.get("_kp", 0)) * float(n) * float(n)
-                e["_kp"] = k
+            e["launches"] += k - marks[f][0]
+            e["ms"] += t - marks[f][1]
+            if f in SELF_FAMILIES:
+                e["pairs"] += (k - marks[f][0]) * float(n) * float(n)
     ctx.set_timing(False)
-    for e in fam.values():
-        for key in ("_k", "_t", "_kp"):
-            e.pop(key, None)
+    return fam
+
+
+FAMILIES = ("particle_vel", "particle_stretch", "particle_curl", "particle_induce", "seg_pairs",
+            "aic", "dense_solve", "closure_near", "closure_far", "closure_train",
+            "closure_check", "mlp_forward", "mlp_train")
+SELF_FAMILIES = ("particle_vel", "particle_stretch", "particle_curl")
+
+
+def run_config(hybrid, particles, steps, warmup, precision, stretch=True, closure=None):
+    run, case = prepare(hybrid, particles, warmup + steps + 3)
+    if precision != "fp64":
+        run.set_precision(precision)
+    net = None
+    if closure is not None:
+        from fluxvortex_amd.mlp import ClosureMLP, closure_params
+        net = ClosureMLP(closure_params(seed=0))
+        run.particles.enable_closure(net, **closure)
+        run.particles.closure_train(rounds=100)
+    run.set_flags(True, stretch)
+    run.step(warmup)
+    out = timed(run, steps)
+    out["families"] = kernel_families(run, 1 if particles > 50000 else 2)
+    if closure is not None:
+        out["closure"] = run.particles.closure_stats()
     run.close()
-    return {"ms_total": ms, "wall_s": wall, "steps": steps, "np_start": np0, "np_end": np1,
-            "seg_pairs": seg1 - seg0, "particle_pairs": par1 - par0, "cl": cl_t,
-            "families": fam, "seed_state": st}
+    return out
 
 
 def cpu_baseline(particles, threads):
@@ -160,8 +187,7 @@ def cpu_baseline(particles, threads):
 
 def roofline(fam, precision):
     """Dominant particle kernel: algorithmic flops per launch / average launch time."""
-    cands = {k: v for k, v in fam.items() if k in ("particle_vel", "particle_stretch",
-                                                    "particle_curl") and v["launches"]}
+    cands = {k: v for k, v in fam.items() if k in SELF_FAMILIES and v["launches"]}
     name = max(cands, key=lambda k: cands[k]["ms"])
     e = cands[name]
     avg_s = e["ms"] / e["launches"] / 1e3
@@ -184,7 +210,7 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--particles", type=int, default=20000)
     ap.add_argument("--big", type=int, default=100000,
-                    help="second particle count measured for FP64 vs mixed (0: skip)")
+                    help="second particle count: FP64 vs mixed vs mixed+learned (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     a = ap.parse_args(argv)
@@ -193,51 +219,63 @@ def main(argv=None):
                           "n_gpus": a.gpus}))
         return 2
     from fluxvortex_amd import _lib, hybrid
-    lib = _lib.load_library()
+    _lib.load_library()
     log(f"[bench] heaving wing, {a.particles} particles, FP64, warmup {a.warmup}, steps {a.steps}")
-    m64 = measure(hybrid, lib, a.particles, a.steps, a.warmup, "fp64")
-    sps = a.steps / (m64["ms_total"] / 1e3)
+    m64 = run_config(hybrid, a.particles, a.steps, a.warmup, "fp64")
+    sec = m64["ms"] / 1e3
     pairs_total = m64["seg_pairs"] + m64["particle_pairs"]
     name, roof = roofline(m64["families"], "fp64")
     out = {
         "metric": "hybrid UVLM-VPM steps/s & G pair-interactions/s: FP64 vs mixed+learned",
-        "value": sps, "unit": "steps/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
-        "ms_per_step": m64["ms_total"] / a.steps, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "heaving-wing hybrid UVLM-VPM step (120 panels, N_keep=10, "
-                               "Winckelmans free wake + stretching + relaxation)",
-                   "particles": [m64["np_start"], m64["np_end"]], "panels": 120,
-                   "precision": "fp64", "parallelism": "single GPU"},
-        "gpairs_per_s": pairs_total / (m64["ms_total"] / 1e3) / 1e9,
-        "segment_gpairs_per_s": m64["seg_pairs"] / (m64["ms_total"] / 1e3) / 1e9,
-        "particle_gpairs_per_s": m64["particle_pairs"] / (m64["ms_total"] / 1e3) / 1e9,
+        "value": a.steps / sec, "unit": "steps/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": m64["ms"] / a.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "heaving-wing hybrid UVLM-VPM step: 120 panels, N_keep=10, "
+                               "Winckelmans sigma=0.5 V dt, free wake + rVPM stretching + "
+                               "relaxation, RK3",
+                   "particles": m64["np"], "panels": 120, "precision": "fp64",
+                   "parallelism": "single GPU"},
+        "gpairs_per_s": pairs_total / sec / 1e9,
+        "segment_gpairs_per_s": m64["seg_pairs"] / sec / 1e9,
+        "particle_gpairs_per_s": m64["particle_pairs"] / sec / 1e9,
         "roofline": roof,
-        "kernel_families_ms": {k: round(v["ms"] / max(v["launches"], 1), 4)
-                               for k, v in m64["families"].items()},
+        "kernel_family_ms_per_launch": {k: round(v["ms"] / v["launches"], 4)
+                                        for k, v in m64["families"].items() if v["launches"]},
     }
     if not a.no_mixed:
         log("[bench] same workload, mixed precision")
-        mmx = measure(hybrid, lib, a.particles, a.steps, a.warmup, "mixed",
-                      seed_state=m64["seed_state"])
-        scl = float(np.max(np.abs(m64["cl"])))
-        out["mixed"] = {"steps_per_s": a.steps / (mmx["ms_total"] / 1e3),
-                        "speedup_vs_fp64": m64["ms_total"] / mmx["ms_total"],
-                        "cl_max_rel_err_vs_fp64": float(np.max(np.abs(mmx["cl"] - m64["cl"])))
-                        / scl}
+        mmx = run_config(hybrid, a.particles, a.steps, a.warmup, "mixed")
+        out["mixed"] = {"steps_per_s": a.steps / (mmx["ms"] / 1e3),
+                        "speedup_vs_fp64": m64["ms"] / mmx["ms"]}
+        log("[bench] accuracy from identical states (unstretched free wake)")
+        r64 = run_config(hybrid, a.particles, a.steps, a.warmup, "fp64", stretch=False)
+        rmx = run_config(hybrid, a.particles, a.steps, a.warmup, "mixed", stretch=False)
+        scl = float(np.max(np.abs(r64["cl"])))
+        out["mixed"]["cl_max_rel_err_vs_fp64"] = float(np.max(np.abs(rmx["cl"] - r64["cl"]))) / scl
     if a.big:
+        big = {}
         log(f"[bench] {a.big} particles, FP64")
-        b64 = measure(hybrid, lib, a.big, 2, 1, "fp64", timing_steps=1)
-        big = {"particles": b64["np_start"], "fp64_steps_per_s": 2 / (b64["ms_total"] / 1e3),
-               "fp64_gpairs_per_s": (b64["seg_pairs"] + b64["particle_pairs"])
-               / (b64["ms_total"] / 1e3) / 1e9}
+        b64 = run_config(hybrid, a.big, 3, 1, "fp64")
+        big["particles"] = b64["np"]
+        big["fp64_steps_per_s"] = 3 / (b64["ms"] / 1e3)
+        big["fp64_gpairs_per_s"] = (b64["seg_pairs"] + b64["particle_pairs"]) / (b64["ms"] / 1e3) / 1e9
         if not a.no_mixed:
             log(f"[bench] {a.big} particles, mixed")
-            bmx = measure(hybrid, lib, a.big, 2, 1, "mixed", timing_steps=1,
-                          seed_state=b64["seed_state"])
-            big["mixed_steps_per_s"] = 2 / (bmx["ms_total"] / 1e3)
-            big["mixed_speedup_vs_fp64"] = b64["ms_total"] / bmx["ms_total"]
-            big["mixed_cl_max_rel_err_vs_fp64"] = float(
-                np.max(np.abs(bmx["cl"] - b64["cl"])) / np.max(np.abs(b64["cl"])))
+            bmx = run_config(hybrid, a.big, 3, 1, "mixed")
+            big["mixed_steps_per_s"] = 3 / (bmx["ms"] / 1e3)
+            log(f"[bench] {a.big} particles, mixed + learned closure")
+            clo = dict(cell=2.0, kappa=4.0, check_every=10, tol=1e-3, train_samples=2048,
+                       train_iters=2, lr=1e-3)
+            bml = run_config(hybrid, a.big, 3, 1, "mixed", closure=clo)
+            big["mixed_learned_steps_per_s"] = 3 / (bml["ms"] / 1e3)
+            big["mixed_learned_speedup_vs_fp64"] = b64["ms"] / bml["ms"]
+            big["closure"] = bml["closure"]
+            big["closure_params"] = clo
+            log("[bench] closure accuracy from identical states (unstretched free wake)")
+            r64 = run_config(hybrid, a.big, 3, 1, "fp64", stretch=False)
+            rml = run_config(hybrid, a.big, 3, 1, "mixed", stretch=False, closure=clo)
+            big["mixed_learned_cl_max_rel_err_vs_fp64"] = float(
+                np.max(np.abs(rml["cl"] - r64["cl"])) / np.max(np.abs(r64["cl"])))
         out["heaving_100k"] = big
     if not a.no_cpu_baseline:
         threads = os.cpu_count() or 1

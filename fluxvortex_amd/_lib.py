@@ -34,6 +34,7 @@ LEGS_CHORDWISE = 0x5
 LEGS_HORSESHOE = 0x7
 
 _dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
 _i32p = C.POINTER(C.c_int32)
 _i64p = C.POINTER(C.c_int64)
 _vp = C.c_void_p
@@ -120,6 +121,26 @@ _SIGNATURES = {
     "fv_hybrid_read": (C.c_int, [_vp, _dp, _dp, _dp]),
     "fv_hybrid_particles": (_vp, [_vp]),
     "fv_hybrid_pair_counts": (C.c_int, [_vp, _dp, _dp]),
+    "fv_hybrid_set_flags": (C.c_int, [_vp, C.c_int32, C.c_int32]),
+    "fv_hybrid_set_precision": (C.c_int, [_vp, C.c_int32]),
+    "fv_vpm_set_closure": (C.c_int, [_vp, _vp, C.c_double, C.c_double, C.c_int32, C.c_double,
+                                     C.c_int32, C.c_int32, C.c_double]),
+    "fv_vpm_clear_closure": (C.c_int, [_vp]),
+    "fv_vpm_closure_stats": (C.c_int, [_vp, _dp, _i64p, _i64p, _i32p, _dp]),
+    "fv_vpm_closure_velocity_host": (C.c_int, [_vp, C.c_int32, _dp, _dp]),
+    "fv_vpm_closure_train": (C.c_int, [_vp, C.c_int32]),
+    "fv_mlp_create": (C.c_int, [_ctx, C.POINTER(_vp)]),
+    "fv_mlp_destroy": (None, [_vp]),
+    "fv_mlp_param_count": (C.c_int, []),
+    "fv_mlp_set_params": (C.c_int, [_vp, _fp]),
+    "fv_mlp_get_params": (C.c_int, [_vp, _fp]),
+    "fv_mlp_forward_dev": (C.c_int, [_vp, _vp, C.c_int64, _vp]),
+    "fv_mlp_gradients_dev": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int64, _vp]),
+    "fv_mlp_adam_dev": (C.c_int, [_vp, _vp, C.c_float]),
+    "fv_mlp_train_step_dev": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int64, C.c_float]),
+    "fv_mlp_forward_host": (C.c_int, [_vp, _fp, C.c_int64, _fp]),
+    "fv_mlp_gradients_host": (C.c_int, [_vp, _fp, _fp, _fp, C.c_int64, _fp]),
+    "fv_mlp_adam_host": (C.c_int, [_vp, _fp, C.c_float]),
 }
 
 _lib = None

@@ -18,6 +18,12 @@ __global__ void k_fill(double *__restrict__ p, int64_t n, double v)
     if (i < n) p[i] = v;
 }
 
+__global__ void k_fill_i32(int32_t *__restrict__ p, int64_t n, int32_t v)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 int fill(fv_ctx *ctx, double *p, int64_t n, double v)
 {
     if (n <= 0) return FV_OK;
@@ -91,8 +97,10 @@ PairPlan plan_pairs(fv_ctx *ctx, int64_t n_tgt, int64_t n_src)
 {
     PairPlan p;
     p.block = (n_tgt >= 65536) ? 256 : 64;
-    p.gx = ceil_div(n_tgt > 0 ? n_tgt : 1, p.block);
-    const int64_t want = 4LL * num_cus(ctx);
+    p.tpt = (n_tgt >= 16384) ? 2 : 1;
+    const int64_t per_block = (int64_t)p.block * p.tpt;
+    p.gx = ceil_div(n_tgt > 0 ? n_tgt : 1, per_block);
+    const int64_t want = 8LL * num_cus(ctx);
     int64_t tiles = ceil_div(n_src > 0 ? n_src : 1, p.block);
     int64_t split = ceil_div(want, p.gx);
     if (split > tiles) split = tiles;
@@ -101,6 +109,16 @@ PairPlan plan_pairs(fv_ctx *ctx, int64_t n_tgt, int64_t n_src)
     p.chunk = ceil_div(tiles, split) * p.block;
     p.split = (int)ceil_div(n_src > 0 ? n_src : 1, p.chunk);
     return p;
+}
+
+template <class Op, int BLOCK, int TPT>
+static void launch_pairs(fv_ctx *ctx, const PairPlan &p, const typename Op::Args &a,
+                         int64_t n_tgt, int64_t n_src, const int64_t *dyn_tgt,
+                         const int64_t *dyn_src, double *dst)
+{
+    hipLaunchKernelGGL((k_pairs<Op, BLOCK, TPT>), dim3((unsigned)p.gx, (unsigned)p.split),
+                       dim3(BLOCK), 0, ctx->stream, a, n_tgt, n_src, dyn_tgt, dyn_src, p.chunk,
+                       n_tgt, dst);
 }
 
 template <class Op>
@@ -118,14 +136,14 @@ static int run_pairs(fv_ctx *ctx, const typename Op::Args &a, int64_t n_tgt, int
     }
     {
         TimedLaunch tl(ctx, family);
-        if (p.block == 256)
-            hipLaunchKernelGGL((k_pairs<Op, 256>), dim3((unsigned)p.gx, (unsigned)p.split),
-                               dim3(256), 0, ctx->stream, a, n_tgt, n_src, dyn_tgt, dyn_src,
-                               p.chunk, n_tgt, dst);
+        if (p.block == 256 && p.tpt == 2)
+            launch_pairs<Op, 256, 2>(ctx, p, a, n_tgt, n_src, dyn_tgt, dyn_src, dst);
+        else if (p.block == 256)
+            launch_pairs<Op, 256, 1>(ctx, p, a, n_tgt, n_src, dyn_tgt, dyn_src, dst);
+        else if (p.tpt == 2)
+            launch_pairs<Op, 64, 2>(ctx, p, a, n_tgt, n_src, dyn_tgt, dyn_src, dst);
         else
-            hipLaunchKernelGGL((k_pairs<Op, 64>), dim3((unsigned)p.gx, (unsigned)p.split),
-                               dim3(64), 0, ctx->stream, a, n_tgt, n_src, dyn_tgt, dyn_src,
-                               p.chunk, n_tgt, dst);
+            launch_pairs<Op, 64, 1>(ctx, p, a, n_tgt, n_src, dyn_tgt, dyn_src, dst);
         FV_LAUNCH_CHECK(ctx);
         tl.done();
     }

@@ -1,0 +1,1006 @@
+// fv_closure.hip -- learned far-field closure of the particle sums.
+//
+// The particles are binned into a uniform grid of cells (edge h).  For a
+// target in cell t and a source cell s:
+//   near (grid-centre distance <= kappa h): exact pair sums, LDS-tiled;
+//   far: the cell's first-order expansion about its centroid (monopole
+//        Sum alpha and dipole M = Sum alpha (x - xbar)^T, in FP64) plus a
+//        learned residual predicted by the closure MLP (BF16 MFMA) from the
+//        target's relative position and the cell's low-order moments,
+//        evaluated once per (target cell, source cell) at the target cell's
+//        centre and carried to each target with the expansion's gradient.
+// The MLP is trained online on exact direct sums the solver generates (random
+// far (target, cell) pairs every step), and the closure is checked against the
+// exact direct sum every `check_every` steps: above `tol` (normwise relative
+// error of the particle-induced velocity) it is rejected and the direct sums
+// are used until the next check passes.
+//
+// The far field of a cell at distance R >> sigma is kernel independent
+// (g(rho) -> 1 for both smoothing kernels), so one expansion serves the
+// velocity, stretching and vorticity passes.
+#include <hipcub/hipcub.hpp>
+
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "fv_vpm.h"
+
+struct fv_mlp;
+extern "C" int fv_mlp_forward_dev(fv_mlp *m, const float *X, int64_t nb, float *Y);
+extern "C" int fv_mlp_train_step_dev(fv_mlp *m, const float *X, const float *T,
+                                     const float *wout, int64_t nb, float lr);
+
+namespace fv {
+
+constexpr int kMomStride = 20;   // n, xbar(3), A(3), M(9), centre(3), pad
+
+struct fv_closure_state {
+    fv_mlp *mlp = nullptr;
+    double h = 2.0, kappa = 4.0, tol = 1e-3, lr = 1e-3;
+    int check_every = 10, train_samples = 2048, train_iters = 2;
+    int64_t max_cells = 1536;
+    // status
+    bool active = false;
+    int64_t steps = 0, checks = 0, rejections = 0;
+    double last_err = -1.0;
+    double near_pairs = 0.0, far_evals = 0.0;
+    uint64_t seed = 0x9E3779B97F4A7C15ull;
+    // geometry of the current build
+    int64_t built_epoch = -1;
+    int64_t far_epoch = -1, far_version = -1;
+    int far_mlp = -1;
+    int64_t mlp_version = 0;
+    int64_t n = 0;                     // particles binned
+    double ox = 0, oy = 0, oz = 0, hh = 2.0;
+    int nx = 1, ny = 1, nz = 1;
+    int64_t ncell = 0, nocc = 0, maxpop = 0;
+    std::vector<int> stencil;          // (di, dj, dk) triples
+    // device buffers
+    void *block = nullptr;
+    size_t block_bytes = 0;
+    int32_t *key = nullptr, *key_s = nullptr, *idx = nullptr, *perm = nullptr;
+    double *sx = nullptr, *sy = nullptr, *sz = nullptr, *sax = nullptr, *say = nullptr,
+           *saz = nullptr, *ssg = nullptr;
+    double *near_out = nullptr;        // 3 per sorted particle
+    void *cells = nullptr;
+    size_t cells_bytes = 0;
+    int32_t *cstart = nullptr, *cend = nullptr, *occ = nullptr, *occ_of = nullptr;
+    double *mom = nullptr;             // kMomStride per occupied cell
+    double *faruj = nullptr;           // 12 per occupied cell (u, J)
+    int32_t *dstencil = nullptr;
+    void *far = nullptr;
+    size_t far_bytes = 0;
+    float *X = nullptr, *Y = nullptr;  // nocc^2 x 16
+    void *tmp = nullptr;
+    size_t tmp_bytes = 0;
+    double *bbox = nullptr;            // 6 + per-block partials
+    float *tX = nullptr, *tT = nullptr, *tw = nullptr;
+    double *vdir = nullptr, *vclo = nullptr, *verr = nullptr;
+};
+
+}  // namespace fv
+
+struct fv_closure : fv::fv_closure_state {};
+
+namespace fv {
+
+// ---------------------------------------------------------------- kernels
+__global__ void k_bbox_partial(const int64_t *__restrict__ np_dev, const double *__restrict__ x,
+                               const double *__restrict__ y, const double *__restrict__ z,
+                               double *__restrict__ part)
+{
+    __shared__ double s[6][256];
+    const int64_t n = *np_dev;
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const double v[3] = {x[i], y[i], z[i]};
+        for (int c = 0; c < 3; ++c) {
+            lo[c] = fmin(lo[c], v[c]);
+            hi[c] = fmax(hi[c], v[c]);
+        }
+    }
+    for (int c = 0; c < 3; ++c) {
+        s[c][threadIdx.x] = lo[c];
+        s[3 + c][threadIdx.x] = hi[c];
+    }
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+            for (int c = 0; c < 3; ++c) {
+                s[c][threadIdx.x] = fmin(s[c][threadIdx.x], s[c][threadIdx.x + w]);
+                s[3 + c][threadIdx.x] = fmax(s[3 + c][threadIdx.x], s[3 + c][threadIdx.x + w]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = s[threadIdx.x][0];
+}
+
+__global__ void k_bbox_final(const double *__restrict__ part, int nb, double *__restrict__ out)
+{
+    if (threadIdx.x >= 6) return;
+    double v = threadIdx.x < 3 ? INFINITY : -INFINITY;
+    for (int b = 0; b < nb; ++b)
+        v = threadIdx.x < 3 ? fmin(v, part[b * 6 + threadIdx.x]) : fmax(v, part[b * 6 + threadIdx.x]);
+    out[threadIdx.x] = v;
+}
+
+__global__ void k_cell_keys(int64_t n, const double *__restrict__ x, const double *__restrict__ y,
+                            const double *__restrict__ z, double ox, double oy, double oz,
+                            double ih, int nx, int ny, int nz, int32_t *__restrict__ key,
+                            int32_t *__restrict__ idx)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int ix = (int)floor((x[i] - ox) * ih), iy = (int)floor((y[i] - oy) * ih),
+        iz = (int)floor((z[i] - oz) * ih);
+    ix = ix < 0 ? 0 : (ix >= nx ? nx - 1 : ix);
+    iy = iy < 0 ? 0 : (iy >= ny ? ny - 1 : iy);
+    iz = iz < 0 ? 0 : (iz >= nz ? nz - 1 : iz);
+    key[i] = (ix * ny + iy) * nz + iz;
+    idx[i] = (int32_t)i;
+}
+
+__global__ void k_cell_ranges(int64_t n, const int32_t *__restrict__ key_s,
+                              int32_t *__restrict__ cstart, int32_t *__restrict__ cend)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t k = key_s[i];
+    if (i == 0 || key_s[i - 1] != k) cstart[k] = (int32_t)i;
+    if (i == n - 1 || key_s[i + 1] != k) cend[k] = (int32_t)(i + 1);
+}
+
+__global__ void k_gather(int64_t n, const int32_t *__restrict__ perm, const double *__restrict__ x,
+                         const double *__restrict__ y, const double *__restrict__ z,
+                         const double *__restrict__ ax, const double *__restrict__ ay,
+                         const double *__restrict__ az, const double *__restrict__ sg,
+                         double *__restrict__ sx, double *__restrict__ sy, double *__restrict__ sz,
+                         double *__restrict__ sax, double *__restrict__ say,
+                         double *__restrict__ saz, double *__restrict__ ssg)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t j = perm[i];
+    sx[i] = x[j]; sy[i] = y[j]; sz[i] = z[j];
+    sax[i] = ax[j]; say[i] = ay[j]; saz[i] = az[j];
+    ssg[i] = sg[j];
+}
+
+// Ordered compaction of the occupied cells (one workgroup).
+__global__ __launch_bounds__(1024) void k_occupied(int64_t ncell, const int32_t *__restrict__ cstart,
+                                                   const int32_t *__restrict__ cend,
+                                                   int32_t *__restrict__ occ,
+                                                   int32_t *__restrict__ occ_of,
+                                                   int64_t *__restrict__ nocc_out)
+{
+    __shared__ int32_t cnt[1024];
+    __shared__ int64_t base;
+    __shared__ int32_t maxpop;
+    if (threadIdx.x == 0) {
+        base = 0;
+        maxpop = 0;
+    }
+    __syncthreads();
+    for (int64_t c0 = 0; c0 < ncell; c0 += 1024) {
+        const int64_t c = c0 + threadIdx.x;
+        const int f = (c < ncell && cend[c] > cstart[c]) ? 1 : 0;
+        if (f) atomicMax(&maxpop, cend[c] - cstart[c]);
+        cnt[threadIdx.x] = f;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int run = 0;
+            for (int k = 0; k < 1024; ++k) {
+                const int v = cnt[k];
+                cnt[k] = run;
+                run += v;
+            }
+        }
+        __syncthreads();
+        if (c < ncell) {
+            if (f) {
+                occ[base + cnt[threadIdx.x]] = (int32_t)c;
+                occ_of[c] = (int32_t)(base + cnt[threadIdx.x]);
+            } else {
+                occ_of[c] = -1;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 1023) base += cnt[1023] + f;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        nocc_out[0] = base;
+        nocc_out[1] = maxpop;
+    }
+}
+
+// Moments of each occupied cell (one wavefront per cell, fixed order).
+__global__ void k_moments(int64_t nocc, const int32_t *__restrict__ occ,
+                          const int32_t *__restrict__ cstart, const int32_t *__restrict__ cend,
+                          const double *__restrict__ sx, const double *__restrict__ sy,
+                          const double *__restrict__ sz, const double *__restrict__ sax,
+                          const double *__restrict__ say, const double *__restrict__ saz,
+                          double ox, double oy, double oz, double h, int ny, int nz,
+                          double *__restrict__ mom)
+{
+    const int64_t o = blockIdx.x;
+    if (o >= nocc) return;
+    const int c = occ[o];
+    const int b = cstart[c], e = cend[c];
+    const int l = threadIdx.x;
+    double px = 0, py = 0, pz = 0;
+    for (int i = b + l; i < e; i += 64) {
+        px += sx[i];
+        py += sy[i];
+        pz += sz[i];
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        px += __shfl_xor(px, off, 64);
+        py += __shfl_xor(py, off, 64);
+        pz += __shfl_xor(pz, off, 64);
+    }
+    const double inv = 1.0 / (double)(e - b);
+    const double xb = px * inv, yb = py * inv, zb = pz * inv;
+    double A[3] = {0, 0, 0}, M[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = b + l; i < e; i += 64) {
+        const double a[3] = {sax[i], say[i], saz[i]};
+        const double d[3] = {sx[i] - xb, sy[i] - yb, sz[i] - zb};
+        for (int r = 0; r < 3; ++r) {
+            A[r] += a[r];
+            for (int k = 0; k < 3; ++k) M[3 * r + k] += a[r] * d[k];
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        for (int r = 0; r < 3; ++r) A[r] += __shfl_xor(A[r], off, 64);
+        for (int r = 0; r < 9; ++r) M[r] += __shfl_xor(M[r], off, 64);
+    }
+    if (l == 0) {
+        double *m = mom + o * kMomStride;
+        m[0] = (double)(e - b);
+        m[1] = xb; m[2] = yb; m[3] = zb;
+        for (int r = 0; r < 3; ++r) m[4 + r] = A[r];
+        for (int r = 0; r < 9; ++r) m[7 + r] = M[r];
+        const int ix = c / (ny * nz), iy = (c / nz) % ny, iz = c % nz;
+        m[16] = ox + (ix + 0.5) * h;
+        m[17] = oy + (iy + 0.5) * h;
+        m[18] = oz + (iz + 0.5) * h;
+        m[19] = 0.0;
+    }
+}
+
+// First-order (monopole + dipole) far field of a cell at point x, FP64:
+//   u = -(1/4pi) d x A / R^3 + (1/4pi) [ e / R^3 - 3 d x (M d) / R^5 ],
+//   e_i = eps_ikl M_lk,  d = x - xbar;  J = du/dx.
+__device__ void far_expansion(const double *m, double x0, double x1, double x2, double *u,
+                              double *J)
+{
+    const double d[3] = {x0 - m[1], x1 - m[2], x2 - m[3]};
+    const double *A = m + 4, *M = m + 7;
+    const double R2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    const double iR2 = 1.0 / R2;
+    const double iR = sqrt(iR2);
+    const double iR3 = iR * iR2, iR5 = iR3 * iR2, iR7 = iR5 * iR2;
+    const double c = 0.07957747154594767;   // 1/(4 pi)
+    const double dA[3] = {d[1] * A[2] - d[2] * A[1], d[2] * A[0] - d[0] * A[2],
+                          d[0] * A[1] - d[1] * A[0]};
+    const double e[3] = {M[7] - M[5], M[2] - M[6], M[3] - M[1]};
+    const double Md[3] = {M[0] * d[0] + M[1] * d[1] + M[2] * d[2],
+                          M[3] * d[0] + M[4] * d[1] + M[5] * d[2],
+                          M[6] * d[0] + M[7] * d[1] + M[8] * d[2]};
+    const double dMd[3] = {d[1] * Md[2] - d[2] * Md[1], d[2] * Md[0] - d[0] * Md[2],
+                           d[0] * Md[1] - d[1] * Md[0]};
+    for (int i = 0; i < 3; ++i) u[i] = c * (-dA[i] * iR3 + e[i] * iR3 - 3.0 * dMd[i] * iR5);
+    if (!J) return;
+    // d/dd_k of each term
+    for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 3; ++k) {
+            // eps_ikl A_l
+            const int i1 = (i + 1) % 3, i2 = (i + 2) % 3;
+            double epsA = 0.0;   // eps_{i k l} A_l
+            if (k == i1) epsA = A[i2];
+            else if (k == i2) epsA = -A[i1];
+            const double t0 = -(epsA * iR3 - 3.0 * dA[i] * d[k] * iR5);
+            const double t1 = -3.0 * e[i] * d[k] * iR5;
+            // d/dd_k (d x Md)_i = eps_ikn Md_n + eps_imn d_m M_nk
+            double epsMd = 0.0;
+            if (k == i1) epsMd = Md[i2];
+            else if (k == i2) epsMd = -Md[i1];
+            const double colk[3] = {M[k], M[3 + k], M[6 + k]};
+            const double dxc = (i == 0) ? d[1] * colk[2] - d[2] * colk[1]
+                               : (i == 1) ? d[2] * colk[0] - d[0] * colk[2]
+                                          : d[0] * colk[1] - d[1] * colk[0];
+            const double t2 = -3.0 * ((epsMd + dxc) * iR5 - 5.0 * dMd[i] * d[k] * iR7);
+            J[3 * i + k] = c * (t0 + t1 + t2);
+        }
+}
+
+// MLP features of a (point, cell) pair and its normalisation scale:
+//   [d/R (3), h/R, A/s (3), M/(s h) (9)],  s = |A| + |M|_F / h.
+// The MLP output (3) is the residual (u_exact - u_expansion) * 4 pi R^2 / s.
+__device__ double far_features(const double *m, double x0, double x1, double x2, double h,
+                               float *f, double *R_out)
+{
+    const double d[3] = {x0 - m[1], x1 - m[2], x2 - m[3]};
+    const double R = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    const double *A = m + 4, *M = m + 7;
+    double mf = 0.0;
+    for (int r = 0; r < 9; ++r) mf += M[r] * M[r];
+    const double s = sqrt(A[0] * A[0] + A[1] * A[1] + A[2] * A[2]) + sqrt(mf) / h + 1e-300;
+    for (int c = 0; c < 3; ++c) f[c] = (float)(d[c] / R);
+    f[3] = (float)(h / R);
+    for (int c = 0; c < 3; ++c) f[4 + c] = (float)(A[c] / s);
+    for (int r = 0; r < 9; ++r) f[7 + r] = (float)(M[r] / (s * h));
+    *R_out = R;
+    return s;
+}
+
+__device__ __forceinline__ bool cells_far(const double *mt, const double *ms, double rc2)
+{
+    const double dx = mt[16] - ms[16], dy = mt[17] - ms[17], dz = mt[18] - ms[18];
+    return dx * dx + dy * dy + dz * dz > rc2;
+}
+
+// MLP inputs for every (target cell, source cell) pair; near pairs get zeros.
+__global__ void k_far_inputs(int64_t nocc, const double *__restrict__ mom, double h, double rc2,
+                             float *__restrict__ X)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nocc * nocc) return;
+    const int64_t t = p / nocc, s = p - t * nocc;
+    const double *mt = mom + t * kMomStride, *ms = mom + s * kMomStride;
+    float f[16];
+    if (cells_far(mt, ms, rc2)) {
+        double R;
+        far_features(ms, mt[16], mt[17], mt[18], h, f, &R);
+    } else {
+        for (int k = 0; k < 16; ++k) f[k] = 0.0f;
+    }
+    for (int k = 0; k < 16; ++k) X[p * 16 + k] = f[k];
+}
+
+// Far field (u, J) at each target cell's centre: one wavefront per target
+// cell, lanes over source cells, fixed-order wave reduction.
+__global__ __launch_bounds__(64) void k_far_reduce(int64_t nocc, const double *__restrict__ mom,
+                                                   double h, double rc2, const float *__restrict__ Y,
+                                                   int use_mlp, double *__restrict__ faruj)
+{
+    const int64_t t = blockIdx.x;
+    if (t >= nocc) return;
+    const double *mt = mom + t * kMomStride;
+    double acc[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int64_t s = threadIdx.x; s < nocc; s += 64) {
+        const double *ms = mom + s * kMomStride;
+        if (!cells_far(mt, ms, rc2)) continue;
+        double u[3], J[9];
+        far_expansion(ms, mt[16], mt[17], mt[18], u, J);
+        if (use_mlp) {
+            float f[16];
+            double R;
+            const double sc = far_features(ms, mt[16], mt[17], mt[18], h, f, &R);
+            const double k = sc * 0.07957747154594767 / (R * R);
+            const float *y = Y + (t * nocc + s) * 16;
+            for (int c = 0; c < 3; ++c) u[c] += k * (double)y[c];
+        }
+        for (int c = 0; c < 3; ++c) acc[c] += u[c];
+        for (int c = 0; c < 9; ++c) acc[3 + c] += J[c];
+    }
+    for (int c = 0; c < 12; ++c)
+        for (int off = 32; off > 0; off >>= 1) acc[c] += __shfl_xor(acc[c], off, 64);
+    if (threadIdx.x == 0)
+        for (int c = 0; c < 12; ++c) faruj[t * 12 + c] = acc[c];
+}
+
+// Near-field pair sums in sorted order: one workgroup per (occupied target
+// cell, 64-target chunk); sources = particles of the stencil cells.
+template <int KERNEL, int PREC, int MODE>
+__global__ __launch_bounds__(64) void k_near(int64_t nocc, const int32_t *__restrict__ occ,
+                                             const int32_t *__restrict__ cstart,
+                                             const int32_t *__restrict__ cend,
+                                             const int32_t *__restrict__ stencil, int nsten,
+                                             int nx, int ny, int nz, const double *__restrict__ sx,
+                                             const double *__restrict__ sy,
+                                             const double *__restrict__ sz,
+                                             const double *__restrict__ sax,
+                                             const double *__restrict__ say,
+                                             const double *__restrict__ saz,
+                                             const double *__restrict__ ssg,
+                                             double *__restrict__ out)
+{
+    using Op = PartOp<KERNEL, PREC, MODE>;
+    __shared__ PartRec tile[64];
+    const int64_t o = blockIdx.x;
+    if (o >= nocc) return;
+    const int c = occ[o];
+    const int tb = cstart[c], te = cend[c];
+    const int t0 = tb + (int)blockIdx.y * 64;
+    if (t0 >= te) return;
+    const int ti = t0 + threadIdx.x;
+    const bool active = ti < te;
+    const int tl = active ? ti : t0;
+    typename Op::Tgt tg;
+    tg.x = sx[tl]; tg.y = sy[tl]; tg.z = sz[tl];
+    tg.ax = sax[tl]; tg.ay = say[tl]; tg.az = saz[tl];
+    typename Op::Acc acc;
+    for (int k = 0; k < 3; ++k) acc.v[k] = 0.0;
+    const int cx = c / (ny * nz), cy = (c / nz) % ny, cz = c % nz;
+    for (int q = 0; q < nsten; ++q) {
+        const int ix = cx + stencil[3 * q], iy = cy + stencil[3 * q + 1], iz = cz + stencil[3 * q + 2];
+        if (ix < 0 || ix >= nx || iy < 0 || iy >= ny || iz < 0 || iz >= nz) continue;
+        const int nc = (ix * ny + iy) * nz + iz;
+        const int sb = cstart[nc], se = cend[nc];
+        for (int base = sb; base < se; base += 64) {
+            const int cnt = (se - base < 64) ? se - base : 64;
+            __syncthreads();
+            if (threadIdx.x < cnt) {
+                const int j = base + threadIdx.x;
+                tile[threadIdx.x] = make_part_rec(KERNEL, sx[j], sy[j], sz[j], sax[j], say[j],
+                                                  saz[j], ssg[j]);
+            }
+            __syncthreads();
+            for (int k = 0; k < cnt; ++k) Op::pair(tg, tile[k], acc);
+        }
+    }
+    if (active)
+        for (int k = 0; k < 3; ++k) out[3 * (int64_t)ti + k] = acc.v[k];
+}
+
+// Adds the far field and scatters to the original particle order.
+//   mode 0: u = near + u_c + J_c (x - centre);  2: S = near + J_c alpha;
+//   3: omega = near + curl(J_c).
+__global__ void k_apply(int64_t n, int mode, const int32_t *__restrict__ key_s,
+                        const int32_t *__restrict__ occ_of, const int32_t *__restrict__ perm,
+                        const double *__restrict__ sx, const double *__restrict__ sy,
+                        const double *__restrict__ sz, const double *__restrict__ sax,
+                        const double *__restrict__ say, const double *__restrict__ saz,
+                        const double *__restrict__ mom, const double *__restrict__ faruj,
+                        const double *__restrict__ near, double *__restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t o = occ_of[key_s[i]];
+    const double *F = faruj + o * 12, *J = F + 3, *m = mom + o * kMomStride;
+    double r[3];
+    if (mode == 0) {
+        const double d[3] = {sx[i] - m[16], sy[i] - m[17], sz[i] - m[18]};
+        for (int a = 0; a < 3; ++a) r[a] = F[a] + J[3 * a] * d[0] + J[3 * a + 1] * d[1] + J[3 * a + 2] * d[2];
+    } else if (mode == 2) {
+        const double al[3] = {sax[i], say[i], saz[i]};
+        for (int a = 0; a < 3; ++a) r[a] = J[3 * a] * al[0] + J[3 * a + 1] * al[1] + J[3 * a + 2] * al[2];
+    } else {
+        r[0] = J[7] - J[5];
+        r[1] = J[2] - J[6];
+        r[2] = J[3] - J[1];
+    }
+    const int32_t j = perm[i];
+    for (int a = 0; a < 3; ++a) out[3 * (int64_t)j + a] = near[3 * i + a] + r[a];
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// Training samples: random (particle, far cell) pairs; exact far-cell velocity
+// by direct sum (one wavefront per sample); residual target normalised.
+__global__ __launch_bounds__(64) void k_train_samples(
+    int64_t n, int64_t nocc, const int32_t *__restrict__ occ, const int32_t *__restrict__ occ_of,
+    const int32_t *__restrict__ key_s, const int32_t *__restrict__ cstart,
+    const int32_t *__restrict__ cend, const double *__restrict__ sx, const double *__restrict__ sy,
+    const double *__restrict__ sz, const double *__restrict__ sax, const double *__restrict__ say,
+    const double *__restrict__ saz, const double *__restrict__ mom, double h, double rc2,
+    uint64_t seed, float *__restrict__ X, float *__restrict__ T)
+{
+    const int64_t k = blockIdx.x;
+    __shared__ int64_t pick[2];
+    if (threadIdx.x == 0) {
+        const int64_t t = (int64_t)(mix64(seed ^ (2 * k + 1)) % (uint64_t)n);
+        const double *mt = mom + (int64_t)occ_of[key_s[t]] * kMomStride;
+        int64_t s = -1;
+        for (int tries = 0; tries < 64 && s < 0; ++tries) {
+            const int64_t cand = (int64_t)(mix64(seed ^ (2 * k + 2) ^ ((uint64_t)tries << 40)) %
+                                           (uint64_t)nocc);
+            if (cells_far(mt, mom + cand * kMomStride, rc2)) s = cand;
+        }
+        pick[0] = t;
+        pick[1] = s;
+    }
+    __syncthreads();
+    const int64_t t = pick[0], s = pick[1];
+    if (s < 0) {
+        // no far cell: a zero sample (all-zero inputs, zero target)
+        if (threadIdx.x < 16) {
+            X[k * 16 + threadIdx.x] = 0.0f;
+            T[k * 16 + threadIdx.x] = 0.0f;
+        }
+        return;
+    }
+    const double x0 = sx[t], x1 = sy[t], x2 = sz[t];
+    const int c = occ[s];
+    double u[3] = {0, 0, 0};
+    for (int j = cstart[c] + threadIdx.x; j < cend[c]; j += 64) {
+        const double d[3] = {x0 - sx[j], x1 - sy[j], x2 - sz[j]};
+        const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        const double ir = 1.0 / sqrt(r2);
+        const double k3 = -0.07957747154594767 * ir * ir * ir;
+        u[0] += k3 * (d[1] * saz[j] - d[2] * say[j]);
+        u[1] += k3 * (d[2] * sax[j] - d[0] * saz[j]);
+        u[2] += k3 * (d[0] * say[j] - d[1] * sax[j]);
+    }
+    for (int a = 0; a < 3; ++a)
+        for (int off = 32; off > 0; off >>= 1) u[a] += __shfl_xor(u[a], off, 64);
+    if (threadIdx.x == 0) {
+        const double *ms = mom + s * kMomStride;
+        float f[16];
+        double R;
+        const double sc = far_features(ms, x0, x1, x2, h, f, &R);
+        double ua[3];
+        far_expansion(ms, x0, x1, x2, ua, nullptr);
+        const double k = R * R / (sc * 0.07957747154594767);
+        for (int a = 0; a < 16; ++a) X[blockIdx.x * 16 + a] = f[a];
+        for (int a = 0; a < 16; ++a) T[blockIdx.x * 16 + a] = 0.0f;
+        for (int a = 0; a < 3; ++a) T[blockIdx.x * 16 + a] = (float)((u[a] - ua[a]) * k);
+    }
+}
+
+// max |a - b| and max |b| over n x 3 (one workgroup, fixed order).
+__global__ __launch_bounds__(256) void k_maxdiff(int64_t n, const double *__restrict__ a,
+                                                 const double *__restrict__ b,
+                                                 double *__restrict__ out)
+{
+    __shared__ double s0[256], s1[256];
+    double m0 = 0.0, m1 = 0.0;
+    for (int64_t i = threadIdx.x; i < 3 * n; i += 256) {
+        m0 = fmax(m0, fabs(a[i] - b[i]));
+        m1 = fmax(m1, fabs(b[i]));
+    }
+    s0[threadIdx.x] = m0;
+    s1[threadIdx.x] = m1;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            s0[threadIdx.x] = fmax(s0[threadIdx.x], s0[threadIdx.x + w]);
+            s1[threadIdx.x] = fmax(s1[threadIdx.x], s1[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = s0[0];
+        out[1] = s1[0];
+    }
+}
+
+// ------------------------------------------------------------- host side
+static int grow(fv_ctx *ctx, void **p, size_t *cap, size_t bytes)
+{
+    if (*cap >= bytes) return FV_OK;
+    if (*p) {
+        FV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        FV_HIP_CHECK(ctx, hipFree(*p));
+        *p = nullptr;
+        *cap = 0;
+    }
+    bytes = bytes + bytes / 4 + 256;
+    FV_HIP_CHECK(ctx, hipMalloc(p, bytes));
+    *cap = bytes;
+    return FV_OK;
+}
+
+static void make_stencil(fv_closure *cl)
+{
+    cl->stencil.clear();
+    const int K = (int)ceil(cl->kappa);
+    const double k2 = cl->kappa * cl->kappa;
+    for (int i = -K; i <= K; ++i)
+        for (int j = -K; j <= K; ++j)
+            for (int k = -K; k <= K; ++k)
+                if ((double)(i * i + j * j + k * k) <= k2 + 1e-9) {
+                    cl->stencil.push_back(i);
+                    cl->stencil.push_back(j);
+                    cl->stencil.push_back(k);
+                }
+}
+
+// Bins the particles and computes the cell moments and the far field.
+static int closure_far(fv_vpm *f, bool use_mlp);
+
+// Bins the particles (cells, moments) for the current state and, when
+// `with_far`, computes the far field.
+static int closure_build(fv_vpm *f, bool use_mlp, bool with_far = true)
+{
+    fv_closure *cl = f->closure;
+    fv_ctx *ctx = f->ctx;
+    if (cl->built_epoch == f->state_epoch) return with_far ? closure_far(f, use_mlp) : FV_OK;
+    int rc = f->np_exact ? FV_OK : vpm_sync_count(f);
+    if (rc) return rc;
+    const int64_t n = f->np_host;
+    cl->n = n;
+    if (n == 0) {
+        cl->nocc = 0;
+        cl->built_epoch = f->state_epoch;
+        return FV_OK;
+    }
+    // bounding box
+    const int nbb = 64;
+    rc = grow(ctx, &cl->tmp, &cl->tmp_bytes, 4096);
+    if (rc) return rc;
+    if (!cl->bbox) {
+        FV_HIP_CHECK(ctx, hipMalloc(&cl->bbox, sizeof(double) * (6 * nbb + 8)));
+    }
+    hipLaunchKernelGGL(k_bbox_partial, dim3(nbb), dim3(256), 0, ctx->stream, f->np_dev, f->x,
+                       f->y, f->z, cl->bbox + 8);
+    FV_LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, ctx->stream, cl->bbox + 8, nbb, cl->bbox);
+    FV_LAUNCH_CHECK(ctx);
+    double bb[6];
+    FV_HIP_CHECK(ctx, hipMemcpyAsync(bb, cl->bbox, sizeof bb, hipMemcpyDeviceToHost, ctx->stream));
+    FV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    for (int c = 0; c < 6; ++c)
+        if (!std::isfinite(bb[c]))
+            return set_error(ctx, FV_E_ARG, "closure: non-finite particle positions");
+    // grid: cell edge h, grown until the grid and the occupied set stay bounded
+    double h = cl->h;
+    for (;;) {
+        cl->nx = (int)floor((bb[3] - bb[0]) / h) + 1;
+        cl->ny = (int)floor((bb[4] - bb[1]) / h) + 1;
+        cl->nz = (int)floor((bb[5] - bb[2]) / h) + 1;
+        const double cells = (double)cl->nx * cl->ny * cl->nz;
+        if (cells <= 262144.0 && (cells <= (double)cl->max_cells || (double)n / 4.0 <= 1.0 ||
+                                  cells <= (double)cl->max_cells * 8.0))
+            break;
+        h *= 1.25;
+    }
+    cl->hh = h;
+    cl->ox = bb[0];
+    cl->oy = bb[1];
+    cl->oz = bb[2];
+    cl->ncell = (int64_t)cl->nx * cl->ny * cl->nz;
+    // particle-side buffers
+    const size_t need = (size_t)n * (4 * sizeof(int32_t) + 10 * sizeof(double)) + 1024;
+    rc = grow(ctx, &cl->block, &cl->block_bytes, need);
+    if (rc) return rc;
+    char *p = static_cast<char *>(cl->block);
+    auto takeD = [&](size_t k) { double *r = reinterpret_cast<double *>(p); p += k * sizeof(double); return r; };
+    cl->sx = takeD(n); cl->sy = takeD(n); cl->sz = takeD(n);
+    cl->sax = takeD(n); cl->say = takeD(n); cl->saz = takeD(n); cl->ssg = takeD(n);
+    cl->near_out = takeD(3 * n);
+    cl->key = reinterpret_cast<int32_t *>(p); p += n * sizeof(int32_t);
+    cl->key_s = reinterpret_cast<int32_t *>(p); p += n * sizeof(int32_t);
+    cl->idx = reinterpret_cast<int32_t *>(p); p += n * sizeof(int32_t);
+    cl->perm = reinterpret_cast<int32_t *>(p); p += n * sizeof(int32_t);
+    const unsigned gb = (unsigned)ceil_div(n, 256);
+    hipLaunchKernelGGL(k_cell_keys, dim3(gb), dim3(256), 0, ctx->stream, n, f->x, f->y, f->z,
+                       cl->ox, cl->oy, cl->oz, 1.0 / h, cl->nx, cl->ny, cl->nz, cl->key, cl->idx);
+    FV_LAUNCH_CHECK(ctx);
+    // stable radix sort of (cell, index)
+    size_t tb = 0;
+    FV_HIP_CHECK(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, tb, cl->key, cl->key_s, cl->idx,
+                                                        cl->perm, (int)n, 0, 32, ctx->stream));
+    rc = grow(ctx, &cl->tmp, &cl->tmp_bytes, tb);
+    if (rc) return rc;
+    FV_HIP_CHECK(ctx, hipcub::DeviceRadixSort::SortPairs(cl->tmp, tb, cl->key, cl->key_s, cl->idx,
+                                                        cl->perm, (int)n, 0, 32, ctx->stream));
+    // per-cell arrays
+    const size_t cneed = (size_t)cl->ncell * 4 * sizeof(int32_t) + 1024;
+    rc = grow(ctx, &cl->cells, &cl->cells_bytes, cneed);
+    if (rc) return rc;
+    cl->cstart = static_cast<int32_t *>(cl->cells);
+    cl->cend = cl->cstart + cl->ncell;
+    cl->occ = cl->cend + cl->ncell;
+    cl->occ_of = cl->occ + cl->ncell;
+    hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)ceil_div(2 * cl->ncell, 256)), dim3(256), 0,
+                       ctx->stream, cl->cstart, 2 * cl->ncell, 0);
+    FV_LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_cell_ranges, dim3(gb), dim3(256), 0, ctx->stream, n, cl->key_s, cl->cstart,
+                       cl->cend);
+    FV_LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_gather, dim3(gb), dim3(256), 0, ctx->stream, n, cl->perm, f->x, f->y, f->z,
+                       f->gx, f->gy, f->gz, f->sig, cl->sx, cl->sy, cl->sz, cl->sax, cl->say,
+                       cl->saz, cl->ssg);
+    FV_LAUNCH_CHECK(ctx);
+    int64_t *nocc_d = reinterpret_cast<int64_t *>(cl->bbox + 6);
+    hipLaunchKernelGGL(k_occupied, dim3(1), dim3(1024), 0, ctx->stream, cl->ncell, cl->cstart,
+                       cl->cend, cl->occ, cl->occ_of, nocc_d);
+    FV_LAUNCH_CHECK(ctx);
+    int64_t occinfo[2] = {0, 0};
+    FV_HIP_CHECK(ctx, hipMemcpyAsync(occinfo, nocc_d, sizeof occinfo, hipMemcpyDeviceToHost,
+                                     ctx->stream));
+    FV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    const int64_t nocc = occinfo[0];
+    cl->nocc = nocc;
+    cl->maxpop = occinfo[1];
+    // moments + far field
+    const size_t fneed = (size_t)nocc * (kMomStride + 12) * sizeof(double) +
+                         (size_t)nocc * nocc * 32 * sizeof(float) + 4096;
+    rc = grow(ctx, &cl->far, &cl->far_bytes, fneed);
+    if (rc) return rc;
+    cl->mom = static_cast<double *>(cl->far);
+    cl->faruj = cl->mom + nocc * kMomStride;
+    cl->X = reinterpret_cast<float *>(cl->faruj + nocc * 12);
+    cl->Y = cl->X + nocc * nocc * 16;
+    hipLaunchKernelGGL(k_moments, dim3((unsigned)nocc), dim3(64), 0, ctx->stream, nocc, cl->occ,
+                       cl->cstart, cl->cend, cl->sx, cl->sy, cl->sz, cl->sax, cl->say, cl->saz,
+                       cl->ox, cl->oy, cl->oz, h, cl->ny, cl->nz, cl->mom);
+    FV_LAUNCH_CHECK(ctx);
+    // stencil of near cells (grid-centre distance <= kappa h), fixed per closure
+    if (!cl->dstencil) {
+        make_stencil(cl);
+        FV_HIP_CHECK(ctx, hipMalloc(&cl->dstencil, sizeof(int32_t) * cl->stencil.size()));
+        FV_HIP_CHECK(ctx, hipMemcpyAsync(cl->dstencil, cl->stencil.data(),
+                                         sizeof(int32_t) * cl->stencil.size(),
+                                         hipMemcpyHostToDevice, ctx->stream));
+    }
+    cl->built_epoch = f->state_epoch;
+    cl->far_epoch = -1;
+    return with_far ? closure_far(f, use_mlp) : FV_OK;
+}
+
+// Far field (u, J) at every occupied cell's centre from all far cells.
+static int closure_far(fv_vpm *f, bool use_mlp)
+{
+    fv_closure *cl = f->closure;
+    fv_ctx *ctx = f->ctx;
+    use_mlp = use_mlp && cl->mlp;
+    if (cl->far_epoch == cl->built_epoch && cl->far_mlp == (int)use_mlp &&
+        (!use_mlp || cl->far_version == cl->mlp_version))
+        return FV_OK;
+    const int64_t nocc = cl->nocc;
+    if (nocc == 0) return FV_OK;
+    const double h = cl->hh;
+    const double rc2 = (cl->kappa * h) * (cl->kappa * h) * (1.0 + 1e-12);
+    TimedLaunch tl(ctx, "closure_far");
+    if (use_mlp) {
+        const int64_t np2 = nocc * nocc;
+        hipLaunchKernelGGL(k_far_inputs, dim3((unsigned)ceil_div(np2, 256)), dim3(256), 0,
+                           ctx->stream, nocc, cl->mom, h, rc2, cl->X);
+        FV_LAUNCH_CHECK(ctx);
+        int rc = fv_mlp_forward_dev(cl->mlp, cl->X, np2, cl->Y);
+        if (rc) return rc;
+        cl->far_evals += (double)np2;
+    }
+    hipLaunchKernelGGL(k_far_reduce, dim3((unsigned)nocc), dim3(64), 0, ctx->stream, nocc, cl->mom,
+                       h, rc2, cl->Y, use_mlp ? 1 : 0, cl->faruj);
+    FV_LAUNCH_CHECK(ctx);
+    tl.done();
+    cl->far_epoch = cl->built_epoch;
+    cl->far_mlp = (int)use_mlp;
+    cl->far_version = cl->mlp_version;
+    return FV_OK;
+}
+
+template <int K, int P, int M>
+static void launch_near(fv_closure *cl, fv_ctx *ctx, unsigned chunks)
+{
+    hipLaunchKernelGGL((k_near<K, P, M>), dim3((unsigned)cl->nocc, chunks), dim3(64), 0,
+                       ctx->stream, cl->nocc, cl->occ, cl->cstart, cl->cend, cl->dstencil,
+                       (int)(cl->stencil.size() / 3), cl->nx, cl->ny, cl->nz, cl->sx, cl->sy,
+                       cl->sz, cl->sax, cl->say, cl->saz, cl->ssg, cl->near_out);
+}
+
+// Closure sum at the particles (original order) for mode 0 (velocity), 2
+// (stretching) or 3 (vorticity); kernel selects the near-field smoothing.
+static int closure_pass(fv_vpm *f, int mode, int kernel, double *out, bool use_mlp)
+{
+    fv_closure *cl = f->closure;
+    fv_ctx *ctx = f->ctx;
+    int rc = closure_build(f, use_mlp);
+    if (rc) return rc;
+    const int64_t n = cl->n;
+    if (n == 0) return FV_OK;
+    // the largest cell population bounds the target chunks per cell
+    const unsigned chunks = (unsigned)ceil_div(cl->maxpop > 0 ? cl->maxpop : 1, 64);
+    {
+        TimedLaunch tl(ctx, "closure_near");
+        const int prec = f->precision;
+#define FV_NEAR(K, P)                                                   \
+    do {                                                                \
+        if (mode == 0) launch_near<K, P, 0>(cl, ctx, chunks);           \
+        else if (mode == 2) launch_near<K, P, 2>(cl, ctx, chunks);      \
+        else launch_near<K, P, 3>(cl, ctx, chunks);                     \
+    } while (0)
+        if (kernel == FV_KERNEL_WINCKELMANS) {
+            if (prec == FV_PREC_FP64) FV_NEAR(1, 0); else FV_NEAR(1, 1);
+        } else {
+            if (prec == FV_PREC_FP64) FV_NEAR(0, 0); else FV_NEAR(0, 1);
+        }
+#undef FV_NEAR
+        FV_LAUNCH_CHECK(ctx);
+        tl.done();
+    }
+    hipLaunchKernelGGL(k_apply, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, ctx->stream, n,
+                       mode, cl->key_s, cl->occ_of, cl->perm, cl->sx, cl->sy, cl->sz, cl->sax,
+                       cl->say, cl->saz, cl->mom, cl->faruj, cl->near_out, out);
+    FV_LAUNCH_CHECK(ctx);
+    return FV_OK;
+}
+
+static int closure_train(fv_vpm *f)
+{
+    fv_closure *cl = f->closure;
+    fv_ctx *ctx = f->ctx;
+    if (!cl->mlp || cl->train_iters <= 0 || cl->n == 0 || cl->nocc < 2) return FV_OK;
+    const int S = cl->train_samples;
+    if (!cl->tX) {
+        FV_HIP_CHECK(ctx, hipMalloc(&cl->tX, sizeof(float) * (32 * (size_t)S + 16)));
+        cl->tT = cl->tX + 16 * (size_t)S;
+        cl->tw = cl->tT + 16 * (size_t)S;
+        const float w[16] = {1, 1, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        FV_HIP_CHECK(ctx, hipMemcpyAsync(cl->tw, w, sizeof w, hipMemcpyHostToDevice, ctx->stream));
+    }
+    const double rc2 = (cl->kappa * cl->hh) * (cl->kappa * cl->hh) * (1.0 + 1e-12);
+    TimedLaunch tl(ctx, "closure_train");
+    cl->seed = cl->seed * 6364136223846793005ull + 1442695040888963407ull;
+    hipLaunchKernelGGL(k_train_samples, dim3((unsigned)S), dim3(64), 0, ctx->stream, cl->n,
+                       cl->nocc, cl->occ, cl->occ_of, cl->key_s, cl->cstart, cl->cend, cl->sx,
+                       cl->sy, cl->sz, cl->sax, cl->say, cl->saz, cl->mom, cl->hh, rc2, cl->seed,
+                       cl->tX, cl->tT);
+    FV_LAUNCH_CHECK(ctx);
+    for (int it = 0; it < cl->train_iters; ++it) {
+        int rc = fv_mlp_train_step_dev(cl->mlp, cl->tX, cl->tT, cl->tw, S, (float)cl->lr);
+        if (rc) return rc;
+    }
+    tl.done();
+    cl->mlp_version++;
+    return FV_OK;
+}
+
+// Normwise relative error of the closure velocity against the exact sum.
+static int closure_check(fv_vpm *f, bool use_mlp, double *err)
+{
+    fv_closure *cl = f->closure;
+    fv_ctx *ctx = f->ctx;
+    int rc = closure_build(f, use_mlp);
+    if (rc) return rc;
+    const int64_t n = cl->n;
+    *err = 0.0;
+    if (n == 0) return FV_OK;
+    if (!cl->vdir) {
+        FV_HIP_CHECK(ctx, hipMalloc(&cl->vdir, sizeof(double) * (6 * (size_t)f->cap + 8)));
+        cl->vclo = cl->vdir + 3 * (size_t)f->cap;
+        cl->verr = cl->vclo + 3 * (size_t)f->cap;
+    }
+    rc = closure_pass(f, 0, f->kernel, cl->vclo, use_mlp);
+    if (rc) return rc;
+    PartSrcArgs s;
+    s.x = f->x; s.y = f->y; s.z = f->z;
+    s.ax = f->gx; s.ay = f->gy; s.az = f->gz;
+    s.sig = f->sig; s.stride = 1; s.kernel = f->kernel;
+    PartTgtArgs t;
+    t.x = f->x; t.y = f->y; t.z = f->z;
+    t.ax = f->gx; t.ay = f->gy; t.az = f->gz; t.stride = 1;
+    rc = part_sum(ctx, f->kernel, f->precision, 0, s, t, n, n, nullptr, nullptr, cl->vdir, S_TMP1,
+                  "closure_check");
+    if (rc) return rc;
+    f->particle_pairs += (double)n * (double)n;
+    hipLaunchKernelGGL(k_maxdiff, dim3(1), dim3(256), 0, ctx->stream, n, cl->vclo, cl->vdir,
+                       cl->verr);
+    FV_LAUNCH_CHECK(ctx);
+    double r[2];
+    FV_HIP_CHECK(ctx, hipMemcpyAsync(r, cl->verr, sizeof r, hipMemcpyDeviceToHost, ctx->stream));
+    FV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    *err = r[1] > 0 ? r[0] / r[1] : r[0];
+    return FV_OK;
+}
+
+int closure_step(fv_vpm *f)
+{
+    fv_closure *cl = f->closure;
+    if (!cl) return FV_OK;
+    int rc = closure_build(f, false, false);
+    if (rc) return rc;
+    rc = closure_train(f);
+    if (rc) return rc;
+    if (cl->steps % cl->check_every == 0) {
+        double err = 0.0;
+        rc = closure_check(f, true, &err);
+        if (rc) return rc;
+        cl->last_err = err;
+        cl->checks++;
+        cl->active = err <= cl->tol;
+        if (!cl->active) cl->rejections++;
+    }
+    cl->steps++;
+    return FV_OK;
+}
+
+bool closure_active(const fv_vpm *f) { return f->closure && f->closure->active; }
+
+int closure_self_sum(fv_vpm *f, int mode, int kernel, double *out)
+{
+    return closure_pass(f, mode, kernel, out, true);
+}
+
+void closure_destroy(fv_closure *cl)
+{
+    if (!cl) return;
+    void *bufs[] = {cl->block, cl->cells, cl->far, cl->tmp, cl->bbox, cl->dstencil, cl->tX,
+                    cl->vdir};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    delete cl;
+}
+
+}  // namespace fv
+
+using namespace fv;
+
+extern "C" {
+
+int fv_vpm_set_closure(fv_vpm *f, fv_mlp *mlp, double cell, double kappa, int32_t check_every,
+                       double tol, int32_t train_samples, int32_t train_iters, double lr)
+{
+    if (!f) return FV_E_ARG;
+    fv_ctx *ctx = f->ctx;
+    FV_REQUIRE(ctx, cell > 0 && kappa >= 1.0 && check_every >= 1 && tol > 0 &&
+                        train_samples >= 16 && train_iters >= 0 && lr >= 0,
+               "invalid closure parameters");
+    closure_destroy(f->closure);
+    fv_closure *cl = new (std::nothrow) fv_closure();
+    if (!cl) return set_error(ctx, FV_E_HIP, "out of host memory");
+    cl->mlp = mlp;
+    cl->h = cell;
+    cl->kappa = kappa;
+    cl->check_every = check_every;
+    cl->tol = tol;
+    cl->train_samples = (train_samples + 15) / 16 * 16;
+    cl->train_iters = train_iters;
+    cl->lr = lr;
+    f->closure = cl;
+    return FV_OK;
+}
+
+int fv_vpm_clear_closure(fv_vpm *f)
+{
+    if (!f) return FV_E_ARG;
+    (void)hipStreamSynchronize(f->ctx->stream);
+    closure_destroy(f->closure);
+    f->closure = nullptr;
+    return FV_OK;
+}
+
+int fv_vpm_closure_stats(fv_vpm *f, double *last_err, int64_t *checks, int64_t *rejections,
+                         int32_t *active, double *far_evals)
+{
+    if (!f || !f->closure) return FV_E_ARG;
+    const fv_closure *cl = f->closure;
+    if (last_err) *last_err = cl->last_err;
+    if (checks) *checks = cl->checks;
+    if (rejections) *rejections = cl->rejections;
+    if (active) *active = cl->active ? 1 : 0;
+    if (far_evals) *far_evals = cl->far_evals;
+    return FV_OK;
+}
+
+// Closure velocity at the particles (host, original order) with or without
+// the learned residual, and its normwise error against the exact sum.
+int fv_vpm_closure_velocity_host(fv_vpm *f, int32_t use_mlp, double *out, double *err)
+{
+    if (!f || !f->closure) return FV_E_ARG;
+    fv_ctx *ctx = f->ctx;
+    double e = 0.0;
+    int rc = closure_check(f, use_mlp != 0, &e);
+    if (rc) return rc;
+    if (err) *err = e;
+    if (out && f->closure->n)
+        FV_HIP_CHECK(ctx, hipMemcpyAsync(out, f->closure->vclo,
+                                         sizeof(double) * 3 * f->closure->n,
+                                         hipMemcpyDeviceToHost, ctx->stream));
+    FV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return FV_OK;
+}
+
+// Runs the closure's online training once on the current particle state.
+int fv_vpm_closure_train(fv_vpm *f, int32_t rounds)
+{
+    if (!f || !f->closure) return FV_E_ARG;
+    int rc = closure_build(f, false, false);
+    for (int r = 0; !rc && r < rounds; ++r) rc = closure_train(f);
+    if (!rc) rc = fv_ctx_synchronize(f->ctx);
+    return rc;
+}
+
+}  // extern "C"

@@ -716,6 +716,24 @@ int fv_hybrid_read(fv_hybrid *h, double *cl, double *cdi, double *gamma)
 
 fv_vpm *fv_hybrid_particles(fv_hybrid *h) { return h ? h->vpm : nullptr; }
 
+int fv_hybrid_set_flags(fv_hybrid *h, int32_t free_wake, int32_t stretch)
+{
+    if (!h) return FV_E_ARG;
+    h->d.free_wake = free_wake ? 1 : 0;
+    h->d.stretch = stretch ? 1 : 0;
+    return FV_OK;
+}
+
+int fv_hybrid_set_precision(fv_hybrid *h, int32_t precision)
+{
+    if (!h) return FV_E_ARG;
+    FV_REQUIRE(h->ctx, precision == FV_PREC_FP64 || precision == FV_PREC_MIXED,
+               "unknown precision %d", precision);
+    h->d.precision = precision;
+    h->vpm->precision = precision;
+    return FV_OK;
+}
+
 int fv_hybrid_pair_counts(fv_hybrid *h, double *seg_pairs, double *particle_pairs)
 {
     if (!h) return FV_E_ARG;

@@ -9,6 +9,7 @@ namespace fv {
 
 struct PairPlan {
     int block;      // 64 or 256 lanes per workgroup
+    int tpt;        // targets per lane (1 or 2)
     int64_t gx;     // target blocks
     int split;      // source chunks (blockIdx.y)
     int64_t chunk;  // sources per chunk (multiple of block)
@@ -18,6 +19,7 @@ PairPlan plan_pairs(fv_ctx *ctx, int64_t n_tgt, int64_t n_src);
 
 // Fills n doubles with v (a kernel, so it is safe under graph capture).
 int fill(fv_ctx *ctx, double *p, int64_t n, double v);
+__global__ void k_fill_i32(int32_t *__restrict__ p, int64_t n, int32_t v);
 
 // Builds the segment table of m vortices x legs(mask), vortex-major
 // (segment m*nl + k is the k-th selected leg of vortex m).  unit_strength

@@ -1,9 +1,9 @@
 // fv_linalg.hip -- batched dense circulation solves.
 //
 // One workgroup per system (the reference's batched solver runs one thread
-// per system, warp_fsi/batched_solver.py:20-63).  LU with partial pivoting
-// (first maximum wins, as LAPACK's idamax), the matrix held in LDS when it
-// fits and in global memory (L2-resident at these sizes) otherwise.
+// per system, warp_fsi/batched_solver.py:20-63).  Blocked LU with partial
+// pivoting (first maximum wins, as LAPACK's idamax), the matrix held in LDS
+// when it fits and in global memory (L2-resident at these sizes) otherwise.
 //   FP64:  factor + solve in FP64 (np.linalg.solve, standalone_uvlm.py:427).
 //   Mixed: factor in FP32, then iterative refinement with FP64 residuals
 //          r = b - A x against the FP64 matrix.
@@ -15,131 +15,150 @@ namespace fv {
 
 constexpr int kLuBlock = 256;
 
+// ---------------------------------------------------------------- blocked
+// Right-looking blocked LU (panel width kNb) executed by one workgroup:
+//   1. wave 0 factors the n-k0 x kb panel with partial pivoting, swapping
+//      whole rows and eliminating the right-hand side as it goes (wave-
+//      synchronous: no workgroup barrier inside the panel);
+//   2. all waves form U12 = L11^{-1} A12 (one column per lane);
+//   3. all waves update A22 -= L21 U12.
+// Three workgroup barriers per panel instead of three per column.
+constexpr int kNb = 16;
+
 template <typename T>
-__device__ void block_argmax(const T *a, int n, int k, int *piv_out, T *best_out)
+__device__ __forceinline__ void wave_sync()
 {
-    __shared__ double sval[kLuBlock / 64];
-    __shared__ int sidx[kLuBlock / 64];
-    double best = -1.0;
-    int bi = n;
-    for (int r = k + threadIdx.x; r < n; r += kLuBlock) {
-        const double v = fabs((double)a[(int64_t)r * n + k]);
-        if (v > best) {
-            best = v;
-            bi = r;
-        }
-    }
-    // wave reduction (first maximum wins)
-    for (int off = 32; off > 0; off >>= 1) {
-        const double ov = __shfl_xor(best, off, 64);
-        const int oi = __shfl_xor(bi, off, 64);
-        if (ov > best || (ov == best && oi < bi)) {
-            best = ov;
-            bi = oi;
-        }
-    }
-    const int w = threadIdx.x / 64;
-    if ((threadIdx.x & 63) == 0) {
-        sval[w] = best;
-        sidx[w] = bi;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double b = sval[0];
-        int i = sidx[0];
-        for (int q = 1; q < kLuBlock / 64; ++q) {
-            if (sval[q] > b || (sval[q] == b && sidx[q] < i)) {
-                b = sval[q];
-                i = sidx[q];
-            }
-        }
-        *piv_out = i;
-        *best_out = (T)b;
-    }
-    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// In-place LU with partial pivoting of the n x n row-major matrix a, applied
-// to the right-hand side b as it goes.  Returns info (0 or k+1).
 template <typename T, typename TB>
-__device__ int lu_factor_apply(T *a, int n, int *perm, TB *b)
+__device__ int lu_blocked(T *a, int n, int *perm, TB *b)
 {
-    __shared__ int piv_s;
-    __shared__ T best_s;
-    int info = 0;
-    for (int k = 0; k < n; ++k) {
-        block_argmax<T>(a, n, k, &piv_s, &best_s);
-        const int piv = piv_s;
-        if (best_s == (T)0) {
-            if (info == 0) info = k + 1;
-            if (threadIdx.x == 0 && perm) perm[k] = k;
-            __syncthreads();
-            continue;
-        }
-        if (threadIdx.x == 0 && perm) perm[k] = piv;
-        if (piv != k) {
-            for (int c = threadIdx.x; c < n; c += kLuBlock) {
-                const T t = a[(int64_t)k * n + c];
-                a[(int64_t)k * n + c] = a[(int64_t)piv * n + c];
-                a[(int64_t)piv * n + c] = t;
-            }
-            if (b && threadIdx.x == 0) {
-                const TB t = b[k];
-                b[k] = b[piv];
-                b[piv] = t;
+    __shared__ int info_s;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    if (threadIdx.x == 0) info_s = 0;
+    __syncthreads();
+    for (int k0 = 0; k0 < n; k0 += kNb) {
+        const int kb = (n - k0 < kNb) ? n - k0 : kNb;
+        if (wave == 0) {
+            for (int j = k0; j < k0 + kb; ++j) {
+                double best = -1.0;
+                int bi = n;
+                for (int r = j + lane; r < n; r += 64) {
+                    const double v = fabs((double)a[(int64_t)r * n + j]);
+                    if (v > best) {
+                        best = v;
+                        bi = r;
+                    }
+                }
+                for (int off = 32; off > 0; off >>= 1) {
+                    const double ov = __shfl_xor(best, off, 64);
+                    const int oi = __shfl_xor(bi, off, 64);
+                    if (ov > best || (ov == best && oi < bi)) {
+                        best = ov;
+                        bi = oi;
+                    }
+                }
+                const int piv = __builtin_amdgcn_readfirstlane(bi);
+                if (perm && lane == 0) perm[j] = piv;
+                if (best == 0.0) {
+                    if (lane == 0 && info_s == 0) info_s = j + 1;
+                    if (perm && lane == 0) perm[j] = j;
+                    wave_sync<T>();
+                    continue;
+                }
+                if (piv != j) {
+                    for (int c = lane; c < n; c += 64) {
+                        const T t = a[(int64_t)j * n + c];
+                        a[(int64_t)j * n + c] = a[(int64_t)piv * n + c];
+                        a[(int64_t)piv * n + c] = t;
+                    }
+                    if (b && lane == 0) {
+                        const TB t = b[j];
+                        b[j] = b[piv];
+                        b[piv] = t;
+                    }
+                }
+                wave_sync<T>();
+                const T inv = (T)1 / a[(int64_t)j * n + j];
+                const TB bj = b ? b[j] : (TB)0;
+                for (int r = j + 1 + lane; r < n; r += 64) {
+                    T *row = a + (int64_t)r * n;
+                    const T l = row[j] * inv;
+                    row[j] = l;
+                    for (int c = j + 1; c < k0 + kb; ++c) row[c] -= l * a[(int64_t)j * n + c];
+                    if (b) b[r] -= (TB)l * bj;
+                }
+                wave_sync<T>();
             }
         }
         __syncthreads();
-        const T inv = (T)1 / a[(int64_t)k * n + k];
-        for (int r = k + 1 + threadIdx.x; r < n; r += kLuBlock) a[(int64_t)r * n + k] *= inv;
+        // U12 = L11^{-1} A12: forward substitution per column
+        for (int c = k0 + kb + threadIdx.x; c < n; c += blockDim.x)
+            for (int i = 1; i < kb; ++i) {
+                T s = a[(int64_t)(k0 + i) * n + c];
+                for (int m = 0; m < i; ++m)
+                    s -= a[(int64_t)(k0 + i) * n + k0 + m] * a[(int64_t)(k0 + m) * n + c];
+                a[(int64_t)(k0 + i) * n + c] = s;
+            }
         __syncthreads();
-        const int m = n - k - 1;
-        for (int64_t e = threadIdx.x; e < (int64_t)m * m; e += kLuBlock) {
-            const int r = k + 1 + (int)(e / m), c = k + 1 + (int)(e % m);
-            a[(int64_t)r * n + c] -= a[(int64_t)r * n + k] * a[(int64_t)k * n + c];
+        // A22 -= L21 U12
+        const int m = n - k0 - kb;
+        if (m > 0) {
+            for (int e = threadIdx.x; e < m * m; e += blockDim.x) {
+                const int r = k0 + kb + e / m, c = k0 + kb + e % m;
+                T s = a[(int64_t)r * n + c];
+                const T *lr = a + (int64_t)r * n + k0;
+                for (int q = 0; q < kb; ++q) s -= lr[q] * a[(int64_t)(k0 + q) * n + c];
+                a[(int64_t)r * n + c] = s;
+            }
         }
-        if (b)
-            for (int r = k + 1 + threadIdx.x; r < n; r += kLuBlock)
-                b[r] -= (TB)a[(int64_t)r * n + k] * b[k];
         __syncthreads();
     }
-    return info;
+    return info_s;
 }
 
-// Back substitution U x = b (b overwritten), column-oriented.
+// x = U^{-1} x in place by wave 0 (row dot products, shuffle reductions).
 template <typename T, typename TB>
-__device__ void lu_back(const T *a, int n, TB *b)
+__device__ void wave_back_subst(const T *a, int n, TB *x)
 {
+    if ((threadIdx.x >> 6) != 0) return;
+    const int lane = threadIdx.x & 63;
     for (int k = n - 1; k >= 0; --k) {
-        __syncthreads();
-        if (threadIdx.x == 0) b[k] = b[k] / (TB)a[(int64_t)k * n + k];
-        __syncthreads();
-        const TB xk = b[k];
-        for (int r = threadIdx.x; r < k; r += kLuBlock) b[r] -= (TB)a[(int64_t)r * n + k] * xk;
+        TB s = 0;
+        for (int c = k + 1 + lane; c < n; c += 64) s += (TB)a[(int64_t)k * n + c] * x[c];
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+        if (lane == 0) x[k] = (x[k] - s) / (TB)a[(int64_t)k * n + k];
+        wave_sync<T>();
     }
-    __syncthreads();
 }
 
-// Forward substitution with unit-lower L and the recorded row swaps.
+// Forward substitution L y = P x with the recorded pivots, by wave 0.
 template <typename T, typename TB>
-__device__ void lu_forward(const T *a, const int *perm, int n, TB *b)
+__device__ void wave_fwd_subst(const T *a, const int *perm, int n, TB *x)
 {
-    for (int k = 0; k < n; ++k) {
-        __syncthreads();
-        if (threadIdx.x == 0) {
+    if ((threadIdx.x >> 6) != 0) return;
+    const int lane = threadIdx.x & 63;
+    if (lane == 0)
+        for (int k = 0; k < n; ++k) {
             const int p = perm[k];
             if (p != k) {
-                const TB t = b[k];
-                b[k] = b[p];
-                b[p] = t;
+                const TB t = x[k];
+                x[k] = x[p];
+                x[p] = t;
             }
         }
-        __syncthreads();
-        const TB bk = b[k];
-        for (int r = k + 1 + threadIdx.x; r < n; r += kLuBlock)
-            b[r] -= (TB)a[(int64_t)r * n + k] * bk;
+    wave_sync<T>();
+    for (int k = 1; k < n; ++k) {
+        TB s = 0;
+        for (int c = lane; c < k; c += 64) s += (TB)a[(int64_t)k * n + c] * x[c];
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+        if (lane == 0) x[k] -= s;
+        wave_sync<T>();
     }
-    __syncthreads();
 }
 
 __global__ __launch_bounds__(kLuBlock) void k_lu_solve_f64(int n, double *__restrict__ A,
@@ -156,8 +175,9 @@ __global__ __launch_bounds__(kLuBlock) void k_lu_solve_f64(int n, double *__rest
         for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += kLuBlock) w[e] = a[e];
         __syncthreads();
     }
-    const int inf = lu_factor_apply<double, double>(w, n, nullptr, b);
-    if (inf == 0) lu_back<double, double>(w, n, b);
+    const int inf = lu_blocked<double, double>(w, n, nullptr, b);
+    if (inf == 0) wave_back_subst<double, double>(w, n, b);
+    __syncthreads();
     if (use_lds)
         for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += kLuBlock) a[e] = w[e];
     if (threadIdx.x == 0 && info) info[blockIdx.x] = inf;
@@ -179,7 +199,7 @@ __global__ __launch_bounds__(kLuBlock) void k_lu_solve_mixed(
     __shared__ double stat[2];
     for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += kLuBlock) lu[e] = (float)a[e];
     __syncthreads();
-    const int inf = lu_factor_apply<float, double>(lu, n, perm, (double *)nullptr);
+    const int inf = lu_blocked<float, double>(lu, n, perm, (double *)nullptr);
     if (inf != 0) {
         if (threadIdx.x == 0) {
             if (info) info[blockIdx.x] = inf;
@@ -191,30 +211,28 @@ __global__ __launch_bounds__(kLuBlock) void k_lu_solve_mixed(
         x[i] = 0.0;
         r[i] = b[i];
     }
+    __syncthreads();
     int it = 0;
     for (; it <= max_iter; ++it) {
-        __syncthreads();
         // correction d = LU32^{-1} r (carried in FP64 storage)
-        lu_forward<float, double>(lu, perm, n, r);
-        lu_back<float, double>(lu, n, r);
-        if (threadIdx.x == 0) {
-            stat[0] = 0.0;
-            stat[1] = 0.0;
-        }
+        wave_fwd_subst<float, double>(lu, perm, n, r);
+        wave_back_subst<float, double>(lu, n, r);
         __syncthreads();
         double dmax = 0.0, xmax = 0.0;
-        for (int i = threadIdx.x; i < n; i += kLuBlock) {
-            x[i] += r[i];
-            dmax = fmax(dmax, fabs(r[i]));
-            xmax = fmax(xmax, fabs(x[i]));
-        }
-        for (int off = 32; off > 0; off >>= 1) {
-            dmax = fmax(dmax, __shfl_xor(dmax, off, 64));
-            xmax = fmax(xmax, __shfl_xor(xmax, off, 64));
-        }
-        if ((threadIdx.x & 63) == 0) {
-            atomicMax((unsigned long long *)&stat[0], (unsigned long long)__double_as_longlong(dmax));
-            atomicMax((unsigned long long *)&stat[1], (unsigned long long)__double_as_longlong(xmax));
+        if (threadIdx.x < 64) {
+            for (int i = threadIdx.x; i < n; i += 64) {
+                x[i] += r[i];
+                dmax = fmax(dmax, fabs(r[i]));
+                xmax = fmax(xmax, fabs(x[i]));
+            }
+            for (int off = 32; off > 0; off >>= 1) {
+                dmax = fmax(dmax, __shfl_xor(dmax, off, 64));
+                xmax = fmax(xmax, __shfl_xor(xmax, off, 64));
+            }
+            if (threadIdx.x == 0) {
+                stat[0] = dmax;
+                stat[1] = xmax;
+            }
         }
         __syncthreads();
         const bool done = stat[0] <= tol * stat[1];
@@ -226,6 +244,7 @@ __global__ __launch_bounds__(kLuBlock) void k_lu_solve_mixed(
             for (int c = 0; c < n; ++c) s = fma(-row[c], x[c], s);
             r[i] = s;
         }
+        __syncthreads();
     }
     __syncthreads();
     for (int i = threadIdx.x; i < n; i += kLuBlock) b[i] = x[i];

@@ -1,0 +1,546 @@
+// fv_mlp.hip -- the far-field closure network: a 16 -> 128 -> 128 -> 128 -> 16
+// SiLU MLP with BF16 weights/activations on MFMA (v_mfma_f32_16x16x16_bf16,
+// FP32 accumulation), and its hand-written backward and Adam kernels.
+//
+// One wavefront owns 16 samples at a time; its activations live in LDS as
+// BF16 (the A operand), weights are read as BF16 rows (the B operand), and the
+// 16x16 FP32 accumulator tiles are biased, activated and rounded to BF16 in
+// registers before the next layer.  Inputs / outputs beyond the network's
+// n_in / n_out are zero-padded to 16.
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "fv_launch.h"
+
+typedef short fv_s4 __attribute__((ext_vector_type(4)));
+typedef float fv_f4 __attribute__((ext_vector_type(4)));
+
+namespace fv {
+
+constexpr int kIn = 16, kHid = 128, kOut = 16;
+// FP32 master parameter layout (row-major weights [out][in]).
+constexpr int kOffW1 = 0;
+constexpr int kOffB1 = kOffW1 + kHid * kIn;
+constexpr int kOffW2 = kOffB1 + kHid;
+constexpr int kOffB2 = kOffW2 + kHid * kHid;
+constexpr int kOffW3 = kOffB2 + kHid;
+constexpr int kOffB3 = kOffW3 + kHid * kHid;
+constexpr int kOffW4 = kOffB3 + kHid;
+constexpr int kOffB4 = kOffW4 + kOut * kHid;
+constexpr int kParams = kOffB4 + kOut;   // 37264
+constexpr int kLda = kHid + 8;           // LDS row stride of activations (bf16)
+
+__device__ __forceinline__ unsigned short f2bf(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);     // round to nearest even
+    return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((uint32_t)h << 16); }
+__device__ __forceinline__ float sigm(float z) { return 1.0f / (1.0f + expf(-z)); }
+__device__ __forceinline__ float silu(float z) { return z * sigm(z); }
+__device__ __forceinline__ float dsilu(float z)
+{
+    const float s = sigm(z);
+    return s * (1.0f + z * (1.0f - s));
+}
+
+struct MlpW {
+    const unsigned short *w[4];    // bf16 [out][in]
+    const unsigned short *wt[4];   // bf16 [in][out] (backward)
+    const float *b[4];
+};
+
+// acc[t] (16 samples x 16 outputs, tile t) = A (16 x K, LDS bf16, stride lda)
+//                                           x W[t*16 .. t*16+15][0..K)^T
+// MFMA 16x16x16 operand layout: lane l supplies row (l & 15) and the four
+// k values 4 (l >> 4) .. +3 of the current 16-wide k slice; the result holds
+// rows 4 (l >> 4) + v, column (l & 15).
+template <int K, int NT>
+__device__ __forceinline__ void wave_gemm_nt(const unsigned short *A, int lda,
+                                             const unsigned short *__restrict__ W, int ldw,
+                                             fv_f4 (&acc)[NT])
+{
+    const int l = threadIdx.x & 63;
+    const int r = l & 15, kq = (l >> 4) * 4;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = fv_f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        const fv_s4 a = *reinterpret_cast<const fv_s4 *>(A + r * lda + k0 + kq);
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const fv_s4 b = *reinterpret_cast<const fv_s4 *>(W + (t * 16 + r) * ldw + k0 + kq);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+        }
+    }
+}
+
+// Forward pass of nb samples; one wavefront per 16-sample group.
+//   X (nb x 16 fp32) -> Y (nb x 16 fp32).  When zs / hs are non-null the
+//   pre-activations (fp32) and activations (bf16) of the 3 hidden layers are
+//   kept for the backward pass: zs[(layer * nb + s) * 128 + n].
+__global__ __launch_bounds__(64) void k_mlp_forward(MlpW m, const float *__restrict__ X,
+                                                    int64_t nb, float *__restrict__ Y,
+                                                    float *__restrict__ zs,
+                                                    unsigned short *__restrict__ hs)
+{
+    __shared__ __attribute__((aligned(16))) unsigned short act[2][16 * kLda];
+    const int l = threadIdx.x;
+    const int col = l & 15, rq = (l >> 4) * 4;
+    for (int64_t g = blockIdx.x; g * 16 < nb; g += gridDim.x) {
+        const int64_t s0 = g * 16;
+        // inputs -> bf16 LDS (16 x 16)
+        for (int e = l; e < 256; e += 64) {
+            const int i = e >> 4, k = e & 15;
+            const float v = (s0 + i < nb) ? X[(s0 + i) * kIn + k] : 0.0f;
+            act[0][i * kLda + k] = f2bf(v);
+        }
+        __syncthreads();
+        int cur = 0;
+#pragma unroll
+        for (int layer = 0; layer < 3; ++layer) {
+            fv_f4 acc[8];
+            if (layer == 0)
+                wave_gemm_nt<kIn, 8>(act[cur], kLda, m.w[0], kIn, acc);
+            else
+                wave_gemm_nt<kHid, 8>(act[cur], kLda, m.w[layer], kHid, acc);
+            const int nxt = cur ^ 1;
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const int n = t * 16 + col;
+                const float bias = m.b[layer][n];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int i = rq + v;
+                    const float z = acc[t][v] + bias;
+                    const unsigned short h = f2bf(silu(z));
+                    act[nxt][i * kLda + n] = h;
+                    if (zs && s0 + i < nb) {
+                        zs[((int64_t)layer * nb + s0 + i) * kHid + n] = z;
+                        hs[((int64_t)layer * nb + s0 + i) * kHid + n] = h;
+                    }
+                }
+            }
+            __syncthreads();
+            cur = nxt;
+        }
+        fv_f4 out[1];
+        wave_gemm_nt<kHid, 1>(act[cur], kLda, m.w[3], kHid, out);
+        const float bias = m.b[3][col];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int64_t s = s0 + rq + v;
+            if (s < nb) Y[s * kOut + col] = out[0][v] + bias;
+        }
+        __syncthreads();
+    }
+}
+
+// Backward of the weighted MSE  L = (1/nb) sum_s sum_j w_j (Y_sj - T_sj)^2
+// down to the layer errors, kept as bf16 (the MFMA inputs of the weight
+// gradients):  dz[(L * nb + s) * 128 + n] for L = 0..2 (hidden layers), and
+// dy[s * 16 + j] for the output layer.
+__global__ __launch_bounds__(64) void k_mlp_backward(MlpW m, const float *__restrict__ Y,
+                                                     const float *__restrict__ T,
+                                                     const float *__restrict__ wout, int64_t nb,
+                                                     const float *__restrict__ zs,
+                                                     unsigned short *__restrict__ dz,
+                                                     unsigned short *__restrict__ dy)
+{
+    __shared__ __attribute__((aligned(16))) unsigned short buf[2][16 * kLda];
+    const int l = threadIdx.x;
+    const int col = l & 15, rq = (l >> 4) * 4;
+    const float inv = 2.0f / (float)nb;
+    for (int64_t g = blockIdx.x; g * 16 < nb; g += gridDim.x) {
+        const int64_t s0 = g * 16;
+        for (int e = l; e < 256; e += 64) {
+            const int i = e >> 4, j = e & 15;
+            const int64_t s = s0 + i;
+            float d = 0.0f;
+            if (s < nb) d = inv * wout[j] * (Y[s * kOut + j] - T[s * kOut + j]);
+            const unsigned short h = f2bf(d);
+            buf[0][i * kLda + j] = h;
+            if (s < nb) dy[s * kOut + j] = h;
+        }
+        __syncthreads();
+        int cur = 0;
+        for (int layer = 2; layer >= 0; --layer) {
+            fv_f4 acc[8];
+            if (layer == 2)
+                wave_gemm_nt<kOut, 8>(buf[cur], kLda, m.wt[3], kOut, acc);
+            else
+                wave_gemm_nt<kHid, 8>(buf[cur], kLda, m.wt[layer + 1], kHid, acc);
+            const int nxt = cur ^ 1;
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const int n = t * 16 + col;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int i = rq + v;
+                    const int64_t s = s0 + i;
+                    float d = 0.0f;
+                    if (s < nb) d = acc[t][v] * dsilu(zs[((int64_t)layer * nb + s) * kHid + n]);
+                    const unsigned short h = f2bf(d);
+                    buf[nxt][i * kLda + n] = h;
+                    if (s < nb) dz[((int64_t)layer * nb + s) * kHid + n] = h;
+                }
+            }
+            __syncthreads();
+            cur = nxt;
+        }
+    }
+}
+
+// Weight gradient dW[n][k] = sum_s E[s][n] H[s][k] for one 16 x 16 tile per
+// wavefront (MFMA over 16-sample slices, fixed order), bias gradient
+// db[n] = sum_s E[s][n] by the tiles with k-tile 0.
+__global__ __launch_bounds__(64) void k_mlp_wgrad(const unsigned short *__restrict__ E, int ne,
+                                                  const unsigned short *__restrict__ H, int nh,
+                                                  const float *__restrict__ X, int64_t nb,
+                                                  float *__restrict__ dW, float *__restrict__ db)
+{
+    const int l = threadIdx.x;
+    const int tiles_k = nh / 16;
+    const int n0 = (blockIdx.x / tiles_k) * 16, k0 = (blockIdx.x % tiles_k) * 16;
+    const int r = l & 15, sq = (l >> 4) * 4;
+    fv_f4 acc = {0.f, 0.f, 0.f, 0.f};
+    float bacc = 0.0f;
+    for (int64_t s0 = 0; s0 < nb; s0 += 16) {
+        fv_s4 a, b;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int64_t s = s0 + sq + q;
+            const bool ok = s < nb;
+            a[q] = ok ? (short)E[s * ne + n0 + r] : (short)0;
+            unsigned short hv = 0;
+            if (ok) hv = H ? H[s * nh + k0 + r] : f2bf(X[s * nh + k0 + r]);
+            b[q] = (short)hv;
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) dW[(n0 + sq + v) * nh + k0 + r] = acc[v];
+    if (k0 == 0 && l < 16) {
+        for (int64_t s = 0; s < nb; ++s) bacc += bf2f(E[s * ne + n0 + l]);
+        db[n0 + l] = bacc;
+    }
+}
+
+// Adam on the FP32 master parameters; refreshes the bf16 weight copies.
+__global__ void k_mlp_adam(float *__restrict__ p, const float *__restrict__ g,
+                           float *__restrict__ mo, float *__restrict__ ve, int n, float lr,
+                           float b1, float b2, float eps, float c1, float c2,
+                           unsigned short *__restrict__ wbf, unsigned short *__restrict__ wtbf)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float m = b1 * mo[i] + (1.0f - b1) * gi;
+    const float v = b2 * ve[i] + (1.0f - b2) * gi * gi;
+    mo[i] = m;
+    ve[i] = v;
+    const float upd = lr * (m / c1) / (sqrtf(v / c2) + eps);
+    const float np = p[i] - upd;
+    p[i] = np;
+    // bf16 copies of the weight matrices (biases stay fp32)
+    int off = -1, rows = 0, cols = 0, base = 0;
+    if (i >= kOffW1 && i < kOffB1) { off = i - kOffW1; rows = kHid; cols = kIn; base = 0; }
+    else if (i >= kOffW2 && i < kOffB2) { off = i - kOffW2; rows = kHid; cols = kHid; base = 1; }
+    else if (i >= kOffW3 && i < kOffB3) { off = i - kOffW3; rows = kHid; cols = kHid; base = 2; }
+    else if (i >= kOffW4 && i < kOffB4) { off = i - kOffW4; rows = kOut; cols = kHid; base = 3; }
+    if (off >= 0) {
+        static constexpr int woff[4] = {0, kHid * kIn, kHid * kIn + kHid * kHid,
+                                        kHid * kIn + 2 * kHid * kHid};
+        const unsigned short h = f2bf(np);
+        wbf[woff[base] + off] = h;
+        const int r = off / cols, c = off % cols;
+        wtbf[woff[base] + c * rows + r] = h;
+    }
+}
+
+}  // namespace fv
+
+using namespace fv;
+
+struct fv_mlp {
+    fv_ctx *ctx = nullptr;
+    void *block = nullptr;
+    float *params = nullptr, *grad = nullptr, *m1 = nullptr, *m2 = nullptr;
+    unsigned short *wbf = nullptr, *wtbf = nullptr;
+    int64_t step = 0;
+    // training scratch (grown on demand)
+    void *scratch = nullptr;
+    int64_t scratch_nb = 0;
+};
+
+static constexpr int kWeightsBf = kHid * kIn + 2 * kHid * kHid + kOut * kHid;
+
+static MlpW mlp_views(const fv_mlp *m)
+{
+    MlpW w;
+    const int woff[4] = {0, kHid * kIn, kHid * kIn + kHid * kHid, kHid * kIn + 2 * kHid * kHid};
+    const int boff[4] = {kOffB1, kOffB2, kOffB3, kOffB4};
+    for (int k = 0; k < 4; ++k) {
+        w.w[k] = m->wbf + woff[k];
+        w.wt[k] = m->wtbf + woff[k];
+        w.b[k] = m->params + boff[k];
+    }
+    return w;
+}
+
+static uint16_t host_f2bf(float f)
+{
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+static int mlp_scratch(fv_mlp *m, int64_t nb)
+{
+    if (m->scratch_nb >= nb) return FV_OK;
+    fv_ctx *ctx = m->ctx;
+    if (m->scratch) {
+        FV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        FV_HIP_CHECK(ctx, hipFree(m->scratch));
+        m->scratch = nullptr;
+    }
+    // Y (16 fp32), zs (3x128 fp32), hs (3x128 bf16), dz (3x128 bf16), dy (16 bf16)
+    const size_t bytes = (size_t)nb * (16 * 4 + 384 * 4 + 384 * 2 + 384 * 2 + 16 * 2) + 1024;
+    FV_HIP_CHECK(ctx, hipMalloc(&m->scratch, bytes));
+    m->scratch_nb = nb;
+    return FV_OK;
+}
+
+extern "C" {
+
+int fv_mlp_create(fv_ctx *ctx, fv_mlp **out)
+{
+    if (!ctx || !out) return FV_E_ARG;
+    *out = nullptr;
+    fv_mlp *m = new (std::nothrow) fv_mlp();
+    if (!m) return set_error(ctx, FV_E_HIP, "out of host memory");
+    m->ctx = ctx;
+    const size_t bytes = sizeof(float) * 4 * kParams + sizeof(unsigned short) * 2 * kWeightsBf + 256;
+    hipError_t e = hipMalloc(&m->block, bytes);
+    if (e != hipSuccess) {
+        delete m;
+        return set_error(ctx, FV_E_HIP, "hipMalloc for the MLP failed: %s", hipGetErrorString(e));
+    }
+    float *p = static_cast<float *>(m->block);
+    m->params = p;
+    m->grad = p + kParams;
+    m->m1 = p + 2 * kParams;
+    m->m2 = p + 3 * kParams;
+    m->wbf = reinterpret_cast<unsigned short *>(p + 4 * kParams);
+    m->wtbf = m->wbf + kWeightsBf;
+    e = hipMemsetAsync(m->block, 0, bytes, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        (void)hipFree(m->block);
+        delete m;
+        return set_error(ctx, FV_E_HIP, "MLP init failed: %s", hipGetErrorString(e));
+    }
+    *out = m;
+    return FV_OK;
+}
+
+void fv_mlp_destroy(fv_mlp *m)
+{
+    if (!m) return;
+    (void)hipStreamSynchronize(m->ctx->stream);
+    if (m->scratch) (void)hipFree(m->scratch);
+    (void)hipFree(m->block);
+    delete m;
+}
+
+int fv_mlp_param_count(void) { return kParams; }
+
+int fv_mlp_set_params(fv_mlp *m, const float *params)
+{
+    if (!m || !params) return FV_E_ARG;
+    fv_ctx *ctx = m->ctx;
+    std::vector<uint16_t> wb(kWeightsBf), wt(kWeightsBf);
+    const int woff[4] = {0, kHid * kIn, kHid * kIn + kHid * kHid, kHid * kIn + 2 * kHid * kHid};
+    const int poff[4] = {kOffW1, kOffW2, kOffW3, kOffW4};
+    const int rows[4] = {kHid, kHid, kHid, kOut}, cols[4] = {kIn, kHid, kHid, kHid};
+    for (int k = 0; k < 4; ++k)
+        for (int r = 0; r < rows[k]; ++r)
+            for (int c = 0; c < cols[k]; ++c) {
+                const uint16_t h = host_f2bf(params[poff[k] + r * cols[k] + c]);
+                wb[woff[k] + r * cols[k] + c] = h;
+                wt[woff[k] + c * rows[k] + r] = h;
+            }
+    FV_HIP_CHECK(ctx, hipMemcpyAsync(m->params, params, sizeof(float) * kParams,
+                                     hipMemcpyHostToDevice, ctx->stream));
+    FV_HIP_CHECK(ctx, hipMemcpyAsync(m->wbf, wb.data(), 2 * kWeightsBf, hipMemcpyHostToDevice,
+                                     ctx->stream));
+    FV_HIP_CHECK(ctx, hipMemcpyAsync(m->wtbf, wt.data(), 2 * kWeightsBf, hipMemcpyHostToDevice,
+                                     ctx->stream));
+    FV_HIP_CHECK(ctx, hipMemsetAsync(m->m1, 0, sizeof(float) * 2 * kParams, ctx->stream));
+    FV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    m->step = 0;
+    return FV_OK;
+}
+
+int fv_mlp_get_params(fv_mlp *m, float *params)
+{
+    if (!m || !params) return FV_E_ARG;
+    return fv_memcpy_d2h(m->ctx, params, m->params, sizeof(float) * kParams);
+}
+
+int fv_mlp_forward_dev(fv_mlp *m, const float *X, int64_t nb, float *Y)
+{
+    if (!m) return FV_E_ARG;
+    fv_ctx *ctx = m->ctx;
+    FV_REQUIRE(ctx, nb >= 0, "negative batch");
+    if (nb == 0) return FV_OK;
+    FV_REQUIRE(ctx, X && Y, "null X/Y");
+    const int64_t groups = ceil_div(nb, 16);
+    const unsigned grid = (unsigned)(groups < 8192 ? groups : 8192);
+    TimedLaunch tl(ctx, "mlp_forward");
+    hipLaunchKernelGGL(k_mlp_forward, dim3(grid), dim3(64), 0, ctx->stream, mlp_views(m), X, nb,
+                       Y, (float *)nullptr, (unsigned short *)nullptr);
+    FV_LAUNCH_CHECK(ctx);
+    tl.done();
+    return FV_OK;
+}
+
+// Gradient of the weighted MSE into m->grad (device); loss not computed here.
+static int mlp_gradients(fv_mlp *m, const float *X, const float *T, const float *wout,
+                         int64_t nb)
+{
+    fv_ctx *ctx = m->ctx;
+    int rc = mlp_scratch(m, nb);
+    if (rc) return rc;
+    char *s = static_cast<char *>(m->scratch);
+    float *Y = reinterpret_cast<float *>(s);
+    float *zs = Y + nb * 16;
+    unsigned short *hs = reinterpret_cast<unsigned short *>(zs + nb * 384);
+    unsigned short *dz = hs + nb * 384;
+    unsigned short *dy = dz + nb * 384;
+    const int64_t groups = ceil_div(nb, 16);
+    const unsigned grid = (unsigned)(groups < 8192 ? groups : 8192);
+    const MlpW w = mlp_views(m);
+    TimedLaunch tl(ctx, "mlp_train");
+    hipLaunchKernelGGL(k_mlp_forward, dim3(grid), dim3(64), 0, ctx->stream, w, X, nb, Y, zs, hs);
+    FV_LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_mlp_backward, dim3(grid), dim3(64), 0, ctx->stream, w, Y, T, wout, nb,
+                       zs, dz, dy);
+    FV_LAUNCH_CHECK(ctx);
+    float *g = m->grad;
+    // W4 / b4: E = dy (16), H = h3
+    hipLaunchKernelGGL(k_mlp_wgrad, dim3((kOut / 16) * (kHid / 16)), dim3(64), 0, ctx->stream, dy,
+                       kOut, hs + 2 * nb * kHid, kHid, (const float *)nullptr, nb, g + kOffW4,
+                       g + kOffB4);
+    FV_LAUNCH_CHECK(ctx);
+    // W3 / b3: E = dz[2], H = h2;  W2 / b2: E = dz[1], H = h1
+    hipLaunchKernelGGL(k_mlp_wgrad, dim3((kHid / 16) * (kHid / 16)), dim3(64), 0, ctx->stream,
+                       dz + 2 * nb * kHid, kHid, hs + nb * kHid, kHid, (const float *)nullptr, nb,
+                       g + kOffW3, g + kOffB3);
+    FV_LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_mlp_wgrad, dim3((kHid / 16) * (kHid / 16)), dim3(64), 0, ctx->stream,
+                       dz + nb * kHid, kHid, hs, kHid, (const float *)nullptr, nb, g + kOffW2,
+                       g + kOffB2);
+    FV_LAUNCH_CHECK(ctx);
+    // W1 / b1: E = dz[0], H = bf16(X)
+    hipLaunchKernelGGL(k_mlp_wgrad, dim3((kHid / 16) * (kIn / 16)), dim3(64), 0, ctx->stream, dz,
+                       kHid, (const unsigned short *)nullptr, kIn, X, nb, g + kOffW1,
+                       g + kOffB1);
+    FV_LAUNCH_CHECK(ctx);
+    tl.done();
+    return FV_OK;
+}
+
+int fv_mlp_gradients_dev(fv_mlp *m, const float *X, const float *T, const float *wout,
+                         int64_t nb, float *grad)
+{
+    if (!m) return FV_E_ARG;
+    fv_ctx *ctx = m->ctx;
+    FV_REQUIRE(ctx, nb > 0 && X && T && wout, "bad gradient arguments");
+    int rc = mlp_gradients(m, X, T, wout, nb);
+    if (rc) return rc;
+    if (grad)
+        FV_HIP_CHECK(ctx, hipMemcpyAsync(grad, m->grad, sizeof(float) * kParams,
+                                         hipMemcpyDeviceToDevice, ctx->stream));
+    return FV_OK;
+}
+
+int fv_mlp_adam_dev(fv_mlp *m, const float *grad, float lr)
+{
+    if (!m) return FV_E_ARG;
+    fv_ctx *ctx = m->ctx;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    m->step += 1;
+    const float c1 = 1.0f - powf(b1, (float)m->step);
+    const float c2 = 1.0f - powf(b2, (float)m->step);
+    TimedLaunch tl(ctx, "mlp_adam");
+    hipLaunchKernelGGL(k_mlp_adam, dim3((unsigned)ceil_div(kParams, 256)), dim3(256), 0,
+                       ctx->stream, m->params, grad ? grad : m->grad, m->m1, m->m2, kParams, lr,
+                       b1, b2, eps, c1, c2, m->wbf, m->wtbf);
+    FV_LAUNCH_CHECK(ctx);
+    tl.done();
+    return FV_OK;
+}
+
+int fv_mlp_train_step_dev(fv_mlp *m, const float *X, const float *T, const float *wout,
+                          int64_t nb, float lr)
+{
+    if (!m) return FV_E_ARG;
+    fv_ctx *ctx = m->ctx;
+    FV_REQUIRE(ctx, nb > 0 && X && T && wout, "bad training arguments");
+    int rc = mlp_gradients(m, X, T, wout, nb);
+    if (rc) return rc;
+    return fv_mlp_adam_dev(m, nullptr, lr);
+}
+
+// Host twins (tests): copy in, run, copy out, synchronise.
+int fv_mlp_forward_host(fv_mlp *m, const float *X, int64_t nb, float *Y)
+{
+    if (!m) return FV_E_ARG;
+    fv_ctx *ctx = m->ctx;
+    FV_REQUIRE(ctx, nb >= 0 && (nb == 0 || (X && Y)), "bad forward arguments");
+    if (nb == 0) return FV_OK;
+    float *d = static_cast<float *>(workspace(ctx, S_TMP0, sizeof(float) * nb * 32));
+    if (!d) return FV_E_HIP;
+    FV_HIP_CHECK(ctx, hipMemcpyAsync(d, X, sizeof(float) * nb * 16, hipMemcpyHostToDevice,
+                                     ctx->stream));
+    int rc = fv_mlp_forward_dev(m, d, nb, d + nb * 16);
+    if (rc) return rc;
+    return fv_memcpy_d2h(ctx, Y, d + nb * 16, sizeof(float) * nb * 16);
+}
+
+int fv_mlp_gradients_host(fv_mlp *m, const float *X, const float *T, const float *wout,
+                          int64_t nb, float *grad)
+{
+    if (!m) return FV_E_ARG;
+    fv_ctx *ctx = m->ctx;
+    FV_REQUIRE(ctx, nb > 0 && X && T && wout && grad, "bad gradient arguments");
+    float *d = static_cast<float *>(workspace(ctx, S_TMP1, sizeof(float) * (nb * 32 + 16)));
+    if (!d) return FV_E_HIP;
+    FV_HIP_CHECK(ctx, hipMemcpyAsync(d, X, sizeof(float) * nb * 16, hipMemcpyHostToDevice,
+                                     ctx->stream));
+    FV_HIP_CHECK(ctx, hipMemcpyAsync(d + nb * 16, T, sizeof(float) * nb * 16,
+                                     hipMemcpyHostToDevice, ctx->stream));
+    FV_HIP_CHECK(ctx, hipMemcpyAsync(d + nb * 32, wout, sizeof(float) * 16, hipMemcpyHostToDevice,
+                                     ctx->stream));
+    int rc = mlp_gradients(m, d, d + nb * 16, d + nb * 32, nb);
+    if (rc) return rc;
+    return fv_memcpy_d2h(ctx, grad, m->grad, sizeof(float) * kParams);
+}
+
+int fv_mlp_adam_host(fv_mlp *m, const float *grad, float lr)
+{
+    if (!m || !grad) return FV_E_ARG;
+    fv_ctx *ctx = m->ctx;
+    FV_HIP_CHECK(ctx, hipMemcpyAsync(m->grad, grad, sizeof(float) * kParams,
+                                     hipMemcpyHostToDevice, ctx->stream));
+    int rc = fv_mlp_adam_dev(m, nullptr, lr);
+    if (rc) return rc;
+    return fv_ctx_synchronize(ctx);
+}
+
+}  // extern "C"

@@ -33,7 +33,7 @@ struct alignas(16) SegRec {
 
 // One vortex particle with per-source constants:
 //   Winckelmans:  k0 = 1/sigma^2, k1 = -1/(4 pi sigma^3), k2 = 1/(4 pi sigma^5)
-//   Gaussian-erf: k0 = 1/sigma,   k1 = sigma,             k2 = 0
+//   Gaussian-erf: k0 = 1/sigma,   k1 = sigma,             k2 = 1/sigma^2
 struct alignas(16) PartRec {
     double x, y, z, ax, ay, az;
     double k0, k1, k2, pad;
@@ -44,8 +44,8 @@ struct Pt3 {
 };
 
 // ------------------------------------------------------------ FP64 helpers
-// 1/sqrt(a) for a >= 1: hardware estimate refined by two Newton steps
-// (relative error ~1 ulp; a >= 1 so no denormal scaling is needed).
+// 1/sqrt(a) for normal a > 0: hardware estimate refined by two Newton steps
+// (relative error ~1 ulp; no denormal scaling, so a must be a normal number).
 __device__ __forceinline__ double rsqrt_refined(double a)
 {
     double y = __builtin_amdgcn_rsq(a);
@@ -56,11 +56,28 @@ __device__ __forceinline__ double rsqrt_refined(double a)
     return y;
 }
 
+// 1/d for normal d: hardware estimate refined by two Newton steps (~1 ulp).
+__device__ __forceinline__ double rcp_refined(double d)
+{
+    double y = __builtin_amdgcn_rcp(d);
+    double e = fma(-d, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-d, y, 1.0);
+    y = fma(y, e, y);
+    return y;
+}
+
 // ------------------------------------------------------- segment, FP64
-// Ptera-form segment velocity (warp_kernels.py:46-96), written for FMA
-// hardware.  The skip rules are the reference's (:72-82); the r3 test is
-// done on squares (|r1 x r2|^2 < (1e-10 |r1||r2|)^2), which differs from the
-// reference's sqrt test only when the two sides round to the same value.
+// Ptera-form segment velocity (warp_kernels.py:46-96), rewritten for gfx950:
+//   c4 = c1 (|r1| + |r2|)(|r1||r2| - r1.r2) / (|r1||r2| (|r1 x r2|^2 + c2))
+//      = c1 (1/|r1| + 1/|r2|)(|r1||r2| - r1.r2) / (|r1 x r2|^2 + c2)
+// with 1/|r1|, 1/|r2| from v_rsq_f64 + Newton and one refined reciprocal, so
+// there is no IEEE sqrt/div expansion in the pair loop.  The skip rules are the
+// reference's (:72-82) evaluated on squares (|r1|^2 < tol^2, |r2|^2 < tol^2,
+// |r1 x r2|^2 < 1e-20 |r1|^2 |r2|^2); they differ from the reference's sqrt
+// tests only when both sides round to the same value.  Skipped pairs are
+// selected away, so their (possibly non-finite) intermediate values never
+// reach the accumulator.
 __device__ __forceinline__ void seg_pair_f64(double px, double py, double pz, const SegRec &s,
                                              double &ax, double &ay, double &az)
 {
@@ -69,14 +86,16 @@ __device__ __forceinline__ void seg_pair_f64(double px, double py, double pz, co
     const double r3x = r1y * r2z - r1z * r2y;
     const double r3y = r1z * r2x - r1x * r2z;
     const double r3z = r1x * r2y - r1y * r2x;
-    const double r1 = sqrt(r1x * r1x + r1y * r1y + r1z * r1z);
-    const double r2 = sqrt(r2x * r2x + r2y * r2y + r2z * r2z);
+    const double r1sq = r1x * r1x + r1y * r1y + r1z * r1z;
+    const double r2sq = r2x * r2x + r2y * r2y + r2z * r2z;
     const double r3sq = r3x * r3x + r3y * r3y + r3z * r3z;
-    const double r1r2 = r1 * r2;
-    const double thr = 1.0e-10 * r1r2;
-    const bool skip = (r1 < s.tol) || (r2 < s.tol) || (r3sq < thr * thr);
     const double c3 = r1x * r2x + r1y * r2y + r1z * r2z;
-    double c4 = s.c1 * (r1 + r2) * (r1r2 - c3) / (r1r2 * (r3sq + s.c2));
+    const double tol2 = s.tol * s.tol;
+    const bool skip = (r1sq < tol2) || (r2sq < tol2) || (r3sq < 1.0e-20 * (r1sq * r2sq));
+    const double i1 = rsqrt_refined(r1sq);
+    const double i2 = rsqrt_refined(r2sq);
+    const double r1r2 = (r1sq * i1) * (r2sq * i2);
+    double c4 = s.c1 * (i1 + i2) * (r1r2 - c3) * rcp_refined(r3sq + s.c2);
     c4 = skip ? 0.0 : c4;
     ax = fma(c4, r3x, ax);
     ay = fma(c4, r3y, ay);
@@ -141,6 +160,14 @@ __device__ __forceinline__ void part_factors_f64(double r2, const PartRec &s, do
         const double t5 = t2 * t2 * t;
         CG = s.k1 * (rho2 + 2.5) * t5;
         if (GRAD) CH = s.k2 * fma(3.0, rho2, 10.5) * (t5 * t2);
+    } else if (r2 * s.k2 > 100.0) {
+        // rho > 10: erf(rho/sqrt2) == 1 and sqrt(2/pi) rho e^(-rho^2/2) < 2e-21 in
+        // FP64, so g == 1 and g' == 0 to the last bit of the full formula.
+        const double ir = rsqrt_refined(r2);
+        const double ir2 = ir * ir;
+        const double ir3 = ir2 * ir;
+        CG = -kInv4Pi * ir3;
+        if (GRAD) CH = 3.0 * kInv4Pi * ir3 * ir2;
     } else {
         const double r = sqrt(r2);
         const double rho = r * s.k0;
@@ -172,6 +199,12 @@ __device__ __forceinline__ void part_factors_mixed(float r2, double r2d, const P
         if (GRAD) CH = (float)s.k2 * fmaf(3.0f, rho2, 10.5f) * (t5 * t2);
     } else {
         const float ir = __builtin_amdgcn_rsqf(r2);
+        if (r2 * (float)s.k2 > 100.0f) {   // g == 1, g' == 0 (see the FP64 path)
+            const float ir3 = ir * ir * ir;
+            CG = -(float)kInv4Pi * ir3;
+            if (GRAD) CH = 3.0f * (float)kInv4Pi * ir3 * (ir * ir);
+            return;
+        }
         const float r = r2 * ir;
         const float rho = r * (float)s.k0;
         if (rho < 0.25f) {
@@ -209,7 +242,7 @@ __device__ __forceinline__ PartRec make_part_rec(int kernel, double x, double y,
     } else {
         r.k0 = 1.0 / sigma;
         r.k1 = sigma;
-        r.k2 = 0.0;
+        r.k2 = r.k0 * r.k0;
     }
     r.pad = 0.0;
     return r;
@@ -390,10 +423,13 @@ struct PartOp {
 };
 
 // --------------------------------------------------- tiled pair-sum kernel
+// Each lane owns TPT targets (i0 + lane + k * BLOCK); the workgroup covers
+// BLOCK * TPT targets and walks its source chunk in LDS tiles of BLOCK
+// records, so every staged record feeds TPT pairs per lane.
 // partial[(blockIdx.y * n_tgt_stride + i) * kComp + c] = sum over this
 // block's source chunk.  n_tgt / n_src come from *dyn_tgt / *dyn_src when
 // those are non-null (device-side particle counts), else from the arguments.
-template <class Op, int BLOCK>
+template <class Op, int BLOCK, int TPT>
 __global__ __launch_bounds__(BLOCK) void k_pairs(typename Op::Args a, int64_t n_tgt_arg,
                                                  int64_t n_src_arg,
                                                  const int64_t *__restrict__ dyn_tgt,
@@ -404,14 +440,17 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(typename Op::Args a, int64_t n_
     __shared__ typename Op::Src tile[BLOCK];
     const int64_t n_tgt = dyn_tgt ? *dyn_tgt : n_tgt_arg;
     const int64_t n_src = dyn_src ? *dyn_src : n_src_arg;
-    const int64_t i0 = (int64_t)blockIdx.x * BLOCK;
+    const int64_t i0 = (int64_t)blockIdx.x * (BLOCK * TPT);
     if (i0 >= n_tgt) return;   // uniform over the block
-    const int64_t i = i0 + threadIdx.x;
-    const bool active = i < n_tgt;
-    typename Op::Tgt t = Op::load_tgt(a, active ? i : i0);
-    typename Op::Acc acc;
+    typename Op::Tgt t[TPT];
+    typename Op::Acc acc[TPT];
 #pragma unroll
-    for (int c = 0; c < Op::kComp; ++c) acc.v[c] = 0.0;
+    for (int k = 0; k < TPT; ++k) {
+        const int64_t i = i0 + threadIdx.x + k * BLOCK;
+        t[k] = Op::load_tgt(a, i < n_tgt ? i : i0);
+#pragma unroll
+        for (int c = 0; c < Op::kComp; ++c) acc[k].v[c] = 0.0;
+    }
     const int64_t s0 = (int64_t)blockIdx.y * chunk;
     const int64_t s1 = (s0 + chunk < n_src) ? s0 + chunk : n_src;
     for (int64_t base = s0; base < s1; base += BLOCK) {
@@ -419,13 +458,21 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(typename Op::Args a, int64_t n_
         __syncthreads();
         if (threadIdx.x < cnt) tile[threadIdx.x] = Op::load_src(a, base + threadIdx.x);
         __syncthreads();
-#pragma unroll 2
-        for (int k = 0; k < cnt; ++k) Op::pair(t, tile[k], acc);
-    }
-    if (active) {
-        double *o = partial + ((int64_t)blockIdx.y * n_tgt_stride + i) * Op::kComp;
+#pragma unroll(TPT == 1 ? 2 : 1)
+        for (int j = 0; j < cnt; ++j) {
+            const typename Op::Src src = tile[j];
 #pragma unroll
-        for (int c = 0; c < Op::kComp; ++c) o[c] = acc.v[c];
+            for (int k = 0; k < TPT; ++k) Op::pair(t[k], src, acc[k]);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < TPT; ++k) {
+        const int64_t i = i0 + threadIdx.x + k * BLOCK;
+        if (i < n_tgt) {
+            double *o = partial + ((int64_t)blockIdx.y * n_tgt_stride + i) * Op::kComp;
+#pragma unroll
+            for (int c = 0; c < Op::kComp; ++c) o[c] = acc[k].v[c];
+        }
     }
 }
 

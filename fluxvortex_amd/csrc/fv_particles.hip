@@ -239,6 +239,7 @@ int vpm_convert_rings(fv_vpm *f, int64_t nr, const double *br, const double *fr,
                        f->z, f->gx, f->gy, f->gz, f->sig, f->age);
     FV_LAUNCH_CHECK(ctx);
     f->np_exact = false;
+    f->state_epoch++;
     f->np_upper += 4 * nr;
     if (f->np_upper > f->cap) f->np_upper = f->cap;
     return FV_OK;
@@ -283,50 +284,61 @@ int vpm_advect(fv_vpm *f, double dt, const double *uinf, int stretch, int free_w
     if (nu <= 0) return FV_OK;
     static const double ra[3] = {0.0, -5.0 / 9.0, -153.0 / 128.0};   // particles.py:114
     static const double rb[3] = {1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0};
-    int rc = fill(ctx, f->rkX, 3 * nu, 0.0);
+    int rc = FV_OK;
+    if (f->closure) {
+        rc = closure_step(f);
+        if (rc) return rc;
+    }
+    const bool clo = closure_active(f);
+    rc = fill(ctx, f->rkX, 3 * nu, 0.0);
     if (!rc && stretch) rc = fill(ctx, f->rkG, 3 * nu, 0.0);
     if (!rc && stretch) rc = fill(ctx, f->rks, nu, 0.0);
     if (rc) return rc;
     const unsigned gb = (unsigned)ceil_div(nu, 256);
     const PartSrcArgs src = field_src(f, f->kernel);
     const PartTgtArgs tgt = field_tgt(f);
+    // self sum of one mode, by the closure when it is in use, else directly
+    auto self_sum = [&](int mode, int kernel) -> int {
+        if (clo) return closure_self_sum(f, mode, kernel, f->sum);
+        f->particle_pairs += (double)nu * (double)nu;
+        return part_sum(ctx, kernel, f->precision, mode,
+                        kernel == f->kernel ? src : field_src(f, kernel), tgt, nu, nu, f->np_dev,
+                        f->np_dev, f->sum, S_TMP0);
+    };
     for (int st = 0; st < 3; ++st) {
         const double *upart = nullptr;
         if (free_wake) {
-            rc = part_sum(ctx, f->kernel, f->precision, 0, src, tgt, nu, nu, f->np_dev, f->np_dev,
-                          f->sum, S_TMP0);
+            rc = self_sum(0, f->kernel);
             if (rc) return rc;
-            f->particle_pairs += (double)nu * (double)nu;
             upart = f->sum;
         }
         hipLaunchKernelGGL(k_vpm_stage_pos, dim3(gb), dim3(256), 0, ctx->stream, f->np_dev, f->x,
                            f->y, f->z, upart, uinf[0], uinf[1], uinf[2], ra[st], rb[st], dt,
                            f->rkX);
         FV_LAUNCH_CHECK(ctx);
+        f->state_epoch++;
         if (stretch) {
-            rc = part_sum(ctx, f->kernel, f->precision, 2, src, tgt, nu, nu, f->np_dev, f->np_dev,
-                          f->sum, S_TMP0);
+            rc = self_sum(2, f->kernel);
             if (rc) return rc;
-            f->particle_pairs += (double)nu * (double)nu;
             hipLaunchKernelGGL(k_vpm_stage_gamma, dim3(gb), dim3(256), 0, ctx->stream, f->np_dev,
                                f->sum, f->gx, f->gy, f->gz, f->sig, ra[st], rb[st], dt, f->nu,
                                f->rkG, f->rks);
             FV_LAUNCH_CHECK(ctx);
+            f->state_epoch++;
         }
     }
     hipLaunchKernelGGL(k_vpm_finish, dim3(gb), dim3(256), 0, ctx->stream, f->np_dev, f->x, f->y,
                        f->z, f->gx, f->gy, f->gz, f->age, dt);
     FV_LAUNCH_CHECK(ctx);
+    f->state_epoch++;
     if (f->rlxf > 0) {
         // the relaxation always uses the Gaussian-erf field (particles.py:219)
-        rc = part_sum(ctx, FV_KERNEL_GAUSSIAN_ERF, f->precision, 3,
-                      field_src(f, FV_KERNEL_GAUSSIAN_ERF), tgt, nu, nu, f->np_dev, f->np_dev,
-                      f->sum, S_TMP0);
+        rc = self_sum(3, FV_KERNEL_GAUSSIAN_ERF);
         if (rc) return rc;
-        f->particle_pairs += (double)nu * (double)nu;
         hipLaunchKernelGGL(k_vpm_relax, dim3(gb), dim3(256), 0, ctx->stream, f->np_dev, f->sum,
                            f->gx, f->gy, f->gz, f->rlxf);
         FV_LAUNCH_CHECK(ctx);
+        f->state_epoch++;
     }
     return FV_OK;
 }
@@ -458,6 +470,8 @@ int fv_vpm_create(fv_ctx *ctx, int64_t max_particles, double nu, double rlxf, in
 void fv_vpm_destroy(fv_vpm *f)
 {
     if (!f) return;
+    closure_destroy(f->closure);
+    f->closure = nullptr;
     (void)hipStreamSynchronize(f->ctx->stream);
     (void)hipFree(f->block);
     delete f;
@@ -545,6 +559,7 @@ int fv_vpm_add_host(fv_vpm *f, const double *pos, const double *gamma, const dou
     if (rc) return rc;
     f->np_host += n;
     f->np_upper = f->np_host;
+    f->state_epoch++;
     FV_HIP_CHECK(ctx, hipMemcpyAsync(f->np_dev, &f->np_host, sizeof(int64_t),
                                      hipMemcpyHostToDevice, ctx->stream));
     FV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
@@ -564,6 +579,7 @@ int fv_vpm_write_host(fv_vpm *f, int64_t np, const double *pos, const double *ga
     f->np_host = np;
     f->np_upper = np;
     f->np_exact = true;
+    f->state_epoch++;
     FV_HIP_CHECK(ctx, hipMemcpyAsync(f->np_dev, &f->np_host, sizeof(int64_t),
                                      hipMemcpyHostToDevice, ctx->stream));
     FV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));

@@ -3,6 +3,9 @@
 
 #include "fv_launch.h"
 
+struct fv_closure;
+struct fv_mlp;
+
 // SoA particle state (one allocation), RK3 low-storage registers and the
 // pair-sum buffer.  np_dev is the authoritative particle count; the host keeps
 // an exact copy when it changed the count itself, else an upper bound.
@@ -25,6 +28,8 @@ struct fv_vpm {
     bool np_exact = true;
     int64_t np_upper = 0;
     double particle_pairs = 0.0;   // pairs evaluated, from the host-side upper bound
+    int64_t state_epoch = 0;       // bumped by every enqueued change of the particle state
+    fv_closure *closure = nullptr; // far-field closure (optional)
 };
 
 namespace fv {
@@ -43,5 +48,16 @@ int vpm_induce(fv_vpm *f, const double *targets, int64_t n, int precision, doubl
 
 // Makes np_host exact (synchronises).
 int vpm_sync_count(fv_vpm *f);
+
+// Far-field closure hooks (fv_closure.hip).
+// Per-step bookkeeping at the start of an advect: online training and the
+// periodic check against the exact sum (sets whether the closure is used).
+int closure_step(fv_vpm *f);
+// True when the closure replaces the direct self sums for this step.
+bool closure_active(const fv_vpm *f);
+// Self sum at the particles (original order): mode 0 velocity, 2 stretching,
+// 3 vorticity; kernel selects the near-field smoothing.
+int closure_self_sum(fv_vpm *f, int mode, int kernel, double *out);
+void closure_destroy(fv_closure *cl);
 
 }  // namespace fv

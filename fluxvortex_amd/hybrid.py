@@ -129,6 +129,13 @@ class HybridRun:
         self._ctx.check(rc)
         return cl, cdi, gam
 
+    def set_flags(self, free_wake: bool, stretch: bool) -> None:
+        self._ctx.check(self._ctx.lib.fv_hybrid_set_flags(self._h, int(free_wake), int(stretch)))
+
+    def set_precision(self, precision: str) -> None:
+        self._ctx.check(self._ctx.lib.fv_hybrid_set_precision(self._h,
+                                                              _lib.PRECISIONS[precision]))
+
     def pair_counts(self):
         a, b = C.c_double(), C.c_double()
         self._ctx.check(self._ctx.lib.fv_hybrid_pair_counts(self._h, C.byref(a), C.byref(b)))
@@ -155,7 +162,9 @@ def run_case(case: HybridCase, steps: int | None = None, particles: bool = False
         out = {"cl": cl, "cdi": cdi, "gamma": gam, "np": run.particles.np}
         out["seg_pairs"], out["particle_pairs"] = run.pair_counts()
         if particles:
-            out.update(run.particles.state_dict())
+            st = run.particles.state_dict()
+            out.update({"pos": st["pos"], "alpha": st["gamma"], "sigma_p": st["sigma"],
+                        "age": st["age"]})
         return out
     finally:
         run.close()

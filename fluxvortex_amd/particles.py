@@ -149,6 +149,43 @@ class VortexParticleField:
                                                         1 if stretch else 0,
                                                         1 if free_wake else 0))
 
+    # -- learned far-field closure -------------------------------------------
+    def enable_closure(self, mlp=None, cell=2.0, kappa=4.0, check_every=10, tol=1e-3,
+                       train_samples=2048, train_iters=2, lr=1e-3):
+        """Replaces the self sums' far field (cells beyond kappa * cell) by the
+        first-order cell expansion plus the learned residual of `mlp`
+        (a fluxvortex_amd.mlp.ClosureMLP, trained online); checked against the
+        exact sum every `check_every` advects and rejected above `tol`."""
+        self._closure_mlp = mlp        # keep the network alive with the field
+        self._ctx.check(self._ctx.lib.fv_vpm_set_closure(
+            self._h, mlp.handle if mlp is not None else None, float(cell), float(kappa),
+            int(check_every), float(tol), int(train_samples), int(train_iters), float(lr)))
+
+    def disable_closure(self):
+        self._ctx.check(self._ctx.lib.fv_vpm_clear_closure(self._h))
+        self._closure_mlp = None
+
+    def closure_stats(self):
+        e, fe = C.c_double(), C.c_double()
+        ch, rj = C.c_int64(), C.c_int64()
+        act = C.c_int32()
+        self._ctx.check(self._ctx.lib.fv_vpm_closure_stats(self._h, C.byref(e), C.byref(ch),
+                                                           C.byref(rj), C.byref(act),
+                                                           C.byref(fe)))
+        return {"last_err": e.value, "checks": ch.value, "rejections": rj.value,
+                "active": bool(act.value), "far_evals": fe.value}
+
+    def closure_velocity(self, use_mlp=True):
+        """(closure velocity at the particles, its normwise error vs the exact sum)."""
+        out = np.zeros((self.np, 3))
+        err = C.c_double()
+        self._ctx.check(self._ctx.lib.fv_vpm_closure_velocity_host(
+            self._h, 1 if use_mlp else 0, _lib.ptr(out), C.byref(err)))
+        return out, err.value
+
+    def closure_train(self, rounds=1):
+        self._ctx.check(self._ctx.lib.fv_vpm_closure_train(self._h, int(rounds)))
+
     def convert_rings(self, br, fr, fl, bl, strengths, sigma):
         """Appends the leg particles of rings with |Gamma| >= 1e-15 and returns the
         zeroed strengths and the number added

@@ -220,6 +220,62 @@ int fv_hybrid_read(fv_hybrid *h, double *cl, double *cdi, double *gamma);
 fv_vpm *fv_hybrid_particles(fv_hybrid *h);
 /* Pair counts (segment pairs, particle pairs) evaluated since creation. */
 int fv_hybrid_pair_counts(fv_hybrid *h, double *seg_pairs, double *particle_pairs);
+/* Switch the particle-field flags / the arithmetic for the following steps
+ * (HybridSolver(free_vpm=...) and advect_rk3(stretch=...),
+ * experiment_hybrid_panel_particle.py:107,140-143). */
+int fv_hybrid_set_flags(fv_hybrid *h, int32_t free_wake, int32_t stretch);
+int fv_hybrid_set_precision(fv_hybrid *h, int32_t precision);
+
+/* ------------------------------------------ learned far-field closure MLP */
+/* 16 -> 128 -> 128 -> 128 -> 16 SiLU network, BF16 weights / activations on
+ * MFMA with FP32 accumulation; FP32 master parameters in the layout
+ * W1[128][16] b1[128] W2[128][128] b2[128] W3[128][128] b3[128] W4[16][128]
+ * b4[16] (fv_mlp_param_count() floats).  No reference counterpart: the
+ * reference has no learned closure; this is the network of the far-field
+ * closure that replaces beyond-cutoff particle sums (the exact direct sum of
+ * kernel.py:69-190 stays the fallback and the training target). */
+typedef struct fv_mlp fv_mlp;
+int fv_mlp_create(fv_ctx *ctx, fv_mlp **out);
+void fv_mlp_destroy(fv_mlp *m);
+int fv_mlp_param_count(void);
+int fv_mlp_set_params(fv_mlp *m, const float *params);   /* host; resets Adam */
+int fv_mlp_get_params(fv_mlp *m, float *params);         /* host */
+/* X (nb x 16), Y (nb x 16) fp32 device arrays. */
+int fv_mlp_forward_dev(fv_mlp *m, const float *X, int64_t nb, float *Y);
+/* Gradient of L = (1/nb) sum_s sum_j wout_j (Y_sj - T_sj)^2; grad (device,
+ * nullable) receives a copy. */
+int fv_mlp_gradients_dev(fv_mlp *m, const float *X, const float *T, const float *wout,
+                         int64_t nb, float *grad);
+/* Adam (beta 0.9 / 0.999, eps 1e-8, bias-corrected) with grad (device; NULL
+ * = the last computed gradient). */
+int fv_mlp_adam_dev(fv_mlp *m, const float *grad, float lr);
+int fv_mlp_train_step_dev(fv_mlp *m, const float *X, const float *T, const float *wout,
+                          int64_t nb, float lr);
+int fv_mlp_forward_host(fv_mlp *m, const float *X, int64_t nb, float *Y);
+int fv_mlp_gradients_host(fv_mlp *m, const float *X, const float *T, const float *wout,
+                          int64_t nb, float *grad);
+int fv_mlp_adam_host(fv_mlp *m, const float *grad, float lr);
+
+/* Far-field closure of a particle field's self sums (velocity, stretching,
+ * vorticity).  Cells of edge `cell`; pairs of cells whose centres are within
+ * kappa * cell are summed exactly, farther cells through their first-order
+ * expansion (sum alpha, sum alpha (x - xbar)^T) plus the residual predicted by
+ * `mlp` (NULL: expansion only).  Every advect trains the network on
+ * `train_samples` exact far (particle, cell) sums (`train_iters` Adam steps at
+ * `lr`), and every `check_every` advects compares the closure velocity with
+ * the exact direct sum: above `tol` (normwise relative) the closure is
+ * rejected and the direct sums are used until the next check passes.
+ * No reference counterpart (the reference sums directly, kernel.py:69-190). */
+int fv_vpm_set_closure(fv_vpm *f, fv_mlp *mlp, double cell, double kappa, int32_t check_every,
+                       double tol, int32_t train_samples, int32_t train_iters, double lr);
+int fv_vpm_clear_closure(fv_vpm *f);
+int fv_vpm_closure_stats(fv_vpm *f, double *last_err, int64_t *checks, int64_t *rejections,
+                         int32_t *active, double *far_evals);
+/* Closure velocity at the particles (host, n x 3; nullable) with or without
+ * the learned residual and its normwise error against the exact sum. */
+int fv_vpm_closure_velocity_host(fv_vpm *f, int32_t use_mlp, double *out, double *err);
+/* Runs `rounds` online-training rounds on the current particle state. */
+int fv_vpm_closure_train(fv_vpm *f, int32_t rounds);
 
 #ifdef __cplusplus
 }

@@ -302,3 +302,88 @@ def hybrid_run(case, steps=None, threads=0):
     k = npo.value
     return {"cl": cl, "cdi": cdi, "gamma": gam, "np": k, "pos": pos[:k].copy(),
             "gamma_p": al[:k].copy(), "sigma": sg[:k].copy()}
+
+
+# ---------------------------------------------------------------------------
+# Closure MLP (16 -> 128 -> 128 -> 128 -> 16, SiLU) in FP64, optionally with
+# the device's bf16 rounding points emulated (inputs, weights, activations,
+# layer errors); the parameter layout is the device's FP32 master layout.
+# ---------------------------------------------------------------------------
+MLP_SHAPES = [("W1", (128, 16)), ("b1", (128,)), ("W2", (128, 128)), ("b2", (128,)),
+              ("W3", (128, 128)), ("b3", (128,)), ("W4", (16, 128)), ("b4", (16,))]
+MLP_NPARAMS = sum(int(np.prod(s)) for _, s in MLP_SHAPES)
+
+
+def mlp_unpack(flat):
+    out, o = {}, 0
+    for name, shp in MLP_SHAPES:
+        n = int(np.prod(shp))
+        out[name] = np.asarray(flat[o:o + n], dtype=np.float64).reshape(shp)
+        o += n
+    return out
+
+
+def mlp_pack(d):
+    return np.concatenate([np.asarray(d[name], dtype=np.float64).ravel() for name, _ in MLP_SHAPES])
+
+
+def bf16(x):
+    """Round to bfloat16 (nearest even) through float32, returned as float64."""
+    f = np.asarray(x, dtype=np.float32)
+    u = f.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return (u.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+def _silu(z):
+    return z / (1.0 + np.exp(-z))
+
+
+def _dsilu(z):
+    s = 1.0 / (1.0 + np.exp(-z))
+    return s * (1.0 + z * (1.0 - s))
+
+
+def mlp_forward(flat, X, emulate_bf16=True):
+    p = mlp_unpack(flat)
+    r = bf16 if emulate_bf16 else (lambda a: np.asarray(a, dtype=np.float64))
+    h = r(X)
+    cache = {"h0": h}
+    for k in (1, 2, 3):
+        z = h @ r(p[f"W{k}"]).T + p[f"b{k}"]
+        h = r(_silu(z))
+        cache[f"z{k}"], cache[f"h{k}"] = z, h
+    y = h @ r(p["W4"]).T + p["b4"]
+    return y, cache
+
+
+def mlp_gradients(flat, X, T, wout, emulate_bf16=True):
+    """Gradient of L = (1/nb) sum_s sum_j w_j (Y_sj - T_sj)^2 (device layout)."""
+    p = mlp_unpack(flat)
+    r = bf16 if emulate_bf16 else (lambda a: np.asarray(a, dtype=np.float64))
+    y, c = mlp_forward(flat, X, emulate_bf16)
+    nb = X.shape[0]
+    dy = r(2.0 / nb * np.asarray(wout)[None, :] * (y - T))
+    g = {"W4": dy.T @ c["h3"], "b4": dy.sum(axis=0)}
+    e = dy
+    for k in (3, 2, 1):
+        wnext = r(p[f"W{k + 1}"])
+        dz = r((e @ wnext) * _dsilu(c[f"z{k}"]))
+        g[f"W{k}"] = dz.T @ c[f"h{k - 1}"]
+        g[f"b{k}"] = dz.sum(axis=0)
+        e = dz
+    return mlp_pack(g)
+
+
+def mlp_loss(flat, X, T, wout, emulate_bf16=False):
+    y, _ = mlp_forward(flat, X, emulate_bf16)
+    return float(np.sum(np.asarray(wout)[None, :] * (y - T) ** 2) / X.shape[0])
+
+
+def adam_step(params, grad, m, v, step, lr, b1=0.9, b2=0.999, eps=1e-8):
+    """One Adam update (Kingma & Ba 2015, bias-corrected); returns new arrays."""
+    m = b1 * m + (1 - b1) * grad
+    v = b2 * v + (1 - b2) * grad * grad
+    mh = m / (1 - b1 ** step)
+    vh = v / (1 - b2 ** step)
+    return params - lr * mh / (np.sqrt(vh) + eps), m, v

@@ -1,0 +1,100 @@
+"""Learned far-field closure: cell binning, near-field exactness, convergence of
+the first-order far expansion with the cutoff, the online-trained residual
+network's gain over the expansion alone, and the check / rejection logic."""
+import numpy as np
+import pytest
+
+from helpers import relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def wake_sheet(nx=1600, ny=12, dx=0.125, span=10.0, seed=3):
+    """Particles of a heaving wing's wake: a sinusoidal sheet of shed
+    (spanwise) and trailing (streamwise) vorticity, 4 per ring as the
+    conversion produces them."""
+    rng = np.random.default_rng(seed)
+    xs = np.arange(nx) * dx
+    ys = np.linspace(-span / 2, span / 2, ny + 1)
+    z = lambda x: 0.1 * np.sin(2 * np.pi * x / 6.283)
+    pos, al = [], []
+    for i, x in enumerate(xs):
+        g = np.cos(2 * np.pi * x / 6.283) * (1 - (np.linspace(-1, 1, ny) ** 2))
+        for j in range(ny):
+            y0, y1 = ys[j], ys[j + 1]
+            c = np.array([x, 0.5 * (y0 + y1), z(x)])
+            pos.append(c)
+            al.append([0.0, g[j] * (y1 - y0) * 0.05, 0.0])            # shed
+            pos.append(np.array([x + dx / 2, y0, z(x + dx / 2)]))
+            gl = g[j] - (g[j - 1] if j else 0.0)
+            al.append([gl * dx * 0.05, 0.0, 0.0])                      # trailing
+    pos = np.array(pos) + rng.normal(0, 1e-3, (len(pos), 3))
+    return pos, np.array(al), np.full(len(pos), 0.0628)
+
+
+@pytest.fixture(scope="module")
+def field(ctx):
+    from fluxvortex_amd.particles import VortexParticleField
+    pos, al, sg = wake_sheet()
+    f = VortexParticleField(max_particles=len(sg) + 64, rlxf=0.0, kernel="winckelmans")
+    f.add_particles_batch(pos, al, sg)
+    return f, pos, al, sg
+
+
+def test_near_only_is_exact(field, oracle):
+    f, pos, al, sg = field
+    f.enable_closure(None, cell=2.0, kappa=1e3)           # every cell is near
+    u, err = f.closure_velocity(use_mlp=False)
+    assert err < 1e-12, err
+    sub = np.arange(0, len(sg), 97)
+    ref = oracle.particle_vel(pos[sub], pos, al, sg, "winckelmans")
+    assert relerr(u[sub], ref) < 1e-12
+
+
+def test_expansion_converges_with_cutoff(field):
+    f, *_ = field
+    errs = []
+    for kappa in (2.0, 4.0, 8.0):
+        f.enable_closure(None, cell=2.0, kappa=kappa)
+        _, e = f.closure_velocity(use_mlp=False)
+        errs.append(e)
+    print("expansion-only error vs kappa (2, 4, 8):", errs)
+    assert errs[0] > errs[1] > errs[2]
+    assert errs[2] < 2e-3
+
+
+def test_learned_residual_gain(field):
+    from fluxvortex_amd.mlp import ClosureMLP, closure_params
+    f, *_ = field
+    net = ClosureMLP(closure_params(seed=1))
+    f.enable_closure(net, cell=2.0, kappa=4.0, train_samples=4096, train_iters=4, lr=2e-3)
+    _, e0 = f.closure_velocity(use_mlp=False)
+    _, e_untrained = f.closure_velocity(use_mlp=True)   # zero output layer: = expansion
+    assert abs(e_untrained - e0) <= 1e-9 + 1e-6 * e0
+    f.closure_train(rounds=150)
+    _, e1 = f.closure_velocity(use_mlp=True)
+    print(f"closure error: expansion only {e0:.3e}, with trained network {e1:.3e}, "
+          f"gain x{e0 / e1:.2f}")
+    assert e1 < e0
+
+
+def test_advect_with_closure_checks_and_stays_close(ctx, oracle):
+    from fluxvortex_amd.mlp import ClosureMLP, closure_params
+    from fluxvortex_amd.particles import VortexParticleField
+    pos, al, sg = wake_sheet(nx=800)
+    U = np.array([10.0, 0.0, 0.0])
+    a = VortexParticleField(max_particles=len(sg) + 8, rlxf=0.3, kernel="winckelmans")
+    b = VortexParticleField(max_particles=len(sg) + 8, rlxf=0.3, kernel="winckelmans")
+    for f in (a, b):
+        f.add_particles_batch(pos, al, sg)
+    net = ClosureMLP(closure_params(seed=2))
+    b.enable_closure(net, cell=2.0, kappa=6.0, check_every=2, tol=1e-3)
+    for _ in range(4):
+        a.advect_rk3(0.0125, U, stretch=False, free_wake=True)
+        b.advect_rk3(0.0125, U, stretch=False, free_wake=True)
+    st = b.closure_stats()
+    assert st["checks"] == 2 and st["last_err"] >= 0.0
+    if st["active"]:
+        assert st["last_err"] <= 1e-3
+    pa, pb = a.positions, b.positions
+    assert relerr(pb - pos, pa - pos) < 1e-3
