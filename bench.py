@@ -143,7 +143,7 @@ def run_config(hybrid, particles, steps, warmup, precision, stretch=True, closur
         from fluxvortex_amd.mlp import ClosureMLP, closure_params
         net = ClosureMLP(closure_params(seed=0))
         run.particles.enable_closure(net, **closure)
-        run.particles.closure_train(rounds=100)
+        run.particles.closure_train(rounds=200)
     run.set_flags(True, stretch)
     run.step(warmup)
     out = timed(run, steps)
@@ -264,8 +264,8 @@ def main(argv=None):
             bmx = run_config(hybrid, a.big, 3, 1, "mixed")
             big["mixed_steps_per_s"] = 3 / (bmx["ms"] / 1e3)
             log(f"[bench] {a.big} particles, mixed + learned closure")
-            clo = dict(cell=2.0, kappa=4.0, check_every=10, tol=1e-3, train_samples=2048,
-                       train_iters=2, lr=1e-3)
+            clo = dict(cell=2.0, kappa=4.0, check_every=10, tol=1e-3, train_samples=4096,
+                       train_iters=4, lr=2e-3)
             bml = run_config(hybrid, a.big, 3, 1, "mixed", closure=clo)
             big["mixed_learned_steps_per_s"] = 3 / (bml["ms"] / 1e3)
             big["mixed_learned_speedup_vs_fp64"] = b64["ms"] / bml["ms"]

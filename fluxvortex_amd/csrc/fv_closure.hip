@@ -1,23 +1,26 @@
 // fv_closure.hip -- learned far-field closure of the particle sums.
 //
 // The particles are binned into a uniform grid of cells (edge h).  For a
-// target in cell t and a source cell s:
+// target particle in cell t and a source cell s:
 //   near (grid-centre distance <= kappa h): exact pair sums, LDS-tiled;
-//   far: the cell's first-order expansion about its centroid (monopole
-//        Sum alpha and dipole M = Sum alpha (x - xbar)^T, in FP64) plus a
-//        learned residual predicted by the closure MLP (BF16 MFMA) from the
-//        target's relative position and the cell's low-order moments,
-//        evaluated once per (target cell, source cell) at the target cell's
-//        centre and carried to each target with the expansion's gradient.
+//   far: the source cell's first-order expansion about its centroid
+//        (monopole A = Sum alpha, dipole M = Sum alpha (x - xbar)^T, FP64),
+//        evaluated at every target particle;
+//   far cells in the shell kappa h < distance <= 2 kappa h also carry a learned
+//        residual (the quadrupole order and beyond): the closure MLP (BF16 on
+//        MFMA) predicts it from the position relative to the cell and the
+//        cell's normalised first and second moments, once per (target cell,
+//        source cell) pair at the target cell's centre.
 // The MLP is trained online on exact direct sums the solver generates (random
-// far (target, cell) pairs every step), and the closure is checked against the
-// exact direct sum every `check_every` steps: above `tol` (normwise relative
-// error of the particle-induced velocity) it is rejected and the direct sums
-// are used until the next check passes.
+// shell pairs of the current state, one random target particle each), and the
+// closure is checked against the exact direct sum every `check_every` steps:
+// above `tol` (max-norm error of the particle-induced velocity relative to its
+// max norm; a non-finite value counts as above) it is rejected and the direct
+// sums are used until the next check passes.
 //
-// The far field of a cell at distance R >> sigma is kernel independent
-// (g(rho) -> 1 for both smoothing kernels), so one expansion serves the
-// velocity, stretching and vorticity passes.
+// Beyond the cutoff the smoothing is inactive (g(rho) -> 1 for both kernels at
+// rho = R / sigma >> 1), so one singular expansion serves the velocity,
+// stretching and vorticity passes.
 #include <hipcub/hipcub.hpp>
 
 #include <cmath>
@@ -34,7 +37,14 @@ extern "C" int fv_mlp_train_step_dev(fv_mlp *m, const float *X, const float *T,
 
 namespace fv {
 
-constexpr int kMomStride = 20;   // n, xbar(3), A(3), M(9), centre(3), pad
+// Per occupied cell: n, xbar(3), A(3), M(9, row-major M[r][k] = Sum a_r d_k),
+// centre(3), S = Sum |alpha|, L2 = Sum |alpha| |d|^2 / S, Q(18: per component r
+// the symmetric Sum a_r d_k d_l for kl = 00 01 02 11 12 22), pad.
+constexpr int kMomStride = 40;
+constexpr int kMomTile = 19;        // n .. centre: what the far-field pass stages
+constexpr int kFeat = 32;           // MLP input width (31 features + pad)
+constexpr int kOutW = 16;           // MLP output width (3 used)
+constexpr double kShell = 2.0;      // residual shell: kappa h < R <= kShell kappa h
 
 struct fv_closure_state {
     fv_mlp *mlp = nullptr;
@@ -45,17 +55,18 @@ struct fv_closure_state {
     bool active = false;
     int64_t steps = 0, checks = 0, rejections = 0;
     double last_err = -1.0;
-    double near_pairs = 0.0, far_evals = 0.0;
+    double far_evals = 0.0;
     uint64_t seed = 0x9E3779B97F4A7C15ull;
     // geometry of the current build
     int64_t built_epoch = -1;
-    int64_t far_epoch = -1, far_version = -1;
-    int far_mlp = -1;
+    int64_t res_epoch = -1, res_version = -1;
+    int res_mlp = -1;
     int64_t mlp_version = 0;
     int64_t n = 0;                     // particles binned
     double ox = 0, oy = 0, oz = 0, hh = 2.0;
     int nx = 1, ny = 1, nz = 1;
-    int64_t ncell = 0, nocc = 0, maxpop = 0;
+    int64_t ncell = 0, nocc = 0, maxpop = 0, nlist = 0;
+    int skx = -1, sky = -1, skz = -1;  // stencil half-widths in cells
     std::vector<int> stencil;          // (di, dj, dk) triples
     // device buffers
     void *block = nullptr;
@@ -67,12 +78,17 @@ struct fv_closure_state {
     void *cells = nullptr;
     size_t cells_bytes = 0;
     int32_t *cstart = nullptr, *cend = nullptr, *occ = nullptr, *occ_of = nullptr;
-    double *mom = nullptr;             // kMomStride per occupied cell
-    double *faruj = nullptr;           // 12 per occupied cell (u, J)
-    int32_t *dstencil = nullptr;
     void *far = nullptr;
     size_t far_bytes = 0;
-    float *X = nullptr, *Y = nullptr;  // nocc^2 x 16
+    double *mom = nullptr;             // kMomStride per occupied cell
+    double *res = nullptr;             // learned residual velocity, 3 per occupied cell
+    int32_t *lcnt = nullptr, *loff = nullptr;   // shell pairs per target cell, offsets
+    void *list = nullptr;
+    size_t list_bytes = 0;
+    int32_t *lt = nullptr, *ls = nullptr;       // shell pair list (target, source)
+    double *lk = nullptr;              // residual scale per pair
+    float *X = nullptr, *Y = nullptr;  // MLP inputs / outputs per pair
+    int32_t *dstencil = nullptr;
     void *tmp = nullptr;
     size_t tmp_bytes = 0;
     double *bbox = nullptr;            // 6 + per-block partials
@@ -85,6 +101,7 @@ struct fv_closure_state {
 struct fv_closure : fv::fv_closure_state {};
 
 namespace fv {
+
 
 // ---------------------------------------------------------------- kernels
 __global__ void k_bbox_partial(const int64_t *__restrict__ np_dev, const double *__restrict__ x,
@@ -217,13 +234,12 @@ __global__ __launch_bounds__(1024) void k_occupied(int64_t ncell, const int32_t 
 }
 
 // Moments of each occupied cell (one wavefront per cell, fixed order).
-__global__ void k_moments(int64_t nocc, const int32_t *__restrict__ occ,
-                          const int32_t *__restrict__ cstart, const int32_t *__restrict__ cend,
-                          const double *__restrict__ sx, const double *__restrict__ sy,
-                          const double *__restrict__ sz, const double *__restrict__ sax,
-                          const double *__restrict__ say, const double *__restrict__ saz,
-                          double ox, double oy, double oz, double h, int ny, int nz,
-                          double *__restrict__ mom)
+__global__ __launch_bounds__(64) void k_moments(
+    int64_t nocc, const int32_t *__restrict__ occ, const int32_t *__restrict__ cstart,
+    const int32_t *__restrict__ cend, const double *__restrict__ sx, const double *__restrict__ sy,
+    const double *__restrict__ sz, const double *__restrict__ sax, const double *__restrict__ say,
+    const double *__restrict__ saz, double ox, double oy, double oz, double h, int ny, int nz,
+    double *__restrict__ mom)
 {
     const int64_t o = blockIdx.x;
     if (o >= nocc) return;
@@ -243,36 +259,45 @@ __global__ void k_moments(int64_t nocc, const int32_t *__restrict__ occ,
     }
     const double inv = 1.0 / (double)(e - b);
     const double xb = px * inv, yb = py * inv, zb = pz * inv;
-    double A[3] = {0, 0, 0}, M[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    // acc: A(3) M(9) Q(18) S L2
+    double acc[32];
+    for (int k = 0; k < 32; ++k) acc[k] = 0.0;
     for (int i = b + l; i < e; i += 64) {
         const double a[3] = {sax[i], say[i], saz[i]};
         const double d[3] = {sx[i] - xb, sy[i] - yb, sz[i] - zb};
+        const double am = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+        const double dd[6] = {d[0] * d[0], d[0] * d[1], d[0] * d[2],
+                              d[1] * d[1], d[1] * d[2], d[2] * d[2]};
         for (int r = 0; r < 3; ++r) {
-            A[r] += a[r];
-            for (int k = 0; k < 3; ++k) M[3 * r + k] += a[r] * d[k];
+            acc[r] += a[r];
+            for (int k = 0; k < 3; ++k) acc[3 + 3 * r + k] += a[r] * d[k];
+            for (int q = 0; q < 6; ++q) acc[12 + 6 * r + q] += a[r] * dd[q];
         }
+        acc[30] += am;
+        acc[31] += am * (dd[0] + dd[3] + dd[5]);
     }
-    for (int off = 32; off > 0; off >>= 1) {
-        for (int r = 0; r < 3; ++r) A[r] += __shfl_xor(A[r], off, 64);
-        for (int r = 0; r < 9; ++r) M[r] += __shfl_xor(M[r], off, 64);
-    }
+    for (int k = 0; k < 32; ++k)
+        for (int off = 32; off > 0; off >>= 1) acc[k] += __shfl_xor(acc[k], off, 64);
     if (l == 0) {
         double *m = mom + o * kMomStride;
         m[0] = (double)(e - b);
         m[1] = xb; m[2] = yb; m[3] = zb;
-        for (int r = 0; r < 3; ++r) m[4 + r] = A[r];
-        for (int r = 0; r < 9; ++r) m[7 + r] = M[r];
+        for (int r = 0; r < 12; ++r) m[4 + r] = acc[r];
         const int ix = c / (ny * nz), iy = (c / nz) % ny, iz = c % nz;
         m[16] = ox + (ix + 0.5) * h;
         m[17] = oy + (iy + 0.5) * h;
         m[18] = oz + (iz + 0.5) * h;
-        m[19] = 0.0;
+        m[19] = acc[30];
+        m[20] = acc[30] > 0.0 ? acc[31] / acc[30] : 0.0;
+        for (int q = 0; q < 18; ++q) m[21 + q] = acc[12 + q];
+        m[39] = 0.0;
     }
 }
 
 // First-order (monopole + dipole) far field of a cell at point x, FP64:
 //   u = -(1/4pi) d x A / R^3 + (1/4pi) [ e / R^3 - 3 d x (M d) / R^5 ],
-//   e_i = eps_ikl M_lk,  d = x - xbar;  J = du/dx.
+//   e_i = eps_ikl M_lk,  d = x - xbar;  J = du/dx (when J is non-null).
+// m holds n, xbar(3), A(3), M(9) at offsets 0..15.
 __device__ void far_expansion(const double *m, double x0, double x1, double x2, double *u,
                               double *J)
 {
@@ -282,7 +307,7 @@ __device__ void far_expansion(const double *m, double x0, double x1, double x2, 
     const double iR2 = 1.0 / R2;
     const double iR = sqrt(iR2);
     const double iR3 = iR * iR2, iR5 = iR3 * iR2, iR7 = iR5 * iR2;
-    const double c = 0.07957747154594767;   // 1/(4 pi)
+    const double c = kInv4Pi;
     const double dA[3] = {d[1] * A[2] - d[2] * A[1], d[2] * A[0] - d[0] * A[2],
                           d[0] * A[1] - d[1] * A[0]};
     const double e[3] = {M[7] - M[5], M[2] - M[6], M[3] - M[1]};
@@ -291,12 +316,10 @@ __device__ void far_expansion(const double *m, double x0, double x1, double x2, 
                           M[6] * d[0] + M[7] * d[1] + M[8] * d[2]};
     const double dMd[3] = {d[1] * Md[2] - d[2] * Md[1], d[2] * Md[0] - d[0] * Md[2],
                            d[0] * Md[1] - d[1] * Md[0]};
-    for (int i = 0; i < 3; ++i) u[i] = c * (-dA[i] * iR3 + e[i] * iR3 - 3.0 * dMd[i] * iR5);
+    for (int i = 0; i < 3; ++i) u[i] += c * (-dA[i] * iR3 + e[i] * iR3 - 3.0 * dMd[i] * iR5);
     if (!J) return;
-    // d/dd_k of each term
     for (int i = 0; i < 3; ++i)
         for (int k = 0; k < 3; ++k) {
-            // eps_ikl A_l
             const int i1 = (i + 1) % 3, i2 = (i + 2) % 3;
             double epsA = 0.0;   // eps_{i k l} A_l
             if (k == i1) epsA = A[i2];
@@ -312,85 +335,124 @@ __device__ void far_expansion(const double *m, double x0, double x1, double x2, 
                                : (i == 1) ? d[2] * colk[0] - d[0] * colk[2]
                                           : d[0] * colk[1] - d[1] * colk[0];
             const double t2 = -3.0 * ((epsMd + dxc) * iR5 - 5.0 * dMd[i] * d[k] * iR7);
-            J[3 * i + k] = c * (t0 + t1 + t2);
+            J[3 * i + k] += c * (t0 + t1 + t2);
         }
 }
 
-// MLP features of a (point, cell) pair and its normalisation scale:
-//   [d/R (3), h/R, A/s (3), M/(s h) (9)],  s = |A| + |M|_F / h.
-// The MLP output (3) is the residual (u_exact - u_expansion) * 4 pi R^2 / s.
+// MLP features of a (point, cell) pair:
+//   [d/R (3), L/R, M/(S L) (9), Q/(S L^2) (18), 0]
+// with S = Sum |alpha| and L^2 = max(Sum |alpha| |d|^2 / S, 1e-6 h^2); every
+// moment feature lies in [-1, 1] (Cauchy-Schwarz).  Returns the residual scale
+// k = S L^2 / (4 pi R^4): the network's output y (3) stands for the velocity
+// residual k y.  A cell with S = 0 gets zero features and k = 0.
 __device__ double far_features(const double *m, double x0, double x1, double x2, double h,
-                               float *f, double *R_out)
+                               float *f)
 {
     const double d[3] = {x0 - m[1], x1 - m[2], x2 - m[3]};
-    const double R = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-    const double *A = m + 4, *M = m + 7;
-    double mf = 0.0;
-    for (int r = 0; r < 9; ++r) mf += M[r] * M[r];
-    const double s = sqrt(A[0] * A[0] + A[1] * A[1] + A[2] * A[2]) + sqrt(mf) / h + 1e-300;
+    const double R2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    const double R = sqrt(R2);
+    const double S = m[19];
+    if (!(S > 0.0)) {
+        for (int k = 0; k < kFeat; ++k) f[k] = 0.0f;
+        return 0.0;
+    }
+    const double L2 = fmax(m[20], 1e-6 * h * h);
+    const double L = sqrt(L2);
     for (int c = 0; c < 3; ++c) f[c] = (float)(d[c] / R);
-    f[3] = (float)(h / R);
-    for (int c = 0; c < 3; ++c) f[4 + c] = (float)(A[c] / s);
-    for (int r = 0; r < 9; ++r) f[7 + r] = (float)(M[r] / (s * h));
-    *R_out = R;
-    return s;
+    f[3] = (float)(L / R);
+    const double sm = 1.0 / (S * L), sq = 1.0 / (S * L2);
+    for (int r = 0; r < 9; ++r) f[4 + r] = (float)(m[7 + r] * sm);
+    for (int q = 0; q < 18; ++q) f[13 + q] = (float)(m[21 + q] * sq);
+    f[31] = 0.0f;
+    return S * L2 * kInv4Pi / (R2 * R2);
 }
 
-__device__ __forceinline__ bool cells_far(const double *mt, const double *ms, double rc2)
+__device__ __forceinline__ double centre_dist2(const double *mt, const double *ms)
 {
     const double dx = mt[16] - ms[16], dy = mt[17] - ms[17], dz = mt[18] - ms[18];
-    return dx * dx + dy * dy + dz * dz > rc2;
+    return dx * dx + dy * dy + dz * dz;
 }
 
-// MLP inputs for every (target cell, source cell) pair; near pairs get zeros.
-__global__ void k_far_inputs(int64_t nocc, const double *__restrict__ mom, double h, double rc2,
-                             float *__restrict__ X)
-{
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= nocc * nocc) return;
-    const int64_t t = p / nocc, s = p - t * nocc;
-    const double *mt = mom + t * kMomStride, *ms = mom + s * kMomStride;
-    float f[16];
-    if (cells_far(mt, ms, rc2)) {
-        double R;
-        far_features(ms, mt[16], mt[17], mt[18], h, f, &R);
-    } else {
-        for (int k = 0; k < 16; ++k) f[k] = 0.0f;
-    }
-    for (int k = 0; k < 16; ++k) X[p * 16 + k] = f[k];
-}
-
-// Far field (u, J) at each target cell's centre: one wavefront per target
-// cell, lanes over source cells, fixed-order wave reduction.
-__global__ __launch_bounds__(64) void k_far_reduce(int64_t nocc, const double *__restrict__ mom,
-                                                   double h, double rc2, const float *__restrict__ Y,
-                                                   int use_mlp, double *__restrict__ faruj)
+// Shell pairs per target cell (one wavefront per occupied target cell).
+__global__ __launch_bounds__(64) void k_shell_count(int64_t nocc, const double *__restrict__ mom,
+                                                    double rc2, double rm2,
+                                                    int32_t *__restrict__ cnt)
 {
     const int64_t t = blockIdx.x;
     if (t >= nocc) return;
     const double *mt = mom + t * kMomStride;
-    double acc[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    int c = 0;
     for (int64_t s = threadIdx.x; s < nocc; s += 64) {
-        const double *ms = mom + s * kMomStride;
-        if (!cells_far(mt, ms, rc2)) continue;
-        double u[3], J[9];
-        far_expansion(ms, mt[16], mt[17], mt[18], u, J);
-        if (use_mlp) {
-            float f[16];
-            double R;
-            const double sc = far_features(ms, mt[16], mt[17], mt[18], h, f, &R);
-            const double k = sc * 0.07957747154594767 / (R * R);
-            const float *y = Y + (t * nocc + s) * 16;
-            for (int c = 0; c < 3; ++c) u[c] += k * (double)y[c];
-        }
-        for (int c = 0; c < 3; ++c) acc[c] += u[c];
-        for (int c = 0; c < 9; ++c) acc[3 + c] += J[c];
+        const double d2 = centre_dist2(mt, mom + s * kMomStride);
+        c += (d2 > rc2 && d2 <= rm2) ? 1 : 0;
     }
-    for (int c = 0; c < 12; ++c)
-        for (int off = 32; off > 0; off >>= 1) acc[c] += __shfl_xor(acc[c], off, 64);
-    if (threadIdx.x == 0)
-        for (int c = 0; c < 12; ++c) faruj[t * 12 + c] = acc[c];
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+    if (threadIdx.x == 0) cnt[t] = c;
 }
+
+// Ordered fill of the shell pair list (source order within each target cell).
+__global__ __launch_bounds__(64) void k_shell_fill(int64_t nocc, const double *__restrict__ mom,
+                                                   double rc2, double rm2,
+                                                   const int32_t *__restrict__ off,
+                                                   int32_t *__restrict__ lt,
+                                                   int32_t *__restrict__ ls)
+{
+    const int64_t t = blockIdx.x;
+    if (t >= nocc) return;
+    const double *mt = mom + t * kMomStride;
+    const int lane = threadIdx.x;
+    const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    int64_t base = off[t];
+    for (int64_t s0 = 0; s0 < nocc; s0 += 64) {
+        const int64_t s = s0 + lane;
+        bool in = false;
+        if (s < nocc) {
+            const double d2 = centre_dist2(mt, mom + s * kMomStride);
+            in = d2 > rc2 && d2 <= rm2;
+        }
+        const unsigned long long mask = __ballot(in);
+        if (in) {
+            const int64_t p = base + __popcll(mask & below);
+            lt[p] = (int32_t)t;
+            ls[p] = (int32_t)s;
+        }
+        base += __popcll(mask);
+    }
+}
+
+// MLP inputs and residual scales of the shell pairs, at the target cell's centre.
+__global__ void k_shell_inputs(int64_t nlist, const int32_t *__restrict__ lt,
+                               const int32_t *__restrict__ ls, const double *__restrict__ mom,
+                               double h, float *__restrict__ X, double *__restrict__ lk)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nlist) return;
+    const double *mt = mom + (int64_t)lt[e] * kMomStride, *ms = mom + (int64_t)ls[e] * kMomStride;
+    float f[kFeat];
+    lk[e] = far_features(ms, mt[16], mt[17], mt[18], h, f);
+    for (int k = 0; k < kFeat; ++k) X[e * kFeat + k] = f[k];
+}
+
+// Learned residual velocity per target cell: Sum over its shell pairs of k y
+// (one wavefront per target cell, fixed order).
+__global__ __launch_bounds__(64) void k_shell_reduce(int64_t nocc, const int32_t *__restrict__ off,
+                                                     const double *__restrict__ lk,
+                                                     const float *__restrict__ Y,
+                                                     double *__restrict__ res)
+{
+    const int64_t t = blockIdx.x;
+    if (t >= nocc) return;
+    double a[3] = {0.0, 0.0, 0.0};
+    for (int64_t e = off[t] + threadIdx.x; e < off[t + 1]; e += 64) {
+        const double k = lk[e];
+        for (int c = 0; c < 3; ++c) a[c] += k * (double)Y[e * kOutW + c];
+    }
+    for (int c = 0; c < 3; ++c)
+        for (int o = 32; o > 0; o >>= 1) a[c] += __shfl_xor(a[c], o, 64);
+    if (threadIdx.x == 0)
+        for (int c = 0; c < 3; ++c) res[3 * t + c] = a[c];
+}
+
 
 // Near-field pair sums in sorted order: one workgroup per (occupied target
 // cell, 64-target chunk); sources = particles of the stencil cells.
@@ -446,26 +508,48 @@ __global__ __launch_bounds__(64) void k_near(int64_t nocc, const int32_t *__rest
         for (int k = 0; k < 3; ++k) out[3 * (int64_t)ti + k] = acc.v[k];
 }
 
-// Adds the far field and scatters to the original particle order.
-//   mode 0: u = near + u_c + J_c (x - centre);  2: S = near + J_c alpha;
-//   3: omega = near + curl(J_c).
-__global__ void k_apply(int64_t n, int mode, const int32_t *__restrict__ key_s,
-                        const int32_t *__restrict__ occ_of, const int32_t *__restrict__ perm,
-                        const double *__restrict__ sx, const double *__restrict__ sy,
-                        const double *__restrict__ sz, const double *__restrict__ sax,
-                        const double *__restrict__ say, const double *__restrict__ saz,
-                        const double *__restrict__ mom, const double *__restrict__ faruj,
-                        const double *__restrict__ near, double *__restrict__ out)
+
+// Far field of every far cell at each particle (sorted order) plus the near
+// sums and (mode 0) the learned residual of its cell; scatters to the original
+// order.  mode 0: velocity; 2: stretching J alpha; 3: vorticity curl u.
+// One particle per lane, source-cell moments staged in LDS.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_far_apply(
+    int64_t n, int64_t nocc, const int32_t *__restrict__ key_s, const int32_t *__restrict__ occ_of,
+    const int32_t *__restrict__ perm, const double *__restrict__ sx, const double *__restrict__ sy,
+    const double *__restrict__ sz, const double *__restrict__ sax, const double *__restrict__ say,
+    const double *__restrict__ saz, const double *__restrict__ mom, double rc2,
+    const double *__restrict__ res, const double *__restrict__ near, double *__restrict__ out)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int64_t o = occ_of[key_s[i]];
-    const double *F = faruj + o * 12, *J = F + 3, *m = mom + o * kMomStride;
+    __shared__ double tile[256 * kMomTile];
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const bool active = i < n;
+    const int64_t il = active ? i : n - 1;
+    const int64_t t = occ_of[key_s[il]];
+    const double *mt = mom + t * kMomStride;
+    const double c0 = mt[16], c1 = mt[17], c2 = mt[18];
+    const double x0 = sx[il], x1 = sy[il], x2 = sz[il];
+    double u[3] = {0.0, 0.0, 0.0}, J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int64_t s0 = 0; s0 < nocc; s0 += 256) {
+        const int cnt = (int)((nocc - s0) < 256 ? (nocc - s0) : 256);
+        __syncthreads();
+        for (int e = threadIdx.x; e < cnt * kMomTile; e += 256) {
+            const int k = e / kMomTile, q = e - k * kMomTile;
+            tile[e] = mom[(s0 + k) * kMomStride + q];
+        }
+        __syncthreads();
+        for (int k = 0; k < cnt; ++k) {
+            const double *ms = tile + k * kMomTile;
+            const double dx = c0 - ms[16], dy = c1 - ms[17], dz = c2 - ms[18];
+            if (dx * dx + dy * dy + dz * dz <= rc2) continue;
+            far_expansion(ms, x0, x1, x2, u, MODE == 0 ? nullptr : J);
+        }
+    }
+    if (!active) return;
     double r[3];
-    if (mode == 0) {
-        const double d[3] = {sx[i] - m[16], sy[i] - m[17], sz[i] - m[18]};
-        for (int a = 0; a < 3; ++a) r[a] = F[a] + J[3 * a] * d[0] + J[3 * a + 1] * d[1] + J[3 * a + 2] * d[2];
-    } else if (mode == 2) {
+    if (MODE == 0) {
+        for (int a = 0; a < 3; ++a) r[a] = u[a] + (res ? res[3 * t + a] : 0.0);
+    } else if (MODE == 2) {
         const double al[3] = {sax[i], say[i], saz[i]};
         for (int a = 0; a < 3; ++a) r[a] = J[3 * a] * al[0] + J[3 * a + 1] * al[1] + J[3 * a + 2] * al[2];
     } else {
@@ -473,8 +557,8 @@ __global__ void k_apply(int64_t n, int mode, const int32_t *__restrict__ key_s,
         r[1] = J[2] - J[6];
         r[2] = J[3] - J[1];
     }
-    const int32_t j = perm[i];
-    for (int a = 0; a < 3; ++a) out[3 * (int64_t)j + a] = near[3 * i + a] + r[a];
+    const int64_t j = perm[i];
+    for (int a = 0; a < 3; ++a) out[3 * j + a] = near[3 * i + a] + r[a];
 }
 
 __device__ __forceinline__ uint64_t mix64(uint64_t z)
@@ -485,48 +569,38 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z)
     return z ^ (z >> 31);
 }
 
-// Training samples: random (particle, far cell) pairs; exact far-cell velocity
-// by direct sum (one wavefront per sample); residual target normalised.
+// Training samples: a random shell pair and a random particle of its target
+// cell; the exact velocity of the source cell's particles at that particle
+// (singular kernel, direct sum, one wavefront per sample) minus the cell's
+// expansion, divided by the residual scale.  Unusable samples (zero scale,
+// non-finite or |target| > 1e4) are zeroed.
 __global__ __launch_bounds__(64) void k_train_samples(
-    int64_t n, int64_t nocc, const int32_t *__restrict__ occ, const int32_t *__restrict__ occ_of,
-    const int32_t *__restrict__ key_s, const int32_t *__restrict__ cstart,
+    int64_t nlist, const int32_t *__restrict__ lt, const int32_t *__restrict__ ls,
+    const int32_t *__restrict__ occ, const int32_t *__restrict__ cstart,
     const int32_t *__restrict__ cend, const double *__restrict__ sx, const double *__restrict__ sy,
     const double *__restrict__ sz, const double *__restrict__ sax, const double *__restrict__ say,
-    const double *__restrict__ saz, const double *__restrict__ mom, double h, double rc2,
-    uint64_t seed, float *__restrict__ X, float *__restrict__ T)
+    const double *__restrict__ saz, const double *__restrict__ mom, double h, uint64_t seed,
+    float *__restrict__ X, float *__restrict__ T)
 {
     const int64_t k = blockIdx.x;
     __shared__ int64_t pick[2];
     if (threadIdx.x == 0) {
-        const int64_t t = (int64_t)(mix64(seed ^ (2 * k + 1)) % (uint64_t)n);
-        const double *mt = mom + (int64_t)occ_of[key_s[t]] * kMomStride;
-        int64_t s = -1;
-        for (int tries = 0; tries < 64 && s < 0; ++tries) {
-            const int64_t cand = (int64_t)(mix64(seed ^ (2 * k + 2) ^ ((uint64_t)tries << 40)) %
-                                           (uint64_t)nocc);
-            if (cells_far(mt, mom + cand * kMomStride, rc2)) s = cand;
-        }
-        pick[0] = t;
-        pick[1] = s;
+        const int64_t e = (int64_t)(mix64(seed ^ (2 * k + 1)) % (uint64_t)nlist);
+        const int ct = occ[lt[e]];
+        const int pop = cend[ct] - cstart[ct];
+        pick[0] = cstart[ct] + (int64_t)(mix64(seed ^ (2 * k + 2)) % (uint64_t)pop);
+        pick[1] = ls[e];
     }
     __syncthreads();
-    const int64_t t = pick[0], s = pick[1];
-    if (s < 0) {
-        // no far cell: a zero sample (all-zero inputs, zero target)
-        if (threadIdx.x < 16) {
-            X[k * 16 + threadIdx.x] = 0.0f;
-            T[k * 16 + threadIdx.x] = 0.0f;
-        }
-        return;
-    }
-    const double x0 = sx[t], x1 = sy[t], x2 = sz[t];
+    const int64_t p = pick[0], s = pick[1];
+    const double x0 = sx[p], x1 = sy[p], x2 = sz[p];
     const int c = occ[s];
     double u[3] = {0, 0, 0};
     for (int j = cstart[c] + threadIdx.x; j < cend[c]; j += 64) {
         const double d[3] = {x0 - sx[j], x1 - sy[j], x2 - sz[j]};
         const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
         const double ir = 1.0 / sqrt(r2);
-        const double k3 = -0.07957747154594767 * ir * ir * ir;
+        const double k3 = -kInv4Pi * ir * ir * ir;
         u[0] += k3 * (d[1] * saz[j] - d[2] * say[j]);
         u[1] += k3 * (d[2] * sax[j] - d[0] * saz[j]);
         u[2] += k3 * (d[0] * say[j] - d[1] * sax[j]);
@@ -535,19 +609,23 @@ __global__ __launch_bounds__(64) void k_train_samples(
         for (int off = 32; off > 0; off >>= 1) u[a] += __shfl_xor(u[a], off, 64);
     if (threadIdx.x == 0) {
         const double *ms = mom + s * kMomStride;
-        float f[16];
-        double R;
-        const double sc = far_features(ms, x0, x1, x2, h, f, &R);
-        double ua[3];
+        float f[kFeat];
+        const double sc = far_features(ms, x0, x1, x2, h, f);
+        double ua[3] = {0.0, 0.0, 0.0};
         far_expansion(ms, x0, x1, x2, ua, nullptr);
-        const double k = R * R / (sc * 0.07957747154594767);
-        for (int a = 0; a < 16; ++a) X[blockIdx.x * 16 + a] = f[a];
-        for (int a = 0; a < 16; ++a) T[blockIdx.x * 16 + a] = 0.0f;
-        for (int a = 0; a < 3; ++a) T[blockIdx.x * 16 + a] = (float)((u[a] - ua[a]) * k);
+        double tv[3];
+        bool ok = sc > 0.0;
+        for (int a = 0; a < 3; ++a) {
+            tv[a] = ok ? (u[a] - ua[a]) / sc : 0.0;
+            ok = ok && isfinite(tv[a]) && fabs(tv[a]) <= 1e4;
+        }
+        for (int a = 0; a < kFeat; ++a) X[k * kFeat + a] = ok ? f[a] : 0.0f;
+        for (int a = 0; a < kOutW; ++a) T[k * kOutW + a] = (ok && a < 3) ? (float)tv[a] : 0.0f;
     }
 }
 
-// max |a - b| and max |b| over n x 3 (one workgroup, fixed order).
+// max |a - b| and max |b| over n x 3 (one workgroup, fixed order); a
+// non-finite a makes the first +inf.
 __global__ __launch_bounds__(256) void k_maxdiff(int64_t n, const double *__restrict__ a,
                                                  const double *__restrict__ b,
                                                  double *__restrict__ out)
@@ -555,7 +633,8 @@ __global__ __launch_bounds__(256) void k_maxdiff(int64_t n, const double *__rest
     __shared__ double s0[256], s1[256];
     double m0 = 0.0, m1 = 0.0;
     for (int64_t i = threadIdx.x; i < 3 * n; i += 256) {
-        m0 = fmax(m0, fabs(a[i] - b[i]));
+        const double d = fabs(a[i] - b[i]);
+        m0 = isfinite(d) ? fmax(m0, d) : INFINITY;
         m1 = fmax(m1, fabs(b[i]));
     }
     s0[threadIdx.x] = m0;
@@ -575,6 +654,7 @@ __global__ __launch_bounds__(256) void k_maxdiff(int64_t n, const double *__rest
 }
 
 // ------------------------------------------------------------- host side
+
 static int grow(fv_ctx *ctx, void **p, size_t *cap, size_t bytes)
 {
     if (*cap >= bytes) return FV_OK;
@@ -590,38 +670,58 @@ static int grow(fv_ctx *ctx, void **p, size_t *cap, size_t bytes)
     return FV_OK;
 }
 
-static void make_stencil(fv_closure *cl)
+// Near stencil: cell offsets whose grid-centre distance is <= kappa h, each
+// axis clamped to the grid (offsets beyond it can never land on a cell).
+static int update_stencil(fv_closure *cl, fv_ctx *ctx)
 {
-    cl->stencil.clear();
     const int K = (int)ceil(cl->kappa);
+    const int kx = K < cl->nx - 1 ? K : cl->nx - 1, ky = K < cl->ny - 1 ? K : cl->ny - 1,
+              kz = K < cl->nz - 1 ? K : cl->nz - 1;
+    if (cl->dstencil && kx == cl->skx && ky == cl->sky && kz == cl->skz) return FV_OK;
+    cl->stencil.clear();
     const double k2 = cl->kappa * cl->kappa;
-    for (int i = -K; i <= K; ++i)
-        for (int j = -K; j <= K; ++j)
-            for (int k = -K; k <= K; ++k)
-                if ((double)(i * i + j * j + k * k) <= k2 + 1e-9) {
+    for (int i = -kx; i <= kx; ++i)
+        for (int j = -ky; j <= ky; ++j)
+            for (int k = -kz; k <= kz; ++k)
+                if ((double)i * i + (double)j * j + (double)k * k <= k2 + 1e-9) {
                     cl->stencil.push_back(i);
                     cl->stencil.push_back(j);
                     cl->stencil.push_back(k);
                 }
+    if (cl->dstencil) {
+        FV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        FV_HIP_CHECK(ctx, hipFree(cl->dstencil));
+        cl->dstencil = nullptr;
+    }
+    FV_HIP_CHECK(ctx, hipMalloc(&cl->dstencil, sizeof(int32_t) * cl->stencil.size()));
+    FV_HIP_CHECK(ctx, hipMemcpyAsync(cl->dstencil, cl->stencil.data(),
+                                     sizeof(int32_t) * cl->stencil.size(), hipMemcpyHostToDevice,
+                                     ctx->stream));
+    FV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    cl->skx = kx;
+    cl->sky = ky;
+    cl->skz = kz;
+    return FV_OK;
 }
 
-// Bins the particles and computes the cell moments and the far field.
-static int closure_far(fv_vpm *f, bool use_mlp);
+static int closure_residual(fv_vpm *f, bool use_mlp);
 
-// Bins the particles (cells, moments) for the current state and, when
-// `with_far`, computes the far field.
-static int closure_build(fv_vpm *f, bool use_mlp, bool with_far = true)
+// Bins the particles (cells, moments, shell pair list) for the current state
+// and, when `with_res`, computes the learned residual.
+static int closure_build(fv_vpm *f, bool use_mlp, bool with_res = true)
 {
     fv_closure *cl = f->closure;
     fv_ctx *ctx = f->ctx;
-    if (cl->built_epoch == f->state_epoch) return with_far ? closure_far(f, use_mlp) : FV_OK;
+    if (cl->built_epoch == f->state_epoch) return with_res ? closure_residual(f, use_mlp) : FV_OK;
     int rc = f->np_exact ? FV_OK : vpm_sync_count(f);
     if (rc) return rc;
     const int64_t n = f->np_host;
     cl->n = n;
     if (n == 0) {
         cl->nocc = 0;
+        cl->nlist = 0;
         cl->built_epoch = f->state_epoch;
+        cl->res_epoch = -1;
         return FV_OK;
     }
     // bounding box
@@ -645,13 +745,15 @@ static int closure_build(fv_vpm *f, bool use_mlp, bool with_far = true)
     // grid: cell edge h, grown until the grid and the occupied set stay bounded
     double h = cl->h;
     for (;;) {
-        cl->nx = (int)floor((bb[3] - bb[0]) / h) + 1;
-        cl->ny = (int)floor((bb[4] - bb[1]) / h) + 1;
-        cl->nz = (int)floor((bb[5] - bb[2]) / h) + 1;
-        const double cells = (double)cl->nx * cl->ny * cl->nz;
-        if (cells <= 262144.0 && (cells <= (double)cl->max_cells || (double)n / 4.0 <= 1.0 ||
-                                  cells <= (double)cl->max_cells * 8.0))
+        const double gx = floor((bb[3] - bb[0]) / h) + 1, gy = floor((bb[4] - bb[1]) / h) + 1,
+                     gz = floor((bb[5] - bb[2]) / h) + 1;
+        const double cells = gx * gy * gz;
+        if (cells <= (double)cl->max_cells * 8.0 && cells <= 262144.0) {
+            cl->nx = (int)gx;
+            cl->ny = (int)gy;
+            cl->nz = (int)gz;
             break;
+        }
         h *= 1.25;
     }
     cl->hh = h;
@@ -713,62 +815,91 @@ static int closure_build(fv_vpm *f, bool use_mlp, bool with_far = true)
     const int64_t nocc = occinfo[0];
     cl->nocc = nocc;
     cl->maxpop = occinfo[1];
-    // moments + far field
-    const size_t fneed = (size_t)nocc * (kMomStride + 12) * sizeof(double) +
-                         (size_t)nocc * nocc * 32 * sizeof(float) + 4096;
+    // moments, residual per cell, shell counts / offsets
+    const size_t fneed = (size_t)nocc * (kMomStride + 3) * sizeof(double) +
+                         (size_t)(2 * nocc + 2) * sizeof(int32_t) + 1024;
     rc = grow(ctx, &cl->far, &cl->far_bytes, fneed);
     if (rc) return rc;
     cl->mom = static_cast<double *>(cl->far);
-    cl->faruj = cl->mom + nocc * kMomStride;
-    cl->X = reinterpret_cast<float *>(cl->faruj + nocc * 12);
-    cl->Y = cl->X + nocc * nocc * 16;
+    cl->res = cl->mom + nocc * kMomStride;
+    cl->lcnt = reinterpret_cast<int32_t *>(cl->res + 3 * nocc);
+    cl->loff = cl->lcnt + (nocc + 1);
     hipLaunchKernelGGL(k_moments, dim3((unsigned)nocc), dim3(64), 0, ctx->stream, nocc, cl->occ,
                        cl->cstart, cl->cend, cl->sx, cl->sy, cl->sz, cl->sax, cl->say, cl->saz,
                        cl->ox, cl->oy, cl->oz, h, cl->ny, cl->nz, cl->mom);
     FV_LAUNCH_CHECK(ctx);
-    // stencil of near cells (grid-centre distance <= kappa h), fixed per closure
-    if (!cl->dstencil) {
-        make_stencil(cl);
-        FV_HIP_CHECK(ctx, hipMalloc(&cl->dstencil, sizeof(int32_t) * cl->stencil.size()));
-        FV_HIP_CHECK(ctx, hipMemcpyAsync(cl->dstencil, cl->stencil.data(),
-                                         sizeof(int32_t) * cl->stencil.size(),
-                                         hipMemcpyHostToDevice, ctx->stream));
+    // shell pair list (geometry only)
+    const double rc2 = (cl->kappa * h) * (cl->kappa * h) * (1.0 + 1e-12);
+    const double rm2 = rc2 * kShell * kShell;
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, ctx->stream, cl->lcnt + nocc, (int64_t)1, 0);
+    FV_LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_shell_count, dim3((unsigned)nocc), dim3(64), 0, ctx->stream, nocc, cl->mom,
+                       rc2, rm2, cl->lcnt);
+    FV_LAUNCH_CHECK(ctx);
+    size_t sb = 0;
+    FV_HIP_CHECK(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, sb, cl->lcnt, cl->loff,
+                                                      (int)(nocc + 1), ctx->stream));
+    rc = grow(ctx, &cl->tmp, &cl->tmp_bytes, sb);
+    if (rc) return rc;
+    FV_HIP_CHECK(ctx, hipcub::DeviceScan::ExclusiveSum(cl->tmp, sb, cl->lcnt, cl->loff,
+                                                      (int)(nocc + 1), ctx->stream));
+    int32_t nl = 0;
+    FV_HIP_CHECK(ctx, hipMemcpyAsync(&nl, cl->loff + nocc, sizeof nl, hipMemcpyDeviceToHost,
+                                     ctx->stream));
+    FV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    cl->nlist = nl;
+    if (nl > 0) {
+        const size_t lneed = (size_t)nl * (2 * sizeof(int32_t) + sizeof(double) +
+                                           (kFeat + kOutW) * sizeof(float)) + 1024;
+        rc = grow(ctx, &cl->list, &cl->list_bytes, lneed);
+        if (rc) return rc;
+        cl->lk = static_cast<double *>(cl->list);
+        cl->X = reinterpret_cast<float *>(cl->lk + nl);
+        cl->Y = cl->X + (size_t)nl * kFeat;
+        cl->lt = reinterpret_cast<int32_t *>(cl->Y + (size_t)nl * kOutW);
+        cl->ls = cl->lt + nl;
+        hipLaunchKernelGGL(k_shell_fill, dim3((unsigned)nocc), dim3(64), 0, ctx->stream, nocc,
+                           cl->mom, rc2, rm2, cl->loff, cl->lt, cl->ls);
+        FV_LAUNCH_CHECK(ctx);
     }
+    rc = update_stencil(cl, ctx);
+    if (rc) return rc;
     cl->built_epoch = f->state_epoch;
-    cl->far_epoch = -1;
-    return with_far ? closure_far(f, use_mlp) : FV_OK;
+    cl->res_epoch = -1;
+    return with_res ? closure_residual(f, use_mlp) : FV_OK;
 }
 
-// Far field (u, J) at every occupied cell's centre from all far cells.
-static int closure_far(fv_vpm *f, bool use_mlp)
+// Learned residual per occupied cell (zero without a network or shell pairs).
+static int closure_residual(fv_vpm *f, bool use_mlp)
 {
     fv_closure *cl = f->closure;
     fv_ctx *ctx = f->ctx;
-    use_mlp = use_mlp && cl->mlp;
-    if (cl->far_epoch == cl->built_epoch && cl->far_mlp == (int)use_mlp &&
-        (!use_mlp || cl->far_version == cl->mlp_version))
+    use_mlp = use_mlp && cl->mlp && cl->nlist > 0;
+    if (cl->res_epoch == cl->built_epoch && cl->res_mlp == (int)use_mlp &&
+        (!use_mlp || cl->res_version == cl->mlp_version))
         return FV_OK;
     const int64_t nocc = cl->nocc;
     if (nocc == 0) return FV_OK;
-    const double h = cl->hh;
-    const double rc2 = (cl->kappa * h) * (cl->kappa * h) * (1.0 + 1e-12);
-    TimedLaunch tl(ctx, "closure_far");
     if (use_mlp) {
-        const int64_t np2 = nocc * nocc;
-        hipLaunchKernelGGL(k_far_inputs, dim3((unsigned)ceil_div(np2, 256)), dim3(256), 0,
-                           ctx->stream, nocc, cl->mom, h, rc2, cl->X);
+        TimedLaunch tl(ctx, "closure_residual");
+        const int64_t nl = cl->nlist;
+        hipLaunchKernelGGL(k_shell_inputs, dim3((unsigned)ceil_div(nl, 256)), dim3(256), 0,
+                           ctx->stream, nl, cl->lt, cl->ls, cl->mom, cl->hh, cl->X, cl->lk);
         FV_LAUNCH_CHECK(ctx);
-        int rc = fv_mlp_forward_dev(cl->mlp, cl->X, np2, cl->Y);
+        int rc = fv_mlp_forward_dev(cl->mlp, cl->X, nl, cl->Y);
         if (rc) return rc;
-        cl->far_evals += (double)np2;
+        hipLaunchKernelGGL(k_shell_reduce, dim3((unsigned)nocc), dim3(64), 0, ctx->stream, nocc,
+                           cl->loff, cl->lk, cl->Y, cl->res);
+        FV_LAUNCH_CHECK(ctx);
+        tl.done();
+        cl->far_evals += (double)nl;
+    } else {
+        int rc = fill(ctx, cl->res, 3 * nocc, 0.0);
+        if (rc) return rc;
     }
-    hipLaunchKernelGGL(k_far_reduce, dim3((unsigned)nocc), dim3(64), 0, ctx->stream, nocc, cl->mom,
-                       h, rc2, cl->Y, use_mlp ? 1 : 0, cl->faruj);
-    FV_LAUNCH_CHECK(ctx);
-    tl.done();
-    cl->far_epoch = cl->built_epoch;
-    cl->far_mlp = (int)use_mlp;
-    cl->far_version = cl->mlp_version;
+    cl->res_epoch = cl->built_epoch;
+    cl->res_mlp = (int)use_mlp;
+    cl->res_version = cl->mlp_version;
     return FV_OK;
 }
 
@@ -781,13 +912,22 @@ static void launch_near(fv_closure *cl, fv_ctx *ctx, unsigned chunks)
                        cl->sz, cl->sax, cl->say, cl->saz, cl->ssg, cl->near_out);
 }
 
+template <int M>
+static void launch_far(fv_closure *cl, fv_ctx *ctx, double rc2, bool res, double *out)
+{
+    hipLaunchKernelGGL((k_far_apply<M>), dim3((unsigned)ceil_div(cl->n, 256)), dim3(256), 0,
+                       ctx->stream, cl->n, cl->nocc, cl->key_s, cl->occ_of, cl->perm, cl->sx,
+                       cl->sy, cl->sz, cl->sax, cl->say, cl->saz, cl->mom, rc2,
+                       res ? cl->res : (const double *)nullptr, cl->near_out, out);
+}
+
 // Closure sum at the particles (original order) for mode 0 (velocity), 2
 // (stretching) or 3 (vorticity); kernel selects the near-field smoothing.
 static int closure_pass(fv_vpm *f, int mode, int kernel, double *out, bool use_mlp)
 {
     fv_closure *cl = f->closure;
     fv_ctx *ctx = f->ctx;
-    int rc = closure_build(f, use_mlp);
+    int rc = closure_build(f, use_mlp, mode == 0);
     if (rc) return rc;
     const int64_t n = cl->n;
     if (n == 0) return FV_OK;
@@ -811,10 +951,15 @@ static int closure_pass(fv_vpm *f, int mode, int kernel, double *out, bool use_m
         FV_LAUNCH_CHECK(ctx);
         tl.done();
     }
-    hipLaunchKernelGGL(k_apply, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, ctx->stream, n,
-                       mode, cl->key_s, cl->occ_of, cl->perm, cl->sx, cl->sy, cl->sz, cl->sax,
-                       cl->say, cl->saz, cl->mom, cl->faruj, cl->near_out, out);
-    FV_LAUNCH_CHECK(ctx);
+    {
+        TimedLaunch tl(ctx, "closure_far");
+        const double rc2 = (cl->kappa * cl->hh) * (cl->kappa * cl->hh) * (1.0 + 1e-12);
+        if (mode == 0) launch_far<0>(cl, ctx, rc2, true, out);
+        else if (mode == 2) launch_far<2>(cl, ctx, rc2, false, out);
+        else launch_far<3>(cl, ctx, rc2, false, out);
+        FV_LAUNCH_CHECK(ctx);
+        tl.done();
+    }
     return FV_OK;
 }
 
@@ -822,22 +967,21 @@ static int closure_train(fv_vpm *f)
 {
     fv_closure *cl = f->closure;
     fv_ctx *ctx = f->ctx;
-    if (!cl->mlp || cl->train_iters <= 0 || cl->n == 0 || cl->nocc < 2) return FV_OK;
+    if (!cl->mlp || cl->train_iters <= 0 || cl->n == 0 || cl->nlist == 0) return FV_OK;
     const int S = cl->train_samples;
     if (!cl->tX) {
-        FV_HIP_CHECK(ctx, hipMalloc(&cl->tX, sizeof(float) * (32 * (size_t)S + 16)));
-        cl->tT = cl->tX + 16 * (size_t)S;
-        cl->tw = cl->tT + 16 * (size_t)S;
-        const float w[16] = {1, 1, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        FV_HIP_CHECK(ctx, hipMalloc(&cl->tX, sizeof(float) * ((kFeat + kOutW) * (size_t)S + kOutW)));
+        cl->tT = cl->tX + kFeat * (size_t)S;
+        cl->tw = cl->tT + kOutW * (size_t)S;
+        float w[kOutW] = {1, 1, 1};
         FV_HIP_CHECK(ctx, hipMemcpyAsync(cl->tw, w, sizeof w, hipMemcpyHostToDevice, ctx->stream));
+        FV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     }
-    const double rc2 = (cl->kappa * cl->hh) * (cl->kappa * cl->hh) * (1.0 + 1e-12);
     TimedLaunch tl(ctx, "closure_train");
     cl->seed = cl->seed * 6364136223846793005ull + 1442695040888963407ull;
-    hipLaunchKernelGGL(k_train_samples, dim3((unsigned)S), dim3(64), 0, ctx->stream, cl->n,
-                       cl->nocc, cl->occ, cl->occ_of, cl->key_s, cl->cstart, cl->cend, cl->sx,
-                       cl->sy, cl->sz, cl->sax, cl->say, cl->saz, cl->mom, cl->hh, rc2, cl->seed,
-                       cl->tX, cl->tT);
+    hipLaunchKernelGGL(k_train_samples, dim3((unsigned)S), dim3(64), 0, ctx->stream, cl->nlist,
+                       cl->lt, cl->ls, cl->occ, cl->cstart, cl->cend, cl->sx, cl->sy, cl->sz,
+                       cl->sax, cl->say, cl->saz, cl->mom, cl->hh, cl->seed, cl->tX, cl->tT);
     FV_LAUNCH_CHECK(ctx);
     for (int it = 0; it < cl->train_iters; ++it) {
         int rc = fv_mlp_train_step_dev(cl->mlp, cl->tX, cl->tT, cl->tw, S, (float)cl->lr);
@@ -848,7 +992,7 @@ static int closure_train(fv_vpm *f)
     return FV_OK;
 }
 
-// Normwise relative error of the closure velocity against the exact sum.
+// Max-norm error of the closure velocity relative to the exact sum's max norm.
 static int closure_check(fv_vpm *f, bool use_mlp, double *err)
 {
     fv_closure *cl = f->closure;
@@ -883,6 +1027,7 @@ static int closure_check(fv_vpm *f, bool use_mlp, double *err)
     FV_HIP_CHECK(ctx, hipMemcpyAsync(r, cl->verr, sizeof r, hipMemcpyDeviceToHost, ctx->stream));
     FV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     *err = r[1] > 0 ? r[0] / r[1] : r[0];
+    if (!std::isfinite(*err)) *err = INFINITY;
     return FV_OK;
 }
 
@@ -917,8 +1062,8 @@ int closure_self_sum(fv_vpm *f, int mode, int kernel, double *out)
 void closure_destroy(fv_closure *cl)
 {
     if (!cl) return;
-    void *bufs[] = {cl->block, cl->cells, cl->far, cl->tmp, cl->bbox, cl->dstencil, cl->tX,
-                    cl->vdir};
+    void *bufs[] = {cl->block, cl->cells, cl->far, cl->list, cl->tmp, cl->bbox, cl->dstencil,
+                    cl->tX, cl->vdir};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     delete cl;
@@ -935,7 +1080,7 @@ int fv_vpm_set_closure(fv_vpm *f, fv_mlp *mlp, double cell, double kappa, int32_
 {
     if (!f) return FV_E_ARG;
     fv_ctx *ctx = f->ctx;
-    FV_REQUIRE(ctx, cell > 0 && kappa >= 1.0 && check_every >= 1 && tol > 0 &&
+    FV_REQUIRE(ctx, cell > 0 && kappa >= 2.0 && check_every >= 1 && tol > 0 &&
                         train_samples >= 16 && train_iters >= 0 && lr >= 0,
                "invalid closure parameters");
     closure_destroy(f->closure);

@@ -1,12 +1,12 @@
-// fv_mlp.hip -- the far-field closure network: a 16 -> 128 -> 128 -> 128 -> 16
+// fv_mlp.hip -- the far-field closure network: a 32 -> 128 -> 128 -> 128 -> 16
 // SiLU MLP with BF16 weights/activations on MFMA (v_mfma_f32_16x16x16_bf16,
 // FP32 accumulation), and its hand-written backward and Adam kernels.
 //
 // One wavefront owns 16 samples at a time; its activations live in LDS as
 // BF16 (the A operand), weights are read as BF16 rows (the B operand), and the
 // 16x16 FP32 accumulator tiles are biased, activated and rounded to BF16 in
-// registers before the next layer.  Inputs / outputs beyond the network's
-// n_in / n_out are zero-padded to 16.
+// registers before the next layer.  Inputs are zero-padded to 32 columns and
+// outputs to 16 (two and one 16-wide MFMA k / n slices).
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -19,7 +19,7 @@ typedef float fv_f4 __attribute__((ext_vector_type(4)));
 
 namespace fv {
 
-constexpr int kIn = 16, kHid = 128, kOut = 16;
+constexpr int kIn = 32, kHid = 128, kOut = 16;
 // FP32 master parameter layout (row-major weights [out][in]).
 constexpr int kOffW1 = 0;
 constexpr int kOffB1 = kOffW1 + kHid * kIn;
@@ -29,7 +29,7 @@ constexpr int kOffW3 = kOffB2 + kHid;
 constexpr int kOffB3 = kOffW3 + kHid * kHid;
 constexpr int kOffW4 = kOffB3 + kHid;
 constexpr int kOffB4 = kOffW4 + kOut * kHid;
-constexpr int kParams = kOffB4 + kOut;   // 37264
+constexpr int kParams = kOffB4 + kOut;   // 39312
 constexpr int kLda = kHid + 8;           // LDS row stride of activations (bf16)
 
 __device__ __forceinline__ unsigned short f2bf(float f)
@@ -92,9 +92,9 @@ __global__ __launch_bounds__(64) void k_mlp_forward(MlpW m, const float *__restr
     const int col = l & 15, rq = (l >> 4) * 4;
     for (int64_t g = blockIdx.x; g * 16 < nb; g += gridDim.x) {
         const int64_t s0 = g * 16;
-        // inputs -> bf16 LDS (16 x 16)
-        for (int e = l; e < 256; e += 64) {
-            const int i = e >> 4, k = e & 15;
+        // inputs -> bf16 LDS (16 x kIn)
+        for (int e = l; e < 16 * kIn; e += 64) {
+            const int i = e / kIn, k = e % kIn;
             const float v = (s0 + i < nb) ? X[(s0 + i) * kIn + k] : 0.0f;
             act[0][i * kLda + k] = f2bf(v);
         }
@@ -504,13 +504,13 @@ int fv_mlp_forward_host(fv_mlp *m, const float *X, int64_t nb, float *Y)
     fv_ctx *ctx = m->ctx;
     FV_REQUIRE(ctx, nb >= 0 && (nb == 0 || (X && Y)), "bad forward arguments");
     if (nb == 0) return FV_OK;
-    float *d = static_cast<float *>(workspace(ctx, S_TMP0, sizeof(float) * nb * 32));
+    float *d = static_cast<float *>(workspace(ctx, S_TMP0, sizeof(float) * nb * (kIn + kOut)));
     if (!d) return FV_E_HIP;
-    FV_HIP_CHECK(ctx, hipMemcpyAsync(d, X, sizeof(float) * nb * 16, hipMemcpyHostToDevice,
+    FV_HIP_CHECK(ctx, hipMemcpyAsync(d, X, sizeof(float) * nb * kIn, hipMemcpyHostToDevice,
                                      ctx->stream));
-    int rc = fv_mlp_forward_dev(m, d, nb, d + nb * 16);
+    int rc = fv_mlp_forward_dev(m, d, nb, d + nb * kIn);
     if (rc) return rc;
-    return fv_memcpy_d2h(ctx, Y, d + nb * 16, sizeof(float) * nb * 16);
+    return fv_memcpy_d2h(ctx, Y, d + nb * kIn, sizeof(float) * nb * kOut);
 }
 
 int fv_mlp_gradients_host(fv_mlp *m, const float *X, const float *T, const float *wout,
@@ -519,15 +519,16 @@ int fv_mlp_gradients_host(fv_mlp *m, const float *X, const float *T, const float
     if (!m) return FV_E_ARG;
     fv_ctx *ctx = m->ctx;
     FV_REQUIRE(ctx, nb > 0 && X && T && wout && grad, "bad gradient arguments");
-    float *d = static_cast<float *>(workspace(ctx, S_TMP1, sizeof(float) * (nb * 32 + 16)));
+    float *d = static_cast<float *>(
+        workspace(ctx, S_TMP1, sizeof(float) * (nb * (kIn + kOut) + kOut)));
     if (!d) return FV_E_HIP;
-    FV_HIP_CHECK(ctx, hipMemcpyAsync(d, X, sizeof(float) * nb * 16, hipMemcpyHostToDevice,
+    FV_HIP_CHECK(ctx, hipMemcpyAsync(d, X, sizeof(float) * nb * kIn, hipMemcpyHostToDevice,
                                      ctx->stream));
-    FV_HIP_CHECK(ctx, hipMemcpyAsync(d + nb * 16, T, sizeof(float) * nb * 16,
+    FV_HIP_CHECK(ctx, hipMemcpyAsync(d + nb * kIn, T, sizeof(float) * nb * kOut,
                                      hipMemcpyHostToDevice, ctx->stream));
-    FV_HIP_CHECK(ctx, hipMemcpyAsync(d + nb * 32, wout, sizeof(float) * 16, hipMemcpyHostToDevice,
-                                     ctx->stream));
-    int rc = mlp_gradients(m, d, d + nb * 16, d + nb * 32, nb);
+    FV_HIP_CHECK(ctx, hipMemcpyAsync(d + nb * (kIn + kOut), wout, sizeof(float) * kOut,
+                                     hipMemcpyHostToDevice, ctx->stream));
+    int rc = mlp_gradients(m, d, d + nb * kIn, d + nb * (kIn + kOut), nb);
     if (rc) return rc;
     return fv_memcpy_d2h(ctx, grad, m->grad, sizeof(float) * kParams);
 }

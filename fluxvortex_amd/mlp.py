@@ -1,8 +1,8 @@
-"""The far-field closure network on the MI355X: 16 -> 128 -> 128 -> 128 -> 16
+"""The far-field closure network on the MI355X: 32 -> 128 -> 128 -> 128 -> 16
 SiLU MLP, BF16 weights and activations on MFMA with FP32 accumulation, with
 hand-written forward, backward and Adam kernels (fv_mlp_*).
 
-Parameters live on the device as FP32 masters (layout W1[128][16], b1[128],
+Parameters live on the device as FP32 masters (layout W1[128][32], b1[128],
 W2[128][128], b2, W3[128][128], b3, W4[16][128], b4[16]) plus BF16 copies used
 by the MFMA kernels.
 """
@@ -14,12 +14,13 @@ import numpy as np
 
 from . import _lib
 
-SHAPES = [("W1", (128, 16)), ("b1", (128,)), ("W2", (128, 128)), ("b2", (128,)),
-          ("W3", (128, 128)), ("b3", (128,)), ("W4", (16, 128)), ("b4", (16,))]
+N_IN, N_OUT = 32, 16
+SHAPES = [("W1", (128, N_IN)), ("b1", (128,)), ("W2", (128, 128)), ("b2", (128,)),
+          ("W3", (128, 128)), ("b3", (128,)), ("W4", (N_OUT, 128)), ("b4", (N_OUT,))]
 NPARAMS = sum(int(np.prod(s)) for _, s in SHAPES)
 
 
-def init_params(n_in=15, n_out=12, seed=0, zero_output=False) -> np.ndarray:
+def init_params(n_in=N_IN, n_out=N_OUT, seed=0, zero_output=False) -> np.ndarray:
     """Uniform(-1/sqrt(fan_in), 1/sqrt(fan_in)) weights, zero biases; padded
     inputs / outputs beyond n_in / n_out get zero weights.  zero_output starts
     the output layer at zero (a residual network that initially predicts 0)."""
@@ -43,9 +44,9 @@ def init_params(n_in=15, n_out=12, seed=0, zero_output=False) -> np.ndarray:
 
 
 def closure_params(seed=0) -> np.ndarray:
-    """Initial parameters of the far-field closure: 16 inputs, 3 outputs (the
-    velocity residual), zero output layer."""
-    return init_params(n_in=16, n_out=3, seed=seed, zero_output=True)
+    """Initial parameters of the far-field closure: 31 inputs (the features of
+    fv_closure.hip), 3 outputs (the velocity residual), zero output layer."""
+    return init_params(n_in=31, n_out=3, seed=seed, zero_output=True)
 
 
 class ClosureMLP:
@@ -81,7 +82,7 @@ class ClosureMLP:
         return p
 
     @staticmethod
-    def _pad(X, width=16):
+    def _pad(X, width=N_IN):
         X = np.asarray(X, dtype=np.float32)
         out = np.zeros((X.shape[0], width), dtype=np.float32)
         out[:, :X.shape[1]] = X
@@ -89,15 +90,15 @@ class ClosureMLP:
 
     def forward(self, X) -> np.ndarray:
         Xp = self._pad(X)
-        Y = np.zeros((Xp.shape[0], 16), dtype=np.float32)
+        Y = np.zeros((Xp.shape[0], N_OUT), dtype=np.float32)
         if Xp.shape[0]:
             self._ctx.check(self._ctx.lib.fv_mlp_forward_host(
                 self._h, Xp.ctypes.data_as(_lib._fp), Xp.shape[0], Y.ctypes.data_as(_lib._fp)))
         return Y
 
     def gradients(self, X, T, wout) -> np.ndarray:
-        Xp, Tp = self._pad(X), self._pad(T)
-        w = self._pad(np.asarray(wout, dtype=np.float32)[None, :])[0]
+        Xp, Tp = self._pad(X), self._pad(T, N_OUT)
+        w = self._pad(np.asarray(wout, dtype=np.float32)[None, :], N_OUT)[0]
         g = np.zeros(NPARAMS, dtype=np.float32)
         self._ctx.check(self._ctx.lib.fv_mlp_gradients_host(
             self._h, Xp.ctypes.data_as(_lib._fp), Tp.ctypes.data_as(_lib._fp),

@@ -227,9 +227,9 @@ int fv_hybrid_set_flags(fv_hybrid *h, int32_t free_wake, int32_t stretch);
 int fv_hybrid_set_precision(fv_hybrid *h, int32_t precision);
 
 /* ------------------------------------------ learned far-field closure MLP */
-/* 16 -> 128 -> 128 -> 128 -> 16 SiLU network, BF16 weights / activations on
+/* 32 -> 128 -> 128 -> 128 -> 16 SiLU network, BF16 weights / activations on
  * MFMA with FP32 accumulation; FP32 master parameters in the layout
- * W1[128][16] b1[128] W2[128][128] b2[128] W3[128][128] b3[128] W4[16][128]
+ * W1[128][32] b1[128] W2[128][128] b2[128] W3[128][128] b3[128] W4[16][128]
  * b4[16] (fv_mlp_param_count() floats).  No reference counterpart: the
  * reference has no learned closure; this is the network of the far-field
  * closure that replaces beyond-cutoff particle sums (the exact direct sum of
@@ -240,7 +240,7 @@ void fv_mlp_destroy(fv_mlp *m);
 int fv_mlp_param_count(void);
 int fv_mlp_set_params(fv_mlp *m, const float *params);   /* host; resets Adam */
 int fv_mlp_get_params(fv_mlp *m, float *params);         /* host */
-/* X (nb x 16), Y (nb x 16) fp32 device arrays. */
+/* X (nb x 32), Y (nb x 16) fp32 device arrays; T of the gradient calls is nb x 16. */
 int fv_mlp_forward_dev(fv_mlp *m, const float *X, int64_t nb, float *Y);
 /* Gradient of L = (1/nb) sum_s sum_j wout_j (Y_sj - T_sj)^2; grad (device,
  * nullable) receives a copy. */

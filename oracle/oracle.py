@@ -305,12 +305,13 @@ def hybrid_run(case, steps=None, threads=0):
 
 
 # ---------------------------------------------------------------------------
-# Closure MLP (16 -> 128 -> 128 -> 128 -> 16, SiLU) in FP64, optionally with
+# Closure MLP (32 -> 128 -> 128 -> 128 -> 16, SiLU) in FP64, optionally with
 # the device's bf16 rounding points emulated (inputs, weights, activations,
 # layer errors); the parameter layout is the device's FP32 master layout.
 # ---------------------------------------------------------------------------
-MLP_SHAPES = [("W1", (128, 16)), ("b1", (128,)), ("W2", (128, 128)), ("b2", (128,)),
-              ("W3", (128, 128)), ("b3", (128,)), ("W4", (16, 128)), ("b4", (16,))]
+MLP_IN, MLP_OUT = 32, 16
+MLP_SHAPES = [("W1", (128, MLP_IN)), ("b1", (128,)), ("W2", (128, 128)), ("b2", (128,)),
+              ("W3", (128, 128)), ("b3", (128,)), ("W4", (MLP_OUT, 128)), ("b4", (MLP_OUT,))]
 MLP_NPARAMS = sum(int(np.prod(s)) for _, s in MLP_SHAPES)
 
 

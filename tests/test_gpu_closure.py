@@ -60,7 +60,7 @@ def test_expansion_converges_with_cutoff(field):
         errs.append(e)
     print("expansion-only error vs kappa (2, 4, 8):", errs)
     assert errs[0] > errs[1] > errs[2]
-    assert errs[2] < 2e-3
+    assert errs[1] < 1e-3 and errs[2] < 2e-4
 
 
 def test_learned_residual_gain(field):
@@ -71,11 +71,13 @@ def test_learned_residual_gain(field):
     _, e0 = f.closure_velocity(use_mlp=False)
     _, e_untrained = f.closure_velocity(use_mlp=True)   # zero output layer: = expansion
     assert abs(e_untrained - e0) <= 1e-9 + 1e-6 * e0
-    f.closure_train(rounds=150)
+    f.closure_train(rounds=200)
     _, e1 = f.closure_velocity(use_mlp=True)
     print(f"closure error: expansion only {e0:.3e}, with trained network {e1:.3e}, "
           f"gain x{e0 / e1:.2f}")
-    assert e1 < e0
+    # the network learns the quadrupole-order residual from the cells' second
+    # moments; an exact residual at the target cell centres would give x3.3 here
+    assert e1 < 0.8 * e0
 
 
 def test_advect_with_closure_checks_and_stays_close(ctx, oracle):

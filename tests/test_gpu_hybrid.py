@@ -16,22 +16,44 @@ def hyb(ctx):
     return hybrid
 
 
-def oracle_sensitivity(oracle, case, ref):
-    """Running max over steps of |CL_oracle(case) - CL_oracle(case with V * (1 + 1e-14))|:
-    how far the reference algorithm itself moves under a 1e-14 input change."""
-    pert = oracle.hybrid_run(replace(case, vinf=case.vinf * (1.0 + 1e-14)))
-    d = np.maximum(np.abs(pert["cl"] - ref["cl"]), np.abs(pert["cdi"] - ref["cdi"]))
-    return np.maximum.accumulate(d)
+def oracle_perturbed(oracle, case):
+    """The oracle under two 1e-14 input changes: a uniform one (V * (1 + 1e-14))
+    and a mirror-antisymmetric one (x of every grid point with y > 0 shifted by
+    1e-14 of the largest |x|), which excites what a symmetric change cannot
+    (relaxation turns particles towards their local vorticity direction, which
+    round-off fixes where that vorticity is tiny)."""
+    a = oracle.hybrid_run(replace(case, vinf=case.vinf * (1.0 + 1e-14)))
+    g = case.grids.copy()
+    g[..., 0] += np.where(g[..., 1] > 0, 1e-14 * np.abs(g[..., 0]).max(), 0.0)
+    b = oracle.hybrid_run(replace(case, grids=g))
+    return a, b
+
+
+def response(ref, runs, key):
+    """Largest normwise relative change of ref[key] over the perturbed runs
+    (inf when a perturbed run has a different particle count)."""
+    out = 0.0
+    for r in runs:
+        if r[key].shape != ref[key].shape:
+            return np.inf
+        out = max(out, relerr(r[key], ref[key]) if ref[key].size else 0.0)
+    return out
 
 
 def compare(hyb, oracle, case, cl_tol=1e-10, part=True):
-    """Device vs oracle CL/CDi histories: within cl_tol at every step where the
-    oracle's own 1e-14-perturbation response stays below 1e-12; beyond that
-    (chaotic free-wake dynamics, relaxation thresholds) within 1e3 x that response."""
+    """Device vs oracle.  CL/CDi histories: within cl_tol at every step where the
+    oracle's own response to the 1e-14 input changes stays below 1e-12, else
+    within 1e3 x that response (chaotic free-wake dynamics, relaxation
+    thresholds).  Final circulations and particles: within max(1e-10, 1e3 x the
+    oracle's own response)."""
     print(f"[hybrid parity] {case.name} steps={case.num_steps}", flush=True)
     dev = hyb.run_case(case, particles=part)
     ref = oracle.hybrid_run(case)
-    sens = oracle_sensitivity(oracle, case, ref)
+    runs = oracle_perturbed(oracle, case)
+    d = np.zeros_like(ref["cl"])
+    for r in runs:
+        d = np.maximum(d, np.maximum(np.abs(r["cl"] - ref["cl"]), np.abs(r["cdi"] - ref["cdi"])))
+    sens = np.maximum.accumulate(d)
     bound = np.where(sens < 1e-12, cl_tol, np.maximum(cl_tol, 1e3 * sens))
     dcl = np.abs(dev["cl"] - ref["cl"])
     dcdi = np.abs(dev["cdi"] - ref["cdi"])
@@ -39,12 +61,13 @@ def compare(hyb, oracle, case, cl_tol=1e-10, part=True):
     assert bad.size == 0, (bad[:5], dcl[bad[:5]], bound[bad[:5]])
     well = int(np.sum(sens < 1e-12))
     assert well >= min(case.num_steps, 8), f"only {well} well-conditioned steps"
-    if np.all(sens < 1e-12):
+    assert relerr(dev["gamma"], ref["gamma"]) < max(1e-10, 1e3 * response(ref, runs, "gamma"))
+    if part and all(r["np"] == ref["np"] for r in runs):
         assert dev["np"] == ref["np"]
-        assert relerr(dev["gamma"], ref["gamma"]) < 1e-10
-        if part and ref["np"]:
-            assert relerr(dev["pos"], ref["pos"]) < 1e-10
-            assert relerr(dev["alpha"], ref["gamma_p"]) < 1e-9
+        if ref["np"]:
+            assert relerr(dev["pos"], ref["pos"]) < max(1e-10, 1e3 * response(ref, runs, "pos"))
+            assert relerr(dev["alpha"], ref["gamma_p"]) < max(
+                1e-9, 1e3 * response(ref, runs, "gamma_p"))
     return dev, ref, sens
 
 

@@ -192,3 +192,23 @@ def test_particle_edge_cases(oracle, ctx):
         assert np.all(np.isfinite(u)) and np.all(np.isfinite(J))
         assert relerr(u, oracle.particle_vel(pos, pos, al, sg, kern)) < TOL64
         assert relerr(J, oracle.particle_jac(pos, pos, al, sg, kern)) < TOL64
+
+
+@pytest.mark.parametrize("kernel", ["gaussianerf", "winckelmans"])
+def test_mixed_tiny_cores_stay_finite(oracle, ctx, kernel):
+    """Cores at the RK3 floor sigma = 1e-6 (where a stretched wake ends up) with
+    targets up to ~300 m away: 1/sigma^5 alone is 1e30, so the FP32 pair terms
+    must be formed without it.  Mixed stays finite and within 1e-5 of FP64."""
+    from fluxvortex_amd import kernel as K
+    rng = np.random.default_rng(11)
+    n = 600
+    pos = rng.uniform([0, -5, -0.5], [300, 5, 0.5], (n, 3))
+    al = rng.normal(0, 1.0, (n, 3))
+    sg = np.where(rng.uniform(size=n) < 0.5, 1e-6, rng.uniform(0.02, 0.1, n))
+    tgt = np.concatenate([pos[:200], rng.uniform([0, -5, -0.5], [300, 5, 0.5], (100, 3))])
+    tal = np.concatenate([al[:200], rng.normal(0, 1.0, (100, 3))])
+    um = K.velocity_from_particles(tgt, pos, al, sg, kernel=kernel, precision="mixed")
+    Jm = K.jacobian_from_particles(tgt, tal, pos, al, sg, kernel=kernel, precision="mixed")
+    assert np.all(np.isfinite(um)) and np.all(np.isfinite(Jm))
+    assert relerr(um, oracle.particle_vel(tgt, pos, al, sg, kernel)) < 1e-5
+    assert relerr(Jm, oracle.particle_jac(tgt, pos, al, sg, kernel)) < 1e-5

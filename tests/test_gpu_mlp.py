@@ -18,8 +18,8 @@ def mlpmod(ctx):
 
 def batch(n, seed=0):
     rng = np.random.default_rng(seed)
-    X = np.zeros((n, 16), np.float32)
-    X[:, :15] = rng.normal(0, 1, (n, 15))
+    X = np.zeros((n, 32), np.float32)
+    X[:, :31] = rng.normal(0, 1, (n, 31))
     T = np.zeros((n, 16), np.float32)
     T[:, :12] = rng.normal(0, 1, (n, 12))
     w = np.zeros(16, np.float32)
@@ -28,7 +28,7 @@ def batch(n, seed=0):
 
 
 def test_forward_matches_oracle(oracle, mlpmod):
-    p = mlpmod.init_params(seed=3)
+    p = mlpmod.init_params(n_in=31, n_out=12, seed=3)
     net = mlpmod.ClosureMLP(p)
     for n in (1, 16, 37, 1000):
         X, _, _ = batch(n, seed=n)
@@ -39,7 +39,7 @@ def test_forward_matches_oracle(oracle, mlpmod):
 
 
 def test_gradients_match_oracle(oracle, mlpmod):
-    p = mlpmod.init_params(seed=4)
+    p = mlpmod.init_params(n_in=31, n_out=12, seed=4)
     net = mlpmod.ClosureMLP(p)
     X, T, w = batch(300, seed=5)
     g = net.gradients(X, T, w)
@@ -51,11 +51,11 @@ def test_gradients_match_oracle(oracle, mlpmod):
             assert relerr(gd[name][:12], rd[name][:12]) < 2e-2
             continue
         assert relerr(gd[name], rd[name]) < 2e-2, name
-    assert np.all(gd["W4"][12:] == 0) and np.all(gd["W1"][:, 15:] == 0)
+    assert np.all(gd["W4"][12:] == 0) and np.all(gd["W1"][:, 31:] == 0)
 
 
 def test_adam_matches_oracle(oracle, mlpmod):
-    p = mlpmod.init_params(seed=6)
+    p = mlpmod.init_params(n_in=31, n_out=12, seed=6)
     net = mlpmod.ClosureMLP(p)
     rng = np.random.default_rng(1)
     m = np.zeros(p.size)
@@ -71,7 +71,7 @@ def test_adam_matches_oracle(oracle, mlpmod):
 def test_training_reduces_loss(oracle, mlpmod):
     rng = np.random.default_rng(2)
     n = 2048
-    X = np.zeros((n, 16), np.float32)
+    X = np.zeros((n, 32), np.float32)
     X[:, :3] = rng.uniform(-1, 1, (n, 3))
     T = np.zeros((n, 16), np.float32)
     T[:, 0] = np.sin(2 * X[:, 0]) * X[:, 1]

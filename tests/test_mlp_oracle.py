@@ -17,10 +17,10 @@ def test_bf16_rounding(oracle):
 def test_backprop_matches_finite_differences(oracle):
     rng = np.random.default_rng(0)
     from fluxvortex_amd.mlp import init_params
-    p = init_params(seed=1).astype(np.float64)
+    p = init_params(n_in=31, n_out=12, seed=1).astype(np.float64)
     p[oracle.MLP_NPARAMS - 16:] = rng.normal(0, 0.1, 16)      # non-zero b4
-    X = np.zeros((7, 16))
-    X[:, :15] = rng.normal(0, 1, (7, 15))
+    X = np.zeros((7, oracle.MLP_IN))
+    X[:, :31] = rng.normal(0, 1, (7, 31))
     T = rng.normal(0, 1, (7, 16))
     w = np.zeros(16)
     w[:12] = rng.uniform(0.5, 2, 12)
