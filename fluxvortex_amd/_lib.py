@@ -99,6 +99,8 @@ _SIGNATURES = {
     "fv_dense_solve_dev": (C.c_int, [_ctx, C.c_int64, C.c_int64, _vp, _vp, _vp]),
     "fv_dense_solve_mixed_dev": (C.c_int, [_ctx, C.c_int64, C.c_int64, _vp, _vp, C.c_int32,
                                            C.c_double, _vp, _vp]),
+    "fv_dense_solve_mixed_host": (C.c_int, [_ctx, C.c_int64, C.c_int64, _dp, _dp, C.c_int32,
+                                           C.c_double, _i32p, _i32p]),
     "fv_particle_velocity_host": (C.c_int, [_ctx, _dp, C.c_int64, _dp, _dp, _dp, C.c_int64,
                                             C.c_int32, C.c_int32, _dp]),
     "fv_particle_jacobian_host": (C.c_int, [_ctx, _dp, C.c_int64, _dp, _dp, _dp, C.c_int64,
@@ -128,6 +130,7 @@ _SIGNATURES = {
     "fv_vpm_clear_closure": (C.c_int, [_vp]),
     "fv_vpm_closure_stats": (C.c_int, [_vp, _dp, _i64p, _i64p, _i32p, _dp]),
     "fv_vpm_closure_velocity_host": (C.c_int, [_vp, C.c_int32, _dp, _dp]),
+    "fv_vpm_closure_sum_host": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32, _dp]),
     "fv_vpm_closure_train": (C.c_int, [_vp, C.c_int32]),
     "fv_mlp_create": (C.c_int, [_ctx, C.POINTER(_vp)]),
     "fv_mlp_destroy": (None, [_vp]),

@@ -45,6 +45,7 @@ constexpr int kMomTile = 19;        // n .. centre: what the far-field pass stag
 constexpr int kFeat = 32;           // MLP input width (31 features + pad)
 constexpr int kOutW = 16;           // MLP output width (3 used)
 constexpr double kShell = 2.0;      // residual shell: kappa h < R <= kShell kappa h
+constexpr int kMaxSlices = 16;      // near (dx ranges) and far (source-cell ranges) slices
 
 struct fv_closure_state {
     fv_mlp *mlp = nullptr;
@@ -59,25 +60,31 @@ struct fv_closure_state {
     uint64_t seed = 0x9E3779B97F4A7C15ull;
     // geometry of the current build
     int64_t built_epoch = -1;
-    int64_t res_epoch = -1, res_version = -1;
+    int64_t bin_id = 0;                // bumped by every full binning
+    int64_t binned_step = -1;          // closure step whose binning is current
+    int64_t res_bin = -1, res_version = -1;
     int res_mlp = -1;
     int64_t mlp_version = 0;
     int64_t n = 0;                     // particles binned
     double ox = 0, oy = 0, oz = 0, hh = 2.0;
     int nx = 1, ny = 1, nz = 1;
     int64_t ncell = 0, nocc = 0, maxpop = 0, nlist = 0;
-    int skx = -1, sky = -1, skz = -1;  // stencil half-widths in cells
-    std::vector<int> stencil;          // (di, dj, dk) triples
+    int skx = -1, sky = -1, skz = -1;  // near-stencil half-widths in cells
+    int nsl = 0;                       // near slices (contiguous dx ranges)
+    std::vector<int> stencil;          // (dx, dy, dz_max) columns, dx-major
+    std::vector<int> slice_off;        // first column of each slice, nsl + 1
     // device buffers
     void *block = nullptr;
     size_t block_bytes = 0;
     int32_t *key = nullptr, *key_s = nullptr, *idx = nullptr, *perm = nullptr;
     double *sx = nullptr, *sy = nullptr, *sz = nullptr, *sax = nullptr, *say = nullptr,
            *saz = nullptr, *ssg = nullptr;
-    double *near_out = nullptr;        // 3 per sorted particle
+    double *npart = nullptr;           // near partial sums, kMaxSlices x 3 per sorted particle
+    double *fpart = nullptr;           // far partial sums, kMaxSlices x 3 per sorted particle
     void *cells = nullptr;
     size_t cells_bytes = 0;
     int32_t *cstart = nullptr, *cend = nullptr, *occ = nullptr, *occ_of = nullptr;
+    int32_t *ccount = nullptr, *pstart = nullptr;   // ncell + 1 each: population, prefix
     void *far = nullptr;
     size_t far_bytes = 0;
     double *mom = nullptr;             // kMomStride per occupied cell
@@ -88,7 +95,7 @@ struct fv_closure_state {
     int32_t *lt = nullptr, *ls = nullptr;       // shell pair list (target, source)
     double *lk = nullptr;              // residual scale per pair
     float *X = nullptr, *Y = nullptr;  // MLP inputs / outputs per pair
-    int32_t *dstencil = nullptr;
+    int32_t *dstencil = nullptr;       // columns, then slice offsets
     void *tmp = nullptr;
     size_t tmp_bytes = 0;
     double *bbox = nullptr;            // 6 + per-block partials
@@ -304,8 +311,8 @@ __device__ void far_expansion(const double *m, double x0, double x1, double x2, 
     const double d[3] = {x0 - m[1], x1 - m[2], x2 - m[3]};
     const double *A = m + 4, *M = m + 7;
     const double R2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-    const double iR2 = 1.0 / R2;
-    const double iR = sqrt(iR2);
+    const double iR = rsqrt_refined(R2);
+    const double iR2 = iR * iR;
     const double iR3 = iR * iR2, iR5 = iR3 * iR2, iR7 = iR5 * iR2;
     const double c = kInv4Pi;
     const double dA[3] = {d[1] * A[2] - d[2] * A[1], d[2] * A[0] - d[0] * A[2],
@@ -337,6 +344,72 @@ __device__ void far_expansion(const double *m, double x0, double x1, double x2, 
             const double t2 = -3.0 * ((epsMd + dxc) * iR5 - 5.0 * dMd[i] * d[k] * iR7);
             J[3 * i + k] += c * (t0 + t1 + t2);
         }
+}
+
+// The far term of one cell at point x for the closure passes, in closed form:
+// with d = x - xbar, w = 3 (d x A - e) / R^5 + 15 d x (M d) / R^7,
+//   J_ik = c ( -eps_ikl A_l / R^3 - 3 (eps_ikn (Md)_n + (d x M_.k)_i) / R^5 + w_i d_k ),
+//   MODE 0: u (as far_expansion);  MODE 2: S = J alpha
+//     = c ( -(alpha x A) / R^3 - 3 (alpha x Md + d x (M alpha)) / R^5 + w (d . alpha) );
+//   MODE 3: omega_i = eps_ijk J_kj.
+template <int MODE>
+__device__ __forceinline__ void far_term(const double *m, double x0, double x1, double x2,
+                                         const double *al, double *acc)
+{
+    const double d[3] = {x0 - m[1], x1 - m[2], x2 - m[3]};
+    const double *A = m + 4, *M = m + 7;
+    const double R2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    const double iR = rsqrt_refined(R2);
+    const double iR2 = iR * iR;
+    const double iR3 = iR * iR2, iR5 = iR3 * iR2;
+    const double c = kInv4Pi;
+    const double dA[3] = {d[1] * A[2] - d[2] * A[1], d[2] * A[0] - d[0] * A[2],
+                          d[0] * A[1] - d[1] * A[0]};
+    const double e[3] = {M[7] - M[5], M[2] - M[6], M[3] - M[1]};
+    const double Md[3] = {M[0] * d[0] + M[1] * d[1] + M[2] * d[2],
+                          M[3] * d[0] + M[4] * d[1] + M[5] * d[2],
+                          M[6] * d[0] + M[7] * d[1] + M[8] * d[2]};
+    const double dMd[3] = {d[1] * Md[2] - d[2] * Md[1], d[2] * Md[0] - d[0] * Md[2],
+                           d[0] * Md[1] - d[1] * Md[0]};
+    if (MODE == 0) {
+        for (int i = 0; i < 3; ++i) acc[i] += c * ((e[i] - dA[i]) * iR3 - 3.0 * dMd[i] * iR5);
+        return;
+    }
+    const double iR7 = iR5 * iR2;
+    double w[3];
+    for (int i = 0; i < 3; ++i) w[i] = 3.0 * (dA[i] - e[i]) * iR5 + 15.0 * dMd[i] * iR7;
+    if (MODE == 2) {
+        const double aA[3] = {al[1] * A[2] - al[2] * A[1], al[2] * A[0] - al[0] * A[2],
+                              al[0] * A[1] - al[1] * A[0]};
+        const double aMd[3] = {al[1] * Md[2] - al[2] * Md[1], al[2] * Md[0] - al[0] * Md[2],
+                               al[0] * Md[1] - al[1] * Md[0]};
+        const double Ma[3] = {M[0] * al[0] + M[1] * al[1] + M[2] * al[2],
+                              M[3] * al[0] + M[4] * al[1] + M[5] * al[2],
+                              M[6] * al[0] + M[7] * al[1] + M[8] * al[2]};
+        const double dMa[3] = {d[1] * Ma[2] - d[2] * Ma[1], d[2] * Ma[0] - d[0] * Ma[2],
+                               d[0] * Ma[1] - d[1] * Ma[0]};
+        const double da = d[0] * al[0] + d[1] * al[1] + d[2] * al[2];
+        for (int i = 0; i < 3; ++i)
+            acc[i] += c * (-aA[i] * iR3 - 3.0 * (aMd[i] + dMa[i]) * iR5 + w[i] * da);
+        return;
+    }
+    // MODE 3: J from the closed form, then its antisymmetric part
+    double J[9];
+    // -eps_ikl A_l / R^3 - 3 eps_ikn Md_n / R^5 = -eps_ikl (A_l / R^3 + 3 Md_l / R^5)
+    const double v[3] = {A[0] * iR3 + 3.0 * Md[0] * iR5, A[1] * iR3 + 3.0 * Md[1] * iR5,
+                         A[2] * iR3 + 3.0 * Md[2] * iR5};
+    J[0] = 0.0;   J[1] = -v[2]; J[2] = v[1];
+    J[3] = v[2];  J[4] = 0.0;   J[5] = -v[0];
+    J[6] = -v[1]; J[7] = v[0];  J[8] = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        const double col[3] = {M[k], M[3 + k], M[6 + k]};
+        const double dc[3] = {d[1] * col[2] - d[2] * col[1], d[2] * col[0] - d[0] * col[2],
+                              d[0] * col[1] - d[1] * col[0]};
+        for (int i = 0; i < 3; ++i) J[3 * i + k] += -3.0 * dc[i] * iR5 + w[i] * d[k];
+    }
+    acc[0] += c * (J[7] - J[5]);
+    acc[1] += c * (J[2] - J[6]);
+    acc[2] += c * (J[3] - J[1]);
 }
 
 // MLP features of a (point, cell) pair:
@@ -454,21 +527,22 @@ __global__ __launch_bounds__(64) void k_shell_reduce(int64_t nocc, const int32_t
 }
 
 
-// Near-field pair sums in sorted order: one workgroup per (occupied target
-// cell, 64-target chunk); sources = particles of the stencil cells.
+// Near-field pair sums in sorted order.  The particles are sorted by cell key
+// (ix * ny + iy) * nz + iz, so the cells of one stencil column (dx, dy, dz in
+// [-dz_max, dz_max]) hold one contiguous particle range [pstart[k0],
+// pstart[k1 + 1]).  One wavefront per (occupied target cell, kNearChunk-target
+// chunk, dx slice), two targets per lane, sources streamed 64 at a time
+// through LDS; each slice writes its own partial sums (summed in order later).
+constexpr int kNearChunk = 128;
+
 template <int KERNEL, int PREC, int MODE>
-__global__ __launch_bounds__(64) void k_near(int64_t nocc, const int32_t *__restrict__ occ,
-                                             const int32_t *__restrict__ cstart,
-                                             const int32_t *__restrict__ cend,
-                                             const int32_t *__restrict__ stencil, int nsten,
-                                             int nx, int ny, int nz, const double *__restrict__ sx,
-                                             const double *__restrict__ sy,
-                                             const double *__restrict__ sz,
-                                             const double *__restrict__ sax,
-                                             const double *__restrict__ say,
-                                             const double *__restrict__ saz,
-                                             const double *__restrict__ ssg,
-                                             double *__restrict__ out)
+__global__ __launch_bounds__(64) void k_near(
+    int64_t n, int64_t nocc, const int32_t *__restrict__ occ, const int32_t *__restrict__ cstart,
+    const int32_t *__restrict__ cend, const int32_t *__restrict__ pstart,
+    const int32_t *__restrict__ cols, const int32_t *__restrict__ slice_off, int nx, int ny,
+    int nz, const double *__restrict__ sx, const double *__restrict__ sy,
+    const double *__restrict__ sz, const double *__restrict__ sax, const double *__restrict__ say,
+    const double *__restrict__ saz, const double *__restrict__ ssg, double *__restrict__ npart)
 {
     using Op = PartOp<KERNEL, PREC, MODE>;
     __shared__ PartRec tile[64];
@@ -476,22 +550,29 @@ __global__ __launch_bounds__(64) void k_near(int64_t nocc, const int32_t *__rest
     if (o >= nocc) return;
     const int c = occ[o];
     const int tb = cstart[c], te = cend[c];
-    const int t0 = tb + (int)blockIdx.y * 64;
+    const int t0 = tb + (int)blockIdx.y * kNearChunk;
     if (t0 >= te) return;
-    const int ti = t0 + threadIdx.x;
-    const bool active = ti < te;
-    const int tl = active ? ti : t0;
-    typename Op::Tgt tg;
-    tg.x = sx[tl]; tg.y = sy[tl]; tg.z = sz[tl];
-    tg.ax = sax[tl]; tg.ay = say[tl]; tg.az = saz[tl];
-    typename Op::Acc acc;
-    for (int k = 0; k < 3; ++k) acc.v[k] = 0.0;
+    typename Op::Tgt tg[2];
+    typename Op::Acc acc[2];
+    bool act[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int ti = t0 + threadIdx.x + 64 * u;
+        act[u] = ti < te;
+        const int tl = act[u] ? ti : t0;
+        tg[u].x = sx[tl]; tg[u].y = sy[tl]; tg[u].z = sz[tl];
+        tg[u].ax = sax[tl]; tg[u].ay = say[tl]; tg[u].az = saz[tl];
+        for (int k = 0; k < 3; ++k) acc[u].v[k] = 0.0;
+    }
     const int cx = c / (ny * nz), cy = (c / nz) % ny, cz = c % nz;
-    for (int q = 0; q < nsten; ++q) {
-        const int ix = cx + stencil[3 * q], iy = cy + stencil[3 * q + 1], iz = cz + stencil[3 * q + 2];
-        if (ix < 0 || ix >= nx || iy < 0 || iy >= ny || iz < 0 || iz >= nz) continue;
-        const int nc = (ix * ny + iy) * nz + iz;
-        const int sb = cstart[nc], se = cend[nc];
+    const int sl = blockIdx.z;
+    for (int e = slice_off[sl]; e < slice_off[sl + 1]; ++e) {
+        const int ix = cx + cols[3 * e], iy = cy + cols[3 * e + 1], dzm = cols[3 * e + 2];
+        if (ix < 0 || ix >= nx || iy < 0 || iy >= ny) continue;
+        const int iz0 = cz - dzm < 0 ? 0 : cz - dzm;
+        const int iz1 = cz + dzm >= nz ? nz - 1 : cz + dzm;
+        const int kb = (ix * ny + iy) * nz;
+        const int sb = pstart[kb + iz0], se = pstart[kb + iz1 + 1];
         for (int base = sb; base < se; base += 64) {
             const int cnt = (se - base < 64) ? se - base : 64;
             __syncthreads();
@@ -501,25 +582,43 @@ __global__ __launch_bounds__(64) void k_near(int64_t nocc, const int32_t *__rest
                                                   saz[j], ssg[j]);
             }
             __syncthreads();
-            for (int k = 0; k < cnt; ++k) Op::pair(tg, tile[k], acc);
+            for (int k = 0; k < cnt; ++k) {
+                const PartRec sr = tile[k];
+                Op::pair(tg[0], sr, acc[0]);
+                Op::pair(tg[1], sr, acc[1]);
+            }
         }
     }
-    if (active)
-        for (int k = 0; k < 3; ++k) out[3 * (int64_t)ti + k] = acc.v[k];
+    double *dst = npart + (int64_t)sl * 3 * n;
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+        if (act[u]) {
+            const int64_t ti = t0 + threadIdx.x + 64 * u;
+            for (int k = 0; k < 3; ++k) dst[3 * ti + k] = acc[u].v[k];
+        }
 }
 
+// Population of every cell (empty cells 0) and one trailing zero, for the
+// exclusive scan that gives pstart.
+__global__ void k_cell_counts(int64_t ncell, const int32_t *__restrict__ cstart,
+                              const int32_t *__restrict__ cend, int32_t *__restrict__ cnt)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > ncell) return;
+    cnt[k] = k < ncell ? cend[k] - cstart[k] : 0;
+}
 
-// Far field of every far cell at each particle (sorted order) plus the near
-// sums and (mode 0) the learned residual of its cell; scatters to the original
-// order.  mode 0: velocity; 2: stretching J alpha; 3: vorticity curl u.
-// One particle per lane, source-cell moments staged in LDS.
+// Far field at each particle (sorted order) from the far cells of one slice of
+// the occupied cells (grid.y), as that slice's partial sums: mode 0 velocity;
+// 2 stretching J alpha; 3 vorticity curl u.  One particle per lane, the
+// slice's cell moments staged in LDS.
 template <int MODE>
-__global__ __launch_bounds__(256) void k_far_apply(
-    int64_t n, int64_t nocc, const int32_t *__restrict__ key_s, const int32_t *__restrict__ occ_of,
-    const int32_t *__restrict__ perm, const double *__restrict__ sx, const double *__restrict__ sy,
-    const double *__restrict__ sz, const double *__restrict__ sax, const double *__restrict__ say,
-    const double *__restrict__ saz, const double *__restrict__ mom, double rc2,
-    const double *__restrict__ res, const double *__restrict__ near, double *__restrict__ out)
+__global__ __launch_bounds__(256) void k_far_part(
+    int64_t n, int64_t nocc, int nfs, const int32_t *__restrict__ key_s,
+    const int32_t *__restrict__ occ_of, const double *__restrict__ sx,
+    const double *__restrict__ sy, const double *__restrict__ sz, const double *__restrict__ sax,
+    const double *__restrict__ say, const double *__restrict__ saz, const double *__restrict__ mom,
+    double rc2, double *__restrict__ fpart)
 {
     __shared__ double tile[256 * kMomTile];
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -529,9 +628,11 @@ __global__ __launch_bounds__(256) void k_far_apply(
     const double *mt = mom + t * kMomStride;
     const double c0 = mt[16], c1 = mt[17], c2 = mt[18];
     const double x0 = sx[il], x1 = sy[il], x2 = sz[il];
-    double u[3] = {0.0, 0.0, 0.0}, J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int64_t s0 = 0; s0 < nocc; s0 += 256) {
-        const int cnt = (int)((nocc - s0) < 256 ? (nocc - s0) : 256);
+    const double al[3] = {sax[il], say[il], saz[il]};
+    const int64_t sb = nocc * blockIdx.y / nfs, se = nocc * (blockIdx.y + 1) / nfs;
+    double r[3] = {0.0, 0.0, 0.0};
+    for (int64_t s0 = sb; s0 < se; s0 += 256) {
+        const int cnt = (int)((se - s0) < 256 ? (se - s0) : 256);
         __syncthreads();
         for (int e = threadIdx.x; e < cnt * kMomTile; e += 256) {
             const int k = e / kMomTile, q = e - k * kMomTile;
@@ -542,23 +643,34 @@ __global__ __launch_bounds__(256) void k_far_apply(
             const double *ms = tile + k * kMomTile;
             const double dx = c0 - ms[16], dy = c1 - ms[17], dz = c2 - ms[18];
             if (dx * dx + dy * dy + dz * dz <= rc2) continue;
-            far_expansion(ms, x0, x1, x2, u, MODE == 0 ? nullptr : J);
+            far_term<MODE>(ms, x0, x1, x2, al, r);
         }
     }
     if (!active) return;
-    double r[3];
-    if (MODE == 0) {
-        for (int a = 0; a < 3; ++a) r[a] = u[a] + (res ? res[3 * t + a] : 0.0);
-    } else if (MODE == 2) {
-        const double al[3] = {sax[i], say[i], saz[i]};
-        for (int a = 0; a < 3; ++a) r[a] = J[3 * a] * al[0] + J[3 * a + 1] * al[1] + J[3 * a + 2] * al[2];
-    } else {
-        r[0] = J[7] - J[5];
-        r[1] = J[2] - J[6];
-        r[2] = J[3] - J[1];
+    double *dst = fpart + (int64_t)blockIdx.y * 3 * n;
+    for (int a = 0; a < 3; ++a) dst[3 * i + a] = r[a];
+}
+
+// Near slices + far slices (+ the learned residual of the particle's cell) in
+// a fixed order, scattered to the original particle order.
+__global__ void k_combine(int64_t n, int nns, int nfs, const int32_t *__restrict__ key_s,
+                          const int32_t *__restrict__ occ_of, const int32_t *__restrict__ perm,
+                          const double *__restrict__ npart, const double *__restrict__ fpart,
+                          const double *__restrict__ res, double *__restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double r[3] = {0.0, 0.0, 0.0};
+    for (int sl = 0; sl < nns; ++sl)
+        for (int a = 0; a < 3; ++a) r[a] += npart[(int64_t)sl * 3 * n + 3 * i + a];
+    for (int sl = 0; sl < nfs; ++sl)
+        for (int a = 0; a < 3; ++a) r[a] += fpart[(int64_t)sl * 3 * n + 3 * i + a];
+    if (res) {
+        const int64_t t = occ_of[key_s[i]];
+        for (int a = 0; a < 3; ++a) r[a] += res[3 * t + a];
     }
     const int64_t j = perm[i];
-    for (int a = 0; a < 3; ++a) out[3 * j + a] = near[3 * i + a] + r[a];
+    for (int a = 0; a < 3; ++a) out[3 * j + a] = r[a];
 }
 
 __device__ __forceinline__ uint64_t mix64(uint64_t z)
@@ -670,8 +782,10 @@ static int grow(fv_ctx *ctx, void **p, size_t *cap, size_t bytes)
     return FV_OK;
 }
 
-// Near stencil: cell offsets whose grid-centre distance is <= kappa h, each
-// axis clamped to the grid (offsets beyond it can never land on a cell).
+// Near stencil as columns: for every (dx, dy) the cells dz in [-dz_max, dz_max]
+// with dx^2 + dy^2 + dz^2 <= kappa^2 (grid-centre distance <= kappa h), each
+// axis clamped to the grid (offsets beyond it can never land on a cell);
+// grouped into at most kMaxSlices contiguous dx ranges.
 static int update_stencil(fv_closure *cl, fv_ctx *ctx)
 {
     const int K = (int)ceil(cl->kappa);
@@ -679,24 +793,39 @@ static int update_stencil(fv_closure *cl, fv_ctx *ctx)
               kz = K < cl->nz - 1 ? K : cl->nz - 1;
     if (cl->dstencil && kx == cl->skx && ky == cl->sky && kz == cl->skz) return FV_OK;
     cl->stencil.clear();
-    const double k2 = cl->kappa * cl->kappa;
-    for (int i = -kx; i <= kx; ++i)
-        for (int j = -ky; j <= ky; ++j)
-            for (int k = -kz; k <= kz; ++k)
-                if ((double)i * i + (double)j * j + (double)k * k <= k2 + 1e-9) {
-                    cl->stencil.push_back(i);
-                    cl->stencil.push_back(j);
-                    cl->stencil.push_back(k);
-                }
+    cl->slice_off.clear();
+    const double k2 = cl->kappa * cl->kappa + 1e-9;
+    const int ndx = 2 * kx + 1;
+    cl->nsl = ndx < kMaxSlices ? ndx : kMaxSlices;
+    for (int sl = 0; sl < cl->nsl; ++sl) {
+        cl->slice_off.push_back((int)(cl->stencil.size() / 3));
+        const int d0 = -kx + (int)((int64_t)ndx * sl / cl->nsl);
+        const int d1 = -kx + (int)((int64_t)ndx * (sl + 1) / cl->nsl);
+        for (int i = d0; i < d1; ++i)
+            for (int j = -ky; j <= ky; ++j) {
+                const double r2 = k2 - (double)i * i - (double)j * j;
+                if (r2 < 0.0) continue;
+                int dz = (int)floor(sqrt(r2));
+                while ((double)(dz + 1) * (dz + 1) <= r2) ++dz;
+                while (dz > 0 && (double)dz * dz > r2) --dz;
+                if (dz > kz) dz = kz;
+                cl->stencil.push_back(i);
+                cl->stencil.push_back(j);
+                cl->stencil.push_back(dz);
+            }
+    }
+    cl->slice_off.push_back((int)(cl->stencil.size() / 3));
     if (cl->dstencil) {
         FV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
         FV_HIP_CHECK(ctx, hipFree(cl->dstencil));
         cl->dstencil = nullptr;
     }
-    FV_HIP_CHECK(ctx, hipMalloc(&cl->dstencil, sizeof(int32_t) * cl->stencil.size()));
-    FV_HIP_CHECK(ctx, hipMemcpyAsync(cl->dstencil, cl->stencil.data(),
-                                     sizeof(int32_t) * cl->stencil.size(), hipMemcpyHostToDevice,
-                                     ctx->stream));
+    const size_t nst = cl->stencil.size(), nso = cl->slice_off.size();
+    FV_HIP_CHECK(ctx, hipMalloc(&cl->dstencil, sizeof(int32_t) * (nst + nso)));
+    FV_HIP_CHECK(ctx, hipMemcpyAsync(cl->dstencil, cl->stencil.data(), sizeof(int32_t) * nst,
+                                     hipMemcpyHostToDevice, ctx->stream));
+    FV_HIP_CHECK(ctx, hipMemcpyAsync(cl->dstencil + nst, cl->slice_off.data(),
+                                     sizeof(int32_t) * nso, hipMemcpyHostToDevice, ctx->stream));
     FV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     cl->skx = kx;
     cl->sky = ky;
@@ -707,8 +836,11 @@ static int update_stencil(fv_closure *cl, fv_ctx *ctx)
 static int closure_residual(fv_vpm *f, bool use_mlp);
 
 // Bins the particles (cells, moments, shell pair list) for the current state
-// and, when `with_res`, computes the learned residual.
-static int closure_build(fv_vpm *f, bool use_mlp, bool with_res = true)
+// and, when `with_res`, computes the learned residual.  With `stage` (the RK
+// stages of one advect step, whose particles move a small fraction of a cell)
+// the step's binning is kept: cell membership, the shell list and the learned
+// residual stay, positions, strengths and moments are refreshed.
+static int closure_build(fv_vpm *f, bool use_mlp, bool with_res = true, bool stage = false)
 {
     fv_closure *cl = f->closure;
     fv_ctx *ctx = f->ctx;
@@ -716,12 +848,26 @@ static int closure_build(fv_vpm *f, bool use_mlp, bool with_res = true)
     int rc = f->np_exact ? FV_OK : vpm_sync_count(f);
     if (rc) return rc;
     const int64_t n = f->np_host;
+    if (stage && cl->binned_step == cl->steps && cl->n == n && n > 0 && cl->nocc > 0) {
+        const unsigned gb = (unsigned)ceil_div(n, 256);
+        hipLaunchKernelGGL(k_gather, dim3(gb), dim3(256), 0, ctx->stream, n, cl->perm, f->x, f->y,
+                           f->z, f->gx, f->gy, f->gz, f->sig, cl->sx, cl->sy, cl->sz, cl->sax,
+                           cl->say, cl->saz, cl->ssg);
+        FV_LAUNCH_CHECK(ctx);
+        hipLaunchKernelGGL(k_moments, dim3((unsigned)cl->nocc), dim3(64), 0, ctx->stream, cl->nocc,
+                           cl->occ, cl->cstart, cl->cend, cl->sx, cl->sy, cl->sz, cl->sax, cl->say,
+                           cl->saz, cl->ox, cl->oy, cl->oz, cl->hh, cl->ny, cl->nz, cl->mom);
+        FV_LAUNCH_CHECK(ctx);
+        cl->built_epoch = f->state_epoch;
+        return with_res ? closure_residual(f, use_mlp) : FV_OK;
+    }
     cl->n = n;
+    cl->bin_id++;
+    cl->binned_step = cl->steps;
     if (n == 0) {
         cl->nocc = 0;
         cl->nlist = 0;
         cl->built_epoch = f->state_epoch;
-        cl->res_epoch = -1;
         return FV_OK;
     }
     // bounding box
@@ -762,14 +908,16 @@ static int closure_build(fv_vpm *f, bool use_mlp, bool with_res = true)
     cl->oz = bb[2];
     cl->ncell = (int64_t)cl->nx * cl->ny * cl->nz;
     // particle-side buffers
-    const size_t need = (size_t)n * (4 * sizeof(int32_t) + 10 * sizeof(double)) + 1024;
+    const size_t need = (size_t)n * (4 * sizeof(int32_t) + (7 + 6 * kMaxSlices) * sizeof(double)) +
+                        1024;
     rc = grow(ctx, &cl->block, &cl->block_bytes, need);
     if (rc) return rc;
     char *p = static_cast<char *>(cl->block);
     auto takeD = [&](size_t k) { double *r = reinterpret_cast<double *>(p); p += k * sizeof(double); return r; };
     cl->sx = takeD(n); cl->sy = takeD(n); cl->sz = takeD(n);
     cl->sax = takeD(n); cl->say = takeD(n); cl->saz = takeD(n); cl->ssg = takeD(n);
-    cl->near_out = takeD(3 * n);
+    cl->npart = takeD(3 * kMaxSlices * n);
+    cl->fpart = takeD(3 * kMaxSlices * n);
     cl->key = reinterpret_cast<int32_t *>(p); p += n * sizeof(int32_t);
     cl->key_s = reinterpret_cast<int32_t *>(p); p += n * sizeof(int32_t);
     cl->idx = reinterpret_cast<int32_t *>(p); p += n * sizeof(int32_t);
@@ -787,19 +935,33 @@ static int closure_build(fv_vpm *f, bool use_mlp, bool with_res = true)
     FV_HIP_CHECK(ctx, hipcub::DeviceRadixSort::SortPairs(cl->tmp, tb, cl->key, cl->key_s, cl->idx,
                                                         cl->perm, (int)n, 0, 32, ctx->stream));
     // per-cell arrays
-    const size_t cneed = (size_t)cl->ncell * 4 * sizeof(int32_t) + 1024;
+    const size_t cneed = ((size_t)cl->ncell * 6 + 2) * sizeof(int32_t) + 1024;
     rc = grow(ctx, &cl->cells, &cl->cells_bytes, cneed);
     if (rc) return rc;
     cl->cstart = static_cast<int32_t *>(cl->cells);
     cl->cend = cl->cstart + cl->ncell;
     cl->occ = cl->cend + cl->ncell;
     cl->occ_of = cl->occ + cl->ncell;
+    cl->ccount = cl->occ_of + cl->ncell;
+    cl->pstart = cl->ccount + cl->ncell + 1;
     hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)ceil_div(2 * cl->ncell, 256)), dim3(256), 0,
                        ctx->stream, cl->cstart, 2 * cl->ncell, 0);
     FV_LAUNCH_CHECK(ctx);
     hipLaunchKernelGGL(k_cell_ranges, dim3(gb), dim3(256), 0, ctx->stream, n, cl->key_s, cl->cstart,
                        cl->cend);
     FV_LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_cell_counts, dim3((unsigned)ceil_div(cl->ncell + 1, 256)), dim3(256), 0,
+                       ctx->stream, cl->ncell, cl->cstart, cl->cend, cl->ccount);
+    FV_LAUNCH_CHECK(ctx);
+    {
+        size_t pb = 0;
+        FV_HIP_CHECK(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, pb, cl->ccount, cl->pstart,
+                                                          (int)(cl->ncell + 1), ctx->stream));
+        rc = grow(ctx, &cl->tmp, &cl->tmp_bytes, pb);
+        if (rc) return rc;
+        FV_HIP_CHECK(ctx, hipcub::DeviceScan::ExclusiveSum(cl->tmp, pb, cl->ccount, cl->pstart,
+                                                          (int)(cl->ncell + 1), ctx->stream));
+    }
     hipLaunchKernelGGL(k_gather, dim3(gb), dim3(256), 0, ctx->stream, n, cl->perm, f->x, f->y, f->z,
                        f->gx, f->gy, f->gz, f->sig, cl->sx, cl->sy, cl->sz, cl->sax, cl->say,
                        cl->saz, cl->ssg);
@@ -865,17 +1027,17 @@ static int closure_build(fv_vpm *f, bool use_mlp, bool with_res = true)
     rc = update_stencil(cl, ctx);
     if (rc) return rc;
     cl->built_epoch = f->state_epoch;
-    cl->res_epoch = -1;
     return with_res ? closure_residual(f, use_mlp) : FV_OK;
 }
 
-// Learned residual per occupied cell (zero without a network or shell pairs).
+// Learned residual per occupied cell (zero without a network or shell pairs),
+// once per binning and network version.
 static int closure_residual(fv_vpm *f, bool use_mlp)
 {
     fv_closure *cl = f->closure;
     fv_ctx *ctx = f->ctx;
     use_mlp = use_mlp && cl->mlp && cl->nlist > 0;
-    if (cl->res_epoch == cl->built_epoch && cl->res_mlp == (int)use_mlp &&
+    if (cl->res_bin == cl->bin_id && cl->res_mlp == (int)use_mlp &&
         (!use_mlp || cl->res_version == cl->mlp_version))
         return FV_OK;
     const int64_t nocc = cl->nocc;
@@ -897,7 +1059,7 @@ static int closure_residual(fv_vpm *f, bool use_mlp)
         int rc = fill(ctx, cl->res, 3 * nocc, 0.0);
         if (rc) return rc;
     }
-    cl->res_epoch = cl->built_epoch;
+    cl->res_bin = cl->bin_id;
     cl->res_mlp = (int)use_mlp;
     cl->res_version = cl->mlp_version;
     return FV_OK;
@@ -906,33 +1068,43 @@ static int closure_residual(fv_vpm *f, bool use_mlp)
 template <int K, int P, int M>
 static void launch_near(fv_closure *cl, fv_ctx *ctx, unsigned chunks)
 {
-    hipLaunchKernelGGL((k_near<K, P, M>), dim3((unsigned)cl->nocc, chunks), dim3(64), 0,
-                       ctx->stream, cl->nocc, cl->occ, cl->cstart, cl->cend, cl->dstencil,
-                       (int)(cl->stencil.size() / 3), cl->nx, cl->ny, cl->nz, cl->sx, cl->sy,
-                       cl->sz, cl->sax, cl->say, cl->saz, cl->ssg, cl->near_out);
+    const size_t nst = cl->stencil.size();
+    hipLaunchKernelGGL((k_near<K, P, M>), dim3((unsigned)cl->nocc, chunks, (unsigned)cl->nsl),
+                       dim3(64), 0, ctx->stream, cl->n, cl->nocc, cl->occ, cl->cstart, cl->cend,
+                       cl->pstart, cl->dstencil, cl->dstencil + nst, cl->nx, cl->ny, cl->nz,
+                       cl->sx, cl->sy, cl->sz, cl->sax, cl->say, cl->saz, cl->ssg, cl->npart);
+}
+
+// Far slices: enough source-cell ranges for ~8 wavefronts per SIMD.
+static int far_slices(const fv_closure *cl)
+{
+    int64_t fs = ceil_div(int64_t(1) << 19, cl->n > 0 ? cl->n : 1);
+    if (fs > kMaxSlices) fs = kMaxSlices;
+    if (fs > cl->nocc) fs = cl->nocc;
+    return fs < 1 ? 1 : (int)fs;
 }
 
 template <int M>
-static void launch_far(fv_closure *cl, fv_ctx *ctx, double rc2, bool res, double *out)
+static void launch_far(fv_closure *cl, fv_ctx *ctx, double rc2, int nfs)
 {
-    hipLaunchKernelGGL((k_far_apply<M>), dim3((unsigned)ceil_div(cl->n, 256)), dim3(256), 0,
-                       ctx->stream, cl->n, cl->nocc, cl->key_s, cl->occ_of, cl->perm, cl->sx,
-                       cl->sy, cl->sz, cl->sax, cl->say, cl->saz, cl->mom, rc2,
-                       res ? cl->res : (const double *)nullptr, cl->near_out, out);
+    hipLaunchKernelGGL((k_far_part<M>), dim3((unsigned)ceil_div(cl->n, 256), (unsigned)nfs),
+                       dim3(256), 0, ctx->stream, cl->n, cl->nocc, nfs, cl->key_s, cl->occ_of,
+                       cl->sx, cl->sy, cl->sz, cl->sax, cl->say, cl->saz, cl->mom, rc2, cl->fpart);
 }
 
 // Closure sum at the particles (original order) for mode 0 (velocity), 2
 // (stretching) or 3 (vorticity); kernel selects the near-field smoothing.
-static int closure_pass(fv_vpm *f, int mode, int kernel, double *out, bool use_mlp)
+static int closure_pass(fv_vpm *f, int mode, int kernel, double *out, bool use_mlp,
+                        bool stage = false)
 {
     fv_closure *cl = f->closure;
     fv_ctx *ctx = f->ctx;
-    int rc = closure_build(f, use_mlp, mode == 0);
+    int rc = closure_build(f, use_mlp, mode == 0, stage);
     if (rc) return rc;
     const int64_t n = cl->n;
     if (n == 0) return FV_OK;
     // the largest cell population bounds the target chunks per cell
-    const unsigned chunks = (unsigned)ceil_div(cl->maxpop > 0 ? cl->maxpop : 1, 64);
+    const unsigned chunks = (unsigned)ceil_div(cl->maxpop > 0 ? cl->maxpop : 1, kNearChunk);
     {
         TimedLaunch tl(ctx, "closure_near");
         const int prec = f->precision;
@@ -951,15 +1123,20 @@ static int closure_pass(fv_vpm *f, int mode, int kernel, double *out, bool use_m
         FV_LAUNCH_CHECK(ctx);
         tl.done();
     }
+    const int nfs = far_slices(cl);
     {
         TimedLaunch tl(ctx, "closure_far");
         const double rc2 = (cl->kappa * cl->hh) * (cl->kappa * cl->hh) * (1.0 + 1e-12);
-        if (mode == 0) launch_far<0>(cl, ctx, rc2, true, out);
-        else if (mode == 2) launch_far<2>(cl, ctx, rc2, false, out);
-        else launch_far<3>(cl, ctx, rc2, false, out);
+        if (mode == 0) launch_far<0>(cl, ctx, rc2, nfs);
+        else if (mode == 2) launch_far<2>(cl, ctx, rc2, nfs);
+        else launch_far<3>(cl, ctx, rc2, nfs);
         FV_LAUNCH_CHECK(ctx);
         tl.done();
     }
+    hipLaunchKernelGGL(k_combine, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, ctx->stream, n,
+                       cl->nsl, nfs, cl->key_s, cl->occ_of, cl->perm, cl->npart, cl->fpart,
+                       mode == 0 ? cl->res : (const double *)nullptr, out);
+    FV_LAUNCH_CHECK(ctx);
     return FV_OK;
 }
 
@@ -1035,11 +1212,13 @@ int closure_step(fv_vpm *f)
 {
     fv_closure *cl = f->closure;
     if (!cl) return FV_OK;
+    cl->steps++;   // this advect step; its stages reuse the binning made here
+    cl->built_epoch = -1;
     int rc = closure_build(f, false, false);
     if (rc) return rc;
     rc = closure_train(f);
     if (rc) return rc;
-    if (cl->steps % cl->check_every == 0) {
+    if ((cl->steps - 1) % cl->check_every == 0) {
         double err = 0.0;
         rc = closure_check(f, true, &err);
         if (rc) return rc;
@@ -1048,7 +1227,6 @@ int closure_step(fv_vpm *f)
         cl->active = err <= cl->tol;
         if (!cl->active) cl->rejections++;
     }
-    cl->steps++;
     return FV_OK;
 }
 
@@ -1056,7 +1234,7 @@ bool closure_active(const fv_vpm *f) { return f->closure && f->closure->active; 
 
 int closure_self_sum(fv_vpm *f, int mode, int kernel, double *out)
 {
-    return closure_pass(f, mode, kernel, out, true);
+    return closure_pass(f, mode, kernel, out, true, true);
 }
 
 void closure_destroy(fv_closure *cl)
@@ -1133,6 +1311,31 @@ int fv_vpm_closure_velocity_host(fv_vpm *f, int32_t use_mlp, double *out, double
     if (out && f->closure->n)
         FV_HIP_CHECK(ctx, hipMemcpyAsync(out, f->closure->vclo,
                                          sizeof(double) * 3 * f->closure->n,
+                                         hipMemcpyDeviceToHost, ctx->stream));
+    FV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return FV_OK;
+}
+
+// One closure pass (mode 0 velocity, 2 stretching J alpha, 3 vorticity) at the
+// particles, host copy in the original order.
+int fv_vpm_closure_sum_host(fv_vpm *f, int32_t mode, int32_t kernel, int32_t use_mlp,
+                            double *out)
+{
+    if (!f || !f->closure || !out) return FV_E_ARG;
+    fv_ctx *ctx = f->ctx;
+    FV_REQUIRE(ctx, mode == 0 || mode == 2 || mode == 3, "closure mode must be 0, 2 or 3");
+    FV_REQUIRE(ctx, kernel == FV_KERNEL_GAUSSIAN_ERF || kernel == FV_KERNEL_WINCKELMANS,
+               "unknown kernel");
+    fv_closure *cl = f->closure;
+    if (!cl->vdir) {
+        FV_HIP_CHECK(ctx, hipMalloc(&cl->vdir, sizeof(double) * (6 * (size_t)f->cap + 8)));
+        cl->vclo = cl->vdir + 3 * (size_t)f->cap;
+        cl->verr = cl->vclo + 3 * (size_t)f->cap;
+    }
+    int rc = closure_pass(f, mode, kernel, cl->vclo, use_mlp != 0);
+    if (rc) return rc;
+    if (cl->n)
+        FV_HIP_CHECK(ctx, hipMemcpyAsync(out, cl->vclo, sizeof(double) * 3 * cl->n,
                                          hipMemcpyDeviceToHost, ctx->stream));
     FV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     return FV_OK;

@@ -427,7 +427,7 @@ static int hybrid_step_one(fv_hybrid *h)
     if (prec == FV_PREC_FP64)
         rc = dense_solve(ctx, 1, P, h->aic, h->rhs, h->info);
     else
-        rc = dense_solve_mixed(ctx, 1, P, h->aic, h->rhs, 8, 1e-14, h->info, nullptr);
+        rc = dense_solve_mixed(ctx, 1, P, h->aic, h->rhs, 8, 1e-12, h->info, nullptr);
     if (rc) return rc;
     hipLaunchKernelGGL(k_copy, dim3((unsigned)ceil_div(P, 128)), dim3(128), 0, ctx->stream,
                        h->rhs, h->gam, P);

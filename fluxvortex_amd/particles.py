@@ -183,6 +183,17 @@ class VortexParticleField:
             self._h, 1 if use_mlp else 0, _lib.ptr(out), C.byref(err)))
         return out, err.value
 
+    def closure_sum(self, mode="velocity", kernel=None, use_mlp=True):
+        """One closure pass at the particles: "velocity", "stretching" (J alpha)
+        or "vorticity"; kernel defaults to the field's (the relaxation uses
+        "gaussianerf")."""
+        m = {"velocity": 0, "stretching": 2, "vorticity": 3}[mode]
+        k = _lib.KERNELS[kernel or self.kernel]
+        out = np.zeros((self.np, 3))
+        self._ctx.check(self._ctx.lib.fv_vpm_closure_sum_host(self._h, m, k, 1 if use_mlp else 0,
+                                                              _lib.ptr(out)))
+        return out
+
     def closure_train(self, rounds=1):
         self._ctx.check(self._ctx.lib.fv_vpm_closure_train(self._h, int(rounds)))
 

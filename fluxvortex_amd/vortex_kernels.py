@@ -149,8 +149,10 @@ def aic(colloc, normals, br, fr, fl, bl, r_c0s, nu=0.0):
     return out
 
 
-def dense_solve(A, b):
-    """x = A^-1 b by LU with partial pivoting on the GPU; A (n, n) or (batch, n, n)."""
+def dense_solve(A, b, precision="fp64", max_iter=8, tol=1e-12, return_iters=False):
+    """x = A^-1 b by LU with partial pivoting on the GPU; A (n, n) or (batch, n, n).
+    precision "mixed": FP32 factors refined with FP64 residuals until the
+    relative correction is below tol (at most max_iter refinement steps)."""
     A = np.array(A, dtype=np.float64, order="C", copy=True)
     x = np.array(b, dtype=np.float64, order="C", copy=True)
     single = A.ndim == 2
@@ -161,10 +163,21 @@ def dense_solve(A, b):
     if n != n2 or x.shape != (bsz, n):
         raise ValueError(f"dense_solve: shapes {A.shape} and {x.shape} do not match")
     info = np.zeros(bsz, dtype=np.int32)
+    iters = np.zeros(bsz, dtype=np.int32)
     if bsz and n:
         ctx = _lib.default_context()
-        ctx.check(ctx.lib.fv_dense_solve_host(ctx.handle, bsz, n, _lib.ptr(A), _lib.ptr(x),
-                                              info.ctypes.data_as(_lib._i32p)))
+        if precision == "fp64":
+            ctx.check(ctx.lib.fv_dense_solve_host(ctx.handle, bsz, n, _lib.ptr(A), _lib.ptr(x),
+                                                  info.ctypes.data_as(_lib._i32p)))
+        elif precision == "mixed":
+            ctx.check(ctx.lib.fv_dense_solve_mixed_host(
+                ctx.handle, bsz, n, _lib.ptr(A), _lib.ptr(x), int(max_iter), float(tol),
+                info.ctypes.data_as(_lib._i32p), iters.ctypes.data_as(_lib._i32p)))
+        else:
+            raise ValueError(f"unknown precision {precision!r}")
     if np.any(info != 0):
         raise np.linalg.LinAlgError(f"Singular matrix (zero pivot, info={info.tolist()})")
-    return x[0] if single else x
+    out = x[0] if single else x
+    if return_iters:
+        return out, (iters[0] if single else iters)
+    return out

@@ -119,7 +119,9 @@ int fv_aic_dev(fv_ctx *ctx, const double *colloc, const double *normals, int64_t
 /* ------------------------------------------------- dense solve (a7) */
 /* Solves A_b x_b = rhs_b for `batch` independent row-major n x n systems
  * (LU with partial pivoting; A overwritten by its factors, rhs by x).
- * info[b] = 0 or k+1 when column k has an exactly zero pivot.
+ * info[b] = 0 or k+1 when column k has an exactly zero pivot.  One
+ * workgroup per system; n <= 1024 (the factorisation stages n x 17 panels
+ * in the 160 KB LDS; n <= 141 keeps the whole FP64 matrix there).
  * Replaces np.linalg.solve in standalone_uvlm.py:427 / PteraSoftware's
  * _calculate_vortex_strengths, and warp_fsi/batched_solver.py:20-63. */
 int fv_dense_solve_host(fv_ctx *ctx, int64_t batch, int64_t n, double *A, double *rhs,
@@ -132,6 +134,11 @@ int fv_dense_solve_dev(fv_ctx *ctx, int64_t batch, int64_t n, double *A, double 
 int fv_dense_solve_mixed_dev(fv_ctx *ctx, int64_t batch, int64_t n, const double *A,
                              double *rhs, int32_t max_iter, double tol, int32_t *info,
                              int32_t *iters);
+/* Host twin of fv_dense_solve_mixed_dev (copies in and out, synchronous);
+ * info and iters (nullable) have batch entries. */
+int fv_dense_solve_mixed_host(fv_ctx *ctx, int64_t batch, int64_t n, const double *A,
+                              double *rhs, int32_t max_iter, double tol, int32_t *info,
+                              int32_t *iters);
 
 /* ------------------------------------------- particle Biot-Savart (a8, a9) */
 /* u(x_t) = -(1/4pi) sum_j g(r/sigma_j)/r^3 (x_t - x_j) x alpha_j        (t, 3)
@@ -274,6 +281,12 @@ int fv_vpm_closure_stats(fv_vpm *f, double *last_err, int64_t *checks, int64_t *
 /* Closure velocity at the particles (host, n x 3; nullable) with or without
  * the learned residual and its normwise error against the exact sum. */
 int fv_vpm_closure_velocity_host(fv_vpm *f, int32_t use_mlp, double *out, double *err);
+/* One closure pass at the particles (original order, n x 3, host): mode 0
+ * velocity, 2 stretching J alpha, 3 vorticity; kernel selects the near-field
+ * smoothing (the reference's relaxation always uses FV_KERNEL_GAUSSIAN_ERF,
+ * particles.py:219).  No reference counterpart (tests of the closure). */
+int fv_vpm_closure_sum_host(fv_vpm *f, int32_t mode, int32_t kernel, int32_t use_mlp,
+                            double *out);
 /* Runs `rounds` online-training rounds on the current particle state. */
 int fv_vpm_closure_train(fv_vpm *f, int32_t rounds);
 

@@ -100,3 +100,31 @@ def test_advect_with_closure_checks_and_stays_close(ctx, oracle):
         assert st["last_err"] <= 1e-3
     pa, pb = a.positions, b.positions
     assert relerr(pb - pos, pa - pos) < 1e-3
+
+
+@pytest.mark.parametrize("kernel", ["winckelmans", "gaussianerf"])
+def test_closure_modes_match_direct_sums(ctx, kernel):
+    """Velocity, stretching J alpha and vorticity passes of the closure (near
+    exact, far first-order expansion, no network) against the direct sums
+    built from the GPU's exact pair kernels; the far share is small at
+    kappa = 5, so each mode agrees to a few 1e-4 of its max norm."""
+    from fluxvortex_amd import kernel as K
+    from fluxvortex_amd.particles import VortexParticleField
+    pos, al, sg = wake_sheet(nx=400)
+    f = VortexParticleField(max_particles=len(sg) + 8, rlxf=0.0, kernel="winckelmans")
+    f.add_particles_batch(pos, al, sg)
+    f.enable_closure(None, cell=2.0, kappa=5.0)
+    u = K.velocity_from_particles(pos, pos, al, sg, kernel=kernel)
+    J = K.jacobian_from_particles(pos, al, pos, al, sg, kernel=kernel)
+    S = np.einsum("nij,nj->ni", J, al)
+    w = np.stack([J[:, 2, 1] - J[:, 1, 2], J[:, 0, 2] - J[:, 2, 0], J[:, 1, 0] - J[:, 0, 1]], 1)
+    for mode, ref in (("velocity", u), ("stretching", S), ("vorticity", w)):
+        got = f.closure_sum(mode, kernel=kernel, use_mlp=False)
+        err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+        print(f"{kernel} {mode}: closure max-norm error {err:.2e}")
+        assert err < 1e-3, (mode, err)
+    # every cell near: exact up to summation order, all three modes
+    f.enable_closure(None, cell=2.0, kappa=1e3)
+    for mode, ref in (("velocity", u), ("stretching", S), ("vorticity", w)):
+        got = f.closure_sum(mode, kernel=kernel, use_mlp=False)
+        assert relerr(got, ref) < 1e-12, mode

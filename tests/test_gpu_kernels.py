@@ -147,7 +147,7 @@ def test_aic_matches_oracle(vk, oracle):
 
 def test_dense_solve(vk):
     rng = np.random.default_rng(5)
-    for n in (1, 5, 64, 126, 252, 300):
+    for n in (1, 5, 64, 126, 141, 142, 252, 300, 1024):
         A = rng.standard_normal((n, n)) + 0.1 * n * np.eye(n)
         b = rng.standard_normal(n)
         x = vk.dense_solve(A, b)
@@ -212,3 +212,23 @@ def test_mixed_tiny_cores_stay_finite(oracle, ctx, kernel):
     assert np.all(np.isfinite(um)) and np.all(np.isfinite(Jm))
     assert relerr(um, oracle.particle_vel(tgt, pos, al, sg, kernel)) < 1e-5
     assert relerr(Jm, oracle.particle_jac(tgt, pos, al, sg, kernel)) < 1e-5
+
+
+def test_dense_solve_mixed(vk):
+    """FP32 LU + FP64-residual refinement reaches FP64 accuracy on
+    well-conditioned systems (whole matrix in LDS up to n = 200, staged
+    panels beyond), within the 8 refinement steps allowed."""
+    rng = np.random.default_rng(6)
+    for n in (1, 7, 64, 120, 200, 201, 300):
+        A = rng.standard_normal((n, n)) + 0.1 * n * np.eye(n)
+        b = rng.standard_normal(n)
+        x, it = vk.dense_solve(A, b, precision="mixed", return_iters=True)
+        ref = np.linalg.solve(A, b)
+        assert relerr(x, ref) < 1e-11, (n, relerr(x, ref))
+        assert it <= 8, (n, it)
+    A = rng.standard_normal((5, 40, 40)) + 4 * np.eye(40)
+    b = rng.standard_normal((5, 40))
+    x = vk.dense_solve(A, b, precision="mixed")
+    assert relerr(x, np.linalg.solve(A, b[..., None])[..., 0]) < 1e-11
+    with pytest.raises(np.linalg.LinAlgError):
+        vk.dense_solve(np.zeros((4, 4)), np.ones(4), precision="mixed")
