@@ -46,6 +46,7 @@ constexpr int kFeat = 32;           // MLP input width (31 features + pad)
 constexpr int kOutW = 16;           // MLP output width (3 used)
 constexpr double kShell = 2.0;      // residual shell: kappa h < R <= kShell kappa h
 constexpr int kMaxSlices = 16;      // near (dx ranges) and far (source-cell ranges) slices
+constexpr int kFarTile = 64;        // source cells staged per far-field tile
 
 struct fv_closure_state {
     fv_mlp *mlp = nullptr;
@@ -620,7 +621,8 @@ __global__ __launch_bounds__(256) void k_far_part(
     const double *__restrict__ say, const double *__restrict__ saz, const double *__restrict__ mom,
     double rc2, double *__restrict__ fpart)
 {
-    __shared__ double tile[256 * kMomTile];
+    // 64-cell tiles (9.7 KB of LDS) keep ~7 wavefronts per SIMD resident
+    __shared__ double tile[kFarTile * kMomTile];
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const bool active = i < n;
     const int64_t il = active ? i : n - 1;
@@ -631,8 +633,8 @@ __global__ __launch_bounds__(256) void k_far_part(
     const double al[3] = {sax[il], say[il], saz[il]};
     const int64_t sb = nocc * blockIdx.y / nfs, se = nocc * (blockIdx.y + 1) / nfs;
     double r[3] = {0.0, 0.0, 0.0};
-    for (int64_t s0 = sb; s0 < se; s0 += 256) {
-        const int cnt = (int)((se - s0) < 256 ? (se - s0) : 256);
+    for (int64_t s0 = sb; s0 < se; s0 += kFarTile) {
+        const int cnt = (int)((se - s0) < kFarTile ? (se - s0) : kFarTile);
         __syncthreads();
         for (int e = threadIdx.x; e < cnt * kMomTile; e += 256) {
             const int k = e / kMomTile, q = e - k * kMomTile;

@@ -123,31 +123,38 @@ __device__ __forceinline__ void trailing_update(double *a, int lda, int n, int k
     }
 }
 
-// FP32 factors (mixed precision): four rows of one column per lane, every
-// load issued before the stores.
+typedef float fv_f4x __attribute__((ext_vector_type(4)));
+
+// FP32 factors (mixed precision) on v_mfma_f32_16x16x4f32: operands as the
+// FP64 form; result element v of lane l: row 4 (l / 16) + v, column l % 16.
 __device__ __forceinline__ void trailing_update(float *a, int lda, int n, int k0, int kb)
 {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     const int r0 = k0 + kb, mm = n - r0;
-    const int rq = (mm + 3) >> 2;
-    for (int e = threadIdx.x; e < rq * mm; e += blockDim.x) {
-        const int q4 = e / mm, c = r0 + (e - q4 * mm);
-        const int rb = r0 + 4 * q4;
-        float u[kNb];
+    const int tiles = (mm + 15) >> 4;
+    const int lr = lane & 15, lk = lane >> 4;
+    for (int t = wave; t < tiles * tiles; t += nw) {
+        const int ti = t / tiles, tj = t - ti * tiles;
+        const int R0 = r0 + 16 * ti, col = r0 + 16 * tj + lr;
+        fv_f4x acc;
 #pragma unroll
-        for (int q = 0; q < kNb; ++q) u[q] = q < kb ? a[(int64_t)(k0 + q) * lda + c] : 0.0f;
-        float s[4];
+        for (int v = 0; v < 4; ++v) {
+            const int row = R0 + 4 * lk + v;
+            acc[v] = (row < n && col < n) ? a[(int64_t)row * lda + col] : 0.0f;
+        }
+        const int ra = R0 + lr;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int r = rb + i;
-            float acc = r < n ? a[(int64_t)r * lda + c] : 0.0f;
-#pragma unroll
-            for (int q = 0; q < kNb; ++q)
-                if (q < kb && r < n) acc -= a[(int64_t)r * lda + k0 + q] * u[q];
-            s[i] = acc;
+        for (int ks = 0; ks < kNb; ks += 4) {
+            const int k = ks + lk;
+            const float av = (ra < n && k < kb) ? a[(int64_t)ra * lda + k0 + k] : 0.0f;
+            const float bv = (col < n && k < kb) ? a[(int64_t)(k0 + k) * lda + col] : 0.0f;
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(-av, bv, acc, 0, 0, 0);
         }
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
-            if (rb + i < n) a[(int64_t)(rb + i) * lda + c] = s[i];
+        for (int v = 0; v < 4; ++v) {
+            const int row = R0 + 4 * lk + v;
+            if (row < n && col < n) a[(int64_t)row * lda + col] = acc[v];
+        }
     }
 }
 
@@ -287,7 +294,8 @@ __device__ __forceinline__ int lu_blocked(T *a, int lda, int n, int *perm, TB *b
 // and column k is eliminated from the rows above.  x lives in registers
 // (lane l holds x[l + 64 q]) for n <= 256, else in LDS.
 template <typename T, typename TB>
-__device__ __forceinline__ void wave_back_subst(const T *a, int lda, int n, TB *x)
+__device__ __forceinline__ void wave_back_subst(const T *a, int lda, int n, TB *x,
+                                                const TB *rdiag)
 {
     if ((threadIdx.x >> 6) != 0) return;
     const int lane = threadIdx.x & 63;
@@ -298,7 +306,7 @@ __device__ __forceinline__ void wave_back_subst(const T *a, int lda, int n, TB *
         for (int k = n - 1; k >= 0; --k) {
             const int ks = k >> 6, kl = k & 63;
             const TB own = ks == 0 ? xr[0] : ks == 1 ? xr[1] : ks == 2 ? xr[2] : xr[3];
-            const TB xk = readlane_d((double)own, kl) / (TB)a[(int64_t)k * lda + k];
+            const TB xk = readlane_d((double)own, kl) * rdiag[k];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int i = lane + 64 * q;
@@ -313,7 +321,7 @@ __device__ __forceinline__ void wave_back_subst(const T *a, int lda, int n, TB *
         return;
     }
     for (int k = n - 1; k >= 0; --k) {
-        const TB xk = x[k] / (TB)a[(int64_t)k * lda + k];
+        const TB xk = x[k] * rdiag[k];
         for (int i = lane; i < k; i += 64) x[i] -= (TB)a[(int64_t)i * lda + k] * xk;
         wave_sync();
         if (lane == 0) x[k] = xk;
@@ -399,7 +407,8 @@ __global__ __launch_bounds__(kLuBlock) void k_lu_solve_f64(int n, double *__rest
     double *bg = B + (int64_t)blockIdx.x * n;
     double *area = reinterpret_cast<double *>(lds_raw);
     double *b = reinterpret_cast<double *>(lds_raw + mat_bytes);
-    int *perm = reinterpret_cast<int *>(b + n);
+    double *rdiag = b + n;
+    int *perm = reinterpret_cast<int *>(rdiag + n);
     double *w = IN_LDS ? area : a;
     const int lda = IN_LDS ? lda_lds : n;
     if (IN_LDS) {
@@ -411,7 +420,9 @@ __global__ __launch_bounds__(kLuBlock) void k_lu_solve_f64(int n, double *__rest
     for (int i = threadIdx.x; i < n; i += kLuBlock) b[i] = bg[i];
     __syncthreads();
     const int inf = lu_blocked<double, double>(w, lda, n, perm, b, IN_LDS ? nullptr : area);
-    if (inf == 0) wave_back_subst<double, double>(w, lda, n, b);
+    for (int i = threadIdx.x; i < n; i += kLuBlock) rdiag[i] = 1.0 / w[(int64_t)i * lda + i];
+    __syncthreads();
+    if (inf == 0) wave_back_subst<double, double>(w, lda, n, b, rdiag);
     __syncthreads();
     if (IN_LDS)
         for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += kLuBlock) {
@@ -437,7 +448,8 @@ __global__ __launch_bounds__(kLuBlock) void k_lu_solve_mixed(
     double *x = reinterpret_cast<double *>(lds_raw + mat_bytes);
     double *r = x + n;
     double *d = r + n;
-    int *perm = reinterpret_cast<int *>(d + n);
+    double *rdiag = d + n;
+    int *perm = reinterpret_cast<int *>(rdiag + n);
     int *order = perm + n;
     float *lu = IN_LDS ? area : f32 + (int64_t)blockIdx.x * n * n;
     const int lda = IN_LDS ? lda_lds : n;
@@ -460,6 +472,7 @@ __global__ __launch_bounds__(kLuBlock) void k_lu_solve_mixed(
         x[i] = 0.0;
         r[i] = bg[i];
         order[i] = i;
+        rdiag[i] = 1.0 / (double)lu[(int64_t)i * lda + i];
     }
     __syncthreads();
     // the pivot sequence composed into one row order: (P v)[i] = v[order[i]]
@@ -479,7 +492,7 @@ __global__ __launch_bounds__(kLuBlock) void k_lu_solve_mixed(
         for (int i = threadIdx.x; i < n; i += kLuBlock) d[i] = r[order[i]];
         __syncthreads();
         wave_fwd_subst<float, double>(lu, lda, n, d);
-        wave_back_subst<float, double>(lu, lda, n, d);
+        wave_back_subst<float, double>(lu, lda, n, d, rdiag);
         __syncthreads();
         double dmax = 0.0, xmax = 0.0;
         if (threadIdx.x < 64) {
@@ -500,12 +513,21 @@ __global__ __launch_bounds__(kLuBlock) void k_lu_solve_mixed(
         __syncthreads();
         const bool done = stat[0] <= tol * stat[1];
         if (done || it == max_iter) break;
-        // FP64 residual r = b - A x
-        for (int i = threadIdx.x; i < n; i += kLuBlock) {
-            double s = bg[i];
+        // FP64 residual r = b - A x: one wavefront per row, coalesced row reads,
+        // DPP sum (row_shr 1/2/4/8, row_bcast 15/31; lane 63 holds the sum)
+        for (int i = threadIdx.x >> 6; i < n; i += kLuBlock >> 6) {
+            const int lane = threadIdx.x & 63;
             const double *row = a + (int64_t)i * n;
-            for (int c = 0; c < n; ++c) s = fma(-row[c], x[c], s);
-            r[i] = s;
+            double s = 0.0;
+            for (int c = lane; c < n; c += 64) s = fma(row[c], x[c], s);
+            s += dpp_d<0x111, 0xf>(0.0, s);
+            s += dpp_d<0x112, 0xf>(0.0, s);
+            s += dpp_d<0x114, 0xf>(0.0, s);
+            s += dpp_d<0x118, 0xf>(0.0, s);
+            s += dpp_d<0x142, 0xa>(0.0, s);
+            s += dpp_d<0x143, 0xc>(0.0, s);
+            const double tot = readlane_d(s, 63);
+            if (lane == 0) r[i] = bg[i] - tot;
         }
         __syncthreads();
     }
@@ -530,7 +552,7 @@ static size_t max_lds(fv_ctx *ctx)
 int dense_solve(fv_ctx *ctx, int64_t batch, int64_t n, double *A, double *rhs, int32_t *info)
 {
     if (batch <= 0 || n <= 0) return FV_OK;
-    const size_t vec = lu_align((size_t)n * sizeof(double)) + lu_align((size_t)n * sizeof(int));
+    const size_t vec = 2 * lu_align((size_t)n * sizeof(double)) + lu_align((size_t)n * sizeof(int));
     const LuLds p = lu_lds_plan<double>(n, vec, max_lds(ctx));
     FV_REQUIRE(ctx, p.total <= max_lds(ctx), "dense solve: n = %lld exceeds the LDS panel limit",
                (long long)n);
@@ -552,7 +574,7 @@ int dense_solve_mixed(fv_ctx *ctx, int64_t batch, int64_t n, const double *A, do
                       int max_iter, double tol, int32_t *info, int32_t *iters)
 {
     if (batch <= 0 || n <= 0) return FV_OK;
-    const size_t vec = 3 * lu_align((size_t)n * sizeof(double)) +
+    const size_t vec = 4 * lu_align((size_t)n * sizeof(double)) +
                        2 * lu_align((size_t)n * sizeof(int));
     const LuLds p = lu_lds_plan<float>(n, vec, max_lds(ctx));
     FV_REQUIRE(ctx, p.total <= max_lds(ctx), "dense solve: n = %lld exceeds the LDS panel limit",
