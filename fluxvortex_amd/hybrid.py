@@ -220,6 +220,25 @@ def static_wing_case(nc=7, ns=18, chord=2.0, half_span=5.0, V=10.0, alpha_deg=5.
                  nu=15.06e-6, name=f"static wing {nc}x{ns * (2 if symmetric else 1)}")
 
 
+def flat_plate_case(nc=4, ns=8, chord=1.0, half_span=4.0, V=10.0, alpha_deg=2.0,
+                    wake_chords=20.0, symmetric=False, spacing="uniform", dt=None,
+                    n_keep=None, precision="fp64") -> HybridCase:
+    """Static rectangular flat plate (symmetric section, zero camber) with a
+    prescribed ring wake grown over `wake_chords` chord lengths; dt defaults to
+    panel chord / V.  With symmetric=False the lattice spans y in [0, half_span]
+    (one semi-span, as PteraSoftware's `symmetric=False` wing)."""
+    dt = (chord / nc) / V if dt is None else dt
+    steps = int(math.ceil(wake_chords * chord / (V * dt)))
+    wing = Surface(root_chord=chord, tip_chord=chord, tip_offset=(0.0, half_span, 0.0), nc=nc,
+                   ns=ns, airfoil="0012", chordwise_spacing=spacing, spanwise_spacing=spacing,
+                   symmetric=symmetric)
+    ap = Airplane([wing])
+    return _case(ap, V, alpha_deg, dt, steps, rc0_bound=0.0, rc0_wake=0.0,
+                 n_keep=steps + 1 if n_keep is None else n_keep, sigma=V * dt,
+                 kernel="gaussianerf", sref=wing.planform_area(), precision=precision,
+                 nu=0.0, name=f"flat plate {nc}x{ns * (2 if symmetric else 1)}")
+
+
 def flapping_case(V=10.0, alpha_deg=1.0, dt=1.0 / 43, cycles=3.0, n_keep=10,
                   kernel="winckelmans", free_wake=False, stretch=False, sigma_factor=1.0,
                   precision="fp64", max_particles=400000, num_steps=None) -> HybridCase:

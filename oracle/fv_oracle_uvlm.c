@@ -26,9 +26,10 @@
  *
  * Pinning: the pieces are the kernel-level functions of fv_oracle.c (pinned
  * there); the step composition itself is checked by tests/test_oracle_uvlm.py
- * against closed forms (flat plate CL -> 2 pi alpha AR/(AR+2) trend, steady
- * lift from the impulsively started plate, Kelvin: shed + bound circulation)
- * -- the CL/CDi histories of the reference's own solver are parity unpinned
+ * against PteraSoftware's published static-wing CL/CDi (reference
+ * README.md:44), Helmbold's lift slope of a rectangular flat plate, linearity
+ * in alpha, and the Wagner-type rise of an impulsively started plate -- the
+ * CL/CDi histories of the reference's own solver are parity unpinned
  * (PteraSoftware is not available).
  */
 #include <math.h>
