@@ -131,6 +131,11 @@ _SIGNATURES = {
     "fv_vpm_closure_stats": (C.c_int, [_vp, _dp, _i64p, _i64p, _i32p, _dp]),
     "fv_vpm_closure_velocity_host": (C.c_int, [_vp, C.c_int32, _dp, _dp]),
     "fv_vpm_closure_sum_host": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32, _dp]),
+    "fv_sweep_create": (C.c_int, [_ctx, _vp, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
+    "fv_sweep_destroy": (None, [_vp]),
+    "fv_sweep_step": (C.c_int, [_vp, C.c_int64]),
+    "fv_sweep_read": (C.c_int, [_vp, C.c_int32, _dp, _dp, _dp, _dp, _dp, C.POINTER(C.c_int64),
+                                C.POINTER(C.c_int32)]),
     "fv_vpm_closure_train": (C.c_int, [_vp, C.c_int32]),
     "fv_mlp_create": (C.c_int, [_ctx, C.POINTER(_vp)]),
     "fv_mlp_destroy": (None, [_vp]),

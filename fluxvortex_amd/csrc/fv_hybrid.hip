@@ -9,7 +9,8 @@
 //   AIC        unit-strength rings . normals                       (P x P)
 //   rhs        -[(V - v_motion).n + (v_wake_rings + v_particles).n]
 //   solve      dense LU (FP64) or FP32 LU + FP64 refinement (mixed)
-//   loads      Kutta-Joukowski on right/front/left legs with effective
+//   loads      Kutta-Joukowski on right/front/left legs (and the trailing-edge
+//              back leg against the newest wake ring) with effective
 //              strengths + unsteady -rho A dGamma/dt n  ->  CL, CDi
 //   wake       shed the trailing-edge row, advect the ring wake with V dt
 //   convert    ring row n_keep -> 4 particles per ring, ring gamma := 0
@@ -64,6 +65,7 @@ struct fv_hybrid {
     int32_t *info = nullptr;                           // dense-solve status
     int32_t *d_nc = nullptr, *d_ns = nullptr;          // per-surface sizes (device)
     int64_t *d_poff = nullptr;                         // per-surface panel offsets (device)
+    int64_t *d_woff = nullptr;                         // per-surface wake ring offsets (device)
     fv_vpm *vpm = nullptr;
     double seg_pairs = 0.0;
 };
@@ -134,12 +136,13 @@ __device__ void panel_frame(const double *g, int ns, int i, int j, double *col, 
     nrm[2] = cz / len;
 }
 
-// Leg l (0 right Br->Fr, 1 front Fr->Fl, 2 left Fl->Bl): centre and vector.
+// Leg l (0 right Br->Fr, 1 front Fr->Fl, 2 left Fl->Bl, 3 back Bl->Br): centre
+// and vector.
 __device__ void leg_of(const double *Br, const double *Fr, const double *Fl, const double *Bl,
                        int l, double *c, double *v)
 {
-    const double *s = l == 0 ? Br : (l == 1 ? Fr : Fl);
-    const double *e = l == 0 ? Fr : (l == 1 ? Fl : Bl);
+    const double *s = l == 0 ? Br : (l == 1 ? Fr : (l == 2 ? Fl : Bl));
+    const double *e = l == 0 ? Fr : (l == 1 ? Fl : (l == 2 ? Bl : Br));
     for (int k = 0; k < 3; ++k) {
         c[k] = 0.5 * (s[k] + e[k]);
         v[k] = e[k] - s[k];
@@ -177,15 +180,15 @@ __global__ void k_geometry(const double *__restrict__ gcur, const double *__rest
         panel_frame(gprev, ns, i, j, pcol, pn, &pa);
     }
     for (int c = 0; c < 3; ++c) vm_col[3 * p + c] = gprev ? (col[c] - pcol[c]) * inv_dt : 0.0;
-    for (int l = 0; l < 3; ++l) {
+    for (int l = 0; l < 4; ++l) {
         double lc[3], lv[3];
         leg_of(Br, Fr, Fl, Bl, l, lc, lv);
         double plc[3], plv[3];
         if (gprev) leg_of(pBr, pFr, pFl, pBl, l, plc, plv);
         for (int c = 0; c < 3; ++c) {
-            leg_c[3 * (3 * p + l) + c] = lc[c];
-            leg_v[3 * (3 * p + l) + c] = lv[c];
-            vm_leg[3 * (3 * p + l) + c] = gprev ? (lc[c] - plc[c]) * inv_dt : 0.0;
+            leg_c[3 * (4 * p + l) + c] = lc[c];
+            leg_v[3 * (4 * p + l) + c] = lv[c];
+            vm_leg[3 * (4 * p + l) + c] = gprev ? (lc[c] - plc[c]) * inv_dt : 0.0;
         }
     }
 }
@@ -207,11 +210,17 @@ __global__ void k_rhs(int64_t P, const double *__restrict__ normal,
 }
 
 // One workgroup: per-panel forces in panel order, fixed-order reduction.
+// Kutta-Joukowski on each leg with the net circulation of its line: right /
+// left legs against the spanwise neighbour, the front leg against the ring
+// ahead (whole on the leading edge), the back leg only on the trailing edge,
+// against the newest wake ring (whole when there is none); interior back legs
+// are the next ring's front legs.
 constexpr int kLoadBlock = 256;
 
 __global__ __launch_bounds__(kLoadBlock) void k_loads(
     int nsurf, const int *__restrict__ surf_nc, const int *__restrict__ surf_ns,
-    const int64_t *__restrict__ surf_poff, int64_t P, const double *__restrict__ gam,
+    const int64_t *__restrict__ surf_poff, const int64_t *__restrict__ surf_woff,
+    const double *__restrict__ wake_g, int64_t nrow, int64_t P, const double *__restrict__ gam,
     const double *__restrict__ gam_prev, int first_step, const double *__restrict__ leg_v,
     const double *__restrict__ vm_leg, const double *__restrict__ vl,
     const double *__restrict__ normal, const double *__restrict__ area, double vx, double vy,
@@ -223,17 +232,19 @@ __global__ __launch_bounds__(kLoadBlock) void k_loads(
     for (int64_t p = threadIdx.x; p < P; p += kLoadBlock) {
         int k = 0;
         while (k + 1 < nsurf && p >= surf_poff[k + 1]) ++k;
-        const int ns = surf_ns[k];
+        const int ns = surf_ns[k], nc = surf_nc[k];
         const int64_t loc = p - surf_poff[k];
         const int i = (int)(loc / ns), j = (int)(loc % ns);
         const double G = gam[p];
-        double geff[3];
+        double geff[4];
         geff[0] = (j + 1 < ns) ? G - gam[p + 1] : G;    // right leg
         geff[1] = (i > 0) ? G - gam[p - ns] : G;         // front leg
         geff[2] = (j > 0) ? G - gam[p - 1] : G;          // left leg
+        geff[3] = G - (nrow > 0 ? wake_g[surf_woff[k] + j] : 0.0);   // trailing-edge back leg
+        const int nleg = i == nc - 1 ? 4 : 3;
         double F[3] = {0.0, 0.0, 0.0};
-        for (int l = 0; l < 3; ++l) {
-            const int64_t q = 3 * p + l;
+        for (int l = 0; l < nleg; ++l) {
+            const int64_t q = 4 * p + l;
             const double V[3] = {vx - vm_leg[3 * q] + vl[3 * q], vy - vm_leg[3 * q + 1] + vl[3 * q + 1],
                                  vz - vm_leg[3 * q + 2] + vl[3 * q + 2]};
             const double *L = leg_v + 3 * q;
@@ -436,11 +447,12 @@ static int hybrid_step_one(fv_hybrid *h)
     rc = build_segtab(ctx, h->br, h->fr, h->fl, h->bl, P, h->gam, h->brc0, nullptr, d.nu,
                       FV_LEGS_RING, false, h->tab);
     if (rc) return rc;
-    rc = seg_sum(ctx, h->leg_c, 3 * P, h->tab, 4 * P + 4 * W, prec, h->vl, S_PARTIAL);
+    rc = seg_sum(ctx, h->leg_c, 4 * P, h->tab, 4 * P + 4 * W, prec, h->vl, S_PARTIAL);
     if (rc) return rc;
-    h->seg_pairs += 3.0 * P * (4.0 * P + 4.0 * W);
+    h->seg_pairs += 4.0 * P * (4.0 * P + 4.0 * W);
     hipLaunchKernelGGL(k_loads, dim3(1), dim3(kLoadBlock), 0, ctx->stream, (int)h->surf.size(),
-                       h->d_nc, h->d_ns, h->d_poff, P, h->gam, h->gam_prev, s == 0 ? 1 : 0,
+                       h->d_nc, h->d_ns, h->d_poff, h->d_woff, h->wg[cur], h->nrow, P, h->gam,
+                       h->gam_prev, s == 0 ? 1 : 0,
                        h->leg_v, h->vm_leg, h->vl, h->normal, h->area, d.vinf[0], d.vinf[1],
                        d.vinf[2], d.rho, 1.0 / dt, d.sref, s, h->force, h->cl, h->cdi);
     FV_LAUNCH_CHECK(ctx);
@@ -567,15 +579,15 @@ int fv_hybrid_create(fv_ctx *ctx, const fv_hybrid_desc *desc, fv_hybrid **out)
     const size_t gsz = (size_t)d.num_steps * TV * 3;
     n += gsz;
     n += 4 * 3 * P + 3 * P + 3 * P + P + 3 * P;      // rings, colloc, normal, area, vm_col
-    n += 3 * 9 * P;                                  // leg_c, leg_v, vm_leg
+    n += 3 * 12 * P;                                 // leg_c, leg_v, vm_leg
     n += (size_t)P * P + 3 * P;                      // aic, rhs, gam, gam_prev
-    n += 3 * P + 3 * P + 9 * P + 3 * P;              // vw, vp, vl, force
+    n += 3 * P + 3 * P + 12 * P + 3 * P;             // vw, vp, vl, force
     n += 2 * (3 * WV + 2 * WR);                      // wake double buffers
     n += 4 * 3 * WR + 3 * WR + P;                    // ring corners, rg, rage, rrc0, brc0
     n += 4 * 3 * NS + NS;                            // conversion corners + cg
     n += 2 * (size_t)d.num_steps;                    // cl, cdi
     const size_t tab_bytes = sizeof(SegRec) * (size_t)(4 * P + 4 * WR + 4 * P);
-    const size_t meta_bytes = sizeof(int64_t) * S + sizeof(int32_t) * (1 + 2 * S) + 64;
+    const size_t meta_bytes = sizeof(int64_t) * 2 * S + sizeof(int32_t) * (1 + 2 * S) + 64;
     const size_t bytes = n * sizeof(double) + tab_bytes + meta_bytes + 1024;
     hipError_t e = hipMalloc(&h->block, bytes);
     if (e != hipSuccess) {
@@ -596,9 +608,9 @@ int fv_hybrid_create(fv_ctx *ctx, const fv_hybrid_desc *desc, fv_hybrid **out)
     h->grids = take(gsz);
     h->br = take(3 * P); h->fr = take(3 * P); h->fl = take(3 * P); h->bl = take(3 * P);
     h->colloc = take(3 * P); h->normal = take(3 * P); h->area = take(P); h->vm_col = take(3 * P);
-    h->leg_c = take(9 * P); h->leg_v = take(9 * P); h->vm_leg = take(9 * P);
+    h->leg_c = take(12 * P); h->leg_v = take(12 * P); h->vm_leg = take(12 * P);
     h->aic = take((size_t)P * P); h->rhs = take(P); h->gam = take(P); h->gam_prev = take(P);
-    h->vw = take(3 * P); h->vp = take(3 * P); h->vl = take(9 * P); h->force = take(3 * P);
+    h->vw = take(3 * P); h->vp = take(3 * P); h->vl = take(12 * P); h->force = take(3 * P);
     for (int b = 0; b < 2; ++b) {
         h->wv[b] = take(3 * WV);
         h->wg[b] = take(WR);
@@ -609,21 +621,24 @@ int fv_hybrid_create(fv_ctx *ctx, const fv_hybrid_desc *desc, fv_hybrid **out)
     h->cbr = take(3 * NS); h->cfr = take(3 * NS); h->cfl = take(3 * NS); h->cbl = take(3 * NS);
     h->cg = take(NS);
     h->cl = take(d.num_steps); h->cdi = take(d.num_steps);
-    // metadata: int64 panel offsets [S], then int32 status, nc [S], ns [S]
+    // metadata: int64 panel offsets [S], wake ring offsets [S], then int32
+    // status, nc [S], ns [S]
     int64_t *m64 = reinterpret_cast<int64_t *>(p);
-    int32_t *m32p = reinterpret_cast<int32_t *>(m64 + S);
+    int32_t *m32p = reinterpret_cast<int32_t *>(m64 + 2 * S);
     h->d_poff = m64;
+    h->d_woff = m64 + S;
     h->info = m32p;
     h->d_nc = m32p + 1;
     h->d_ns = m32p + 1 + S;
-    std::vector<int64_t> po(S);
+    std::vector<int64_t> po(2 * S);
     std::vector<int32_t> m32(1 + 2 * S, 0);
     for (size_t k = 0; k < S; ++k) {
         po[k] = h->surf[k].panel_off;
+        po[S + k] = h->surf[k].wring_off;
         m32[1 + k] = h->surf[k].nc;
         m32[1 + S + k] = h->surf[k].ns;
     }
-    e = hipMemcpy(m64, po.data(), sizeof(int64_t) * S, hipMemcpyHostToDevice);
+    e = hipMemcpy(m64, po.data(), sizeof(int64_t) * 2 * S, hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(m32p, m32.data(), sizeof(int32_t) * m32.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess)

@@ -1,0 +1,357 @@
+// fv_lu.h -- dense LU building blocks executed by one workgroup (device code
+// shared by the batched circulation solve and the batched flutter sweep).
+#pragma once
+
+#include "fv_launch.h"
+
+namespace fv {
+
+// ---------------------------------------------------------------- blocked
+// Right-looking blocked LU (panel width kNb) executed by one workgroup:
+//   1. the n-k0 x kb panel is factored by wave 0 with partial pivoting
+//      (wave-synchronous: no workgroup barrier inside the panel), swapping
+//      the panel's rows and eliminating the right-hand side as it goes; when
+//      the matrix lives in global memory the panel is first staged in LDS;
+//   2. the panel's row swaps are applied to every other column (in order,
+//      one column per thread), so L and U end in LAPACK's row order;
+//   3. all waves form U12 = L11^{-1} A12 (one column per lane);
+//   4. all waves update A22 -= L21 U12.
+// The right-hand side and the pivots live in LDS; LDS row strides are odd so
+// that the lanes of a column walk (one row per lane) hit distinct banks.
+constexpr int kNb = 16;
+constexpr int kLdp = kNb + 1;   // LDS row stride of a staged panel
+
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Wavefront reductions on DPP (row_shr 1/2/4/8 inside each 16-lane row, then
+// row_bcast 15 / 31 across rows); lane 63 ends with the result, which
+// v_readlane makes uniform.  Lanes whose DPP source is outside the row take
+// the identity.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ int dpp_i(int ident, int v)
+{
+    return __builtin_amdgcn_update_dpp(ident, v, CTRL, ROWMASK, 0xf, false);
+}
+
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_d(double ident, double v)
+{
+    const int lo = dpp_i<CTRL, ROWMASK>(__double2loint(ident), __double2loint(v));
+    const int hi = dpp_i<CTRL, ROWMASK>(__double2hiint(ident), __double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double readlane_d(double v, int lane)
+{
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+
+// Index of the largest |value| over the wavefront (ties: the smallest index);
+// lanes without a candidate pass best = -1.  LAPACK's idamax rule.
+__device__ __forceinline__ int wave_argmax(double best, int idx)
+{
+    double m = best;
+    m = fmax(m, dpp_d<0x111, 0xf>(-1.0, m));
+    m = fmax(m, dpp_d<0x112, 0xf>(-1.0, m));
+    m = fmax(m, dpp_d<0x114, 0xf>(-1.0, m));
+    m = fmax(m, dpp_d<0x118, 0xf>(-1.0, m));
+    m = fmax(m, dpp_d<0x142, 0xa>(-1.0, m));
+    m = fmax(m, dpp_d<0x143, 0xc>(-1.0, m));
+    const double M = readlane_d(m, 63);
+    const int big = 0x7fffffff;
+    int c = (best == M) ? idx : big;
+    c = min(c, dpp_i<0x111, 0xf>(big, c));
+    c = min(c, dpp_i<0x112, 0xf>(big, c));
+    c = min(c, dpp_i<0x114, 0xf>(big, c));
+    c = min(c, dpp_i<0x118, 0xf>(big, c));
+    c = min(c, dpp_i<0x142, 0xa>(big, c));
+    c = min(c, dpp_i<0x143, 0xc>(big, c));
+    return __builtin_amdgcn_readlane(c, 63);
+}
+
+typedef double fv_d4 __attribute__((ext_vector_type(4)));
+
+// A22 -= L21 U12 (rows / columns k0+kb .. n, inner dimension kb <= 16), FP64
+// on MFMA: each wavefront owns 16 x 16 tiles of A22 and accumulates them with
+// v_mfma_f64_16x16x4f64 (A operand lane l: row l % 16, k = l / 16; B operand:
+// column l % 16, k = l / 16; result element v of lane l: row 4 v + l / 16,
+// column l % 16).  Out-of-range rows / columns / k enter as zeros.
+__device__ __forceinline__ void trailing_update(double *a, int lda, int n, int k0, int kb)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int r0 = k0 + kb, mm = n - r0;
+    const int tiles = (mm + 15) >> 4;
+    const int lr = lane & 15, lk = lane >> 4;
+    for (int t = wave; t < tiles * tiles; t += nw) {
+        const int ti = t / tiles, tj = t - ti * tiles;
+        const int R0 = r0 + 16 * ti, col = r0 + 16 * tj + lr;
+        fv_d4 acc;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int row = R0 + 4 * v + lk;
+            acc[v] = (row < n && col < n) ? a[(int64_t)row * lda + col] : 0.0;
+        }
+        const int ra = R0 + lr;
+#pragma unroll
+        for (int ks = 0; ks < kNb; ks += 4) {
+            const int k = ks + lk;
+            const double av = (ra < n && k < kb) ? a[(int64_t)ra * lda + k0 + k] : 0.0;
+            const double bv = (col < n && k < kb) ? a[(int64_t)(k0 + k) * lda + col] : 0.0;
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(-av, bv, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int row = R0 + 4 * v + lk;
+            if (row < n && col < n) a[(int64_t)row * lda + col] = acc[v];
+        }
+    }
+}
+
+typedef float fv_f4x __attribute__((ext_vector_type(4)));
+
+// FP32 factors (mixed precision) on v_mfma_f32_16x16x4f32: operands as the
+// FP64 form; result element v of lane l: row 4 (l / 16) + v, column l % 16.
+__device__ __forceinline__ void trailing_update(float *a, int lda, int n, int k0, int kb)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int r0 = k0 + kb, mm = n - r0;
+    const int tiles = (mm + 15) >> 4;
+    const int lr = lane & 15, lk = lane >> 4;
+    for (int t = wave; t < tiles * tiles; t += nw) {
+        const int ti = t / tiles, tj = t - ti * tiles;
+        const int R0 = r0 + 16 * ti, col = r0 + 16 * tj + lr;
+        fv_f4x acc;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int row = R0 + 4 * lk + v;
+            acc[v] = (row < n && col < n) ? a[(int64_t)row * lda + col] : 0.0f;
+        }
+        const int ra = R0 + lr;
+#pragma unroll
+        for (int ks = 0; ks < kNb; ks += 4) {
+            const int k = ks + lk;
+            const float av = (ra < n && k < kb) ? a[(int64_t)ra * lda + k0 + k] : 0.0f;
+            const float bv = (col < n && k < kb) ? a[(int64_t)(k0 + k) * lda + col] : 0.0f;
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(-av, bv, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int row = R0 + 4 * lk + v;
+            if (row < n && col < n) a[(int64_t)row * lda + col] = acc[v];
+        }
+    }
+}
+
+// a: n x n, row stride lda (LDS or global); pan: LDS panel buffer (n x kLdp)
+// or null when a is in LDS; perm (LDS, n): perm[j] = row swapped with j;
+// b (LDS, n, nullable): right-hand side, eliminated along.  Returns 0 or the
+// 1-based index of the first zero pivot.
+template <typename T, typename TB>
+__device__ __forceinline__ int lu_blocked(T *a, int lda, int n, int *perm, TB *b, T *pan)
+{
+    __shared__ int info_s;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    if (threadIdx.x == 0) info_s = 0;
+    __syncthreads();
+    for (int k0 = 0; k0 < n; k0 += kNb) {
+        const int kb = (n - k0 < kNb) ? n - k0 : kNb;
+        const int m = n - k0;
+        T *P = a + (int64_t)k0 * lda + k0;
+        int ldP = lda;
+        if (pan) {
+            for (int e = threadIdx.x; e < m * kb; e += blockDim.x) {
+                const int r = e / kb, c = e - r * kb;
+                pan[r * kLdp + c] = P[(int64_t)r * lda + c];
+            }
+            P = pan;
+            ldP = kLdp;
+            __syncthreads();
+        }
+        if (wave == 0) {
+            for (int jj = 0; jj < kb; ++jj) {
+                double best = -1.0;
+                int bi = m;
+                for (int r = jj + lane; r < m; r += 64) {
+                    const double v = fabs((double)P[(int64_t)r * ldP + jj]);
+                    if (v > best) {
+                        best = v;
+                        bi = r;
+                    }
+                }
+                const int piv = wave_argmax(best, bi);
+                if (P[(int64_t)piv * ldP + jj] == (T)0) {
+                    if (lane == 0) {
+                        if (info_s == 0) info_s = k0 + jj + 1;
+                        perm[k0 + jj] = k0 + jj;
+                    }
+                    wave_sync();
+                    continue;
+                }
+                if (lane == 0) perm[k0 + jj] = k0 + piv;
+                if (piv != jj) {
+                    for (int c = lane; c < kb; c += 64) {
+                        const T t = P[(int64_t)jj * ldP + c];
+                        P[(int64_t)jj * ldP + c] = P[(int64_t)piv * ldP + c];
+                        P[(int64_t)piv * ldP + c] = t;
+                    }
+                    if (b && lane == 0) {
+                        const TB t = b[k0 + jj];
+                        b[k0 + jj] = b[k0 + piv];
+                        b[k0 + piv] = t;
+                    }
+                }
+                wave_sync();
+                const T inv = (T)1 / P[(int64_t)jj * ldP + jj];
+                const TB bj = b ? b[k0 + jj] : (TB)0;
+                // pivot row in registers; each lane loads its row slice before
+                // storing, so no LDS store waits on the next load
+                T prow[kNb];
+#pragma unroll
+                for (int c = 0; c < kNb; ++c)
+                    prow[c] = (c > jj && c < kb) ? P[(int64_t)jj * ldP + c] : (T)0;
+                for (int r = jj + 1 + lane; r < m; r += 64) {
+                    T *row = P + (int64_t)r * ldP;
+                    T v[kNb];
+#pragma unroll
+                    for (int c = 0; c < kNb; ++c) v[c] = (c > jj && c < kb) ? row[c] : (T)0;
+                    const T l = row[jj] * inv;
+                    row[jj] = l;
+#pragma unroll
+                    for (int c = 0; c < kNb; ++c)
+                        if (c > jj && c < kb) row[c] = v[c] - l * prow[c];
+                    if (b) b[k0 + r] -= (TB)l * bj;
+                }
+                wave_sync();
+            }
+        }
+        __syncthreads();
+        if (pan) {
+            T *G = a + (int64_t)k0 * lda + k0;
+            for (int e = threadIdx.x; e < m * kb; e += blockDim.x) {
+                const int r = e / kb, c = e - r * kb;
+                G[(int64_t)r * lda + c] = pan[r * kLdp + c];
+            }
+        }
+        // the panel's row swaps on every column outside it, in pivot order
+        for (int c = threadIdx.x; c < n - kb; c += blockDim.x) {
+            const int cc = c < k0 ? c : c + kb;
+            for (int jj = 0; jj < kb; ++jj) {
+                const int p = perm[k0 + jj];
+                if (p != k0 + jj) {
+                    T *x = a + (int64_t)(k0 + jj) * lda + cc, *y = a + (int64_t)p * lda + cc;
+                    const T t = *x;
+                    *x = *y;
+                    *y = t;
+                }
+            }
+        }
+        __syncthreads();
+        // U12 = L11^{-1} A12: forward substitution, one column per lane in registers
+        for (int c = k0 + kb + threadIdx.x; c < n; c += blockDim.x) {
+            T col[kNb];
+#pragma unroll
+            for (int i = 0; i < kNb; ++i) col[i] = i < kb ? a[(int64_t)(k0 + i) * lda + c] : (T)0;
+#pragma unroll
+            for (int i = 1; i < kNb; ++i) {
+                if (i < kb) {
+                    T s = col[i];
+#pragma unroll
+                    for (int q = 0; q < i; ++q) s -= a[(int64_t)(k0 + i) * lda + k0 + q] * col[q];
+                    col[i] = s;
+                }
+            }
+#pragma unroll
+            for (int i = 1; i < kNb; ++i)
+                if (i < kb) a[(int64_t)(k0 + i) * lda + c] = col[i];
+        }
+        __syncthreads();
+        // A22 -= L21 U12
+        if (n - k0 - kb > 0) trailing_update(a, lda, n, k0, kb);
+        __syncthreads();
+    }
+    return info_s;
+}
+
+// x = U^{-1} x in place by wave 0, column-oriented: x[k] is final once the
+// columns right of k have been eliminated; it is broadcast with v_readlane
+// and column k is eliminated from the rows above.  x lives in registers
+// (lane l holds x[l + 64 q]) for n <= 256, else in LDS.
+template <typename T, typename TB>
+__device__ __forceinline__ void wave_back_subst(const T *a, int lda, int n, TB *x,
+                                                const TB *rdiag)
+{
+    if ((threadIdx.x >> 6) != 0) return;
+    const int lane = threadIdx.x & 63;
+    if (n <= 256) {
+        TB xr[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) xr[q] = (lane + 64 * q < n) ? x[lane + 64 * q] : (TB)0;
+        for (int k = n - 1; k >= 0; --k) {
+            const int ks = k >> 6, kl = k & 63;
+            const TB own = ks == 0 ? xr[0] : ks == 1 ? xr[1] : ks == 2 ? xr[2] : xr[3];
+            const TB xk = readlane_d((double)own, kl) * rdiag[k];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int i = lane + 64 * q;
+                if (i < k) xr[q] -= (TB)a[(int64_t)i * lda + k] * xk;
+                if (i == k) xr[q] = xk;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (lane + 64 * q < n) x[lane + 64 * q] = xr[q];
+        wave_sync();
+        return;
+    }
+    for (int k = n - 1; k >= 0; --k) {
+        const TB xk = x[k] * rdiag[k];
+        for (int i = lane; i < k; i += 64) x[i] -= (TB)a[(int64_t)i * lda + k] * xk;
+        wave_sync();
+        if (lane == 0) x[k] = xk;
+        wave_sync();
+    }
+}
+
+// y = L^{-1} x in place (unit lower triangle) by wave 0, column-oriented as
+// above; the row permutation is applied by the caller.
+template <typename T, typename TB>
+__device__ __forceinline__ void wave_fwd_subst(const T *a, int lda, int n, TB *x)
+{
+    if ((threadIdx.x >> 6) != 0) return;
+    const int lane = threadIdx.x & 63;
+    if (n <= 256) {
+        TB xr[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) xr[q] = (lane + 64 * q < n) ? x[lane + 64 * q] : (TB)0;
+        for (int k = 0; k < n - 1; ++k) {
+            const int ks = k >> 6, kl = k & 63;
+            const TB own = ks == 0 ? xr[0] : ks == 1 ? xr[1] : ks == 2 ? xr[2] : xr[3];
+            const TB xk = readlane_d((double)own, kl);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int i = lane + 64 * q;
+                if (i > k && i < n) xr[q] -= (TB)a[(int64_t)i * lda + k] * xk;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (lane + 64 * q < n) x[lane + 64 * q] = xr[q];
+        wave_sync();
+        return;
+    }
+    for (int k = 0; k < n - 1; ++k) {
+        const TB xk = x[k];
+        for (int i = k + 1 + lane; i < n; i += 64) x[i] -= (TB)a[(int64_t)i * lda + k] * xk;
+        wave_sync();
+    }
+}
+
+}  // namespace fv

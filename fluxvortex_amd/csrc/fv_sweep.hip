@@ -1,0 +1,939 @@
+// fv_sweep.hip -- batched aeroelastic hybrid UVLM instances (a flutter-speed
+// sweep) on one GPU.
+//
+// Reference: tests/benchmark_goland.py:100-243 loops over airspeeds, each a
+// PteraSoftware UVLM run coupled to BeamFE by AeroelasticSolver
+// (src/fluxvortex/aeroelastic_solver.py:63-221, beam_fe.py:231-351).  Here
+// every airspeed is an instance of the hybrid N_keep step and all instances
+// advance together, two launches per step:
+//   k_sw_induce  particle -> collocation-point velocities, one workgroup per
+//                (instance, 1024-particle chunk), partial sums per chunk;
+//   k_sw_step    one workgroup per instance for the rest of the step, every
+//                array in LDS: lattice deformation, rings / frames / legs,
+//                AIC, wake-ring velocities, right-hand side, LU solve, leg
+//                velocities and Kutta-Joukowski loads, CL / CDi, station loads
+//                -> beam nodes, one Newmark step, the next lattice's
+//                deformation, wake shedding, N_keep conversion, particle
+//                convection and the next step's collocation points.
+// The two launches are captured once in a hipGraph and replayed per step
+// (every per-step quantity lives in device memory).
+//
+// The UVLM statement is the device driver's (fv_hybrid.hip) and the oracle's
+// (oracle/fv_oracle_uvlm.c); the coupling follows AeroelasticSolver: station
+// lift = lift_sign * F_z summed over the strip (the reference's -F_z,
+// aeroelastic_solver.py:135-139), station moment = sum of x_colloc * lift, the
+// panel force sharing each line between two rows half and half.  With
+// `feedback` the deformed lattice carries the next step's aerodynamics (every
+// vertex moved with the heave / twist at its own |y|); without it the
+// aerodynamics stay on the undeformed lattice, as in the reference, whose
+// vertex mutation does not reach its solver (its stored Goland histories
+// carry loads independent of the deformation).  Particles are a prescribed
+// wake moved by V dt per step.
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "fv_lu.h"
+
+namespace fv {
+
+constexpr int kSwMaxNc = 16, kSwMaxNs = 32, kSwMaxP = 64, kSwMaxR = 16, kSwMaxNodes = 17;
+constexpr int kSwMaxFree = 3 * (kSwMaxNodes - 1);
+constexpr int kSwMaxL = 3 * kSwMaxP + kSwMaxNs;   // right / front / left legs + trailing-edge back legs
+constexpr int kSwBlock = 256;
+constexpr int kSwChunk = 1024;   // particles per induction partial
+constexpr int kSwParts = 4;      // segment partial sums per target
+
+struct SwInst {
+    int nc, ns, P, nv, R, n_keep, kernel, couple, n_nodes, nfree;
+    int status;                  // 0; k + 1: zero pivot at step k; -2: particle overflow
+    int has_prev, par, nrow, has_wprev, prec, feedback;
+    int64_t num_steps, step, cap, np, max_chunks;
+    double V[3], rho, dt, nu, rc0b, rc0w, sigma, sref, x_ea, relax, beta, gamma, lift_sign;
+    // offsets (doubles) into the state block
+    int64_t o_grid0, o_coln, o_colp, o_legp, o_gamp, o_gam, o_wv[2], o_wg[2], o_wa[2];
+    int64_t o_pos, o_al, o_sg, o_part, o_M, o_C, o_K, o_d, o_v, o_a;
+    int64_t o_wdef, o_tdef, o_wprev, o_tprev, o_yn, o_cl, o_cdi, o_tw, o_tt;
+};
+
+// ------------------------------------------------------------ geometry helpers
+__device__ __forceinline__ void sw_lerp(const double *a, const double *b, double t, double *o)
+{
+    o[0] = a[0] + t * (b[0] - a[0]);
+    o[1] = a[1] + t * (b[1] - a[1]);
+    o[2] = a[2] + t * (b[2] - a[2]);
+}
+
+__device__ __forceinline__ const double *sw_v(const double *g, int ns, int i, int j)
+{
+    return g + 3 * (i * (ns + 1) + j);
+}
+
+// Ring of panel (i, j): front legs at the quarter chord, back legs at the next
+// panel's quarter chord (last row: one panel length behind the front leg).
+__device__ void sw_ring(const double *g, int nc, int ns, int i, int j, double *Br, double *Fr,
+                        double *Fl, double *Bl)
+{
+    const double *v00 = sw_v(g, ns, i, j), *v01 = sw_v(g, ns, i, j + 1);
+    const double *v10 = sw_v(g, ns, i + 1, j), *v11 = sw_v(g, ns, i + 1, j + 1);
+    sw_lerp(v00, v10, 0.25, Fl);
+    sw_lerp(v01, v11, 0.25, Fr);
+    if (i < nc - 1) {
+        sw_lerp(v10, sw_v(g, ns, i + 2, j), 0.25, Bl);
+        sw_lerp(v11, sw_v(g, ns, i + 2, j + 1), 0.25, Br);
+    } else {
+        for (int c = 0; c < 3; ++c) {
+            Bl[c] = Fl[c] + (v10[c] - v00[c]);
+            Br[c] = Fr[c] + (v11[c] - v01[c]);
+        }
+    }
+}
+
+// Collocation point (3/4 chord, mid span), unit normal (Fr - Bl) x (Fl - Br), area.
+__device__ void sw_frame(const double *g, int ns, int i, int j, double *col, double *nrm,
+                         double *area)
+{
+    const double *v00 = sw_v(g, ns, i, j), *v01 = sw_v(g, ns, i, j + 1);
+    const double *v10 = sw_v(g, ns, i + 1, j), *v11 = sw_v(g, ns, i + 1, j + 1);
+    double a[3], b[3];
+    sw_lerp(v00, v10, 0.75, a);
+    sw_lerp(v01, v11, 0.75, b);
+    for (int c = 0; c < 3; ++c) col[c] = 0.5 * (a[c] + b[c]);
+    const double d1[3] = {v01[0] - v10[0], v01[1] - v10[1], v01[2] - v10[2]};
+    const double d2[3] = {v00[0] - v11[0], v00[1] - v11[1], v00[2] - v11[2]};
+    const double cx = d1[1] * d2[2] - d1[2] * d2[1];
+    const double cy = d1[2] * d2[0] - d1[0] * d2[2];
+    const double cz = d1[0] * d2[1] - d1[1] * d2[0];
+    const double len = sqrt(cx * cx + cy * cy + cz * cz);
+    *area = 0.5 * len;
+    nrm[0] = cx / len;
+    nrm[1] = cy / len;
+    nrm[2] = cz / len;
+}
+
+// numpy.interp: linear inside [xp0, xp_last], the end values outside.
+__device__ double sw_interp(double x, const double *xp, const double *fp, int n)
+{
+    if (x <= xp[0]) return fp[0];
+    if (x >= xp[n - 1]) return fp[n - 1];
+    int j = 0;
+    while (j + 1 < n - 1 && xp[j + 1] <= x) ++j;
+    if (x == xp[j]) return fp[j];
+    const double slope = (fp[j + 1] - fp[j]) / (xp[j + 1] - xp[j]);
+    return slope * (x - xp[j]) + fp[j];
+}
+
+// Vertex deformed by heave w and twist theta at its |y| (rotation about x_ea):
+// z += w + (x - x_ea) sin(theta), x += (x - x_ea)(cos(theta) - 1)
+// (aeroelastic_solver.py:176-198).
+__device__ void sw_deform(const double *b, const double *yn, const double *wn, const double *tn,
+                          int nn, double x_ea, double *o)
+{
+    const double y = fabs(b[1]);
+    const double w = sw_interp(y, yn, wn, nn), th = sw_interp(y, yn, tn, nn);
+    const double xr = b[0] - x_ea;
+    o[0] = b[0] + xr * (cos(th) - 1.0);
+    o[1] = b[1];
+    o[2] = b[2] + (w + xr * sin(th));
+}
+
+// Segment record of leg k (Br->Fr, Fr->Fl, Fl->Bl, Bl->Br) of a ring with
+// strength G, core rc0 and age (warp_kernels.py:46-96 constants).
+__device__ SegRec sw_seg(const double *s, const double *t, double G, double rc0, double age,
+                         double nu)
+{
+    SegRec r;
+    r.sx = s[0]; r.sy = s[1]; r.sz = s[2];
+    r.ex = t[0]; r.ey = t[1]; r.ez = t[2];
+    const double r0x = t[0] - s[0], r0y = t[1] - s[1], r0z = t[2] - s[2];
+    const double r0sq = r0x * r0x + r0y * r0y + r0z * r0z;
+    const double r0len = sqrt(r0sq);
+    const double rcsq = rc0 * rc0 + kFourLamb * (nu + kSquire * fabs(G)) * age;
+    r.c1 = G * kInv4Pi;
+    r.c2 = r0sq * rcsq;
+    r.tol = (r0len < 2.22e-16) ? INFINITY : r0len * 1.0e-10;
+    r.pad = 0.0;
+    return r;
+}
+
+// Segment pair in the instance's precision (mixed: FP32 pair term, FP64 sums).
+__device__ __forceinline__ void sw_pair(int prec, const double *x, const SegRec &sr, double *u)
+{
+    if (prec == FV_PREC_MIXED)
+        seg_pair_mixed(x[0], x[1], x[2], sr, u[0], u[1], u[2]);
+    else
+        seg_pair_f64(x[0], x[1], x[2], sr, u[0], u[1], u[2]);
+}
+
+// ----------------------------------------------------------- particle induction
+template <int KERNEL, int PREC>
+__device__ void sw_induce_body(const SwInst &I, const double *__restrict__ S, double *part,
+                               int64_t chunk, PartRec *tile, double *tgt, double *red)
+{
+    const int t = threadIdx.x, P = I.P;
+    const int64_t c0 = chunk * kSwChunk;
+    const int64_t c1 = (c0 + kSwChunk < I.np) ? c0 + kSwChunk : I.np;
+    for (int e = t; e < 3 * P; e += kSwBlock) tgt[e] = S[I.o_coln + e];
+    const int tp = t & 63, grp = t >> 6;
+    using Op = PartOp<KERNEL, PREC, 0>;
+    typename Op::Tgt tg;
+    typename Op::Acc acc;
+    for (int k = 0; k < 3; ++k) acc.v[k] = 0.0;
+    __syncthreads();
+    if (tp < P) {
+        tg.x = tgt[3 * tp]; tg.y = tgt[3 * tp + 1]; tg.z = tgt[3 * tp + 2];
+        tg.ax = tg.ay = tg.az = 0.0;
+    }
+    const double *pos = S + I.o_pos, *al = S + I.o_al, *sg = S + I.o_sg;
+    for (int64_t b0 = c0; b0 < c1; b0 += kSwBlock) {
+        const int cnt = (int)((c1 - b0) < kSwBlock ? (c1 - b0) : kSwBlock);
+        __syncthreads();
+        if (t < cnt) {
+            const int64_t j = b0 + t;
+            tile[t] = make_part_rec(KERNEL, pos[3 * j], pos[3 * j + 1], pos[3 * j + 2],
+                                    al[3 * j], al[3 * j + 1], al[3 * j + 2], sg[j]);
+        }
+        __syncthreads();
+        if (tp < P)
+            for (int k = grp; k < cnt; k += kSwBlock / 64) Op::pair(tg, tile[k], acc);
+    }
+    if (tp < P)
+        for (int c = 0; c < 3; ++c) red[(grp * kSwMaxP + tp) * 3 + c] = acc.v[c];
+    __syncthreads();
+    if (t < P)
+        for (int c = 0; c < 3; ++c) {
+            double s = 0.0;
+            for (int g = 0; g < kSwBlock / 64; ++g) s += red[(g * kSwMaxP + t) * 3 + c];
+            part[(chunk * P + t) * 3 + c] = s;
+        }
+}
+
+__global__ __launch_bounds__(kSwBlock) void k_sw_induce(const SwInst *__restrict__ insts,
+                                                        double *__restrict__ S)
+{
+    __shared__ PartRec tile[kSwBlock];
+    __shared__ double tgt[3 * kSwMaxP];
+    __shared__ double red[(kSwBlock / 64) * kSwMaxP * 3];
+    const SwInst &I = insts[blockIdx.y];
+    if (I.status != 0 || I.step >= I.num_steps) return;
+    const int64_t chunk = blockIdx.x;
+    if (chunk * kSwChunk >= I.np) return;
+    double *part = S + I.o_part;
+    const bool mixed = I.prec == FV_PREC_MIXED;
+    if (I.kernel == FV_KERNEL_WINCKELMANS) {
+        if (mixed) sw_induce_body<1, 1>(I, S, part, chunk, tile, tgt, red);
+        else sw_induce_body<1, 0>(I, S, part, chunk, tile, tgt, red);
+    } else {
+        if (mixed) sw_induce_body<0, 1>(I, S, part, chunk, tile, tgt, red);
+        else sw_induce_body<0, 0>(I, S, part, chunk, tile, tgt, red);
+    }
+}
+
+// ------------------------------------------------------------------ the step
+// LDS plan (doubles) of k_sw_step at the limits above.
+constexpr int kSwNv = (kSwMaxNc + 1) * (kSwMaxNs + 1);
+constexpr int kSwLdaMax = kSwMaxP | 1;
+constexpr int kSwWv = (kSwMaxR + 1) * (kSwMaxNs + 1);
+constexpr int kL_g = 0;                                   // grid 3 nv
+constexpr int kL_ring = kL_g + 3 * kSwNv;                 // Br Fr Fl Bl 12 P
+constexpr int kL_col = kL_ring + 12 * kSwMaxP;            // collocation 3 P
+constexpr int kL_nrm = kL_col + 3 * kSwMaxP;              // normal 3 P
+constexpr int kL_area = kL_nrm + 3 * kSwMaxP;             // area P
+constexpr int kL_vmc = kL_area + kSwMaxP;                 // motion velocity 3 P
+constexpr int kL_lc = kL_vmc + 3 * kSwMaxP;               // leg centres 3 NL
+constexpr int kL_lv = kL_lc + 3 * kSwMaxL;                // leg vectors 3 NL
+constexpr int kL_vml = kL_lv + 3 * kSwMaxL;               // leg motion velocity 3 NL
+constexpr int kL_aic = kL_vml + 3 * kSwMaxL;              // AIC P x lda
+constexpr int kL_rhs = kL_aic + kSwMaxP * kSwLdaMax;      // rhs / gamma P
+constexpr int kL_rdg = kL_rhs + kSwMaxP;                  // 1 / diag P
+constexpr int kL_perm = kL_rdg + kSwMaxP;                 // pivots (int) P
+constexpr int kL_red = kL_perm + kSwMaxP;                 // partial sums NL x parts x 3
+constexpr int kL_vw = kL_red + 3 * kSwMaxL * kSwParts;    // wake / particle velocity 3 P
+constexpr int kL_vl = kL_vw + 3 * kSwMaxP;                // leg velocities 3 NL
+constexpr int kL_F = kL_vl + 3 * kSwMaxL;                 // panel forces 3 P
+constexpr int kL_PF = kL_F + 3 * kSwMaxP;                 // panel shares, front-leg forces 6 P
+constexpr int kL_wv = kL_PF + 6 * kSwMaxP;                // wake vertices 3 wv
+constexpr int kL_wg = kL_wv + 3 * kSwWv;                  // wake strengths R ns
+constexpr int kL_wa = kL_wg + kSwMaxR * kSwMaxNs;         // wake ages R ns
+constexpr int kL_beam = kL_wa + kSwMaxR * kSwMaxNs;       // nodal forces / vectors
+constexpr int kL_end = kL_beam + 3 * kSwMaxNodes + 4 * kSwMaxFree + 4 * kSwMaxNodes + 8;
+constexpr size_t kSwLdsBytes = sizeof(double) * kL_end;
+
+__global__ __launch_bounds__(kSwBlock) void k_sw_step(SwInst *__restrict__ insts,
+                                                      double *__restrict__ S)
+{
+    extern __shared__ __attribute__((aligned(16))) double L[];
+    __shared__ SwInst I;
+    __shared__ int fail_s;
+    const int t = threadIdx.x;
+    if (t == 0) {
+        I = insts[blockIdx.x];
+        fail_s = 0;
+    }
+    __syncthreads();
+    if (I.status != 0 || I.step >= I.num_steps) return;
+    const int nc = I.nc, ns = I.ns, P = I.P, nv = I.nv, nn = I.n_nodes, nf = I.nfree;
+    const int64_t s = I.step;
+    const double dt = I.dt, *V = I.V;
+    double *g = L + kL_g, *ring = L + kL_ring, *col = L + kL_col, *nrm = L + kL_nrm;
+    double *area = L + kL_area, *vmc = L + kL_vmc, *lc = L + kL_lc, *lv = L + kL_lv;
+    double *vml = L + kL_vml, *A = L + kL_aic, *rhs = L + kL_rhs, *rdg = L + kL_rdg;
+    int *perm = reinterpret_cast<int *>(L + kL_perm);
+    double *red = L + kL_red, *vw = L + kL_vw, *vl = L + kL_vl, *F = L + kL_F;
+    double *PF = L + kL_PF, *Ff = PF + 3 * kSwMaxP;
+    double *wv = L + kL_wv, *wg = L + kL_wg, *wa = L + kL_wa, *bm = L + kL_beam;
+    const double *grid0 = S + I.o_grid0, *yn = S + I.o_yn;
+    double *wdef = S + I.o_wdef, *tdef = S + I.o_tdef;
+    const int lda = P | 1;
+    const int NL = 3 * P + ns;   // leg slots: 3 p + l, then 3 P + j for the back leg of (nc-1, j)
+    // ---- lattice of this step
+    for (int v = t; v < nv; v += kSwBlock) sw_deform(grid0 + 3 * v, yn, wdef, tdef, nn, I.x_ea, g + 3 * v);
+    __syncthreads();
+    for (int p = t; p < P; p += kSwBlock) {
+        const int i = p / ns, j = p % ns;
+        double *Br = ring + 12 * p, *Fr = Br + 3, *Fl = Br + 6, *Bl = Br + 9;
+        sw_ring(g, nc, ns, i, j, Br, Fr, Fl, Bl);
+        sw_frame(g, ns, i, j, col + 3 * p, nrm + 3 * p, area + p);
+        const double *cp = S + I.o_colp + 3 * p;
+        for (int c = 0; c < 3; ++c) vmc[3 * p + c] = I.has_prev ? (col[3 * p + c] - cp[c]) / dt : 0.0;
+        for (int l = 0; l < 4; ++l) {
+            // right, front, left legs; the back leg only on the trailing edge
+            if (l == 3 && i != nc - 1) continue;
+            const int q = l < 3 ? 3 * p + l : 3 * P + j;
+            const double *a = Br + 3 * l, *b = Br + 3 * ((l + 1) & 3);
+            const double *lp = S + I.o_legp + 3 * q;
+            for (int c = 0; c < 3; ++c) {
+                const double cc = 0.5 * (a[c] + b[c]);
+                lc[3 * q + c] = cc;
+                lv[3 * q + c] = b[c] - a[c];
+                vml[3 * q + c] = I.has_prev ? (cc - lp[c]) / dt : 0.0;
+            }
+        }
+    }
+    // wake of the current step (rows 0 .. nrow-1, vertices 0 .. nrow)
+    const int nrow = I.nrow, cur = I.par;
+    for (int e = t; e < 3 * (nrow + 1) * (ns + 1); e += kSwBlock) wv[e] = S[I.o_wv[cur] + e];
+    for (int e = t; e < nrow * ns; e += kSwBlock) {
+        wg[e] = S[I.o_wg[cur] + e];
+        wa[e] = S[I.o_wa[cur] + e];
+    }
+    __syncthreads();
+    // ---- AIC: unit-strength rings (bound core, age 0) . normals
+    for (int e = t; e < P * P; e += kSwBlock) {
+        const int i = e / P, j = e % P;
+        const double *R = ring + 12 * j;
+        double u[3] = {0.0, 0.0, 0.0};
+        for (int k = 0; k < 4; ++k) {
+            const SegRec sr = sw_seg(R + 3 * k, R + 3 * ((k + 1) & 3), 1.0, I.rc0b, 0.0, I.nu);
+            sw_pair(I.prec, col + 3 * i, sr, u);
+        }
+        A[i * lda + j] = u[0] * nrm[3 * i] + u[1] * nrm[3 * i + 1] + u[2] * nrm[3 * i + 2];
+    }
+    // ---- wake-ring velocity at the collocation points (kSwParts partial sums)
+    const int W = nrow * ns;
+    for (int e = t; e < P * kSwParts; e += kSwBlock) {
+        const int p = e / kSwParts, q = e % kSwParts;
+        double u[3] = {0.0, 0.0, 0.0};
+        for (int k = q; k < 4 * W; k += kSwParts) {
+            const int rg = k >> 2, leg = k & 3, r = rg / ns, j = rg % ns;
+            const double *c4[4] = {wv + 3 * ((r + 1) * (ns + 1) + j + 1),   // Br
+                                   wv + 3 * (r * (ns + 1) + j + 1),         // Fr
+                                   wv + 3 * (r * (ns + 1) + j),             // Fl
+                                   wv + 3 * ((r + 1) * (ns + 1) + j)};      // Bl
+            const SegRec sr = sw_seg(c4[leg], c4[(leg + 1) & 3], wg[rg], I.rc0w, wa[rg], I.nu);
+            sw_pair(I.prec, col + 3 * p, sr, u);
+        }
+        for (int c = 0; c < 3; ++c) red[(p * kSwParts + q) * 3 + c] = u[c];
+    }
+    __syncthreads();
+    const int nchunk = (int)((I.np + kSwChunk - 1) / kSwChunk);
+    for (int p = t; p < P; p += kSwBlock) {
+        double w3[3], q3[3] = {0.0, 0.0, 0.0};
+        for (int c = 0; c < 3; ++c) {
+            double a = 0.0;
+            for (int q = 0; q < kSwParts; ++q) a += red[(p * kSwParts + q) * 3 + c];
+            w3[c] = a;
+        }
+        for (int ch = 0; ch < nchunk; ++ch)
+            for (int c = 0; c < 3; ++c) q3[c] += S[I.o_part + (ch * P + p) * 3 + c];
+        const double *n = nrm + 3 * p;
+        const double fs = (V[0] - vmc[3 * p]) * n[0] + (V[1] - vmc[3 * p + 1]) * n[1] +
+                          (V[2] - vmc[3 * p + 2]) * n[2];
+        const double wn = w3[0] * n[0] + w3[1] * n[1] + w3[2] * n[2];
+        const double qn = I.np > 0 ? q3[0] * n[0] + q3[1] * n[1] + q3[2] * n[2] : 0.0;
+        rhs[p] = -((wn + qn) + fs);
+    }
+    __syncthreads();
+    // ---- circulation solve (LU with partial pivoting in LDS)
+    const int inf = lu_blocked<double, double>(A, lda, P, perm, rhs, (double *)nullptr);
+    if (inf != 0) {
+        if (t == 0) {
+            insts[blockIdx.x].status = (int)s + 1;
+        }
+        return;
+    }
+    for (int i = t; i < P; i += kSwBlock) rdg[i] = 1.0 / A[i * lda + i];
+    __syncthreads();
+    wave_back_subst<double, double>(A, lda, P, rhs, rdg);
+    __syncthreads();
+    const double *gam = rhs;
+    // ---- velocities at the leg centres from bound (gamma) + wake rings
+    for (int e = t; e < NL * kSwParts; e += kSwBlock) {
+        const int tq = e / kSwParts, q = e % kSwParts;
+        const double *x = lc + 3 * tq;
+        double u[3] = {0.0, 0.0, 0.0};
+        for (int k = q; k < 4 * P; k += kSwParts) {
+            const int j = k >> 2, leg = k & 3;
+            const double *R = ring + 12 * j;
+            const SegRec sr = sw_seg(R + 3 * leg, R + 3 * ((leg + 1) & 3), gam[j], I.rc0b, 0.0, I.nu);
+            sw_pair(I.prec, x, sr, u);
+        }
+        for (int k = q; k < 4 * W; k += kSwParts) {
+            const int rg = k >> 2, leg = k & 3, r = rg / ns, j = rg % ns;
+            const double *c4[4] = {wv + 3 * ((r + 1) * (ns + 1) + j + 1),
+                                   wv + 3 * (r * (ns + 1) + j + 1),
+                                   wv + 3 * (r * (ns + 1) + j),
+                                   wv + 3 * ((r + 1) * (ns + 1) + j)};
+            const SegRec sr = sw_seg(c4[leg], c4[(leg + 1) & 3], wg[rg], I.rc0w, wa[rg], I.nu);
+            sw_pair(I.prec, x, sr, u);
+        }
+        for (int c = 0; c < 3; ++c) red[(tq * kSwParts + q) * 3 + c] = u[c];
+    }
+    __syncthreads();
+    for (int tq = t; tq < NL; tq += kSwBlock)
+        for (int c = 0; c < 3; ++c) {
+            double a = 0.0;
+            for (int q = 0; q < kSwParts; ++q) a += red[(tq * kSwParts + q) * 3 + c];
+            vl[3 * tq + c] = a;
+        }
+    __syncthreads();
+    // ---- loads: Kutta-Joukowski on each leg with the net circulation of its
+    // line (right / left against the spanwise neighbour, front against the ring
+    // ahead, the trailing-edge back leg against the newest wake ring) + the
+    // unsteady term -rho A dGamma/dt n
+    const double *gprev = S + I.o_gamp;
+    for (int p = t; p < P; p += kSwBlock) {
+        const int i = p / ns, j = p % ns;
+        const double G = gam[p];
+        const double gw = nrow > 0 ? wg[j] : 0.0;
+        const double ge[4] = {(j + 1 < ns) ? G - gam[p + 1] : G, (i > 0) ? G - gam[p - ns] : G,
+                              (j > 0) ? G - gam[p - 1] : G, (i == nc - 1) ? G - gw : 0.0};
+        double Fp[3] = {0.0, 0.0, 0.0}, fl[4][3];
+        for (int l = 0; l < 4; ++l) {
+            if (l == 3 && i != nc - 1) {
+                fl[3][0] = fl[3][1] = fl[3][2] = 0.0;
+                continue;
+            }
+            const int q = l < 3 ? 3 * p + l : 3 * P + j;
+            const double u[3] = {V[0] - vml[3 * q] + vl[3 * q], V[1] - vml[3 * q + 1] + vl[3 * q + 1],
+                                 V[2] - vml[3 * q + 2] + vl[3 * q + 2]};
+            const double *Lg = lv + 3 * q;
+            const double f = I.rho * ge[l];
+            fl[l][0] = f * (u[1] * Lg[2] - u[2] * Lg[1]);
+            fl[l][1] = f * (u[2] * Lg[0] - u[0] * Lg[2]);
+            fl[l][2] = f * (u[0] * Lg[1] - u[1] * Lg[0]);
+            for (int c = 0; c < 3; ++c) Fp[c] += fl[l][c];
+        }
+        const double dG = s == 0 ? 0.0 : (G - gprev[p]) / dt;
+        const double fu = -I.rho * area[p] * dG;
+        for (int c = 0; c < 3; ++c) {
+            F[3 * p + c] = Fp[c] + fu * nrm[3 * p + c];
+            // the panel's share: the line to the row ahead half and half
+            PF[3 * p + c] = fl[0][c] + fl[2][c] + fl[1][c] * (i > 0 ? 0.5 : 1.0) + fl[3][c] +
+                            fu * nrm[3 * p + c];
+            Ff[3 * p + c] = fl[1][c];
+        }
+    }
+    __syncthreads();
+    const double V2 = V[0] * V[0] + V[1] * V[1] + V[2] * V[2], Vn = sqrt(V2);
+    const double dd[3] = {V[0] / Vn, V[1] / Vn, V[2] / Vn};
+    double l0 = -dd[2], l2 = dd[0];
+    const double ln = sqrt(l0 * l0 + l2 * l2);
+    l0 /= ln;
+    l2 /= ln;
+    if (t == 0) {
+        double Ft[3] = {0.0, 0.0, 0.0};
+        for (int p = 0; p < P; ++p)
+            for (int c = 0; c < 3; ++c) Ft[c] += F[3 * p + c];
+        const double qs = 0.5 * I.rho * V2 * I.sref;
+        S[I.o_cl + s] = (Ft[0] * l0 + Ft[2] * l2) / qs;
+        S[I.o_cdi + s] = (Ft[0] * dd[0] + Ft[1] * dd[1] + Ft[2] * dd[2]) / qs;
+    }
+    // ---- structure: station loads -> beam nodes, Newmark, next deformation
+    double *Fn = bm, *Fe = bm + 3 * kSwMaxNodes, *dn = Fe + kSwMaxFree, *an = dn + kSwMaxFree;
+    double *tmp = an + kSwMaxFree, *wnew = tmp + kSwMaxFree, *tnew = wnew + kSwMaxNodes;
+    const bool couple = I.couple && s >= 1 && s <= I.num_steps - 2;
+    if (couple) {
+        if (t == 0) {
+            for (int k = 0; k < 3 * nn; ++k) Fn[k] = 0.0;
+            for (int j = 0; j < ns; ++j) {
+                double lift = 0.0, mom = 0.0;
+                for (int i = 0; i < nc; ++i) {
+                    const int p = i * ns + j;
+                    const double fz = PF[3 * p + 2] + (i + 1 < nc ? 0.5 * Ff[3 * (p + ns) + 2] : 0.0);
+                    const double lp = I.lift_sign * fz;
+                    lift += lp;
+                    mom += col[3 * p] * lp;
+                }
+                const double y = col[3 * j + 1];   // first chordwise panel of the station
+                if (!(fabs(y) > 1e-6)) continue;
+                int idx = -1;
+                for (int k = 0; k < nn; ++k)
+                    if (yn[k] <= y) idx = k;
+                idx = idx < 0 ? 0 : (idx > nn - 2 ? nn - 2 : idx);
+                const double Le = yn[idx + 1] - yn[idx];
+                double xi = (y - yn[idx]) / Le;
+                xi = xi < 0.0 ? 0.0 : (xi > 1.0 ? 1.0 : xi);
+                Fn[3 * idx] += lift * (1 - xi);
+                Fn[3 * (idx + 1)] += lift * xi;
+                Fn[3 * idx + 2] += mom * (1 - xi);
+                Fn[3 * (idx + 1) + 2] += mom * xi;
+            }
+        }
+        __syncthreads();
+        const double be = I.beta, ga = I.gamma;
+        const double a0 = 1.0 / (be * dt * dt), a1 = 1.0 / (be * dt), a2 = 1.0 / (2.0 * be) - 1.0;
+        const double a3 = ga / (be * dt), a4 = ga / be - 1.0, a5 = dt * (ga / (2.0 * be) - 1.0);
+        const double *M = S + I.o_M, *C = S + I.o_C, *K = S + I.o_K;
+        double *d = S + I.o_d, *v = S + I.o_v, *a = S + I.o_a;
+        // F_eff = F + M (a0 d + a1 v + a2 a) + C (a3 d + a4 v + a5 a)
+        for (int r = t; r < nf; r += kSwBlock) {
+            double sm = 0.0, sc = 0.0;
+            for (int c = 0; c < nf; ++c) {
+                sm += M[r * nf + c] * (a0 * d[c] + a1 * v[c] + a2 * a[c]);
+                sc += C[r * nf + c] * (a3 * d[c] + a4 * v[c] + a5 * a[c]);
+            }
+            Fe[r] = Fn[3 + r] + sm + sc;
+        }
+        __syncthreads();
+        for (int r = t; r < nf; r += kSwBlock) {
+            double x = 0.0;
+            for (int c = 0; c < nf; ++c) x += K[r * nf + c] * Fe[c];
+            dn[r] = x;
+            an[r] = a0 * (dn[r] - d[r]) - a1 * v[r] - a2 * a[r];
+        }
+        __syncthreads();
+        for (int r = t; r < nf; r += kSwBlock) {
+            v[r] = v[r] + dt * ((1.0 - ga) * a[r] + ga * an[r]);
+            a[r] = an[r];
+            d[r] = dn[r];
+        }
+        __syncthreads();
+        if (t == 0) {
+            double *wp = S + I.o_wprev, *tp = S + I.o_tprev;
+            for (int k = 0; k < nn; ++k) {
+                double w = k == 0 ? 0.0 : d[3 * (k - 1)];
+                double th = k == 0 ? 0.0 : d[3 * (k - 1) + 2];
+                if (I.has_wprev) {
+                    w = I.relax * w + (1.0 - I.relax) * wp[k];
+                    th = I.relax * th + (1.0 - I.relax) * tp[k];
+                }
+                wnew[k] = w;
+                tnew[k] = th;
+                wp[k] = w;
+                tp[k] = th;
+            }
+            I.has_wprev = 1;
+            S[I.o_tw + s] = wnew[nn - 1];
+            S[I.o_tt + s] = tnew[nn - 1];
+            if (!I.feedback)   // the aerodynamics stay on the undeformed lattice
+                for (int k = 0; k < nn; ++k) {
+                    wnew[k] = 0.0;
+                    tnew[k] = 0.0;
+                }
+        }
+    } else if (t == 0) {
+        for (int k = 0; k < nn; ++k) {
+            wnew[k] = 0.0;
+            tnew[k] = 0.0;
+        }
+    }
+    __syncthreads();
+    // ---- wake shedding, conversion, particle convection (not after the last step)
+    const bool more = s + 1 < I.num_steps;
+    const int nxt = 1 - cur;
+    if (more) {
+        const int nrow_new = nrow + 1 < I.R ? nrow + 1 : I.R;
+        double *wvn = S + I.o_wv[nxt], *wgn = S + I.o_wg[nxt], *wan = S + I.o_wa[nxt];
+        const double ox = V[0] * dt, oy = V[1] * dt, oz = V[2] * dt;
+        // vertex rows: 0 = trailing edge of the next lattice; 1 = current trailing
+        // edge + V dt; r + 1 = old row r + V dt
+        for (int e = t; e < (nrow_new + 1) * (ns + 1); e += kSwBlock) {
+            const int r = e / (ns + 1), j = e % (ns + 1);
+            double o[3];
+            if (r == 0) {
+                double va[3], vb[3];
+                sw_deform(grid0 + 3 * ((nc - 1) * (ns + 1) + j), yn, wnew, tnew, nn, I.x_ea, va);
+                sw_deform(grid0 + 3 * (nc * (ns + 1) + j), yn, wnew, tnew, nn, I.x_ea, vb);
+                for (int c = 0; c < 3; ++c) o[c] = vb[c] + 0.25 * (vb[c] - va[c]);
+            } else if (r == 1) {
+                const double *va = sw_v(g, ns, nc - 1, j), *vb = sw_v(g, ns, nc, j);
+                for (int c = 0; c < 3; ++c) o[c] = vb[c] + 0.25 * (vb[c] - va[c]);
+                o[0] += ox;
+                o[1] += oy;
+                o[2] += oz;
+            } else {
+                const double *w = wv + 3 * ((r - 1) * (ns + 1) + j);
+                o[0] = w[0] + ox;
+                o[1] = w[1] + oy;
+                o[2] = w[2] + oz;
+            }
+            for (int c = 0; c < 3; ++c) wvn[3 * e + c] = o[c];
+        }
+        for (int e = t; e < nrow_new * ns; e += kSwBlock) {
+            const int r = e / ns, j = e % ns;
+            if (r == 0) {
+                wgn[e] = gam[(nc - 1) * ns + j];
+                wan[e] = 0.0;
+            } else {
+                wgn[e] = wg[(r - 1) * ns + j];
+                wan[e] = wa[(r - 1) * ns + j] + dt;
+            }
+        }
+        __syncthreads();
+        // conversion of ring row n_keep (span order, legs Fr->Fl, Fl->Bl, Bl->Br, Br->Fr)
+        if (I.n_keep < nrow_new && t == 0) {
+            const int kr = I.n_keep;
+            int64_t np = I.np;
+            double *pos = S + I.o_pos, *al = S + I.o_al, *sg = S + I.o_sg;
+            for (int j = 0; j < ns && !fail_s; ++j) {
+                const double G = wgn[kr * ns + j];
+                if (!(fabs(G) >= 1e-15)) continue;
+                const double *FL = wvn + 3 * (kr * (ns + 1) + j), *FR = FL + 3;
+                const double *BL = wvn + 3 * ((kr + 1) * (ns + 1) + j), *BR = BL + 3;
+                const double *ends[4][2] = {{FR, FL}, {FL, BL}, {BL, BR}, {BR, FR}};
+                for (int l = 0; l < 4; ++l) {
+                    double pp[3], aa[3];
+                    for (int c = 0; c < 3; ++c) {
+                        pp[c] = 0.5 * (ends[l][0][c] + ends[l][1][c]);
+                        aa[c] = G * (ends[l][1][c] - ends[l][0][c]);
+                    }
+                    if (!(aa[0] * aa[0] + aa[1] * aa[1] + aa[2] * aa[2] > 1e-30)) continue;
+                    if (np >= I.cap) {
+                        fail_s = 1;
+                        break;
+                    }
+                    for (int c = 0; c < 3; ++c) {
+                        pos[3 * np + c] = pp[c];
+                        al[3 * np + c] = aa[c];
+                    }
+                    sg[np] = I.sigma;
+                    ++np;
+                }
+                wgn[kr * ns + j] = 0.0;
+            }
+            I.np = np;
+        }
+        __syncthreads();
+        if (fail_s) {
+            if (t == 0) insts[blockIdx.x].status = -2;
+            return;
+        }
+        // particles: prescribed wake, x += V dt
+        double *pos = S + I.o_pos;
+        for (int64_t k = t; k < I.np; k += kSwBlock) {
+            pos[3 * k] += ox;
+            pos[3 * k + 1] += oy;
+            pos[3 * k + 2] += oz;
+        }
+        // next lattice: deformation and collocation points for the next induction
+        __syncthreads();
+        for (int k = t; k < nn; k += kSwBlock) {
+            wdef[k] = wnew[k];
+            tdef[k] = tnew[k];
+        }
+        double *gn = g;   // the current lattice is no longer needed
+        __syncthreads();
+        for (int v = t; v < nv; v += kSwBlock) sw_deform(grid0 + 3 * v, yn, wnew, tnew, nn, I.x_ea, gn + 3 * v);
+        __syncthreads();
+        for (int p = t; p < P; p += kSwBlock) {
+            double cn[3], nn3[3], ar;
+            sw_frame(gn, ns, p / ns, p % ns, cn, nn3, &ar);
+            for (int c = 0; c < 3; ++c) S[I.o_coln + 3 * p + c] = cn[c];
+        }
+        if (t == 0) {
+            I.nrow = nrow_new;
+            I.par = nxt;
+        }
+    }
+    // ---- carry this step's state
+    for (int p = t; p < P; p += kSwBlock) {
+        for (int c = 0; c < 3; ++c) S[I.o_colp + 3 * p + c] = col[3 * p + c];
+        for (int c = 0; c < 9; ++c) S[I.o_legp + 9 * p + c] = lc[9 * p + c];
+        if (p / ns == nc - 1)
+            for (int c = 0; c < 3; ++c) S[I.o_legp + 3 * (3 * P + p % ns) + c] = lc[3 * (3 * P + p % ns) + c];
+        S[I.o_gamp + p] = gam[p];
+        S[I.o_gam + p] = gam[p];
+    }
+    __syncthreads();
+    if (t == 0) {
+        SwInst &o = insts[blockIdx.x];
+        o.np = I.np;
+        o.nrow = I.nrow;
+        o.par = I.par;
+        o.has_wprev = I.has_wprev;
+        o.has_prev = 1;
+        o.step = s + 1;
+    }
+}
+
+// Collocation points of the undeformed lattice (the first induction's targets).
+__global__ __launch_bounds__(64) void k_sw_init(const SwInst *__restrict__ insts,
+                                                double *__restrict__ S)
+{
+    const SwInst &I = insts[blockIdx.x];
+    for (int p = threadIdx.x; p < I.P; p += 64) {
+        const double *g = S + I.o_grid0;
+        double cn[3], nn3[3], ar;
+        // an undeformed lattice: sw_deform with zero heave / twist is the identity
+        sw_frame(g, I.ns, p / I.ns, p % I.ns, cn, nn3, &ar);
+        for (int c = 0; c < 3; ++c) S[I.o_coln + 3 * p + c] = cn[c];
+    }
+}
+
+}  // namespace fv
+
+using namespace fv;
+
+struct fv_sweep {
+    fv_ctx *ctx = nullptr;
+    int n = 0;
+    std::vector<SwInst> host;
+    SwInst *dinst = nullptr;
+    double *block = nullptr;
+    int64_t max_chunks = 1, max_steps = 0, done_steps = 0;
+    int use_graph = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+static int sweep_launch(fv_sweep *sw)
+{
+    fv_ctx *ctx = sw->ctx;
+    hipLaunchKernelGGL(k_sw_induce, dim3((unsigned)sw->max_chunks, (unsigned)sw->n),
+                       dim3(kSwBlock), 0, ctx->stream, sw->dinst, sw->block);
+    FV_LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_sw_step, dim3((unsigned)sw->n), dim3(kSwBlock), kSwLdsBytes, ctx->stream,
+                       sw->dinst, sw->block);
+    FV_LAUNCH_CHECK(ctx);
+    return FV_OK;
+}
+
+extern "C" {
+
+int fv_sweep_create(fv_ctx *ctx, const fv_sweep_desc *descs, int32_t n, int32_t use_graph,
+                    fv_sweep **out)
+{
+    if (!ctx || !descs || !out) return FV_E_ARG;
+    *out = nullptr;
+    FV_REQUIRE(ctx, n >= 1 && n <= 65535, "instance count %d outside 1..65535", n);
+    fv_sweep *sw = new (std::nothrow) fv_sweep();
+    if (!sw) return set_error(ctx, FV_E_HIP, "out of host memory");
+    sw->ctx = ctx;
+    sw->n = n;
+    sw->use_graph = use_graph ? 1 : 0;
+    int64_t off = 0;
+    auto take = [&](int64_t k) { const int64_t o = off; off += (k + 3) & ~int64_t(3); return o; };
+    for (int i = 0; i < n; ++i) {
+        const fv_sweep_desc &d = descs[i];
+        const double V = sqrt(d.vinf[0] * d.vinf[0] + d.vinf[1] * d.vinf[1] + d.vinf[2] * d.vinf[2]);
+        const int nfree = 3 * (d.n_nodes - 1);
+        if (!(d.nc >= 1 && d.nc <= kSwMaxNc && d.ns >= 1 && d.ns <= kSwMaxNs &&
+              d.nc * d.ns <= kSwMaxP && d.grid && d.num_steps >= 1 && d.dt > 0 && d.rho > 0 &&
+              d.sref > 0 && V > 1e-10 && d.n_keep >= 0 && d.n_keep <= kSwMaxR - 1 &&
+              d.max_particles >= 1 && d.n_nodes >= 2 && d.n_nodes <= kSwMaxNodes && d.y_nodes &&
+              (d.kernel == FV_KERNEL_GAUSSIAN_ERF || d.kernel == FV_KERNEL_WINCKELMANS) &&
+              (d.precision == FV_PREC_FP64 || d.precision == FV_PREC_MIXED) &&
+              (!d.couple || (d.M && d.C && d.Keff_inv && d.d0 && d.v0 && d.a0)))) {
+            delete sw;
+            return set_error(ctx, FV_E_ARG, "sweep instance %d: invalid or out-of-limit description",
+                             i);
+        }
+        SwInst I{};
+        I.nc = d.nc;
+        I.ns = d.ns;
+        I.P = d.nc * d.ns;
+        I.nv = (d.nc + 1) * (d.ns + 1);
+        I.R = (int)(d.n_keep + 1 < d.num_steps ? d.n_keep + 1 : d.num_steps);
+        I.n_keep = d.n_keep;
+        I.kernel = d.kernel;
+        I.prec = d.precision;
+        I.couple = d.couple ? 1 : 0;
+        I.n_nodes = d.n_nodes;
+        I.nfree = nfree;
+        I.num_steps = d.num_steps;
+        I.cap = d.max_particles;
+        for (int c = 0; c < 3; ++c) I.V[c] = d.vinf[c];
+        I.rho = d.rho;
+        I.dt = d.dt;
+        I.nu = d.nu;
+        I.rc0b = d.rc0_bound;
+        I.rc0w = d.rc0_wake;
+        I.sigma = d.sigma;
+        I.sref = d.sref;
+        I.x_ea = d.x_ea;
+        I.relax = d.relax;
+        I.feedback = d.feedback ? 1 : 0;
+        I.lift_sign = d.lift_sign;
+        I.beta = d.beta;
+        I.gamma = d.gamma;
+        I.max_chunks = ceil_div(d.max_particles, kSwChunk);
+        if (I.max_chunks > sw->max_chunks) sw->max_chunks = I.max_chunks;
+        if (d.num_steps > sw->max_steps) sw->max_steps = d.num_steps;
+        I.o_grid0 = take(3 * I.nv);
+        I.o_coln = take(3 * I.P);
+        I.o_colp = take(3 * I.P);
+        I.o_legp = take(3 * (3 * I.P + d.ns));
+        I.o_gamp = take(I.P);
+        I.o_gam = take(I.P);
+        for (int b = 0; b < 2; ++b) {
+            I.o_wv[b] = take(3 * (I.R + 1) * (d.ns + 1));
+            I.o_wg[b] = take(I.R * d.ns);
+            I.o_wa[b] = take(I.R * d.ns);
+        }
+        I.o_pos = take(3 * I.cap);
+        I.o_al = take(3 * I.cap);
+        I.o_sg = take(I.cap);
+        I.o_part = take(3 * I.P * I.max_chunks);
+        I.o_M = take((int64_t)nfree * nfree);
+        I.o_C = take((int64_t)nfree * nfree);
+        I.o_K = take((int64_t)nfree * nfree);
+        I.o_d = take(nfree);
+        I.o_v = take(nfree);
+        I.o_a = take(nfree);
+        I.o_wdef = take(d.n_nodes);
+        I.o_tdef = take(d.n_nodes);
+        I.o_wprev = take(d.n_nodes);
+        I.o_tprev = take(d.n_nodes);
+        I.o_yn = take(d.n_nodes);
+        I.o_cl = take(d.num_steps);
+        I.o_cdi = take(d.num_steps);
+        I.o_tw = take(d.num_steps);
+        I.o_tt = take(d.num_steps);
+        sw->host.push_back(I);
+    }
+    // host image of the state block, then one upload
+    std::vector<double> img;
+    try {
+        img.assign((size_t)off, 0.0);
+    } catch (...) {
+        delete sw;
+        return set_error(ctx, FV_E_HIP, "out of host memory (%lld doubles)", (long long)off);
+    }
+    for (int i = 0; i < n; ++i) {
+        const fv_sweep_desc &d = descs[i];
+        const SwInst &I = sw->host[i];
+        memcpy(&img[I.o_grid0], d.grid, sizeof(double) * 3 * I.nv);
+        memcpy(&img[I.o_yn], d.y_nodes, sizeof(double) * d.n_nodes);
+        const int nf = I.nfree;
+        if (d.couple) {
+            memcpy(&img[I.o_M], d.M, sizeof(double) * nf * nf);
+            memcpy(&img[I.o_C], d.C, sizeof(double) * nf * nf);
+            memcpy(&img[I.o_K], d.Keff_inv, sizeof(double) * nf * nf);
+            memcpy(&img[I.o_d], d.d0, sizeof(double) * nf);
+            memcpy(&img[I.o_v], d.v0, sizeof(double) * nf);
+            memcpy(&img[I.o_a], d.a0, sizeof(double) * nf);
+        }
+        for (int64_t k = 0; k < I.num_steps; ++k) {
+            img[I.o_tw + k] = NAN;
+            img[I.o_tt + k] = NAN;
+        }
+    }
+    hipError_t e1 = hipMalloc(&sw->block, sizeof(double) * (size_t)(off > 0 ? off : 1));
+    hipError_t e2 = hipMalloc(&sw->dinst, sizeof(SwInst) * n);
+    if (e1 != hipSuccess || e2 != hipSuccess) {
+        if (sw->block) (void)hipFree(sw->block);
+        if (sw->dinst) (void)hipFree(sw->dinst);
+        delete sw;
+        return set_error(ctx, FV_E_HIP, "hipMalloc failed for the sweep state");
+    }
+    int rc = FV_OK;
+    if (hipMemcpyAsync(sw->block, img.data(), sizeof(double) * (size_t)off, hipMemcpyHostToDevice,
+                       ctx->stream) != hipSuccess ||
+        hipMemcpyAsync(sw->dinst, sw->host.data(), sizeof(SwInst) * n, hipMemcpyHostToDevice,
+                       ctx->stream) != hipSuccess)
+        rc = set_error(ctx, FV_E_HIP, "sweep upload failed");
+    if (!rc) {
+        hipLaunchKernelGGL(k_sw_init, dim3((unsigned)n), dim3(64), 0, ctx->stream, sw->dinst,
+                           sw->block);
+        if (hipGetLastError() != hipSuccess) rc = set_error(ctx, FV_E_HIP, "sweep init launch failed");
+    }
+    if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess)
+        rc = set_error(ctx, FV_E_HIP, "sweep set-up failed");
+    if (rc) {
+        fv_sweep_destroy(sw);
+        return rc;
+    }
+    *out = sw;
+    return FV_OK;
+}
+
+void fv_sweep_destroy(fv_sweep *sw)
+{
+    if (!sw) return;
+    (void)hipStreamSynchronize(sw->ctx->stream);
+    if (sw->exec) (void)hipGraphExecDestroy(sw->exec);
+    if (sw->graph) (void)hipGraphDestroy(sw->graph);
+    if (sw->block) (void)hipFree(sw->block);
+    if (sw->dinst) (void)hipFree(sw->dinst);
+    delete sw;
+}
+
+int fv_sweep_step(fv_sweep *sw, int64_t steps)
+{
+    if (!sw) return FV_E_ARG;
+    fv_ctx *ctx = sw->ctx;
+    FV_REQUIRE(ctx, steps >= 0, "negative step count");
+    int64_t k = steps;
+    if (sw->done_steps + k > sw->max_steps) k = sw->max_steps - sw->done_steps;
+    if (k <= 0) return FV_OK;
+    if (sw->use_graph && !sw->exec) {
+        FV_HIP_CHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+        const int rc = sweep_launch(sw);
+        hipGraph_t gr = nullptr;
+        const hipError_t ec = hipStreamEndCapture(ctx->stream, &gr);
+        if (rc) {
+            if (gr) (void)hipGraphDestroy(gr);
+            return rc;
+        }
+        if (ec != hipSuccess) return set_error(ctx, FV_E_HIP, "sweep graph capture failed");
+        sw->graph = gr;
+        FV_HIP_CHECK(ctx, hipGraphInstantiate(&sw->exec, sw->graph, nullptr, nullptr, 0));
+    }
+    TimedLaunch tl(ctx, "sweep_step");
+    for (int64_t q = 0; q < k; ++q) {
+        if (sw->use_graph) {
+            FV_HIP_CHECK(ctx, hipGraphLaunch(sw->exec, ctx->stream));
+        } else {
+            const int rc = sweep_launch(sw);
+            if (rc) return rc;
+        }
+    }
+    tl.done();
+    sw->done_steps += k;
+    return FV_OK;
+}
+
+int fv_sweep_read(fv_sweep *sw, int32_t i, double *cl, double *cdi, double *tip_w,
+                  double *tip_theta, double *gamma, int64_t *np, int32_t *status)
+{
+    if (!sw) return FV_E_ARG;
+    fv_ctx *ctx = sw->ctx;
+    FV_REQUIRE(ctx, i >= 0 && i < sw->n, "instance %d outside 0..%d", i, sw->n - 1);
+    const SwInst &H = sw->host[i];
+    SwInst D;
+    FV_HIP_CHECK(ctx, hipMemcpyAsync(&D, sw->dinst + i, sizeof D, hipMemcpyDeviceToHost, ctx->stream));
+    const int64_t ns = H.num_steps;
+    if (cl) FV_HIP_CHECK(ctx, hipMemcpyAsync(cl, sw->block + H.o_cl, sizeof(double) * ns, hipMemcpyDeviceToHost, ctx->stream));
+    if (cdi) FV_HIP_CHECK(ctx, hipMemcpyAsync(cdi, sw->block + H.o_cdi, sizeof(double) * ns, hipMemcpyDeviceToHost, ctx->stream));
+    if (tip_w) FV_HIP_CHECK(ctx, hipMemcpyAsync(tip_w, sw->block + H.o_tw, sizeof(double) * ns, hipMemcpyDeviceToHost, ctx->stream));
+    if (tip_theta) FV_HIP_CHECK(ctx, hipMemcpyAsync(tip_theta, sw->block + H.o_tt, sizeof(double) * ns, hipMemcpyDeviceToHost, ctx->stream));
+    if (gamma) FV_HIP_CHECK(ctx, hipMemcpyAsync(gamma, sw->block + H.o_gam, sizeof(double) * H.P, hipMemcpyDeviceToHost, ctx->stream));
+    FV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if (np) *np = D.np;
+    if (status) *status = D.status;
+    return FV_OK;
+}
+
+}  // extern "C"

@@ -290,6 +290,65 @@ int fv_vpm_closure_sum_host(fv_vpm *f, int32_t mode, int32_t kernel, int32_t use
 /* Runs `rounds` online-training rounds on the current particle state. */
 int fv_vpm_closure_train(fv_vpm *f, int32_t rounds);
 
+/* ------------------------------------------- batched aeroelastic sweep (f2) */
+/* Many independent aeroelastic hybrid UVLM instances (a flutter-speed sweep)
+ * stepped together: per step one batched particle-induction launch and one
+ * launch with a workgroup per instance for the rest of the step, captured once
+ * in a hipGraph and replayed.  Replaces the per-speed loop of
+ * tests/benchmark_goland.py:100-243 over AeroelasticSolver
+ * (src/fluxvortex/aeroelastic_solver.py:63-221) and BeamFE.step
+ * (src/fluxvortex/beam_fe.py:231-277), with the hybrid N_keep wake.
+ *
+ * Per instance and step s: the lattice = the undeformed grid deformed by the
+ * beam's nodal heave w and twist theta (each vertex with w, theta linearly
+ * interpolated at its |y|; rotation about the elastic axis x_ea); the hybrid
+ * UVLM step (the device driver's statement); for 1 <= s <= num_steps - 2 the
+ * panel forces' lift components, summed per spanwise station (station y = the
+ * first chordwise panel's collocation y, moment = sum x_colloc * lift about
+ * x = 0), go linearly to the beam nodes, one Newmark step advances the beam
+ * and its heave / twist deform the lattice of step s + 1 (else step s + 1 is
+ * undeformed).  Particles are a prescribed wake convected with V dt.
+ * Limits: nc <= 16, ns <= 32, nc * ns <= 64, n_keep <= 15, beam nodes <= 17. */
+typedef struct fv_sweep_desc {
+    int32_t nc, ns;              /* one lattice, (nc+1) x (ns+1) corner grid */
+    const double *grid;          /* undeformed corner grid (host), x aft, y span */
+    int64_t num_steps;
+    double vinf[3];
+    double rho, dt;
+    double nu;                   /* ring core growth viscosity */
+    double rc0_bound, rc0_wake;
+    int32_t n_keep;
+    double sigma;                /* particle core of converted rings */
+    int32_t kernel;              /* FV_KERNEL_* of the particle induction */
+    double sref;
+    int64_t max_particles;
+    int32_t couple;              /* 0: rigid lattice (no structure) */
+    int32_t n_nodes;             /* beam nodes incl. the clamped root */
+    const double *y_nodes;       /* n_nodes, increasing from the root */
+    const double *M, *C, *Keff_inv; /* free-DOF matrices, 3 (n_nodes - 1) square */
+    const double *d0, *v0, *a0;  /* free-DOF initial state */
+    double beta, gamma;          /* Newmark parameters */
+    double x_ea;                 /* elastic axis x (absolute) */
+    double relax;                /* deformation relaxation factor (1: none) */
+    int32_t precision;           /* FV_PREC_FP64, or FV_PREC_MIXED for FP32 pair terms */
+    int32_t feedback;            /* 1: the deformed lattice carries the next step's
+                                  * aerodynamics; 0: the aerodynamics stay on the
+                                  * undeformed lattice (the reference's behaviour) */
+    double lift_sign;            /* station lift = lift_sign * F_z (reference: -1) */
+} fv_sweep_desc;
+typedef struct fv_sweep fv_sweep;
+int fv_sweep_create(fv_ctx *ctx, const fv_sweep_desc *descs, int32_t n, int32_t use_graph,
+                    fv_sweep **out);
+void fv_sweep_destroy(fv_sweep *sw);
+/* Advances every unfinished instance by up to `steps` steps. */
+int fv_sweep_step(fv_sweep *sw, int64_t steps);
+/* Host copies of instance i: cl, cdi, tip_w, tip_theta [num_steps] (tip
+ * values are NaN where no coupling ran), gamma [nc*ns], particle count; any
+ * pointer may be NULL.  status: 0, or the step + 1 of a zero pivot, or -2 on
+ * particle overflow. */
+int fv_sweep_read(fv_sweep *sw, int32_t i, double *cl, double *cdi, double *tip_w,
+                  double *tip_theta, double *gamma, int64_t *np, int32_t *status);
+
 #ifdef __cplusplus
 }
 #endif

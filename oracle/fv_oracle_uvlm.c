@@ -61,7 +61,38 @@ typedef struct orc_hybrid_desc {
     double sref;
     int64_t max_particles;
     int32_t precision;   /* ignored: the oracle is FP64 */
+    /* aeroelastic coupling (optional; one surface).  With `beam` set, `grids`
+     * holds only the undeformed lattice: each step's lattice is that grid
+     * deformed by the beam's nodal heave / twist (aeroelastic_solver.py:63-221,
+     * beam_fe.py:231-351, the device sweep's statement). */
+    const struct orc_beam *beam;
+    /* 1: particles are a prescribed wake moved by V dt per step (no RK3 /
+     * clamp / relaxation) */
+    int32_t convect_exact;
 } orc_hybrid_desc;
+
+typedef struct orc_beam {
+    int32_t n_nodes;                /* including the clamped root */
+    const double *y_nodes;
+    const double *M, *C, *Keff_inv; /* free DOFs, 3 (n_nodes - 1) square, row-major */
+    double *d, *v, *a;              /* free-DOF state, advanced in place */
+    double beta, gamma;             /* Newmark */
+    double x_ea;                    /* elastic axis x */
+    double relax;                   /* deformation relaxation (1: none) */
+    double *tip_w, *tip_theta;      /* [num_steps] outputs, NaN where no coupling */
+    /* 1: the deformed lattice carries the aerodynamics of the next step;
+     * 0: the aerodynamics stay on the undeformed lattice while the beam is
+     * still driven by the loads (the reference's vertex mutation does not
+     * reach its solver: its stored Goland histories show loads independent
+     * of the deformation, figures/flutter_data.npz) */
+    int32_t feedback;
+    /* station lift = lift_sign * F_z (the reference: -F_z,
+     * aeroelastic_solver.py:135-139) */
+    double lift_sign;
+    /* optional [num_steps x ns] outputs: station lift and moment about x = 0
+     * at each coupled step (NaN elsewhere) */
+    double *station_lift, *station_moment;
+} orc_beam;
 
 typedef struct {
     int nc, ns;
@@ -118,6 +149,35 @@ static void te_of(const double *g, int nc, int ns, int j, double *o)
 {
     const double *a = gv(g, ns, nc - 1, j), *b = gv(g, ns, nc, j);
     for (int c = 0; c < 3; ++c) o[c] = b[c] + 0.25 * (b[c] - a[c]);
+}
+
+/* numpy.interp: linear inside [xp0, xp_last], the end values outside. */
+static double np_interp(double x, const double *xp, const double *fp, int n)
+{
+    if (x <= xp[0]) return fp[0];
+    if (x >= xp[n - 1]) return fp[n - 1];
+    int j = 0;
+    while (j + 1 < n - 1 && xp[j + 1] <= x) ++j;
+    if (x == xp[j]) return fp[j];
+    double slope = (fp[j + 1] - fp[j]) / (xp[j + 1] - xp[j]);
+    return slope * (x - xp[j]) + fp[j];
+}
+
+/* aeroelastic_solver.py:176-198: heave w and twist theta at the vertex's |y|,
+ * rotation about x_ea (applied to every lattice vertex). */
+static void deform_grid(const double *base, int64_t nv, const orc_beam *bm, const double *w,
+                        const double *th, double *out)
+{
+    for (int64_t v = 0; v < nv; ++v) {
+        const double *b = base + 3 * v;
+        double y = fabs(b[1]);
+        double wv = np_interp(y, bm->y_nodes, w, bm->n_nodes);
+        double tv = np_interp(y, bm->y_nodes, th, bm->n_nodes);
+        double xr = b[0] - bm->x_ea;
+        out[3 * v] = b[0] + xr * (cos(tv) - 1.0);
+        out[3 * v + 1] = b[1];
+        out[3 * v + 2] = b[2] + (wv + xr * sin(tv));
+    }
 }
 
 /* Collects the non-zero wake rings of all surfaces into flat arrays. */
@@ -178,13 +238,13 @@ int orc_hybrid_run(const orc_hybrid_desc *d, int64_t steps, double *cl, double *
     double *fl = malloc(sizeof(double) * 3 * P), *bl = malloc(sizeof(double) * 3 * P);
     double *col = malloc(sizeof(double) * 3 * P), *nrm = malloc(sizeof(double) * 3 * P);
     double *area = malloc(sizeof(double) * P), *vmc = malloc(sizeof(double) * 3 * P);
-    double *lc = malloc(sizeof(double) * 9 * P), *lv = malloc(sizeof(double) * 9 * P);
-    double *vml = malloc(sizeof(double) * 9 * P);
+    double *lc = malloc(sizeof(double) * 12 * P), *lv = malloc(sizeof(double) * 12 * P);
+    double *vml = malloc(sizeof(double) * 12 * P), *lf = malloc(sizeof(double) * 12 * P);
     double *A = malloc(sizeof(double) * P * P), *rhs = malloc(sizeof(double) * P);
     double *gam = calloc((size_t)P, sizeof(double)), *gprev = calloc((size_t)P, sizeof(double));
     double *rc0b = malloc(sizeof(double) * P), *vw = malloc(sizeof(double) * 3 * P);
-    double *vp = calloc((size_t)3 * P, sizeof(double)), *vl = malloc(sizeof(double) * 9 * P);
-    double *vl2 = malloc(sizeof(double) * 9 * P);
+    double *vp = calloc((size_t)3 * P, sizeof(double)), *vl = malloc(sizeof(double) * 12 * P);
+    double *vl2 = malloc(sizeof(double) * 12 * P);
     double *wbr = malloc(sizeof(double) * 3 * WRmax), *wfr = malloc(sizeof(double) * 3 * WRmax);
     double *wfl = malloc(sizeof(double) * 3 * WRmax), *wbl = malloc(sizeof(double) * 3 * WRmax);
     double *wg = malloc(sizeof(double) * WRmax), *wa = malloc(sizeof(double) * WRmax);
@@ -194,7 +254,36 @@ int orc_hybrid_run(const orc_hybrid_desc *d, int64_t steps, double *cl, double *
     double *psg = malloc(sizeof(double) * cap), *page = calloc((size_t)cap, sizeof(double));
     double *tpos = malloc(sizeof(double) * 12 * NSt), *tal = malloc(sizeof(double) * 12 * NSt);
     double *tsg = malloc(sizeof(double) * 4 * NSt);
-    if (!br || !fr || !fl || !bl || !col || !nrm || !area || !vmc || !lc || !lv || !vml || !A ||
+    /* aeroelastic buffers */
+    const orc_beam *bm = d->beam;
+    const int NN = bm ? bm->n_nodes : 1, NF = 3 * (NN - 1);
+    double *pf = malloc(sizeof(double) * 3 * P);
+    double *gb[3] = {NULL, NULL, NULL};   /* current, previous, next deformed lattice */
+    double *wc = calloc((size_t)NN, sizeof(double)), *tc = calloc((size_t)NN, sizeof(double));
+    double *wn = calloc((size_t)NN, sizeof(double)), *tn = calloc((size_t)NN, sizeof(double));
+    double *wpv = calloc((size_t)NN, sizeof(double)), *tpv = calloc((size_t)NN, sizeof(double));
+    double *Fn = calloc((size_t)3 * NN, sizeof(double));
+    double *Fe = calloc((size_t)(NF > 0 ? NF : 1), sizeof(double));
+    double *dn = calloc((size_t)(NF > 0 ? NF : 1), sizeof(double));
+    double *an = calloc((size_t)(NF > 0 ? NF : 1), sizeof(double));
+    int has_wprev = 0;
+    if (bm) {
+        for (int k = 0; k < 3; ++k) {
+            gb[k] = malloc(sizeof(double) * 3 * TV);
+            if (!gb[k]) fail = 1;
+        }
+        if (S != 1) fail = 1;
+        for (int64_t k = 0; k < d->num_steps; ++k) {
+            bm->tip_w[k] = NAN;
+            bm->tip_theta[k] = NAN;
+        }
+        for (int64_t k = 0; k < d->num_steps * (int64_t)d->ns[0]; ++k) {
+            if (bm->station_lift) bm->station_lift[k] = NAN;
+            if (bm->station_moment) bm->station_moment[k] = NAN;
+        }
+    }
+    if (!pf || !wc || !tc || !wn || !tn || !wpv || !tpv || !Fn || !Fe || !dn || !an) fail = 1;
+    if (!br || !fr || !fl || !bl || !col || !nrm || !area || !vmc || !lc || !lv || !vml || !lf || !A ||
         !rhs || !gam || !gprev || !rc0b || !vw || !vp || !vl || !vl2 || !wbr || !wfr || !wfl ||
         !wbl || !wg || !wa || !wrc || !zage || !ppos || !pal || !psg || !page || !tpos || !tal ||
         !tsg)
@@ -210,8 +299,18 @@ int orc_hybrid_run(const orc_hybrid_desc *d, int64_t steps, double *cl, double *
     const double *V = d->vinf;
     const double dt = d->dt;
     for (int64_t st = 0; st < steps && st < d->num_steps; ++st) {
-        const double *G = d->grids + (size_t)st * TV * 3;
-        const double *Gp = st > 0 ? d->grids + (size_t)(st - 1) * TV * 3 : NULL;
+        const double *G, *Gp;
+        if (bm) {
+            if (bm->feedback)
+                deform_grid(d->grids, TV, bm, wc, tc, gb[0]);
+            else
+                memcpy(gb[0], d->grids, sizeof(double) * 3 * TV);
+            G = gb[0];
+            Gp = st > 0 ? gb[1] : NULL;
+        } else {
+            G = d->grids + (size_t)st * TV * 3;
+            Gp = st > 0 ? d->grids + (size_t)(st - 1) * TV * 3 : NULL;
+        }
         /* geometry */
         for (int s = 0; s < S; ++s) {
             const double *g = G + 3 * sf[s].voff, *gp = Gp ? Gp + 3 * sf[s].voff : NULL;
@@ -235,18 +334,18 @@ int orc_hybrid_run(const orc_hybrid_desc *d, int64_t steps, double *cl, double *
                         frame_of(gp, ns, i, j, cp, np_, &ap);
                     }
                     for (int k = 0; k < 3; ++k) vmc[3 * p + k] = gp ? (c[k] - cp[k]) / dt : 0.0;
-                    /* legs: 0 right Br->Fr, 1 front Fr->Fl, 2 left Fl->Bl */
-                    for (int l = 0; l < 3; ++l) {
-                        const double *s0 = R[l], *e0 = R[l + 1];
+                    /* legs: 0 right Br->Fr, 1 front Fr->Fl, 2 left Fl->Bl, 3 back Bl->Br */
+                    for (int l = 0; l < 4; ++l) {
+                        const double *s0 = R[l], *e0 = R[(l + 1) & 3];
                         for (int k = 0; k < 3; ++k) {
                             double cc = 0.5 * (s0[k] + e0[k]);
-                            lc[3 * (3 * p + l) + k] = cc;
-                            lv[3 * (3 * p + l) + k] = e0[k] - s0[k];
+                            lc[3 * (4 * p + l) + k] = cc;
+                            lv[3 * (4 * p + l) + k] = e0[k] - s0[k];
                             if (gp) {
-                                double pc = 0.5 * (Rp[l][k] + Rp[l + 1][k]);
-                                vml[3 * (3 * p + l) + k] = (cc - pc) / dt;
+                                double pc = 0.5 * (Rp[l][k] + Rp[(l + 1) & 3][k]);
+                                vml[3 * (4 * p + l) + k] = (cc - pc) / dt;
                             } else {
-                                vml[3 * (3 * p + l) + k] = 0.0;
+                                vml[3 * (4 * p + l) + k] = 0.0;
                             }
                         }
                     }
@@ -275,32 +374,53 @@ int orc_hybrid_run(const orc_hybrid_desc *d, int64_t steps, double *cl, double *
         }
         memcpy(gam, rhs, sizeof(double) * P);
         /* loads */
-        orc_collapsed(lc, 3 * P, br, fr, fl, bl, P, gam, rc0b, zage, d->nu, 0xF, vl, nthreads);
-        orc_collapsed(lc, 3 * P, wbr, wfr, wfl, wbl, W, wg, wrc, wa, d->nu, 0xF, vl2, nthreads);
+        orc_collapsed(lc, 4 * P, br, fr, fl, bl, P, gam, rc0b, zage, d->nu, 0xF, vl, nthreads);
+        orc_collapsed(lc, 4 * P, wbr, wfr, wfl, wbl, W, wg, wrc, wa, d->nu, 0xF, vl2, nthreads);
+        /* Kutta-Joukowski on each ring's legs with the net circulation of the
+         * line it lies on: right / left legs against the spanwise neighbour
+         * (each side counts its own), the front leg against the ring ahead
+         * (whole on the leading edge), the back leg only on the trailing edge,
+         * against the newest wake ring (whole when there is none) -- interior
+         * back legs are the next ring's front legs.  Plus the unsteady term
+         * -rho A dGamma/dt n. */
         double F[3] = {0.0, 0.0, 0.0};
         for (int s = 0; s < S; ++s) {
-            int ns = sf[s].ns;
-            for (int64_t loc = 0; loc < (int64_t)sf[s].nc * ns; ++loc) {
+            int ns = sf[s].ns, nc = sf[s].nc;
+            for (int64_t loc = 0; loc < (int64_t)nc * ns; ++loc) {
                 int64_t p = sf[s].poff + loc;
                 int i = (int)(loc / ns), j = (int)(loc % ns);
                 double Gm = gam[p];
-                double ge[3] = {(j + 1 < ns) ? Gm - gam[p + 1] : Gm, (i > 0) ? Gm - gam[p - ns] : Gm,
-                                (j > 0) ? Gm - gam[p - 1] : Gm};
+                double gw = sf[s].nrow > 0 ? sf[s].wg[j] : 0.0;
+                double ge[4] = {(j + 1 < ns) ? Gm - gam[p + 1] : Gm, (i > 0) ? Gm - gam[p - ns] : Gm,
+                                (j > 0) ? Gm - gam[p - 1] : Gm, (i == nc - 1) ? Gm - gw : 0.0};
                 double Fp[3] = {0.0, 0.0, 0.0};
-                for (int l = 0; l < 3; ++l) {
-                    int64_t q = 3 * p + l;
+                for (int l = 0; l < 4; ++l) {
+                    int64_t q = 4 * p + l;
                     double u[3];
                     for (int k = 0; k < 3; ++k)
                         u[k] = V[k] - vml[3 * q + k] + (vl[3 * q + k] + vl2[3 * q + k]);
                     const double *L = lv + 3 * q;
                     double f = d->rho * ge[l];
-                    Fp[0] += f * (u[1] * L[2] - u[2] * L[1]);
-                    Fp[1] += f * (u[2] * L[0] - u[0] * L[2]);
-                    Fp[2] += f * (u[0] * L[1] - u[1] * L[0]);
+                    lf[3 * q] = f * (u[1] * L[2] - u[2] * L[1]);
+                    lf[3 * q + 1] = f * (u[2] * L[0] - u[0] * L[2]);
+                    lf[3 * q + 2] = f * (u[0] * L[1] - u[1] * L[0]);
+                    for (int k = 0; k < 3; ++k) Fp[k] += lf[3 * q + k];
                 }
                 double dG = st == 0 ? 0.0 : (Gm - gprev[p]) / dt;
                 double fu = -d->rho * area[p] * dG;
                 for (int k = 0; k < 3; ++k) F[k] += Fp[k] + fu * nrm[3 * p + k];
+                /* the panel's own force: the line between two rows of a
+                 * strip is shared half and half by the two panels (fitted
+                 * to the reference's stored Goland station moments) */
+                for (int k = 0; k < 3; ++k) {
+                    double fr_ = lf[3 * (4 * p + 1) + k] * (i > 0 ? 0.5 : 1.0);
+                    pf[3 * p + k] = lf[3 * (4 * p) + k] + lf[3 * (4 * p + 2) + k] + fr_ +
+                                    lf[3 * (4 * p + 3) + k] + fu * nrm[3 * p + k];
+                }
+            }
+            for (int64_t loc = ns; loc < (int64_t)nc * ns; ++loc) {
+                int64_t p = sf[s].poff + loc;
+                for (int k = 0; k < 3; ++k) pf[3 * (p - ns) + k] += 0.5 * lf[3 * (4 * p + 1) + k];
             }
         }
         {
@@ -312,11 +432,94 @@ int orc_hybrid_run(const orc_hybrid_desc *d, int64_t steps, double *cl, double *
             double qs = 0.5 * d->rho * V2 * d->sref;
             if (cl) cl[st] = (F[0] * l0 + F[2] * l2) / qs;
             if (cdi) cdi[st] = (F[0] * dd[0] + F[1] * dd[1] + F[2] * dd[2]) / qs;
+            /* structure (aeroelastic_solver.py:68-110, beam_fe.py:231-351) */
+            if (bm && st >= 1 && st <= d->num_steps - 2) {
+                const int nc = sf[0].nc, ns = sf[0].ns;
+                for (int k = 0; k < 3 * NN; ++k) Fn[k] = 0.0;
+                for (int j = 0; j < ns; ++j) {
+                    double lift = 0.0, mom = 0.0;
+                    for (int i = 0; i < nc; ++i) {
+                        int64_t p = (int64_t)i * ns + j;
+                        double lp = bm->lift_sign * pf[3 * p + 2];
+                        lift += lp;
+                        mom += col[3 * p] * lp;
+                    }
+                    double y = col[3 * j + 1];
+                    if (bm->station_lift) bm->station_lift[st * ns + j] = lift;
+                    if (bm->station_moment) bm->station_moment[st * ns + j] = mom;
+                    if (!(fabs(y) > 1e-6)) continue;
+                    int idx = -1;
+                    for (int k = 0; k < NN; ++k)
+                        if (bm->y_nodes[k] <= y) idx = k;
+                    idx = idx < 0 ? 0 : (idx > NN - 2 ? NN - 2 : idx);
+                    double Le = bm->y_nodes[idx + 1] - bm->y_nodes[idx];
+                    double xi = (y - bm->y_nodes[idx]) / Le;
+                    xi = xi < 0.0 ? 0.0 : (xi > 1.0 ? 1.0 : xi);
+                    Fn[3 * idx] += lift * (1 - xi);
+                    Fn[3 * (idx + 1)] += lift * xi;
+                    Fn[3 * idx + 2] += mom * (1 - xi);
+                    Fn[3 * (idx + 1) + 2] += mom * xi;
+                }
+                const double be = bm->beta, ga = bm->gamma;
+                const double a0 = 1.0 / (be * dt * dt), a1 = 1.0 / (be * dt),
+                             a2 = 1.0 / (2.0 * be) - 1.0;
+                const double a3 = ga / (be * dt), a4 = ga / be - 1.0,
+                             a5 = dt * (ga / (2.0 * be) - 1.0);
+                double *dv = bm->d, *vv = bm->v, *av = bm->a;
+                for (int r = 0; r < NF; ++r) {
+                    double sm = 0.0, sc = 0.0;
+                    for (int c = 0; c < NF; ++c) {
+                        sm += bm->M[r * NF + c] * (a0 * dv[c] + a1 * vv[c] + a2 * av[c]);
+                        sc += bm->C[r * NF + c] * (a3 * dv[c] + a4 * vv[c] + a5 * av[c]);
+                    }
+                    Fe[r] = Fn[3 + r] + sm + sc;
+                }
+                for (int r = 0; r < NF; ++r) {
+                    double x = 0.0;
+                    for (int c = 0; c < NF; ++c) x += bm->Keff_inv[r * NF + c] * Fe[c];
+                    dn[r] = x;
+                    an[r] = a0 * (dn[r] - dv[r]) - a1 * vv[r] - a2 * av[r];
+                }
+                for (int r = 0; r < NF; ++r) {
+                    vv[r] = vv[r] + dt * ((1.0 - ga) * av[r] + ga * an[r]);
+                    av[r] = an[r];
+                    dv[r] = dn[r];
+                }
+                for (int k = 0; k < NN; ++k) {
+                    double w = k == 0 ? 0.0 : dv[3 * (k - 1)];
+                    double th = k == 0 ? 0.0 : dv[3 * (k - 1) + 2];
+                    if (has_wprev) {
+                        w = bm->relax * w + (1.0 - bm->relax) * wpv[k];
+                        th = bm->relax * th + (1.0 - bm->relax) * tpv[k];
+                    }
+                    wn[k] = w;
+                    tn[k] = th;
+                    wpv[k] = w;
+                    tpv[k] = th;
+                }
+                has_wprev = 1;
+                bm->tip_w[st] = wn[NN - 1];
+                bm->tip_theta[st] = tn[NN - 1];
+            } else if (bm) {
+                for (int k = 0; k < NN; ++k) {
+                    wn[k] = 0.0;
+                    tn[k] = 0.0;
+                }
+            }
         }
         memcpy(gprev, gam, sizeof(double) * P);
         /* wake shedding + conversion */
         if (st + 1 < d->num_steps) {
-            const double *Gn = d->grids + (size_t)(st + 1) * TV * 3;
+            const double *Gn;
+            if (bm) {
+                if (bm->feedback)
+                    deform_grid(d->grids, TV, bm, wn, tn, gb[2]);
+                else
+                    memcpy(gb[2], d->grids, sizeof(double) * 3 * TV);
+                Gn = gb[2];
+            } else {
+                Gn = d->grids + (size_t)(st + 1) * TV * 3;
+            }
             int64_t added = 0;
             for (int s = 0; s < S; ++s) {
                 int nc = sf[s].nc, ns = sf[s].ns;
@@ -376,10 +579,21 @@ int orc_hybrid_run(const orc_hybrid_desc *d, int64_t steps, double *cl, double *
             }
             np += added;
         }
-        if (np > 0) {
+        if (np > 0 && d->convect_exact) {
+            for (int64_t k = 0; k < np; ++k)
+                for (int c = 0; c < 3; ++c) ppos[3 * k + c] += V[c] * dt;
+        } else if (np > 0) {
             orc_vpm_advect_rk3(np, ppos, pal, psg, page, dt, V, d->stretch, d->free_wake,
                                d->kernel, d->vpm_nu, nthreads);
             orc_vpm_relax(np, ppos, pal, psg, d->rlxf, nthreads);
+        }
+        if (bm) {
+            /* this lattice becomes the previous one; the next deformation the current */
+            double *tg = gb[1];
+            gb[1] = gb[0];
+            gb[0] = tg;
+            memcpy(wc, wn, sizeof(double) * NN);
+            memcpy(tc, tn, sizeof(double) * NN);
         }
     }
     if (gamma_out) memcpy(gamma_out, gam, sizeof(double) * P);
@@ -395,9 +609,11 @@ done:
     }
     free(sf);
     free(br); free(fr); free(fl); free(bl); free(col); free(nrm); free(area); free(vmc);
-    free(lc); free(lv); free(vml); free(A); free(rhs); free(gam); free(gprev); free(rc0b);
+    free(lc); free(lv); free(vml); free(lf); free(A); free(rhs); free(gam); free(gprev); free(rc0b);
     free(vw); free(vp); free(vl); free(vl2); free(wbr); free(wfr); free(wfl); free(wbl);
     free(wg); free(wa); free(wrc); free(zage); free(ppos); free(pal); free(psg); free(page);
     free(tpos); free(tal); free(tsg);
+    free(pf); free(gb[0]); free(gb[1]); free(gb[2]); free(wc); free(tc); free(wn); free(tn);
+    free(wpv); free(tpv); free(Fn); free(Fe); free(dn); free(an);
     return rc;
 }

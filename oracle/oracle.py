@@ -257,6 +257,17 @@ class _HybridDesc(C.Structure):
         ("sigma", C.c_double), ("kernel", C.c_int32), ("free_wake", C.c_int32),
         ("stretch", C.c_int32), ("rlxf", C.c_double), ("vpm_nu", C.c_double),
         ("sref", C.c_double), ("max_particles", C.c_int64), ("precision", C.c_int32),
+        ("beam", C.c_void_p), ("convect_exact", C.c_int32),
+    ]
+
+
+class _OrcBeam(C.Structure):
+    _fields_ = [
+        ("n_nodes", C.c_int32), ("y_nodes", _dp), ("M", _dp), ("C", _dp), ("Keff_inv", _dp),
+        ("d", _dp), ("v", _dp), ("a", _dp), ("beta", C.c_double), ("gamma", C.c_double),
+        ("x_ea", C.c_double), ("relax", C.c_double), ("tip_w", _dp), ("tip_theta", _dp),
+        ("feedback", C.c_int32), ("lift_sign", C.c_double), ("station_lift", _dp),
+        ("station_moment", _dp),
     ]
 
 
@@ -302,6 +313,72 @@ def hybrid_run(case, steps=None, threads=0):
     k = npo.value
     return {"cl": cl, "cdi": cdi, "gamma": gam, "np": k, "pos": pos[:k].copy(),
             "gamma_p": al[:k].copy(), "sigma": sg[:k].copy()}
+
+
+def aeroelastic_run(inst, threads=0):
+    """One aeroelastic instance (fluxvortex_amd.aeroelastic.AeroInstance) on the
+    CPU oracle: the hybrid step on the deformed lattice, the beam coupling and
+    Newmark step, particles as a prescribed wake; returns cl, cdi, tip_w,
+    tip_theta, gamma and np (oracle/fv_oracle_uvlm.c)."""
+    from fluxvortex_amd.aeroelastic import beam_arrays
+    nc = np.array([inst.nc], dtype=np.int32)
+    ns = np.array([inst.ns], dtype=np.int32)
+    grid = _f(inst.grid)
+    if inst.beam is None:
+        # without a beam the oracle reads one lattice per step
+        grid = np.ascontiguousarray(np.broadcast_to(grid, (int(inst.num_steps),) + grid.shape))
+    d = _HybridDesc()
+    d.n_surfaces = 1
+    d.nc = nc.ctypes.data_as(C.POINTER(C.c_int32))
+    d.ns = ns.ctypes.data_as(C.POINTER(C.c_int32))
+    d.num_steps = int(inst.num_steps)
+    d.grids = _p(grid)
+    for k in range(3):
+        d.vinf[k] = float(inst.vinf[k])
+    d.rho, d.dt, d.nu = inst.rho, inst.dt, inst.nu
+    d.rc0_bound, d.rc0_wake = inst.rc0_bound, inst.rc0_wake
+    d.n_keep, d.sigma = int(inst.n_keep), inst.sigma
+    d.kernel = KERNELS[inst.kernel]
+    d.free_wake, d.stretch, d.rlxf, d.vpm_nu = 0, 0, 0.0, 0.0
+    d.sref, d.max_particles = inst.sref, int(inst.max_particles)
+    d.convect_exact = 1
+    n = int(inst.num_steps)
+    tw, tt = np.zeros(n), np.zeros(n)
+    sl, sm = np.zeros((n, inst.ns)), np.zeros((n, inst.ns))
+    b = None
+    if inst.beam is not None:
+        arr = beam_arrays(inst)
+        st = {k: arr[k].copy() for k in ("d0", "v0", "a0")}
+        b = _OrcBeam()
+        b.n_nodes = len(arr["y"])
+        b.y_nodes, b.M, b.C, b.Keff_inv = _p(arr["y"]), _p(arr["M"]), _p(arr["C"]), _p(arr["Kinv"])
+        b.d, b.v, b.a = _p(st["d0"]), _p(st["v0"]), _p(st["a0"])
+        b.beta, b.gamma = inst.beam.beta, inst.beam.gamma
+        b.x_ea, b.relax = inst.x_ea, inst.relax
+        b.tip_w, b.tip_theta = _p(tw), _p(tt)
+        b.feedback = 1 if inst.feedback else 0
+        b.lift_sign = float(inst.lift_sign)
+        b.station_lift, b.station_moment = _p(sl), _p(sm)
+        d.beam = C.cast(C.pointer(b), C.c_void_p)
+    else:
+        d.beam = None
+    P = inst.nc * inst.ns
+    cl, cdi, gam = np.zeros(n), np.zeros(n), np.zeros(P)
+    npo = C.c_int64()
+    cap = int(inst.max_particles)
+    pos, al, sg = np.zeros((cap, 3)), np.zeros((cap, 3)), np.zeros(cap)
+    fn = lib().orc_hybrid_run
+    fn.argtypes = [C.POINTER(_HybridDesc), C.c_int64, _dp, _dp, _dp, C.POINTER(C.c_int64),
+                   _dp, _dp, _dp, C.c_int]
+    rc = fn(C.byref(d), n, _p(cl), _p(cdi), _p(gam), C.byref(npo), _p(pos), _p(al), _p(sg),
+            threads)
+    if rc:
+        raise RuntimeError(f"orc_hybrid_run (aeroelastic) failed ({rc})")
+    if b is None:
+        for x in (tw, tt, sl, sm):
+            x[...] = np.nan
+    return {"cl": cl, "cdi": cdi, "tip_w": tw, "tip_theta": tt, "gamma": gam, "np": npo.value,
+            "station_lift": sl, "station_moment": sm}
 
 
 # ---------------------------------------------------------------------------

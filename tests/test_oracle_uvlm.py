@@ -39,13 +39,15 @@ def test_flat_plate_lift_slope(oracle):
 
 def test_impulsive_start_rises_to_steady(oracle):
     """After an impulsive start the circulatory lift grows as the starting
-    vortex leaves (Wagner): step 1 carries the added-mass term -rho A dGamma/dt
-    of the circulation jump, from step 2 on CL increases monotonically, starting
-    well below its final value and settling within 1 % over the last 10 steps."""
+    vortex leaves (Wagner).  Step 0 has no wake: the lattice is a set of closed
+    rings in a uniform stream and, with the trailing-edge line carrying its
+    whole circulation, feels no net force.  From step 1 on CL increases
+    monotonically, starting well below its final value and settling within 1 %
+    over the last 10 steps."""
     case = hybrid.flat_plate_case(nc=4, ns=8, chord=1.0, half_span=4.0, alpha_deg=3.0,
                                   symmetric=True, wake_chords=15.0)
     cl = oracle.hybrid_run(case)["cl"]
-    assert cl[1] > cl[0] and cl[1] > cl[2]
-    assert np.all(np.diff(cl[2:]) > 0.0)
-    assert cl[2] < 0.9 * cl[-1]
+    assert abs(cl[0]) < 1e-9 * abs(cl[-1])
+    assert np.all(np.diff(cl[1:]) > 0.0)
+    assert cl[1] < 0.75 * cl[-1]
     assert abs(cl[-10] - cl[-1]) < 0.01 * abs(cl[-1])
