@@ -164,25 +164,28 @@ def cpu_baseline(particles, threads):
     pos = rng.uniform([0, -5, -1], [200, 5, 1], (n, 3))
     al = rng.normal(0, 0.01, (n, 3))
     sg = np.full(n, 0.0628)
-    sub = min(n, 2048)
+    sub = n
     tgt = pos[:sub]
     oracle.particle_vel(tgt[:16], pos, al, sg, "winckelmans", threads)   # warm-up
-    t0 = time.perf_counter()
-    oracle.particle_vel(tgt, pos, al, sg, "winckelmans", threads)
-    t_vel = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    oracle.particle_jac(tgt, pos, al, sg, "winckelmans", threads)
-    t_jw = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    oracle.particle_jac(tgt, pos, al, sg, "gaussianerf", threads)
-    t_jg = time.perf_counter() - t0
+
+    def best(fn, *args, reps=3):
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn(*args)
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)), sum(ts)
+
+    t_vel, s1 = best(oracle.particle_vel, tgt, pos, al, sg, "winckelmans", threads)
+    t_jw, s2 = best(oracle.particle_jac, tgt, pos, al, sg, "winckelmans", threads)
+    t_jg, s3 = best(oracle.particle_jac, tgt, pos, al, sg, "gaussianerf", threads)
     scale = n / sub
     t_step = scale * (3 * t_vel + 3 * t_jw + t_jg)
     return {"value": 1.0 / t_step, "unit": "steps/s", "cores": threads, "kind": "port",
-            "sample": (f"oracle particle sums (3 Winckelmans velocity + 3 Winckelmans Jacobian "
-                       f"+ 1 Gaussian Jacobian per step) for {sub} of {n} targets x {n} sources, "
-                       f"scaled x{scale:.1f}; ring/UVLM part (<0.1%) omitted"),
-            "sample_seconds": t_vel + t_jw + t_jg}
+            "sample": (f"oracle particle sums of one step (3 Winckelmans velocity + 3 Winckelmans "
+                       f"Jacobian + 1 Gaussian Jacobian) for all {n} targets x {n} sources, each "
+                       f"sum the median of 3 runs; ring/UVLM part (<0.1%) omitted"),
+            "sample_seconds": s1 + s2 + s3}
 
 
 def roofline(fam, precision):
@@ -203,6 +206,55 @@ def roofline(fam, precision):
                            "FP32 pair terms with FP64 accumulation, priced at the FP64 peak")}
 
 
+def flutter_block(n_speeds=64, cpu_sample=4):
+    """The batched Goland flutter sweep: n_speeds airspeeds uniform over
+    80-180 m/s, 100 chords of wake, N_keep 10 (uneven step and particle
+    counts), every instance stepped together (one induction + one step launch
+    per time step, replayed from a hipGraph).  FP64 and mixed; the oracle is
+    timed on cpu_sample speeds and scaled to the sweep."""
+    from fluxvortex_amd import aeroelastic as ae
+    speeds = np.linspace(80.0, 180.0, n_speeds)
+    res = {"speeds": [80.0, 180.0, n_speeds], "n_keep": 10, "num_chords": 100}
+    for prec in ("fp64", "mixed"):
+        insts = [ae.goland_instance(float(V), precision=prec) for V in speeds]
+        sw = ae.Sweep(insts, use_graph=True)
+        sw.step(1)
+        sw.read(0)
+        t0 = time.perf_counter()
+        sw.step()
+        sw.read(0)
+        sec = time.perf_counter() - t0
+        sig = []
+        inst_steps = 0
+        for i, it in enumerate(insts):
+            r = sw.read(i)
+            if r["status"] != 0:
+                raise RuntimeError(f"flutter instance {i} failed ({r['status']})")
+            sig.append(ae.envelope_growth(r["tip_w"][~np.isnan(r["tip_w"])], it.dt))
+            inst_steps += it.num_steps - 1
+        sw.close()
+        res[prec] = {"seconds": sec, "time_steps": sw.max_steps - 1,
+                     "instance_steps_per_s": inst_steps / sec,
+                     "flutter_speed": ae.flutter_speed(speeds, sig),
+                     "sigma_w_80": sig[0], "sigma_w_180": sig[-1]}
+    f64, mx = res["fp64"]["flutter_speed"], res["mixed"]["flutter_speed"]
+    if f64 and mx:
+        res["mixed_flutter_speed_rel_err_vs_fp64"] = abs(mx - f64) / f64
+    try:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import oracle
+        pick = np.linspace(0, n_speeds - 1, cpu_sample).round().astype(int)
+        t0 = time.perf_counter()
+        for k in pick:
+            oracle.aeroelastic_run(ae.goland_instance(float(speeds[k])), threads=1)
+        per = (time.perf_counter() - t0) / cpu_sample
+        res["cpu_oracle"] = {"seconds_est": per * n_speeds, "threads": 1,
+                             "sample": f"{cpu_sample} of the {n_speeds} speeds, scaled"}
+    except Exception as e:   # the oracle is optional on a user's machine
+        res["cpu_oracle"] = {"error": str(e)}
+    return res
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -213,6 +265,7 @@ def main(argv=None):
                     help="second particle count: FP64 vs mixed vs mixed+learned (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-flutter", action="store_true")
     a = ap.parse_args(argv)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
@@ -282,6 +335,9 @@ def main(argv=None):
             big["mixed_learned_cl_max_rel_err_vs_fp64"] = float(
                 np.max(np.abs(rml["cl"] - r64["cl"])) / np.max(np.abs(r64["cl"])))
         out["heaving_100k"] = big
+    if not a.no_flutter:
+        log("[bench] batched Goland flutter sweep, 64 speeds")
+        out["flutter_sweep"] = flutter_block()
     if not a.no_cpu_baseline:
         threads = os.cpu_count() or 1
         log(f"[bench] CPU baseline (oracle, {threads} threads)")

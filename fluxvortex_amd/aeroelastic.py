@@ -402,3 +402,34 @@ def flutter_sweep(speeds, use_graph=True, **kw):
     sw.close()
     Vf = flutter_speed([r["V"] for r in out], [r["sigma_w"] for r in out])
     return {"runs": out, "flutter_speed": Vf}
+
+
+def main(argv=None):
+    """The Goland flutter sweep from the command line (benchmark_goland.py's
+    run): prints sigma_w per speed, the flutter speed and the sweep time."""
+    import argparse
+    import json
+    import time
+
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--speeds", type=int, default=64)
+    ap.add_argument("--vmin", type=float, default=80.0)
+    ap.add_argument("--vmax", type=float, default=180.0)
+    ap.add_argument("--num-chords", type=float, default=100.0)
+    ap.add_argument("--n-keep", type=int, default=10)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--feedback", action="store_true")
+    ap.add_argument("--eager", action="store_true", help="launch without the hipGraph")
+    a = ap.parse_args(argv)
+    speeds = np.linspace(a.vmin, a.vmax, a.speeds)
+    t0 = time.perf_counter()
+    res = flutter_sweep(speeds, use_graph=not a.eager, num_chords=a.num_chords, n_keep=a.n_keep,
+                        precision=a.precision, feedback=a.feedback)
+    sec = time.perf_counter() - t0
+    print(json.dumps({"flutter_speed": res["flutter_speed"], "seconds": sec,
+                      "sigma_w": {f"{r['V']:.3f}": round(r["sigma_w"], 6) for r in res["runs"]}}))
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -41,9 +41,10 @@ namespace fv {
 constexpr int kSwMaxNc = 16, kSwMaxNs = 32, kSwMaxP = 64, kSwMaxR = 16, kSwMaxNodes = 17;
 constexpr int kSwMaxFree = 3 * (kSwMaxNodes - 1);
 constexpr int kSwMaxL = 3 * kSwMaxP + kSwMaxNs;   // right / front / left legs + trailing-edge back legs
-constexpr int kSwBlock = 256;
+constexpr int kSwBlock = 256;       // induction workgroup
+constexpr int kSwStepBlock = 1024;  // step workgroup: 16 waves, one instance per CU
 constexpr int kSwChunk = 1024;   // particles per induction partial
-constexpr int kSwParts = 4;      // segment partial sums per target
+constexpr int kSwParts = 8;      // segment partial sums per target
 
 struct SwInst {
     int nc, ns, P, nv, R, n_keep, kernel, couple, n_nodes, nfree;
@@ -260,7 +261,7 @@ constexpr int kL_beam = kL_wa + kSwMaxR * kSwMaxNs;       // nodal forces / vect
 constexpr int kL_end = kL_beam + 3 * kSwMaxNodes + 4 * kSwMaxFree + 4 * kSwMaxNodes + 8;
 constexpr size_t kSwLdsBytes = sizeof(double) * kL_end;
 
-__global__ __launch_bounds__(kSwBlock) void k_sw_step(SwInst *__restrict__ insts,
+__global__ __launch_bounds__(kSwStepBlock) void k_sw_step(SwInst *__restrict__ insts,
                                                       double *__restrict__ S)
 {
     extern __shared__ __attribute__((aligned(16))) double L[];
@@ -288,9 +289,9 @@ __global__ __launch_bounds__(kSwBlock) void k_sw_step(SwInst *__restrict__ insts
     const int lda = P | 1;
     const int NL = 3 * P + ns;   // leg slots: 3 p + l, then 3 P + j for the back leg of (nc-1, j)
     // ---- lattice of this step
-    for (int v = t; v < nv; v += kSwBlock) sw_deform(grid0 + 3 * v, yn, wdef, tdef, nn, I.x_ea, g + 3 * v);
+    for (int v = t; v < nv; v += kSwStepBlock) sw_deform(grid0 + 3 * v, yn, wdef, tdef, nn, I.x_ea, g + 3 * v);
     __syncthreads();
-    for (int p = t; p < P; p += kSwBlock) {
+    for (int p = t; p < P; p += kSwStepBlock) {
         const int i = p / ns, j = p % ns;
         double *Br = ring + 12 * p, *Fr = Br + 3, *Fl = Br + 6, *Bl = Br + 9;
         sw_ring(g, nc, ns, i, j, Br, Fr, Fl, Bl);
@@ -313,14 +314,14 @@ __global__ __launch_bounds__(kSwBlock) void k_sw_step(SwInst *__restrict__ insts
     }
     // wake of the current step (rows 0 .. nrow-1, vertices 0 .. nrow)
     const int nrow = I.nrow, cur = I.par;
-    for (int e = t; e < 3 * (nrow + 1) * (ns + 1); e += kSwBlock) wv[e] = S[I.o_wv[cur] + e];
-    for (int e = t; e < nrow * ns; e += kSwBlock) {
+    for (int e = t; e < 3 * (nrow + 1) * (ns + 1); e += kSwStepBlock) wv[e] = S[I.o_wv[cur] + e];
+    for (int e = t; e < nrow * ns; e += kSwStepBlock) {
         wg[e] = S[I.o_wg[cur] + e];
         wa[e] = S[I.o_wa[cur] + e];
     }
     __syncthreads();
     // ---- AIC: unit-strength rings (bound core, age 0) . normals
-    for (int e = t; e < P * P; e += kSwBlock) {
+    for (int e = t; e < P * P; e += kSwStepBlock) {
         const int i = e / P, j = e % P;
         const double *R = ring + 12 * j;
         double u[3] = {0.0, 0.0, 0.0};
@@ -332,7 +333,7 @@ __global__ __launch_bounds__(kSwBlock) void k_sw_step(SwInst *__restrict__ insts
     }
     // ---- wake-ring velocity at the collocation points (kSwParts partial sums)
     const int W = nrow * ns;
-    for (int e = t; e < P * kSwParts; e += kSwBlock) {
+    for (int e = t; e < P * kSwParts; e += kSwStepBlock) {
         const int p = e / kSwParts, q = e % kSwParts;
         double u[3] = {0.0, 0.0, 0.0};
         for (int k = q; k < 4 * W; k += kSwParts) {
@@ -348,7 +349,7 @@ __global__ __launch_bounds__(kSwBlock) void k_sw_step(SwInst *__restrict__ insts
     }
     __syncthreads();
     const int nchunk = (int)((I.np + kSwChunk - 1) / kSwChunk);
-    for (int p = t; p < P; p += kSwBlock) {
+    for (int p = t; p < P; p += kSwStepBlock) {
         double w3[3], q3[3] = {0.0, 0.0, 0.0};
         for (int c = 0; c < 3; ++c) {
             double a = 0.0;
@@ -373,13 +374,13 @@ __global__ __launch_bounds__(kSwBlock) void k_sw_step(SwInst *__restrict__ insts
         }
         return;
     }
-    for (int i = t; i < P; i += kSwBlock) rdg[i] = 1.0 / A[i * lda + i];
+    for (int i = t; i < P; i += kSwStepBlock) rdg[i] = 1.0 / A[i * lda + i];
     __syncthreads();
     wave_back_subst<double, double>(A, lda, P, rhs, rdg);
     __syncthreads();
     const double *gam = rhs;
     // ---- velocities at the leg centres from bound (gamma) + wake rings
-    for (int e = t; e < NL * kSwParts; e += kSwBlock) {
+    for (int e = t; e < NL * kSwParts; e += kSwStepBlock) {
         const int tq = e / kSwParts, q = e % kSwParts;
         const double *x = lc + 3 * tq;
         double u[3] = {0.0, 0.0, 0.0};
@@ -401,7 +402,7 @@ __global__ __launch_bounds__(kSwBlock) void k_sw_step(SwInst *__restrict__ insts
         for (int c = 0; c < 3; ++c) red[(tq * kSwParts + q) * 3 + c] = u[c];
     }
     __syncthreads();
-    for (int tq = t; tq < NL; tq += kSwBlock)
+    for (int tq = t; tq < NL; tq += kSwStepBlock)
         for (int c = 0; c < 3; ++c) {
             double a = 0.0;
             for (int q = 0; q < kSwParts; ++q) a += red[(tq * kSwParts + q) * 3 + c];
@@ -413,7 +414,7 @@ __global__ __launch_bounds__(kSwBlock) void k_sw_step(SwInst *__restrict__ insts
     // ahead, the trailing-edge back leg against the newest wake ring) + the
     // unsteady term -rho A dGamma/dt n
     const double *gprev = S + I.o_gamp;
-    for (int p = t; p < P; p += kSwBlock) {
+    for (int p = t; p < P; p += kSwStepBlock) {
         const int i = p / ns, j = p % ns;
         const double G = gam[p];
         const double gw = nrow > 0 ? wg[j] : 0.0;
@@ -498,7 +499,7 @@ __global__ __launch_bounds__(kSwBlock) void k_sw_step(SwInst *__restrict__ insts
         const double *M = S + I.o_M, *C = S + I.o_C, *K = S + I.o_K;
         double *d = S + I.o_d, *v = S + I.o_v, *a = S + I.o_a;
         // F_eff = F + M (a0 d + a1 v + a2 a) + C (a3 d + a4 v + a5 a)
-        for (int r = t; r < nf; r += kSwBlock) {
+        for (int r = t; r < nf; r += kSwStepBlock) {
             double sm = 0.0, sc = 0.0;
             for (int c = 0; c < nf; ++c) {
                 sm += M[r * nf + c] * (a0 * d[c] + a1 * v[c] + a2 * a[c]);
@@ -507,14 +508,14 @@ __global__ __launch_bounds__(kSwBlock) void k_sw_step(SwInst *__restrict__ insts
             Fe[r] = Fn[3 + r] + sm + sc;
         }
         __syncthreads();
-        for (int r = t; r < nf; r += kSwBlock) {
+        for (int r = t; r < nf; r += kSwStepBlock) {
             double x = 0.0;
             for (int c = 0; c < nf; ++c) x += K[r * nf + c] * Fe[c];
             dn[r] = x;
             an[r] = a0 * (dn[r] - d[r]) - a1 * v[r] - a2 * a[r];
         }
         __syncthreads();
-        for (int r = t; r < nf; r += kSwBlock) {
+        for (int r = t; r < nf; r += kSwStepBlock) {
             v[r] = v[r] + dt * ((1.0 - ga) * a[r] + ga * an[r]);
             a[r] = an[r];
             d[r] = dn[r];
@@ -559,7 +560,7 @@ __global__ __launch_bounds__(kSwBlock) void k_sw_step(SwInst *__restrict__ insts
         const double ox = V[0] * dt, oy = V[1] * dt, oz = V[2] * dt;
         // vertex rows: 0 = trailing edge of the next lattice; 1 = current trailing
         // edge + V dt; r + 1 = old row r + V dt
-        for (int e = t; e < (nrow_new + 1) * (ns + 1); e += kSwBlock) {
+        for (int e = t; e < (nrow_new + 1) * (ns + 1); e += kSwStepBlock) {
             const int r = e / (ns + 1), j = e % (ns + 1);
             double o[3];
             if (r == 0) {
@@ -581,7 +582,7 @@ __global__ __launch_bounds__(kSwBlock) void k_sw_step(SwInst *__restrict__ insts
             }
             for (int c = 0; c < 3; ++c) wvn[3 * e + c] = o[c];
         }
-        for (int e = t; e < nrow_new * ns; e += kSwBlock) {
+        for (int e = t; e < nrow_new * ns; e += kSwStepBlock) {
             const int r = e / ns, j = e % ns;
             if (r == 0) {
                 wgn[e] = gam[(nc - 1) * ns + j];
@@ -632,22 +633,22 @@ __global__ __launch_bounds__(kSwBlock) void k_sw_step(SwInst *__restrict__ insts
         }
         // particles: prescribed wake, x += V dt
         double *pos = S + I.o_pos;
-        for (int64_t k = t; k < I.np; k += kSwBlock) {
+        for (int64_t k = t; k < I.np; k += kSwStepBlock) {
             pos[3 * k] += ox;
             pos[3 * k + 1] += oy;
             pos[3 * k + 2] += oz;
         }
         // next lattice: deformation and collocation points for the next induction
         __syncthreads();
-        for (int k = t; k < nn; k += kSwBlock) {
+        for (int k = t; k < nn; k += kSwStepBlock) {
             wdef[k] = wnew[k];
             tdef[k] = tnew[k];
         }
         double *gn = g;   // the current lattice is no longer needed
         __syncthreads();
-        for (int v = t; v < nv; v += kSwBlock) sw_deform(grid0 + 3 * v, yn, wnew, tnew, nn, I.x_ea, gn + 3 * v);
+        for (int v = t; v < nv; v += kSwStepBlock) sw_deform(grid0 + 3 * v, yn, wnew, tnew, nn, I.x_ea, gn + 3 * v);
         __syncthreads();
-        for (int p = t; p < P; p += kSwBlock) {
+        for (int p = t; p < P; p += kSwStepBlock) {
             double cn[3], nn3[3], ar;
             sw_frame(gn, ns, p / ns, p % ns, cn, nn3, &ar);
             for (int c = 0; c < 3; ++c) S[I.o_coln + 3 * p + c] = cn[c];
@@ -658,7 +659,7 @@ __global__ __launch_bounds__(kSwBlock) void k_sw_step(SwInst *__restrict__ insts
         }
     }
     // ---- carry this step's state
-    for (int p = t; p < P; p += kSwBlock) {
+    for (int p = t; p < P; p += kSwStepBlock) {
         for (int c = 0; c < 3; ++c) S[I.o_colp + 3 * p + c] = col[3 * p + c];
         for (int c = 0; c < 9; ++c) S[I.o_legp + 9 * p + c] = lc[9 * p + c];
         if (p / ns == nc - 1)
@@ -714,7 +715,7 @@ static int sweep_launch(fv_sweep *sw)
     hipLaunchKernelGGL(k_sw_induce, dim3((unsigned)sw->max_chunks, (unsigned)sw->n),
                        dim3(kSwBlock), 0, ctx->stream, sw->dinst, sw->block);
     FV_LAUNCH_CHECK(ctx);
-    hipLaunchKernelGGL(k_sw_step, dim3((unsigned)sw->n), dim3(kSwBlock), kSwLdsBytes, ctx->stream,
+    hipLaunchKernelGGL(k_sw_step, dim3((unsigned)sw->n), dim3(kSwStepBlock), kSwLdsBytes, ctx->stream,
                        sw->dinst, sw->block);
     FV_LAUNCH_CHECK(ctx);
     return FV_OK;
