@@ -116,12 +116,20 @@ _SIGNATURES = {
     "fv_vpm_advect_rk3": (C.c_int, [_vp, C.c_double, _dp, C.c_int32, C.c_int32]),
     "fv_convert_rings_host": (C.c_int, [_vp, C.c_int64, _dp, _dp, _dp, _dp, _dp, C.c_double,
                                         _i64p]),
+    "fv_vpm_drop_oldest": (C.c_int, [_vp, C.c_int64]),
+    "fv_vpm_merge_host": (C.c_int, [_vp, _dp, C.c_int32, C.c_double, C.c_double, C.c_int64,
+                                    C.c_double, _i64p]),
+    "fv_vpm_merge_stats": (C.c_int, [_vp, _i64p, _i64p]),
+    "fv_vpm_ledger_host": (C.c_int, [_vp, _dp]),
     "fv_hybrid_create": (C.c_int, [_ctx, C.POINTER(HybridDesc), C.POINTER(_vp)]),
     "fv_hybrid_destroy": (None, [_vp]),
     "fv_hybrid_step": (C.c_int, [_vp, C.c_int64]),
     "fv_hybrid_current_step": (C.c_int, [_vp, _i64p]),
     "fv_hybrid_read": (C.c_int, [_vp, _dp, _dp, _dp]),
     "fv_hybrid_particles": (_vp, [_vp]),
+    "fv_hybrid_set_population": (C.c_int, [_vp, C.c_int32, C.c_int64, C.c_double, C.c_int64,
+                                           C.c_double]),
+    "fv_hybrid_ledger_host": (C.c_int, [_vp, _dp]),
     "fv_hybrid_pair_counts": (C.c_int, [_vp, _dp, _dp]),
     "fv_hybrid_set_flags": (C.c_int, [_vp, C.c_int32, C.c_int32]),
     "fv_hybrid_set_precision": (C.c_int, [_vp, C.c_int32]),

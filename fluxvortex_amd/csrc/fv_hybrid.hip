@@ -67,6 +67,10 @@ struct fv_hybrid {
     int64_t *d_poff = nullptr;                         // per-surface panel offsets (device)
     int64_t *d_woff = nullptr;                         // per-surface wake ring offsets (device)
     fv_vpm *vpm = nullptr;
+    // population control after each advect (0 none, 1 drop oldest, 2 merge)
+    int pop_scheme = 0;
+    int64_t pop_max = 0, pop_protect = 64;
+    double pop_eps = 1e-3, pop_protect_dist = 0.0;
     double seg_pairs = 0.0;
 };
 
@@ -512,6 +516,16 @@ static int hybrid_step_one(fv_hybrid *h)
         rc = vpm_advect(h->vpm, dt, d.vinf, d.stretch, d.free_wake);
         if (rc) return rc;
     }
+    // population control (newton_pc/adapters/flap.py:466-475)
+    if (h->pop_scheme == 1 && h->vpm->np_upper > h->pop_max) {
+        rc = vpm_drop_oldest(h->vpm, h->pop_max);
+        if (rc) return rc;
+    } else if (h->pop_scheme == 2 && h->vpm->np_upper >= 2) {
+        const double vn = sqrt(d.vinf[0] * d.vinf[0] + d.vinf[1] * d.vinf[1] + d.vinf[2] * d.vinf[2]);
+        rc = vpm_merge_colloc(h->vpm, h->colloc, P, h->pop_eps, vn, h->pop_protect,
+                              h->pop_protect_dist);
+        if (rc) return rc;
+    }
     h->step = s + 1;
     return FV_OK;
 }
@@ -736,6 +750,47 @@ int fv_hybrid_set_flags(fv_hybrid *h, int32_t free_wake, int32_t stretch)
     if (!h) return FV_E_ARG;
     h->d.free_wake = free_wake ? 1 : 0;
     h->d.stretch = stretch ? 1 : 0;
+    return FV_OK;
+}
+
+int fv_hybrid_set_population(fv_hybrid *h, int32_t scheme, int64_t max_particles, double eps,
+                             int64_t protect, double protect_dist)
+{
+    if (!h) return FV_E_ARG;
+    FV_REQUIRE(h->ctx, scheme >= 0 && scheme <= 2, "unknown population scheme %d", scheme);
+    FV_REQUIRE(h->ctx, max_particles >= 0 && eps >= 0.0 && protect >= 0 && protect_dist >= 0.0,
+               "negative population parameter");
+    h->pop_scheme = scheme;
+    h->pop_max = max_particles;
+    h->pop_eps = eps;
+    h->pop_protect = protect;
+    h->pop_protect_dist = protect_dist;
+    return FV_OK;
+}
+
+int fv_hybrid_ledger_host(fv_hybrid *h, double *out3)
+{
+    if (!h || !out3) return FV_E_ARG;
+    fv_ctx *ctx = h->ctx;
+    // the stored wake rings as corners, then particles + rings
+    int64_t W = 0;
+    const int cur = h->par;
+    for (auto &sf : h->surf) {
+        if (h->nrow > 0) {
+            hipLaunchKernelGGL(k_wake_corners, dim3((unsigned)ceil_div(h->nrow * sf.ns, 128)),
+                               dim3(128), 0, ctx->stream, h->wv[cur] + 3 * sf.wvert_off,
+                               h->wg[cur] + sf.wring_off, h->wa[cur] + sf.wring_off, sf.ns,
+                               (int64_t)0, h->nrow, W, h->rbr, h->rfr, h->rfl, h->rbl, h->rg,
+                               h->rage);
+            FV_LAUNCH_CHECK(ctx);
+        }
+        W += h->nrow * sf.ns;
+    }
+    int rc = vpm_ledger(h->vpm, W, h->rbr, h->rfr, h->rfl, h->rbl, h->rg, h->vw);
+    if (rc) return rc;
+    FV_HIP_CHECK(ctx, hipMemcpyAsync(out3, h->vw, 3 * sizeof(double), hipMemcpyDeviceToHost,
+                                     ctx->stream));
+    FV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     return FV_OK;
 }
 

@@ -473,6 +473,7 @@ void fv_vpm_destroy(fv_vpm *f)
     closure_destroy(f->closure);
     f->closure = nullptr;
     (void)hipStreamSynchronize(f->ctx->stream);
+    pop_destroy(f);
     (void)hipFree(f->block);
     delete f;
 }

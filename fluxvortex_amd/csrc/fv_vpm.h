@@ -30,6 +30,7 @@ struct fv_vpm {
     double particle_pairs = 0.0;   // pairs evaluated, from the host-side upper bound
     int64_t state_epoch = 0;       // bumped by every enqueued change of the particle state
     fv_closure *closure = nullptr; // far-field closure (optional)
+    void *pop = nullptr;           // population-control scratch (fv_population.hip), lazily made
 };
 
 namespace fv {
@@ -48,6 +49,19 @@ int vpm_induce(fv_vpm *f, const double *targets, int64_t n, int precision, doubl
 
 // Makes np_host exact (synchronises).
 int vpm_sync_count(fv_vpm *f);
+
+// Population control and the circulation ledger (fv_population.hip).
+int vpm_drop_oldest(fv_vpm *f, int64_t max_particles);
+int vpm_merge(fv_vpm *f, const double *ref_dev, int nref, double eps, double vinf_norm,
+              int64_t protect, double protect_dist);
+int vpm_merge_stats(fv_vpm *f, int64_t *merged, int64_t *out_of_range);
+// Merge with the hybrid driver's reference points (9 strided collocation
+// points, flap.py:426).
+int vpm_merge_colloc(fv_vpm *f, const double *colloc, int64_t P, double eps, double vinf_norm,
+                     int64_t protect, double protect_dist);
+int vpm_ledger(fv_vpm *f, int64_t nr, const double *br, const double *fr, const double *fl,
+               const double *bl, const double *g, double *out3);
+void pop_destroy(fv_vpm *f);
 
 // Far-field closure hooks (fv_closure.hip).
 // Per-step bookkeeping at the start of an advect: online training and the

@@ -104,7 +104,7 @@ class HybridRun:
         vh = self._ctx.lib.fv_hybrid_particles(self._h)
         self.particles = VortexParticleField(max_particles=case.max_particles, rlxf=case.rlxf,
                                              kernel=case.kernel, precision=case.precision,
-                                             context=self._ctx, _handle=vh)
+                                             context=self._ctx, _handle=vh, _owner=self)
 
     def step(self, n: int = 1) -> None:
         rc = self._ctx.lib.fv_hybrid_step(self._h, int(n))
@@ -135,6 +135,23 @@ class HybridRun:
     def set_precision(self, precision: str) -> None:
         self._ctx.check(self._ctx.lib.fv_hybrid_set_precision(self._h,
                                                               _lib.PRECISIONS[precision]))
+
+    def set_population(self, scheme: str = "none", max_particles: int = 60000, eps: float = 1e-3,
+                       protect: int = 64, protect_dist: float = 0.0) -> None:
+        """Particle population control after every advect
+        (newton_pc/adapters/flap.py:466-475): "none", "drop" (keep the newest
+        max_particles) or "merge" (moment-conserving pairs, error eps |V_inf|
+        at 9 collocation points)."""
+        code = {"none": 0, "drop": 1, "merge": 2}[scheme]
+        self._ctx.check(self._ctx.lib.fv_hybrid_set_population(
+            self._h, code, int(max_particles), float(eps), int(protect), float(protect_dist)))
+
+    def circulation_ledger(self):
+        """Total vector circulation of particles + stored wake rings
+        (flap.py:565-578)."""
+        out = np.zeros(3)
+        self._ctx.check(self._ctx.lib.fv_hybrid_ledger_host(self._h, _lib.ptr(out)))
+        return out
 
     def pair_counts(self):
         a, b = C.c_double(), C.c_double()

@@ -19,7 +19,7 @@ class VortexParticleField:
     """Structure-of-arrays vortex particle field on the MI355X."""
 
     def __init__(self, max_particles=50000, nu=0.0, rlxf=0.3, kernel="gaussianerf",
-                 precision="fp64", context=None, _handle=None):
+                 precision="fp64", context=None, _handle=None, _owner=None):
         self.max_particles = int(max_particles)
         self.nu = float(nu)
         self.rlxf = float(rlxf)
@@ -27,6 +27,8 @@ class VortexParticleField:
         self.precision = precision
         self._ctx = context or _lib.default_context()
         self._owned = _handle is None
+        # a borrowed field (a hybrid case's) keeps its owner alive while in use
+        self._owner = _owner
         if _handle is None:
             h = C.c_void_p()
             self._ctx.check(self._ctx.lib.fv_vpm_create(
@@ -196,6 +198,31 @@ class VortexParticleField:
 
     def closure_train(self, rounds=1):
         self._ctx.check(self._ctx.lib.fv_vpm_closure_train(self._h, int(rounds)))
+
+    # -- population control and the circulation ledger (newton_pc/adapters/flap.py:466-578)
+    def drop_oldest(self, max_particles):
+        """Keeps the newest max_particles particles (flap.py:469-473)."""
+        self._ctx.check(self._ctx.lib.fv_vpm_drop_oldest(self._h, int(max_particles)))
+
+    def merge(self, ref_points, eps=1e-3, vinf_norm=1.0, protect=64, protect_dist=0.0):
+        """Pairwise moment-conserving merge (flap.py:478-563): aligned particle
+        pairs in one distance-graded hash cell merge when the velocity change
+        at ref_points stays below eps * vinf_norm; the newest `protect`
+        particles (or, with protect_dist > 0, those within protect_dist of the
+        reference centre) never merge.  Returns the number of merged pairs."""
+        ref = _lib.f64(ref_points, (3,), "ref_points")
+        n = C.c_int64()
+        self._ctx.check(self._ctx.lib.fv_vpm_merge_host(self._h, _lib.ptr(ref), ref.shape[0],
+                                                        float(eps), float(vinf_norm),
+                                                        int(protect), float(protect_dist),
+                                                        C.byref(n)))
+        return n.value
+
+    def circulation_ledger(self):
+        """Total vector circulation sum(alpha) of the particles (flap.py:565-578)."""
+        out = np.zeros(3)
+        self._ctx.check(self._ctx.lib.fv_vpm_ledger_host(self._h, _lib.ptr(out)))
+        return out
 
     def convert_rings(self, br, fr, fl, bl, strengths, sigma):
         """Appends the leg particles of rings with |Gamma| >= 1e-15 and returns the

@@ -184,6 +184,25 @@ int fv_convert_rings_host(fv_vpm *f, int64_t nr, const double *br, const double 
                           const double *fl, const double *bl, double *gamma, double sigma,
                           int64_t *added);
 
+/* Particle population control (newton_pc/adapters/flap.py:466-563).
+ * fv_vpm_drop_oldest: beyond max_particles the oldest particles are removed
+ * (flap.py:469-473).  fv_vpm_merge_host: pairwise moment-conserving merge
+ * (flap.py:478-563): the oldest np - protect particles (or, with
+ * protect_dist > 0, those farther than protect_dist from the centre of the
+ * nref <= 64 reference points, host xyz) are hashed into cells of size
+ * 0.1 + 0.15 d (d: distance to that centre); consecutive aligned pairs of one
+ * cell merge into (a1 + a2, |a|-weighted centroid, max sigma) when the
+ * velocity change at the reference points stays below eps * vinf_norm;
+ * n_merged (may be NULL: no synchronisation) receives the pairs merged by
+ * this call.  Both run on the stream without host synchronisation. */
+int fv_vpm_drop_oldest(fv_vpm *f, int64_t max_particles);
+int fv_vpm_merge_host(fv_vpm *f, const double *ref, int32_t nref, double eps, double vinf_norm,
+                      int64_t protect, double protect_dist, int64_t *n_merged);
+/* Cumulative merged pairs and candidates skipped for |cell index| >= 2^20. */
+int fv_vpm_merge_stats(fv_vpm *f, int64_t *merged, int64_t *out_of_range);
+/* Total vector circulation sum(alpha) of the particles (flap.py:565-578). */
+int fv_vpm_ledger_host(fv_vpm *f, double *out3);
+
 /* -------------------------------------------------- hybrid step (a12-a14) */
 /* One lattice per lifting surface; grids hold the panel corner vertices of
  * every surface for every time step:
@@ -231,6 +250,15 @@ int fv_hybrid_pair_counts(fv_hybrid *h, double *seg_pairs, double *particle_pair
  * (HybridSolver(free_vpm=...) and advect_rk3(stretch=...),
  * experiment_hybrid_panel_particle.py:107,140-143). */
 int fv_hybrid_set_flags(fv_hybrid *h, int32_t free_wake, int32_t stretch);
+/* Population control after every particle advect (newton_pc/adapters/
+ * flap.py:466-475): scheme 0 none, 1 keep the newest max_particles, 2 merge
+ * (fv_vpm_merge_host with the step's collocation points 0, s, 2s, ... (s =
+ * max(1, P / 9), at most 9, flap.py:426) as reference points and |V_inf|). */
+int fv_hybrid_set_population(fv_hybrid *h, int32_t scheme, int64_t max_particles, double eps,
+                             int64_t protect, double protect_dist);
+/* Total vector circulation: particle alpha + stored wake rings' G (b - a)
+ * over their legs (flap.py:565-578). */
+int fv_hybrid_ledger_host(fv_hybrid *h, double *out3);
 int fv_hybrid_set_precision(fv_hybrid *h, int32_t precision);
 
 /* ------------------------------------------ learned far-field closure MLP */

@@ -28,5 +28,12 @@ int orc_vpm_relax(int64_t n, const double *pos, double *gamma, const double *sig
 int64_t orc_convert_rings(int64_t nr, const double *Br, const double *Fr, const double *Fl,
                           const double *Bl, double *gamma, double sigma, double *pos_out,
                           double *alpha_out, double *sig_out);
+int64_t orc_pop_drop(int64_t np, int64_t max_particles, double *pos, double *alpha,
+                     double *sigma, double *age);
+int64_t orc_pop_merge(int64_t np, double *pos, double *alpha, double *sigma, double *age,
+                      const double *ref, int64_t nref, double eps, double vinf_norm,
+                      int64_t protect, double protect_dist, int64_t *n_merged);
+void orc_ledger(int64_t np, const double *alpha, int64_t nr, const double *Br, const double *Fr,
+                const double *Fl, const double *Bl, const double *gamma, double *out);
 
 #endif

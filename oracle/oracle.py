@@ -148,6 +148,50 @@ def convert_rings(br, fr, fl, bl, gamma, sigma):
     return pos[:k], al[:k], sg[:k], g
 
 
+def pop_drop(pos, alpha, sigma, age, max_particles):
+    """Keeps the newest max_particles (flap.py:469-473); returns copies."""
+    pos, alpha, sigma, age = (_f(x).copy() for x in (pos, alpha, sigma, age))
+    fn = lib().orc_pop_drop
+    fn.restype = C.c_int64
+    fn.argtypes = [C.c_int64, C.c_int64, _dp, _dp, _dp, _dp]
+    n = fn(len(sigma), int(max_particles), _p(pos), _p(alpha), _p(sigma), _p(age))
+    return pos[:n], alpha[:n], sigma[:n], age[:n]
+
+
+def pop_merge(pos, alpha, sigma, age, ref, eps, vinf_norm, protect=64, protect_dist=0.0):
+    """Pairwise moment-conserving merge (flap.py:478-563); returns copies and
+    the number of merged pairs."""
+    pos, alpha, sigma, age = (_f(x).copy() for x in (pos, alpha, sigma, age))
+    ref = _f(ref)
+    fn = lib().orc_pop_merge
+    fn.restype = C.c_int64
+    fn.argtypes = [C.c_int64, _dp, _dp, _dp, _dp, _dp, C.c_int64, C.c_double, C.c_double,
+                   C.c_int64, C.c_double, C.POINTER(C.c_int64)]
+    nm = C.c_int64()
+    n = fn(len(sigma), _p(pos), _p(alpha), _p(sigma), _p(age), _p(ref), ref.shape[0], eps,
+           vinf_norm, int(protect), float(protect_dist), C.byref(nm))
+    if n < 0:
+        raise MemoryError("orc_pop_merge")
+    return pos[:n], alpha[:n], sigma[:n], age[:n], nm.value
+
+
+def ledger(alpha, br=None, fr=None, fl=None, bl=None, gamma=None):
+    """Total vector circulation: sum of particle alpha + sum over ring legs of
+    G (b - a) (flap.py:565-578)."""
+    alpha = _f(alpha).reshape(-1, 3)
+    out = np.zeros(3)
+    fn = lib().orc_ledger
+    fn.argtypes = [C.c_int64, _dp, C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp]
+    if gamma is None:
+        z = np.zeros((0, 3))
+        fn(alpha.shape[0], _p(alpha), 0, _p(z), _p(z), _p(z), _p(z), _p(np.zeros(0)), _p(out))
+    else:
+        c = [_f(x) for x in (br, fr, fl, bl)]
+        g = _f(gamma)
+        fn(alpha.shape[0], _p(alpha), g.shape[0], *map(_p, c), _p(g), _p(out))
+    return out
+
+
 # ---------------------------------------------------------------------------
 # NumPy restatements (small inputs only)
 # ---------------------------------------------------------------------------
