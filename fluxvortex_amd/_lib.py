@@ -130,6 +130,7 @@ _SIGNATURES = {
     "fv_hybrid_set_population": (C.c_int, [_vp, C.c_int32, C.c_int64, C.c_double, C.c_int64,
                                            C.c_double]),
     "fv_hybrid_ledger_host": (C.c_int, [_vp, _dp]),
+    "fv_hybrid_set_ring_wake": (C.c_int, [_vp, C.c_int32, C.c_double]),
     "fv_hybrid_pair_counts": (C.c_int, [_vp, _dp, _dp]),
     "fv_hybrid_set_flags": (C.c_int, [_vp, C.c_int32, C.c_int32]),
     "fv_hybrid_set_precision": (C.c_int, [_vp, C.c_int32]),

@@ -69,6 +69,11 @@ struct fv_hybrid {
     fv_vpm *vpm = nullptr;
     // population control after each advect (0 none, 1 drop oldest, 2 merge)
     int pop_scheme = 0;
+    // free ring wake: vertices move with V + the bound / wake-ring induction
+    // (frozen snapshot, forward Euler); rings beyond trunc_x get gamma = 0
+    int ring_free = 0;
+    double trunc_x = INFINITY;
+    double *wpts = nullptr, *wvel = nullptr;   // 3 WV each
     int64_t pop_max = 0, pop_protect = 64;
     double pop_eps = 1e-3, pop_protect_dist = 0.0;
     double seg_pairs = 0.0;
@@ -294,13 +299,31 @@ __global__ void k_copy(const double *__restrict__ a, double *__restrict__ b, int
     if (i < n) b[i] = a[i];
 }
 
+// Old wake vertex rows 0 .. nr-1 of one surface as points: row 0 = trailing
+// edge of the current geometry, row r = old row r.
+__global__ void k_wake_points(const double *__restrict__ gcur, int nc, int ns,
+                              const double *__restrict__ wold, int64_t nr,
+                              double *__restrict__ pts)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nr * (ns + 1)) return;
+    const int64_t r = t / (ns + 1);
+    const int j = (int)(t % (ns + 1));
+    double v[3];
+    if (r == 0) te_vertex(gcur, nc, ns, j, v);
+    else
+        for (int c = 0; c < 3; ++c) v[c] = wold[3 * t + c];
+    for (int c = 0; c < 3; ++c) pts[3 * t + c] = v[c];
+}
+
 // New wake vertex rows of one surface: row 0 = trailing edge of the next
 // geometry, row r+1 = (r == 0 ? trailing edge of the current geometry : old
-// row r) + V dt, for r < nrow_new.
+// row r) + (V + u_r) dt for r < nrow_new, with u_r the induced velocity at
+// that old vertex (vel; nullptr for the prescribed wake, u = 0).
 __global__ void k_wake_vertices(const double *__restrict__ gnext, const double *__restrict__ gcur,
                                 int nc, int ns, const double *__restrict__ wold,
-                                double *__restrict__ wnew, int64_t nrow_new, double ox, double oy,
-                                double oz)
+                                const double *__restrict__ vel, double *__restrict__ wnew,
+                                int64_t nrow_new, double vx, double vy, double vz, double dt)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (nrow_new + 1) * (ns + 1)) return;
@@ -309,18 +332,31 @@ __global__ void k_wake_vertices(const double *__restrict__ gnext, const double *
     double v[3];
     if (r == 0) {
         te_vertex(gnext, nc, ns, j, v);
-    } else if (r == 1) {
-        te_vertex(gcur, nc, ns, j, v);
-        v[0] += ox;
-        v[1] += oy;
-        v[2] += oz;
     } else {
-        const double *o = wold + 3 * ((r - 1) * (ns + 1) + j);
-        v[0] = o[0] + ox;
-        v[1] = o[1] + oy;
-        v[2] = o[2] + oz;
+        if (r == 1) te_vertex(gcur, nc, ns, j, v);
+        else
+            for (int c = 0; c < 3; ++c) v[c] = wold[3 * ((r - 1) * (ns + 1) + j) + c];
+        const double *u = vel ? vel + 3 * ((r - 1) * (ns + 1) + j) : nullptr;
+        v[0] += (u ? vx + u[0] : vx) * dt;
+        v[1] += (u ? vy + u[1] : vy) * dt;
+        v[2] += (u ? vz + u[2] : vz) * dt;
     }
     for (int c = 0; c < 3; ++c) wnew[3 * t + c] = v[c];
+}
+
+// Rings of one surface's new wake whose corner-mean x exceeds max_x lose
+// their circulation (warp_fsi/kernels_wake.py:110-126).
+__global__ void k_wake_truncate(const double *__restrict__ w, int ns, int64_t nrow,
+                                double max_x, double *__restrict__ g)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nrow * ns) return;
+    const int64_t r = t / ns;
+    const int j = (int)(t % ns);
+    const double *a = w + 3 * (r * (ns + 1) + j), *b = a + 3;
+    const double *c = w + 3 * ((r + 1) * (ns + 1) + j), *e = c + 3;
+    const double cx = (b[0] + a[0] + c[0] + e[0]) * 0.25;
+    if (cx > max_x) g[t] = 0.0;
 }
 
 // New ring strengths / ages of one surface: row 0 = trailing-edge bound
@@ -468,14 +504,31 @@ static int hybrid_step_one(fv_hybrid *h)
         const double *gnext = h->grids + (size_t)(s + 1) * h->TV * 3;
         const int nxt = 1 - cur;
         const int64_t nrow_new = (h->nrow + 1 < h->R) ? h->nrow + 1 : h->R;
-        const double ox = d.vinf[0] * dt, oy = d.vinf[1] * dt, oz = d.vinf[2] * dt;
+        if (h->ring_free) {
+            // frozen snapshot: induced velocity at every old vertex that moves,
+            // from the bound rings (this step's gamma) and the wake rings
+            for (auto &sf : h->surf) {
+                const int64_t np_ = nrow_new * (sf.ns + 1);
+                hipLaunchKernelGGL(k_wake_points, dim3((unsigned)ceil_div(np_, 128)), dim3(128), 0,
+                                   ctx->stream, gcur + 3 * sf.vert_off, sf.nc, sf.ns,
+                                   h->wv[cur] + 3 * sf.wvert_off, nrow_new,
+                                   h->wpts + 3 * sf.wvert_off);
+                FV_LAUNCH_CHECK(ctx);
+                rc = seg_sum(ctx, h->wpts + 3 * sf.wvert_off, np_, h->tab, 4 * P + 4 * W, prec,
+                             h->wvel + 3 * sf.wvert_off, S_PARTIAL);
+                if (rc) return rc;
+                h->seg_pairs += (double)np_ * (4.0 * P + 4.0 * W);
+            }
+        }
         for (auto &sf : h->surf) {
             hipLaunchKernelGGL(k_wake_vertices,
                                dim3((unsigned)ceil_div((nrow_new + 1) * (sf.ns + 1), 128)),
                                dim3(128), 0, ctx->stream, gnext + 3 * sf.vert_off,
                                gcur + 3 * sf.vert_off, sf.nc, sf.ns,
-                               h->wv[cur] + 3 * sf.wvert_off, h->wv[nxt] + 3 * sf.wvert_off,
-                               nrow_new, ox, oy, oz);
+                               h->wv[cur] + 3 * sf.wvert_off,
+                               h->ring_free ? h->wvel + 3 * sf.wvert_off : nullptr,
+                               h->wv[nxt] + 3 * sf.wvert_off, nrow_new, d.vinf[0], d.vinf[1],
+                               d.vinf[2], dt);
             FV_LAUNCH_CHECK(ctx);
             const double *gam_te = h->gam + sf.panel_off + (int64_t)(sf.nc - 1) * sf.ns;
             hipLaunchKernelGGL(k_wake_rings, dim3((unsigned)ceil_div(nrow_new * sf.ns, 128)),
@@ -484,6 +537,13 @@ static int hybrid_step_one(fv_hybrid *h)
                                h->wa[nxt] + sf.wring_off, nrow_new, dt);
             FV_LAUNCH_CHECK(ctx);
         }
+        if (std::isfinite(h->trunc_x))
+            for (auto &sf : h->surf) {
+                hipLaunchKernelGGL(k_wake_truncate, dim3((unsigned)ceil_div(nrow_new * sf.ns, 128)),
+                                   dim3(128), 0, ctx->stream, h->wv[nxt] + 3 * sf.wvert_off, sf.ns,
+                                   nrow_new, h->trunc_x, h->wg[nxt] + sf.wring_off);
+                FV_LAUNCH_CHECK(ctx);
+            }
         h->par = nxt;
         h->nrow = nrow_new;
         const int64_t kr = d.n_keep;
@@ -597,6 +657,7 @@ int fv_hybrid_create(fv_ctx *ctx, const fv_hybrid_desc *desc, fv_hybrid **out)
     n += (size_t)P * P + 3 * P;                      // aic, rhs, gam, gam_prev
     n += 3 * P + 3 * P + 12 * P + 3 * P;             // vw, vp, vl, force
     n += 2 * (3 * WV + 2 * WR);                      // wake double buffers
+    n += 2 * 3 * WV;                                 // free-wake points, velocities
     n += 4 * 3 * WR + 3 * WR + P;                    // ring corners, rg, rage, rrc0, brc0
     n += 4 * 3 * NS + NS;                            // conversion corners + cg
     n += 2 * (size_t)d.num_steps;                    // cl, cdi
@@ -630,6 +691,7 @@ int fv_hybrid_create(fv_ctx *ctx, const fv_hybrid_desc *desc, fv_hybrid **out)
         h->wg[b] = take(WR);
         h->wa[b] = take(WR);
     }
+    h->wpts = take(3 * WV); h->wvel = take(3 * WV);
     h->rbr = take(3 * WR); h->rfr = take(3 * WR); h->rfl = take(3 * WR); h->rbl = take(3 * WR);
     h->rg = take(WR); h->rage = take(WR); h->rrc0 = take(WR); h->brc0 = take(P);
     h->cbr = take(3 * NS); h->cfr = take(3 * NS); h->cfl = take(3 * NS); h->cbl = take(3 * NS);
@@ -765,6 +827,15 @@ int fv_hybrid_set_population(fv_hybrid *h, int32_t scheme, int64_t max_particles
     h->pop_eps = eps;
     h->pop_protect = protect;
     h->pop_protect_dist = protect_dist;
+    return FV_OK;
+}
+
+int fv_hybrid_set_ring_wake(fv_hybrid *h, int32_t free_wake, double truncate_x)
+{
+    if (!h) return FV_E_ARG;
+    FV_REQUIRE(h->ctx, !std::isnan(truncate_x), "truncation x is NaN");
+    h->ring_free = free_wake ? 1 : 0;
+    h->trunc_x = truncate_x;
     return FV_OK;
 }
 

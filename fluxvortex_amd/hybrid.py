@@ -47,6 +47,8 @@ class HybridCase:
     sref: float = 1.0
     max_particles: int = 200000
     precision: str = "fp64"
+    ring_free_wake: bool = False  # ring wake moves with V + induction (prescribed: V)
+    truncate_x: float = math.inf  # rings whose corner-mean x exceeds this lose gamma
     name: str = "case"
     meta: dict = field(default_factory=dict)
 
@@ -105,6 +107,8 @@ class HybridRun:
         self.particles = VortexParticleField(max_particles=case.max_particles, rlxf=case.rlxf,
                                              kernel=case.kernel, precision=case.precision,
                                              context=self._ctx, _handle=vh, _owner=self)
+        if case.ring_free_wake or math.isfinite(case.truncate_x):
+            self.set_ring_wake(case.ring_free_wake, case.truncate_x)
 
     def step(self, n: int = 1) -> None:
         rc = self._ctx.lib.fv_hybrid_step(self._h, int(n))
@@ -145,6 +149,14 @@ class HybridRun:
         code = {"none": 0, "drop": 1, "merge": 2}[scheme]
         self._ctx.check(self._ctx.lib.fv_hybrid_set_population(
             self._h, code, int(max_particles), float(eps), int(protect), float(protect_dist)))
+
+    def set_ring_wake(self, free: bool = False, truncate_x: float = math.inf) -> None:
+        """Ring-wake motion: prescribed (V dt) or free (V + bound / wake-ring
+        induction, one snapshot, forward Euler: PteraSoftware's
+        prescribed_wake=False); rings whose corner-mean x exceeds truncate_x
+        lose their circulation (warp_fsi/kernels_wake.py:110-126)."""
+        self._ctx.check(self._ctx.lib.fv_hybrid_set_ring_wake(self._h, int(bool(free)),
+                                                              float(truncate_x)))
 
     def circulation_ledger(self):
         """Total vector circulation of particles + stored wake rings

@@ -259,6 +259,14 @@ int fv_hybrid_set_population(fv_hybrid *h, int32_t scheme, int64_t max_particles
 /* Total vector circulation: particle alpha + stored wake rings' G (b - a)
  * over their legs (flap.py:565-578). */
 int fv_hybrid_ledger_host(fv_hybrid *h, double *out3);
+/* Ring-wake motion.  free_wake 0: prescribed (every vertex moves by V dt, the
+ * default); 1: free (PteraSoftware's prescribed_wake=False path): each wake
+ * vertex moves by (V + u) dt with u induced at its pre-step position by the
+ * bound rings (this step's circulation) and the wake rings, all vertices from
+ * one snapshot.  Rings whose corner-mean x exceeds truncate_x lose their
+ * circulation after shedding (warp_fsi/kernels_wake.py:110-126; INFINITY:
+ * never). */
+int fv_hybrid_set_ring_wake(fv_hybrid *h, int32_t free_wake, double truncate_x);
 int fv_hybrid_set_precision(fv_hybrid *h, int32_t precision);
 
 /* ------------------------------------------ learned far-field closure MLP */

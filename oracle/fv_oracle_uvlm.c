@@ -69,6 +69,13 @@ typedef struct orc_hybrid_desc {
     /* 1: particles are a prescribed wake moved by V dt per step (no RK3 /
      * clamp / relaxation) */
     int32_t convect_exact;
+    /* 1: free ring wake -- every live wake vertex moves by (V + u) dt, u the
+     * bound (this step's circulation) + wake-ring induction at its pre-step
+     * position, all from one snapshot (PteraSoftware's prescribed_wake=False;
+     * warp_fsi/kernels_wake.py:34-108); rings whose corner-mean x exceeds
+     * truncate_x lose their circulation after shedding (:110-126) */
+    int32_t free_ring_wake;
+    double truncate_x;
 } orc_hybrid_desc;
 
 typedef struct orc_beam {
@@ -524,6 +531,33 @@ int orc_hybrid_run(const orc_hybrid_desc *d, int64_t steps, double *cl, double *
             for (int s = 0; s < S; ++s) {
                 int nc = sf[s].nc, ns = sf[s].ns;
                 int64_t nr = sf[s].nrow;
+                /* free ring wake: induced velocity at the live old vertex rows
+                 * 0 .. min(nr, n_keep) (rows beyond hold no circulation) */
+                const int64_t nfree = d->free_ring_wake ? (nr < d->n_keep ? nr : d->n_keep) + 1 : 0;
+                double *fp = NULL, *fu = NULL, *fu2 = NULL;
+                if (nfree > 0) {
+                    const int64_t npt = nfree * (ns + 1);
+                    fp = malloc(sizeof(double) * 3 * npt);
+                    fu = malloc(sizeof(double) * 3 * npt);
+                    fu2 = malloc(sizeof(double) * 3 * npt);
+                    if (!fp || !fu || !fu2) {
+                        free(fp); free(fu); free(fu2);
+                        rc = -1;
+                        goto done;
+                    }
+                    for (int64_t r = 0; r < nfree; ++r)
+                        for (int j = 0; j <= ns; ++j) {
+                            double *q = fp + 3 * (r * (ns + 1) + j);
+                            if (r == 0)
+                                te_of(G + 3 * sf[s].voff, nc, ns, j, q);
+                            else
+                                memcpy(q, sf[s].wv + 3 * (r * (ns + 1) + j), 3 * sizeof(double));
+                        }
+                    orc_collapsed(fp, npt, br, fr, fl, bl, P, gam, rc0b, zage, d->nu, 0xF, fu,
+                                  nthreads);
+                    orc_collapsed(fp, npt, wbr, wfr, wfl, wbl, W, wg, wrc, wa, d->nu, 0xF, fu2,
+                                  nthreads);
+                }
                 /* shift rows back-to-front in place */
                 for (int64_t r = nr; r >= 0; --r)
                     for (int j = 0; j <= ns; ++j) {
@@ -533,9 +567,16 @@ int orc_hybrid_run(const orc_hybrid_desc *d, int64_t steps, double *cl, double *
                         else
                             memcpy(v, sf[s].wv + 3 * (r * (ns + 1) + j), sizeof v);
                         /* old vertex row r (r = 0: current trailing edge) -> row r+1 */
-                        for (int k = 0; k < 3; ++k)
-                            sf[s].wv[3 * ((r + 1) * (ns + 1) + j) + k] = v[k] + V[k] * dt;
+                        double *o = sf[s].wv + 3 * ((r + 1) * (ns + 1) + j);
+                        if (r < nfree) {
+                            const int64_t q = 3 * (r * (ns + 1) + j);
+                            for (int k = 0; k < 3; ++k)
+                                o[k] = v[k] + (V[k] + (fu[q + k] + fu2[q + k])) * dt;
+                        } else {
+                            for (int k = 0; k < 3; ++k) o[k] = v[k] + V[k] * dt;
+                        }
                     }
+                free(fp); free(fu); free(fu2);
                 for (int j = 0; j <= ns; ++j) te_of(Gn + 3 * sf[s].voff, nc, ns, j,
                                                     sf[s].wv + 3 * j);
                 for (int64_t r = nr; r >= 1; --r)
@@ -549,6 +590,18 @@ int orc_hybrid_run(const orc_hybrid_desc *d, int64_t steps, double *cl, double *
                 }
                 sf[s].nrow = nr + 1;
             }
+            /* truncation by ring corner-mean x (Fr + Fl + Bl + Br) / 4 */
+            if (isfinite(d->truncate_x))
+                for (int s = 0; s < S; ++s) {
+                    const int ns = sf[s].ns;
+                    for (int64_t r = 0; r < sf[s].nrow; ++r)
+                        for (int j = 0; j < ns; ++j) {
+                            const double *a_ = sf[s].wv + 3 * (r * (ns + 1) + j), *b_ = a_ + 3;
+                            const double *c_ = sf[s].wv + 3 * ((r + 1) * (ns + 1) + j), *e_ = c_ + 3;
+                            const double cx = (b_[0] + a_[0] + c_[0] + e_[0]) * 0.25;
+                            if (cx > d->truncate_x) sf[s].wg[r * ns + j] = 0.0;
+                        }
+                }
             /* conversion of rows >= n_keep (surface, row, span order) */
             for (int s = 0; s < S; ++s) {
                 int ns = sf[s].ns;

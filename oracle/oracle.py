@@ -302,6 +302,7 @@ class _HybridDesc(C.Structure):
         ("stretch", C.c_int32), ("rlxf", C.c_double), ("vpm_nu", C.c_double),
         ("sref", C.c_double), ("max_particles", C.c_int64), ("precision", C.c_int32),
         ("beam", C.c_void_p), ("convect_exact", C.c_int32),
+        ("free_ring_wake", C.c_int32), ("truncate_x", C.c_double),
     ]
 
 
@@ -337,6 +338,8 @@ def hybrid_run(case, steps=None, threads=0):
     d.free_wake, d.stretch = int(bool(case.free_wake)), int(bool(case.stretch))
     d.rlxf, d.vpm_nu, d.sref = case.rlxf, case.vpm_nu, case.sref
     d.max_particles = int(case.max_particles)
+    d.free_ring_wake = int(bool(getattr(case, "ring_free_wake", False)))
+    d.truncate_x = float(getattr(case, "truncate_x", np.inf))
     n = grids.shape[0]
     steps = n if steps is None else steps
     P = int(sum(int(a) * int(b) for a, b in zip(nc, ns)))
@@ -386,6 +389,7 @@ def aeroelastic_run(inst, threads=0):
     d.free_wake, d.stretch, d.rlxf, d.vpm_nu = 0, 0, 0.0, 0.0
     d.sref, d.max_particles = inst.sref, int(inst.max_particles)
     d.convect_exact = 1
+    d.free_ring_wake, d.truncate_x = 0, np.inf
     n = int(inst.num_steps)
     tw, tt = np.zeros(n), np.zeros(n)
     sl, sm = np.zeros((n, inst.ns)), np.zeros((n, inst.ns))
