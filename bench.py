@@ -142,7 +142,11 @@ def run_config(hybrid, particles, steps, warmup, precision, stretch=True, closur
     if closure is not None:
         from fluxvortex_amd.mlp import ClosureMLP, closure_params
         net = ClosureMLP(closure_params(seed=0))
-        run.particles.enable_closure(net, **closure)
+        clo = dict(closure)
+        kl = clo.pop("kappa_local", 0.0)
+        run.particles.enable_closure(net, **clo)
+        if kl:
+            run.particles.set_closure_local(kl)
         run.particles.closure_train(rounds=200)
     run.set_flags(True, stretch)
     run.step(warmup)
@@ -321,7 +325,7 @@ def main(argv=None):
             for kappa in (2.5, 3.0, 4.0):
                 log(f"[bench] {a.big} particles, mixed + learned closure, kappa {kappa}")
                 clo = dict(cell=2.0, kappa=kappa, check_every=10, tol=1e-3, train_samples=4096,
-                           train_iters=4, lr=2e-3)
+                           train_iters=2, lr=2e-3, kappa_local=2.0 * kappa)
                 bml = run_config(hybrid, a.big, 3, 1, "mixed", closure=clo)
                 if bml["closure"]["active"]:
                     break

@@ -136,6 +136,7 @@ _SIGNATURES = {
     "fv_hybrid_set_precision": (C.c_int, [_vp, C.c_int32]),
     "fv_vpm_set_closure": (C.c_int, [_vp, _vp, C.c_double, C.c_double, C.c_int32, C.c_double,
                                      C.c_int32, C.c_int32, C.c_double]),
+    "fv_vpm_set_closure_local": (C.c_int, [_vp, C.c_double]),
     "fv_vpm_clear_closure": (C.c_int, [_vp]),
     "fv_vpm_closure_stats": (C.c_int, [_vp, _dp, _i64p, _i64p, _i32p, _dp]),
     "fv_vpm_closure_velocity_host": (C.c_int, [_vp, C.c_int32, _dp, _dp]),

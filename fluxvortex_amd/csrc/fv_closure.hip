@@ -47,10 +47,12 @@ constexpr int kOutW = 16;           // MLP output width (3 used)
 constexpr double kShell = 2.0;      // residual shell: kappa h < R <= kShell kappa h
 constexpr int kMaxSlices = 16;      // near (dx ranges) and far (source-cell ranges) slices
 constexpr int kFarTile = 64;        // source cells staged per far-field tile
+constexpr int kLocStride = 32;      // local expansion per cell: u0 (3), J0 (9), H (18), pad
 
 struct fv_closure_state {
     fv_mlp *mlp = nullptr;
     double h = 2.0, kappa = 4.0, tol = 1e-3, lr = 1e-3;
+    double kappa_local = 0.0;          // > 0: cells beyond kappa_local h via local expansions
     int check_every = 10, train_samples = 2048, train_iters = 2;
     int64_t max_cells = 1536;
     // status
@@ -90,7 +92,12 @@ struct fv_closure_state {
     size_t far_bytes = 0;
     double *mom = nullptr;             // kMomStride per occupied cell
     double *res = nullptr;             // learned residual velocity, 3 per occupied cell
+    double *loc = nullptr;             // local expansions, kLocStride per occupied cell
     int32_t *lcnt = nullptr, *loff = nullptr;   // shell pairs per target cell, offsets
+    void *band = nullptr;              // far band list (kappa h, kappa_local h] per target cell
+    size_t band_bytes = 0;
+    int32_t *bcnt = nullptr, *boff = nullptr, *bt = nullptr, *bs = nullptr;
+    int64_t nband = 0;
     void *list = nullptr;
     size_t list_bytes = 0;
     int32_t *lt = nullptr, *ls = nullptr;       // shell pair list (target, source)
@@ -345,6 +352,125 @@ __device__ void far_expansion(const double *m, double x0, double x1, double x2, 
             const double t2 = -3.0 * ((epsMd + dxc) * iR5 - 5.0 * dMd[i] * d[k] * iR7);
             J[3 * i + k] += c * (t0 + t1 + t2);
         }
+}
+
+// Local (target-cell) expansion of the far field of every occupied cell whose
+// centre lies beyond kappa_local h from the target cell's centre.  With phi =
+// 1/R and d = c_t - xbar_s, a cell's first-order field is
+//   u_i = -c eps_ijk (A_j dk phi - M_jl dl dk phi),
+// so u0, J0_im = dm u_i and H_imn = dn dm u_i at c_t follow from the
+// derivatives of phi up to the fourth (contracted with M through v = M d, the
+// eps sums written as cross products); the particles of the target cell then
+// take u0 + J0 (x - c_t) + H (x - c_t)(x - c_t) / 2 (and J0 + H (x - c_t) for
+// the gradient).  One workgroup per target cell, a fixed-order reduction.
+__global__ __launch_bounds__(256) void k_local_exp(int64_t nocc, const double *__restrict__ mom,
+                                                   double rl2, double *__restrict__ loc)
+{
+    __shared__ double red[4][30];
+    const int64_t t = blockIdx.x;
+    const double *mt = mom + t * kMomStride;
+    const double ct[3] = {mt[16], mt[17], mt[18]};
+    double acc[30];
+    for (int q = 0; q < 30; ++q) acc[q] = 0.0;
+    const double c = kInv4Pi;
+    for (int64_t s = threadIdx.x; s < nocc; s += 256) {
+        const double *ms = mom + s * kMomStride;
+        const double gx = ct[0] - ms[16], gy = ct[1] - ms[17], gz = ct[2] - ms[18];
+        if (gx * gx + gy * gy + gz * gz <= rl2) continue;
+        const double d[3] = {ct[0] - ms[1], ct[1] - ms[2], ct[2] - ms[3]};
+        const double *A = ms + 4, *M = ms + 7;   // M[3 j + l] = Sum a_j s_l
+        const double R2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        const double iR = rsqrt_refined(R2);
+        const double iR2 = iR * iR;
+        const double iR3 = iR * iR2, iR5 = iR3 * iR2, iR7 = iR5 * iR2, iR9 = iR7 * iR2;
+        // v = M d; with (j1, k1) = (i+1, i+2) every eps_ijk sum becomes a
+        // cross product or an antisymmetric pair
+        const double v[3] = {M[0] * d[0] + M[1] * d[1] + M[2] * d[2],
+                             M[3] * d[0] + M[4] * d[1] + M[5] * d[2],
+                             M[6] * d[0] + M[7] * d[1] + M[8] * d[2]};
+        const double Ad[3] = {A[1] * d[2] - A[2] * d[1], A[2] * d[0] - A[0] * d[2],
+                              A[0] * d[1] - A[1] * d[0]};
+        const double vd[3] = {v[1] * d[2] - v[2] * d[1], v[2] * d[0] - v[0] * d[2],
+                              v[0] * d[1] - v[1] * d[0]};
+        const double em[3] = {M[5] - M[7], M[6] - M[2], M[1] - M[3]};   // M_j1k1 - M_k1j1
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const int j1 = (i + 1) % 3, k1 = (i + 2) % 3;
+            acc[i] += -c * (-iR3 * Ad[i] - (3.0 * iR5 * vd[i] - iR3 * em[i]));
+            double Y[3], Am[3], Vm[3];
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                Y[m] = M[3 * j1 + m] * d[k1] - M[3 * k1 + m] * d[j1];
+                Am[m] = (k1 == m ? A[j1] : 0.0) - (j1 == m ? A[k1] : 0.0);
+                Vm[m] = (k1 == m ? v[j1] : 0.0) - (j1 == m ? v[k1] : 0.0);
+            }
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                const double t2 = 3.0 * iR5 * Ad[i] * d[m] - iR3 * Am[m];
+                const double s3 = -15.0 * iR7 * vd[i] * d[m] + 3.0 * iR5 * (em[i] * d[m] + Y[m] + Vm[m]);
+                acc[3 + 3 * i + m] += -c * (t2 - s3);
+            }
+            int o = 0;
+#pragma unroll
+            for (int m = 0; m < 3; ++m)
+#pragma unroll
+                for (int n = m; n < 3; ++n, ++o) {
+                    const double dmn = m == n ? 1.0 : 0.0;
+                    const double t3 = -15.0 * iR7 * Ad[i] * d[m] * d[n] +
+                                      3.0 * iR5 * (Am[m] * d[n] + Am[n] * d[m] + dmn * Ad[i]);
+                    const double z = (k1 == n ? M[3 * j1 + m] : 0.0) - (j1 == n ? M[3 * k1 + m] : 0.0) +
+                                     (k1 == m ? M[3 * j1 + n] : 0.0) - (j1 == m ? M[3 * k1 + n] : 0.0);
+                    const double s4 = 105.0 * iR9 * vd[i] * d[m] * d[n] -
+                                      15.0 * iR7 * (em[i] * d[m] * d[n] + Y[m] * d[n] + Y[n] * d[m] +
+                                                    Vm[m] * d[n] + Vm[n] * d[m] + dmn * vd[i]) +
+                                      3.0 * iR5 * (em[i] * dmn + z);
+                    acc[12 + 6 * i + o] += -c * (t3 - s4);
+                }
+        }
+    }
+    for (int q = 0; q < 30; ++q)
+        for (int off = 32; off > 0; off >>= 1) acc[q] += __shfl_xor(acc[q], off, 64);
+    if ((threadIdx.x & 63) == 0)
+        for (int q = 0; q < 30; ++q) red[threadIdx.x >> 6][q] = acc[q];
+    __syncthreads();
+    if (threadIdx.x < 30) {
+        const int q = threadIdx.x;
+        loc[t * kLocStride + q] = ((red[0][q] + red[1][q]) + red[2][q]) + red[3][q];
+    }
+    if (threadIdx.x == 30) {
+        loc[t * kLocStride + 30] = 0.0;
+        loc[t * kLocStride + 31] = 0.0;
+    }
+}
+
+// Adds the local expansion of the particle's cell (offset dd from the centre)
+// for mode 0 (velocity), 2 (stretching J alpha) or 3 (vorticity).
+template <int MODE>
+__device__ __forceinline__ void local_term(const double *L, const double *dd, const double *al,
+                                           double *r)
+{
+    double J[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const double *H = L + 12 + 6 * i;   // mn = 00 01 02 11 12 22
+        const double Hd[3] = {H[0] * dd[0] + H[1] * dd[1] + H[2] * dd[2],
+                              H[1] * dd[0] + H[3] * dd[1] + H[4] * dd[2],
+                              H[2] * dd[0] + H[4] * dd[1] + H[5] * dd[2]};
+        if (MODE == 0) {
+            double v = L[i];
+            for (int m = 0; m < 3; ++m) v += (L[3 + 3 * i + m] + 0.5 * Hd[m]) * dd[m];
+            r[i] += v;
+        } else {
+            for (int m = 0; m < 3; ++m) J[3 * i + m] = L[3 + 3 * i + m] + Hd[m];
+        }
+    }
+    if (MODE == 2)
+        for (int i = 0; i < 3; ++i) r[i] += J[3 * i] * al[0] + J[3 * i + 1] * al[1] + J[3 * i + 2] * al[2];
+    if (MODE == 3) {
+        r[0] += J[7] - J[5];
+        r[1] += J[2] - J[6];
+        r[2] += J[3] - J[1];
+    }
 }
 
 // The far term of one cell at point x for the closure passes, in closed form:
@@ -619,7 +745,7 @@ __global__ __launch_bounds__(256) void k_far_part(
     const int32_t *__restrict__ occ_of, const double *__restrict__ sx,
     const double *__restrict__ sy, const double *__restrict__ sz, const double *__restrict__ sax,
     const double *__restrict__ say, const double *__restrict__ saz, const double *__restrict__ mom,
-    double rc2, double *__restrict__ fpart)
+    double rc2, double rl2, const double *__restrict__ loc, double *__restrict__ fpart)
 {
     // 64-cell tiles (9.7 KB of LDS) keep ~7 wavefronts per SIMD resident
     __shared__ double tile[kFarTile * kMomTile];
@@ -644,13 +770,60 @@ __global__ __launch_bounds__(256) void k_far_part(
         for (int k = 0; k < cnt; ++k) {
             const double *ms = tile + k * kMomTile;
             const double dx = c0 - ms[16], dy = c1 - ms[17], dz = c2 - ms[18];
-            if (dx * dx + dy * dy + dz * dz <= rc2) continue;
+            const double g2 = dx * dx + dy * dy + dz * dz;
+            if (g2 <= rc2 || g2 > rl2) continue;
             far_term<MODE>(ms, x0, x1, x2, al, r);
         }
     }
     if (!active) return;
+    if (loc && blockIdx.y == 0) {
+        const double dd[3] = {x0 - c0, x1 - c1, x2 - c2};
+        local_term<MODE>(loc + t * kLocStride, dd, al, r);
+    }
     double *dst = fpart + (int64_t)blockIdx.y * 3 * n;
     for (int a = 0; a < 3; ++a) dst[3 * i + a] = r[a];
+}
+
+// Far field with the local expansions on: one workgroup per (target cell,
+// 256-particle chunk); the cells of the target cell's band list (kappa h <
+// distance <= kappa_local h) staged in LDS, each particle's sum per lane, plus
+// the target cell's local expansion (everything beyond the band).
+template <int MODE>
+__global__ __launch_bounds__(256) void k_far_band(
+    int64_t n, const int32_t *__restrict__ occ, const int32_t *__restrict__ cstart,
+    const int32_t *__restrict__ cend, const int32_t *__restrict__ boff,
+    const int32_t *__restrict__ bs, const double *__restrict__ sx,
+    const double *__restrict__ sy, const double *__restrict__ sz, const double *__restrict__ sax,
+    const double *__restrict__ say, const double *__restrict__ saz, const double *__restrict__ mom,
+    const double *__restrict__ loc, double *__restrict__ fpart)
+{
+    __shared__ double tile[kFarTile * kMomTile];
+    const int64_t t = blockIdx.x;
+    const int c = occ[t];
+    const int64_t p0 = cstart[c] + (int64_t)blockIdx.y * 256, p1 = cend[c];
+    if (p0 >= p1) return;
+    const int64_t i = p0 + threadIdx.x;
+    const bool active = i < p1;
+    const int64_t il = active ? i : p0;
+    const double *mt = mom + t * kMomStride;
+    const double x0 = sx[il], x1 = sy[il], x2 = sz[il];
+    const double al[3] = {sax[il], say[il], saz[il]};
+    double r[3] = {0.0, 0.0, 0.0};
+    const int64_t b0 = boff[t], b1 = boff[t + 1];
+    for (int64_t s0 = b0; s0 < b1; s0 += kFarTile) {
+        const int cnt = (int)((b1 - s0) < kFarTile ? (b1 - s0) : kFarTile);
+        __syncthreads();
+        for (int e = threadIdx.x; e < cnt * kMomTile; e += 256) {
+            const int k = e / kMomTile, q = e - k * kMomTile;
+            tile[e] = mom[(int64_t)bs[s0 + k] * kMomStride + q];
+        }
+        __syncthreads();
+        for (int k = 0; k < cnt; ++k) far_term<MODE>(tile + k * kMomTile, x0, x1, x2, al, r);
+    }
+    if (!active) return;
+    const double dd[3] = {x0 - mt[16], x1 - mt[17], x2 - mt[18]};
+    local_term<MODE>(loc + t * kLocStride, dd, al, r);
+    for (int a = 0; a < 3; ++a) fpart[3 * i + a] = r[a];
 }
 
 // Near slices + far slices (+ the learned residual of the particle's cell) in
@@ -842,6 +1015,63 @@ static int closure_residual(fv_vpm *f, bool use_mlp);
 // stages of one advect step, whose particles move a small fraction of a cell)
 // the step's binning is kept: cell membership, the shell list and the learned
 // residual stay, positions, strengths and moments are refreshed.
+static int build_local(fv_closure *cl, fv_ctx *ctx);
+
+// Band list of every target cell: source cells with kappa h < distance <=
+// kappa_local h (geometry only, once per binning).
+static int build_band(fv_closure *cl, fv_ctx *ctx, double rc2)
+{
+    cl->nband = 0;
+    if (!(cl->kappa_local > 0.0) || cl->nocc == 0) return FV_OK;
+    const int64_t nocc = cl->nocc;
+    const double rl = cl->kappa_local * cl->hh;
+    const double rl2 = rl * rl * (1.0 + 1e-12);
+    int rc = grow(ctx, &cl->band, &cl->band_bytes, (size_t)(2 * nocc + 4) * sizeof(int32_t) + 256);
+    if (rc) return rc;
+    cl->bcnt = static_cast<int32_t *>(cl->band);
+    cl->boff = cl->bcnt + (nocc + 1);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, ctx->stream, cl->bcnt + nocc, (int64_t)1, 0);
+    FV_LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_shell_count, dim3((unsigned)nocc), dim3(64), 0, ctx->stream, nocc, cl->mom,
+                       rc2, rl2, cl->bcnt);
+    FV_LAUNCH_CHECK(ctx);
+    size_t sb = 0;
+    FV_HIP_CHECK(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, sb, cl->bcnt, cl->boff,
+                                                      (int)(nocc + 1), ctx->stream));
+    rc = grow(ctx, &cl->tmp, &cl->tmp_bytes, sb);
+    if (rc) return rc;
+    FV_HIP_CHECK(ctx, hipcub::DeviceScan::ExclusiveSum(cl->tmp, sb, cl->bcnt, cl->boff,
+                                                      (int)(nocc + 1), ctx->stream));
+    int32_t nb = 0;
+    FV_HIP_CHECK(ctx, hipMemcpyAsync(&nb, cl->boff + nocc, sizeof nb, hipMemcpyDeviceToHost,
+                                     ctx->stream));
+    FV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    cl->nband = nb;
+    // the list follows the counts in the same buffer (which may move)
+    const size_t need = (size_t)(2 * nocc + 4) * sizeof(int32_t) + (size_t)2 * (nb + 1) * sizeof(int32_t) + 256;
+    if (need > cl->band_bytes) {
+        void *nbuf = nullptr;
+        if (hipMalloc(&nbuf, need) != hipSuccess)
+            return set_error(ctx, FV_E_HIP, "hipMalloc(%zu) for the closure band list failed", need);
+        FV_HIP_CHECK(ctx, hipMemcpyAsync(nbuf, cl->band, (size_t)(2 * nocc + 2) * sizeof(int32_t),
+                                         hipMemcpyDeviceToDevice, ctx->stream));
+        FV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        (void)hipFree(cl->band);
+        cl->band = nbuf;
+        cl->band_bytes = need;
+        cl->bcnt = static_cast<int32_t *>(cl->band);
+        cl->boff = cl->bcnt + (nocc + 1);
+    }
+    cl->bt = cl->boff + (nocc + 1);
+    cl->bs = cl->bt + (nb + 1);
+    if (nb > 0) {
+        hipLaunchKernelGGL(k_shell_fill, dim3((unsigned)nocc), dim3(64), 0, ctx->stream, nocc,
+                           cl->mom, rc2, rl2, cl->boff, cl->bt, cl->bs);
+        FV_LAUNCH_CHECK(ctx);
+    }
+    return FV_OK;
+}
+
 static int closure_build(fv_vpm *f, bool use_mlp, bool with_res = true, bool stage = false)
 {
     fv_closure *cl = f->closure;
@@ -860,6 +1090,8 @@ static int closure_build(fv_vpm *f, bool use_mlp, bool with_res = true, bool sta
                            cl->occ, cl->cstart, cl->cend, cl->sx, cl->sy, cl->sz, cl->sax, cl->say,
                            cl->saz, cl->ox, cl->oy, cl->oz, cl->hh, cl->ny, cl->nz, cl->mom);
         FV_LAUNCH_CHECK(ctx);
+        rc = build_local(cl, ctx);
+        if (rc) return rc;
         cl->built_epoch = f->state_epoch;
         return with_res ? closure_residual(f, use_mlp) : FV_OK;
     }
@@ -980,18 +1212,21 @@ static int closure_build(fv_vpm *f, bool use_mlp, bool with_res = true, bool sta
     cl->nocc = nocc;
     cl->maxpop = occinfo[1];
     // moments, residual per cell, shell counts / offsets
-    const size_t fneed = (size_t)nocc * (kMomStride + 3) * sizeof(double) +
+    const size_t fneed = (size_t)nocc * (kMomStride + 3 + kLocStride) * sizeof(double) +
                          (size_t)(2 * nocc + 2) * sizeof(int32_t) + 1024;
     rc = grow(ctx, &cl->far, &cl->far_bytes, fneed);
     if (rc) return rc;
     cl->mom = static_cast<double *>(cl->far);
     cl->res = cl->mom + nocc * kMomStride;
-    cl->lcnt = reinterpret_cast<int32_t *>(cl->res + 3 * nocc);
+    cl->loc = cl->res + 3 * nocc;
+    cl->lcnt = reinterpret_cast<int32_t *>(cl->loc + kLocStride * nocc);
     cl->loff = cl->lcnt + (nocc + 1);
     hipLaunchKernelGGL(k_moments, dim3((unsigned)nocc), dim3(64), 0, ctx->stream, nocc, cl->occ,
                        cl->cstart, cl->cend, cl->sx, cl->sy, cl->sz, cl->sax, cl->say, cl->saz,
                        cl->ox, cl->oy, cl->oz, h, cl->ny, cl->nz, cl->mom);
     FV_LAUNCH_CHECK(ctx);
+    rc = build_local(cl, ctx);
+    if (rc) return rc;
     // shell pair list (geometry only)
     const double rc2 = (cl->kappa * h) * (cl->kappa * h) * (1.0 + 1e-12);
     const double rm2 = rc2 * kShell * kShell;
@@ -1026,6 +1261,8 @@ static int closure_build(fv_vpm *f, bool use_mlp, bool with_res = true, bool sta
                            cl->mom, rc2, rm2, cl->loff, cl->lt, cl->ls);
         FV_LAUNCH_CHECK(ctx);
     }
+    rc = build_band(cl, ctx, rc2);
+    if (rc) return rc;
     rc = update_stencil(cl, ctx);
     if (rc) return rc;
     cl->built_epoch = f->state_epoch;
@@ -1089,9 +1326,32 @@ static int far_slices(const fv_closure *cl)
 template <int M>
 static void launch_far(fv_closure *cl, fv_ctx *ctx, double rc2, int nfs)
 {
+    if (cl->kappa_local > 0.0) {
+        const unsigned chunks = (unsigned)ceil_div(cl->maxpop > 0 ? cl->maxpop : 1, 256);
+        hipLaunchKernelGGL((k_far_band<M>), dim3((unsigned)cl->nocc, chunks), dim3(256), 0,
+                           ctx->stream, cl->n, cl->occ, cl->cstart, cl->cend, cl->boff, cl->bs,
+                           cl->sx, cl->sy, cl->sz, cl->sax, cl->say, cl->saz, cl->mom, cl->loc,
+                           cl->fpart);
+        return;
+    }
+    const bool lx = cl->kappa_local > 0.0;
+    const double rl = cl->kappa_local * cl->hh;
+    const double rl2 = lx ? rl * rl * (1.0 + 1e-12) : INFINITY;
     hipLaunchKernelGGL((k_far_part<M>), dim3((unsigned)ceil_div(cl->n, 256), (unsigned)nfs),
                        dim3(256), 0, ctx->stream, cl->n, cl->nocc, nfs, cl->key_s, cl->occ_of,
-                       cl->sx, cl->sy, cl->sz, cl->sax, cl->say, cl->saz, cl->mom, rc2, cl->fpart);
+                       cl->sx, cl->sy, cl->sz, cl->sax, cl->say, cl->saz, cl->mom, rc2, rl2,
+                       lx ? cl->loc : (const double *)nullptr, cl->fpart);
+}
+
+// Local expansions of every occupied cell (after its moments are current).
+static int build_local(fv_closure *cl, fv_ctx *ctx)
+{
+    if (!(cl->kappa_local > 0.0) || cl->nocc == 0) return FV_OK;
+    const double rl = cl->kappa_local * cl->hh;
+    hipLaunchKernelGGL(k_local_exp, dim3((unsigned)cl->nocc), dim3(256), 0, ctx->stream, cl->nocc,
+                       cl->mom, rl * rl * (1.0 + 1e-12), cl->loc);
+    FV_LAUNCH_CHECK(ctx);
+    return FV_OK;
 }
 
 // Closure sum at the particles (original order) for mode 0 (velocity), 2
@@ -1125,7 +1385,7 @@ static int closure_pass(fv_vpm *f, int mode, int kernel, double *out, bool use_m
         FV_LAUNCH_CHECK(ctx);
         tl.done();
     }
-    const int nfs = far_slices(cl);
+    const int nfs = cl->kappa_local > 0.0 ? 1 : far_slices(cl);
     {
         TimedLaunch tl(ctx, "closure_far");
         const double rc2 = (cl->kappa * cl->hh) * (cl->kappa * cl->hh) * (1.0 + 1e-12);
@@ -1242,7 +1502,7 @@ int closure_self_sum(fv_vpm *f, int mode, int kernel, double *out)
 void closure_destroy(fv_closure *cl)
 {
     if (!cl) return;
-    void *bufs[] = {cl->block, cl->cells, cl->far, cl->list, cl->tmp, cl->bbox, cl->dstencil,
+    void *bufs[] = {cl->block, cl->cells, cl->far, cl->band, cl->list, cl->tmp, cl->bbox, cl->dstencil,
                     cl->tX, cl->vdir};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -1275,6 +1535,21 @@ int fv_vpm_set_closure(fv_vpm *f, fv_mlp *mlp, double cell, double kappa, int32_
     cl->train_iters = train_iters;
     cl->lr = lr;
     f->closure = cl;
+    return FV_OK;
+}
+
+int fv_vpm_set_closure_local(fv_vpm *f, double kappa_local)
+{
+    if (!f) return FV_E_ARG;
+    fv_ctx *ctx = f->ctx;
+    FV_REQUIRE(ctx, f->closure, "no closure enabled");
+    fv_closure *cl = f->closure;
+    FV_REQUIRE(ctx, kappa_local == 0.0 || kappa_local >= kShell * cl->kappa,
+               "kappa_local %g must be 0 or at least %g (outside the learned shell)", kappa_local,
+               kShell * cl->kappa);
+    cl->kappa_local = kappa_local;
+    cl->built_epoch = -1;   // a full rebuild: band lists and local expansions
+    cl->binned_step = -1;
     return FV_OK;
 }
 

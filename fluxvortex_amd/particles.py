@@ -163,6 +163,11 @@ class VortexParticleField:
             self._h, mlp.handle if mlp is not None else None, float(cell), float(kappa),
             int(check_every), float(tol), int(train_samples), int(train_iters), float(lr)))
 
+    def set_closure_local(self, kappa_local):
+        """Cells beyond kappa_local * cell enter through a second-order local
+        expansion at each target cell (0: off; else >= 2 kappa)."""
+        self._ctx.check(self._ctx.lib.fv_vpm_set_closure_local(self._h, float(kappa_local)))
+
     def disable_closure(self):
         self._ctx.check(self._ctx.lib.fv_vpm_clear_closure(self._h))
         self._closure_mlp = None

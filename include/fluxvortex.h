@@ -311,6 +311,12 @@ int fv_mlp_adam_host(fv_mlp *m, const float *grad, float lr);
  * No reference counterpart (the reference sums directly, kernel.py:69-190). */
 int fv_vpm_set_closure(fv_vpm *f, fv_mlp *mlp, double cell, double kappa, int32_t check_every,
                        double tol, int32_t train_samples, int32_t train_iters, double lr);
+/* Cells farther than kappa_local * cell from a particle's cell (centre to
+ * centre) enter through a second-order local expansion at that cell's centre,
+ * built once per binning from every such cell's monopole + dipole, instead of
+ * per particle (0: off, the default; else >= 2 kappa, beyond the learned
+ * shell).  No counterpart in the reference (the closure is this build's). */
+int fv_vpm_set_closure_local(fv_vpm *f, double kappa_local);
 int fv_vpm_clear_closure(fv_vpm *f);
 int fv_vpm_closure_stats(fv_vpm *f, double *last_err, int64_t *checks, int64_t *rejections,
                          int32_t *active, double *far_evals);

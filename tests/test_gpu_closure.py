@@ -128,3 +128,65 @@ def test_closure_modes_match_direct_sums(ctx, kernel):
     for mode, ref in (("velocity", u), ("stretching", S), ("vorticity", w)):
         got = f.closure_sum(mode, kernel=kernel, use_mlp=False)
         assert relerr(got, ref) < 1e-12, mode
+
+
+def test_closure_local_expansion_order():
+    """Two clusters, each inside one cell: the far field of the source cell at
+    the target particles enters per particle (first-order source expansion)
+    or through the target cell's second-order local expansion of the same
+    field.  Their difference is the local expansion's truncation error:
+    relative to the far velocity it falls as (delta / D)^3 (x8 per doubling
+    of D) and for the stretching (gradient) as (delta / D)^2 (x4) -- a wrong
+    Hessian would leave the velocity at the lower order."""
+    from fluxvortex_amd.particles import VortexParticleField
+    rng = np.random.default_rng(3)
+    n = 400
+    errs = {}
+    for D in (24.0, 48.0):
+        tgt = rng.uniform(0.05, 1.95, (n, 3))
+        src = rng.uniform(0.05, 1.95, (n, 3)) + [D, 0.0, 0.0]
+        pos = np.concatenate([tgt, src])
+        al = rng.normal(0, 0.01, (2 * n, 3))
+        sg = np.full(2 * n, 0.2)
+        f = VortexParticleField(max_particles=2 * n + 8, rlxf=0.0, kernel="winckelmans")
+        f.add_particles_batch(pos, al, sg)
+        f.enable_closure(None, cell=2.0, kappa=2.0)
+        ref = {m: f.closure_sum(m, use_mlp=False)[:n] for m in ("velocity", "stretching")}
+        f.set_closure_local(4.0)
+        got = {m: f.closure_sum(m, use_mlp=False)[:n] for m in ("velocity", "stretching")}
+        f.set_closure_local(0.0)
+        assert np.array_equal(f.closure_sum("velocity", use_mlp=False)[:n], ref["velocity"])
+        # the far part alone: closure minus the target cell's own exact sum
+        f2 = VortexParticleField(max_particles=n + 8, rlxf=0.0, kernel="winckelmans")
+        f2.add_particles_batch(tgt, al[:n], sg[:n])
+        f2.enable_closure(None, cell=2.0, kappa=2.0)
+        own = {m: f2.closure_sum(m, use_mlp=False) for m in ("velocity", "stretching")}
+        for m in ref:
+            far = ref[m] - own[m]
+            errs[(m, D)] = np.abs(got[m] - ref[m]).max() / np.abs(far).max()
+    print("local expansion errors", errs)
+    assert errs[("velocity", 24.0)] < 1e-3
+    assert 5.5 < errs[("velocity", 24.0)] / errs[("velocity", 48.0)] < 11.0
+    assert 2.5 < errs[("stretching", 24.0)] / errs[("stretching", 48.0)] < 6.0
+
+
+def test_closure_local_expansion_on_a_wake():
+    """On the wake sheet the local expansion beyond 6 cells changes the
+    closure's velocity / stretching / vorticity by under 1e-4 of their max
+    norm, and the closure with it stays within 1e-3 of the direct sums."""
+    from fluxvortex_amd import kernel as K
+    from fluxvortex_amd.particles import VortexParticleField
+    pos, al, sg = wake_sheet(nx=400)
+    f = VortexParticleField(max_particles=len(sg) + 8, rlxf=0.0, kernel="winckelmans")
+    f.add_particles_batch(pos, al, sg)
+    f.enable_closure(None, cell=2.0, kappa=2.5)
+    base = {m: f.closure_sum(m, use_mlp=False) for m in ("velocity", "stretching", "vorticity")}
+    f.set_closure_local(6.0)
+    u = K.velocity_from_particles(pos, pos, al, sg, kernel="winckelmans")
+    for m in base:
+        got = f.closure_sum(m, use_mlp=False)
+        d = np.abs(got - base[m]).max() / np.abs(base[m]).max()
+        print(f"local expansion {m}: change {d:.2e}")
+        assert d < 1e-4, (m, d)
+    got = f.closure_sum("velocity", use_mlp=False)
+    assert np.abs(got - u).max() / np.abs(u).max() < 1e-3
