@@ -39,6 +39,11 @@ FLOPS = {"particle_vel": 31, "particle_stretch": 66, "particle_curl": 72,
          "particle_induce": 31, "seg_pairs": 51}
 PEAK_FP64 = 78.6e12
 PEAK_FP32 = 157.3e12
+# HBM bytes per launch of the stretching pair sum at the default 20k workload
+# (20 096 particles at the timed steps), from two rocprofv3 counter passes
+# (FETCH_SIZE x 2.00, WRITE_SIZE x 1.00, scaled by a plain 512 MiB copy):
+# profiles/k_pairs_stretch_heaving_wing_20k_pmc.md
+TRAFFIC_STRETCH = {"particles": 20096, "bytes": (4181.0 * 2.0 + 6586.1) * 1024}
 
 
 def log(msg):
@@ -192,7 +197,7 @@ def cpu_baseline(particles, threads):
             "sample_seconds": s1 + s2 + s3}
 
 
-def roofline(fam, precision):
+def roofline(fam, precision, particles=None):
     """Dominant particle kernel: algorithmic flops per launch / average launch time."""
     cands = {k: v for k, v in fam.items() if k in SELF_FAMILIES and v["launches"]}
     name = max(cands, key=lambda k: cands[k]["ms"])
@@ -201,8 +206,11 @@ def roofline(fam, precision):
     flops = FLOPS[name] * e["pairs"] / e["launches"]
     achieved = flops / avg_s
     peak = PEAK_FP64
+    traffic = None
+    if name == "particle_stretch" and precision == "fp64" and particles == TRAFFIC_STRETCH["particles"]:
+        traffic = TRAFFIC_STRETCH["bytes"]
     return name, {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
-                  "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                  "unit": "TFLOP/s", "frac": achieved / peak, "traffic": traffic,
                   "kernel": f"k_pairs<PartOp> ({name})", "avg_launch_ms": avg_s * 1e3,
                   "flops_per_launch": flops,
                   "note": ("FP64 VALU pair sum; MI355X FP64 vector peak = FP64 matrix peak = "
@@ -281,7 +289,8 @@ def main(argv=None):
     m64 = run_config(hybrid, a.particles, a.steps, a.warmup, "fp64")
     sec = m64["ms"] / 1e3
     pairs_total = m64["seg_pairs"] + m64["particle_pairs"]
-    name, roof = roofline(m64["families"], "fp64")
+    np_timed = m64["np"][0] if isinstance(m64["np"], (list, tuple)) else m64["np"]
+    name, roof = roofline(m64["families"], "fp64", np_timed)
     out = {
         "metric": "hybrid UVLM-VPM steps/s & G pair-interactions/s: FP64 vs mixed+learned",
         "value": a.steps / sec, "unit": "steps/s", "n_gpus": 1, "steps": a.steps,
