@@ -278,6 +278,7 @@ def main(argv=None):
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-flutter", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     a = ap.parse_args(argv)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
@@ -285,6 +286,9 @@ def main(argv=None):
         return 2
     from fluxvortex_amd import _lib, hybrid
     _lib.load_library()
+    # tile shapes of the pair sums (and the closure's near field) timed at
+    # first use, for every configuration below alike
+    _lib.default_context().set_autotune(not a.no_autotune)
     log(f"[bench] heaving wing, {a.particles} particles, FP64, warmup {a.warmup}, steps {a.steps}")
     m64 = run_config(hybrid, a.particles, a.steps, a.warmup, "fp64")
     sec = m64["ms"] / 1e3
@@ -351,6 +355,10 @@ def main(argv=None):
     if not a.no_flutter:
         log("[bench] batched Goland flutter sweep, 64 speeds")
         out["flutter_sweep"] = flutter_block()
+    out["autotune"] = [dict(zip(("family", "log2_targets", "log2_sources", "lanes",
+                                 "targets_per_lane", "workgroups_per_cu", "best_ms"), e))
+                       for e in _lib.default_context().autotune_report()
+                       if e[1] >= 14]
     if not a.no_cpu_baseline:
         threads = os.cpu_count() or 1
         log(f"[bench] CPU baseline (oracle, {threads} threads)")

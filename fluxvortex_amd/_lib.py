@@ -79,6 +79,8 @@ _SIGNATURES = {
     "fv_device_free": (C.c_int, [_ctx, _vp]),
     "fv_memcpy_h2d": (C.c_int, [_ctx, _vp, _vp, C.c_size_t]),
     "fv_memcpy_d2h": (C.c_int, [_ctx, _vp, _vp, C.c_size_t]),
+    "fv_ctx_set_autotune": (C.c_int, [_ctx, C.c_int32]),
+    "fv_autotune_report": (C.c_int, [_ctx, C.c_char_p, C.c_int64]),
     "fv_ctx_set_timing": (C.c_int, [_ctx, C.c_int]),
     "fv_ctx_kernel_stats": (C.c_int, [_ctx, C.c_char_p, _i64p, _dp]),
     "fv_ctx_event_record": (C.c_int, [_ctx, C.c_int32]),
@@ -218,6 +220,21 @@ class Context:
 
     def set_timing(self, enabled: bool) -> None:
         self.check(self.lib.fv_ctx_set_timing(self.handle, 1 if enabled else 0))
+
+    def set_autotune(self, enabled: bool) -> None:
+        """Pair-sum tile shapes timed at first use per kind (see fluxvortex.h)."""
+        self.check(self.lib.fv_ctx_set_autotune(self.handle, 1 if enabled else 0))
+
+    def autotune_report(self) -> list:
+        """[(family, log2_targets, log2_sources, lanes, targets_per_lane,
+        workgroups_per_cu, best_ms)] of the tuned kinds."""
+        buf = C.create_string_buffer(1 << 16)
+        self.check(self.lib.fv_autotune_report(self.handle, buf, len(buf)))
+        out = []
+        for line in buf.value.decode().splitlines():
+            f = line.split()
+            out.append((f[0], int(f[1]), int(f[2]), int(f[3]), int(f[4]), int(f[5]), float(f[6])))
+        return out
 
     def kernel_stats(self, family: str) -> tuple[int, float]:
         n = C.c_int64()

@@ -6,7 +6,10 @@
 // preprocessed once into a segment table and every target's sum runs in one
 // tiled, atomic-free pass.
 #include <cmath>
+#include <cstdio>
 #include <limits>
+#include <map>
+#include <mutex>
 
 #include "fv_launch.h"
 
@@ -95,12 +98,20 @@ int build_segtab(fv_ctx *ctx, const double *br, const double *fr, const double *
 
 PairPlan plan_pairs(fv_ctx *ctx, int64_t n_tgt, int64_t n_src)
 {
+    // the shapes the autotuner picks on MI355X for these size classes
+    return plan_pairs_shape(ctx, n_tgt, n_src, (n_tgt >= 65536) ? 256 : 64,
+                            (n_tgt >= 65536) ? 4 : (n_tgt >= 16384) ? 2 : 1, 16);
+}
+
+PairPlan plan_pairs_shape(fv_ctx *ctx, int64_t n_tgt, int64_t n_src, int block, int tpt,
+                          int per_cu)
+{
     PairPlan p;
-    p.block = (n_tgt >= 65536) ? 256 : 64;
-    p.tpt = (n_tgt >= 16384) ? 2 : 1;
+    p.block = block;
+    p.tpt = tpt;
     const int64_t per_block = (int64_t)p.block * p.tpt;
     p.gx = ceil_div(n_tgt > 0 ? n_tgt : 1, per_block);
-    const int64_t want = 8LL * num_cus(ctx);
+    const int64_t want = (int64_t)per_cu * num_cus(ctx);
     int64_t tiles = ceil_div(n_src > 0 ? n_src : 1, p.block);
     int64_t split = ceil_div(want, p.gx);
     if (split > tiles) split = tiles;
@@ -122,12 +133,121 @@ static void launch_pairs(fv_ctx *ctx, const PairPlan &p, const typename Op::Args
 }
 
 template <class Op>
+static void dispatch_pairs(fv_ctx *ctx, const PairPlan &p, const typename Op::Args &a,
+                           int64_t n_tgt, int64_t n_src, const int64_t *dyn_tgt,
+                           const int64_t *dyn_src, double *dst)
+{
+    if (p.block == 256) {
+        if (p.tpt == 4) launch_pairs<Op, 256, 4>(ctx, p, a, n_tgt, n_src, dyn_tgt, dyn_src, dst);
+        else if (p.tpt == 2) launch_pairs<Op, 256, 2>(ctx, p, a, n_tgt, n_src, dyn_tgt, dyn_src, dst);
+        else launch_pairs<Op, 256, 1>(ctx, p, a, n_tgt, n_src, dyn_tgt, dyn_src, dst);
+    } else {
+        if (p.tpt == 4) launch_pairs<Op, 64, 4>(ctx, p, a, n_tgt, n_src, dyn_tgt, dyn_src, dst);
+        else if (p.tpt == 2) launch_pairs<Op, 64, 2>(ctx, p, a, n_tgt, n_src, dyn_tgt, dyn_src, dst);
+        else launch_pairs<Op, 64, 1>(ctx, p, a, n_tgt, n_src, dyn_tgt, dyn_src, dst);
+    }
+}
+
+// ---------------------------------------------------------------- autotuner
+// With fv_ctx_set_autotune on, the first pair sum of a kind (operator, device,
+// power-of-two size class of targets and sources) times every candidate tile
+// shape -- 64 / 256 lanes, 1 / 2 / 4 targets per lane, 4 / 8 / 16 workgroups
+// per CU -- with HIP events (each run twice, the second counted), keeps the
+// fastest and uses it for every later call of that kind.  The shapes differ
+// only in how the sources are split, so results agree to round-off.
+struct TuneKey {
+    const void *op;
+    int device, lt, ls;
+    bool operator<(const TuneKey &o) const
+    {
+        if (op != o.op) return op < o.op;
+        if (device != o.device) return device < o.device;
+        if (lt != o.lt) return lt < o.lt;
+        return ls < o.ls;
+    }
+};
+struct TuneVal {
+    int block, tpt, per_cu;
+    double ms;
+    const char *family;
+    int prec;
+};
+static std::map<TuneKey, TuneVal> g_tune;
+static std::mutex g_tune_mu;
+
+static int ilog2_ceil(int64_t n)
+{
+    int l = 0;
+    while ((int64_t(1) << l) < n) ++l;
+    return l;
+}
+
+template <class Op>
+static int tuned_plan(fv_ctx *ctx, const typename Op::Args &a, int64_t n_tgt, int64_t n_src,
+                      const int64_t *dyn_tgt, const int64_t *dyn_src, int pslot,
+                      const char *family, PairPlan &out)
+{
+    out = plan_pairs(ctx, n_tgt, n_src);
+    if (!ctx->autotune) return FV_OK;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(ctx->stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
+        return FV_OK;
+    const TuneKey key{reinterpret_cast<const void *>(&k_pairs<Op, 64, 1>), ctx->device,
+                      ilog2_ceil(n_tgt), ilog2_ceil(n_src > 0 ? n_src : 1)};
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune.find(key);
+        if (it != g_tune.end()) {
+            out = plan_pairs_shape(ctx, n_tgt, n_src, it->second.block, it->second.tpt,
+                                   it->second.per_cu);
+            return FV_OK;
+        }
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    FV_HIP_CHECK(ctx, hipEventCreate(&e0));
+    FV_HIP_CHECK(ctx, hipEventCreate(&e1));
+    TuneVal best{out.block, out.tpt, 16, INFINITY, family, Op::kPrec};
+    static const int kBlocks[2] = {64, 256}, kTpt[3] = {1, 2, 4}, kPerCu[3] = {4, 8, 16};
+    int rc = FV_OK;
+    for (int b : kBlocks)
+        for (int t : kTpt)
+            for (int w : kPerCu) {
+                const PairPlan p = plan_pairs_shape(ctx, n_tgt, n_src, b, t, w);
+                double *dst = static_cast<double *>(
+                    workspace(ctx, pslot, sizeof(double) * (size_t)p.split * n_tgt * Op::kComp));
+                if (!dst) { rc = FV_E_HIP; break; }
+                float ms = 0.0f;
+                for (int rep = 0; rep < 2 && !rc; ++rep) {
+                    if (hipEventRecord(e0, ctx->stream) != hipSuccess) rc = FV_E_HIP;
+                    dispatch_pairs<Op>(ctx, p, a, n_tgt, n_src, dyn_tgt, dyn_src, dst);
+                    if (hipGetLastError() != hipSuccess) rc = FV_E_HIP;
+                    if (hipEventRecord(e1, ctx->stream) != hipSuccess) rc = FV_E_HIP;
+                    if (hipEventSynchronize(e1) != hipSuccess) rc = FV_E_HIP;
+                    if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = FV_E_HIP;
+                }
+                if (rc) break;
+                if (ms < best.ms) best = TuneVal{b, t, w, (double)ms, family, Op::kPrec};
+            }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return set_error(ctx, rc, "autotuning the %s pair sum failed", family);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune[key] = best;
+    }
+    out = plan_pairs_shape(ctx, n_tgt, n_src, best.block, best.tpt, best.per_cu);
+    return FV_OK;
+}
+
+template <class Op>
 static int run_pairs(fv_ctx *ctx, const typename Op::Args &a, int64_t n_tgt, int64_t n_src,
                      const int64_t *dyn_tgt, const int64_t *dyn_src, double *out, int pslot,
                      const char *family)
 {
     if (n_tgt <= 0) return FV_OK;
-    PairPlan p = plan_pairs(ctx, n_tgt, n_src);
+    PairPlan p;
+    int rc = tuned_plan<Op>(ctx, a, n_tgt, n_src, dyn_tgt, dyn_src, pslot, family, p);
+    if (rc) return rc;
     double *dst = out;
     if (p.split > 1) {
         dst = static_cast<double *>(
@@ -136,14 +256,7 @@ static int run_pairs(fv_ctx *ctx, const typename Op::Args &a, int64_t n_tgt, int
     }
     {
         TimedLaunch tl(ctx, family);
-        if (p.block == 256 && p.tpt == 2)
-            launch_pairs<Op, 256, 2>(ctx, p, a, n_tgt, n_src, dyn_tgt, dyn_src, dst);
-        else if (p.block == 256)
-            launch_pairs<Op, 256, 1>(ctx, p, a, n_tgt, n_src, dyn_tgt, dyn_src, dst);
-        else if (p.tpt == 2)
-            launch_pairs<Op, 64, 2>(ctx, p, a, n_tgt, n_src, dyn_tgt, dyn_src, dst);
-        else
-            launch_pairs<Op, 64, 1>(ctx, p, a, n_tgt, n_src, dyn_tgt, dyn_src, dst);
+        dispatch_pairs<Op>(ctx, p, a, n_tgt, n_src, dyn_tgt, dyn_src, dst);
         FV_LAUNCH_CHECK(ctx);
         tl.done();
     }
@@ -415,6 +528,38 @@ int fv_aic_host(fv_ctx *ctx, const double *colloc, const double *normals, int64_
     int rc = fv_aic_dev(ctx, dc, dn, p, dbr, dfr, dfl, dbl, drc, nu, dout);
     if (rc) return rc;
     return fv_memcpy_d2h(ctx, aic, dout, sizeof(double) * p * p);
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int fv_ctx_set_autotune(fv_ctx *ctx, int32_t enabled)
+{
+    if (!ctx) return FV_E_ARG;
+    ctx->autotune = enabled != 0;
+    return FV_OK;
+}
+
+// One line per tuned kind: family (+ "_mixed" for FP32 pair terms), size
+// classes (log2 targets, sources),
+// lanes per workgroup, targets per lane, workgroups per CU, best time (ms).
+int fv_autotune_report(fv_ctx *ctx, char *buf, int64_t cap)
+{
+    if (!ctx || !buf || cap <= 0) return FV_E_ARG;
+    std::lock_guard<std::mutex> lk(fv::g_tune_mu);
+    int64_t used = 0;
+    buf[0] = 0;
+    for (const auto &kv : fv::g_tune) {
+        if (kv.first.device != ctx->device) continue;
+        const int w = snprintf(buf + used, (size_t)(cap - used), "%s%s %d %d %d %d %d %.4f\n",
+                               kv.second.family ? kv.second.family : "?",
+                               kv.second.prec ? "_mixed" : "", kv.first.lt, kv.first.ls,
+                               kv.second.block, kv.second.tpt, kv.second.per_cu, kv.second.ms);
+        if (w < 0 || used + w >= cap) break;
+        used += w;
+    }
+    return FV_OK;
 }
 
 }  // extern "C"

@@ -53,6 +53,7 @@ struct fv_closure_state {
     fv_mlp *mlp = nullptr;
     double h = 2.0, kappa = 4.0, tol = 1e-3, lr = 1e-3;
     double kappa_local = 0.0;          // > 0: cells beyond kappa_local h via local expansions
+    int near_tpt[4][2][2] = {};        // targets per lane of k_near per (mode, kernel, precision)
     int check_every = 10, train_samples = 2048, train_iters = 2;
     int64_t max_cells = 1536;
     // status
@@ -657,12 +658,11 @@ __global__ __launch_bounds__(64) void k_shell_reduce(int64_t nocc, const int32_t
 // Near-field pair sums in sorted order.  The particles are sorted by cell key
 // (ix * ny + iy) * nz + iz, so the cells of one stencil column (dx, dy, dz in
 // [-dz_max, dz_max]) hold one contiguous particle range [pstart[k0],
-// pstart[k1 + 1]).  One wavefront per (occupied target cell, kNearChunk-target
-// chunk, dx slice), two targets per lane, sources streamed 64 at a time
-// through LDS; each slice writes its own partial sums (summed in order later).
-constexpr int kNearChunk = 128;
-
-template <int KERNEL, int PREC, int MODE>
+// pstart[k1 + 1]).  One wavefront per (occupied target cell, 64 TPT-target
+// chunk, dx slice), TPT targets per lane (1, 2 or 4; timed at first use when
+// the context autotunes, else 2), sources streamed 64 at a time through LDS;
+// each slice writes its own partial sums (summed in order later).
+template <int KERNEL, int PREC, int MODE, int TPT>
 __global__ __launch_bounds__(64) void k_near(
     int64_t n, int64_t nocc, const int32_t *__restrict__ occ, const int32_t *__restrict__ cstart,
     const int32_t *__restrict__ cend, const int32_t *__restrict__ pstart,
@@ -677,13 +677,13 @@ __global__ __launch_bounds__(64) void k_near(
     if (o >= nocc) return;
     const int c = occ[o];
     const int tb = cstart[c], te = cend[c];
-    const int t0 = tb + (int)blockIdx.y * kNearChunk;
+    const int t0 = tb + (int)blockIdx.y * (64 * TPT);
     if (t0 >= te) return;
-    typename Op::Tgt tg[2];
-    typename Op::Acc acc[2];
-    bool act[2];
+    typename Op::Tgt tg[TPT];
+    typename Op::Acc acc[TPT];
+    bool act[TPT];
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
+    for (int u = 0; u < TPT; ++u) {
         const int ti = t0 + threadIdx.x + 64 * u;
         act[u] = ti < te;
         const int tl = act[u] ? ti : t0;
@@ -711,14 +711,14 @@ __global__ __launch_bounds__(64) void k_near(
             __syncthreads();
             for (int k = 0; k < cnt; ++k) {
                 const PartRec sr = tile[k];
-                Op::pair(tg[0], sr, acc[0]);
-                Op::pair(tg[1], sr, acc[1]);
+#pragma unroll
+                for (int u = 0; u < TPT; ++u) Op::pair(tg[u], sr, acc[u]);
             }
         }
     }
     double *dst = npart + (int64_t)sl * 3 * n;
 #pragma unroll
-    for (int u = 0; u < 2; ++u)
+    for (int u = 0; u < TPT; ++u)
         if (act[u]) {
             const int64_t ti = t0 + threadIdx.x + 64 * u;
             for (int k = 0; k < 3; ++k) dst[3 * ti + k] = acc[u].v[k];
@@ -1305,13 +1305,20 @@ static int closure_residual(fv_vpm *f, bool use_mlp)
 }
 
 template <int K, int P, int M>
-static void launch_near(fv_closure *cl, fv_ctx *ctx, unsigned chunks)
+static void launch_near(fv_closure *cl, fv_ctx *ctx, int tpt)
 {
     const size_t nst = cl->stencil.size();
-    hipLaunchKernelGGL((k_near<K, P, M>), dim3((unsigned)cl->nocc, chunks, (unsigned)cl->nsl),
-                       dim3(64), 0, ctx->stream, cl->n, cl->nocc, cl->occ, cl->cstart, cl->cend,
-                       cl->pstart, cl->dstencil, cl->dstencil + nst, cl->nx, cl->ny, cl->nz,
-                       cl->sx, cl->sy, cl->sz, cl->sax, cl->say, cl->saz, cl->ssg, cl->npart);
+    const unsigned chunks = (unsigned)ceil_div(cl->maxpop > 0 ? cl->maxpop : 1, 64 * tpt);
+    const dim3 grid((unsigned)cl->nocc, chunks, (unsigned)cl->nsl);
+#define FV_NEAR_T(T)                                                                             \
+    hipLaunchKernelGGL((k_near<K, P, M, T>), grid, dim3(64), 0, ctx->stream, cl->n, cl->nocc,   \
+                       cl->occ, cl->cstart, cl->cend, cl->pstart, cl->dstencil,                 \
+                       cl->dstencil + nst, cl->nx, cl->ny, cl->nz, cl->sx, cl->sy, cl->sz,     \
+                       cl->sax, cl->say, cl->saz, cl->ssg, cl->npart)
+    if (tpt == 4) FV_NEAR_T(4);
+    else if (tpt == 1) FV_NEAR_T(1);
+    else FV_NEAR_T(2);
+#undef FV_NEAR_T
 }
 
 // Far slices: enough source-cell ranges for ~8 wavefronts per SIMD.
@@ -1365,16 +1372,13 @@ static int closure_pass(fv_vpm *f, int mode, int kernel, double *out, bool use_m
     if (rc) return rc;
     const int64_t n = cl->n;
     if (n == 0) return FV_OK;
-    // the largest cell population bounds the target chunks per cell
-    const unsigned chunks = (unsigned)ceil_div(cl->maxpop > 0 ? cl->maxpop : 1, kNearChunk);
-    {
-        TimedLaunch tl(ctx, "closure_near");
-        const int prec = f->precision;
+    const int prec = f->precision;
+    auto near = [&](int tpt) {
 #define FV_NEAR(K, P)                                                   \
     do {                                                                \
-        if (mode == 0) launch_near<K, P, 0>(cl, ctx, chunks);           \
-        else if (mode == 2) launch_near<K, P, 2>(cl, ctx, chunks);      \
-        else launch_near<K, P, 3>(cl, ctx, chunks);                     \
+        if (mode == 0) launch_near<K, P, 0>(cl, ctx, tpt);              \
+        else if (mode == 2) launch_near<K, P, 2>(cl, ctx, tpt);         \
+        else launch_near<K, P, 3>(cl, ctx, tpt);                        \
     } while (0)
         if (kernel == FV_KERNEL_WINCKELMANS) {
             if (prec == FV_PREC_FP64) FV_NEAR(1, 0); else FV_NEAR(1, 1);
@@ -1382,6 +1386,40 @@ static int closure_pass(fv_vpm *f, int mode, int kernel, double *out, bool use_m
             if (prec == FV_PREC_FP64) FV_NEAR(0, 0); else FV_NEAR(0, 1);
         }
 #undef FV_NEAR
+    };
+    int &tsel = cl->near_tpt[mode & 3][kernel == FV_KERNEL_WINCKELMANS][prec == FV_PREC_MIXED];
+    if (tsel == 0) {
+        tsel = 2;
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (ctx->autotune && hipStreamIsCapturing(ctx->stream, &cs) == hipSuccess &&
+            cs == hipStreamCaptureStatusNone) {
+            // time 1, 2 and 4 targets per lane once (each twice, the second counted)
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            FV_HIP_CHECK(ctx, hipEventCreate(&e0));
+            FV_HIP_CHECK(ctx, hipEventCreate(&e1));
+            float best = INFINITY;
+            for (int t : {1, 2, 4}) {
+                float ms = 0.0f;
+                for (int rep = 0; rep < 2; ++rep) {
+                    FV_HIP_CHECK(ctx, hipEventRecord(e0, ctx->stream));
+                    near(t);
+                    FV_LAUNCH_CHECK(ctx);
+                    FV_HIP_CHECK(ctx, hipEventRecord(e1, ctx->stream));
+                    FV_HIP_CHECK(ctx, hipEventSynchronize(e1));
+                    FV_HIP_CHECK(ctx, hipEventElapsedTime(&ms, e0, e1));
+                }
+                if (ms < best) {
+                    best = ms;
+                    tsel = t;
+                }
+            }
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+        }
+    }
+    {
+        TimedLaunch tl(ctx, "closure_near");
+        near(tsel);
         FV_LAUNCH_CHECK(ctx);
         tl.done();
     }

@@ -39,6 +39,7 @@ struct fv_ctx {
     std::vector<hipEvent_t> event_pool;   // recycled events
     hipEvent_t user_events[8] = {};
     int cus = 0;
+    bool autotune = false;                // pair-sum tile shapes timed at first use
 };
 
 namespace fv {

@@ -9,13 +9,17 @@ namespace fv {
 
 struct PairPlan {
     int block;      // 64 or 256 lanes per workgroup
-    int tpt;        // targets per lane (1 or 2)
+    int tpt;        // targets per lane (1, 2 or 4)
     int64_t gx;     // target blocks
     int split;      // source chunks (blockIdx.y)
     int64_t chunk;  // sources per chunk (multiple of block)
 };
 
+// The default shape (by problem size) and an explicit one: `per_cu`
+// workgroups per CU are aimed for by splitting the sources.
 PairPlan plan_pairs(fv_ctx *ctx, int64_t n_tgt, int64_t n_src);
+PairPlan plan_pairs_shape(fv_ctx *ctx, int64_t n_tgt, int64_t n_src, int block, int tpt,
+                          int per_cu);
 
 // Fills n doubles with v (a kernel, so it is safe under graph capture).
 int fill(fv_ctx *ctx, double *p, int64_t n, double v);

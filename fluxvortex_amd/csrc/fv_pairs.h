@@ -259,6 +259,7 @@ __device__ __forceinline__ PartRec make_part_rec(int kernel, double x, double y,
 template <int PREC>
 struct SegOp {
     static constexpr int kComp = 3;
+    static constexpr int kPrec = PREC;
     struct Args {
         const double *pts;    // AoS (n, 3)
         const SegRec *segs;
@@ -306,6 +307,7 @@ struct PartTgtArgs {
 template <int KERNEL, int PREC, int MODE>
 struct PartOp {
     static constexpr int kComp = (MODE == 1) ? 9 : 3;
+    static constexpr int kPrec = PREC;
     struct Args {
         PartSrcArgs src;
         PartTgtArgs tgt;

@@ -71,6 +71,17 @@ int fv_memcpy_d2h(fv_ctx *ctx, void *dst, const void *src, size_t bytes);
  * synchronises and returns the launch count and summed duration.  Enabling
  * clears earlier records. */
 int fv_ctx_set_timing(fv_ctx *ctx, int enabled);
+/* Project-written autotuner of the pair-sum tile shapes (64 / 256 lanes,
+ * 1 / 2 / 4 targets per lane, 4 / 8 / 16 workgroups per CU): when enabled,
+ * the first pair sum of each kind (operator, device, power-of-two size class
+ * of targets and sources) times every shape with HIP events (synchronising
+ * once) and every later call of that kind uses the fastest.  Shapes only
+ * change how sources are split, so results differ at round-off level from
+ * the default shape.  Off by default (no counterpart in the reference). */
+int fv_ctx_set_autotune(fv_ctx *ctx, int32_t enabled);
+/* The tuned kinds, one text line each: family log2_targets log2_sources
+ * lanes targets_per_lane workgroups_per_cu best_ms. */
+int fv_autotune_report(fv_ctx *ctx, char *buf, int64_t cap);
 int fv_ctx_kernel_stats(fv_ctx *ctx, const char *family, int64_t *launches, double *total_ms);
 /* Eight user events on the context stream (which = 0..7). */
 int fv_ctx_event_record(fv_ctx *ctx, int32_t which);
