@@ -40,10 +40,11 @@ FLOPS = {"particle_vel": 31, "particle_stretch": 66, "particle_curl": 72,
 PEAK_FP64 = 78.6e12
 PEAK_FP32 = 157.3e12
 # HBM bytes per launch of the stretching pair sum at the default 20k workload
-# (20 096 particles at the timed steps), from two rocprofv3 counter passes
+# (20 096 particles at the timed steps, 64 lanes x 2 targets, 27 source slices),
+# from two rocprofv3 counter passes
 # (FETCH_SIZE x 2.00, WRITE_SIZE x 1.00, scaled by a plain 512 MiB copy):
 # profiles/k_pairs_stretch_heaving_wing_20k_pmc.md
-TRAFFIC_STRETCH = {"particles": 20096, "bytes": (4181.0 * 2.0 + 6586.1) * 1024}
+TRAFFIC_STRETCH = {"particles": 20096, "bytes": (4182.5 * 2.0 + 12701.8) * 1024}
 
 
 def log(msg):
