@@ -689,7 +689,7 @@ __global__ __launch_bounds__(64) void k_near(
         const int tl = act[u] ? ti : t0;
         tg[u].x = sx[tl]; tg[u].y = sy[tl]; tg[u].z = sz[tl];
         tg[u].ax = sax[tl]; tg[u].ay = say[tl]; tg[u].az = saz[tl];
-        for (int k = 0; k < 3; ++k) acc[u].v[k] = 0.0;
+        Op::init(acc[u]);
     }
     const int cx = c / (ny * nz), cy = (c / nz) % ny, cz = c % nz;
     const int sl = blockIdx.z;
@@ -705,8 +705,8 @@ __global__ __launch_bounds__(64) void k_near(
             __syncthreads();
             if (threadIdx.x < cnt) {
                 const int j = base + threadIdx.x;
-                tile[threadIdx.x] = make_part_rec(KERNEL, sx[j], sy[j], sz[j], sax[j], say[j],
-                                                  saz[j], ssg[j]);
+                tile[threadIdx.x] = make_part_rec<PREC>(KERNEL, sx[j], sy[j], sz[j], sax[j],
+                                                        say[j], saz[j], ssg[j]);
             }
             __syncthreads();
             for (int k = 0; k < cnt; ++k) {
@@ -718,11 +718,13 @@ __global__ __launch_bounds__(64) void k_near(
     }
     double *dst = npart + (int64_t)sl * 3 * n;
 #pragma unroll
-    for (int u = 0; u < TPT; ++u)
+    for (int u = 0; u < TPT; ++u) {
+        Op::finish(tg[u], acc[u]);
         if (act[u]) {
             const int64_t ti = t0 + threadIdx.x + 64 * u;
             for (int k = 0; k < 3; ++k) dst[3 * ti + k] = acc[u].v[k];
         }
+    }
 }
 
 // Population of every cell (empty cells 0) and one trailing zero, for the

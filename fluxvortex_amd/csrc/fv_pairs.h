@@ -16,6 +16,7 @@
 namespace fv {
 
 constexpr double kInv4Pi = 0.07957747154594767;   // warp_kernels.py:20
+constexpr double kFourPi = 12.566370614359172;
 constexpr double kFourLamb = 5.02572;             // warp_kernels.py:18 (4 * 1.25643)
 constexpr double kSquire = 1.0e-4;                // warp_kernels.py:19
 constexpr double kSqrt2OverPi = 0.7978845608028654;   // warp_vpm.py:14
@@ -31,12 +32,25 @@ struct alignas(16) SegRec {
     double c1, c2, tol, pad;
 };
 
-// One vortex particle with per-source constants:
-//   Winckelmans:  k0 = 1/sigma^2, k1 = -1/(4 pi sigma^3), k2 = 1/(4 pi sigma^5)
-//   Gaussian-erf: k0 = 1/sigma,   k1 = sigma,             k2 = 1/sigma^2
+// One vortex particle with per-source constants, built when a tile is staged:
+//   Winckelmans:  k0 = 1/sigma^2, k1 = -1/(128 pi sigma^3), k2 = 1.5 k1,
+//                 k3 = 3/(512 pi sigma^5), k4 = 2.5 k3, k5 = sigma^2
+//                 (k1..k4 carry the 2^-5 / 2^-7 of the doubled inverse square
+//                 root Z = 2 t, see part_factors_f64)
+//   Gaussian-erf: k0 = 1/sigma, k1 = sigma, k2 = 1/sigma^2, k3 = k4 = k5 = 0
+// Mixed-precision records (make_part_rec<1>) hold the primed strengths
+// ax..az = -alpha / 4pi of part_factors_mixed, keep k0 and k2 for the
+// Gaussian, and overlay k3..k5 with floats: Winckelmans f[3] = sigma^2,
+// f[4] = 2.5 sigma^2, f[5] = 10.5 sigma^2.
 struct alignas(16) PartRec {
     double x, y, z, ax, ay, az;
-    double k0, k1, k2, pad;
+    double k0, k1, k2;
+    union {
+        struct {
+            double k3, k4, k5;
+        };
+        float f[6];
+    };
 };
 
 struct Pt3 {
@@ -53,6 +67,15 @@ __device__ __forceinline__ double rsqrt_refined(double a)
     const double y = __builtin_amdgcn_rsq(a);
     const double r = fma(-a * y, y, 1.0);
     return fma(y, 0.5 * r, y);
+}
+
+// 2/sqrt(a) for normal a > 0: the Newton step of rsqrt_refined without its
+// halving, y (3 - a y^2), three operations after the estimate.  Callers fold
+// the factor 2 into their constants.
+__device__ __forceinline__ double rsqrt2_refined(double a)
+{
+    const double y = __builtin_amdgcn_rsq(a);
+    return y * fma(-(a * y), y, 3.0);
 }
 
 // 1/d for normal d: the v_rcp_f64 estimate refined by one Newton step
@@ -137,98 +160,114 @@ __device__ __forceinline__ void seg_pair_mixed(double px, double py, double pz, 
 }
 
 // ----------------------------------------------------- particle kernels
-// Smoothing-kernel factors for one pair at squared distance r2 (> 0):
+// Smoothing-kernel factors for one pair at squared distance r2:
 //   CG = -(1/4pi) g(rho)/r^3                       (velocity factor)
 //   CH = -(1/4pi) (g'(rho)/(sigma r) - 3 g/r^2)/r^3  (gradient factor)
 // so that u += CG (dx x alpha) and
 //   J_ij += CH (dx x alpha)_i dx_j + CG eps_ijl alpha_l     (kernel.py:163-188).
-// Winckelmans (kernel.py:42-48): with t = (rho^2+1)^(-1/2),
-//   g/r^3 = (rho^2+2.5) t^5 / sigma^3,  (g'/(sigma r) - 3g/r^2)/r^3 = -(3 rho^2 + 10.5) t^7 / sigma^5.
+// Pairs with keep == false (the self pair and coincident particles) get
+// CG = CH = 0 by selection, so non-finite intermediates never leak.
+// Winckelmans (kernel.py:42-48): with q = rho^2 + 1 and t = q^(-1/2),
+//   g/r^3 = (q + 1.5) t^5 / sigma^3,  (g'/(sigma r) - 3g/r^2)/r^3 = -(3q + 7.5) t^7 / sigma^5;
+// evaluated with Z = 2t from rsqrt2_refined, Z^5 = 32 t^5 and Z^7 = 128 t^7
+// folded into k1..k4: CG = (k1 q + k2) Z^5, CH = (k3 q + k4) Z^7.
 // Gaussian-erf (kernel.py:26-31): g = erf(rho/sqrt2) - sqrt(2/pi) rho e^(-rho^2/2),
 //   g' = sqrt(2/pi) rho^2 e^(-rho^2/2).
 template <int KERNEL, bool GRAD>
-__device__ __forceinline__ void part_factors_f64(double r2, const PartRec &s, double &CG,
-                                                 double &CH)
+__device__ __forceinline__ void part_factors_f64(double r2, bool keep, const PartRec &s,
+                                                 double &CG, double &CH)
 {
     if (KERNEL == 1) {
-        const double rho2 = r2 * s.k0;
-        const double t = rsqrt_refined(rho2 + 1.0);
-        const double t2 = t * t;
-        const double t5 = t2 * t2 * t;
-        CG = s.k1 * (rho2 + 2.5) * t5;
-        if (GRAD) CH = s.k2 * fma(3.0, rho2, 10.5) * (t5 * t2);
-    } else if (r2 * s.k2 > 100.0) {
-        // rho > 10: erf(rho/sqrt2) == 1 and sqrt(2/pi) rho e^(-rho^2/2) < 2e-21 in
-        // FP64, so g == 1 and g' == 0 to the last bit of the full formula.
-        const double ir = rsqrt_refined(r2);
-        const double ir2 = ir * ir;
-        const double ir3 = ir2 * ir;
-        CG = -kInv4Pi * ir3;
-        if (GRAD) CH = 3.0 * kInv4Pi * ir3 * ir2;
+        const double q = fma(r2, s.k0, 1.0);
+        const double Z = rsqrt2_refined(q);
+        const double Z2 = Z * Z;
+        double Z5 = Z2 * Z2 * Z;
+        Z5 = keep ? Z5 : 0.0;
+        CG = fma(q, s.k1, s.k2) * Z5;
+        if (GRAD) CH = fma(q, s.k3, s.k4) * (Z5 * Z2);
     } else {
-        const double r = sqrt(r2);
-        const double rho = r * s.k0;
-        const double ev = exp(-0.5 * rho * rho);
-        const double g = erf(rho * kInvSqrt2) - kSqrt2OverPi * rho * ev;
-        const double ir = 1.0 / r;
-        const double ir3 = ir * ir * ir;
-        CG = -kInv4Pi * g * ir3;
-        if (GRAD) {
-            const double dgdr = kSqrt2OverPi * rho * rho * ev;
-            const double aux = dgdr * (s.k0 * ir) - 3.0 * g * (ir * ir);
-            CH = -kInv4Pi * aux * ir3;
+        if (r2 * s.k2 > 100.0) {
+            // rho > 10: erf(rho/sqrt2) == 1 and sqrt(2/pi) rho e^(-rho^2/2) < 2e-21 in
+            // FP64, so g == 1 and g' == 0 to the last bit of the full formula:
+            // CG = -ir^3 / 4pi, CH = 3 ir^5 / 4pi, with ir = Z / 2.
+            const double Z = rsqrt2_refined(r2);
+            const double Z2 = Z * Z;
+            const double Z3 = Z2 * Z;
+            CG = (-kInv4Pi / 8.0) * Z3;
+            if (GRAD) CH = (3.0 * kInv4Pi / 32.0) * (Z3 * Z2);
+        } else {
+            const double ir = 0.5 * rsqrt2_refined(r2);
+            const double r = r2 * ir;
+            const double rho = r * s.k0;
+            const double ev = exp(-0.5 * rho * rho);
+            const double g = erf(rho * kInvSqrt2) - kSqrt2OverPi * rho * ev;
+            const double ir3 = ir * ir * ir;
+            CG = -kInv4Pi * g * ir3;
+            if (GRAD) {
+                const double dgdr = kSqrt2OverPi * rho * rho * ev;
+                const double aux = dgdr * (s.k0 * ir) - 3.0 * g * (ir * ir);
+                CH = -kInv4Pi * aux * ir3;
+            }
         }
+        CG = keep ? CG : 0.0;
+        if (GRAD) CH = keep ? CH : 0.0;
     }
 }
 
-// FP32 factors; the Gaussian pairs with rho < 0.25 (where g cancels in FP32)
-// are evaluated in FP64 instead.
+// FP32 factors, primed: CG' = -4pi CG and CH' = -4pi CH, which pair with the
+// primed strengths a' = -a / 4pi of the record (CG' a' = CG a).  The Gaussian
+// pairs with rho < 0.25 (where g cancels in FP32) are evaluated in FP64
+// instead, from the FP64 differences.
 template <int KERNEL, bool GRAD>
-__device__ __forceinline__ void part_factors_mixed(float r2, double r2d, const PartRec &s,
-                                                   float &CG, float &CH)
+__device__ __forceinline__ void part_factors_mixed(float r2, bool keep, double dxd, double dyd,
+                                                   double dzd, const PartRec &s, float &CG,
+                                                   float &CH)
 {
     if (KERNEL == 1) {
-        // The same factors in a form with no 1/sigma^n (which overflows FP32
-        // once a stretched core shrinks towards its 1e-6 floor): with
+        // The factors in a form with no 1/sigma^n (which overflows FP32 once a
+        // stretched core shrinks towards its 1e-6 floor): with
         // u = (r^2 + sigma^2)^(-1/2),
         //   (rho^2+2.5) t^5 / sigma^3      = (r^2 + 2.5 sigma^2) u^2 u^3,
         //   (3 rho^2+10.5) t^7 / sigma^5   = (3 r^2 + 10.5 sigma^2) u^2 u^5,
-        // every factor bounded for any sigma.
-        const float sg2 = (float)s.pad;
-        const float u = __builtin_amdgcn_rsqf(r2 + sg2);
+        // every partial product bounded for any sigma.
+        float u = __builtin_amdgcn_rsqf(r2 + s.f[3]);   // finite: sigma > 0
+        u = keep ? u : 0.0f;
         const float u2 = u * u;
         const float u3 = u2 * u;
-        CG = -(float)kInv4Pi * (fmaf(2.5f, sg2, r2) * u2) * u3;
-        if (GRAD) CH = (float)kInv4Pi * (fmaf(10.5f, sg2, 3.0f * r2) * u2) * (u3 * u2);
+        CG = ((r2 + s.f[4]) * u2) * u3;
+        if (GRAD) CH = -(fmaf(3.0f, r2, s.f[5]) * u2) * (u3 * u2);
     } else {
         const float ir = __builtin_amdgcn_rsqf(r2);
         if (r2 * (float)s.k2 > 100.0f) {   // g == 1, g' == 0 (see the FP64 path)
             const float ir3 = ir * ir * ir;
-            CG = -(float)kInv4Pi * ir3;
-            if (GRAD) CH = 3.0f * (float)kInv4Pi * ir3 * (ir * ir);
-            return;
+            CG = ir3;
+            if (GRAD) CH = -3.0f * ir3 * (ir * ir);
+        } else {
+            const float r = r2 * ir;
+            const float rho = r * (float)s.k0;
+            if (rho < 0.25f) {
+                const double r2d = dxd * dxd + dyd * dyd + dzd * dzd;
+                double cg, ch = 0.0;
+                part_factors_f64<0, GRAD>(r2d, true, s, cg, ch);
+                CG = (float)(-kFourPi * cg);
+                if (GRAD) CH = (float)(-kFourPi * ch);
+            } else {
+                const float ev = __expf(-0.5f * rho * rho);
+                const float g = erff(rho * (float)kInvSqrt2) - (float)kSqrt2OverPi * rho * ev;
+                const float ir3 = ir * ir * ir;
+                CG = g * ir3;
+                if (GRAD) {
+                    const float dgdr = (float)kSqrt2OverPi * rho * rho * ev;
+                    CH = (dgdr * ((float)s.k0 * ir) - 3.0f * g * (ir * ir)) * ir3;
+                }
+            }
         }
-        const float r = r2 * ir;
-        const float rho = r * (float)s.k0;
-        if (rho < 0.25f) {
-            double cg, ch = 0.0;
-            part_factors_f64<0, GRAD>(r2d, s, cg, ch);
-            CG = (float)cg;
-            if (GRAD) CH = (float)ch;
-            return;
-        }
-        const float ev = __expf(-0.5f * rho * rho);
-        const float g = erff(rho * (float)kInvSqrt2) - (float)kSqrt2OverPi * rho * ev;
-        const float ir3 = ir * ir * ir;
-        CG = -(float)kInv4Pi * g * ir3;
-        if (GRAD) {
-            const float dgdr = (float)kSqrt2OverPi * rho * rho * ev;
-            const float aux = dgdr * ((float)s.k0 * ir) - 3.0f * g * (ir * ir);
-            CH = -(float)kInv4Pi * aux * ir3;
-        }
+        CG = keep ? CG : 0.0f;
+        if (GRAD) CH = keep ? CH : 0.0f;
     }
 }
 
+template <int PREC>
 __device__ __forceinline__ PartRec make_part_rec(int kernel, double x, double y, double z,
                                                  double ax, double ay, double az, double sigma)
 {
@@ -240,20 +279,34 @@ __device__ __forceinline__ PartRec make_part_rec(int kernel, double x, double y,
         const double is2 = is * is;
         const double is3 = is2 * is;
         r.k0 = is2;
-        r.k1 = -kInv4Pi * is3;
-        r.k2 = kInv4Pi * is3 * is2;
-        r.pad = sigma * sigma;   // the mixed-precision factors' form
+        r.k1 = (-kInv4Pi / 32.0) * is3;
+        r.k2 = 1.5 * r.k1;
+        if (PREC == 0) {
+            r.k3 = (3.0 * kInv4Pi / 128.0) * (is3 * is2);
+            r.k4 = 2.5 * r.k3;
+            r.k5 = 0.0;
+        }
     } else {
         r.k0 = 1.0 / sigma;
         r.k1 = sigma;
         r.k2 = r.k0 * r.k0;
-        r.pad = 0.0;
+        if (PREC == 0) r.k3 = r.k4 = r.k5 = 0.0;
+    }
+    if (PREC == 1) {
+        r.ax = -kInv4Pi * ax;
+        r.ay = -kInv4Pi * ay;
+        r.az = -kInv4Pi * az;
+        const double sg2 = sigma * sigma;
+        r.f[3] = (float)sg2;
+        r.f[4] = (float)(2.5 * sg2);
+        r.f[5] = (float)(10.5 * sg2);
     }
     return r;
 }
 
 // ------------------------------------------------------------ pair ops
-// Each op defines: Args, Src, Tgt, kComp, load_src, load_tgt, pair, store.
+// Each op defines: Args, Src, Tgt, Acc, kComp, load_src, load_tgt, init (zero
+// accumulators), pair, finish (per-target epilogue before the sums are stored).
 
 // Segment sums at points (collapsed Biot-Savart).
 template <int PREC>
@@ -274,6 +327,8 @@ struct SegOp {
     {
         return Pt3{a.pts[3 * i], a.pts[3 * i + 1], a.pts[3 * i + 2]};
     }
+    __device__ static void init(Acc &c) { c.v[0] = c.v[1] = c.v[2] = 0.0; }
+    __device__ static void finish(const Tgt &, Acc &) {}
     __device__ static void pair(const Tgt &t, const Src &s, Acc &c)
     {
         if (PREC == 0)
@@ -290,10 +345,12 @@ struct PartSrcArgs {
     int kernel;
 };
 
+template <int PREC>
 __device__ __forceinline__ PartRec load_part(const PartSrcArgs &p, int64_t j)
 {
     const int64_t k = j * p.stride;
-    return make_part_rec(p.kernel, p.x[k], p.y[k], p.z[k], p.ax[k], p.ay[k], p.az[k], p.sig[j]);
+    return make_part_rec<PREC>(p.kernel, p.x[k], p.y[k], p.z[k], p.ax[k], p.ay[k], p.az[k],
+                               p.sig[j]);
 }
 
 struct PartTgtArgs {
@@ -316,10 +373,13 @@ struct PartOp {
     struct Tgt {
         double x, y, z, ax, ay, az;
     };
+    // v: the sums; g: sum of CG alpha_s (stretching / Jacobian), whose cross
+    // product with the target's alpha finish() adds once per target.
     struct Acc {
         double v[kComp];
+        double g[3];
     };
-    __device__ static Src load_src(const Args &a, int64_t j) { return load_part(a.src, j); }
+    __device__ static Src load_src(const Args &a, int64_t j) { return load_part<PREC>(a.src, j); }
     __device__ static Tgt load_tgt(const Args &a, int64_t i)
     {
         const int64_t k = i * a.tgt.stride;
@@ -332,98 +392,89 @@ struct PartOp {
         }
         return t;
     }
+    __device__ static void init(Acc &c)
+    {
+#pragma unroll
+        for (int k = 0; k < kComp; ++k) c.v[k] = 0.0;
+        c.g[0] = c.g[1] = c.g[2] = 0.0;
+    }
+    // S = sum CH (dx . a_t)(dx x a_s) + a_t x sum CG a_s  (stretching);
+    // J_ij = sum CH (dx x a_s)_i dx_j + eps_ijl (sum CG a_s)_l  (Jacobian).
+    __device__ static void finish(const Tgt &t, Acc &c)
+    {
+        if (MODE == 2) {
+            c.v[0] += t.ay * c.g[2] - t.az * c.g[1];
+            c.v[1] += t.az * c.g[0] - t.ax * c.g[2];
+            c.v[2] += t.ax * c.g[1] - t.ay * c.g[0];
+        } else if (MODE == 1) {
+            c.v[1] += c.g[2]; c.v[2] -= c.g[1];
+            c.v[3] -= c.g[2]; c.v[5] += c.g[0];
+            c.v[6] += c.g[1]; c.v[7] -= c.g[0];
+        }
+    }
     __device__ static void pair(const Tgt &t, const Src &s, Acc &c)
     {
         const double dxd = t.x - s.x, dyd = t.y - s.y, dzd = t.z - s.z;
-        const double r2d = dxd * dxd + dyd * dyd + dzd * dzd;
-        const bool keep = r2d >= 1.0e-24;   // self pair excluded, warp_vpm.py:37-38
         if (PREC == 0) {
+            const double r2d = dxd * dxd + dyd * dyd + dzd * dzd;
+            const bool keep = r2d >= 1.0e-24;   // self pair excluded, warp_vpm.py:37-38
             double CG, CH = 0.0;
-            part_factors_f64<KERNEL, MODE != 0>(r2d, s, CG, CH);
-            CG = keep ? CG : 0.0;
-            CH = keep ? CH : 0.0;
-            const double cx = dyd * s.az - dzd * s.ay;
-            const double cy = dzd * s.ax - dxd * s.az;
-            const double cz = dxd * s.ay - dyd * s.ax;
-            accumulate(dxd, dyd, dzd, cx, cy, cz, CG, CH, s.ax, s.ay, s.az, t, c);
+            part_factors_f64<KERNEL, MODE != 0>(r2d, keep, s, CG, CH);
+            accumulate(dxd, dyd, dzd, r2d, CG, CH, s, t, c);
         } else {
-            const float dx = (float)dxd, dy = (float)dyd, dz = (float)dzd;
-            const float r2 = dx * dx + dy * dy + dz * dz;
-            float CG, CH = 0.0f;
-            part_factors_mixed<KERNEL, MODE != 0>(r2, r2d, s, CG, CH);
-            CG = keep ? CG : 0.0f;
-            CH = keep ? CH : 0.0f;
-            const float fax = (float)s.ax, fay = (float)s.ay, faz = (float)s.az;
-            const float cx = dy * faz - dz * fay;
-            const float cy = dz * fax - dx * faz;
-            const float cz = dx * fay - dy * fax;
-            accumulate_mixed(dx, dy, dz, cx, cy, cz, CG, CH, fax, fay, faz, t, c);
+            // FP32 kernel factors (the rsqrt / erf / exp work), FP64 geometry and
+            // sums: on gfx950 without packed FP32 an FP32 operation issues at the
+            // FP64 rate, so FP32 pays only where it removes work (no Newton
+            // step, FP32 erf / exp), and FP64 sums keep exactly cancelling pair
+            // terms (coincident particles of opposite strength) cancelling.
+            const double r2d = dxd * dxd + dyd * dyd + dzd * dzd;
+            const bool keep = r2d >= 1.0e-24;
+            float CGf, CHf = 0.0f;
+            part_factors_mixed<KERNEL, MODE != 0>((float)r2d, keep, dxd, dyd, dzd, s, CGf, CHf);
+            accumulate(dxd, dyd, dzd, r2d, (double)CGf, (double)CHf, s, t, c);
         }
     }
-    __device__ static void accumulate(double dx, double dy, double dz, double cx, double cy,
-                                      double cz, double CG, double CH, double sax, double say,
-                                      double saz, const Tgt &t, Acc &c)
+    __device__ static void accumulate(double dx, double dy, double dz, double r2, double CG,
+                                      double CH, const Src &s, const Tgt &t, Acc &c)
     {
+        if (MODE == 3) {
+            // omega = CH dx x (dx x a_s) - 2 CG a_s = CH (dx . a_s) dx - (CH r^2 + 2 CG) a_s
+            const double a = CH * (dx * s.ax + dy * s.ay + dz * s.az);
+            const double b = fma(CH, r2, CG + CG);
+            c.v[0] = fma(-b, s.ax, fma(a, dx, c.v[0]));
+            c.v[1] = fma(-b, s.ay, fma(a, dy, c.v[1]));
+            c.v[2] = fma(-b, s.az, fma(a, dz, c.v[2]));
+            return;
+        }
+        const double cx = dy * s.az - dz * s.ay;
+        const double cy = dz * s.ax - dx * s.az;
+        const double cz = dx * s.ay - dy * s.ax;
         if (MODE == 0) {
             c.v[0] = fma(CG, cx, c.v[0]);
             c.v[1] = fma(CG, cy, c.v[1]);
             c.v[2] = fma(CG, cz, c.v[2]);
         } else if (MODE == 1) {
             const double hx = CH * cx, hy = CH * cy, hz = CH * cz;
-            const double gx = CG * sax, gy = CG * say, gz = CG * saz;
             c.v[0] = fma(hx, dx, c.v[0]);
-            c.v[1] = fma(hx, dy, c.v[1]) + gz;
-            c.v[2] = fma(hx, dz, c.v[2]) - gy;
-            c.v[3] = fma(hy, dx, c.v[3]) - gz;
+            c.v[1] = fma(hx, dy, c.v[1]);
+            c.v[2] = fma(hx, dz, c.v[2]);
+            c.v[3] = fma(hy, dx, c.v[3]);
             c.v[4] = fma(hy, dy, c.v[4]);
-            c.v[5] = fma(hy, dz, c.v[5]) + gx;
-            c.v[6] = fma(hz, dx, c.v[6]) + gy;
-            c.v[7] = fma(hz, dy, c.v[7]) - gx;
+            c.v[5] = fma(hy, dz, c.v[5]);
+            c.v[6] = fma(hz, dx, c.v[6]);
+            c.v[7] = fma(hz, dy, c.v[7]);
             c.v[8] = fma(hz, dz, c.v[8]);
-        } else if (MODE == 2) {
-            // S_i = CH c_i (dx . a_t) + CG (a_t x a_s)_i
-            const double d = dx * t.ax + dy * t.ay + dz * t.az;
-            const double h = CH * d;
-            c.v[0] = fma(h, cx, c.v[0]) + CG * (t.ay * saz - t.az * say);
-            c.v[1] = fma(h, cy, c.v[1]) + CG * (t.az * sax - t.ax * saz);
-            c.v[2] = fma(h, cz, c.v[2]) + CG * (t.ax * say - t.ay * sax);
+            c.g[0] = fma(CG, s.ax, c.g[0]);
+            c.g[1] = fma(CG, s.ay, c.g[1]);
+            c.g[2] = fma(CG, s.az, c.g[2]);
         } else {
-            // omega = CH (dx x c) - 2 CG a_s
-            c.v[0] = fma(CH, dy * cz - dz * cy, c.v[0]) - 2.0 * CG * sax;
-            c.v[1] = fma(CH, dz * cx - dx * cz, c.v[1]) - 2.0 * CG * say;
-            c.v[2] = fma(CH, dx * cy - dy * cx, c.v[2]) - 2.0 * CG * saz;
-        }
-    }
-    __device__ static void accumulate_mixed(float dx, float dy, float dz, float cx, float cy,
-                                            float cz, float CG, float CH, float sax, float say,
-                                            float saz, const Tgt &t, Acc &c)
-    {
-        if (MODE == 0) {
-            c.v[0] += (double)(CG * cx);
-            c.v[1] += (double)(CG * cy);
-            c.v[2] += (double)(CG * cz);
-        } else if (MODE == 1) {
-            const float hx = CH * cx, hy = CH * cy, hz = CH * cz;
-            const float gx = CG * sax, gy = CG * say, gz = CG * saz;
-            c.v[0] += (double)(hx * dx);
-            c.v[1] += (double)(fmaf(hx, dy, gz));
-            c.v[2] += (double)(fmaf(hx, dz, -gy));
-            c.v[3] += (double)(fmaf(hy, dx, -gz));
-            c.v[4] += (double)(hy * dy);
-            c.v[5] += (double)(fmaf(hy, dz, gx));
-            c.v[6] += (double)(fmaf(hz, dx, gy));
-            c.v[7] += (double)(fmaf(hz, dy, -gx));
-            c.v[8] += (double)(hz * dz);
-        } else if (MODE == 2) {
-            const float tax = (float)t.ax, tay = (float)t.ay, taz = (float)t.az;
-            const float h = CH * (dx * tax + dy * tay + dz * taz);
-            c.v[0] += (double)fmaf(h, cx, CG * (tay * saz - taz * say));
-            c.v[1] += (double)fmaf(h, cy, CG * (taz * sax - tax * saz));
-            c.v[2] += (double)fmaf(h, cz, CG * (tax * say - tay * sax));
-        } else {
-            c.v[0] += (double)fmaf(CH, dy * cz - dz * cy, -2.0f * CG * sax);
-            c.v[1] += (double)fmaf(CH, dz * cx - dx * cz, -2.0f * CG * say);
-            c.v[2] += (double)fmaf(CH, dx * cy - dy * cx, -2.0f * CG * saz);
+            const double h = CH * (dx * t.ax + dy * t.ay + dz * t.az);
+            c.v[0] = fma(h, cx, c.v[0]);
+            c.v[1] = fma(h, cy, c.v[1]);
+            c.v[2] = fma(h, cz, c.v[2]);
+            c.g[0] = fma(CG, s.ax, c.g[0]);
+            c.g[1] = fma(CG, s.ay, c.g[1]);
+            c.g[2] = fma(CG, s.az, c.g[2]);
         }
     }
 };
@@ -454,8 +505,7 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(typename Op::Args a, int64_t n_
     for (int k = 0; k < TPT; ++k) {
         const int64_t i = i0 + threadIdx.x + k * BLOCK;
         t[k] = Op::load_tgt(a, i < n_tgt ? i : i0);
-#pragma unroll
-        for (int c = 0; c < Op::kComp; ++c) acc[k].v[c] = 0.0;
+        Op::init(acc[k]);
     }
     const int64_t s0 = (int64_t)blockIdx.y * chunk;
     const int64_t s1 = (s0 + chunk < n_src) ? s0 + chunk : n_src;
@@ -474,6 +524,7 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(typename Op::Args a, int64_t n_
 #pragma unroll
     for (int k = 0; k < TPT; ++k) {
         const int64_t i = i0 + threadIdx.x + k * BLOCK;
+        Op::finish(t[k], acc[k]);
         if (i < n_tgt) {
             double *o = partial + ((int64_t)blockIdx.y * n_tgt_stride + i) * Op::kComp;
 #pragma unroll

@@ -180,7 +180,7 @@ __device__ void sw_induce_body(const SwInst &I, const double *__restrict__ S, do
     using Op = PartOp<KERNEL, PREC, 0>;
     typename Op::Tgt tg;
     typename Op::Acc acc;
-    for (int k = 0; k < 3; ++k) acc.v[k] = 0.0;
+    Op::init(acc);
     __syncthreads();
     if (tp < P) {
         tg.x = tgt[3 * tp]; tg.y = tgt[3 * tp + 1]; tg.z = tgt[3 * tp + 2];
@@ -192,13 +192,14 @@ __device__ void sw_induce_body(const SwInst &I, const double *__restrict__ S, do
         __syncthreads();
         if (t < cnt) {
             const int64_t j = b0 + t;
-            tile[t] = make_part_rec(KERNEL, pos[3 * j], pos[3 * j + 1], pos[3 * j + 2],
-                                    al[3 * j], al[3 * j + 1], al[3 * j + 2], sg[j]);
+            tile[t] = make_part_rec<PREC>(KERNEL, pos[3 * j], pos[3 * j + 1], pos[3 * j + 2],
+                                          al[3 * j], al[3 * j + 1], al[3 * j + 2], sg[j]);
         }
         __syncthreads();
         if (tp < P)
             for (int k = grp; k < cnt; k += kSwBlock / 64) Op::pair(tg, tile[k], acc);
     }
+    Op::finish(tg, acc);
     if (tp < P)
         for (int c = 0; c < 3; ++c) red[(grp * kSwMaxP + tp) * 3 + c] = acc.v[c];
     __syncthreads();

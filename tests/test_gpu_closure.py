@@ -130,6 +130,29 @@ def test_closure_modes_match_direct_sums(ctx, kernel):
         assert relerr(got, ref) < 1e-12, mode
 
 
+@pytest.mark.parametrize("kernel", ["winckelmans", "gaussianerf"])
+def test_mixed_near_field_matches_direct_sums(ctx, kernel):
+    """Every cell near: the mixed-precision near field (FP32 kernel factors,
+    FP64 geometry and sums) of all three modes within 1e-5 of the FP64 direct
+    sums."""
+    from fluxvortex_amd import kernel as K
+    from fluxvortex_amd.particles import VortexParticleField
+    pos, al, sg = wake_sheet(nx=400)
+    f = VortexParticleField(max_particles=len(sg) + 8, rlxf=0.0, kernel="winckelmans",
+                            precision="mixed")
+    f.add_particles_batch(pos, al, sg)
+    f.enable_closure(None, cell=2.0, kappa=1e3)
+    u = K.velocity_from_particles(pos, pos, al, sg, kernel=kernel)
+    J = K.jacobian_from_particles(pos, al, pos, al, sg, kernel=kernel)
+    S = np.einsum("nij,nj->ni", J, al)
+    w = np.stack([J[:, 2, 1] - J[:, 1, 2], J[:, 0, 2] - J[:, 2, 0], J[:, 1, 0] - J[:, 0, 1]], 1)
+    for mode, ref in (("velocity", u), ("stretching", S), ("vorticity", w)):
+        got = f.closure_sum(mode, kernel=kernel, use_mlp=False)
+        e = relerr(got, ref)
+        print(f"mixed near field {kernel} {mode}: {e:.2e}")
+        assert e < 1e-5, (mode, e)
+
+
 def test_closure_local_expansion_order():
     """Two clusters, each inside one cell: the far field of the source cell at
     the target particles enters per particle (first-order source expansion)
