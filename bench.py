@@ -334,11 +334,13 @@ def main(argv=None):
             log(f"[bench] {a.big} particles, mixed")
             bmx = run_config(hybrid, a.big, 3, 1, "mixed")
             big["mixed_steps_per_s"] = 3 / (bmx["ms"] / 1e3)
-            # the smallest cutoff whose closure passes its own error check (the
-            # learned residual is what lets the cutoff shrink)
-            for kappa in (2.5, 3.0, 4.0):
-                log(f"[bench] {a.big} particles, mixed + learned closure, kappa {kappa}")
-                clo = dict(cell=2.0, kappa=kappa, check_every=10, tol=1e-3, train_samples=4096,
+            # the cheapest (cell edge, cutoff) whose closure passes its own error
+            # check (the learned residual is what lets the cutoff shrink); the
+            # near field reaches cell * kappa, the local expansions begin at
+            # twice that
+            for cell, kappa in ((2.5, 2.0), (2.0, 2.5), (2.0, 3.0), (2.0, 4.0)):
+                log(f"[bench] {a.big} particles, mixed + learned closure, cell {cell} kappa {kappa}")
+                clo = dict(cell=cell, kappa=kappa, check_every=10, tol=1e-3, train_samples=4096,
                            train_iters=2, lr=2e-3, kappa_local=2.0 * kappa)
                 bml = run_config(hybrid, a.big, 3, 1, "mixed", closure=clo)
                 if bml["closure"]["active"]:

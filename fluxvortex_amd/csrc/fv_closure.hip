@@ -94,6 +94,8 @@ struct fv_closure_state {
     double *mom = nullptr;             // kMomStride per occupied cell
     double *res = nullptr;             // learned residual velocity, 3 per occupied cell
     double *loc = nullptr;             // local expansions, kLocStride per occupied cell
+    void *lpart = nullptr;             // local-expansion partial sums per source chunk
+    size_t lpart_bytes = 0;
     int32_t *lcnt = nullptr, *loff = nullptr;   // shell pairs per target cell, offsets
     void *band = nullptr;              // far band list (kappa h, kappa_local h] per target cell
     size_t band_bytes = 0;
@@ -363,18 +365,28 @@ __device__ void far_expansion(const double *m, double x0, double x1, double x2, 
 // derivatives of phi up to the fourth (contracted with M through v = M d, the
 // eps sums written as cross products); the particles of the target cell then
 // take u0 + J0 (x - c_t) + H (x - c_t)(x - c_t) / 2 (and J0 + H (x - c_t) for
-// the gradient).  One workgroup per target cell, a fixed-order reduction.
-__global__ __launch_bounds__(256) void k_local_exp(int64_t nocc, const double *__restrict__ mom,
-                                                   double rl2, double *__restrict__ loc)
+// the gradient).  k_local_part: one target cell per lane, the source cells
+// of one chunk per workgroup, every fourth one per wavefront (the same source
+// for every lane of a wavefront: scalar loads), the four wavefronts' sums
+// added in order through LDS: 30 partial coefficients per (chunk, target);
+// k_local_sum adds the chunks in order.
+__global__ __launch_bounds__(256) void k_local_part(int64_t nocc, int64_t chunk,
+                                                    const double *__restrict__ mom, double rl2,
+                                                    double *__restrict__ lpart)
 {
-    __shared__ double red[4][30];
-    const int64_t t = blockIdx.x;
-    const double *mt = mom + t * kMomStride;
+    __shared__ double red[3][30][64];
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t t = (int64_t)blockIdx.x * 64 + lane;
+    const bool act = t < nocc;
+    const double *mt = mom + (act ? t : nocc - 1) * kMomStride;
     const double ct[3] = {mt[16], mt[17], mt[18]};
     double acc[30];
     for (int q = 0; q < 30; ++q) acc[q] = 0.0;
     const double c = kInv4Pi;
-    for (int64_t s = threadIdx.x; s < nocc; s += 256) {
+    const int64_t s0 = (int64_t)blockIdx.y * chunk;
+    const int64_t s1 = (s0 + chunk < nocc) ? s0 + chunk : nocc;
+    for (int64_t s = s0 + w; s < s1; s += 4) {
         const double *ms = mom + s * kMomStride;
         const double gx = ct[0] - ms[16], gy = ct[1] - ms[17], gz = ct[2] - ms[18];
         if (gx * gx + gy * gy + gz * gz <= rl2) continue;
@@ -429,19 +441,27 @@ __global__ __launch_bounds__(256) void k_local_exp(int64_t nocc, const double *_
                 }
         }
     }
-    for (int q = 0; q < 30; ++q)
-        for (int off = 32; off > 0; off >>= 1) acc[q] += __shfl_xor(acc[q], off, 64);
-    if ((threadIdx.x & 63) == 0)
-        for (int q = 0; q < 30; ++q) red[threadIdx.x >> 6][q] = acc[q];
+    if (w > 0)
+        for (int q = 0; q < 30; ++q) red[w - 1][q][lane] = acc[q];
     __syncthreads();
-    if (threadIdx.x < 30) {
-        const int q = threadIdx.x;
-        loc[t * kLocStride + q] = ((red[0][q] + red[1][q]) + red[2][q]) + red[3][q];
+    if (w == 0 && act) {
+        double *o = lpart + ((int64_t)blockIdx.y * nocc + t) * 30;
+        for (int q = 0; q < 30; ++q) o[q] = ((acc[q] + red[0][q][lane]) + red[1][q][lane]) + red[2][q][lane];
     }
-    if (threadIdx.x == 30) {
-        loc[t * kLocStride + 30] = 0.0;
-        loc[t * kLocStride + 31] = 0.0;
-    }
+}
+
+// loc[t][q] = sum over chunks (in chunk order) of lpart[chunk][t][q]; pads 0.
+__global__ void k_local_sum(int64_t nocc, int nchunk, const double *__restrict__ lpart,
+                            double *__restrict__ loc)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nocc * kLocStride) return;
+    const int64_t t = e / kLocStride;
+    const int q = (int)(e % kLocStride);
+    double v = 0.0;
+    if (q < 30)
+        for (int k = 0; k < nchunk; ++k) v += lpart[((int64_t)k * nocc + t) * 30 + q];
+    loc[e] = v;
 }
 
 // Adds the local expansion of the particle's cell (offset dd from the centre)
@@ -1357,8 +1377,23 @@ static int build_local(fv_closure *cl, fv_ctx *ctx)
 {
     if (!(cl->kappa_local > 0.0) || cl->nocc == 0) return FV_OK;
     const double rl = cl->kappa_local * cl->hh;
-    hipLaunchKernelGGL(k_local_exp, dim3((unsigned)cl->nocc), dim3(256), 0, ctx->stream, cl->nocc,
-                       cl->mom, rl * rl * (1.0 + 1e-12), cl->loc);
+    const int64_t nocc = cl->nocc;
+    // about 1024 workgroups of four wavefronts: target blocks of 64 cells x
+    // source chunks
+    const int64_t tb = ceil_div(nocc, 64);
+    int64_t nchunk = ceil_div(1024, tb);
+    if (nchunk > 64) nchunk = 64;
+    if (nchunk > nocc) nchunk = nocc;
+    const int64_t chunk = ceil_div(nocc, nchunk);
+    nchunk = ceil_div(nocc, chunk);
+    int rc = grow(ctx, &cl->lpart, &cl->lpart_bytes, sizeof(double) * 30 * (size_t)(nchunk * nocc));
+    if (rc) return rc;
+    double *lp = static_cast<double *>(cl->lpart);
+    hipLaunchKernelGGL(k_local_part, dim3((unsigned)tb, (unsigned)nchunk), dim3(256), 0, ctx->stream,
+                       nocc, chunk, cl->mom, rl * rl * (1.0 + 1e-12), lp);
+    FV_LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_local_sum, dim3((unsigned)ceil_div(nocc * kLocStride, 256)), dim3(256), 0,
+                       ctx->stream, nocc, (int)nchunk, lp, cl->loc);
     FV_LAUNCH_CHECK(ctx);
     return FV_OK;
 }
@@ -1543,7 +1578,7 @@ void closure_destroy(fv_closure *cl)
 {
     if (!cl) return;
     void *bufs[] = {cl->block, cl->cells, cl->far, cl->band, cl->list, cl->tmp, cl->bbox, cl->dstencil,
-                    cl->tX, cl->vdir};
+                    cl->tX, cl->vdir, cl->lpart};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     delete cl;

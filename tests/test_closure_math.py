@@ -2,7 +2,7 @@
 field of a cell's monopole A + dipole M, u_i = -c eps_ijk (A_j dk phi -
 M_jl dl dk phi) with phi = 1/R, its gradient J and Hessian H, checked by
 finite differences, and the cross-product forms the device kernel evaluates
-(fv_closure.hip, k_local_exp) against the tensor forms."""
+(fv_closure.hip, k_local_part) against the tensor forms."""
 import itertools
 
 import numpy as np
