@@ -165,21 +165,25 @@ __global__ __launch_bounds__(kLuBlock) void k_lu_solve_mixed(
         __syncthreads();
         const bool done = stat[0] <= tol * stat[1];
         if (done || it == max_iter) break;
-        // FP64 residual r = b - A x: one wavefront per row, coalesced row reads,
-        // DPP sum (row_shr 1/2/4/8, row_bcast 15/31; lane 63 holds the sum)
-        for (int i = threadIdx.x >> 6; i < n; i += kLuBlock >> 6) {
-            const int lane = threadIdx.x & 63;
+        // FP64 residual r = b - A x: one row per lane, the row's loads
+        // independent of each other (8 in flight), x broadcast from LDS;
+        // rows beyond the workgroup's lanes take further passes
+        for (int i = threadIdx.x; i < n; i += kLuBlock) {
             const double *row = a + (int64_t)i * n;
-            double s = 0.0;
-            for (int c = lane; c < n; c += 64) s = fma(row[c], x[c], s);
-            s += dpp_d<0x111, 0xf>(0.0, s);
-            s += dpp_d<0x112, 0xf>(0.0, s);
-            s += dpp_d<0x114, 0xf>(0.0, s);
-            s += dpp_d<0x118, 0xf>(0.0, s);
-            s += dpp_d<0x142, 0xa>(0.0, s);
-            s += dpp_d<0x143, 0xc>(0.0, s);
-            const double tot = readlane_d(s, 63);
-            if (lane == 0) r[i] = bg[i] - tot;
+            double s0 = 0.0, s1 = 0.0;
+            int c = 0;
+            for (; c + 8 <= n; c += 8) {
+                double v[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) v[k] = row[c + k];
+#pragma unroll
+                for (int k = 0; k < 8; k += 2) {
+                    s0 = fma(v[k], x[c + k], s0);
+                    s1 = fma(v[k + 1], x[c + k + 1], s1);
+                }
+            }
+            for (; c < n; ++c) s0 = fma(row[c], x[c], s0);
+            r[i] = bg[i] - (s0 + s1);
         }
         __syncthreads();
     }

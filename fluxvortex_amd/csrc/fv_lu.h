@@ -76,6 +76,110 @@ __device__ __forceinline__ int wave_argmax(double best, int idx)
     return __builtin_amdgcn_readlane(c, 63);
 }
 
+__device__ __forceinline__ double readlane_t(double v, int lane) { return readlane_d(v, lane); }
+__device__ __forceinline__ float readlane_t(float v, int lane)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// Panel factorization with the panel in registers (m <= 128 rows), wave 0:
+// lane l holds rows l and l + 64 of the m x kb panel P.  Rows are not moved
+// while the panel is factored: each register row carries its current row
+// position pos (LAPACK's row order after the swaps so far), a swap exchanges
+// two positions, and the rows are stored at their final positions at the
+// end, so L and U come out in LAPACK's order.  The pivot is the first
+// maximum |value| in row-position order (idamax), broadcast with v_readlane.
+// b (nullable, LDS): the right-hand side, swapped and eliminated along.
+// Returns the 1-based column of the first zero pivot in this panel, or 0.
+template <typename T, typename TB>
+__device__ __forceinline__ int panel_factor_regs(T *P, int ldP, int m, int kb, int *perm, TB *b)
+{
+    const int lane = threadIdx.x & 63;
+    constexpr int kNone = 1 << 30;
+    T v[2][kNb];
+    int pos[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int r = lane + 64 * q;
+        pos[q] = r < m ? r : kNone;
+#pragma unroll
+        for (int c = 0; c < kNb; ++c) v[q][c] = (r < m && c < kb) ? P[(int64_t)r * ldP + c] : (T)0;
+    }
+    int info = 0;
+    // fully unrolled over the panel's columns (jj a constant in every copy), so
+    // the register rows are never indexed at run time
+#pragma unroll
+    for (int jj = 0; jj < kNb; ++jj) {
+        if (jj < kb) {
+            double best = -1.0;
+            int bi = kNone;
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const double a = fabs((double)v[q][jj]);
+                if (pos[q] >= jj && pos[q] < m && (a > best || (a == best && pos[q] < bi))) {
+                    best = a;
+                    bi = pos[q];
+                }
+            }
+            const int pl = wave_argmax(best, bi);
+            // the register row holding position pl: (lane lp, slot qp)
+            const uint64_t own = __ballot(pos[0] == pl || pos[1] == pl);
+            const int lp = __builtin_ctzll(own);
+            const int qp = __builtin_amdgcn_readlane(pos[1] == pl ? 1 : 0, lp);
+            const T pv = readlane_t(qp ? v[1][jj] : v[0][jj], lp);
+            if (pv == (T)0) {
+                if (info == 0) info = jj + 1;
+                if (lane == 0) perm[jj] = jj;
+            } else {
+                if (lane == 0) perm[jj] = pl;
+                if (pl != jj) {
+#pragma unroll
+                    for (int q = 0; q < 2; ++q)
+                        pos[q] = pos[q] == pl ? jj : (pos[q] == jj ? pl : pos[q]);
+                    if (b && lane == 0) {
+                        const TB t = b[jj];
+                        b[jj] = b[pl];
+                        b[pl] = t;
+                    }
+                }
+                wave_sync();
+                // pivot row broadcast (qp is uniform: a scalar branch)
+                T prow[kNb];
+                if (qp) {
+#pragma unroll
+                    for (int c = 0; c < kNb; ++c) prow[c] = c > jj ? readlane_t(v[1][c], lp) : (T)0;
+                } else {
+#pragma unroll
+                    for (int c = 0; c < kNb; ++c) prow[c] = c > jj ? readlane_t(v[0][c], lp) : (T)0;
+                }
+                const T inv = (T)1 / pv;
+                const TB bj = b ? b[jj] : (TB)0;
+                // rows below the pivot take l = v / pivot, the others l = 0 (an
+                // exact no-op update), so the update is branch-free
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    const bool below = pos[q] > jj && pos[q] < m;
+                    const T l = below ? v[q][jj] * inv : (T)0;
+                    v[q][jj] = below ? l : v[q][jj];
+#pragma unroll
+                    for (int c = 0; c < kNb; ++c)
+                        if (c > jj) v[q][c] = fma(-l, prow[c], v[q][c]);
+                    if (b && below) b[pos[q]] -= (TB)l * bj;
+                }
+            }
+            wave_sync();
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+        if (pos[q] < m)
+#pragma unroll
+            for (int c = 0; c < kNb; ++c)
+                if (c < kb) P[(int64_t)pos[q] * ldP + c] = v[q][c];
+    wave_sync();
+    return info;
+}
+
 typedef double fv_d4 __attribute__((ext_vector_type(4)));
 
 // A22 -= L21 U12 (rows / columns k0+kb .. n, inner dimension kb <= 16), FP64
@@ -149,11 +253,76 @@ __device__ __forceinline__ void trailing_update(float *a, int lda, int n, int k0
     }
 }
 
+// Panel factorization staged through LDS (any m), wave 0: the m x kb panel P
+// with row stride ldP, swapping the panel's rows and eliminating b as it goes.
+template <typename T, typename TB>
+__device__ __forceinline__ int panel_factor_lds(T *P, int ldP, int m, int kb, int *perm, TB *b)
+{
+    const int lane = threadIdx.x & 63;
+    int info = 0;
+    for (int jj = 0; jj < kb; ++jj) {
+        double best = -1.0;
+        int bi = m;
+        for (int r = jj + lane; r < m; r += 64) {
+            const double v = fabs((double)P[(int64_t)r * ldP + jj]);
+            if (v > best) {
+                best = v;
+                bi = r;
+            }
+        }
+        const int piv = wave_argmax(best, bi);
+        if (P[(int64_t)piv * ldP + jj] == (T)0) {
+            if (info == 0) info = jj + 1;
+            if (lane == 0) perm[jj] = jj;
+            wave_sync();
+            continue;
+        }
+        if (lane == 0) perm[jj] = piv;
+        if (piv != jj) {
+            for (int c = lane; c < kb; c += 64) {
+                const T t = P[(int64_t)jj * ldP + c];
+                P[(int64_t)jj * ldP + c] = P[(int64_t)piv * ldP + c];
+                P[(int64_t)piv * ldP + c] = t;
+            }
+            if (b && lane == 0) {
+                const TB t = b[jj];
+                b[jj] = b[piv];
+                b[piv] = t;
+            }
+        }
+        wave_sync();
+        const T inv = (T)1 / P[(int64_t)jj * ldP + jj];
+        const TB bj = b ? b[jj] : (TB)0;
+        // pivot row in registers; each lane loads its row slice before
+        // storing, so no LDS store waits on the next load
+        T prow[kNb];
+#pragma unroll
+        for (int c = 0; c < kNb; ++c)
+            prow[c] = (c > jj && c < kb) ? P[(int64_t)jj * ldP + c] : (T)0;
+        for (int r = jj + 1 + lane; r < m; r += 64) {
+            T *row = P + (int64_t)r * ldP;
+            T v[kNb];
+#pragma unroll
+            for (int c = 0; c < kNb; ++c) v[c] = (c > jj && c < kb) ? row[c] : (T)0;
+            const T l = row[jj] * inv;
+            row[jj] = l;
+#pragma unroll
+            for (int c = 0; c < kNb; ++c)
+                if (c > jj && c < kb) row[c] = v[c] - l * prow[c];
+            if (b) b[r] -= (TB)l * bj;
+        }
+        wave_sync();
+    }
+    return info;
+}
+
 // a: n x n, row stride lda (LDS or global); pan: LDS panel buffer (n x kLdp)
 // or null when a is in LDS; perm (LDS, n): perm[j] = row swapped with j;
 // b (LDS, n, nullable): right-hand side, eliminated along.  Returns 0 or the
-// 1-based index of the first zero pivot.
-template <typename T, typename TB>
+// 1-based index of the first zero pivot.  With REGS, panels of at most 128
+// rows are factored in registers (panel_factor_regs: about 100 VGPRs, for
+// kernels with that budget), taller ones through LDS.
+template <typename T, typename TB, bool REGS = true>
 __device__ __forceinline__ int lu_blocked(T *a, int lda, int n, int *perm, TB *b, T *pan)
 {
     __shared__ int info_s;
@@ -165,79 +334,29 @@ __device__ __forceinline__ int lu_blocked(T *a, int lda, int n, int *perm, TB *b
         const int kb = (n - k0 < kNb) ? n - k0 : kNb;
         const int m = n - k0;
         T *P = a + (int64_t)k0 * lda + k0;
-        int ldP = lda;
-        if (pan) {
+        const bool regs = REGS && m <= 128;
+        if (!regs && pan) {
             for (int e = threadIdx.x; e < m * kb; e += blockDim.x) {
                 const int r = e / kb, c = e - r * kb;
                 pan[r * kLdp + c] = P[(int64_t)r * lda + c];
             }
-            P = pan;
-            ldP = kLdp;
             __syncthreads();
         }
         if (wave == 0) {
-            for (int jj = 0; jj < kb; ++jj) {
-                double best = -1.0;
-                int bi = m;
-                for (int r = jj + lane; r < m; r += 64) {
-                    const double v = fabs((double)P[(int64_t)r * ldP + jj]);
-                    if (v > best) {
-                        best = v;
-                        bi = r;
-                    }
-                }
-                const int piv = wave_argmax(best, bi);
-                if (P[(int64_t)piv * ldP + jj] == (T)0) {
-                    if (lane == 0) {
-                        if (info_s == 0) info_s = k0 + jj + 1;
-                        perm[k0 + jj] = k0 + jj;
-                    }
-                    wave_sync();
-                    continue;
-                }
-                if (lane == 0) perm[k0 + jj] = k0 + piv;
-                if (piv != jj) {
-                    for (int c = lane; c < kb; c += 64) {
-                        const T t = P[(int64_t)jj * ldP + c];
-                        P[(int64_t)jj * ldP + c] = P[(int64_t)piv * ldP + c];
-                        P[(int64_t)piv * ldP + c] = t;
-                    }
-                    if (b && lane == 0) {
-                        const TB t = b[k0 + jj];
-                        b[k0 + jj] = b[k0 + piv];
-                        b[k0 + piv] = t;
-                    }
-                }
-                wave_sync();
-                const T inv = (T)1 / P[(int64_t)jj * ldP + jj];
-                const TB bj = b ? b[k0 + jj] : (TB)0;
-                // pivot row in registers; each lane loads its row slice before
-                // storing, so no LDS store waits on the next load
-                T prow[kNb];
-#pragma unroll
-                for (int c = 0; c < kNb; ++c)
-                    prow[c] = (c > jj && c < kb) ? P[(int64_t)jj * ldP + c] : (T)0;
-                for (int r = jj + 1 + lane; r < m; r += 64) {
-                    T *row = P + (int64_t)r * ldP;
-                    T v[kNb];
-#pragma unroll
-                    for (int c = 0; c < kNb; ++c) v[c] = (c > jj && c < kb) ? row[c] : (T)0;
-                    const T l = row[jj] * inv;
-                    row[jj] = l;
-#pragma unroll
-                    for (int c = 0; c < kNb; ++c)
-                        if (c > jj && c < kb) row[c] = v[c] - l * prow[c];
-                    if (b) b[k0 + r] -= (TB)l * bj;
-                }
-                wave_sync();
+            TB *bk = b ? b + k0 : (TB *)nullptr;
+            const int inf = regs ? panel_factor_regs<T, TB>(P, lda, m, kb, perm + k0, bk)
+                            : pan ? panel_factor_lds<T, TB>(pan, kLdp, m, kb, perm + k0, bk)
+                                  : panel_factor_lds<T, TB>(P, lda, m, kb, perm + k0, bk);
+            if (lane == 0) {
+                for (int jj = 0; jj < kb; ++jj) perm[k0 + jj] += k0;
+                if (inf && info_s == 0) info_s = k0 + inf;
             }
         }
         __syncthreads();
-        if (pan) {
-            T *G = a + (int64_t)k0 * lda + k0;
+        if (!regs && pan) {
             for (int e = threadIdx.x; e < m * kb; e += blockDim.x) {
                 const int r = e / kb, c = e - r * kb;
-                G[(int64_t)r * lda + c] = pan[r * kLdp + c];
+                P[(int64_t)r * lda + c] = pan[r * kLdp + c];
             }
         }
         // the panel's row swaps on every column outside it, in pivot order
@@ -280,10 +399,28 @@ __device__ __forceinline__ int lu_blocked(T *a, int lda, int n, int *perm, TB *b
     return info_s;
 }
 
-// x = U^{-1} x in place by wave 0, column-oriented: x[k] is final once the
-// columns right of k have been eliminated; it is broadcast with v_readlane
-// and column k is eliminated from the rows above.  x lives in registers
-// (lane l holds x[l + 64 q]) for n <= 256, else in LDS.
+// Register form of the substitutions (n <= 256): lane l holds x[l + 64 q].
+// Blocks of 16 columns: the block's own rows are solved sequentially from
+// the 16 x 16 diagonal block held in registers (x broadcast with
+// v_readlane), then every row outside the block is updated at once with
+// the block's 16 finished values (independent LDS loads, no chain).
+template <typename TB>
+__device__ __forceinline__ TB slot_get(const TB *xr, int q)
+{
+    return q == 0 ? xr[0] : q == 1 ? xr[1] : q == 2 ? xr[2] : xr[3];
+}
+
+template <typename TB>
+__device__ __forceinline__ void slot_set(TB *xr, int q, TB v)
+{
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (k == q) xr[k] = v;
+}
+
+// x = U^{-1} x in place by wave 0 (U upper triangular, 1 / U_kk in rdiag).
+// x lives in registers for n <= 256 (blocked, above), else in LDS
+// (column-oriented, one broadcast per column).
 template <typename T, typename TB>
 __device__ __forceinline__ void wave_back_subst(const T *a, int lda, int n, TB *x,
                                                 const TB *rdiag)
@@ -294,15 +431,37 @@ __device__ __forceinline__ void wave_back_subst(const T *a, int lda, int n, TB *
         TB xr[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) xr[q] = (lane + 64 * q < n) ? x[lane + 64 * q] : (TB)0;
-        for (int k = n - 1; k >= 0; --k) {
-            const int ks = k >> 6, kl = k & 63;
-            const TB own = ks == 0 ? xr[0] : ks == 1 ? xr[1] : ks == 2 ? xr[2] : xr[3];
-            const TB xk = readlane_d((double)own, kl) * rdiag[k];
+        for (int kb0 = ((n - 1) / kNb) * kNb; kb0 >= 0; kb0 -= kNb) {
+            const int kbl = (n - kb0 < kNb) ? n - kb0 : kNb;
+            const int q0 = kb0 >> 6, l0 = kb0 & 63, ib = lane - l0;
+            const bool inblk = ib >= 0 && ib < kbl;
+            T u[kNb];
+#pragma unroll
+            for (int j = 0; j < kNb; ++j)
+                u[j] = (inblk && j > ib && j < kbl) ? a[(int64_t)(kb0 + ib) * lda + kb0 + j] : (T)0;
+            const TB rd = inblk ? rdiag[kb0 + ib] : (TB)0;
+            TB xb = slot_get(xr, q0);
+#pragma unroll
+            for (int j = kNb - 1; j >= 0; --j) {
+                if (j < kbl) {
+                    const TB xj = readlane_t(xb, l0 + j) * readlane_t(rd, l0 + j);
+                    xb = ib == j ? xj : fma(-(TB)u[j], xj, xb);   // u[j] = 0 off the block's upper part
+                }
+            }
+            slot_set(xr, q0, xb);
+            TB xs[kNb];
+#pragma unroll
+            for (int j = 0; j < kNb; ++j) xs[j] = j < kbl ? readlane_t(xb, l0 + j) : (TB)0;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int i = lane + 64 * q;
-                if (i < k) xr[q] -= (TB)a[(int64_t)i * lda + k] * xk;
-                if (i == k) xr[q] = xk;
+                if (i < kb0) {
+                    TB acc = xr[q];
+#pragma unroll
+                    for (int j = 0; j < kNb; ++j)
+                        if (j < kbl) acc -= (TB)a[(int64_t)i * lda + kb0 + j] * xs[j];
+                    xr[q] = acc;
+                }
             }
         }
 #pragma unroll
@@ -320,8 +479,9 @@ __device__ __forceinline__ void wave_back_subst(const T *a, int lda, int n, TB *
     }
 }
 
-// y = L^{-1} x in place (unit lower triangle) by wave 0, column-oriented as
-// above; the row permutation is applied by the caller.
+// y = L^{-1} x in place (unit lower triangle) by wave 0, blocked in registers
+// for n <= 256 as above, else column-oriented in LDS; the row permutation is
+// applied by the caller.
 template <typename T, typename TB>
 __device__ __forceinline__ void wave_fwd_subst(const T *a, int lda, int n, TB *x)
 {
@@ -331,14 +491,33 @@ __device__ __forceinline__ void wave_fwd_subst(const T *a, int lda, int n, TB *x
         TB xr[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) xr[q] = (lane + 64 * q < n) ? x[lane + 64 * q] : (TB)0;
-        for (int k = 0; k < n - 1; ++k) {
-            const int ks = k >> 6, kl = k & 63;
-            const TB own = ks == 0 ? xr[0] : ks == 1 ? xr[1] : ks == 2 ? xr[2] : xr[3];
-            const TB xk = readlane_d((double)own, kl);
+        for (int kb0 = 0; kb0 < n; kb0 += kNb) {
+            const int kbl = (n - kb0 < kNb) ? n - kb0 : kNb;
+            const int q0 = kb0 >> 6, l0 = kb0 & 63, ib = lane - l0;
+            const bool inblk = ib >= 0 && ib < kbl;
+            T lrow[kNb];
+#pragma unroll
+            for (int j = 0; j < kNb; ++j)
+                lrow[j] = (inblk && j < ib) ? a[(int64_t)(kb0 + ib) * lda + kb0 + j] : (T)0;
+            TB xb = slot_get(xr, q0);
+#pragma unroll
+            for (int j = 0; j < kNb; ++j) {
+                if (j < kbl) xb = fma(-(TB)lrow[j], readlane_t(xb, l0 + j), xb);   // lrow[j] = 0 off L
+            }
+            slot_set(xr, q0, xb);
+            TB xs[kNb];
+#pragma unroll
+            for (int j = 0; j < kNb; ++j) xs[j] = j < kbl ? readlane_t(xb, l0 + j) : (TB)0;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int i = lane + 64 * q;
-                if (i > k && i < n) xr[q] -= (TB)a[(int64_t)i * lda + k] * xk;
+                if (i >= kb0 + kbl && i < n) {
+                    TB acc = xr[q];
+#pragma unroll
+                    for (int j = 0; j < kNb; ++j)
+                        if (j < kbl) acc -= (TB)a[(int64_t)i * lda + kb0 + j] * xs[j];
+                    xr[q] = acc;
+                }
             }
         }
 #pragma unroll

@@ -368,7 +368,8 @@ __global__ __launch_bounds__(kSwStepBlock) void k_sw_step(SwInst *__restrict__ i
     }
     __syncthreads();
     // ---- circulation solve (LU with partial pivoting in LDS)
-    const int inf = lu_blocked<double, double>(A, lda, P, perm, rhs, (double *)nullptr);
+    // LDS panels: the step kernel's 1024-lane workgroup leaves 128 VGPRs a lane
+    const int inf = lu_blocked<double, double, false>(A, lda, P, perm, rhs, (double *)nullptr);
     if (inf != 0) {
         if (t == 0) {
             insts[blockIdx.x].status = (int)s + 1;
