@@ -223,13 +223,17 @@ def flutter_block(n_speeds=64, cpu_sample=4):
     """The batched Goland flutter sweep: n_speeds airspeeds uniform over
     80-180 m/s, 100 chords of wake, N_keep 10 (uneven step and particle
     counts), every instance stepped together (one induction + one step launch
-    per time step, replayed from a hipGraph).  FP64 and mixed; the oracle is
-    timed on cpu_sample speeds and scaled to the sweep."""
+    per time step, replayed from a hipGraph).  FP64 and mixed; then the
+    reference benchmark's own wake (every row kept as rings, held in device
+    memory, one more launch per step) in FP64 against its published flutter
+    speed.  The oracle is timed on cpu_sample speeds of the N_keep 10 sweep
+    and scaled to the sweep."""
     from fluxvortex_amd import aeroelastic as ae
     speeds = np.linspace(80.0, 180.0, n_speeds)
     res = {"speeds": [80.0, 180.0, n_speeds], "n_keep": 10, "num_chords": 100}
-    for prec in ("fp64", "mixed"):
-        insts = [ae.goland_instance(float(V), precision=prec) for V in speeds]
+    for prec, n_keep in (("fp64", 10), ("mixed", 10), ("ring_wake_fp64", 100000)):
+        insts = [ae.goland_instance(float(V), precision=prec.replace("ring_wake_", ""),
+                                    n_keep=n_keep) for V in speeds]
         sw = ae.Sweep(insts, use_graph=True)
         sw.step(1)
         sw.read(0)
@@ -250,6 +254,8 @@ def flutter_block(n_speeds=64, cpu_sample=4):
                      "instance_steps_per_s": inst_steps / sec,
                      "flutter_speed": ae.flutter_speed(speeds, sig),
                      "sigma_w_80": sig[0], "sigma_w_180": sig[-1]}
+    # benchmark_goland.py's published result (README.md:121-132)
+    res["ring_wake_fp64"]["published_flutter_speed"] = 140.2
     f64, mx = res["fp64"]["flutter_speed"], res["mixed"]["flutter_speed"]
     if f64 and mx:
         res["mixed_flutter_speed_rel_err_vs_fp64"] = abs(mx - f64) / f64

@@ -15,8 +15,14 @@
 //                -> beam nodes, one Newmark step, the next lattice's
 //                deformation, wake shedding, N_keep conversion, particle
 //                convection and the next step's collocation points.
-// The two launches are captured once in a hipGraph and replayed per step
-// (every per-step quantity lives in device memory).
+// Instances keeping more than kSwMaxR - 1 ring rows (up to the whole ring
+// wake, the reference benchmark's configuration) keep the wake in HBM and add
+// a third launch ahead of the step:
+//   k_sw_wake    wake-ring velocities at the collocation points and leg
+//                centres of the step's lattice, one workgroup per (instance,
+//                2048-segment chunk), partial sums per chunk.
+// The launches are captured once in a hipGraph and replayed per step (every
+// per-step quantity lives in device memory).
 //
 // The UVLM statement is the device driver's (fv_hybrid.hip) and the oracle's
 // (oracle/fv_oracle_uvlm.c); the coupling follows AeroelasticSolver: station
@@ -45,17 +51,21 @@ constexpr int kSwBlock = 256;       // induction workgroup
 constexpr int kSwStepBlock = 1024;  // step workgroup: 16 waves, one instance per CU
 constexpr int kSwChunk = 1024;   // particles per induction partial
 constexpr int kSwParts = 8;      // segment partial sums per target
+constexpr int kSwWChunk = 2048;  // wake segments per partial sum (HBM wake)
+constexpr int kSwWTile = 256;    // wake segments staged in LDS at a time
 
 struct SwInst {
     int nc, ns, P, nv, R, n_keep, kernel, couple, n_nodes, nfree;
     int status;                  // 0; k + 1: zero pivot at step k; -2: particle overflow
     int has_prev, par, nrow, has_wprev, prec, feedback;
-    int64_t num_steps, step, cap, np, max_chunks;
+    int hbm;                     // wake rings in HBM, velocities from k_sw_wake partials
+    int64_t num_steps, step, cap, np, max_chunks, max_wchunks;
     double V[3], rho, dt, nu, rc0b, rc0w, sigma, sref, x_ea, relax, beta, gamma, lift_sign;
     // offsets (doubles) into the state block
     int64_t o_grid0, o_coln, o_colp, o_legp, o_gamp, o_gam, o_wv[2], o_wg[2], o_wa[2];
     int64_t o_pos, o_al, o_sg, o_part, o_M, o_C, o_K, o_d, o_v, o_a;
     int64_t o_wdef, o_tdef, o_wprev, o_tprev, o_yn, o_cl, o_cdi, o_tw, o_tt;
+    int64_t o_legn, o_wpart;     // the step's leg centres; wake partial sums (HBM wake)
 };
 
 // ------------------------------------------------------------ geometry helpers
@@ -167,6 +177,21 @@ __device__ __forceinline__ void sw_pair(int prec, const double *x, const SegRec 
         seg_pair_f64(x[0], x[1], x[2], sr, u[0], u[1], u[2]);
 }
 
+// Leg centres of panel p (slots 3 p + l for the right / front / left legs, 3 P
+// + j for the trailing-edge back leg of (nc - 1, j)), as k_sw_step forms them.
+__device__ void sw_leg_centres(const double *g, int nc, int ns, int P, int p, double *lc)
+{
+    const int i = p / ns, j = p % ns;
+    double R[12];
+    sw_ring(g, nc, ns, i, j, R, R + 3, R + 6, R + 9);
+    for (int l = 0; l < 4; ++l) {
+        if (l == 3 && i != nc - 1) continue;
+        const int q = l < 3 ? 3 * p + l : 3 * P + j;
+        const double *a = R + 3 * l, *b = R + 3 * ((l + 1) & 3);
+        for (int c = 0; c < 3; ++c) lc[3 * q + c] = 0.5 * (a[c] + b[c]);
+    }
+}
+
 // ----------------------------------------------------------- particle induction
 template <int KERNEL, int PREC>
 __device__ void sw_induce_body(const SwInst &I, const double *__restrict__ S, double *part,
@@ -232,6 +257,64 @@ __global__ __launch_bounds__(kSwBlock) void k_sw_induce(const SwInst *__restrict
     }
 }
 
+// ------------------------------------------------------------ HBM ring wake
+// Wake-ring velocities at the T = P + NL targets of the step (collocation
+// points, then the leg centres in slot order) from the segments [c0, c1) of
+// this chunk (segment k: leg k & 3 of ring k >> 2), partial sums
+// wpart[(chunk T + target) 3 + c].
+__global__ __launch_bounds__(kSwBlock) void k_sw_wake(const SwInst *__restrict__ insts,
+                                                      double *__restrict__ S)
+{
+    __shared__ SegRec tile[kSwWTile];
+    const SwInst &I = insts[blockIdx.y];
+    if (!I.hbm || I.status != 0 || I.step >= I.num_steps) return;
+    const int ns = I.ns, P = I.P, NL = 3 * P + ns, T = P + NL;
+    const int64_t nseg = 4 * (int64_t)I.nrow * ns;
+    const int64_t c0 = (int64_t)blockIdx.x * kSwWChunk;
+    if (c0 >= nseg) return;   // uniform over the workgroup
+    const int64_t c1 = (c0 + kSwWChunk < nseg) ? c0 + kSwWChunk : nseg;
+    const double *wv = S + I.o_wv[I.par], *wg = S + I.o_wg[I.par], *wa = S + I.o_wa[I.par];
+    constexpr int kTpt = (kSwMaxP + 3 * kSwMaxP + kSwMaxNs + kSwBlock - 1) / kSwBlock;
+    double u[kTpt][3];
+    const double *x[kTpt];
+#pragma unroll
+    for (int q = 0; q < kTpt; ++q) {
+        const int tq = threadIdx.x + q * kSwBlock;
+        const int tc = tq < T ? tq : 0;
+        x[q] = tc < P ? S + I.o_coln + 3 * tc : S + I.o_legn + 3 * (tc - P);
+        u[q][0] = u[q][1] = u[q][2] = 0.0;
+    }
+    for (int64_t base = c0; base < c1; base += kSwWTile) {
+        const int cnt = (int)((c1 - base < kSwWTile) ? c1 - base : kSwWTile);
+        __syncthreads();
+        if ((int)threadIdx.x < cnt) {
+            const int64_t k = base + threadIdx.x;
+            const int64_t rg = k >> 2;
+            const int leg = (int)(k & 3), r = (int)(rg / ns), j = (int)(rg % ns);
+            const double *c4[4] = {wv + 3 * ((r + 1) * (ns + 1) + j + 1),   // Br
+                                   wv + 3 * (r * (ns + 1) + j + 1),         // Fr
+                                   wv + 3 * (r * (ns + 1) + j),             // Fl
+                                   wv + 3 * ((r + 1) * (ns + 1) + j)};      // Bl
+            tile[threadIdx.x] = sw_seg(c4[leg], c4[(leg + 1) & 3], wg[rg], I.rc0w, wa[rg], I.nu);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < kTpt; ++q) {
+            if ((int)threadIdx.x + q * kSwBlock < T) {
+                const double xp[3] = {x[q][0], x[q][1], x[q][2]};
+                for (int k = 0; k < cnt; ++k) sw_pair(I.prec, xp, tile[k], u[q]);
+            }
+        }
+    }
+    double *part = S + I.o_wpart + (int64_t)blockIdx.x * T * 3;
+#pragma unroll
+    for (int q = 0; q < kTpt; ++q) {
+        const int tq = threadIdx.x + q * kSwBlock;
+        if (tq < T)
+            for (int c = 0; c < 3; ++c) part[3 * tq + c] = u[q][c];
+    }
+}
+
 // ------------------------------------------------------------------ the step
 // LDS plan (doubles) of k_sw_step at the limits above.
 constexpr int kSwNv = (kSwMaxNc + 1) * (kSwMaxNs + 1);
@@ -284,7 +367,12 @@ __global__ __launch_bounds__(kSwStepBlock) void k_sw_step(SwInst *__restrict__ i
     int *perm = reinterpret_cast<int *>(L + kL_perm);
     double *red = L + kL_red, *vw = L + kL_vw, *vl = L + kL_vl, *F = L + kL_F;
     double *PF = L + kL_PF, *Ff = PF + 3 * kSwMaxP;
-    double *wv = L + kL_wv, *wg = L + kL_wg, *wa = L + kL_wa, *bm = L + kL_beam;
+    const bool hbm = I.hbm != 0;
+    // the wake of this step: staged in LDS, or read in place (HBM wake)
+    const double *wv = hbm ? S + I.o_wv[I.par] : L + kL_wv;
+    const double *wg = hbm ? S + I.o_wg[I.par] : L + kL_wg;
+    const double *wa = hbm ? S + I.o_wa[I.par] : L + kL_wa;
+    double *bm = L + kL_beam;
     const double *grid0 = S + I.o_grid0, *yn = S + I.o_yn;
     double *wdef = S + I.o_wdef, *tdef = S + I.o_tdef;
     const int lda = P | 1;
@@ -315,10 +403,13 @@ __global__ __launch_bounds__(kSwStepBlock) void k_sw_step(SwInst *__restrict__ i
     }
     // wake of the current step (rows 0 .. nrow-1, vertices 0 .. nrow)
     const int nrow = I.nrow, cur = I.par;
-    for (int e = t; e < 3 * (nrow + 1) * (ns + 1); e += kSwStepBlock) wv[e] = S[I.o_wv[cur] + e];
-    for (int e = t; e < nrow * ns; e += kSwStepBlock) {
-        wg[e] = S[I.o_wg[cur] + e];
-        wa[e] = S[I.o_wa[cur] + e];
+    if (!hbm) {
+        double *lwv = L + kL_wv, *lwg = L + kL_wg, *lwa = L + kL_wa;
+        for (int e = t; e < 3 * (nrow + 1) * (ns + 1); e += kSwStepBlock) lwv[e] = S[I.o_wv[cur] + e];
+        for (int e = t; e < nrow * ns; e += kSwStepBlock) {
+            lwg[e] = S[I.o_wg[cur] + e];
+            lwa[e] = S[I.o_wa[cur] + e];
+        }
     }
     __syncthreads();
     // ---- AIC: unit-strength rings (bound core, age 0) . normals
@@ -332,9 +423,12 @@ __global__ __launch_bounds__(kSwStepBlock) void k_sw_step(SwInst *__restrict__ i
         }
         A[i * lda + j] = u[0] * nrm[3 * i] + u[1] * nrm[3 * i + 1] + u[2] * nrm[3 * i + 2];
     }
-    // ---- wake-ring velocity at the collocation points (kSwParts partial sums)
+    // ---- wake-ring velocity at the collocation points (kSwParts partial sums;
+    // the HBM wake's come from k_sw_wake)
     const int W = nrow * ns;
-    for (int e = t; e < P * kSwParts; e += kSwStepBlock) {
+    const int T = P + NL;
+    const int nwch = (int)((4 * (int64_t)W + kSwWChunk - 1) / kSwWChunk);
+    for (int e = t; e < P * kSwParts && !hbm; e += kSwStepBlock) {
         const int p = e / kSwParts, q = e % kSwParts;
         double u[3] = {0.0, 0.0, 0.0};
         for (int k = q; k < 4 * W; k += kSwParts) {
@@ -354,7 +448,10 @@ __global__ __launch_bounds__(kSwStepBlock) void k_sw_step(SwInst *__restrict__ i
         double w3[3], q3[3] = {0.0, 0.0, 0.0};
         for (int c = 0; c < 3; ++c) {
             double a = 0.0;
-            for (int q = 0; q < kSwParts; ++q) a += red[(p * kSwParts + q) * 3 + c];
+            if (hbm)
+                for (int ch = 0; ch < nwch; ++ch) a += S[I.o_wpart + ((int64_t)ch * T + p) * 3 + c];
+            else
+                for (int q = 0; q < kSwParts; ++q) a += red[(p * kSwParts + q) * 3 + c];
             w3[c] = a;
         }
         for (int ch = 0; ch < nchunk; ++ch)
@@ -392,7 +489,7 @@ __global__ __launch_bounds__(kSwStepBlock) void k_sw_step(SwInst *__restrict__ i
             const SegRec sr = sw_seg(R + 3 * leg, R + 3 * ((leg + 1) & 3), gam[j], I.rc0b, 0.0, I.nu);
             sw_pair(I.prec, x, sr, u);
         }
-        for (int k = q; k < 4 * W; k += kSwParts) {
+        for (int k = q; k < 4 * W && !hbm; k += kSwParts) {
             const int rg = k >> 2, leg = k & 3, r = rg / ns, j = rg % ns;
             const double *c4[4] = {wv + 3 * ((r + 1) * (ns + 1) + j + 1),
                                    wv + 3 * (r * (ns + 1) + j + 1),
@@ -408,6 +505,9 @@ __global__ __launch_bounds__(kSwStepBlock) void k_sw_step(SwInst *__restrict__ i
         for (int c = 0; c < 3; ++c) {
             double a = 0.0;
             for (int q = 0; q < kSwParts; ++q) a += red[(tq * kSwParts + q) * 3 + c];
+            if (hbm)
+                for (int ch = 0; ch < nwch; ++ch)
+                    a += S[I.o_wpart + ((int64_t)ch * T + P + tq) * 3 + c];
             vl[3 * tq + c] = a;
         }
     __syncthreads();
@@ -654,6 +754,7 @@ __global__ __launch_bounds__(kSwStepBlock) void k_sw_step(SwInst *__restrict__ i
             double cn[3], nn3[3], ar;
             sw_frame(gn, ns, p / ns, p % ns, cn, nn3, &ar);
             for (int c = 0; c < 3; ++c) S[I.o_coln + 3 * p + c] = cn[c];
+            if (hbm) sw_leg_centres(gn, nc, ns, P, p, S + I.o_legn);
         }
         if (t == 0) {
             I.nrow = nrow_new;
@@ -681,7 +782,8 @@ __global__ __launch_bounds__(kSwStepBlock) void k_sw_step(SwInst *__restrict__ i
     }
 }
 
-// Collocation points of the undeformed lattice (the first induction's targets).
+// Collocation points (and, for an HBM wake, leg centres) of the undeformed
+// lattice: the first step's induction targets.
 __global__ __launch_bounds__(64) void k_sw_init(const SwInst *__restrict__ insts,
                                                 double *__restrict__ S)
 {
@@ -692,6 +794,7 @@ __global__ __launch_bounds__(64) void k_sw_init(const SwInst *__restrict__ insts
         // an undeformed lattice: sw_deform with zero heave / twist is the identity
         sw_frame(g, I.ns, p / I.ns, p % I.ns, cn, nn3, &ar);
         for (int c = 0; c < 3; ++c) S[I.o_coln + 3 * p + c] = cn[c];
+        if (I.hbm) sw_leg_centres(g, I.nc, I.ns, I.P, p, S + I.o_legn);
     }
 }
 
@@ -705,7 +808,7 @@ struct fv_sweep {
     std::vector<SwInst> host;
     SwInst *dinst = nullptr;
     double *block = nullptr;
-    int64_t max_chunks = 1, max_steps = 0, done_steps = 0;
+    int64_t max_chunks = 1, max_wchunks = 0, max_steps = 0, done_steps = 0;
     int use_graph = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -714,6 +817,11 @@ struct fv_sweep {
 static int sweep_launch(fv_sweep *sw)
 {
     fv_ctx *ctx = sw->ctx;
+    if (sw->max_wchunks > 0) {
+        hipLaunchKernelGGL(k_sw_wake, dim3((unsigned)sw->max_wchunks, (unsigned)sw->n),
+                           dim3(kSwBlock), 0, ctx->stream, sw->dinst, sw->block);
+        FV_LAUNCH_CHECK(ctx);
+    }
     hipLaunchKernelGGL(k_sw_induce, dim3((unsigned)sw->max_chunks, (unsigned)sw->n),
                        dim3(kSwBlock), 0, ctx->stream, sw->dinst, sw->block);
     FV_LAUNCH_CHECK(ctx);
@@ -744,7 +852,7 @@ int fv_sweep_create(fv_ctx *ctx, const fv_sweep_desc *descs, int32_t n, int32_t 
         const int nfree = 3 * (d.n_nodes - 1);
         if (!(d.nc >= 1 && d.nc <= kSwMaxNc && d.ns >= 1 && d.ns <= kSwMaxNs &&
               d.nc * d.ns <= kSwMaxP && d.grid && d.num_steps >= 1 && d.dt > 0 && d.rho > 0 &&
-              d.sref > 0 && V > 1e-10 && d.n_keep >= 0 && d.n_keep <= kSwMaxR - 1 &&
+              d.sref > 0 && V > 1e-10 && d.n_keep >= 0 &&
               d.max_particles >= 1 && d.n_nodes >= 2 && d.n_nodes <= kSwMaxNodes && d.y_nodes &&
               (d.kernel == FV_KERNEL_GAUSSIAN_ERF || d.kernel == FV_KERNEL_WINCKELMANS) &&
               (d.precision == FV_PREC_FP64 || d.precision == FV_PREC_MIXED) &&
@@ -758,7 +866,14 @@ int fv_sweep_create(fv_ctx *ctx, const fv_sweep_desc *descs, int32_t n, int32_t 
         I.ns = d.ns;
         I.P = d.nc * d.ns;
         I.nv = (d.nc + 1) * (d.ns + 1);
-        I.R = (int)(d.n_keep + 1 < d.num_steps ? d.n_keep + 1 : d.num_steps);
+        const int64_t R = (int64_t)d.n_keep + 1 < d.num_steps ? (int64_t)d.n_keep + 1 : d.num_steps;
+        if (R > (int64_t)1 << 24) {
+            delete sw;
+            return set_error(ctx, FV_E_ARG, "sweep instance %d: %lld kept wake rows exceed the limit",
+                             i, (long long)R);
+        }
+        I.R = (int)R;
+        I.hbm = I.R > kSwMaxR ? 1 : 0;
         I.n_keep = d.n_keep;
         I.kernel = d.kernel;
         I.prec = d.precision;
@@ -799,6 +914,14 @@ int fv_sweep_create(fv_ctx *ctx, const fv_sweep_desc *descs, int32_t n, int32_t 
         I.o_al = take(3 * I.cap);
         I.o_sg = take(I.cap);
         I.o_part = take(3 * I.P * I.max_chunks);
+        if (I.hbm) {
+            // the wake holds at most R rows when a step starts
+            I.max_wchunks = ceil_div(4 * (int64_t)I.R * d.ns, kSwWChunk);
+            if (I.max_wchunks < 1) I.max_wchunks = 1;
+            if (I.max_wchunks > sw->max_wchunks) sw->max_wchunks = I.max_wchunks;
+            I.o_legn = take(3 * (3 * I.P + d.ns));
+            I.o_wpart = take(3 * (int64_t)(4 * I.P + d.ns) * I.max_wchunks);
+        }
         I.o_M = take((int64_t)nfree * nfree);
         I.o_C = take((int64_t)nfree * nfree);
         I.o_K = take((int64_t)nfree * nfree);

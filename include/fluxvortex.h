@@ -361,7 +361,10 @@ int fv_vpm_closure_train(fv_vpm *f, int32_t rounds);
  * x = 0), go linearly to the beam nodes, one Newmark step advances the beam
  * and its heave / twist deform the lattice of step s + 1 (else step s + 1 is
  * undeformed).  Particles are a prescribed wake convected with V dt.
- * Limits: nc <= 16, ns <= 32, nc * ns <= 64, n_keep <= 15, beam nodes <= 17. */
+ * n_keep >= num_steps keeps the whole ring wake (no particles, the reference
+ * benchmark's wake); instances keeping more than 15 rows hold their wake in
+ * device memory and add one wake-induction launch per step.
+ * Limits: nc <= 16, ns <= 32, nc * ns <= 64, beam nodes <= 17. */
 typedef struct fv_sweep_desc {
     int32_t nc, ns;              /* one lattice, (nc+1) x (ns+1) corner grid */
     const double *grid;          /* undeformed corner grid (host), x aft, y span */

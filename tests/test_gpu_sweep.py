@@ -104,3 +104,49 @@ def test_small_flutter_sweep_matches_oracle(oracle):
     assert np.abs(np.array(sig) - sig_ref).max() < 1e-6
     Vf_ref = ae.flutter_speed(speeds, sig_ref)
     assert Vf_ref is not None and abs(res["flutter_speed"] - Vf_ref) < 1e-6 * Vf_ref
+
+
+def test_sweep_hbm_ring_wake_matches_oracle(oracle):
+    """Instances keeping more than 15 ring rows hold their wake in device
+    memory (k_sw_wake): the whole ring wake (no particles) at two speeds and
+    n_keep 20 (rows 20+ converted) in FP64 against the oracle to 1e-10, and
+    mixed within 1e-5 of FP64."""
+    insts = [ae.goland_instance(100.0, num_chords=10, n_keep=100000),
+             ae.goland_instance(160.0, num_chords=12, n_keep=100000, kernel="winckelmans"),
+             ae.goland_instance(130.0, num_chords=10, n_keep=20),
+             ae.goland_instance(90.0, num_chords=6)]          # LDS wake beside them
+    print("hbm sweep:", [(it.speed, it.num_steps, it.n_keep) for it in insts], flush=True)
+    sw = ae.Sweep(insts, use_graph=True)
+    sw.step()
+    for i, it in enumerate(insts):
+        got = sw.read(i)
+        ref = oracle.aeroelastic_run(it)
+        assert got["status"] == 0
+        assert got["np"] == ref["np"]
+        assert _rel(got["cl"], ref["cl"]) < 1e-10, i
+        assert _rel(got["cdi"], ref["cdi"]) < 1e-10, i
+        assert _rel(got["gamma"], ref["gamma"]) < 1e-10, i
+        ok = ~np.isnan(ref["tip_w"])
+        assert _rel(got["tip_w"][ok], ref["tip_w"][ok]) < 1e-10, i
+    assert sw.read(0)["np"] == 0 and sw.read(2)["np"] > 0
+    mixed = [ae.goland_instance(it.speed, num_chords=10, n_keep=100000, precision="mixed")
+             for it in insts[:1]]
+    mx = ae.Sweep(mixed, use_graph=True)
+    mx.step()
+    assert _rel(mx.read(0)["cl"], sw.read(0)["cl"]) < 1e-5
+    sw.close()
+    mx.close()
+
+
+def test_ring_wake_flutter_sweep_matches_oracle_and_published():
+    """benchmark_goland.py's configuration (ring wake, 100 chords) on the
+    device: the eight published speeds' sigma_w and the flutter speed within
+    0.5 % of the published 140.2 m/s (README.md:121-132), the same bar the
+    oracle meets."""
+    Vs = np.array([80, 100, 120, 130, 140, 144, 160, 180], float)
+    pub = [-0.628, -0.899, -0.249, -0.101, -0.037, 0.564, 0.794, 1.413]
+    res = ae.flutter_sweep(Vs, use_graph=True, n_keep=100000)
+    sig = [r["sigma_w"] for r in res["runs"]]
+    print("ring-wake sweep sigma_w:", np.round(sig, 3), "V_f", res["flutter_speed"])
+    assert np.abs(np.array(sig) - pub).max() < 0.13, sig
+    assert abs(res["flutter_speed"] - 140.2) < 0.005 * 140.2
