@@ -44,7 +44,7 @@ PEAK_FP32 = 157.3e12
 # from two rocprofv3 counter passes
 # (FETCH_SIZE x 2.00, WRITE_SIZE x 1.00, scaled by a plain 512 MiB copy):
 # profiles/k_pairs_stretch_heaving_wing_20k_pmc.md
-TRAFFIC_STRETCH = {"particles": 20096, "bytes": (4182.5 * 2.0 + 12701.8) * 1024}
+TRAFFIC_STRETCH = {"particles": 20096, "bytes": (4187.5 * 2.0 + 12717.0) * 1024}
 
 
 def log(msg):
@@ -254,8 +254,13 @@ def flutter_block(n_speeds=64, cpu_sample=4):
                      "instance_steps_per_s": inst_steps / sec,
                      "flutter_speed": ae.flutter_speed(speeds, sig),
                      "sigma_w_80": sig[0], "sigma_w_180": sig[-1]}
-    # benchmark_goland.py's published result (README.md:121-132)
-    res["ring_wake_fp64"]["published_flutter_speed"] = 140.2
+    # benchmark_goland.py's own eight speeds and published result (README.md:121-132)
+    pub_V = [80.0, 100.0, 120.0, 130.0, 140.0, 144.0, 160.0, 180.0]
+    pub = ae.flutter_sweep(pub_V, use_graph=True, n_keep=100000)
+    res["ring_wake_published_speeds"] = {
+        "speeds": pub_V, "sigma_w": [r["sigma_w"] for r in pub["runs"]],
+        "flutter_speed": pub["flutter_speed"], "published_flutter_speed": 140.2,
+        "published_sigma_w": [-0.628, -0.899, -0.249, -0.101, -0.037, 0.564, 0.794, 1.413]}
     f64, mx = res["fp64"]["flutter_speed"], res["mixed"]["flutter_speed"]
     if f64 and mx:
         res["mixed_flutter_speed_rel_err_vs_fp64"] = abs(mx - f64) / f64
