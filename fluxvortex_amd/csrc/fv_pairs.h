@@ -178,11 +178,15 @@ __device__ __forceinline__ void part_factors_f64(double r2, bool keep, const Par
                                                  double &CG, double &CH)
 {
     if (KERNEL == 1) {
-        const double q = fma(r2, s.k0, 1.0);
+        // excluded pairs: r^2 replaced by a value in [2^512, 2^512 (1 + 2^-20))
+        // (only the high word is selected, one instruction), so that Z^5
+        // underflows to exactly 0 while q and the factor polynomials stay
+        // finite for any core sigma >= 1e-6: CG = CH = 0 exactly
+        const double r2k = keep ? r2 : __hiloint2double(0x5FF00000, __double2loint(r2));
+        const double q = fma(r2k, s.k0, 1.0);
         const double Z = rsqrt2_refined(q);
         const double Z2 = Z * Z;
-        double Z5 = Z2 * Z2 * Z;
-        Z5 = keep ? Z5 : 0.0;
+        const double Z5 = Z2 * Z2 * Z;
         CG = fma(q, s.k1, s.k2) * Z5;
         if (GRAD) CH = fma(q, s.k3, s.k4) * (Z5 * Z2);
     } else {

@@ -162,6 +162,26 @@ def test_dense_solve(vk):
         vk.dense_solve(np.zeros((4, 4)), np.ones(4))
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_dense_solve_with_row_exchanges(vk, precision):
+    """Rows of a diagonally dominant matrix shuffled, so nearly every column
+    pivots on another row (the register panels track row positions instead of
+    moving rows), and small-integer matrices with many equal magnitudes (the
+    first-maximum pivot rule): solutions within 1e-11 of numpy's, for panels
+    factored in registers (<= 128 rows), through LDS and from global memory."""
+    rng = np.random.default_rng(8)
+    for n in (5, 17, 64, 120, 129, 142, 300):
+        A = rng.standard_normal((n, n)) + 0.1 * n * np.eye(n)
+        A = A[rng.permutation(n)]
+        b = rng.standard_normal(n)
+        x = vk.dense_solve(A, b, precision=precision)
+        assert relerr(x, np.linalg.solve(A, b)) < 1e-11, (n, precision)
+        B = rng.integers(-3, 4, (n, n)).astype(float) + 4.0 * np.sqrt(n) * np.eye(n)
+        B = B[rng.permutation(n)]
+        x = vk.dense_solve(B, b, precision=precision)
+        assert relerr(x, np.linalg.solve(B, b)) < 1e-11, (n, precision, "ties")
+
+
 @pytest.mark.parametrize("kernel", ["gaussianerf", "winckelmans"])
 def test_particle_velocity_and_jacobian(oracle, ctx, kernel):
     from fluxvortex_amd import kernel as K
@@ -192,6 +212,28 @@ def test_particle_edge_cases(oracle, ctx):
         assert np.all(np.isfinite(u)) and np.all(np.isfinite(J))
         assert relerr(u, oracle.particle_vel(pos, pos, al, sg, kern)) < TOL64
         assert relerr(J, oracle.particle_jac(pos, pos, al, sg, kern)) < TOL64
+    # pairs just inside (r^2 = 1e-26) and just outside (r^2 = 4e-24) the
+    # exclusion radius, with cores from the 1e-6 floor to 1: excluded pairs
+    # contribute exactly nothing, included ones their full term.  The
+    # Gaussian-erf factor of the included pair (rho <= 2e-6) is the
+    # difference of two nearly equal terms in the reference formula itself
+    # (g ~ rho^3 from erf - rho e^..), so that pair enters the Winckelmans
+    # check only.
+    base = np.array([[0.0, 0, 0], [1e-13, 0, 0], [0.3, 0.1, 0], [0.3, 0.1, 1e-13]])
+    bal = np.array([[0.0, 0, 1], [0, 1, 0], [0.2, 0.1, 0.3], [-0.1, 0.4, 0]])
+    for sgv in (1e-6, 0.1, 1.0):
+        for kern in ("gaussianerf", "winckelmans"):
+            pos, al = base, bal
+            if kern == "winckelmans":
+                pos = np.concatenate([base, [[0.0, 2e-12, 0.0]]])
+                al = np.concatenate([bal, [[1.0, 0.0, 0.0]]])
+            tgt = np.concatenate([pos, [[0.05, 0.0, 0.0]]])
+            sg = np.full(len(pos), sgv)
+            u = K.velocity_from_particles(tgt, pos, al, sg, kernel=kern)
+            J = K.jacobian_from_particles(tgt, np.ones_like(tgt), pos, al, sg, kernel=kern)
+            assert np.all(np.isfinite(u)) and np.all(np.isfinite(J)), (sgv, kern)
+            assert relerr(u, oracle.particle_vel(tgt, pos, al, sg, kern)) < TOL64, (sgv, kern)
+            assert relerr(J, oracle.particle_jac(tgt, pos, al, sg, kern)) < TOL64, (sgv, kern)
 
 
 @pytest.mark.parametrize("kernel", ["gaussianerf", "winckelmans"])
