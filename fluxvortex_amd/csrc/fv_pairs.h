@@ -1,12 +1,13 @@
 // fv_pairs.h -- device math of the pair interactions and the tiled pair-sum
 // kernel shared by every source/target sum in the library.
 //
-// Mapping (gfx950, wave64): one target per lane; a workgroup of BLOCK lanes
-// stages BLOCK source records at a time in LDS from coalesced HBM reads and
-// every lane walks the tile in the same order (LDS broadcast reads).  Each lane
-// keeps its own accumulators, so sums are atomic-free and bitwise
-// reproducible.  Large source sets are split over blockIdx.y into fixed chunks
-// whose partial sums are added in chunk order by k_reduce_partials.
+// Mapping (gfx950, wave64): TPT targets per lane (1, 2 or 4); a workgroup of
+// BLOCK lanes stages BLOCK source records at a time in LDS from coalesced HBM
+// reads (per-source constants formed once there) and every lane walks the
+// tile in the same order (LDS broadcast reads).  Each lane keeps its own
+// accumulators, so sums are atomic-free and bitwise reproducible.  Large
+// source sets are split over blockIdx.y into fixed chunks whose partial sums
+// are added in chunk order by k_reduce_partials.
 #pragma once
 
 #include <hip/hip_runtime.h>
