@@ -236,6 +236,10 @@ class AeroInstance:
     tip_w: float = 0.0
     tip_theta: float = 0.0
     precision: str = "fp64"
+    # True: the wake's vertex rows up to n_keep move with V + the induced
+    # velocity (PteraSoftware prescribed_wake=False); False: with V (the
+    # reference benchmark's prescribed wake)
+    ring_free_wake: bool = False
 
     @property
     def speed(self) -> float:
@@ -245,14 +249,15 @@ class AeroInstance:
 def goland_instance(V, dt=0.003, num_chords=100.0, alpha_deg=2.0, nc=4, ns=8, n_keep=10,
                     kernel="gaussianerf", tip_w=0.05, tip_theta_deg=2.0, x_ea_chord=0.33,
                     couple=True, num_steps=None, precision="fp64", feedback=False,
-                    lift_sign=None) -> AeroInstance:
+                    lift_sign=None, ring_free_wake=False) -> AeroInstance:
     """The Goland wing at airspeed V (benchmark_goland.py:21-63, 100-119):
     rectangular NACA 0012 wing (zero camber), chord 1.8288 m, semi-span
     6.096 m, nc x ns uniform panels from the root (y = 0) outwards, alpha 2 deg,
     dt 0.003 s, num_chords chord lengths of wake (num_steps = ceil(num_chords c
     / (V dt))), tip perturbation w = 0.05 m, theta = 2 deg, elastic axis at 33 %
-    chord; the hybrid wake keeps n_keep ring rows.  feedback / lift_sign as in
-    AeroInstance (lift_sign defaults to -1 without feedback, +1 with it)."""
+    chord; the hybrid wake keeps n_keep ring rows.  feedback / lift_sign /
+    ring_free_wake as in AeroInstance (lift_sign defaults to -1 without
+    feedback, +1 with it)."""
     chord, span = 1.8288, 6.096
     xs = np.linspace(0.0, chord, nc + 1)
     ys = np.linspace(0.0, span, ns + 1)
@@ -268,7 +273,8 @@ def goland_instance(V, dt=0.003, num_chords=100.0, alpha_deg=2.0, nc=4, ns=8, n_
                         x_ea=x_ea_chord * chord, tip_w=tip_w,
                         tip_theta=math.radians(tip_theta_deg), precision=precision,
                         feedback=bool(feedback),
-                        lift_sign=(1.0 if feedback else -1.0) if lift_sign is None else lift_sign)
+                        lift_sign=(1.0 if feedback else -1.0) if lift_sign is None else lift_sign,
+                        ring_free_wake=bool(ring_free_wake))
 
 
 # ---------------------------------------------------------------------------
@@ -289,6 +295,7 @@ class SweepDesc(C.Structure):
         ("v0", C.POINTER(C.c_double)), ("a0", C.POINTER(C.c_double)), ("beta", C.c_double),
         ("gamma", C.c_double), ("x_ea", C.c_double), ("relax", C.c_double),
         ("precision", C.c_int32), ("feedback", C.c_int32), ("lift_sign", C.c_double),
+        ("free_ring_wake", C.c_int32),
     ]
 
 
@@ -332,6 +339,7 @@ class Sweep:
             d.precision = _lib.PRECISIONS[it.precision]
             d.feedback = 1 if it.feedback else 0
             d.lift_sign = float(it.lift_sign)
+            d.free_ring_wake = 1 if it.ring_free_wake else 0
             if it.beam is not None:
                 arr = beam_arrays(it)
                 self._keep.append(arr)

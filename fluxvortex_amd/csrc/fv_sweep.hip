@@ -59,14 +59,24 @@ struct SwInst {
     int status;                  // 0; k + 1: zero pivot at step k; -2: particle overflow
     int has_prev, par, nrow, has_wprev, prec, feedback;
     int hbm;                     // wake rings in HBM, velocities from k_sw_wake partials
-    int64_t num_steps, step, cap, np, max_chunks, max_wchunks;
+    int free_wake;               // free ring wake: vertex rows move with V + u
+    int64_t num_steps, step, cap, np, max_chunks, max_wchunks, max_targets;
     double V[3], rho, dt, nu, rc0b, rc0w, sigma, sref, x_ea, relax, beta, gamma, lift_sign;
     // offsets (doubles) into the state block
     int64_t o_grid0, o_coln, o_colp, o_legp, o_gamp, o_gam, o_wv[2], o_wg[2], o_wa[2];
     int64_t o_pos, o_al, o_sg, o_part, o_M, o_C, o_K, o_d, o_v, o_a;
     int64_t o_wdef, o_tdef, o_wprev, o_tprev, o_yn, o_cl, o_cdi, o_tw, o_tt;
     int64_t o_legn, o_wpart;     // the step's leg centres; wake partial sums (HBM wake)
+    int64_t o_ten, o_vvel;       // the step's trailing-edge points; free-vertex velocities
 };
+
+// Free ring wake: vertex rows 0 .. nfree - 1 move with the induced velocity
+// (row 0: the trailing edge of the step's lattice; rows >= 1: the wake's old
+// vertex rows; rows beyond n_keep hold no circulation).
+__device__ __forceinline__ int sw_nfree(const SwInst &I)
+{
+    return I.free_wake ? (I.nrow < I.n_keep ? I.nrow : I.n_keep) + 1 : 0;
+}
 
 // ------------------------------------------------------------ geometry helpers
 __device__ __forceinline__ void sw_lerp(const double *a, const double *b, double t, double *o)
@@ -258,30 +268,37 @@ __global__ __launch_bounds__(kSwBlock) void k_sw_induce(const SwInst *__restrict
 }
 
 // ------------------------------------------------------------ HBM ring wake
-// Wake-ring velocities at the T = P + NL targets of the step (collocation
-// points, then the leg centres in slot order) from the segments [c0, c1) of
-// this chunk (segment k: leg k & 3 of ring k >> 2), partial sums
-// wpart[(chunk T + target) 3 + c].
+// Wake-ring velocities at the T = P + NL + NV targets of the step (collocation
+// points, the leg centres in slot order, then with a free ring wake the NV =
+// nfree (ns + 1) vertices of rows 0 .. nfree - 1) from the segments [c0, c1)
+// of this chunk (segment k: leg k & 3 of ring k >> 2), 256 targets per
+// workgroup (blockIdx.z), partial sums wpart[(chunk T + target) 3 + c].
 __global__ __launch_bounds__(kSwBlock) void k_sw_wake(const SwInst *__restrict__ insts,
                                                       double *__restrict__ S)
 {
     __shared__ SegRec tile[kSwWTile];
     const SwInst &I = insts[blockIdx.y];
     if (!I.hbm || I.status != 0 || I.step >= I.num_steps) return;
-    const int ns = I.ns, P = I.P, NL = 3 * P + ns, T = P + NL;
+    const int ns = I.ns, P = I.P, NL = 3 * P + ns;
+    const int T = P + NL + sw_nfree(I) * (ns + 1);
     const int64_t nseg = 4 * (int64_t)I.nrow * ns;
     const int64_t c0 = (int64_t)blockIdx.x * kSwWChunk;
-    if (c0 >= nseg) return;   // uniform over the workgroup
+    const int tz = (int)blockIdx.z * kSwBlock;
+    if (c0 >= nseg || tz >= T) return;   // uniform over the workgroup
     const int64_t c1 = (c0 + kSwWChunk < nseg) ? c0 + kSwWChunk : nseg;
     const double *wv = S + I.o_wv[I.par], *wg = S + I.o_wg[I.par], *wa = S + I.o_wa[I.par];
-    constexpr int kTpt = (kSwMaxP + 3 * kSwMaxP + kSwMaxNs + kSwBlock - 1) / kSwBlock;
+    constexpr int kTpt = 1;
     double u[kTpt][3];
     const double *x[kTpt];
 #pragma unroll
     for (int q = 0; q < kTpt; ++q) {
-        const int tq = threadIdx.x + q * kSwBlock;
+        const int tq = tz + threadIdx.x + q * kSwBlock;
         const int tc = tq < T ? tq : 0;
-        x[q] = tc < P ? S + I.o_coln + 3 * tc : S + I.o_legn + 3 * (tc - P);
+        const int tv = tc - P - NL;   // free vertex: row tv / (ns + 1), column tv % (ns + 1)
+        x[q] = tc < P ? S + I.o_coln + 3 * tc
+             : tv < 0 ? S + I.o_legn + 3 * (tc - P)
+             : tv < ns + 1 ? S + I.o_ten + 3 * tv
+                           : wv + 3 * tv;
         u[q][0] = u[q][1] = u[q][2] = 0.0;
     }
     for (int64_t base = c0; base < c1; base += kSwWTile) {
@@ -300,7 +317,7 @@ __global__ __launch_bounds__(kSwBlock) void k_sw_wake(const SwInst *__restrict__
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < kTpt; ++q) {
-            if ((int)threadIdx.x + q * kSwBlock < T) {
+            if (tz + (int)threadIdx.x + q * kSwBlock < T) {
                 const double xp[3] = {x[q][0], x[q][1], x[q][2]};
                 for (int k = 0; k < cnt; ++k) sw_pair(I.prec, xp, tile[k], u[q]);
             }
@@ -309,7 +326,7 @@ __global__ __launch_bounds__(kSwBlock) void k_sw_wake(const SwInst *__restrict__
     double *part = S + I.o_wpart + (int64_t)blockIdx.x * T * 3;
 #pragma unroll
     for (int q = 0; q < kTpt; ++q) {
-        const int tq = threadIdx.x + q * kSwBlock;
+        const int tq = tz + threadIdx.x + q * kSwBlock;
         if (tq < T)
             for (int c = 0; c < 3; ++c) part[3 * tq + c] = u[q][c];
     }
@@ -426,7 +443,8 @@ __global__ __launch_bounds__(kSwStepBlock) void k_sw_step(SwInst *__restrict__ i
     // ---- wake-ring velocity at the collocation points (kSwParts partial sums;
     // the HBM wake's come from k_sw_wake)
     const int W = nrow * ns;
-    const int T = P + NL;
+    const int nfree = sw_nfree(I), NV = nfree * (ns + 1);
+    const int T = P + NL + NV;
     const int nwch = (int)((4 * (int64_t)W + kSwWChunk - 1) / kSwWChunk);
     for (int e = t; e < P * kSwParts && !hbm; e += kSwStepBlock) {
         const int p = e / kSwParts, q = e % kSwParts;
@@ -511,6 +529,34 @@ __global__ __launch_bounds__(kSwStepBlock) void k_sw_step(SwInst *__restrict__ i
             vl[3 * tq + c] = a;
         }
     __syncthreads();
+    // ---- free ring wake: velocities at vertex rows 0 .. nfree - 1 from the
+    // bound rings (this step's circulation) + the wake rings (k_sw_wake's
+    // partial sums), stored for the shedding below
+    if (NV > 0) {
+        double *vv = S + I.o_vvel;
+        for (int e = t; e < NV; e += kSwStepBlock) {
+            const int r = e / (ns + 1), j = e % (ns + 1);
+            double x[3];
+            if (r == 0) {
+                const double *va = sw_v(g, ns, nc - 1, j), *vb = sw_v(g, ns, nc, j);
+                for (int c = 0; c < 3; ++c) x[c] = vb[c] + 0.25 * (vb[c] - va[c]);
+            } else {
+                for (int c = 0; c < 3; ++c) x[c] = wv[3 * e + c];
+            }
+            double u[3] = {0.0, 0.0, 0.0};
+            for (int k = 0; k < 4 * P; ++k) {
+                const int jr = k >> 2, leg = k & 3;
+                const double *R = ring + 12 * jr;
+                const SegRec sr = sw_seg(R + 3 * leg, R + 3 * ((leg + 1) & 3), gam[jr], I.rc0b, 0.0, I.nu);
+                sw_pair(I.prec, x, sr, u);
+            }
+            double w3[3] = {0.0, 0.0, 0.0};
+            for (int ch = 0; ch < nwch; ++ch)
+                for (int c = 0; c < 3; ++c)
+                    w3[c] += S[I.o_wpart + ((int64_t)ch * T + P + NL + e) * 3 + c];
+            for (int c = 0; c < 3; ++c) vv[3 * e + c] = u[c] + w3[c];
+        }
+    }
     // ---- loads: Kutta-Joukowski on each leg with the net circulation of its
     // line (right / left against the spanwise neighbour, front against the ring
     // ahead, the trailing-edge back leg against the newest wake ring) + the
@@ -670,6 +716,17 @@ __global__ __launch_bounds__(kSwStepBlock) void k_sw_step(SwInst *__restrict__ i
                 sw_deform(grid0 + 3 * ((nc - 1) * (ns + 1) + j), yn, wnew, tnew, nn, I.x_ea, va);
                 sw_deform(grid0 + 3 * (nc * (ns + 1) + j), yn, wnew, tnew, nn, I.x_ea, vb);
                 for (int c = 0; c < 3; ++c) o[c] = vb[c] + 0.25 * (vb[c] - va[c]);
+            } else if (r - 1 < nfree) {
+                // free vertex: old row r - 1 moves with V + u (frozen snapshot)
+                const double *w = S + I.o_vvel + 3 * ((r - 1) * (ns + 1) + j);
+                double x0[3];
+                if (r == 1) {
+                    const double *va = sw_v(g, ns, nc - 1, j), *vb = sw_v(g, ns, nc, j);
+                    for (int c = 0; c < 3; ++c) x0[c] = vb[c] + 0.25 * (vb[c] - va[c]);
+                } else {
+                    for (int c = 0; c < 3; ++c) x0[c] = wv[3 * ((r - 1) * (ns + 1) + j) + c];
+                }
+                for (int c = 0; c < 3; ++c) o[c] = x0[c] + (V[c] + w[c]) * dt;
             } else if (r == 1) {
                 const double *va = sw_v(g, ns, nc - 1, j), *vb = sw_v(g, ns, nc, j);
                 for (int c = 0; c < 3; ++c) o[c] = vb[c] + 0.25 * (vb[c] - va[c]);
@@ -756,6 +813,12 @@ __global__ __launch_bounds__(kSwStepBlock) void k_sw_step(SwInst *__restrict__ i
             for (int c = 0; c < 3; ++c) S[I.o_coln + 3 * p + c] = cn[c];
             if (hbm) sw_leg_centres(gn, nc, ns, P, p, S + I.o_legn);
         }
+        if (I.free_wake)
+            for (int j = t; j <= ns; j += kSwStepBlock) {
+                const double *va = sw_v(gn, ns, nc - 1, j), *vb = sw_v(gn, ns, nc, j);
+                for (int c = 0; c < 3; ++c)
+                    S[I.o_ten + 3 * j + c] = vb[c] + 0.25 * (vb[c] - va[c]);
+            }
         if (t == 0) {
             I.nrow = nrow_new;
             I.par = nxt;
@@ -796,6 +859,12 @@ __global__ __launch_bounds__(64) void k_sw_init(const SwInst *__restrict__ insts
         for (int c = 0; c < 3; ++c) S[I.o_coln + 3 * p + c] = cn[c];
         if (I.hbm) sw_leg_centres(g, I.nc, I.ns, I.P, p, S + I.o_legn);
     }
+    if (I.free_wake)
+        for (int j = threadIdx.x; j <= I.ns; j += 64) {
+            const double *va = sw_v(S + I.o_grid0, I.ns, I.nc - 1, j);
+            const double *vb = sw_v(S + I.o_grid0, I.ns, I.nc, j);
+            for (int c = 0; c < 3; ++c) S[I.o_ten + 3 * j + c] = vb[c] + 0.25 * (vb[c] - va[c]);
+        }
 }
 
 }  // namespace fv
@@ -808,7 +877,7 @@ struct fv_sweep {
     std::vector<SwInst> host;
     SwInst *dinst = nullptr;
     double *block = nullptr;
-    int64_t max_chunks = 1, max_wchunks = 0, max_steps = 0, done_steps = 0;
+    int64_t max_chunks = 1, max_wchunks = 0, max_tchunks = 1, max_steps = 0, done_steps = 0;
     int use_graph = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -818,7 +887,8 @@ static int sweep_launch(fv_sweep *sw)
 {
     fv_ctx *ctx = sw->ctx;
     if (sw->max_wchunks > 0) {
-        hipLaunchKernelGGL(k_sw_wake, dim3((unsigned)sw->max_wchunks, (unsigned)sw->n),
+        hipLaunchKernelGGL(k_sw_wake,
+                           dim3((unsigned)sw->max_wchunks, (unsigned)sw->n, (unsigned)sw->max_tchunks),
                            dim3(kSwBlock), 0, ctx->stream, sw->dinst, sw->block);
         FV_LAUNCH_CHECK(ctx);
     }
@@ -873,7 +943,8 @@ int fv_sweep_create(fv_ctx *ctx, const fv_sweep_desc *descs, int32_t n, int32_t 
                              i, (long long)R);
         }
         I.R = (int)R;
-        I.hbm = I.R > kSwMaxR ? 1 : 0;
+        I.free_wake = d.free_ring_wake ? 1 : 0;
+        I.hbm = (I.R > kSwMaxR || I.free_wake) ? 1 : 0;
         I.n_keep = d.n_keep;
         I.kernel = d.kernel;
         I.prec = d.precision;
@@ -915,12 +986,21 @@ int fv_sweep_create(fv_ctx *ctx, const fv_sweep_desc *descs, int32_t n, int32_t 
         I.o_sg = take(I.cap);
         I.o_part = take(3 * I.P * I.max_chunks);
         if (I.hbm) {
-            // the wake holds at most R rows when a step starts
+            // the wake holds at most R rows when a step starts; the free vertex
+            // rows are at most min(R, n_keep) + 1
             I.max_wchunks = ceil_div(4 * (int64_t)I.R * d.ns, kSwWChunk);
             if (I.max_wchunks < 1) I.max_wchunks = 1;
             if (I.max_wchunks > sw->max_wchunks) sw->max_wchunks = I.max_wchunks;
+            const int64_t nv = I.free_wake
+                                   ? ((int64_t)(I.R < d.n_keep ? I.R : d.n_keep) + 1) * (d.ns + 1)
+                                   : 0;
+            I.max_targets = 4 * (int64_t)I.P + d.ns + nv;
+            const int64_t tch = ceil_div(I.max_targets, kSwBlock);
+            if (tch > sw->max_tchunks) sw->max_tchunks = tch;
             I.o_legn = take(3 * (3 * I.P + d.ns));
-            I.o_wpart = take(3 * (int64_t)(4 * I.P + d.ns) * I.max_wchunks);
+            I.o_wpart = take(3 * I.max_targets * I.max_wchunks);
+            I.o_ten = take(3 * (d.ns + 1));
+            I.o_vvel = take(3 * (nv > 0 ? nv : 1));
         }
         I.o_M = take((int64_t)nfree * nfree);
         I.o_C = take((int64_t)nfree * nfree);

@@ -391,6 +391,9 @@ typedef struct fv_sweep_desc {
                                   * aerodynamics; 0: the aerodynamics stay on the
                                   * undeformed lattice (the reference's behaviour) */
     double lift_sign;            /* station lift = lift_sign * F_z (reference: -1) */
+    int32_t free_ring_wake;      /* 1: wake vertex rows 0 .. n_keep move with V + the
+                                  * induced velocity (frozen snapshot, as
+                                  * fv_hybrid_set_ring_wake); 0: prescribed (V dt) */
 } fv_sweep_desc;
 typedef struct fv_sweep fv_sweep;
 int fv_sweep_create(fv_ctx *ctx, const fv_sweep_desc *descs, int32_t n, int32_t use_graph,

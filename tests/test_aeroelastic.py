@@ -161,3 +161,14 @@ def test_oracle_feedback_switch(oracle):
     fb = oracle.aeroelastic_run(ae.goland_instance(140.0, feedback=True, **base))
     assert np.array_equal(rigid["cl"], nofb["cl"])
     assert np.abs(fb["cl"] - rigid["cl"]).max() > 1e-4
+
+
+def test_oracle_sweep_free_ring_wake_differs_from_prescribed(oracle):
+    """The oracle's aeroelastic run with a free ring wake: finite, the same
+    first step as the prescribed wake (no wake yet) and different loads once
+    the wake has rolled."""
+    a = oracle.aeroelastic_run(ae.goland_instance(120.0, num_chords=5, ring_free_wake=True))
+    b = oracle.aeroelastic_run(ae.goland_instance(120.0, num_chords=5))
+    assert np.all(np.isfinite(a["cl"])) and np.all(np.isfinite(a["gamma"]))
+    assert a["cl"][0] == b["cl"][0]
+    assert np.abs(a["cl"] - b["cl"]).max() > 1e-6
