@@ -240,6 +240,9 @@ class AeroInstance:
     # velocity (PteraSoftware prescribed_wake=False); False: with V (the
     # reference benchmark's prescribed wake)
     ring_free_wake: bool = False
+    # rings whose corner-mean x exceeds truncate_x lose their circulation after
+    # shedding (PteraSoftware's wake truncation); inf: none
+    truncate_x: float = math.inf
 
     @property
     def speed(self) -> float:
@@ -249,14 +252,14 @@ class AeroInstance:
 def goland_instance(V, dt=0.003, num_chords=100.0, alpha_deg=2.0, nc=4, ns=8, n_keep=10,
                     kernel="gaussianerf", tip_w=0.05, tip_theta_deg=2.0, x_ea_chord=0.33,
                     couple=True, num_steps=None, precision="fp64", feedback=False,
-                    lift_sign=None, ring_free_wake=False) -> AeroInstance:
+                    lift_sign=None, ring_free_wake=False, truncate_x=math.inf) -> AeroInstance:
     """The Goland wing at airspeed V (benchmark_goland.py:21-63, 100-119):
     rectangular NACA 0012 wing (zero camber), chord 1.8288 m, semi-span
     6.096 m, nc x ns uniform panels from the root (y = 0) outwards, alpha 2 deg,
     dt 0.003 s, num_chords chord lengths of wake (num_steps = ceil(num_chords c
     / (V dt))), tip perturbation w = 0.05 m, theta = 2 deg, elastic axis at 33 %
     chord; the hybrid wake keeps n_keep ring rows.  feedback / lift_sign /
-    ring_free_wake as in AeroInstance (lift_sign defaults to -1 without
+    ring_free_wake / truncate_x as in AeroInstance (lift_sign defaults to -1 without
     feedback, +1 with it)."""
     chord, span = 1.8288, 6.096
     xs = np.linspace(0.0, chord, nc + 1)
@@ -274,7 +277,7 @@ def goland_instance(V, dt=0.003, num_chords=100.0, alpha_deg=2.0, nc=4, ns=8, n_
                         tip_theta=math.radians(tip_theta_deg), precision=precision,
                         feedback=bool(feedback),
                         lift_sign=(1.0 if feedback else -1.0) if lift_sign is None else lift_sign,
-                        ring_free_wake=bool(ring_free_wake))
+                        ring_free_wake=bool(ring_free_wake), truncate_x=float(truncate_x))
 
 
 # ---------------------------------------------------------------------------
@@ -295,7 +298,7 @@ class SweepDesc(C.Structure):
         ("v0", C.POINTER(C.c_double)), ("a0", C.POINTER(C.c_double)), ("beta", C.c_double),
         ("gamma", C.c_double), ("x_ea", C.c_double), ("relax", C.c_double),
         ("precision", C.c_int32), ("feedback", C.c_int32), ("lift_sign", C.c_double),
-        ("free_ring_wake", C.c_int32),
+        ("free_ring_wake", C.c_int32), ("truncate_wake", C.c_int32), ("truncate_x", C.c_double),
     ]
 
 
@@ -340,6 +343,8 @@ class Sweep:
             d.feedback = 1 if it.feedback else 0
             d.lift_sign = float(it.lift_sign)
             d.free_ring_wake = 1 if it.ring_free_wake else 0
+            d.truncate_wake = 1 if math.isfinite(it.truncate_x) else 0
+            d.truncate_x = float(it.truncate_x) if math.isfinite(it.truncate_x) else 0.0
             if it.beam is not None:
                 arr = beam_arrays(it)
                 self._keep.append(arr)

@@ -60,8 +60,10 @@ struct SwInst {
     int has_prev, par, nrow, has_wprev, prec, feedback;
     int hbm;                     // wake rings in HBM, velocities from k_sw_wake partials
     int free_wake;               // free ring wake: vertex rows move with V + u
+    int truncate;                // rings whose corner-mean x exceeds trunc_x lose Gamma
     int64_t num_steps, step, cap, np, max_chunks, max_wchunks, max_targets;
     double V[3], rho, dt, nu, rc0b, rc0w, sigma, sref, x_ea, relax, beta, gamma, lift_sign;
+    double trunc_x;
     // offsets (doubles) into the state block
     int64_t o_grid0, o_coln, o_colp, o_legp, o_gamp, o_gam, o_wv[2], o_wg[2], o_wa[2];
     int64_t o_pos, o_al, o_sg, o_part, o_M, o_C, o_K, o_d, o_v, o_a;
@@ -752,6 +754,17 @@ __global__ __launch_bounds__(kSwStepBlock) void k_sw_step(SwInst *__restrict__ i
             }
         }
         __syncthreads();
+        // truncation by ring corner-mean x (Fr + Fl + Bl + Br) / 4, after shedding
+        if (I.truncate) {
+            for (int e = t; e < nrow_new * ns; e += kSwStepBlock) {
+                const int r = e / ns, j = e % ns;
+                const double *a = wvn + 3 * (r * (ns + 1) + j), *b = a + 3;
+                const double *c = wvn + 3 * ((r + 1) * (ns + 1) + j), *f = c + 3;
+                const double cx = (b[0] + a[0] + c[0] + f[0]) * 0.25;
+                if (cx > I.trunc_x) wgn[e] = 0.0;
+            }
+            __syncthreads();
+        }
         // conversion of ring row n_keep (span order, legs Fr->Fl, Fl->Bl, Bl->Br, Br->Fr)
         if (I.n_keep < nrow_new && t == 0) {
             const int kr = I.n_keep;
@@ -944,6 +957,8 @@ int fv_sweep_create(fv_ctx *ctx, const fv_sweep_desc *descs, int32_t n, int32_t 
         }
         I.R = (int)R;
         I.free_wake = d.free_ring_wake ? 1 : 0;
+        I.truncate = d.truncate_wake ? 1 : 0;
+        I.trunc_x = d.truncate_x;
         I.hbm = (I.R > kSwMaxR || I.free_wake) ? 1 : 0;
         I.n_keep = d.n_keep;
         I.kernel = d.kernel;

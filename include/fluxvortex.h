@@ -394,6 +394,9 @@ typedef struct fv_sweep_desc {
     int32_t free_ring_wake;      /* 1: wake vertex rows 0 .. n_keep move with V + the
                                   * induced velocity (frozen snapshot, as
                                   * fv_hybrid_set_ring_wake); 0: prescribed (V dt) */
+    int32_t truncate_wake;       /* 1: after shedding, rings whose corner-mean x
+                                  * exceeds truncate_x lose their circulation */
+    double truncate_x;
 } fv_sweep_desc;
 typedef struct fv_sweep fv_sweep;
 int fv_sweep_create(fv_ctx *ctx, const fv_sweep_desc *descs, int32_t n, int32_t use_graph,

@@ -390,7 +390,7 @@ def aeroelastic_run(inst, threads=0):
     d.sref, d.max_particles = inst.sref, int(inst.max_particles)
     d.convect_exact = 1
     d.free_ring_wake = int(bool(getattr(inst, "ring_free_wake", False)))
-    d.truncate_x = np.inf
+    d.truncate_x = float(getattr(inst, "truncate_x", np.inf))
     n = int(inst.num_steps)
     tw, tt = np.zeros(n), np.zeros(n)
     sl, sm = np.zeros((n, inst.ns)), np.zeros((n, inst.ns))

@@ -172,3 +172,16 @@ def test_oracle_sweep_free_ring_wake_differs_from_prescribed(oracle):
     assert np.all(np.isfinite(a["cl"])) and np.all(np.isfinite(a["gamma"]))
     assert a["cl"][0] == b["cl"][0]
     assert np.abs(a["cl"] - b["cl"]).max() > 1e-6
+
+
+def test_oracle_sweep_wake_truncation(oracle):
+    """Truncating the ring wake at x = 9 m changes the loads once the wake has
+    passed it, and leaves the earlier steps unchanged."""
+    a = oracle.aeroelastic_run(ae.goland_instance(140.0, num_chords=10, n_keep=100000,
+                                                  truncate_x=9.0))
+    b = oracle.aeroelastic_run(ae.goland_instance(140.0, num_chords=10, n_keep=100000))
+    # the wake starts at the trailing edge (x ~ 1.9 m) and moves V dt per step:
+    # no ring passes x = 9 m in the first k steps
+    k = int((9.0 - 2.5) / (140.0 * 0.003))
+    assert np.array_equal(a["cl"][:k], b["cl"][:k])
+    assert np.abs(a["cl"] - b["cl"]).max() > 1e-8
