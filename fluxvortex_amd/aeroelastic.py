@@ -432,12 +432,16 @@ def main(argv=None):
     ap.add_argument("--n-keep", type=int, default=10)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--feedback", action="store_true")
+    ap.add_argument("--ring-free-wake", action="store_true",
+                    help="wake vertex rows move with the induced velocity")
+    ap.add_argument("--truncate-x", type=float, default=math.inf)
     ap.add_argument("--eager", action="store_true", help="launch without the hipGraph")
     a = ap.parse_args(argv)
     speeds = np.linspace(a.vmin, a.vmax, a.speeds)
     t0 = time.perf_counter()
     res = flutter_sweep(speeds, use_graph=not a.eager, num_chords=a.num_chords, n_keep=a.n_keep,
-                        precision=a.precision, feedback=a.feedback)
+                        precision=a.precision, feedback=a.feedback,
+                        ring_free_wake=a.ring_free_wake, truncate_x=a.truncate_x)
     sec = time.perf_counter() - t0
     print(json.dumps({"flutter_speed": res["flutter_speed"], "seconds": sec,
                       "sigma_w": {f"{r['V']:.3f}": round(r["sigma_w"], 6) for r in res["runs"]}}))

@@ -19,8 +19,8 @@
 // wake, the reference benchmark's configuration) keep the wake in HBM and add
 // a third launch ahead of the step:
 //   k_sw_wake    wake-ring velocities at the collocation points and leg
-//                centres of the step's lattice, one workgroup per (instance,
-//                2048-segment chunk), partial sums per chunk.
+//                centres of the step's lattice, one wavefront per (instance,
+//                512-segment chunk, 64 targets), partial sums per chunk.
 // The launches are captured once in a hipGraph and replayed per step (every
 // per-step quantity lives in device memory).
 //
@@ -51,8 +51,10 @@ constexpr int kSwBlock = 256;       // induction workgroup
 constexpr int kSwStepBlock = 1024;  // step workgroup: 16 waves, one instance per CU
 constexpr int kSwChunk = 1024;   // particles per induction partial
 constexpr int kSwParts = 8;      // segment partial sums per target
-constexpr int kSwWChunk = 2048;  // wake segments per partial sum (HBM wake)
-constexpr int kSwWTile = 256;    // wake segments staged in LDS at a time
+constexpr int kSwWChunk = 256;   // wake segments per partial sum (HBM wake)
+constexpr int kSwWBlock = 64;    // wake induction: one wavefront,
+constexpr int kSwWTpt = 1;       // one target per lane (many small workgroups)
+constexpr int kSwWTile = 64;     // wake segments staged in LDS at a time
 
 struct SwInst {
     int nc, ns, P, nv, R, n_keep, kernel, couple, n_nodes, nfree;
@@ -70,6 +72,7 @@ struct SwInst {
     int64_t o_wdef, o_tdef, o_wprev, o_tprev, o_yn, o_cl, o_cdi, o_tw, o_tt;
     int64_t o_legn, o_wpart;     // the step's leg centres; wake partial sums (HBM wake)
     int64_t o_ten, o_vvel;       // the step's trailing-edge points; free-vertex velocities
+    int64_t o_wsum;              // the wake partial sums added up, 3 per target
 };
 
 // Free ring wake: vertex rows 0 .. nfree - 1 move with the induced velocity
@@ -273,10 +276,10 @@ __global__ __launch_bounds__(kSwBlock) void k_sw_induce(const SwInst *__restrict
 // Wake-ring velocities at the T = P + NL + NV targets of the step (collocation
 // points, the leg centres in slot order, then with a free ring wake the NV =
 // nfree (ns + 1) vertices of rows 0 .. nfree - 1) from the segments [c0, c1)
-// of this chunk (segment k: leg k & 3 of ring k >> 2), 256 targets per
+// of this chunk (segment k: leg k & 3 of ring k >> 2), 64 targets per
 // workgroup (blockIdx.z), partial sums wpart[(chunk T + target) 3 + c].
-__global__ __launch_bounds__(kSwBlock) void k_sw_wake(const SwInst *__restrict__ insts,
-                                                      double *__restrict__ S)
+__global__ __launch_bounds__(kSwWBlock) void k_sw_wake(const SwInst *__restrict__ insts,
+                                                       double *__restrict__ S)
 {
     __shared__ SegRec tile[kSwWTile];
     const SwInst &I = insts[blockIdx.y];
@@ -285,16 +288,16 @@ __global__ __launch_bounds__(kSwBlock) void k_sw_wake(const SwInst *__restrict__
     const int T = P + NL + sw_nfree(I) * (ns + 1);
     const int64_t nseg = 4 * (int64_t)I.nrow * ns;
     const int64_t c0 = (int64_t)blockIdx.x * kSwWChunk;
-    const int tz = (int)blockIdx.z * kSwBlock;
+    const int tz = (int)blockIdx.z * (kSwWBlock * kSwWTpt);
     if (c0 >= nseg || tz >= T) return;   // uniform over the workgroup
     const int64_t c1 = (c0 + kSwWChunk < nseg) ? c0 + kSwWChunk : nseg;
     const double *wv = S + I.o_wv[I.par], *wg = S + I.o_wg[I.par], *wa = S + I.o_wa[I.par];
-    constexpr int kTpt = 1;
+    constexpr int kTpt = kSwWTpt;
     double u[kTpt][3];
     const double *x[kTpt];
 #pragma unroll
     for (int q = 0; q < kTpt; ++q) {
-        const int tq = tz + threadIdx.x + q * kSwBlock;
+        const int tq = tz + threadIdx.x + q * kSwWBlock;
         const int tc = tq < T ? tq : 0;
         const int tv = tc - P - NL;   // free vertex: row tv / (ns + 1), column tv % (ns + 1)
         x[q] = tc < P ? S + I.o_coln + 3 * tc
@@ -303,13 +306,21 @@ __global__ __launch_bounds__(kSwBlock) void k_sw_wake(const SwInst *__restrict__
                            : wv + 3 * tv;
         u[q][0] = u[q][1] = u[q][2] = 0.0;
     }
+    const int prec = I.prec;
+    bool act[kTpt];
+    double xp[kTpt][3];
+#pragma unroll
+    for (int q = 0; q < kTpt; ++q) {
+        act[q] = tz + (int)threadIdx.x + q * kSwWBlock < T;
+        for (int c = 0; c < 3; ++c) xp[q][c] = x[q][c];
+    }
     for (int64_t base = c0; base < c1; base += kSwWTile) {
         const int cnt = (int)((c1 - base < kSwWTile) ? c1 - base : kSwWTile);
         __syncthreads();
         if ((int)threadIdx.x < cnt) {
-            const int64_t k = base + threadIdx.x;
-            const int64_t rg = k >> 2;
-            const int leg = (int)(k & 3), r = (int)(rg / ns), j = (int)(rg % ns);
+            const int k = (int)(base + threadIdx.x);   // < 4 nrow ns < 2^31
+            const int rg = k >> 2;
+            const int leg = k & 3, r = rg / ns, j = rg % ns;
             const double *c4[4] = {wv + 3 * ((r + 1) * (ns + 1) + j + 1),   // Br
                                    wv + 3 * (r * (ns + 1) + j + 1),         // Fr
                                    wv + 3 * (r * (ns + 1) + j),             // Fl
@@ -317,18 +328,18 @@ __global__ __launch_bounds__(kSwBlock) void k_sw_wake(const SwInst *__restrict__
             tile[threadIdx.x] = sw_seg(c4[leg], c4[(leg + 1) & 3], wg[rg], I.rc0w, wa[rg], I.nu);
         }
         __syncthreads();
+#pragma unroll 2
+        for (int k = 0; k < cnt; ++k) {
+            const SegRec sr = tile[k];
 #pragma unroll
-        for (int q = 0; q < kTpt; ++q) {
-            if (tz + (int)threadIdx.x + q * kSwBlock < T) {
-                const double xp[3] = {x[q][0], x[q][1], x[q][2]};
-                for (int k = 0; k < cnt; ++k) sw_pair(I.prec, xp, tile[k], u[q]);
-            }
+            for (int q = 0; q < kTpt; ++q)
+                if (act[q]) sw_pair(prec, xp[q], sr, u[q]);
         }
     }
     double *part = S + I.o_wpart + (int64_t)blockIdx.x * T * 3;
 #pragma unroll
     for (int q = 0; q < kTpt; ++q) {
-        const int tq = tz + threadIdx.x + q * kSwBlock;
+        const int tq = tz + threadIdx.x + q * kSwWBlock;
         if (tq < T)
             for (int c = 0; c < 3; ++c) part[3 * tq + c] = u[q][c];
     }
@@ -463,13 +474,31 @@ __global__ __launch_bounds__(kSwStepBlock) void k_sw_step(SwInst *__restrict__ i
         for (int c = 0; c < 3; ++c) red[(p * kSwParts + q) * 3 + c] = u[c];
     }
     __syncthreads();
+    if (hbm) {
+        // k_sw_wake's chunk partials added per (target, component), four
+        // interleaved running sums combined in a fixed order
+        for (int e = t; e < 3 * T; e += kSwStepBlock) {
+            const double *pp = S + I.o_wpart + e;
+            double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+            int ch = 0;
+            for (; ch + 4 <= nwch; ch += 4) {
+                a0 += pp[(int64_t)ch * 3 * T];
+                a1 += pp[(int64_t)(ch + 1) * 3 * T];
+                a2 += pp[(int64_t)(ch + 2) * 3 * T];
+                a3 += pp[(int64_t)(ch + 3) * 3 * T];
+            }
+            for (; ch < nwch; ++ch) a0 += pp[(int64_t)ch * 3 * T];
+            S[I.o_wsum + e] = (a0 + a1) + (a2 + a3);
+        }
+        __syncthreads();
+    }
     const int nchunk = (int)((I.np + kSwChunk - 1) / kSwChunk);
     for (int p = t; p < P; p += kSwStepBlock) {
         double w3[3], q3[3] = {0.0, 0.0, 0.0};
         for (int c = 0; c < 3; ++c) {
             double a = 0.0;
             if (hbm)
-                for (int ch = 0; ch < nwch; ++ch) a += S[I.o_wpart + ((int64_t)ch * T + p) * 3 + c];
+                a = S[I.o_wsum + 3 * p + c];
             else
                 for (int q = 0; q < kSwParts; ++q) a += red[(p * kSwParts + q) * 3 + c];
             w3[c] = a;
@@ -525,9 +554,7 @@ __global__ __launch_bounds__(kSwStepBlock) void k_sw_step(SwInst *__restrict__ i
         for (int c = 0; c < 3; ++c) {
             double a = 0.0;
             for (int q = 0; q < kSwParts; ++q) a += red[(tq * kSwParts + q) * 3 + c];
-            if (hbm)
-                for (int ch = 0; ch < nwch; ++ch)
-                    a += S[I.o_wpart + ((int64_t)ch * T + P + tq) * 3 + c];
+            if (hbm) a += S[I.o_wsum + 3 * (P + tq) + c];
             vl[3 * tq + c] = a;
         }
     __syncthreads();
@@ -552,11 +579,7 @@ __global__ __launch_bounds__(kSwStepBlock) void k_sw_step(SwInst *__restrict__ i
                 const SegRec sr = sw_seg(R + 3 * leg, R + 3 * ((leg + 1) & 3), gam[jr], I.rc0b, 0.0, I.nu);
                 sw_pair(I.prec, x, sr, u);
             }
-            double w3[3] = {0.0, 0.0, 0.0};
-            for (int ch = 0; ch < nwch; ++ch)
-                for (int c = 0; c < 3; ++c)
-                    w3[c] += S[I.o_wpart + ((int64_t)ch * T + P + NL + e) * 3 + c];
-            for (int c = 0; c < 3; ++c) vv[3 * e + c] = u[c] + w3[c];
+            for (int c = 0; c < 3; ++c) vv[3 * e + c] = u[c] + S[I.o_wsum + 3 * (P + NL + e) + c];
         }
     }
     // ---- loads: Kutta-Joukowski on each leg with the net circulation of its
@@ -902,7 +925,7 @@ static int sweep_launch(fv_sweep *sw)
     if (sw->max_wchunks > 0) {
         hipLaunchKernelGGL(k_sw_wake,
                            dim3((unsigned)sw->max_wchunks, (unsigned)sw->n, (unsigned)sw->max_tchunks),
-                           dim3(kSwBlock), 0, ctx->stream, sw->dinst, sw->block);
+                           dim3(kSwWBlock), 0, ctx->stream, sw->dinst, sw->block);
         FV_LAUNCH_CHECK(ctx);
     }
     hipLaunchKernelGGL(k_sw_induce, dim3((unsigned)sw->max_chunks, (unsigned)sw->n),
@@ -1010,10 +1033,11 @@ int fv_sweep_create(fv_ctx *ctx, const fv_sweep_desc *descs, int32_t n, int32_t 
                                    ? ((int64_t)(I.R < d.n_keep ? I.R : d.n_keep) + 1) * (d.ns + 1)
                                    : 0;
             I.max_targets = 4 * (int64_t)I.P + d.ns + nv;
-            const int64_t tch = ceil_div(I.max_targets, kSwBlock);
+            const int64_t tch = ceil_div(I.max_targets, kSwWBlock * kSwWTpt);
             if (tch > sw->max_tchunks) sw->max_tchunks = tch;
             I.o_legn = take(3 * (3 * I.P + d.ns));
             I.o_wpart = take(3 * I.max_targets * I.max_wchunks);
+            I.o_wsum = take(3 * I.max_targets);
             I.o_ten = take(3 * (d.ns + 1));
             I.o_vvel = take(3 * (nv > 0 ? nv : 1));
         }
