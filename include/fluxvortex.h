@@ -41,7 +41,8 @@ extern "C" {
 
 /* Arithmetic of the pair sums. */
 #define FV_PREC_FP64 0    /* plain FP64 */
-#define FV_PREC_MIXED 1   /* FP32 pairs, compensated FP32 accumulation */
+#define FV_PREC_MIXED 1   /* FP32 pair terms (segments) / kernel factors (particles),
+                            * FP64 sums; FP32 LU + FP64-residual refinement */
 
 /* Leg masks over ring corners (Br, Fr, Fl, Bl); leg k runs corner k -> k+1.
  * warp_kernels.py:268-273 (ring), :296-299 (chordwise), :350-354 (horseshoe). */
@@ -386,7 +387,7 @@ typedef struct fv_sweep_desc {
     double beta, gamma;          /* Newmark parameters */
     double x_ea;                 /* elastic axis x (absolute) */
     double relax;                /* deformation relaxation factor (1: none) */
-    int32_t precision;           /* FV_PREC_FP64, or FV_PREC_MIXED for FP32 pair terms */
+    int32_t precision;           /* FV_PREC_FP64, or FV_PREC_MIXED (FV_PREC_* above) */
     int32_t feedback;            /* 1: the deformed lattice carries the next step's
                                   * aerodynamics; 0: the aerodynamics stay on the
                                   * undeformed lattice (the reference's behaviour) */
