@@ -200,3 +200,23 @@ def test_sweep_wake_truncation_matches_oracle(oracle):
         assert _rel(got["tip_w"][ok], ref["tip_w"][ok]) < 1e-10, i
     assert np.abs(sw.read(1)["cl"] - sw.read(3)["cl"]).max() > 1e-8
     sw.close()
+
+
+def test_sweep_hbm_wake_at_the_panel_limit(oracle):
+    """64 panels (4 x 16, the largest lattice): 272 wake-induction targets per
+    instance (five 64-target groups), whole ring wake, prescribed and free,
+    FP64 against the oracle to 1e-10."""
+    insts = [ae.goland_instance(150.0, num_chords=6, nc=4, ns=16, n_keep=100000),
+             ae.goland_instance(150.0, num_chords=6, nc=4, ns=16, n_keep=100000,
+                                ring_free_wake=True)]
+    sw = ae.Sweep(insts, use_graph=True)
+    sw.step()
+    for i, it in enumerate(insts):
+        got = sw.read(i)
+        ref = oracle.aeroelastic_run(it)
+        assert got["status"] == 0
+        assert _rel(got["cl"], ref["cl"]) < 1e-10, i
+        assert _rel(got["gamma"], ref["gamma"]) < 1e-10, i
+        ok = ~np.isnan(ref["tip_w"])
+        assert _rel(got["tip_w"][ok], ref["tip_w"][ok]) < 1e-10, i
+    sw.close()
