@@ -97,6 +97,7 @@ __device__ __forceinline__ int panel_factor_regs(T *P, int ldP, int m, int kb, i
     const int lane = threadIdx.x & 63;
     constexpr int kNone = 1 << 30;
     T v[2][kNb];
+    TB bv[2];   // right-hand side entries of the register rows (they travel with the rows)
     int pos[2];
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
@@ -104,10 +105,12 @@ __device__ __forceinline__ int panel_factor_regs(T *P, int ldP, int m, int kb, i
         pos[q] = r < m ? r : kNone;
 #pragma unroll
         for (int c = 0; c < kNb; ++c) v[q][c] = (r < m && c < kb) ? P[(int64_t)r * ldP + c] : (T)0;
+        bv[q] = (b && r < m) ? b[r] : (TB)0;
     }
     int info = 0;
     // fully unrolled over the panel's columns (jj a constant in every copy), so
-    // the register rows are never indexed at run time
+    // the register rows are never indexed at run time; everything the column
+    // steps touch is in registers, so they need no memory ordering
 #pragma unroll
     for (int jj = 0; jj < kNb; ++jj) {
         if (jj < kb) {
@@ -136,24 +139,20 @@ __device__ __forceinline__ int panel_factor_regs(T *P, int ldP, int m, int kb, i
 #pragma unroll
                     for (int q = 0; q < 2; ++q)
                         pos[q] = pos[q] == pl ? jj : (pos[q] == jj ? pl : pos[q]);
-                    if (b && lane == 0) {
-                        const TB t = b[jj];
-                        b[jj] = b[pl];
-                        b[pl] = t;
-                    }
                 }
-                wave_sync();
                 // pivot row broadcast (qp is uniform: a scalar branch)
                 T prow[kNb];
+                TB bj;
                 if (qp) {
 #pragma unroll
                     for (int c = 0; c < kNb; ++c) prow[c] = c > jj ? readlane_t(v[1][c], lp) : (T)0;
+                    bj = readlane_t(bv[1], lp);
                 } else {
 #pragma unroll
                     for (int c = 0; c < kNb; ++c) prow[c] = c > jj ? readlane_t(v[0][c], lp) : (T)0;
+                    bj = readlane_t(bv[0], lp);
                 }
                 const T inv = (T)1 / pv;
-                const TB bj = b ? b[jj] : (TB)0;
                 // rows below the pivot take l = v / pivot, the others l = 0 (an
                 // exact no-op update), so the update is branch-free
 #pragma unroll
@@ -164,18 +163,19 @@ __device__ __forceinline__ int panel_factor_regs(T *P, int ldP, int m, int kb, i
 #pragma unroll
                     for (int c = 0; c < kNb; ++c)
                         if (c > jj) v[q][c] = fma(-l, prow[c], v[q][c]);
-                    if (b && below) b[pos[q]] -= (TB)l * bj;
+                    bv[q] = below ? bv[q] - (TB)l * bj : bv[q];
                 }
             }
-            wave_sync();
         }
     }
 #pragma unroll
     for (int q = 0; q < 2; ++q)
-        if (pos[q] < m)
+        if (pos[q] < m) {
 #pragma unroll
             for (int c = 0; c < kNb; ++c)
                 if (c < kb) P[(int64_t)pos[q] * ldP + c] = v[q][c];
+            if (b) b[pos[q]] = bv[q];
+        }
     wave_sync();
     return info;
 }
