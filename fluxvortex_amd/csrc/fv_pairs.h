@@ -519,7 +519,7 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(typename Op::Args a, int64_t n_
         __syncthreads();
         if (threadIdx.x < cnt) tile[threadIdx.x] = Op::load_src(a, base + threadIdx.x);
         __syncthreads();
-#pragma unroll(TPT == 1 ? 2 : 1)
+#pragma unroll(TPT <= 2 ? 2 : 1)
         for (int j = 0; j < cnt; ++j) {
             const typename Op::Src src = tile[j];
 #pragma unroll
