@@ -370,6 +370,9 @@ template <int KERNEL, int PREC, int MODE>
 struct PartOp {
     static constexpr int kComp = (MODE == 1) ? 9 : 3;
     static constexpr int kPrec = PREC;
+    // source-loop unroll at two targets per lane: the Gaussian's erf / exp
+    // branch doubled would exceed 128 VGPRs (three wavefronts per SIMD)
+    static constexpr int kUnroll2 = (KERNEL == 0) ? 1 : 2;
     struct Args {
         PartSrcArgs src;
         PartTgtArgs tgt;
@@ -491,6 +494,16 @@ struct PartOp {
 // partial[(blockIdx.y * n_tgt_stride + i) * kComp + c] = sum over this
 // block's source chunk.  n_tgt / n_src come from *dyn_tgt / *dyn_src when
 // those are non-null (device-side particle counts), else from the arguments.
+// Source-loop unroll at two targets per lane: the op's kUnroll2, else 2.
+template <class Op, class = void>
+struct PairUnroll2 {
+    static constexpr int value = 2;
+};
+template <class Op>
+struct PairUnroll2<Op, decltype((void)Op::kUnroll2)> {
+    static constexpr int value = Op::kUnroll2;
+};
+
 template <class Op, int BLOCK, int TPT>
 __global__ __launch_bounds__(BLOCK) void k_pairs(typename Op::Args a, int64_t n_tgt_arg,
                                                  int64_t n_src_arg,
@@ -519,7 +532,7 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(typename Op::Args a, int64_t n_
         __syncthreads();
         if (threadIdx.x < cnt) tile[threadIdx.x] = Op::load_src(a, base + threadIdx.x);
         __syncthreads();
-#pragma unroll(TPT <= 2 ? 2 : 1)
+#pragma unroll(TPT == 1 ? 2 : TPT == 2 ? PairUnroll2<Op>::value : 1)
         for (int j = 0; j < cnt; ++j) {
             const typename Op::Src src = tile[j];
 #pragma unroll
