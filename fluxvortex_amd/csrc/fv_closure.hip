@@ -682,6 +682,10 @@ __global__ __launch_bounds__(64) void k_shell_reduce(int64_t nocc, const int32_t
 // chunk, dx slice), TPT targets per lane (1, 2 or 4; timed at first use when
 // the context autotunes, else 2), sources streamed 64 at a time through LDS;
 // each slice writes its own partial sums (summed in order later).
+// At one target per lane a chunk of p <= 32 targets (a cell's last, partly
+// filled chunk) runs as g = 64 / p lane groups of p: group q takes the tile's
+// sources q, q + g, q + 2g, ..., and the groups' sums are added in group order
+// before finish(), so the wavefront is not left mostly idle.
 template <int KERNEL, int PREC, int MODE, int TPT>
 __global__ __launch_bounds__(64) void k_near(
     int64_t n, int64_t nocc, const int32_t *__restrict__ occ, const int32_t *__restrict__ cstart,
@@ -699,14 +703,19 @@ __global__ __launch_bounds__(64) void k_near(
     const int tb = cstart[c], te = cend[c];
     const int t0 = tb + (int)blockIdx.y * (64 * TPT);
     if (t0 >= te) return;
+    // lane groups (TPT == 1 only; wave-uniform p and g)
+    const int p = (TPT == 1 && te - t0 < 64) ? te - t0 : 64;
+    const int g = 64 / p;
+    const int grp = (int)threadIdx.x / p;            // >= g: idle lanes past g * p
+    const int slot = (int)threadIdx.x - grp * p;
     typename Op::Tgt tg[TPT];
     typename Op::Acc acc[TPT];
     bool act[TPT];
 #pragma unroll
     for (int u = 0; u < TPT; ++u) {
-        const int ti = t0 + threadIdx.x + 64 * u;
-        act[u] = ti < te;
-        const int tl = act[u] ? ti : t0;
+        const int ti = t0 + slot + 64 * u;
+        act[u] = ti < te && grp == 0;
+        const int tl = ti < te ? ti : t0;
         tg[u].x = sx[tl]; tg[u].y = sy[tl]; tg[u].z = sz[tl];
         tg[u].ax = sax[tl]; tg[u].ay = say[tl]; tg[u].az = saz[tl];
         Op::init(acc[u]);
@@ -729,10 +738,36 @@ __global__ __launch_bounds__(64) void k_near(
                                                         say[j], saz[j], ssg[j]);
             }
             __syncthreads();
-            for (int k = 0; k < cnt; ++k) {
-                const PartRec sr = tile[k];
+            if (g == 1) {
+                for (int k = 0; k < cnt; ++k) {
+                    const PartRec sr = tile[k];
 #pragma unroll
-                for (int u = 0; u < TPT; ++u) Op::pair(tg[u], sr, acc[u]);
+                    for (int u = 0; u < TPT; ++u) Op::pair(tg[u], sr, acc[u]);
+                }
+            } else {
+                for (int k = grp; k < cnt; k += g) {
+                    const PartRec sr = tile[k];
+#pragma unroll
+                    for (int u = 0; u < TPT; ++u) Op::pair(tg[u], sr, acc[u]);
+                }
+            }
+        }
+    }
+    if (TPT == 1 && g > 1) {
+        constexpr int kW = Op::kComp + 3;
+        __shared__ double red[64 * kW];
+#pragma unroll
+        for (int k = 0; k < Op::kComp; ++k) red[threadIdx.x * kW + k] = acc[0].v[k];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) red[threadIdx.x * kW + Op::kComp + k] = acc[0].g[k];
+        __syncthreads();
+        if (grp == 0) {
+            for (int q = 1; q < g; ++q) {
+                const double *r = red + (slot + q * p) * kW;
+#pragma unroll
+                for (int k = 0; k < Op::kComp; ++k) acc[0].v[k] += r[k];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) acc[0].g[k] += r[Op::kComp + k];
             }
         }
     }
@@ -741,7 +776,7 @@ __global__ __launch_bounds__(64) void k_near(
     for (int u = 0; u < TPT; ++u) {
         Op::finish(tg[u], acc[u]);
         if (act[u]) {
-            const int64_t ti = t0 + threadIdx.x + 64 * u;
+            const int64_t ti = t0 + slot + 64 * u;
             for (int k = 0; k < 3; ++k) dst[3 * ti + k] = acc[u].v[k];
         }
     }
