@@ -154,7 +154,7 @@ def run_config(hybrid, particles, steps, warmup, precision, stretch=True, closur
         run.particles.enable_closure(net, **clo)
         if kl:
             run.particles.set_closure_local(kl)
-        run.particles.closure_train(rounds=200)
+        run.particles.closure_train(rounds=800)
     run.set_flags(True, stretch)
     run.step(warmup)
     out = timed(run, steps)
