@@ -118,6 +118,54 @@ def test_sheet_sweep_large(vk, oracle):
     assert np.array_equal(out, out2)
 
 
+@pytest.mark.parametrize("m_rings,n_part,kernel", [(1000, 10_000, "gaussianerf"),
+                                                   (10_000, 100_000, "winckelmans"),
+                                                   (10_000, 1_000_000, "winckelmans")])
+def test_raw_kernel_sweep_full_sizes(vk, oracle, m_rings, n_part, kernel):
+    """The seeded synthetic raw-kernel sweep at its full sizes: M rings on the
+    perturbed sheet, N particles uniform in the 20 x 10 x 2 slab with
+    alpha ~ N(0, 0.01^2) per component and sigma = 0.05; targets are the ring
+    centroids plus every particle.  The device evaluates every target; 256
+    random targets (and 256 particles for the stretching term S = J alpha)
+    are checked against the oracle's direct sums over all sources: FP64 to
+    1e-12, mixed to 1e-5 (normwise relative)."""
+    from fluxvortex_amd import kernel as K
+    nx = 40 if m_rings == 1000 else 125
+    br, fr, fl, bl, g, rc, age = sheet_rings(nx, m_rings // nx, seed=11)
+    rng = np.random.default_rng(n_part)
+    pos = rng.uniform([0.0, -5.0, -1.0], [20.0, 5.0, 1.0], (n_part, 3))
+    al = rng.normal(0.0, 0.01, (n_part, 3))
+    sg = np.full(n_part, 0.05)
+    tgt = np.concatenate([0.25 * (br + fr + fl + bl), pos])
+    pick = np.sort(rng.choice(len(tgt), 256, replace=False))
+    pick[0] = 0                                   # a collocation point is always in
+    ppick = np.sort(rng.choice(n_part, 256, replace=False))
+    s = np.zeros(4, np.int64)
+    nu = 1.5e-5
+    ref_ring = oracle.collapsed(tgt[pick], br, fr, fl, bl, g, rc, age, nu)
+    ref_part = oracle.particle_vel(tgt[pick], pos, al, sg, kernel)
+    ref_S = np.einsum("tij,tj->ti", oracle.particle_jac(pos[ppick], pos, al, sg, kernel),
+                      al[ppick])
+    errs = {}
+    for prec, bar in (("fp64", TOL64), ("mixed", 1e-5)):
+        vk.set_precision(prec)
+        try:
+            u_ring = vk.collapsed_velocities_from_ring_vortices(tgt, br, fr, fl, bl, g, rc, s,
+                                                                age, nu)
+        finally:
+            vk.set_precision("fp64")
+        u_part = K.velocity_from_particles(tgt, pos, al, sg, kernel=kernel, precision=prec)
+        J = K.jacobian_from_particles(pos, al, pos, al, sg, kernel=kernel, precision=prec)
+        S = np.einsum("tij,tj->ti", J[ppick], al[ppick])
+        assert u_ring.shape == (len(tgt), 3) and u_part.shape == (len(tgt), 3)
+        assert np.all(np.isfinite(u_part)) and np.all(np.isfinite(J))
+        errs[prec] = (relerr(u_ring[pick], ref_ring), relerr(u_part[pick], ref_part),
+                      relerr(S, ref_S))
+        print(f"M={m_rings} N={n_part} {kernel} {prec}: ring {errs[prec][0]:.2e} "
+              f"particles {errs[prec][1]:.2e} stretching {errs[prec][2]:.2e}", flush=True)
+        assert max(errs[prec]) < bar, (prec, errs[prec])
+
+
 def test_mixed_precision_collapsed(vk, oracle):
     br, fr, fl, bl, g, rc, age = sheet_rings(40, 25)
     rng = np.random.default_rng(1)
