@@ -213,3 +213,72 @@ def test_closure_local_expansion_on_a_wake():
         assert d < 1e-4, (m, d)
     got = f.closure_sum("velocity", use_mlp=False)
     assert np.abs(got - u).max() / np.abs(u).max() < 1e-3
+
+
+def _flapping_at(hybrid, n_particles, prefill=129):
+    """The 192-panel flapping airplane advanced three flapping periods (43
+    steps each) with a free, unstretched particle wake, its particle wake then
+    extended downstream by copies of the oldest period's particles shifted by
+    k V T until n_particles exist.  Deterministic: every call gives the same
+    state."""
+    case = hybrid.flapping_case(num_steps=prefill + 8, free_wake=True, stretch=False)
+    run = hybrid.HybridRun(case)
+    run.set_flags(True, False)
+    run.step(prefill)
+    st = run.particles.state_dict()
+    n0 = st["pos"].shape[0]
+    block = min(n0, 43 * 128)
+    parts = {k: [v] for k, v in st.items()}
+    need, k = n_particles - n0, 1
+    while need > 0:
+        take = min(need, block)
+        p = st["pos"][:take].copy()
+        p[:, 0] += k * 10.0 * 1.0          # V T, V = 10 m/s, T = 1 s
+        parts["pos"].append(p)
+        parts["gamma"].append(st["gamma"][:take])
+        parts["sigma"].append(st["sigma"][:take])
+        parts["age"].append(st["age"][:take] + k * 1.0)
+        need -= take
+        k += 1
+    run.particles.load_state_dict({k: np.concatenate(v) for k, v in parts.items()})
+    return run, prefill
+
+
+def test_flapping_airplane_closure_at_100k(ctx):
+    """The learned closure on the multi-surface flapping configuration (three
+    surfaces, 192 panels) at 100 000 particles, mixed precision: the cheapest
+    (cell, cutoff) whose check passes, three free-wake steps, CL within 1e-3
+    (relative to max |CL|) of the FP64 direct sums from the identical state."""
+    from fluxvortex_amd import hybrid
+    from fluxvortex_amd.mlp import ClosureMLP, closure_params
+    n = 100_000
+    ref, s0 = _flapping_at(hybrid, n)
+    assert ref.particles.np == n
+    ref.set_flags(True, False)
+    ref.step(3)
+    cl_ref = np.asarray(ref.read()[0][s0:s0 + 3]).copy()
+    ref.close()
+    chosen = None
+    for cell, kappa in ((2.5, 2.0), (2.0, 2.5), (2.0, 3.0), (2.0, 4.0)):
+        run, _ = _flapping_at(hybrid, n)
+        run.set_precision("mixed")
+        run.particles.enable_closure(ClosureMLP(closure_params(seed=0)), cell=cell, kappa=kappa,
+                                     check_every=10, tol=1e-3, train_samples=4096,
+                                     train_iters=2, lr=2e-3)
+        run.particles.set_closure_local(2.0 * kappa)
+        run.particles.closure_train(rounds=800)
+        run.set_flags(True, False)
+        run.step(3)
+        st = run.particles.closure_stats()
+        cl = np.asarray(run.read()[0][s0:s0 + 3]).copy()
+        run.close()
+        print(f"flapping 100k cell {cell} kappa {kappa}: {st}", flush=True)
+        if st["active"] and st["rejections"] == 0:
+            chosen = (cell, kappa, st, cl)
+            break
+    assert chosen is not None, "no closure candidate passed its check"
+    cell, kappa, st, cl = chosen
+    assert st["checks"] == 1 and st["last_err"] <= 1e-3 and st["far_evals"] > 0
+    err = float(np.max(np.abs(cl - cl_ref)) / np.max(np.abs(cl_ref)))
+    print(f"flapping 100k: CL error vs FP64 {err:.2e} (cell {cell}, kappa {kappa})", flush=True)
+    assert np.all(np.isfinite(cl)) and err < 1e-3
