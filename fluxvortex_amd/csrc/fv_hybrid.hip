@@ -49,6 +49,8 @@ struct fv_hybrid {
     int64_t step = 0;            // next step to run
     int64_t nrow = 0;            // ring rows currently in the wake
     int par = 0;                 // wake double-buffer parity
+    hipStream_t side = nullptr;  // the circulation solve's stream (overlaps the advection)
+    hipEvent_t ev_rhs = nullptr, ev_solved = nullptr;
     void *block = nullptr;
     double *grids = nullptr;
     double *br = nullptr, *fr = nullptr, *fl = nullptr, *bl = nullptr;
@@ -344,32 +346,34 @@ __global__ void k_wake_vertices(const double *__restrict__ gnext, const double *
     for (int c = 0; c < 3; ++c) wnew[3 * t + c] = v[c];
 }
 
-// Rings of one surface's new wake whose corner-mean x exceeds max_x lose
-// their circulation (warp_fsi/kernels_wake.py:110-126).
-__global__ void k_wake_truncate(const double *__restrict__ w, int ns, int64_t nrow,
+// Rings in rows [r0, r1) of one surface's new wake whose corner-mean x
+// exceeds max_x lose their circulation (warp_fsi/kernels_wake.py:110-126).
+__global__ void k_wake_truncate(const double *__restrict__ w, int ns, int64_t r0, int64_t r1,
                                 double max_x, double *__restrict__ g)
 {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nrow * ns) return;
-    const int64_t r = t / ns;
-    const int j = (int)(t % ns);
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (r1 - r0) * ns) return;
+    const int64_t r = r0 + q / ns;
+    const int j = (int)(q % ns);
+    const int64_t t = r * ns + j;
     const double *a = w + 3 * (r * (ns + 1) + j), *b = a + 3;
     const double *c = w + 3 * ((r + 1) * (ns + 1) + j), *e = c + 3;
     const double cx = (b[0] + a[0] + c[0] + e[0]) * 0.25;
     if (cx > max_x) g[t] = 0.0;
 }
 
-// New ring strengths / ages of one surface: row 0 = trailing-edge bound
-// circulations (age 0), row r+1 = old row r (age + dt).
+// New ring strengths / ages of rows [r0, r1) of one surface: row 0 =
+// trailing-edge bound circulations (age 0), row r+1 = old row r (age + dt).
 __global__ void k_wake_rings(const double *__restrict__ gam_te, int ns,
                              const double *__restrict__ gold, const double *__restrict__ aold,
-                             double *__restrict__ gnew, double *__restrict__ anew,
-                             int64_t nrow_new, double dt)
+                             double *__restrict__ gnew, double *__restrict__ anew, int64_t r0,
+                             int64_t r1, double dt)
 {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nrow_new * ns) return;
-    const int64_t r = t / ns;
-    const int j = (int)(t % ns);
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (r1 - r0) * ns) return;
+    const int64_t r = r0 + e / ns;
+    const int j = (int)(e % ns);
+    const int64_t t = r * ns + j;
     if (r == 0) {
         gnew[t] = gam_te[j];
         anew[t] = 0.0;
@@ -474,35 +478,83 @@ static int hybrid_step_one(fv_hybrid *h)
     hipLaunchKernelGGL(k_rhs, dim3((unsigned)ceil_div(P, 128)), dim3(128), 0, ctx->stream, P,
                        h->normal, h->vm_col, h->vw, vp, d.vinf[0], d.vinf[1], d.vinf[2], h->rhs);
     FV_LAUNCH_CHECK(ctx);
-    // circulation solve
-    if (prec == FV_PREC_FP64)
-        rc = dense_solve(ctx, 1, P, h->aic, h->rhs, h->info);
-    else
-        rc = dense_solve_mixed(ctx, 1, P, h->aic, h->rhs, 8, 1e-12, h->info, nullptr);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_copy, dim3((unsigned)ceil_div(P, 128)), dim3(128), 0, ctx->stream,
-                       h->rhs, h->gam, P);
-    FV_LAUNCH_CHECK(ctx);
-    // loads: velocities at leg centres from bound + wake rings
-    rc = build_segtab(ctx, h->br, h->fr, h->fl, h->bl, P, h->gam, h->brc0, nullptr, d.nu,
-                      FV_LEGS_RING, false, h->tab);
-    if (rc) return rc;
-    rc = seg_sum(ctx, h->leg_c, 4 * P, h->tab, 4 * P + 4 * W, prec, h->vl, S_PARTIAL);
-    if (rc) return rc;
-    h->seg_pairs += 4.0 * P * (4.0 * P + 4.0 * W);
-    hipLaunchKernelGGL(k_loads, dim3(1), dim3(kLoadBlock), 0, ctx->stream, (int)h->surf.size(),
-                       h->d_nc, h->d_ns, h->d_poff, h->d_woff, h->wg[cur], h->nrow, P, h->gam,
-                       h->gam_prev, s == 0 ? 1 : 0,
-                       h->leg_v, h->vm_leg, h->vl, h->normal, h->area, d.vinf[0], d.vinf[1],
-                       d.vinf[2], d.rho, 1.0 / dt, d.sref, s, h->force, h->cl, h->cdi);
-    FV_LAUNCH_CHECK(ctx);
-    hipLaunchKernelGGL(k_copy, dim3((unsigned)ceil_div(P, 128)), dim3(128), 0, ctx->stream,
-                       h->gam, h->gam_prev, P);
-    FV_LAUNCH_CHECK(ctx);
+    // circulation solve, loads, shedding, conversion, advection.  When the ring
+    // row that converts this step is at least one step old (n_keep >= 1) and the
+    // ring wake is prescribed, only the loads and the newest wake row need this
+    // step's circulation: the solve then runs on the side stream while this
+    // stream sheds the older rows, converts and advects the particles, and the
+    // loads and the newest row follow once the solve is done (the same kernels
+    // on the same inputs as the sequential order, so the results are identical).
+    const bool shed = s + 1 < d.num_steps;
+    const bool overlap = shed && d.n_keep >= 1 && !h->ring_free && h->side;
+    const int64_t nrow_old = h->nrow;
+    auto solve = [&]() -> int {
+        if (prec == FV_PREC_FP64) return dense_solve(ctx, 1, P, h->aic, h->rhs, h->info);
+        return dense_solve_mixed(ctx, 1, P, h->aic, h->rhs, 8, 1e-12, h->info, nullptr);
+    };
+    // loads: velocities at leg centres from bound + wake rings (this step's wake)
+    auto loads = [&]() -> int {
+        hipLaunchKernelGGL(k_copy, dim3((unsigned)ceil_div(P, 128)), dim3(128), 0, ctx->stream,
+                           h->rhs, h->gam, P);
+        FV_LAUNCH_CHECK(ctx);
+        int r = build_segtab(ctx, h->br, h->fr, h->fl, h->bl, P, h->gam, h->brc0, nullptr, d.nu,
+                             FV_LEGS_RING, false, h->tab);
+        if (r) return r;
+        r = seg_sum(ctx, h->leg_c, 4 * P, h->tab, 4 * P + 4 * W, prec, h->vl, S_PARTIAL);
+        if (r) return r;
+        h->seg_pairs += 4.0 * P * (4.0 * P + 4.0 * W);
+        hipLaunchKernelGGL(k_loads, dim3(1), dim3(kLoadBlock), 0, ctx->stream,
+                           (int)h->surf.size(), h->d_nc, h->d_ns, h->d_poff, h->d_woff, h->wg[cur],
+                           nrow_old, P, h->gam, h->gam_prev, s == 0 ? 1 : 0, h->leg_v, h->vm_leg,
+                           h->vl, h->normal, h->area, d.vinf[0], d.vinf[1], d.vinf[2], d.rho,
+                           1.0 / dt, d.sref, s, h->force, h->cl, h->cdi);
+        FV_LAUNCH_CHECK(ctx);
+        hipLaunchKernelGGL(k_copy, dim3((unsigned)ceil_div(P, 128)), dim3(128), 0, ctx->stream,
+                           h->gam, h->gam_prev, P);
+        FV_LAUNCH_CHECK(ctx);
+        return FV_OK;
+    };
+    // new ring strengths / ages of rows [r0, r1) of the next wake, then the
+    // truncation of those rows
+    auto wake_rows = [&](int nxt, int64_t r0, int64_t r1) -> int {
+        for (auto &sf : h->surf) {
+            const double *gam_te = h->gam + sf.panel_off + (int64_t)(sf.nc - 1) * sf.ns;
+            hipLaunchKernelGGL(k_wake_rings, dim3((unsigned)ceil_div((r1 - r0) * sf.ns, 128)),
+                               dim3(128), 0, ctx->stream, gam_te, sf.ns, h->wg[cur] + sf.wring_off,
+                               h->wa[cur] + sf.wring_off, h->wg[nxt] + sf.wring_off,
+                               h->wa[nxt] + sf.wring_off, r0, r1, dt);
+            FV_LAUNCH_CHECK(ctx);
+        }
+        if (std::isfinite(h->trunc_x))
+            for (auto &sf : h->surf) {
+                hipLaunchKernelGGL(k_wake_truncate,
+                                   dim3((unsigned)ceil_div((r1 - r0) * sf.ns, 128)), dim3(128), 0,
+                                   ctx->stream, h->wv[nxt] + 3 * sf.wvert_off, sf.ns, r0, r1,
+                                   h->trunc_x, h->wg[nxt] + sf.wring_off);
+                FV_LAUNCH_CHECK(ctx);
+            }
+        return FV_OK;
+    };
+    if (overlap) {
+        FV_HIP_CHECK(ctx, hipEventRecord(h->ev_rhs, ctx->stream));
+        FV_HIP_CHECK(ctx, hipStreamWaitEvent(h->side, h->ev_rhs, 0));
+        hipStream_t own = ctx->stream;
+        ctx->stream = h->side;
+        rc = solve();
+        ctx->stream = own;
+        if (rc) return rc;
+        FV_HIP_CHECK(ctx, hipEventRecord(h->ev_solved, h->side));
+    } else {
+        rc = solve();
+        if (rc) return rc;
+        rc = loads();
+        if (rc) return rc;
+    }
     // wake shedding + conversion (not after the last step)
-    if (s + 1 < d.num_steps) {
+    int nxt = cur;
+    if (shed) {
         const double *gnext = h->grids + (size_t)(s + 1) * h->TV * 3;
-        const int nxt = 1 - cur;
+        nxt = 1 - cur;
         const int64_t nrow_new = (h->nrow + 1 < h->R) ? h->nrow + 1 : h->R;
         if (h->ring_free) {
             // frozen snapshot: induced velocity at every old vertex that moves,
@@ -530,20 +582,12 @@ static int hybrid_step_one(fv_hybrid *h)
                                h->wv[nxt] + 3 * sf.wvert_off, nrow_new, d.vinf[0], d.vinf[1],
                                d.vinf[2], dt);
             FV_LAUNCH_CHECK(ctx);
-            const double *gam_te = h->gam + sf.panel_off + (int64_t)(sf.nc - 1) * sf.ns;
-            hipLaunchKernelGGL(k_wake_rings, dim3((unsigned)ceil_div(nrow_new * sf.ns, 128)),
-                               dim3(128), 0, ctx->stream, gam_te, sf.ns, h->wg[cur] + sf.wring_off,
-                               h->wa[cur] + sf.wring_off, h->wg[nxt] + sf.wring_off,
-                               h->wa[nxt] + sf.wring_off, nrow_new, dt);
-            FV_LAUNCH_CHECK(ctx);
         }
-        if (std::isfinite(h->trunc_x))
-            for (auto &sf : h->surf) {
-                hipLaunchKernelGGL(k_wake_truncate, dim3((unsigned)ceil_div(nrow_new * sf.ns, 128)),
-                                   dim3(128), 0, ctx->stream, h->wv[nxt] + 3 * sf.wvert_off, sf.ns,
-                                   nrow_new, h->trunc_x, h->wg[nxt] + sf.wring_off);
-                FV_LAUNCH_CHECK(ctx);
-            }
+        // rows 1.. hold older circulations; row 0 (this step's) waits for the
+        // solve when it runs on the side stream
+        rc = overlap ? (nrow_new > 1 ? wake_rows(nxt, 1, nrow_new) : FV_OK)
+                     : wake_rows(nxt, 0, nrow_new);
+        if (rc) return rc;
         h->par = nxt;
         h->nrow = nrow_new;
         const int64_t kr = d.n_keep;
@@ -584,6 +628,13 @@ static int hybrid_step_one(fv_hybrid *h)
         const double vn = sqrt(d.vinf[0] * d.vinf[0] + d.vinf[1] * d.vinf[1] + d.vinf[2] * d.vinf[2]);
         rc = vpm_merge_colloc(h->vpm, h->colloc, P, h->pop_eps, vn, h->pop_protect,
                               h->pop_protect_dist);
+        if (rc) return rc;
+    }
+    if (overlap) {
+        FV_HIP_CHECK(ctx, hipStreamWaitEvent(ctx->stream, h->ev_solved, 0));
+        rc = loads();
+        if (rc) return rc;
+        rc = wake_rows(nxt, 0, 1);
         if (rc) return rc;
     }
     h->step = s + 1;
@@ -741,6 +792,17 @@ int fv_hybrid_create(fv_ctx *ctx, const fv_hybrid_desc *desc, fv_hybrid **out)
         delete h;
         return rc;
     }
+    // the side stream (default priority) and its two events; without them the
+    // solve stays on the context's stream
+    if (hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&h->ev_rhs, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&h->ev_solved, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        if (h->side) (void)hipStreamDestroy(h->side);
+        if (h->ev_rhs) (void)hipEventDestroy(h->ev_rhs);
+        h->side = nullptr;
+        h->ev_rhs = h->ev_solved = nullptr;
+    }
     FV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     *out = h;
     return FV_OK;
@@ -750,6 +812,12 @@ void fv_hybrid_destroy(fv_hybrid *h)
 {
     if (!h) return;
     (void)hipStreamSynchronize(h->ctx->stream);
+    if (h->side) {
+        (void)hipStreamSynchronize(h->side);
+        (void)hipStreamDestroy(h->side);
+    }
+    if (h->ev_rhs) (void)hipEventDestroy(h->ev_rhs);
+    if (h->ev_solved) (void)hipEventDestroy(h->ev_solved);
     fv_vpm_destroy(h->vpm);
     (void)hipFree(h->block);
     delete h;
