@@ -137,3 +137,50 @@ def test_capacity_error(hyb):
     case = hyb.heaving_wing_case(nc=3, ns=2, num_steps=12, n_keep=2, max_particles=20)
     with pytest.raises(RuntimeError, match="overflow"):
         hyb.run_case(case)
+
+
+def _theodorsen_cl(k, h0c, omega, t, c=1.0):
+    """Theodorsen's plunging-airfoil lift coefficient with C(k) from Hankel
+    functions (experiment_hybrid_panel_particle.py:28-36)."""
+    from scipy.special import hankel2
+    ck = hankel2(1, k) / (hankel2(1, k) + 1j * hankel2(0, k))
+    U = omega * c / (2 * k)
+    h0 = h0c * c
+    return np.real(np.pi * (-h0 * omega ** 2 * np.sin(omega * t)) * c / (2 * U ** 2)
+                   - 2 * np.pi * ck * h0 * omega * np.cos(omega * t) / U)
+
+
+def _plunge_ratio(hyb, k, n_keep, free_wake):
+    """CL amplitude after two periods over Theodorsen's for the plunging wing
+    (NACA 0012, c = 1 m, half-span 5 m, h0/c = 0.1, V = 10 m/s, nc = 10,
+    ns = 6, dt = period / 50, 3 cycles, Gaussian particle core sigma = V dt,
+    unstretched; experiment_hybrid_panel_particle.py:206-262)."""
+    case = hyb.heaving_wing_case(k=k, kernel="gaussianerf", sigma_factor=1.0, n_keep=n_keep,
+                                 free_wake=free_wake, stretch=False)
+    cl = np.asarray(hyb.run_case(case)["cl"])
+    assert np.all(np.isfinite(cl))
+    t = np.arange(len(cl)) * case.dt
+    m = t > 2 * case.meta["period"]
+    th = _theodorsen_cl(k, 0.1, case.meta["omega"], t)
+    return (cl[m].max() - cl[m].min()) / (th[m].max() - th[m].min())
+
+
+def test_plunging_ring_wake_against_published_ptera_ratio(hyb):
+    """Ring wake only (every row kept) at k = 0.5: the CL amplitude is 92.9 %
+    of Theodorsen's in PteraSoftware's published run (README.md:59-69); this
+    UVLM statement gives 91.3 %."""
+    ratio = _plunge_ratio(hyb, 0.5, 1000, False)
+    print(f"ring wake k=0.5: {ratio:.1%} of Theodorsen (PteraSoftware 92.9%)", flush=True)
+    assert abs(ratio - 0.929) < 0.02
+
+
+@pytest.mark.xfail(reason="the published hybrid ratios (97.4 / 94.6 / 92.5 %) are not "
+                          "reproduced: this statement gives 90 / 75 / 68 %", strict=False)
+@pytest.mark.parametrize("k,published", [(0.5, 0.974), (0.2, 0.946), (0.1, 0.925)])
+def test_plunging_sweep_against_published_hybrid_ratios(hyb, k, published):
+    """The reduced-frequency sweep with the hybrid wake (N_keep = 10, free
+    particle wake): CL amplitude over Theodorsen's against the reference's
+    published FLUXVortex N=10 FREE ratios (README.md:59-69)."""
+    ratio = _plunge_ratio(hyb, k, 10, True)
+    print(f"hybrid k={k}: {ratio:.1%} of Theodorsen (published {published:.1%})", flush=True)
+    assert abs(ratio - published) < 0.03
