@@ -272,6 +272,28 @@ __device__ __forceinline__ void part_factors_mixed(float r2, bool keep, double d
     }
 }
 
+// Gaussian factors of the vorticity pair omega = CH (dx . a_s) dx - B a_s,
+// B = CH r^2 + 2 CG, FP64.  For rho > 10 (g == 1, g' == 0 as in
+// part_factors_f64) CH r^2 = 3/4pi ir^3 and 2 CG = -2/4pi ir^3, so
+// B = ir^3 / 4pi = Z^3 / (32 pi): one product instead of the sum.
+__device__ __forceinline__ void gauss_curl_factors_f64(double r2, bool keep, const PartRec &s,
+                                                       double &CH, double &B)
+{
+    if (r2 * s.k2 > 100.0) {
+        const double Z = rsqrt2_refined(r2);
+        const double Z2 = Z * Z;
+        const double Z3 = Z2 * Z;
+        CH = (3.0 * kInv4Pi / 32.0) * (Z3 * Z2);
+        B = (kInv4Pi / 8.0) * Z3;
+    } else {
+        double CG;
+        part_factors_f64<0, true>(r2, true, s, CG, CH);
+        B = fma(CH, r2, CG + CG);
+    }
+    CH = keep ? CH : 0.0;
+    B = keep ? B : 0.0;
+}
+
 template <int PREC>
 __device__ __forceinline__ PartRec make_part_rec(int kernel, double x, double y, double z,
                                                  double ax, double ay, double az, double sigma)
@@ -427,6 +449,15 @@ struct PartOp {
             const double r2d = dxd * dxd + dyd * dyd + dzd * dzd;
             const bool keep = r2d >= 1.0e-24;   // self pair excluded, warp_vpm.py:37-38
             double CG, CH = 0.0;
+            if (KERNEL == 0 && MODE == 3) {
+                double B;
+                gauss_curl_factors_f64(r2d, keep, s, CH, B);
+                const double a = CH * (dxd * s.ax + dyd * s.ay + dzd * s.az);
+                c.v[0] = fma(-B, s.ax, fma(a, dxd, c.v[0]));
+                c.v[1] = fma(-B, s.ay, fma(a, dyd, c.v[1]));
+                c.v[2] = fma(-B, s.az, fma(a, dzd, c.v[2]));
+                return;
+            }
             part_factors_f64<KERNEL, MODE != 0>(r2d, keep, s, CG, CH);
             accumulate(dxd, dyd, dzd, r2d, CG, CH, s, t, c);
         } else {
