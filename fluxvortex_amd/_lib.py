@@ -28,6 +28,9 @@ KERNELS = {"gaussianerf": KERNEL_GAUSSIAN_ERF, "winckelmans": KERNEL_WINCKELMANS
 PREC_FP64 = 0
 PREC_MIXED = 1
 PRECISIONS = {"fp64": PREC_FP64, "mixed": PREC_MIXED}
+# particles closer than 1e-12: "skip" as warp_vpm.py:37,103; "clamp" r to 1e-12
+# as the reference's CPU field (kernel.py:105,156), which HybridSolver runs
+PAIR_RULES = {"skip": 0, "clamp": 1}
 
 LEGS_RING = 0xF
 LEGS_CHORDWISE = 0x5
@@ -110,6 +113,7 @@ _SIGNATURES = {
     "fv_vpm_create": (C.c_int, [_ctx, C.c_int64, C.c_double, C.c_double, C.c_int32, C.c_int32,
                                 C.POINTER(_vp)]),
     "fv_vpm_destroy": (None, [_vp]),
+    "fv_vpm_set_pair_rule": (C.c_int, [_vp, C.c_int32]),
     "fv_vpm_add_host": (C.c_int, [_vp, _dp, _dp, _dp, C.c_int64]),
     "fv_vpm_count": (C.c_int, [_vp, _i64p]),
     "fv_vpm_read_host": (C.c_int, [_vp, _dp, _dp, _dp, _dp]),

@@ -299,9 +299,9 @@ int part_sum(fv_ctx *ctx, int kernel, int precision, int mode, const PartSrcArgs
 {
     static const char *const kFamilies[4] = {"particle_vel", "particle_jac", "particle_stretch",
                                              "particle_curl"};
-    if (!family) family = kFamilies[mode & 3];
+    if (!family) family = kFamilies[mode == 4 ? 3 : mode & 3];
     if (n_tgt <= 0) return FV_OK;
-    const int comp = (mode == 1) ? 9 : 3;
+    const int comp = (mode == 1) ? 9 : (mode == 4) ? 6 : 3;
     if (n_src <= 0) return fill(ctx, out, comp * n_tgt, 0.0);
     PartSrcArgs s = src;
     s.kernel = kernel;
@@ -313,6 +313,12 @@ int part_sum(fv_ctx *ctx, int kernel, int precision, int mode, const PartSrcArgs
     case 1: FV_PS(K, P, 1);     \
     case 2: FV_PS(K, P, 2);     \
     default: FV_PS(K, P, 3);    \
+    }
+    if (mode == 4) {   // Gaussian vorticity under the reference CPU pair rule
+        if (kernel != FV_KERNEL_GAUSSIAN_ERF)
+            return set_error(ctx, FV_E_ARG, "mode 4 is the Gaussian field's vorticity");
+        if (precision == FV_PREC_FP64) FV_PS(0, 0, 4);
+        FV_PS(0, 1, 4);
     }
     if (kernel == FV_KERNEL_WINCKELMANS) {
         if (precision == FV_PREC_FP64) { FV_PS_MODE(1, 0) }

@@ -734,8 +734,8 @@ __global__ __launch_bounds__(64) void k_near(
             __syncthreads();
             if (threadIdx.x < cnt) {
                 const int j = base + threadIdx.x;
-                tile[threadIdx.x] = make_part_rec<PREC>(KERNEL, sx[j], sy[j], sz[j], sax[j],
-                                                        say[j], saz[j], ssg[j]);
+                tile[threadIdx.x] = Op::make(KERNEL, sx[j], sy[j], sz[j], sax[j], say[j],
+                                             saz[j], ssg[j]);
             }
             __syncthreads();
             if (g == 1) {
@@ -777,7 +777,7 @@ __global__ __launch_bounds__(64) void k_near(
         Op::finish(tg[u], acc[u]);
         if (act[u]) {
             const int64_t ti = t0 + slot + 64 * u;
-            for (int k = 0; k < 3; ++k) dst[3 * ti + k] = acc[u].v[k];
+            for (int k = 0; k < 3; ++k) dst[3 * ti + k] = Op::result(acc[u], k);
         }
     }
 }
@@ -1440,6 +1440,8 @@ static int closure_pass(fv_vpm *f, int mode, int kernel, double *out, bool use_m
 {
     fv_closure *cl = f->closure;
     fv_ctx *ctx = f->ctx;
+    FV_REQUIRE(ctx, mode != 4 || kernel == FV_KERNEL_GAUSSIAN_ERF,
+               "the clamped-pair vorticity (mode 4) is the Gaussian field's");
     int rc = closure_build(f, use_mlp, mode == 0, stage);
     if (rc) return rc;
     const int64_t n = cl->n;
@@ -1454,12 +1456,17 @@ static int closure_pass(fv_vpm *f, int mode, int kernel, double *out, bool use_m
     } while (0)
         if (kernel == FV_KERNEL_WINCKELMANS) {
             if (prec == FV_PREC_FP64) FV_NEAR(1, 0); else FV_NEAR(1, 1);
+        } else if (mode == 4) {
+            if (prec == FV_PREC_FP64) launch_near<0, 0, 4>(cl, ctx, tpt);
+            else launch_near<0, 1, 4>(cl, ctx, tpt);
         } else {
             if (prec == FV_PREC_FP64) FV_NEAR(0, 0); else FV_NEAR(0, 1);
         }
 #undef FV_NEAR
     };
-    int &tsel = cl->near_tpt[mode & 3][kernel == FV_KERNEL_WINCKELMANS][prec == FV_PREC_MIXED];
+    // tuning slot per (mode, kernel, precision); mode 4 shares mode 3's work
+    int &tsel = cl->near_tpt[mode == 4 ? 3 : mode & 3][kernel == FV_KERNEL_WINCKELMANS]
+                            [prec == FV_PREC_MIXED];
     if (tsel == 0) {
         tsel = 2;
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
@@ -1710,7 +1717,8 @@ int fv_vpm_closure_sum_host(fv_vpm *f, int32_t mode, int32_t kernel, int32_t use
 {
     if (!f || !f->closure || !out) return FV_E_ARG;
     fv_ctx *ctx = f->ctx;
-    FV_REQUIRE(ctx, mode == 0 || mode == 2 || mode == 3, "closure mode must be 0, 2 or 3");
+    FV_REQUIRE(ctx, mode == 0 || mode == 2 || mode == 3 || mode == 4,
+               "closure mode must be 0, 2, 3 or 4");
     FV_REQUIRE(ctx, kernel == FV_KERNEL_GAUSSIAN_ERF || kernel == FV_KERNEL_WINCKELMANS,
                "unknown kernel");
     fv_closure *cl = f->closure;

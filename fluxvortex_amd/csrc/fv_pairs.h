@@ -294,6 +294,48 @@ __device__ __forceinline__ void gauss_curl_factors_f64(double r2, bool keep, con
     B = keep ? B : 0.0;
 }
 
+// The reference CPU path's Gaussian factors at r = 1e-12 (kernel.py:156:
+// r = sqrt(max(r^2, 1e-24))), used for the self pair and exactly coincident
+// particles (pair rule 1): CG = -(1/4pi) g / r^3, CH = -(1/4pi) aux / r^3 with
+// aux = g'/(sigma r) - 3 g / r^2.  There g = erf(rho/sqrt2) - sqrt(2/pi) rho
+// e^(-rho^2/2) is pure round-off, and its sign decides whether the
+// relaxation turns a particle's alpha against itself, so it is evaluated with
+// the reference's operations in the reference's order and no contraction:
+// erf as scipy.special.erf computes it below 1 (Cephes: x T(x^2) / U(x^2),
+// Horner, the published coefficients), IEEE divisions, 1/r^3 = 1e36 and
+// r^2 = 1e-24 exactly as NumPy forms them.  rho = 1e-12 / sigma < 1 for every
+// core sigma >= 1e-6 the field allows.
+__device__ inline double erf_cephes_below1(double x)
+{
+#pragma clang fp contract(off)
+    const double z = x * x;
+    double t = 9.60497373987051638749E0;
+    t = t * z + 9.00260197203842689217E1;
+    t = t * z + 2.23200534594684319226E3;
+    t = t * z + 7.00332514112805075473E3;
+    t = t * z + 5.55923013010394962768E4;
+    double u = z + 3.35617141647503099647E1;
+    u = u * z + 5.21357949780152679795E2;
+    u = u * z + 4.59432382970980127987E3;
+    u = u * z + 2.26290000613890934246E4;
+    u = u * z + 4.92673942608635921086E4;
+    return x * t / u;
+}
+
+__device__ inline void gauss_clamped_factors(double sigma, double &CG, double &CH)
+{
+#pragma clang fp contract(off)
+    const double r = 1.0e-12;
+    const double rb = r / sigma;
+    const double rb2 = rb * rb;
+    const double ev = exp(-0.5 * rb2);
+    const double g = erf_cephes_below1(rb / 1.4142135623730951) - (kSqrt2OverPi * rb) * ev;
+    const double dgdr = (kSqrt2OverPi * rb2) * ev;
+    const double aux = dgdr / (sigma * r) - (3.0 * g) / 1.0e-24;
+    CG = (-kInv4Pi * g) * 1.0e36;
+    CH = (-kInv4Pi * aux) * 1.0e36;
+}
+
 template <int PREC>
 __device__ __forceinline__ PartRec make_part_rec(int kernel, double x, double y, double z,
                                                  double ax, double ay, double az, double sigma)
@@ -387,10 +429,19 @@ struct PartTgtArgs {
 };
 
 // mode: 0 velocity (3), 1 Jacobian (9), 2 stretching S = J alpha_t (3),
-//       3 vorticity (curl, 3)
+//       3 vorticity (curl, 3), 4 vorticity of the Gaussian field under the
+//       reference CPU path's pair rule (kernel.py:156): pairs closer than
+//       1e-12 are not skipped but take the factors of r = 1e-12
+//       (gauss_clamped_factors, carried in the record's k3 / k4; primed by
+//       -4pi in mixed records), omega += CH (dx . a) dx - (CH r^2 + 2 CG) a.
+//       Those clamped terms (~1e8 |a|) are summed apart from the others
+//       (components 3..5 of the sums, added by result() at the very end), so
+//       the terms of coincident particles of opposite alpha cancel exactly in
+//       any tile / chunk order instead of leaving round-off of their size.
 template <int KERNEL, int PREC, int MODE>
 struct PartOp {
-    static constexpr int kComp = (MODE == 1) ? 9 : 3;
+    static_assert(MODE != 4 || KERNEL == 0, "mode 4 is the Gaussian vorticity");
+    static constexpr int kComp = (MODE == 1) ? 9 : (MODE == 4) ? 6 : 3;
     static constexpr int kPrec = PREC;
     // source-loop unroll at two targets per lane: the Gaussian's erf / exp
     // branch doubled would exceed 128 VGPRs (three wavefronts per SIMD)
@@ -409,7 +460,26 @@ struct PartOp {
         double v[kComp];
         double g[3];
     };
-    __device__ static Src load_src(const Args &a, int64_t j) { return load_part<PREC>(a.src, j); }
+    // a source record of this op (the clamped-pair factors for mode 4)
+    __device__ static Src make(int kernel, double x, double y, double z, double ax, double ay,
+                               double az, double sigma)
+    {
+        PartRec r = make_part_rec<PREC>(kernel, x, y, z, ax, ay, az, sigma);
+        if (MODE == 4) {
+            double cg, ch;
+            gauss_clamped_factors(sigma, cg, ch);
+            const double pf = PREC == 0 ? 1.0 : -kFourPi;
+            r.k3 = pf * ch;
+            r.k4 = pf * (cg + cg);
+        }
+        return r;
+    }
+    __device__ static Src load_src(const Args &a, int64_t j)
+    {
+        const int64_t k = j * a.src.stride;
+        return make(a.src.kernel, a.src.x[k], a.src.y[k], a.src.z[k], a.src.ax[k], a.src.ay[k],
+                    a.src.az[k], a.src.sig[j]);
+    }
     __device__ static Tgt load_tgt(const Args &a, int64_t i)
     {
         const int64_t k = i * a.tgt.stride;
@@ -427,6 +497,11 @@ struct PartOp {
 #pragma unroll
         for (int k = 0; k < kComp; ++k) c.v[k] = 0.0;
         c.g[0] = c.g[1] = c.g[2] = 0.0;
+    }
+    // the k-th component (k < 3; mode 1: k < 9) of a finished sum
+    __device__ static double result(const Acc &c, int k)
+    {
+        return MODE == 4 ? c.v[k] + c.v[k + 3] : c.v[k];
     }
     // S = sum CH (dx . a_t)(dx x a_s) + a_t x sum CG a_s  (stretching);
     // J_ij = sum CH (dx x a_s)_i dx_j + eps_ijl (sum CG a_s)_l  (Jacobian).
@@ -449,9 +524,10 @@ struct PartOp {
             const double r2d = dxd * dxd + dyd * dyd + dzd * dzd;
             const bool keep = r2d >= 1.0e-24;   // self pair excluded, warp_vpm.py:37-38
             double CG, CH = 0.0;
-            if (KERNEL == 0 && MODE == 3) {
+            if (KERNEL == 0 && (MODE == 3 || MODE == 4)) {
                 double B;
                 gauss_curl_factors_f64(r2d, keep, s, CH, B);
+                if (MODE == 4 && !keep) clamped_pair(dxd, dyd, dzd, r2d, s, c);
                 const double a = CH * (dxd * s.ax + dyd * s.ay + dzd * s.az);
                 c.v[0] = fma(-B, s.ax, fma(a, dxd, c.v[0]));
                 c.v[1] = fma(-B, s.ay, fma(a, dyd, c.v[1]));
@@ -470,13 +546,25 @@ struct PartOp {
             const bool keep = r2d >= 1.0e-24;
             float CGf, CHf = 0.0f;
             part_factors_mixed<KERNEL, MODE != 0>((float)r2d, keep, dxd, dyd, dzd, s, CGf, CHf);
+            if (MODE == 4 && !keep) clamped_pair(dxd, dyd, dzd, r2d, s, c);
             accumulate(dxd, dyd, dzd, r2d, (double)CGf, (double)CHf, s, t, c);
         }
+    }
+    // a pair closer than 1e-12 under the clamped rule (mode 4), into the
+    // separate components 3..5; rare (the self pair, coincident particles)
+    __device__ static void clamped_pair(double dx, double dy, double dz, double r2,
+                                        const Src &s, Acc &c)
+    {
+        const double B = fma(s.k3, r2, s.k4);
+        const double a = s.k3 * (dx * s.ax + dy * s.ay + dz * s.az);
+        c.v[3] = fma(-B, s.ax, fma(a, dx, c.v[3]));
+        c.v[4] = fma(-B, s.ay, fma(a, dy, c.v[4]));
+        c.v[5] = fma(-B, s.az, fma(a, dz, c.v[5]));
     }
     __device__ static void accumulate(double dx, double dy, double dz, double r2, double CG,
                                       double CH, const Src &s, const Tgt &t, Acc &c)
     {
-        if (MODE == 3) {
+        if (MODE == 3 || MODE == 4) {
             // omega = CH dx x (dx x a_s) - 2 CG a_s = CH (dx . a_s) dx - (CH r^2 + 2 CG) a_s
             const double a = CH * (dx * s.ax + dy * s.ay + dz * s.az);
             const double b = fma(CH, r2, CG + CG);

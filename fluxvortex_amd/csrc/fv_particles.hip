@@ -107,16 +107,20 @@ __global__ void k_vpm_finish(const int64_t *__restrict__ np_dev, const double *_
 }
 
 // Pedrizzetti relaxation, particles.py:212-246, with the vorticity
-// omega_i = eps_ijk J_kj of the Gaussian-erf field precomputed in W.
+// omega_i = eps_ijk J_kj of the Gaussian-erf field precomputed in W: three
+// components per particle, or six (split = true: the clamped pairs' sum in
+// 3..5, added last).
 __global__ void k_vpm_relax(const int64_t *__restrict__ np_dev, const double *__restrict__ W,
-                            double *__restrict__ gx, double *__restrict__ gy,
+                            int split, double *__restrict__ gx, double *__restrict__ gy,
                             double *__restrict__ gz, double rlxf)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= *np_dev) return;
     double g[3] = {gx[i], gy[i], gz[i]};
     const double gn = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
-    const double w[3] = {W[3 * i], W[3 * i + 1], W[3 * i + 2]};
+    const double *Wi = W + (split ? 6 : 3) * i;
+    const double w[3] = {split ? Wi[0] + Wi[3] : Wi[0], split ? Wi[1] + Wi[4] : Wi[1],
+                         split ? Wi[2] + Wi[5] : Wi[2]};
     const double wn = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
     if (!(gn > 1e-15 && wn > 1e-12)) return;
     const double wh[3] = {w[0] / wn, w[1] / wn, w[2] / wn};
@@ -332,11 +336,17 @@ int vpm_advect(fv_vpm *f, double dt, const double *uinf, int stretch, int free_w
     FV_LAUNCH_CHECK(ctx);
     f->state_epoch++;
     if (f->rlxf > 0) {
-        // the relaxation always uses the Gaussian-erf field (particles.py:219)
-        rc = self_sum(3, FV_KERNEL_GAUSSIAN_ERF);
+        // the relaxation always uses the Gaussian-erf field (particles.py:219);
+        // under the reference CPU pair rule its Jacobian keeps the self pair
+        // and coincident particles with r = 1e-12 (kernel.py:156), mode 4
+        const int mode = f->pair_rule == FV_PAIRS_CLAMP ? 4 : 3;
+        rc = self_sum(mode, FV_KERNEL_GAUSSIAN_ERF);
         if (rc) return rc;
+        // the direct mode-4 sum keeps the clamped pairs apart (6 components);
+        // the closure's near field has already added them
+        const int split = (mode == 4 && !clo) ? 1 : 0;
         hipLaunchKernelGGL(k_vpm_relax, dim3(gb), dim3(256), 0, ctx->stream, f->np_dev, f->sum,
-                           f->gx, f->gy, f->gz, f->rlxf);
+                           split, f->gx, f->gy, f->gz, f->rlxf);
         FV_LAUNCH_CHECK(ctx);
         f->state_epoch++;
     }
@@ -476,6 +486,15 @@ void fv_vpm_destroy(fv_vpm *f)
     pop_destroy(f);
     (void)hipFree(f->block);
     delete f;
+}
+
+int fv_vpm_set_pair_rule(fv_vpm *f, int32_t rule)
+{
+    if (!f) return FV_E_ARG;
+    FV_REQUIRE(f->ctx, rule == FV_PAIRS_SKIP_CLOSE || rule == FV_PAIRS_CLAMP,
+               "unknown pair rule %d", rule);
+    f->pair_rule = rule;
+    return FV_OK;
 }
 
 int fv_vpm_count(fv_vpm *f, int64_t *np)

@@ -15,6 +15,7 @@ struct fv_vpm {
     double nu = 0.0, rlxf = 0.3;
     int kernel = FV_KERNEL_GAUSSIAN_ERF;
     int precision = FV_PREC_FP64;
+    int pair_rule = 0;             // FV_PAIRS_SKIP_CLOSE or FV_PAIRS_CLAMP (relaxation)
     void *block = nullptr;
     double *x = nullptr, *y = nullptr, *z = nullptr;
     double *gx = nullptr, *gy = nullptr, *gz = nullptr;

@@ -49,6 +49,10 @@ class HybridCase:
     precision: str = "fp64"
     ring_free_wake: bool = False  # ring wake moves with V + induction (prescribed: V)
     truncate_x: float = math.inf  # rings whose corner-mean x exceeds this lose gamma
+    # particles closer than 1e-12 in the relaxation: "clamp" (r = 1e-12), as the
+    # reference's CPU field that HybridSolver runs (kernel.py:156), or "skip"
+    # (warp_vpm.py:103); the oracle reads the same field (0 skip, 1 clamp)
+    pair_rule: str = "clamp"
     name: str = "case"
     meta: dict = field(default_factory=dict)
 
@@ -106,7 +110,8 @@ class HybridRun:
         vh = self._ctx.lib.fv_hybrid_particles(self._h)
         self.particles = VortexParticleField(max_particles=case.max_particles, rlxf=case.rlxf,
                                              kernel=case.kernel, precision=case.precision,
-                                             context=self._ctx, _handle=vh, _owner=self)
+                                             context=self._ctx, pair_rule=case.pair_rule,
+                                             _handle=vh, _owner=self)
         if case.ring_free_wake or math.isfinite(case.truncate_x):
             self.set_ring_wake(case.ring_free_wake, case.truncate_x)
 
@@ -210,7 +215,8 @@ def _case(airplane: Airplane, V: float, alpha_deg: float, dt: float, num_steps: 
 def heaving_wing_case(nc=10, ns=6, chord=1.0, half_span=5.0, V=10.0, k=0.5, h0c=0.1,
                       cycles=3.0, steps_per_cycle=50, n_keep=10, kernel="winckelmans",
                       sigma_factor=0.5, free_wake=True, stretch=True, rlxf=0.3,
-                      precision="fp64", max_particles=200000, num_steps=None) -> HybridCase:
+                      precision="fp64", max_particles=200000, num_steps=None,
+                      pair_rule="clamp") -> HybridCase:
     """Heaving rectangular wing: NACA 0012, uniform panels mirrored into one
     lattice, h(t) = h0 sin(2 k V t / c), dt = period / 50, N_keep wake rows kept
     as rings, older rows converted with sigma = sigma_factor |V| dt
@@ -227,7 +233,7 @@ def heaving_wing_case(nc=10, ns=6, chord=1.0, half_span=5.0, V=10.0, k=0.5, h0c=
                  n_keep=n_keep, sigma=sigma_factor * V * dt, kernel=kernel, free_wake=free_wake,
                  stretch=stretch, rlxf=rlxf, sref=wing.planform_area(),
                  max_particles=max_particles, precision=precision, nu=15.06e-6,
-                 name=f"heaving wing k={k} nc={nc} ns={2 * ns} N_keep={n_keep}",
+                 pair_rule=pair_rule, name=f"heaving wing k={k} nc={nc} ns={2 * ns} N_keep={n_keep}",
                  meta={"period": period, "omega": omega, "h0": h0c * chord, "V": V, "k": k})
 
 
@@ -315,7 +321,7 @@ class HybridSolver:
 
     def __init__(self, prob: UnsteadyProblem, n_keep=5, kernel="gaussianerf", free_vpm=False,
                  stretch=False, sigma_factor=1.0, precision="fp64", max_particles=200000,
-                 rlxf=0.3):
+                 rlxf=0.3, pair_rule="clamp"):
         self.problem = prob
         self._n_keep = n_keep
         self._free_vpm = free_vpm
@@ -327,7 +333,8 @@ class HybridSolver:
                           sigma=sigma_factor * prob.V * prob.dt, kernel=kernel,
                           free_wake=free_vpm, stretch=stretch, rlxf=rlxf,
                           sref=prob.sref or prob.airplane.reference_area(),
-                          max_particles=max_particles, precision=precision)
+                          max_particles=max_particles, precision=precision,
+                          pair_rule=pair_rule)
         self._run = HybridRun(self.case)
         self._vpm = self._run.particles
         self.cl = self.cdi = self.gamma = None

@@ -19,7 +19,14 @@ class VortexParticleField:
     """Structure-of-arrays vortex particle field on the MI355X."""
 
     def __init__(self, max_particles=50000, nu=0.0, rlxf=0.3, kernel="gaussianerf",
-                 precision="fp64", context=None, _handle=None, _owner=None):
+                 precision="fp64", context=None, pair_rule="clamp", _handle=None,
+                 _owner=None):
+        """pair_rule: how particles closer than 1e-12 (the self pair, coincident
+        particles) enter the sums -- "clamp" (default) with r = 1e-12, as the
+        reference's CPU field (kernel.py:105,156); "skip" as its GPU twin
+        (warp_vpm.py:37,103).  They differ only in the relaxation, where the
+        clamped Gaussian self pair contributes a round-off-sized g at r = 1e-12
+        divided by r^3 (particles.py:219)."""
         self.max_particles = int(max_particles)
         self.nu = float(nu)
         self.rlxf = float(rlxf)
@@ -37,6 +44,8 @@ class VortexParticleField:
             self._h = h
         else:
             self._h = C.c_void_p(_handle)
+        self.pair_rule = pair_rule
+        self._ctx.check(self._ctx.lib.fv_vpm_set_pair_rule(self._h, _lib.PAIR_RULES[pair_rule]))
 
     def __del__(self):
         try:
@@ -191,10 +200,11 @@ class VortexParticleField:
         return out, err.value
 
     def closure_sum(self, mode="velocity", kernel=None, use_mlp=True):
-        """One closure pass at the particles: "velocity", "stretching" (J alpha)
-        or "vorticity"; kernel defaults to the field's (the relaxation uses
+        """One closure pass at the particles: "velocity", "stretching" (J alpha),
+        "vorticity" or "vorticity_clamped" (the Gaussian vorticity under the
+        clamped pair rule); kernel defaults to the field's (the relaxation uses
         "gaussianerf")."""
-        m = {"velocity": 0, "stretching": 2, "vorticity": 3}[mode]
+        m = {"velocity": 0, "stretching": 2, "vorticity": 3, "vorticity_clamped": 4}[mode]
         k = _lib.KERNELS[kernel or self.kernel]
         out = np.zeros((self.np, 3))
         self._ctx.check(self._ctx.lib.fv_vpm_closure_sum_host(self._h, m, k, 1 if use_mlp else 0,

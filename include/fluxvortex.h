@@ -44,6 +44,11 @@ extern "C" {
 #define FV_PREC_MIXED 1   /* FP32 pair terms (segments) / kernel factors (particles),
                             * FP64 sums; FP32 LU + FP64-residual refinement */
 
+/* Particles closer than 1e-12 (the self pair, exactly coincident particles). */
+#define FV_PAIRS_SKIP_CLOSE 0   /* skipped, as the reference's GPU twin warp_vpm.py:37,103 */
+#define FV_PAIRS_CLAMP 1        /* r clamped to 1e-12, as the reference's CPU field
+                                 * (kernel.py:105,156) that HybridSolver runs */
+
 /* Leg masks over ring corners (Br, Fr, Fl, Bl); leg k runs corner k -> k+1.
  * warp_kernels.py:268-273 (ring), :296-299 (chordwise), :350-354 (horseshoe). */
 #define FV_LEGS_RING 0xF
@@ -182,6 +187,14 @@ int fv_vpm_write_host(fv_vpm *f, int64_t np, const double *pos, const double *ga
                       const double *sigma, const double *age);
 /* particles.py:72-82 */
 int fv_vpm_induce_host(fv_vpm *f, const double *targets, int64_t t, double *out);
+/* Rule for particles closer than 1e-12 (FV_PAIRS_*, default SKIP_CLOSE).
+ * Velocities and stretching are the same under both up to round-off (their
+ * dx = 0 terms vanish); the relaxation's Gaussian Jacobian is not: under
+ * FV_PAIRS_CLAMP the self pair adds -2 aux2 alpha to omega with aux2 =
+ * -g(1e-12/sigma)/(4 pi 1e-36), g the round-off of erf(rho/sqrt2) -
+ * sqrt(2/pi) rho, evaluated bit for bit as the reference's CPU path
+ * (particles.py:219 -> kernel.py:156-172). */
+int fv_vpm_set_pair_rule(fv_vpm *f, int32_t rule);
 /* particles.py:85-199 followed by _relax_pedrizzetti (:202-246), with a
  * constant freestream uinf[3]. */
 int fv_vpm_advect_rk3(fv_vpm *f, double dt, const double *uinf, int32_t stretch,

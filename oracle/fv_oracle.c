@@ -221,6 +221,33 @@ int orc_lu_solve(int64_t n, double *A, double *b)
 /* Vortex particles.  kernel.py:10-190, warp_vpm.py:19-141                   */
 /* kernel id: 0 = Gaussian-erf (kernel.py:26-31), 1 = Winckelmans (:51-55)  */
 /* ------------------------------------------------------------------------ */
+/* erf as scipy.special.erf evaluates it (SciPy 1.15.3, the library kernel.py
+ * imports; its erf is Moshier's Cephes ndtr.c): for |x| < 1 the rational
+ * x T(x^2) / U(x^2) with Horner evaluation (polevl / p1evl), coefficients
+ * as published in Cephes.  Bit-identical to scipy there (checked over 2e5
+ * arguments by tests/test_oracle.py); beyond, libm's erf (within an ulp, no
+ * cancellation in g for rho >= 1.4).  The bits matter where g cancels: at the
+ * clamped self pair, r = 1e-12, g is pure round-off whose sign decides the
+ * direction the relaxation turns a particle. */
+static double orc_erf_ref(double x)
+{
+    if (fabs(x) >= 1.0) return erf(x);
+    static const double T[5] = {9.60497373987051638749E0, 9.00260197203842689217E1,
+                                2.23200534594684319226E3, 7.00332514112805075473E3,
+                                5.55923013010394962768E4};
+    static const double U[5] = {3.35617141647503099647E1, 5.21357949780152679795E2,
+                                4.59432382970980127987E3, 2.26290000613890934246E4,
+                                4.92673942608635921086E4};
+    double z = x * x;
+    double t = T[0];
+    for (int i = 1; i < 5; ++i) t = t * z + T[i];
+    double u = z + U[0];
+    for (int i = 1; i < 5; ++i) u = u * z + U[i];
+    return x * t / u;
+}
+
+double orc_erf(double x) { return orc_erf_ref(x); }
+
 static inline void g_dgdr(int kernel, double rb, double *g, double *dgdr)
 {
     if (kernel == 1) {
@@ -230,7 +257,7 @@ static inline void g_dgdr(int kernel, double rb, double *g, double *dgdr)
     } else {
         const double c = 0.7978845608028654;          /* sqrt(2/pi) */
         double ev = exp(-0.5 * rb * rb);
-        *g = erf(rb / 1.4142135623730951) - c * rb * ev;
+        *g = orc_erf_ref(rb / 1.4142135623730951) - c * rb * ev;
         *dgdr = c * rb * rb * ev;
     }
 }
@@ -240,6 +267,15 @@ static inline void g_dgdr(int kernel, double rb, double *g, double *dgdr)
  * velocity this is identical to kernel.py's clamp because dx = 0 there. */
 int orc_particle_vel(const double *tgt, int64_t T, const double *pos, const double *alpha,
                      const double *sigma, int64_t N, int kernel, double *out, int nthreads)
+{
+    return orc_particle_vel_rule(tgt, T, pos, alpha, sigma, N, kernel, 0, out, nthreads);
+}
+
+/* rule 0: pairs closer than 1e-12 are skipped (warp_vpm.py:37-38);
+ * rule 1: r = max(r, 1e-12) (kernel.py:105-106), the reference CPU path. */
+int orc_particle_vel_rule(const double *tgt, int64_t T, const double *pos, const double *alpha,
+                          const double *sigma, int64_t N, int kernel, int rule, double *out,
+                          int nthreads)
 {
     const double C = -1.0 / (4.0 * M_PI);
     memset(out, 0, sizeof(double) * 3 * (size_t)T);
@@ -252,7 +288,10 @@ int orc_particle_vel(const double *tgt, int64_t T, const double *pos, const doub
             double dx = x[0] - pos[3 * j], dy = x[1] - pos[3 * j + 1], dz = x[2] - pos[3 * j + 2];
             double r2 = dx * dx + dy * dy + dz * dz;
             double r = sqrt(r2);
-            if (r < 1.0e-12) continue;
+            if (r < 1.0e-12) {
+                if (rule == 0) continue;
+                r = 1.0e-12;
+            }
             double rb = r / sigma[j];
             double g, dg;
             g_dgdr(kernel, rb, &g, &dg);
@@ -276,18 +315,40 @@ int orc_particle_vel(const double *tgt, int64_t T, const double *pos, const doub
 int orc_particle_jac(const double *tgt, int64_t T, const double *pos, const double *alpha,
                      const double *sigma, int64_t N, int kernel, double *J, int nthreads)
 {
+    return orc_particle_jac_rule(tgt, T, pos, alpha, sigma, N, kernel, 0, J, nthreads);
+}
+
+/* rule 0: pairs closer than 1e-12 skipped (warp_vpm.py:103); rule 1:
+ * r = sqrt(max(r^2, 1e-24)) (kernel.py:156), which keeps the self pair and
+ * exactly coincident pairs: their dx = 0 terms vanish, the aux2 terms
+ * (aux2 = -g(1e-12/sigma) / (4 pi 1e-36), ~1e8 |alpha|) do not.  Those
+ * clamped pairs are summed apart and added last, so coincident particles of
+ * opposite alpha cancel exactly; the reference sums them among the others,
+ * where they leave round-off of their own size (~1e-8 |alpha|) that depends
+ * on NumPy's summation order. */
+int orc_particle_jac_rule(const double *tgt, int64_t T, const double *pos, const double *alpha,
+                          const double *sigma, int64_t N, int kernel, int rule, double *J,
+                          int nthreads)
+{
     const double C = -1.0 / (4.0 * M_PI);
     memset(J, 0, sizeof(double) * 9 * (size_t)T);
     if (N == 0) return 0;
 #pragma omp parallel for schedule(static) num_threads(orc_threads(nthreads))
     for (int64_t t = 0; t < T; ++t) {
-        double acc[9] = {0};
+        double main_acc[9] = {0}, clamped[9] = {0};
         const double *x = tgt + 3 * t;
         for (int64_t j = 0; j < N; ++j) {
             double d[3] = {x[0] - pos[3 * j], x[1] - pos[3 * j + 1], x[2] - pos[3 * j + 2]};
             double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-            double r = sqrt(r2);
-            if (r < 1.0e-12) continue;
+            double r;
+            double *acc = main_acc;
+            if (rule == 0) {
+                r = sqrt(r2);
+                if (r < 1.0e-12) continue;
+            } else {
+                r = sqrt(r2 > 1.0e-24 ? r2 : 1.0e-24);
+                if (r2 < 1.0e-24) acc = clamped;
+            }
             double s = sigma[j];
             double rb = r / s;
             double g, dg;
@@ -309,7 +370,7 @@ int orc_particle_jac(const double *tgt, int64_t T, const double *pos, const doub
             acc[7] += aux * k[2] * d[1] - aux2 * a[0];
             acc[8] += aux * k[2] * d[2];
         }
-        memcpy(J + 9 * t, acc, sizeof acc);
+        for (int c = 0; c < 9; ++c) J[9 * t + c] = main_acc[c] + clamped[c];
     }
     return 0;
 }
@@ -322,6 +383,14 @@ int orc_particle_jac(const double *tgt, int64_t T, const double *pos, const doub
 int orc_vpm_advect_rk3(int64_t n, double *pos, double *gamma, double *sigma, double *age,
                        double dt, const double *Uinf, int stretch, int free_wake, int kernel,
                        double nu, int nthreads)
+{
+    return orc_vpm_advect_rk3_rule(n, pos, gamma, sigma, age, dt, Uinf, stretch, free_wake,
+                                   kernel, nu, 0, nthreads);
+}
+
+int orc_vpm_advect_rk3_rule(int64_t n, double *pos, double *gamma, double *sigma, double *age,
+                            double dt, const double *Uinf, int stretch, int free_wake,
+                            int kernel, double nu, int rule, int nthreads)
 {
     if (n == 0) return 0;
     static const double ra[3] = {0.0, -5.0 / 9.0, -153.0 / 128.0};   /* :114 */
@@ -340,7 +409,7 @@ int orc_vpm_advect_rk3(int64_t n, double *pos, double *gamma, double *sigma, dou
     for (int st = 0; st < 3; ++st) {
         double a = ra[st], b = rb[st];
         if (free_wake)
-            orc_particle_vel(pos, n, pos, gamma, sigma, n, kernel, U, nthreads);
+            orc_particle_vel_rule(pos, n, pos, gamma, sigma, n, kernel, rule, U, nthreads);
         else
             memset(U, 0, n3 * sizeof(double));
         for (int64_t i = 0; i < n; ++i) {
@@ -358,7 +427,8 @@ int orc_vpm_advect_rk3(int64_t n, double *pos, double *gamma, double *sigma, dou
             }
         }
         if (stretch) {
-            orc_particle_jac(pos, n, pos, gamma, sigma, n, kernel, J, nthreads);   /* :148 */
+            orc_particle_jac_rule(pos, n, pos, gamma, sigma, n, kernel, rule, J,
+                                  nthreads);                                    /* :148 */
             double zmax = 10.0 / (dt > 1e-10 ? dt : 1e-10);                     /* :161 */
             for (int64_t i = 0; i < n; ++i) {
                 const double *Ji = J + 9 * i;
@@ -407,6 +477,12 @@ int orc_vpm_advect_rk3(int64_t n, double *pos, double *gamma, double *sigma, dou
 int orc_vpm_relax(int64_t n, const double *pos, double *gamma, const double *sigma,
                   double rlxf, int nthreads)
 {
+    return orc_vpm_relax_rule(n, pos, gamma, sigma, rlxf, 0, nthreads);
+}
+
+int orc_vpm_relax_rule(int64_t n, const double *pos, double *gamma, const double *sigma,
+                       double rlxf, int rule, int nthreads)
+{
     if (rlxf <= 0 || n == 0) return 0;
     int any = 0;
     for (int64_t i = 0; i < n && !any; ++i) {
@@ -416,7 +492,7 @@ int orc_vpm_relax(int64_t n, const double *pos, double *gamma, const double *sig
     if (!any) return 0;                                   /* :214-216 */
     double *J = calloc(9 * (size_t)n, sizeof(double));
     if (!J) return -1;
-    orc_particle_jac(pos, n, pos, gamma, sigma, n, 0, J, nthreads);
+    orc_particle_jac_rule(pos, n, pos, gamma, sigma, n, 0, rule, J, nthreads);
     for (int64_t i = 0; i < n; ++i) {
         double *g = gamma + 3 * i;
         const double *Ji = J + 9 * i;

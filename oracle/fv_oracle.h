@@ -25,6 +25,20 @@ int orc_vpm_advect_rk3(int64_t n, double *pos, double *gamma, double *sigma, dou
                        double nu, int nthreads);
 int orc_vpm_relax(int64_t n, const double *pos, double *gamma, const double *sigma,
                   double rlxf, int nthreads);
+/* Pair rule for particles closer than 1e-12: 0 skips them (warp_vpm.py:37,103),
+ * 1 clamps r to 1e-12 as the reference's CPU kernels do (kernel.py:105,156). */
+int orc_particle_vel_rule(const double *tgt, int64_t T, const double *pos, const double *alpha,
+                          const double *sigma, int64_t N, int kernel, int rule, double *out,
+                          int nthreads);
+int orc_particle_jac_rule(const double *tgt, int64_t T, const double *pos, const double *alpha,
+                          const double *sigma, int64_t N, int kernel, int rule, double *J,
+                          int nthreads);
+int orc_vpm_advect_rk3_rule(int64_t n, double *pos, double *gamma, double *sigma, double *age,
+                            double dt, const double *Uinf, int stretch, int free_wake,
+                            int kernel, double nu, int rule, int nthreads);
+int orc_vpm_relax_rule(int64_t n, const double *pos, double *gamma, const double *sigma,
+                       double rlxf, int rule, int nthreads);
+double orc_erf(double x);
 int64_t orc_convert_rings(int64_t nr, const double *Br, const double *Fr, const double *Fl,
                           const double *Bl, double *gamma, double sigma, double *pos_out,
                           double *alpha_out, double *sig_out);

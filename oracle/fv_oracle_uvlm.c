@@ -76,6 +76,9 @@ typedef struct orc_hybrid_desc {
      * truncate_x lose their circulation after shedding (:110-126) */
     int32_t free_ring_wake;
     double truncate_x;
+    /* particles closer than 1e-12: 0 skipped (warp_vpm.py:37,103), 1 r clamped
+     * to 1e-12 as the reference's CPU field does (kernel.py:105,156) */
+    int32_t pair_rule;
 } orc_hybrid_desc;
 
 typedef struct orc_beam {
@@ -636,9 +639,9 @@ int orc_hybrid_run(const orc_hybrid_desc *d, int64_t steps, double *cl, double *
             for (int64_t k = 0; k < np; ++k)
                 for (int c = 0; c < 3; ++c) ppos[3 * k + c] += V[c] * dt;
         } else if (np > 0) {
-            orc_vpm_advect_rk3(np, ppos, pal, psg, page, dt, V, d->stretch, d->free_wake,
-                               d->kernel, d->vpm_nu, nthreads);
-            orc_vpm_relax(np, ppos, pal, psg, d->rlxf, nthreads);
+            orc_vpm_advect_rk3_rule(np, ppos, pal, psg, page, dt, V, d->stretch, d->free_wake,
+                                    d->kernel, d->vpm_nu, d->pair_rule, nthreads);
+            orc_vpm_relax_rule(np, ppos, pal, psg, d->rlxf, d->pair_rule, nthreads);
         }
         if (bm) {
             /* this lattice becomes the previous one; the next deformation the current */

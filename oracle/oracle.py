@@ -48,6 +48,14 @@ def lib() -> C.CDLL:
         L.orc_vpm_advect_rk3.argtypes = [i64, _dp, _dp, _dp, _dp, C.c_double, _dp, C.c_int,
                                          C.c_int, C.c_int, C.c_double, C.c_int]
         L.orc_vpm_relax.argtypes = [i64, _dp, _dp, _dp, C.c_double, C.c_int]
+        L.orc_particle_vel_rule.argtypes = [_dp, i64, _dp, _dp, _dp, i64, C.c_int, C.c_int, _dp,
+                                            C.c_int]
+        L.orc_particle_jac_rule.argtypes = L.orc_particle_vel_rule.argtypes
+        L.orc_vpm_advect_rk3_rule.argtypes = [i64, _dp, _dp, _dp, _dp, C.c_double, _dp, C.c_int,
+                                              C.c_int, C.c_int, C.c_double, C.c_int, C.c_int]
+        L.orc_vpm_relax_rule.argtypes = [i64, _dp, _dp, _dp, C.c_double, C.c_int, C.c_int]
+        L.orc_erf.argtypes = [C.c_double]
+        L.orc_erf.restype = C.c_double
         L.orc_convert_rings.argtypes = [i64, _dp, _dp, _dp, _dp, _dp, C.c_double, _dp, _dp, _dp]
         L.orc_convert_rings.restype = i64
         if hasattr(L, "orc_hybrid_run"):
@@ -105,34 +113,43 @@ def lu_solve(A, b):
     return x
 
 
-def particle_vel(tgt, pos, alpha, sigma, kernel="gaussianerf", threads=0):
+def particle_vel(tgt, pos, alpha, sigma, kernel="gaussianerf", threads=0, rule=0):
+    """rule 0: pairs closer than 1e-12 skipped (warp_vpm.py:37); rule 1: r
+    clamped to 1e-12 (kernel.py:105-106, the reference CPU path)."""
     t, p, a, s = _f(tgt), _f(pos), _f(alpha), _f(sigma)
     out = np.zeros((t.shape[0], 3))
-    lib().orc_particle_vel(_p(t), t.shape[0], _p(p), _p(a), _p(s), p.shape[0], KERNELS[kernel],
-                           _p(out), threads)
+    lib().orc_particle_vel_rule(_p(t), t.shape[0], _p(p), _p(a), _p(s), p.shape[0],
+                                KERNELS[kernel], int(rule), _p(out), threads)
     return out
 
 
-def particle_jac(tgt, pos, alpha, sigma, kernel="gaussianerf", threads=0):
+def particle_jac(tgt, pos, alpha, sigma, kernel="gaussianerf", threads=0, rule=0):
+    """rule as particle_vel (kernel.py:156: r = sqrt(max(r^2, 1e-24)))."""
     t, p, a, s = _f(tgt), _f(pos), _f(alpha), _f(sigma)
     out = np.zeros((t.shape[0], 3, 3))
-    lib().orc_particle_jac(_p(t), t.shape[0], _p(p), _p(a), _p(s), p.shape[0], KERNELS[kernel],
-                           _p(out), threads)
+    lib().orc_particle_jac_rule(_p(t), t.shape[0], _p(p), _p(a), _p(s), p.shape[0],
+                                KERNELS[kernel], int(rule), _p(out), threads)
     return out
+
+
+def oracle_erf(x):
+    """The oracle's erf (Cephes form below 1, as scipy.special.erf)."""
+    return lib().orc_erf(float(x))
 
 
 def vpm_advect_rk3(pos, gamma, sigma, age, dt, uinf, stretch, free_wake, kernel, nu=0.0,
-                   rlxf=0.3, threads=0):
+                   rlxf=0.3, threads=0, rule=0):
     """advect_rk3 followed by _relax_pedrizzetti (particles.py:85-246); returns new arrays."""
     pos, gamma = _f(pos).copy(), _f(gamma).copy()
     sigma, age = _f(sigma).copy(), _f(age).copy()
     u = _f(uinf)
     n = pos.shape[0]
     L = lib()
-    rc = L.orc_vpm_advect_rk3(n, _p(pos), _p(gamma), _p(sigma), _p(age), dt, _p(u),
-                              int(stretch), int(free_wake), KERNELS[kernel], nu, threads)
+    rc = L.orc_vpm_advect_rk3_rule(n, _p(pos), _p(gamma), _p(sigma), _p(age), dt, _p(u),
+                                   int(stretch), int(free_wake), KERNELS[kernel], nu, int(rule),
+                                   threads)
     assert rc == 0
-    rc = L.orc_vpm_relax(n, _p(pos), _p(gamma), _p(sigma), rlxf, threads)
+    rc = L.orc_vpm_relax_rule(n, _p(pos), _p(gamma), _p(sigma), rlxf, int(rule), threads)
     assert rc == 0
     return pos, gamma, sigma, age
 
@@ -303,6 +320,7 @@ class _HybridDesc(C.Structure):
         ("sref", C.c_double), ("max_particles", C.c_int64), ("precision", C.c_int32),
         ("beam", C.c_void_p), ("convect_exact", C.c_int32),
         ("free_ring_wake", C.c_int32), ("truncate_x", C.c_double),
+        ("pair_rule", C.c_int32),
     ]
 
 
@@ -340,6 +358,8 @@ def hybrid_run(case, steps=None, threads=0):
     d.max_particles = int(case.max_particles)
     d.free_ring_wake = int(bool(getattr(case, "ring_free_wake", False)))
     d.truncate_x = float(getattr(case, "truncate_x", np.inf))
+    rule = getattr(case, "pair_rule", 0)
+    d.pair_rule = {"skip": 0, "clamp": 1}.get(rule, rule) if isinstance(rule, str) else int(rule)
     n = grids.shape[0]
     steps = n if steps is None else steps
     P = int(sum(int(a) * int(b) for a, b in zip(nc, ns)))

@@ -51,6 +51,27 @@ def test_near_only_is_exact(field, oracle):
     assert relerr(u[sub], ref) < 1e-12
 
 
+def test_near_only_clamped_vorticity_matches_oracle(ctx, oracle):
+    """The relaxation's vorticity under the reference CPU pair rule (self pair
+    kept with r = 1e-12), closure with every cell near, against the oracle's
+    curl of its clamped Gaussian Jacobian (kernel.py:156)."""
+    from fluxvortex_amd.particles import VortexParticleField
+    pos, al, _ = wake_sheet(nx=200)
+    sg = np.full(len(pos), 0.12566370614359174)   # V dt of the k = 0.5 plunging wing
+    f = VortexParticleField(max_particles=len(sg) + 8, rlxf=0.3, kernel="gaussianerf")
+    f.add_particles_batch(pos, al, sg)
+    f.enable_closure(None, cell=2.0, kappa=1e3)
+    w = f.closure_sum("vorticity_clamped", kernel="gaussianerf", use_mlp=False)
+    J = oracle.particle_jac(pos, pos, al, sg, "gaussianerf", rule=1)
+    ref = np.stack([J[:, 2, 1] - J[:, 1, 2], J[:, 0, 2] - J[:, 2, 0], J[:, 1, 0] - J[:, 0, 1]], 1)
+    assert relerr(w, ref) < 1e-12
+    # the self pair dominates: omega points against alpha for every particle
+    nz = np.linalg.norm(al, axis=1) > 0
+    cosang = np.sum(w[nz] * al[nz], 1) / (np.linalg.norm(w[nz], axis=1)
+                                          * np.linalg.norm(al[nz], axis=1))
+    assert np.all(cosang < -0.999)
+
+
 def test_expansion_converges_with_cutoff(field):
     f, *_ = field
     errs = []

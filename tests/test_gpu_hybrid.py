@@ -40,35 +40,68 @@ def response(ref, runs, key):
     return out
 
 
-def compare(hyb, oracle, case, cl_tol=1e-10, part=True):
-    """Device vs oracle.  CL/CDi histories: within cl_tol at every step where the
-    oracle's own response to the 1e-14 input changes stays below 1e-12, else
-    within 1e3 x that response (chaotic free-wake dynamics, relaxation
-    thresholds).  Final circulations and particles: within max(1e-10, 1e3 x the
-    oracle's own response)."""
-    print(f"[hybrid parity] {case.name} steps={case.num_steps}", flush=True)
+def compare(hyb, oracle, case, cl_tol=1e-10, part=True, flat=None):
+    """Device vs oracle.  Flat (no relaxation, no stretching: the default
+    there): the CL/CDi histories agree within cl_tol at every step, the final
+    circulations within 1e-10, particle positions within 1e-10 and strengths
+    within 1e-9.  Otherwise the oracle's own response to two 1e-14 input
+    changes is measured: steps where it stays below 1e-12 agree within cl_tol,
+    the rest (and the final state) within 30 x that response (the device's
+    own round-off, summed over ~1e4 sources in another order, is itself
+    ~1e-14 relative, the size of the probe).  Every step before the first
+    particles are relaxed must be well conditioned.  The
+    relaxation turns alpha towards omega / |omega|, which round-off sets
+    wherever omega cancels (the mirrored sheet's root legs, whose coincident
+    particles carry opposite alpha), and the stretched free wake amplifies
+    round-off about tenfold per step."""
+    if flat is None:
+        flat = (case.rlxf == 0.0 or case.n_keep >= case.num_steps) and not case.stretch
+    print(f"[hybrid parity] {case.name} steps={case.num_steps} flat={flat}", flush=True)
     dev = hyb.run_case(case, particles=part)
     ref = oracle.hybrid_run(case)
+    dcl = np.abs(dev["cl"] - ref["cl"])
+    dcdi = np.abs(dev["cdi"] - ref["cdi"])
+    if flat:
+        sens = np.zeros_like(ref["cl"])
+        bad = np.nonzero((dcl > cl_tol) | (dcdi > cl_tol))[0]
+        assert bad.size == 0, (bad[:5], dcl[bad[:5]], dcdi[bad[:5]])
+        assert relerr(dev["gamma"], ref["gamma"]) < 1e-10
+        if part:
+            assert dev["np"] == ref["np"]
+            if ref["np"]:
+                assert relerr(dev["pos"], ref["pos"]) < 1e-10
+                assert relerr(dev["alpha"], ref["gamma_p"]) < 1e-9
+        return dev, ref, sens
     runs = oracle_perturbed(oracle, case)
     d = np.zeros_like(ref["cl"])
     for r in runs:
         d = np.maximum(d, np.maximum(np.abs(r["cl"] - ref["cl"]), np.abs(r["cdi"] - ref["cdi"])))
     sens = np.maximum.accumulate(d)
-    bound = np.where(sens < 1e-12, cl_tol, np.maximum(cl_tol, 1e3 * sens))
-    dcl = np.abs(dev["cl"] - ref["cl"])
-    dcdi = np.abs(dev["cdi"] - ref["cdi"])
+    bound = np.where(sens < 1e-12, cl_tol, np.maximum(cl_tol, 30.0 * sens))
     bad = np.nonzero((dcl > bound) | (dcdi > bound))[0]
     assert bad.size == 0, (bad[:5], dcl[bad[:5]], bound[bad[:5]])
     well = int(np.sum(sens < 1e-12))
-    assert well >= min(case.num_steps, 8), f"only {well} well-conditioned steps"
-    assert relerr(dev["gamma"], ref["gamma"]) < max(1e-10, 1e3 * response(ref, runs, "gamma"))
+    assert well >= min(case.num_steps, case.n_keep + 1), f"only {well} well-conditioned steps"
+    assert relerr(dev["gamma"], ref["gamma"]) < max(1e-10, 30.0 * response(ref, runs, "gamma"))
     if part and all(r["np"] == ref["np"] for r in runs):
         assert dev["np"] == ref["np"]
         if ref["np"]:
-            assert relerr(dev["pos"], ref["pos"]) < max(1e-10, 1e3 * response(ref, runs, "pos"))
+            assert relerr(dev["pos"], ref["pos"]) < max(1e-10, 30.0 * response(ref, runs, "pos"))
             assert relerr(dev["alpha"], ref["gamma_p"]) < max(
-                1e-9, 1e3 * response(ref, runs, "gamma_p"))
+                1e-9, 30.0 * response(ref, runs, "gamma_p"))
     return dev, ref, sens
+
+
+@pytest.mark.parametrize("kernel,free,stretch", [("gaussianerf", False, False),
+                                                 ("gaussianerf", True, False),
+                                                 ("winckelmans", True, False)])
+def test_heaving_small_without_relaxation(hyb, oracle, kernel, free, stretch):
+    """Flat 1e-10 at every step: without relaxation the step is well
+    conditioned (prescribed or free particle wake, no stretching)."""
+    case = hyb.heaving_wing_case(nc=4, ns=3, cycles=1.0, steps_per_cycle=24, n_keep=4,
+                                 kernel=kernel, free_wake=free, stretch=stretch, rlxf=0.0)
+    dev, ref, sens = compare(hyb, oracle, case)
+    assert dev["np"] > 0 and not np.any(sens)
 
 
 @pytest.mark.parametrize("kernel,free,stretch", [("winckelmans", True, True),
@@ -76,16 +109,41 @@ def compare(hyb, oracle, case, cl_tol=1e-10, part=True):
                                                  ("gaussianerf", True, False),
                                                  ("winckelmans", False, False)])
 def test_heaving_small(hyb, oracle, kernel, free, stretch):
+    # With stretching the cores change, and under the clamp rule the sign of
+    # the relaxation's round-off self term follows each core's last bit (the
+    # reference CPU path's behaviour), so a 1e-14 input change re-steers
+    # particles at once; stretched runs are compared under the skip rule.
     case = hyb.heaving_wing_case(nc=4, ns=3, cycles=1.0, steps_per_cycle=24, n_keep=4,
-                                 kernel=kernel, free_wake=free, stretch=stretch)
+                                 kernel=kernel, free_wake=free, stretch=stretch,
+                                 pair_rule="skip" if stretch else "clamp")
     dev, ref, sens = compare(hyb, oracle, case)
     assert dev["np"] > 0
 
 
 def test_heaving_config_geometry_short(hyb, oracle):
-    """The heaving-wing configuration (10 x 12 panels, N_keep = 10), 40 steps."""
-    case = hyb.heaving_wing_case(num_steps=40)
+    """The heaving-wing configuration (10 x 12 panels, N_keep = 10), 40 steps
+    of free wake + stretching."""
+    case = hyb.heaving_wing_case(num_steps=40, pair_rule="skip")
     compare(hyb, oracle, case)
+
+
+def test_heaving_config_prescribed_particles_full(hyb, oracle):
+    """The heaving-wing configuration over all 150 steps (3 cycles, 6.6k
+    particles) with a prescribed particle wake (RK3 at V) and no relaxation:
+    CL/CDi within 1e-10 of the oracle at every step, flat."""
+    case = hyb.heaving_wing_case(free_wake=False, stretch=False, rlxf=0.0)
+    assert case.num_steps == 150
+    dev, ref, sens = compare(hyb, oracle, case)
+    assert dev["np"] > 6000 and not np.any(sens)
+
+
+@pytest.mark.parametrize("rule", ["clamp", "skip"])
+def test_heaving_config_prescribed_particles_relaxed(hyb, oracle, rule):
+    """The same 150 steps with the relaxation on (rlxf = 0.3), under both
+    pair rules, against the oracle's own sensitivity."""
+    case = hyb.heaving_wing_case(free_wake=False, stretch=False, pair_rule=rule)
+    dev, ref, sens = compare(hyb, oracle, case)
+    assert dev["np"] > 6000
 
 
 def test_static_wing_ring_wake(hyb, oracle):
@@ -165,6 +223,12 @@ def _plunge_ratio(hyb, k, n_keep, free_wake):
     return (cl[m].max() - cl[m].min()) / (th[m].max() - th[m].min())
 
 
+# README.md:59-69: CL amplitude over Theodorsen's, (PteraSoftware ring wake,
+# FLUXVortex N=10 FREE), the latter run by the reference's HybridSolver on its
+# CPU VortexParticleField (experiment_hybrid_panel_particle.py:206-329)
+PUBLISHED_PLUNGE = {0.5: (0.929, 0.974), 0.2: (0.930, 0.946), 0.1: (0.920, 0.925)}
+
+
 def test_plunging_ring_wake_against_published_ptera_ratio(hyb):
     """Ring wake only (every row kept) at k = 0.5: the CL amplitude is 92.9 %
     of Theodorsen's in PteraSoftware's published run (README.md:59-69); this
@@ -174,13 +238,38 @@ def test_plunging_ring_wake_against_published_ptera_ratio(hyb):
     assert abs(ratio - 0.929) < 0.02
 
 
-@pytest.mark.xfail(reason="the published hybrid ratios (97.4 / 94.6 / 92.5 %) are not "
-                          "reproduced: this statement gives 90 / 75 / 68 %", strict=False)
-@pytest.mark.parametrize("k,published", [(0.5, 0.974), (0.2, 0.946), (0.1, 0.925)])
-def test_plunging_sweep_against_published_hybrid_ratios(hyb, k, published):
+@pytest.mark.parametrize("k", [0.5, 0.2, 0.1])
+def test_plunging_hybrid_gain_over_ring_matches_published(hyb, k):
+    """The hybrid wake (N_keep = 10, free particle wake, relaxation under the
+    reference CPU pair rule) against the ring wake of the same UVLM statement:
+    the amplitude gain matches the published gain of the reference's hybrid
+    over PteraSoftware's ring wake (+4.5 / +1.6 / +0.5 points at k = 0.5 /
+    0.2 / 0.1) within one point.  The gain comes from the relaxation's
+    clamped Gaussian self pair, whose round-off g(1e-12/sigma) < 0 turns every
+    isolated particle's alpha against itself (Gamma x 0.62 per step), so the
+    shed far wake fades; with the self pair skipped (warp_vpm.py's rule) the
+    gain is -1.2 / -10.8 / -14.9 points."""
+    ring = _plunge_ratio(hyb, k, 1000, False)
+    hybr = _plunge_ratio(hyb, k, 10, True)
+    pub = PUBLISHED_PLUNGE[k][1] - PUBLISHED_PLUNGE[k][0]
+    print(f"k={k}: ring {ring:.1%}, hybrid {hybr:.1%}, gain {hybr - ring:+.2%} "
+          f"(published {pub:+.2%})", flush=True)
+    assert abs((hybr - ring) - pub) < 0.01
+
+
+_RING_GAP = pytest.mark.xfail(
+    reason="absolute ratios at k = 0.2 / 0.1 inherit this UVLM statement's ring-wake baseline "
+           "(85.9 / 83.1 % vs PteraSoftware's 93.0 / 92.0 %); the hybrid gain over it matches "
+           "the published one", strict=True)
+
+
+@pytest.mark.parametrize("k", [0.5, pytest.param(0.2, marks=_RING_GAP),
+                               pytest.param(0.1, marks=_RING_GAP)])
+def test_plunging_sweep_against_published_hybrid_ratios(hyb, k):
     """The reduced-frequency sweep with the hybrid wake (N_keep = 10, free
     particle wake): CL amplitude over Theodorsen's against the reference's
     published FLUXVortex N=10 FREE ratios (README.md:59-69)."""
+    published = PUBLISHED_PLUNGE[k][1]
     ratio = _plunge_ratio(hyb, k, 10, True)
     print(f"hybrid k={k}: {ratio:.1%} of Theodorsen (published {published:.1%})", flush=True)
-    assert abs(ratio - published) < 0.03
+    assert abs(ratio - published) < 0.02
