@@ -189,6 +189,9 @@ static int tuned_plan(fv_ctx *ctx, const typename Op::Args &a, int64_t n_tgt, in
 {
     out = plan_pairs(ctx, n_tgt, n_src);
     if (!ctx->autotune) return FV_OK;
+    // tuning runs 36 trial sums: beyond 2^36 pairs (~50 ms a trial) the
+    // default shape (what the tuner picks at 2^17 x 2^17) is kept
+    if ((double)n_tgt * (double)n_src > 68719476736.0) return FV_OK;
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(ctx->stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
         return FV_OK;

@@ -947,7 +947,14 @@ int fv_hybrid_pair_counts(fv_hybrid *h, double *seg_pairs, double *particle_pair
 {
     if (!h) return FV_E_ARG;
     if (seg_pairs) *seg_pairs = h->seg_pairs;
-    if (particle_pairs) *particle_pairs = h->vpm->particle_pairs;
+    if (particle_pairs) {
+        fv_ctx *ctx = h->ctx;
+        double dev[2] = {0.0, 0.0};
+        FV_HIP_CHECK(ctx, hipMemcpyAsync(dev, h->vpm->pairs_dev, sizeof dev,
+                                         hipMemcpyDeviceToHost, ctx->stream));
+        FV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        *particle_pairs = dev[0] + dev[1] + h->vpm->particle_pairs;
+    }
     return FV_OK;
 }
 

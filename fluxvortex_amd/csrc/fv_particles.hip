@@ -89,13 +89,18 @@ __global__ void k_vpm_stage_gamma(const int64_t *__restrict__ np_dev,
 }
 
 // particles.py:185-196: non-finite position or circulation -> Gamma = 0; age += dt.
+// Thread 0 also counts this advect's direct pair sums: passes x n^2 pairs
+// into pairs_dev[0] (exact count, no host synchronisation).
 __global__ void k_vpm_finish(const int64_t *__restrict__ np_dev, const double *__restrict__ x,
                              const double *__restrict__ y, const double *__restrict__ z,
                              double *__restrict__ gx, double *__restrict__ gy,
-                             double *__restrict__ gz, double *__restrict__ age, double dt)
+                             double *__restrict__ gz, double *__restrict__ age, double dt,
+                             double passes, double *__restrict__ pairs_dev)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= *np_dev) return;
+    const int64_t n = *np_dev;
+    if (i == 0) pairs_dev[0] += passes * (double)n * (double)n;
+    if (i >= n) return;
     const bool bad = !isfinite(x[i]) || !isfinite(y[i]) || !isfinite(z[i]) ||
                      !isfinite(gx[i]) || !isfinite(gy[i]) || !isfinite(gz[i]);
     if (bad) {
@@ -238,10 +243,12 @@ int vpm_convert_rings(fv_vpm *f, int64_t nr, const double *br, const double *fr,
 {
     fv_ctx *ctx = f->ctx;
     if (nr <= 0) return FV_OK;
+    TimedLaunch tl(ctx, "convert");
     hipLaunchKernelGGL(k_convert_rings, dim3(1), dim3(kConvertBlock), 0, ctx->stream, nr, br,
                        fr, fl, bl, gamma, sigma, f->cap, f->np_dev, f->overflow_dev, f->x, f->y,
                        f->z, f->gx, f->gy, f->gz, f->sig, f->age);
     FV_LAUNCH_CHECK(ctx);
+    tl.done();
     f->np_exact = false;
     f->state_epoch++;
     f->np_upper += 4 * nr;
@@ -269,6 +276,13 @@ static PartTgtArgs field_tgt(const fv_vpm *f)
     return t;
 }
 
+// pairs_dev[1] += targets x the device particle count (one thread)
+__global__ void k_count_induce(const int64_t *__restrict__ np_dev, double targets,
+                               double *__restrict__ pairs_dev)
+{
+    pairs_dev[1] += targets * (double)*np_dev;
+}
+
 int vpm_induce(fv_vpm *f, const double *targets, int64_t n, int precision, double *out,
                int pslot)
 {
@@ -276,9 +290,13 @@ int vpm_induce(fv_vpm *f, const double *targets, int64_t n, int precision, doubl
     t.x = targets; t.y = targets + 1; t.z = targets + 2;
     t.ax = t.ay = t.az = nullptr;
     t.stride = 3;
-    f->particle_pairs += (double)n * (double)f->np_upper;
-    return part_sum(f->ctx, f->kernel, precision, 0, field_src(f, f->kernel), t, n, f->np_upper,
-                    nullptr, f->np_dev, out, pslot, "particle_induce");
+    int rc = part_sum(f->ctx, f->kernel, precision, 0, field_src(f, f->kernel), t, n,
+                      f->np_upper, nullptr, f->np_dev, out, pslot, "particle_induce");
+    if (rc || n <= 0 || f->np_upper <= 0) return rc;
+    hipLaunchKernelGGL(k_count_induce, dim3(1), dim3(1), 0, f->ctx->stream, f->np_dev,
+                       (double)n, f->pairs_dev);
+    FV_LAUNCH_CHECK(f->ctx);
+    return FV_OK;
 }
 
 int vpm_advect(fv_vpm *f, double dt, const double *uinf, int stretch, int free_wake)
@@ -302,9 +320,12 @@ int vpm_advect(fv_vpm *f, double dt, const double *uinf, int stretch, int free_w
     const PartSrcArgs src = field_src(f, f->kernel);
     const PartTgtArgs tgt = field_tgt(f);
     // self sum of one mode, by the closure when it is in use, else directly
+    // direct self-sum passes of this advect (counted on the device)
+    const double passes = clo ? 0.0
+                              : (free_wake ? 3.0 : 0.0) + (stretch ? 3.0 : 0.0) +
+                                    (f->rlxf > 0 ? 1.0 : 0.0);
     auto self_sum = [&](int mode, int kernel) -> int {
         if (clo) return closure_self_sum(f, mode, kernel, f->sum);
-        f->particle_pairs += (double)nu * (double)nu;
         return part_sum(ctx, kernel, f->precision, mode,
                         kernel == f->kernel ? src : field_src(f, kernel), tgt, nu, nu, f->np_dev,
                         f->np_dev, f->sum, S_TMP0);
@@ -332,7 +353,7 @@ int vpm_advect(fv_vpm *f, double dt, const double *uinf, int stretch, int free_w
         }
     }
     hipLaunchKernelGGL(k_vpm_finish, dim3(gb), dim3(256), 0, ctx->stream, f->np_dev, f->x, f->y,
-                       f->z, f->gx, f->gy, f->gz, f->age, dt);
+                       f->z, f->gx, f->gy, f->gz, f->age, dt, passes, f->pairs_dev);
     FV_LAUNCH_CHECK(ctx);
     f->state_epoch++;
     if (f->rlxf > 0) {
@@ -465,7 +486,8 @@ int fv_vpm_create(fv_ctx *ctx, int64_t max_particles, double nu, double rlxf, in
     f->stage = p; p += 9 * cap;
     f->np_dev = reinterpret_cast<int64_t *>(p);
     f->overflow_dev = reinterpret_cast<int32_t *>(p + 1);
-    const int64_t zero[2] = {0, 0};
+    f->pairs_dev = p + 2;
+    const int64_t zero[4] = {0, 0, 0, 0};
     e = hipMemcpyAsync(f->np_dev, zero, sizeof zero, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {

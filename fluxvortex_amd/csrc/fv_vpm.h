@@ -28,7 +28,9 @@ struct fv_vpm {
     int64_t np_host = 0;
     bool np_exact = true;
     int64_t np_upper = 0;
-    double particle_pairs = 0.0;   // pairs evaluated, from the host-side upper bound
+    double particle_pairs = 0.0;   // pairs of host-counted passes (closure checks)
+    double *pairs_dev = nullptr;   // [0] direct self-sum pairs, [1] induced-target pairs,
+                                   // counted on the device from the exact particle count
     int64_t state_epoch = 0;       // bumped by every enqueued change of the particle state
     fv_closure *closure = nullptr; // far-field closure (optional)
     void *pop = nullptr;           // population-control scratch (fv_population.hip), lazily made

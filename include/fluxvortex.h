@@ -83,7 +83,9 @@ int fv_ctx_set_timing(fv_ctx *ctx, int enabled);
  * of targets and sources) times every shape with HIP events (synchronising
  * once) and every later call of that kind uses the fastest.  Shapes only
  * change how sources are split, so results differ at round-off level from
- * the default shape.  Off by default (no counterpart in the reference). */
+ * the default shape.  Sums beyond 2^36 pairs are not tuned (36 trial sums
+ * would take seconds); they keep the default shape.  Off by default (no
+ * counterpart in the reference). */
 int fv_ctx_set_autotune(fv_ctx *ctx, int32_t enabled);
 /* The tuned kinds, one text line each: family log2_targets log2_sources
  * lanes targets_per_lane workgroups_per_cu best_ms. */
@@ -269,7 +271,9 @@ int fv_hybrid_current_step(fv_hybrid *h, int64_t *step);
 int fv_hybrid_read(fv_hybrid *h, double *cl, double *cdi, double *gamma);
 /* The particle field driven by the hybrid case (owned by it). */
 fv_vpm *fv_hybrid_particles(fv_hybrid *h);
-/* Pair counts (segment pairs, particle pairs) evaluated since creation. */
+/* Pair counts (segment pairs, particle pairs) evaluated since creation; the
+ * particle pairs are counted on the device from the exact particle count of
+ * each pass (this call synchronises the context stream). */
 int fv_hybrid_pair_counts(fv_hybrid *h, double *seg_pairs, double *particle_pairs);
 /* Switch the particle-field flags / the arithmetic for the following steps
  * (HybridSolver(free_vpm=...) and advect_rk3(stretch=...),
