@@ -92,6 +92,9 @@ _SIGNATURES = {
                                            _dp, _dp, _dp, C.c_double, C.c_int32, C.c_int32, _dp]),
     "fv_vortex_collapsed_dev": (C.c_int, [_ctx, _vp, C.c_int64, _vp, _vp, _vp, _vp, C.c_int64,
                                           _vp, _vp, _vp, C.c_double, C.c_int32, C.c_int32, _vp]),
+    "fv_vortex_collapsed_dev_noalloc": (C.c_int, [_ctx, _vp, C.c_int64, _vp, _vp, _vp, _vp,
+                                                  C.c_int64, _vp, _vp, _vp, C.c_double,
+                                                  C.c_int32, C.c_int32, _vp]),
     "fv_vortex_expanded_host": (C.c_int, [_ctx, _dp, C.c_int64, _dp, _dp, _dp, _dp, C.c_int64,
                                           _dp, _dp, _dp, C.c_double, C.c_int32, _dp]),
     "fv_vortex_expanded_dev": (C.c_int, [_ctx, _vp, C.c_int64, _vp, _vp, _vp, _vp, C.c_int64,

@@ -400,6 +400,40 @@ static int check_vortex_args(fv_ctx *ctx, const double *points, int64_t n, const
     return FV_OK;
 }
 
+// ------------------------------------------------ no-allocation collapsed sum
+// fv_vortex_collapsed_dev_noalloc keeps its segment table and per-split partial
+// sums in this file-scope device array instead of context workspaces, so it
+// never allocates.  Each device the library runs on holds its own copy until
+// the process ends.  In host code `scratch` only names the symbol: its device
+// address comes from hipGetSymbolAddress, looked up once per device (with that
+// device current) at the entry point's first call there, and kept below.
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;   // 32 MiB per device
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_dev[kMaxDevices];
+
+static int noalloc_scratch(fv_ctx *ctx, void **out)
+{
+    FV_REQUIRE(ctx, ctx->device >= 0 && ctx->device < kMaxDevices,
+               "device %d outside the scratch table", ctx->device);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[ctx->device]) {
+        int cur = -1;
+        FV_HIP_CHECK(ctx, hipGetDevice(&cur));
+        if (cur != ctx->device) FV_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (cur != ctx->device) (void)hipSetDevice(cur);
+        if (e != hipSuccess || !ptr)
+            return set_error(ctx, FV_E_HIP, "hipGetSymbolAddress(scratch) failed: %s",
+                             hipGetErrorString(e));
+        g_scratch_dev[ctx->device] = ptr;
+    }
+    *out = g_scratch_dev[ctx->device];
+    return FV_OK;
+}
+
 extern "C" {
 
 int fv_vortex_collapsed_dev(fv_ctx *ctx, const double *points, int64_t n, const double *br,
@@ -418,6 +452,60 @@ int fv_vortex_collapsed_dev(fv_ctx *ctx, const double *points, int64_t n, const 
     rc = build_segtab(ctx, br, fr, fl, bl, m, strengths, r_c0s, ages, nu, leg_mask, false, tab);
     if (rc) return rc;
     return seg_sum(ctx, points, n, tab, m * nl, precision, out, S_PARTIAL);
+}
+
+int fv_vortex_collapsed_dev_noalloc(fv_ctx *ctx, const double *points, int64_t n,
+                                    const double *br, const double *fr, const double *fl,
+                                    const double *bl, int64_t m, const double *strengths,
+                                    const double *r_c0s, const double *ages, double nu,
+                                    int32_t leg_mask, int32_t precision, double *out)
+{
+    int rc = check_vortex_args(ctx, points, n, br, fr, fl, bl, m, strengths, r_c0s, leg_mask,
+                               precision, out);
+    if (rc) return rc;
+    if (n == 0) return FV_OK;
+    if (m == 0) return fill(ctx, out, 3 * n, 0.0);
+    const int nl = legs_in_mask(leg_mask);
+    const int64_t nseg = m * nl;
+    const size_t seg_bytes = (sizeof(SegRec) * (size_t)nseg + 255) & ~size_t(255);
+    FV_REQUIRE(ctx, seg_bytes <= kNoAllocScratchBytes,
+               "no-allocation sum: %lld segments need %zu bytes of segment table, over the "
+               "%zu-byte scratch (at most %zu segments)", (long long)nseg, seg_bytes,
+               kNoAllocScratchBytes, kNoAllocScratchBytes / sizeof(SegRec));
+    void *base = nullptr;
+    rc = noalloc_scratch(ctx, &base);
+    if (rc) return rc;
+    SegRec *tab = static_cast<SegRec *>(base);
+    double *part = reinterpret_cast<double *>(static_cast<unsigned char *>(base) + seg_bytes);
+    // the default tile shape (that of fv_vortex_collapsed_dev without the
+    // autotuner), its source split capped so the partial sums fit
+    PairPlan p = plan_pairs(ctx, n, nseg);
+    const int64_t max_split =
+        (int64_t)((kNoAllocScratchBytes - seg_bytes) / (sizeof(double) * 3 * (size_t)n));
+    if (p.split > 1 && p.split > max_split) {
+        const int64_t tiles = ceil_div(nseg, p.block);
+        const int64_t cap = max_split > 1 ? max_split : 1;
+        p.chunk = ceil_div(tiles, cap) * p.block;
+        p.split = (int)ceil_div(nseg, p.chunk);
+    }
+    rc = build_segtab(ctx, br, fr, fl, bl, m, strengths, r_c0s, ages, nu, leg_mask, false, tab);
+    if (rc) return rc;
+    double *dst = p.split > 1 ? part : out;
+    if (precision == FV_PREC_FP64) {
+        SegOp<0>::Args a{points, tab};
+        dispatch_pairs<SegOp<0>>(ctx, p, a, n, nseg, nullptr, nullptr, dst);
+    } else {
+        SegOp<1>::Args a{points, tab};
+        dispatch_pairs<SegOp<1>>(ctx, p, a, n, nseg, nullptr, nullptr, dst);
+    }
+    FV_LAUNCH_CHECK(ctx);
+    if (p.split > 1) {
+        hipLaunchKernelGGL((k_reduce_partials<3>), dim3((unsigned)ceil_div(3 * n, 256)),
+                           dim3(256), 0, ctx->stream, part, n, (const int64_t *)nullptr, n,
+                           p.split, out);
+        FV_LAUNCH_CHECK(ctx);
+    }
+    return FV_OK;
 }
 
 int fv_vortex_collapsed_host(fv_ctx *ctx, const double *points, int64_t n, const double *br,

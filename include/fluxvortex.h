@@ -110,6 +110,21 @@ int fv_vortex_collapsed_dev(fv_ctx *ctx, const double *points, int64_t n, const 
                             const double *fr, const double *fl, const double *bl, int64_t m,
                             const double *strengths, const double *r_c0s, const double *ages,
                             double nu, int32_t leg_mask, int32_t precision, double *out);
+/* fv_vortex_collapsed_dev without any allocation.  No counterpart in the
+ * reference: this entry point is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments and results as fv_vortex_collapsed_dev
+ * with the autotuner off (the source split may be capped, which moves the
+ * result by round-off only).  Its segment table and partial sums live in a
+ * 32 MiB file-scope device array that every GPU the library runs on holds
+ * until the process ends: at most 419 430 segments (m x legs); beyond that
+ * FV_E_ARG with a message.  Calls that overlap on one GPU (several contexts
+ * or streams at once) race on that array; calls on one stream are ordered. */
+int fv_vortex_collapsed_dev_noalloc(fv_ctx *ctx, const double *points, int64_t n,
+                                    const double *br, const double *fr, const double *fl,
+                                    const double *bl, int64_t m, const double *strengths,
+                                    const double *r_c0s, const double *ages, double nu,
+                                    int32_t leg_mask, int32_t precision, double *out);
 /* Expanded: out[i][m] = sum_legs v(points[i]; leg of vortex m)       (n, m, 3)
  * Replaces warp_kernels.py:305-330 (ring) and :360-382 (horseshoe). */
 int fv_vortex_expanded_host(fv_ctx *ctx, const double *points, int64_t n, const double *br,
