@@ -143,6 +143,8 @@ _SIGNATURES = {
     "fv_hybrid_pair_counts": (C.c_int, [_vp, _dp, _dp]),
     "fv_hybrid_set_flags": (C.c_int, [_vp, C.c_int32, C.c_int32]),
     "fv_hybrid_set_precision": (C.c_int, [_vp, C.c_int32]),
+    "fv_hybrid_set_solve_reuse": (C.c_int, [_vp, C.c_int32]),
+    "fv_hybrid_solve_reuse_count": (C.c_int, [_vp, _i64p]),
     "fv_vpm_set_closure": (C.c_int, [_vp, _vp, C.c_double, C.c_double, C.c_int32, C.c_double,
                                      C.c_int32, C.c_int32, C.c_double]),
     "fv_vpm_set_closure_local": (C.c_int, [_vp, C.c_double]),

@@ -22,12 +22,15 @@
 #include <new>
 #include <vector>
 
+#include "fv_lu.h"
 #include "fv_vpm.h"
 
 namespace fv {
-int dense_solve(fv_ctx *ctx, int64_t batch, int64_t n, double *A, double *rhs, int32_t *info);
+int dense_solve(fv_ctx *ctx, int64_t batch, int64_t n, double *A, double *rhs, int32_t *info,
+                const LuReuse *reuse);
 int dense_solve_mixed(fv_ctx *ctx, int64_t batch, int64_t n, const double *A, double *rhs,
-                      int max_iter, double tol, int32_t *info, int32_t *iters);
+                      int max_iter, double tol, int32_t *info, int32_t *iters,
+                      const LuReuse *reuse);
 }
 
 struct fv_surface {
@@ -65,6 +68,11 @@ struct fv_hybrid {
     double *cl = nullptr, *cdi = nullptr, *force = nullptr;
     fv::SegRec *unit_tab = nullptr, *tab = nullptr;
     int32_t *info = nullptr;                           // dense-solve status
+    // factor reuse of the circulation solve: a step whose AIC equals the last
+    // factored one bit for bit (a lattice that does not move) skips the
+    // factorization (the solution is the same bit for bit)
+    fv::LuReuse reuse{nullptr, nullptr, nullptr, nullptr, nullptr};
+    int reuse_on = 1;
     int32_t *d_nc = nullptr, *d_ns = nullptr;          // per-surface sizes (device)
     int64_t *d_poff = nullptr;                         // per-surface panel offsets (device)
     int64_t *d_woff = nullptr;                         // per-surface wake ring offsets (device)
@@ -489,8 +497,9 @@ static int hybrid_step_one(fv_hybrid *h)
     const bool overlap = shed && d.n_keep >= 1 && !h->ring_free && h->side;
     const int64_t nrow_old = h->nrow;
     auto solve = [&]() -> int {
-        if (prec == FV_PREC_FP64) return dense_solve(ctx, 1, P, h->aic, h->rhs, h->info);
-        return dense_solve_mixed(ctx, 1, P, h->aic, h->rhs, 8, 1e-12, h->info, nullptr);
+        const LuReuse *ru = h->reuse_on ? &h->reuse : nullptr;
+        if (prec == FV_PREC_FP64) return dense_solve(ctx, 1, P, h->aic, h->rhs, h->info, ru);
+        return dense_solve_mixed(ctx, 1, P, h->aic, h->rhs, 8, 1e-12, h->info, nullptr, ru);
     };
     // loads: velocities at leg centres from bound + wake rings (this step's wake)
     auto loads = [&]() -> int {
@@ -706,6 +715,7 @@ int fv_hybrid_create(fv_ctx *ctx, const fv_hybrid_desc *desc, fv_hybrid **out)
     n += 4 * 3 * P + 3 * P + 3 * P + P + 3 * P;      // rings, colloc, normal, area, vm_col
     n += 3 * 12 * P;                                 // leg_c, leg_v, vm_leg
     n += (size_t)P * P + 3 * P;                      // aic, rhs, gam, gam_prev
+    n += 2 * (size_t)P * P;                          // factor reuse: key, factors
     n += 3 * P + 3 * P + 12 * P + 3 * P;             // vw, vp, vl, force
     n += 2 * (3 * WV + 2 * WR);                      // wake double buffers
     n += 2 * 3 * WV;                                 // free-wake points, velocities
@@ -713,7 +723,8 @@ int fv_hybrid_create(fv_ctx *ctx, const fv_hybrid_desc *desc, fv_hybrid **out)
     n += 4 * 3 * NS + NS;                            // conversion corners + cg
     n += 2 * (size_t)d.num_steps;                    // cl, cdi
     const size_t tab_bytes = sizeof(SegRec) * (size_t)(4 * P + 4 * WR + 4 * P);
-    const size_t meta_bytes = sizeof(int64_t) * 2 * S + sizeof(int32_t) * (1 + 2 * S) + 64;
+    const size_t meta_bytes =
+        sizeof(int64_t) * 2 * S + sizeof(int32_t) * (1 + 2 * S + 2 + P) + 64;
     const size_t bytes = n * sizeof(double) + tab_bytes + meta_bytes + 1024;
     hipError_t e = hipMalloc(&h->block, bytes);
     if (e != hipSuccess) {
@@ -736,6 +747,8 @@ int fv_hybrid_create(fv_ctx *ctx, const fv_hybrid_desc *desc, fv_hybrid **out)
     h->colloc = take(3 * P); h->normal = take(3 * P); h->area = take(P); h->vm_col = take(3 * P);
     h->leg_c = take(12 * P); h->leg_v = take(12 * P); h->vm_leg = take(12 * P);
     h->aic = take((size_t)P * P); h->rhs = take(P); h->gam = take(P); h->gam_prev = take(P);
+    h->reuse.key = take((size_t)P * P);
+    h->reuse.fac = take((size_t)P * P);
     h->vw = take(3 * P); h->vp = take(3 * P); h->vl = take(12 * P); h->force = take(3 * P);
     for (int b = 0; b < 2; ++b) {
         h->wv[b] = take(3 * WV);
@@ -757,8 +770,11 @@ int fv_hybrid_create(fv_ctx *ctx, const fv_hybrid_desc *desc, fv_hybrid **out)
     h->info = m32p;
     h->d_nc = m32p + 1;
     h->d_ns = m32p + 1 + S;
+    h->reuse.state = m32p + 1 + 2 * S;
+    h->reuse.hits = m32p + 2 + 2 * S;
+    h->reuse.order = m32p + 3 + 2 * S;
     std::vector<int64_t> po(2 * S);
-    std::vector<int32_t> m32(1 + 2 * S, 0);
+    std::vector<int32_t> m32(1 + 2 * S + 2, 0);   // status, nc, ns, reuse state + hits
     for (size_t k = 0; k < S; ++k) {
         po[k] = h->surf[k].panel_off;
         po[S + k] = h->surf[k].wring_off;
@@ -938,8 +954,33 @@ int fv_hybrid_set_precision(fv_hybrid *h, int32_t precision)
     if (!h) return FV_E_ARG;
     FV_REQUIRE(h->ctx, precision == FV_PREC_FP64 || precision == FV_PREC_MIXED,
                "unknown precision %d", precision);
+    if (precision != h->d.precision) {
+        // the stored factors are of the other precision's type
+        FV_HIP_CHECK(h->ctx, hipMemsetAsync(h->reuse.state, 0, sizeof(int32_t), h->ctx->stream));
+    }
     h->d.precision = precision;
     h->vpm->precision = precision;
+    return FV_OK;
+}
+
+int fv_hybrid_set_solve_reuse(fv_hybrid *h, int32_t on)
+{
+    if (!h) return FV_E_ARG;
+    if (!on) FV_HIP_CHECK(h->ctx, hipMemsetAsync(h->reuse.state, 0, sizeof(int32_t), h->ctx->stream));
+    h->reuse_on = on ? 1 : 0;
+    return FV_OK;
+}
+
+int fv_hybrid_solve_reuse_count(fv_hybrid *h, int64_t *reused)
+{
+    if (!h) return FV_E_ARG;
+    fv_ctx *ctx = h->ctx;
+    FV_REQUIRE(ctx, reused, "null output");
+    int32_t v = 0;
+    FV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if (h->side) FV_HIP_CHECK(ctx, hipStreamSynchronize(h->side));
+    FV_HIP_CHECK(ctx, hipMemcpy(&v, h->reuse.hits, sizeof v, hipMemcpyDeviceToHost));
+    *reused = v;
     return FV_OK;
 }
 

@@ -26,22 +26,26 @@ struct LuLds {
 
 __host__ __device__ inline size_t lu_align(size_t v) { return (v + 15) & ~size_t(15); }
 
+// vec_lds / vec_glob: bytes of the vectors when the matrix is in LDS / in
+// global memory; limit: the dynamic LDS a workgroup may take (the kernels'
+// own static LDS already subtracted).
 template <typename T>
-static LuLds lu_lds_plan(int64_t n, size_t vec_bytes, size_t limit)
+static LuLds lu_lds_plan(int64_t n, size_t vec_lds, size_t vec_glob, size_t limit)
 {
     LuLds p;
     const int64_t ldo = n | 1;
     const size_t full = lu_align((size_t)n * ldo * sizeof(T));
-    if (full + vec_bytes <= limit) {
+    if (full + vec_lds <= limit) {
         p.in_lds = 1;
         p.lda = (int)ldo;
         p.mat = full;
+        p.total = p.mat + vec_lds;
     } else {
         p.in_lds = 0;
         p.lda = (int)n;
         p.mat = lu_align((size_t)n * kLdp * sizeof(T));
+        p.total = p.mat + vec_glob;
     }
-    p.total = p.mat + vec_bytes;
     return p;
 }
 
@@ -52,32 +56,77 @@ template <int IN_LDS>
 __global__ __launch_bounds__(kLuBlock) void k_lu_solve_f64(int n, double *__restrict__ A,
                                                            double *__restrict__ B,
                                                            int32_t *__restrict__ info,
-                                                           int lda_lds, unsigned mat_bytes)
+                                                           int lda_lds, unsigned mat_bytes,
+                                                           LuReuse ru)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
-    double *a = A + (int64_t)blockIdx.x * n * n;
+    const int64_t nn = (int64_t)n * n;
+    double *a = A + (int64_t)blockIdx.x * nn;
     double *bg = B + (int64_t)blockIdx.x * n;
     double *area = reinterpret_cast<double *>(lds_raw);
     double *b = reinterpret_cast<double *>(lds_raw + mat_bytes);
     double *rdiag = b + n;
     int *perm = reinterpret_cast<int *>(rdiag + n);
+    // the row order: after the pivots in LDS, or in the staged-panel area
+    // (free once the factorization is done) when the matrix is in global memory
+    int *order = IN_LDS ? perm + n : reinterpret_cast<int *>(area);
     double *w = IN_LDS ? area : a;
     const int lda = IN_LDS ? lda_lds : n;
-    if (IN_LDS) {
-        for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += kLuBlock) {
+    double *key = ru.key ? ru.key + (int64_t)blockIdx.x * nn : nullptr;
+    double *fac = ru.key ? static_cast<double *>(ru.fac) + (int64_t)blockIdx.x * nn : nullptr;
+    int32_t *ord = ru.key ? ru.order + (int64_t)blockIdx.x * n : nullptr;
+    bool differ = !key || ru.state[blockIdx.x] != 1;
+    for (int64_t e = threadIdx.x; e < nn; e += kLuBlock) {
+        const double v = a[e];
+        if (IN_LDS) {
             const int r = (int)(e / n), c = (int)(e - (int64_t)r * n);
-            w[(int64_t)r * lda + c] = a[e];
+            w[(int64_t)r * lda + c] = v;
+        }
+        if (key && !differ) differ = bits_differ(v, key[e]);
+    }
+    const bool reuse = !__syncthreads_or(differ ? 1 : 0);
+    int inf = 0;
+    if (reuse) {
+        // the stored factors and row order of this matrix
+        for (int64_t e = threadIdx.x; e < nn; e += kLuBlock) {
+            const int r = (int)(e / n), c = (int)(e - (int64_t)r * n);
+            w[(int64_t)r * lda + c] = fac[e];
+        }
+        for (int i = threadIdx.x; i < n; i += kLuBlock) order[i] = ord[i];
+        if (threadIdx.x == 0 && ru.hits) ru.hits[blockIdx.x] += 1;
+    } else {
+        if (key) {
+            for (int64_t e = threadIdx.x; e < nn; e += kLuBlock) key[e] = a[e];
+            if (threadIdx.x == 0) ru.state[blockIdx.x] = 0;
+        }
+        inf = lu_blocked<double, double>(w, lda, n, perm, (double *)nullptr,
+                                         IN_LDS ? nullptr : area);
+        // the pivot sequence composed into one row order: (P v)[i] = v[order[i]]
+        for (int i = threadIdx.x; i < n; i += kLuBlock) order[pivot_track(perm, n, i)] = i;
+        if (key && inf == 0) {
+            for (int64_t e = threadIdx.x; e < nn; e += kLuBlock) {
+                const int r = (int)(e / n), c = (int)(e - (int64_t)r * n);
+                fac[e] = w[(int64_t)r * lda + c];
+            }
+            __syncthreads();
+            for (int i = threadIdx.x; i < n; i += kLuBlock) ord[i] = order[i];
+            if (threadIdx.x == 0) ru.state[blockIdx.x] = 1;
         }
     }
-    for (int i = threadIdx.x; i < n; i += kLuBlock) b[i] = bg[i];
     __syncthreads();
-    const int inf = lu_blocked<double, double>(w, lda, n, perm, b, IN_LDS ? nullptr : area);
-    for (int i = threadIdx.x; i < n; i += kLuBlock) rdiag[i] = 1.0 / w[(int64_t)i * lda + i];
+    // both paths solve with the same two substitutions: L y = P b, U x = y
+    for (int i = threadIdx.x; i < n; i += kLuBlock) {
+        b[i] = bg[order[i]];
+        rdiag[i] = 1.0 / w[(int64_t)i * lda + i];
+    }
     __syncthreads();
-    if (inf == 0) wave_back_subst<double, double>(w, lda, n, b, rdiag);
+    if (inf == 0) {
+        wave_fwd_subst<double, double>(w, lda, n, b);
+        wave_back_subst<double, double>(w, lda, n, b, rdiag);
+    }
     __syncthreads();
     if (IN_LDS)
-        for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += kLuBlock) {
+        for (int64_t e = threadIdx.x; e < nn; e += kLuBlock) {
             const int r = (int)(e / n), c = (int)(e - (int64_t)r * n);
             a[e] = w[(int64_t)r * lda + c];
         }
@@ -91,7 +140,7 @@ template <int IN_LDS>
 __global__ __launch_bounds__(kLuBlock) void k_lu_solve_mixed(
     int n, const double *__restrict__ A, double *__restrict__ B, float *__restrict__ f32,
     int max_iter, double tol, int32_t *__restrict__ info, int32_t *__restrict__ iters,
-    int lda_lds, unsigned mat_bytes)
+    int lda_lds, unsigned mat_bytes, LuReuse ru)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
     const double *a = A + (int64_t)blockIdx.x * n * n;
@@ -106,37 +155,59 @@ __global__ __launch_bounds__(kLuBlock) void k_lu_solve_mixed(
     float *lu = IN_LDS ? area : f32 + (int64_t)blockIdx.x * n * n;
     const int lda = IN_LDS ? lda_lds : n;
     __shared__ double stat[2];
-    for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += kLuBlock) {
+    const int64_t nn = (int64_t)n * n;
+    double *key = ru.key ? ru.key + (int64_t)blockIdx.x * nn : nullptr;
+    float *fac = ru.key ? static_cast<float *>(ru.fac) + (int64_t)blockIdx.x * nn : nullptr;
+    int32_t *ord = ru.key ? ru.order + (int64_t)blockIdx.x * n : nullptr;
+    bool differ = !key || ru.state[blockIdx.x] != 1;
+    for (int64_t e = threadIdx.x; e < nn; e += kLuBlock) {
+        const double v = a[e];
         const int rr = (int)(e / n), c = (int)(e - (int64_t)rr * n);
-        lu[(int64_t)rr * lda + c] = (float)a[e];
+        lu[(int64_t)rr * lda + c] = (float)v;
+        if (key && !differ) differ = bits_differ(v, key[e]);
+    }
+    const bool reuse = !__syncthreads_or(differ ? 1 : 0);
+    if (reuse) {
+        // the stored FP32 factors and row order of this matrix
+        for (int64_t e = threadIdx.x; e < nn; e += kLuBlock) {
+            const int rr = (int)(e / n), c = (int)(e - (int64_t)rr * n);
+            lu[(int64_t)rr * lda + c] = fac[e];
+        }
+        for (int i = threadIdx.x; i < n; i += kLuBlock) order[i] = ord[i];
+        if (threadIdx.x == 0 && ru.hits) ru.hits[blockIdx.x] += 1;
+    } else {
+        if (key) {
+            for (int64_t e = threadIdx.x; e < nn; e += kLuBlock) key[e] = a[e];
+            if (threadIdx.x == 0) ru.state[blockIdx.x] = 0;
+        }
+        __syncthreads();
+        const int inf = lu_blocked<float, double>(lu, lda, n, perm, (double *)nullptr,
+                                                  IN_LDS ? nullptr : area);
+        if (inf != 0) {
+            if (threadIdx.x == 0) {
+                if (info) info[blockIdx.x] = inf;
+                if (iters) iters[blockIdx.x] = 0;
+            }
+            return;
+        }
+        // the pivot sequence composed into one row order: (P v)[i] = v[order[i]]
+        for (int i = threadIdx.x; i < n; i += kLuBlock) order[pivot_track(perm, n, i)] = i;
+        if (key) {
+            for (int64_t e = threadIdx.x; e < nn; e += kLuBlock) {
+                const int rr = (int)(e / n), c = (int)(e - (int64_t)rr * n);
+                fac[e] = lu[(int64_t)rr * lda + c];
+            }
+            __syncthreads();
+            for (int i = threadIdx.x; i < n; i += kLuBlock) ord[i] = order[i];
+            if (threadIdx.x == 0) ru.state[blockIdx.x] = 1;
+        }
     }
     __syncthreads();
-    const int inf = lu_blocked<float, double>(lu, lda, n, perm, (double *)nullptr,
-                                              IN_LDS ? nullptr : area);
-    if (inf != 0) {
-        if (threadIdx.x == 0) {
-            if (info) info[blockIdx.x] = inf;
-            if (iters) iters[blockIdx.x] = 0;
-        }
-        return;
-    }
     for (int i = threadIdx.x; i < n; i += kLuBlock) {
         x[i] = 0.0;
         r[i] = bg[i];
-        order[i] = i;
         rdiag[i] = 1.0 / (double)lu[(int64_t)i * lda + i];
     }
-    __syncthreads();
-    // the pivot sequence composed into one row order: (P v)[i] = v[order[i]]
-    if (threadIdx.x == 0)
-        for (int k = 0; k < n; ++k) {
-            const int q = perm[k];
-            if (q != k) {
-                const int t = order[k];
-                order[k] = order[q];
-                order[q] = t;
-            }
-        }
     __syncthreads();
     int it = 0;
     for (; it <= max_iter; ++it) {
@@ -195,6 +266,10 @@ __global__ __launch_bounds__(kLuBlock) void k_lu_solve_mixed(
     }
 }
 
+// Dynamic LDS a solve may take: the device's per-workgroup limit less a
+// margin for the kernels' static LDS (a few hundred bytes).
+constexpr size_t kLuStaticLds = 1024;
+
 static size_t max_lds(fv_ctx *ctx)
 {
     int v = 0;
@@ -202,37 +277,42 @@ static size_t max_lds(fv_ctx *ctx)
             hipSuccess ||
         v <= 0)
         v = 65536;
-    return (size_t)v;
+    return (size_t)v - kLuStaticLds;
 }
 
-int dense_solve(fv_ctx *ctx, int64_t batch, int64_t n, double *A, double *rhs, int32_t *info)
+int dense_solve(fv_ctx *ctx, int64_t batch, int64_t n, double *A, double *rhs, int32_t *info,
+                const LuReuse *reuse)
 {
+    const LuReuse ru = reuse ? *reuse : LuReuse{nullptr, nullptr, nullptr, nullptr, nullptr};
     if (batch <= 0 || n <= 0) return FV_OK;
     const size_t vec = 2 * lu_align((size_t)n * sizeof(double)) + lu_align((size_t)n * sizeof(int));
-    const LuLds p = lu_lds_plan<double>(n, vec, max_lds(ctx));
+    const LuLds p = lu_lds_plan<double>(n, vec + lu_align((size_t)n * sizeof(int)), vec,
+                                        max_lds(ctx));
     FV_REQUIRE(ctx, p.total <= max_lds(ctx), "dense solve: n = %lld exceeds the LDS panel limit",
                (long long)n);
     TimedLaunch tl(ctx, "dense_solve");
     if (p.in_lds)
         hipLaunchKernelGGL(k_lu_solve_f64<1>, dim3((unsigned)batch), dim3(kLuBlock),
                            (unsigned)p.total, ctx->stream, (int)n, A, rhs, info, p.lda,
-                           (unsigned)p.mat);
+                           (unsigned)p.mat, ru);
     else
         hipLaunchKernelGGL(k_lu_solve_f64<0>, dim3((unsigned)batch), dim3(kLuBlock),
                            (unsigned)p.total, ctx->stream, (int)n, A, rhs, info, p.lda,
-                           (unsigned)p.mat);
+                           (unsigned)p.mat, ru);
     FV_LAUNCH_CHECK(ctx);
     tl.done();
     return FV_OK;
 }
 
 int dense_solve_mixed(fv_ctx *ctx, int64_t batch, int64_t n, const double *A, double *rhs,
-                      int max_iter, double tol, int32_t *info, int32_t *iters)
+                      int max_iter, double tol, int32_t *info, int32_t *iters,
+                      const LuReuse *reuse)
 {
+    const LuReuse ru = reuse ? *reuse : LuReuse{nullptr, nullptr, nullptr, nullptr, nullptr};
     if (batch <= 0 || n <= 0) return FV_OK;
     const size_t vec = 4 * lu_align((size_t)n * sizeof(double)) +
                        2 * lu_align((size_t)n * sizeof(int));
-    const LuLds p = lu_lds_plan<float>(n, vec, max_lds(ctx));
+    const LuLds p = lu_lds_plan<float>(n, vec, vec, max_lds(ctx));
     FV_REQUIRE(ctx, p.total <= max_lds(ctx), "dense solve: n = %lld exceeds the LDS panel limit",
                (long long)n);
     float *f32 = nullptr;
@@ -245,11 +325,11 @@ int dense_solve_mixed(fv_ctx *ctx, int64_t batch, int64_t n, const double *A, do
     if (p.in_lds)
         hipLaunchKernelGGL(k_lu_solve_mixed<1>, dim3((unsigned)batch), dim3(kLuBlock),
                            (unsigned)p.total, ctx->stream, (int)n, A, rhs, f32, max_iter, tol,
-                           info, iters, p.lda, (unsigned)p.mat);
+                           info, iters, p.lda, (unsigned)p.mat, ru);
     else
         hipLaunchKernelGGL(k_lu_solve_mixed<0>, dim3((unsigned)batch), dim3(kLuBlock),
                            (unsigned)p.total, ctx->stream, (int)n, A, rhs, f32, max_iter, tol,
-                           info, iters, p.lda, (unsigned)p.mat);
+                           info, iters, p.lda, (unsigned)p.mat, ru);
     FV_LAUNCH_CHECK(ctx);
     tl.done();
     return FV_OK;
@@ -269,7 +349,7 @@ int fv_dense_solve_dev(fv_ctx *ctx, int64_t batch, int64_t n, double *A, double 
     if (batch == 0 || n == 0) return FV_OK;
     FV_REQUIRE(ctx, A && rhs, "null matrix/rhs");
     FV_REQUIRE(ctx, n <= 1024, "dense solve supports n <= 1024 (got %lld)", (long long)n);
-    return dense_solve(ctx, batch, n, A, rhs, info);
+    return dense_solve(ctx, batch, n, A, rhs, info, nullptr);
 }
 
 int fv_dense_solve_host(fv_ctx *ctx, int64_t batch, int64_t n, double *A, double *rhs,
@@ -304,7 +384,7 @@ int fv_dense_solve_mixed_dev(fv_ctx *ctx, int64_t batch, int64_t n, const double
     if (batch == 0 || n == 0) return FV_OK;
     FV_REQUIRE(ctx, A && rhs, "null matrix/rhs");
     FV_REQUIRE(ctx, n <= 1024, "dense solve supports n <= 1024 (got %lld)", (long long)n);
-    return dense_solve_mixed(ctx, batch, n, A, rhs, max_iter, tol, info, iters);
+    return dense_solve_mixed(ctx, batch, n, A, rhs, max_iter, tol, info, iters, nullptr);
 }
 
 int fv_dense_solve_mixed_host(fv_ctx *ctx, int64_t batch, int64_t n, const double *A,

@@ -21,6 +21,42 @@ namespace fv {
 constexpr int kNb = 16;
 constexpr int kLdp = kNb + 1;   // LDS row stride of a staged panel
 
+// Factor reuse across solves of one system slot (all arrays per batch entry,
+// device memory owned by the caller): the matrix the slot last factored
+// (key, n x n FP64), its factors (fac, n x n of the factor type), the
+// composed row order of its pivots ((P v)[i] = v[order[i]]) and a state word
+// (1: key / fac / order describe a nonsingular factorization).  A solve whose
+// matrix equals key bit for bit skips the factorization and takes fac and
+// order instead; either way the solution comes from the same two
+// substitutions (L y = P b, U x = y) on the same factors, so it is the same
+// bit for bit.  key == nullptr: no reuse.
+struct LuReuse {
+    double *key;
+    void *fac;
+    int32_t *order;
+    int32_t *state;
+    int32_t *hits;   // nullable: solves that took the stored factors
+};
+
+// Bitwise inequality of two doubles (NaN payloads and signed zeros count).
+__device__ __forceinline__ bool bits_differ(double x, double y)
+{
+    return __double_as_longlong(x) != __double_as_longlong(y);
+}
+
+// Final position of row i under the pivot interchanges perm[0..n) applied in
+// order (row k swapped with row perm[k]); order[pivot_track(perm, n, i)] = i
+// composes the sequence into one gather order, one row per lane.
+__device__ __forceinline__ int pivot_track(const int *perm, int n, int i)
+{
+    int p = i;
+    for (int k = 0; k < n; ++k) {
+        const int q = perm[k];
+        p = p == k ? q : (p == q ? k : p);
+    }
+    return p;
+}
+
 __device__ __forceinline__ void wave_sync()
 {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -480,14 +516,14 @@ __device__ __forceinline__ void wave_back_subst(const T *a, int lda, int n, TB *
 }
 
 // y = L^{-1} x in place (unit lower triangle) by wave 0, blocked in registers
-// for n <= 256 as above, else column-oriented in LDS; the row permutation is
-// applied by the caller.
-template <typename T, typename TB>
+// for n <= 256 as above (REGS; about 80 VGPRs), else column-oriented in LDS;
+// the row permutation is applied by the caller.
+template <typename T, typename TB, bool REGS = true>
 __device__ __forceinline__ void wave_fwd_subst(const T *a, int lda, int n, TB *x)
 {
     if ((threadIdx.x >> 6) != 0) return;
     const int lane = threadIdx.x & 63;
-    if (n <= 256) {
+    if (REGS && n <= 256) {
         TB xr[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) xr[q] = (lane + 64 * q < n) ? x[lane + 64 * q] : (TB)0;

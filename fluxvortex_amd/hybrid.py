@@ -145,6 +145,18 @@ class HybridRun:
         self._ctx.check(self._ctx.lib.fv_hybrid_set_precision(self._h,
                                                               _lib.PRECISIONS[precision]))
 
+    def set_solve_reuse(self, on: bool) -> None:
+        """Factor reuse of the circulation solve (on by default): a step whose
+        AIC equals the last factored one bit for bit skips the factorization;
+        the circulation is the same bit for bit."""
+        self._ctx.check(self._ctx.lib.fv_hybrid_set_solve_reuse(self._h, int(bool(on))))
+
+    def solve_reuse_count(self) -> int:
+        """Steps so far whose circulation solve took the stored factors."""
+        v = C.c_int64()
+        self._ctx.check(self._ctx.lib.fv_hybrid_solve_reuse_count(self._h, C.byref(v)))
+        return v.value
+
     def set_population(self, scheme: str = "none", max_particles: int = 60000, eps: float = 1e-3,
                        protect: int = 64, protect_dist: float = 0.0) -> None:
         """Particle population control after every advect

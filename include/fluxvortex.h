@@ -312,6 +312,15 @@ int fv_hybrid_ledger_host(fv_hybrid *h, double *out3);
  * never). */
 int fv_hybrid_set_ring_wake(fv_hybrid *h, int32_t free_wake, double truncate_x);
 int fv_hybrid_set_precision(fv_hybrid *h, int32_t precision);
+/* Factor reuse of the per-step circulation solve (no counterpart in the
+ * reference, which factors every step: standalone_uvlm.py:427): on (the
+ * default), a step whose AIC equals the last factored one bit for bit (a
+ * lattice that does not move) skips the factorization and runs the two
+ * substitutions on the stored factors; the circulation is the same bit for
+ * bit.  0 turns it off (every step factors). */
+int fv_hybrid_set_solve_reuse(fv_hybrid *h, int32_t on);
+/* Steps so far whose circulation solve took the stored factors. */
+int fv_hybrid_solve_reuse_count(fv_hybrid *h, int64_t *reused);
 
 /* ------------------------------------------ learned far-field closure MLP */
 /* 32 -> 128 -> 128 -> 128 -> 16 SiLU network, BF16 weights / activations on
