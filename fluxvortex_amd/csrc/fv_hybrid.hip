@@ -27,7 +27,7 @@
 
 namespace fv {
 int dense_solve(fv_ctx *ctx, int64_t batch, int64_t n, double *A, double *rhs, int32_t *info,
-                const LuReuse *reuse);
+                const LuReuse *reuse, bool store_factors);
 int dense_solve_mixed(fv_ctx *ctx, int64_t batch, int64_t n, const double *A, double *rhs,
                       int max_iter, double tol, int32_t *info, int32_t *iters,
                       const LuReuse *reuse);
@@ -498,7 +498,7 @@ static int hybrid_step_one(fv_hybrid *h)
     const int64_t nrow_old = h->nrow;
     auto solve = [&]() -> int {
         const LuReuse *ru = h->reuse_on ? &h->reuse : nullptr;
-        if (prec == FV_PREC_FP64) return dense_solve(ctx, 1, P, h->aic, h->rhs, h->info, ru);
+        if (prec == FV_PREC_FP64) return dense_solve(ctx, 1, P, h->aic, h->rhs, h->info, ru, false);
         return dense_solve_mixed(ctx, 1, P, h->aic, h->rhs, 8, 1e-12, h->info, nullptr, ru);
     };
     // loads: velocities at leg centres from bound + wake rings (this step's wake)

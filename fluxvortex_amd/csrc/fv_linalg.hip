@@ -57,7 +57,7 @@ __global__ __launch_bounds__(kLuBlock) void k_lu_solve_f64(int n, double *__rest
                                                            double *__restrict__ B,
                                                            int32_t *__restrict__ info,
                                                            int lda_lds, unsigned mat_bytes,
-                                                           LuReuse ru)
+                                                           LuReuse ru, int store_factors)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
     const int64_t nn = (int64_t)n * n;
@@ -75,28 +75,19 @@ __global__ __launch_bounds__(kLuBlock) void k_lu_solve_f64(int n, double *__rest
     double *key = ru.key ? ru.key + (int64_t)blockIdx.x * nn : nullptr;
     double *fac = ru.key ? static_cast<double *>(ru.fac) + (int64_t)blockIdx.x * nn : nullptr;
     int32_t *ord = ru.key ? ru.order + (int64_t)blockIdx.x * n : nullptr;
-    bool differ = !key || ru.state[blockIdx.x] != 1;
-    for (int64_t e = threadIdx.x; e < nn; e += kLuBlock) {
-        const double v = a[e];
-        if (IN_LDS) {
-            const int r = (int)(e / n), c = (int)(e - (int64_t)r * n);
-            w[(int64_t)r * lda + c] = v;
-        }
-        if (key && !differ) differ = bits_differ(v, key[e]);
-    }
-    const bool reuse = !__syncthreads_or(differ ? 1 : 0);
+    const bool check = key && ru.state[blockIdx.x] == 1;
+    const bool differ = mat_move<kLuBlock>(a, n, IN_LDS ? w : (double *)nullptr, lda, n,
+                                           check ? key : (const double *)nullptr);
+    const bool reuse = !__syncthreads_or((!check || differ) ? 1 : 0);
     int inf = 0;
     if (reuse) {
         // the stored factors and row order of this matrix
-        for (int64_t e = threadIdx.x; e < nn; e += kLuBlock) {
-            const int r = (int)(e / n), c = (int)(e - (int64_t)r * n);
-            w[(int64_t)r * lda + c] = fac[e];
-        }
+        mat_move<kLuBlock>(fac, n, w, lda, n, (const double *)nullptr);
         for (int i = threadIdx.x; i < n; i += kLuBlock) order[i] = ord[i];
         if (threadIdx.x == 0 && ru.hits) ru.hits[blockIdx.x] += 1;
     } else {
         if (key) {
-            for (int64_t e = threadIdx.x; e < nn; e += kLuBlock) key[e] = a[e];
+            mat_move<kLuBlock>(a, n, key, n, n, (const double *)nullptr);
             if (threadIdx.x == 0) ru.state[blockIdx.x] = 0;
         }
         inf = lu_blocked<double, double>(w, lda, n, perm, (double *)nullptr,
@@ -104,10 +95,7 @@ __global__ __launch_bounds__(kLuBlock) void k_lu_solve_f64(int n, double *__rest
         // the pivot sequence composed into one row order: (P v)[i] = v[order[i]]
         for (int i = threadIdx.x; i < n; i += kLuBlock) order[pivot_track(perm, n, i)] = i;
         if (key && inf == 0) {
-            for (int64_t e = threadIdx.x; e < nn; e += kLuBlock) {
-                const int r = (int)(e / n), c = (int)(e - (int64_t)r * n);
-                fac[e] = w[(int64_t)r * lda + c];
-            }
+            mat_move<kLuBlock>(w, lda, fac, n, n, (const double *)nullptr);
             __syncthreads();
             for (int i = threadIdx.x; i < n; i += kLuBlock) ord[i] = order[i];
             if (threadIdx.x == 0) ru.state[blockIdx.x] = 1;
@@ -125,11 +113,9 @@ __global__ __launch_bounds__(kLuBlock) void k_lu_solve_f64(int n, double *__rest
         wave_back_subst<double, double>(w, lda, n, b, rdiag);
     }
     __syncthreads();
-    if (IN_LDS)
-        for (int64_t e = threadIdx.x; e < nn; e += kLuBlock) {
-            const int r = (int)(e / n), c = (int)(e - (int64_t)r * n);
-            a[e] = w[(int64_t)r * lda + c];
-        }
+    // A takes its factors (LAPACK's getrf / gesv contract); in global memory
+    // it holds the stored factors after a reuse
+    if (IN_LDS && store_factors) mat_move<kLuBlock>(w, lda, a, n, n, (const double *)nullptr);
     for (int i = threadIdx.x; i < n; i += kLuBlock) bg[i] = b[i];
     if (threadIdx.x == 0 && info) info[blockIdx.x] = inf;
 }
@@ -159,25 +145,17 @@ __global__ __launch_bounds__(kLuBlock) void k_lu_solve_mixed(
     double *key = ru.key ? ru.key + (int64_t)blockIdx.x * nn : nullptr;
     float *fac = ru.key ? static_cast<float *>(ru.fac) + (int64_t)blockIdx.x * nn : nullptr;
     int32_t *ord = ru.key ? ru.order + (int64_t)blockIdx.x * n : nullptr;
-    bool differ = !key || ru.state[blockIdx.x] != 1;
-    for (int64_t e = threadIdx.x; e < nn; e += kLuBlock) {
-        const double v = a[e];
-        const int rr = (int)(e / n), c = (int)(e - (int64_t)rr * n);
-        lu[(int64_t)rr * lda + c] = (float)v;
-        if (key && !differ) differ = bits_differ(v, key[e]);
-    }
-    const bool reuse = !__syncthreads_or(differ ? 1 : 0);
+    const bool check = key && ru.state[blockIdx.x] == 1;
+    const bool differ = mat_move<kLuBlock>(a, n, lu, lda, n, check ? key : (const double *)nullptr);
+    const bool reuse = !__syncthreads_or((!check || differ) ? 1 : 0);
     if (reuse) {
         // the stored FP32 factors and row order of this matrix
-        for (int64_t e = threadIdx.x; e < nn; e += kLuBlock) {
-            const int rr = (int)(e / n), c = (int)(e - (int64_t)rr * n);
-            lu[(int64_t)rr * lda + c] = fac[e];
-        }
+        mat_move<kLuBlock>(fac, n, lu, lda, n, (const double *)nullptr);
         for (int i = threadIdx.x; i < n; i += kLuBlock) order[i] = ord[i];
         if (threadIdx.x == 0 && ru.hits) ru.hits[blockIdx.x] += 1;
     } else {
         if (key) {
-            for (int64_t e = threadIdx.x; e < nn; e += kLuBlock) key[e] = a[e];
+            mat_move<kLuBlock>(a, n, key, n, n, (const double *)nullptr);
             if (threadIdx.x == 0) ru.state[blockIdx.x] = 0;
         }
         __syncthreads();
@@ -193,10 +171,7 @@ __global__ __launch_bounds__(kLuBlock) void k_lu_solve_mixed(
         // the pivot sequence composed into one row order: (P v)[i] = v[order[i]]
         for (int i = threadIdx.x; i < n; i += kLuBlock) order[pivot_track(perm, n, i)] = i;
         if (key) {
-            for (int64_t e = threadIdx.x; e < nn; e += kLuBlock) {
-                const int rr = (int)(e / n), c = (int)(e - (int64_t)rr * n);
-                fac[e] = lu[(int64_t)rr * lda + c];
-            }
+            mat_move<kLuBlock>(lu, lda, fac, n, n, (const double *)nullptr);
             __syncthreads();
             for (int i = threadIdx.x; i < n; i += kLuBlock) ord[i] = order[i];
             if (threadIdx.x == 0) ru.state[blockIdx.x] = 1;
@@ -280,8 +255,10 @@ static size_t max_lds(fv_ctx *ctx)
     return (size_t)v - kLuStaticLds;
 }
 
+// store_factors: A takes its LU factors (else A is left as it was when the
+// matrix is factored in LDS).
 int dense_solve(fv_ctx *ctx, int64_t batch, int64_t n, double *A, double *rhs, int32_t *info,
-                const LuReuse *reuse)
+                const LuReuse *reuse, bool store_factors)
 {
     const LuReuse ru = reuse ? *reuse : LuReuse{nullptr, nullptr, nullptr, nullptr, nullptr};
     if (batch <= 0 || n <= 0) return FV_OK;
@@ -294,11 +271,11 @@ int dense_solve(fv_ctx *ctx, int64_t batch, int64_t n, double *A, double *rhs, i
     if (p.in_lds)
         hipLaunchKernelGGL(k_lu_solve_f64<1>, dim3((unsigned)batch), dim3(kLuBlock),
                            (unsigned)p.total, ctx->stream, (int)n, A, rhs, info, p.lda,
-                           (unsigned)p.mat, ru);
+                           (unsigned)p.mat, ru, store_factors ? 1 : 0);
     else
         hipLaunchKernelGGL(k_lu_solve_f64<0>, dim3((unsigned)batch), dim3(kLuBlock),
                            (unsigned)p.total, ctx->stream, (int)n, A, rhs, info, p.lda,
-                           (unsigned)p.mat, ru);
+                           (unsigned)p.mat, ru, 1);
     FV_LAUNCH_CHECK(ctx);
     tl.done();
     return FV_OK;
@@ -349,7 +326,7 @@ int fv_dense_solve_dev(fv_ctx *ctx, int64_t batch, int64_t n, double *A, double 
     if (batch == 0 || n == 0) return FV_OK;
     FV_REQUIRE(ctx, A && rhs, "null matrix/rhs");
     FV_REQUIRE(ctx, n <= 1024, "dense solve supports n <= 1024 (got %lld)", (long long)n);
-    return dense_solve(ctx, batch, n, A, rhs, info, nullptr);
+    return dense_solve(ctx, batch, n, A, rhs, info, nullptr, true);
 }
 
 int fv_dense_solve_host(fv_ctx *ctx, int64_t batch, int64_t n, double *A, double *rhs,

@@ -44,6 +44,39 @@ __device__ __forceinline__ bool bits_differ(double x, double y)
     return __double_as_longlong(x) != __double_as_longlong(y);
 }
 
+// dst[r ldd + c] = src[r lds + c] over an n x n matrix by a workgroup of NT
+// lanes, kU elements per lane in flight (32-bit indices: n <= 1024); dst
+// nullable (compare only).  Returns whether any moved element differs bit for
+// bit from cmp (dense, row stride n; nullable: false).
+template <int NT, int kU = 8, typename TS, typename TD>
+__device__ __forceinline__ bool mat_move(const TS *__restrict__ src, int lds, TD *__restrict__ dst,
+                                         int ldd, int n, const double *__restrict__ cmp)
+{
+    const int nn = n * n;
+    bool diff = false;
+    for (int e0 = threadIdx.x; e0 < nn; e0 += kU * NT) {
+        TS v[kU];
+        double k[kU];
+        int rr[kU], cc[kU];
+#pragma unroll
+        for (int j = 0; j < kU; ++j) {
+            const int e = e0 + j * NT;
+            rr[j] = e / n;
+            cc[j] = e - rr[j] * n;
+            v[j] = e < nn ? src[rr[j] * lds + cc[j]] : (TS)0;
+            k[j] = (cmp && e < nn) ? cmp[e] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < kU; ++j) {
+            if (e0 + j * NT < nn) {
+                if (dst) dst[rr[j] * ldd + cc[j]] = (TD)v[j];
+                if (cmp) diff |= bits_differ((double)v[j], k[j]);
+            }
+        }
+    }
+    return diff;
+}
+
 // Final position of row i under the pivot interchanges perm[0..n) applied in
 // order (row k swapped with row perm[k]); order[pivot_track(perm, n, i)] = i
 // composes the sequence into one gather order, one row per lane.

@@ -522,14 +522,13 @@ __global__ __launch_bounds__(kSwStepBlock) void k_sw_step(SwInst *__restrict__ i
     // factorization is skipped when the AIC equals the last factored one
     double *lkey = S + I.o_lkey, *lfac = S + I.o_lfac, *lord = S + I.o_lord;
     int *order = perm + P;   // the pivots composed into one gather order (kL_perm holds 2 P ints)
-    bool differ = S[I.o_lst] != 1.0;
-    for (int e = t; e < P * P && !differ; e += kSwStepBlock)
-        differ = bits_differ(A[(e / P) * lda + e % P], lkey[e]);
-    if (!__syncthreads_or(differ ? 1 : 0)) {
-        for (int e = t; e < P * P; e += kSwStepBlock) A[(e / P) * lda + e % P] = lfac[e];
+    const bool check = S[I.o_lst] == 1.0;
+    const bool differ = check && mat_move<kSwStepBlock, 2>(A, lda, (double *)nullptr, 0, P, lkey);
+    if (!__syncthreads_or((!check || differ) ? 1 : 0)) {
+        mat_move<kSwStepBlock, 2>(lfac, P, A, lda, P, (const double *)nullptr);
         for (int i = t; i < P; i += kSwStepBlock) order[i] = (int)lord[i];
     } else {
-        for (int e = t; e < P * P; e += kSwStepBlock) lkey[e] = A[(e / P) * lda + e % P];
+        mat_move<kSwStepBlock, 2>(A, lda, lkey, P, P, (const double *)nullptr);
         if (t == 0) S[I.o_lst] = 0.0;
         // LDS panels: the step kernel's 1024-lane workgroup leaves 128 VGPRs a lane
         const int inf = lu_blocked<double, double, false>(A, lda, P, perm, (double *)nullptr,
@@ -541,7 +540,7 @@ __global__ __launch_bounds__(kSwStepBlock) void k_sw_step(SwInst *__restrict__ i
             return;
         }
         for (int i = t; i < P; i += kSwStepBlock) order[pivot_track(perm, P, i)] = i;
-        for (int e = t; e < P * P; e += kSwStepBlock) lfac[e] = A[(e / P) * lda + e % P];
+        mat_move<kSwStepBlock, 2>(A, lda, lfac, P, P, (const double *)nullptr);
         __syncthreads();
         for (int i = t; i < P; i += kSwStepBlock) lord[i] = (double)order[i];
         if (t == 0) S[I.o_lst] = 1.0;
