@@ -119,6 +119,9 @@ def test_sheet_sweep_large(vk, oracle):
 
 
 @pytest.mark.parametrize("m_rings,n_part,kernel", [(1000, 10_000, "gaussianerf"),
+                                                   (1000, 100_000, "gaussianerf"),
+                                                   (1000, 1_000_000, "gaussianerf"),
+                                                   (10_000, 10_000, "winckelmans"),
                                                    (10_000, 100_000, "winckelmans"),
                                                    (10_000, 1_000_000, "winckelmans")])
 def test_raw_kernel_sweep_full_sizes(vk, oracle, m_rings, n_part, kernel):
