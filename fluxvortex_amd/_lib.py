@@ -145,6 +145,12 @@ _SIGNATURES = {
     "fv_hybrid_set_precision": (C.c_int, [_vp, C.c_int32]),
     "fv_hybrid_set_solve_reuse": (C.c_int, [_vp, C.c_int32]),
     "fv_hybrid_solve_reuse_count": (C.c_int, [_vp, _i64p]),
+    "fv_hybrid_set_graph": (C.c_int, [_vp, C.c_int32]),
+    "fv_hybrid_graph_stats": (C.c_int, [_vp, _i64p, _i64p]),
+    "fv_hybrid_batch_create": (C.c_int, [C.POINTER(_vp), C.c_int32, C.POINTER(_vp)]),
+    "fv_hybrid_batch_step": (C.c_int, [_vp, C.c_int64]),
+    "fv_hybrid_batch_stats": (C.c_int, [_vp, _i64p, _i64p]),
+    "fv_hybrid_batch_destroy": (None, [_vp]),
     "fv_vpm_set_closure": (C.c_int, [_vp, _vp, C.c_double, C.c_double, C.c_int32, C.c_double,
                                      C.c_int32, C.c_int32, C.c_double]),
     "fv_vpm_set_closure_local": (C.c_int, [_vp, C.c_double]),

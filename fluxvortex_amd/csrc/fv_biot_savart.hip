@@ -188,6 +188,10 @@ static int tuned_plan(fv_ctx *ctx, const typename Op::Args &a, int64_t n_tgt, in
                       const char *family, PairPlan &out)
 {
     out = plan_pairs(ctx, n_tgt, n_src);
+    if (ctx->fixed_plans && dyn_src) {
+        out.chunk = 0;   // sized in the kernel from the device count
+        return FV_OK;
+    }
     if (!ctx->autotune) return FV_OK;
     // tuning runs 36 trial sums: beyond 2^36 pairs (~50 ms a trial) the
     // default shape (what the tuner picks at 2^17 x 2^17) is kept

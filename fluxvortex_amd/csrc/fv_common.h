@@ -40,6 +40,10 @@ struct fv_ctx {
     hipEvent_t user_events[8] = {};
     int cus = 0;
     bool autotune = false;                // pair-sum tile shapes timed at first use
+    // particle pair sums with device counts: the default shape for the upper
+    // bound and each source slice's length from the device count (launches
+    // that depend on no host-side count, for graph capture)
+    bool fixed_plans = false;
 };
 
 namespace fv {

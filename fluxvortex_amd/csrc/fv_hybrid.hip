@@ -68,6 +68,7 @@ struct fv_hybrid {
     double *cl = nullptr, *cdi = nullptr, *force = nullptr;
     fv::SegRec *unit_tab = nullptr, *tab = nullptr;
     int32_t *info = nullptr;                           // dense-solve status
+    int64_t *step_dev = nullptr;                       // the step counter kernels read
     // factor reuse of the circulation solve: a step whose AIC equals the last
     // factored one bit for bit (a lattice that does not move) skips the
     // factorization (the solution is the same bit for bit)
@@ -87,6 +88,44 @@ struct fv_hybrid {
     int64_t pop_max = 0, pop_protect = 64;
     double pop_eps = 1e-3, pop_protect_dist = 0.0;
     double seg_pairs = 0.0;
+    // graph mode: launches sized by capacities and device counts, so every
+    // steady step (below) enqueues the same work and replays from a graph
+    // (1), or the same launches run eagerly (2)
+    int graph_mode = 0;
+    bool warm = false;               // a steady step ran eagerly in graph mode
+    fv_hybrid_batch *self = nullptr; // the one-run batch fv_hybrid_step replays through
+};
+
+// Host-side state a step advances (restored after a capture, which runs the
+// host code of one step without executing it).
+struct HybridHost {
+    int64_t step, nrow;
+    int par;
+    double seg_pairs;
+    int64_t np_upper, np_host, state_epoch;
+    bool np_exact;
+};
+
+// Everything the enqueued work of a steady step depends on besides the
+// device state; a change makes the captured graphs stale.
+struct HybridSig {
+    int32_t free_wake, stretch, precision, pop_scheme, ring_free, reuse_on;
+    int32_t kernel, vprec, pair_rule;
+    double trunc_x, rlxf, nu;
+    const void *closure;
+    bool operator==(const HybridSig &o) const { return memcmp(this, &o, sizeof o) == 0; }
+};
+
+struct fv_hybrid_batch {
+    std::vector<fv_hybrid *> runs;
+    hipStream_t master = nullptr;
+    hipEvent_t ev_fork = nullptr;
+    std::vector<hipEvent_t> ev_join;
+    hipGraphExec_t exec[2] = {nullptr, nullptr};   // by wake parity
+    std::vector<HybridHost> before[2], after[2];   // each run's host state around the capture
+    std::vector<HybridSig> sig[2];
+    bool on_master = false;   // the runs' streams wait for the master's graph launches
+    int64_t graph_launches = 0, eager_steps = 0;
 };
 
 namespace fv {
@@ -168,8 +207,22 @@ __device__ void leg_of(const double *Br, const double *Fr, const double *Fl, con
     }
 }
 
-__global__ void k_geometry(const double *__restrict__ gcur, const double *__restrict__ gprev,
-                           int nc, int ns, int64_t poff, double inv_dt,
+// The corner grids of every step (step-major, `stride` doubles a step) and the
+// device step counter: kernels find this step's grid (and its neighbours)
+// themselves, so a step's launches do not change from step to step.
+struct StepGrid {
+    const double *g;
+    int64_t stride;
+    const int64_t *step;
+    // grid of step (current + delta), offset by off doubles; null before step 0
+    __device__ const double *at(int delta, int64_t off) const
+    {
+        const int64_t s = *step + delta;
+        return s < 0 ? nullptr : g + s * stride + off;
+    }
+};
+
+__global__ void k_geometry(StepGrid sg, int64_t voff, int nc, int ns, int64_t poff, double inv_dt,
                            double *__restrict__ br, double *__restrict__ fr,
                            double *__restrict__ fl, double *__restrict__ bl,
                            double *__restrict__ colloc, double *__restrict__ normal,
@@ -179,6 +232,7 @@ __global__ void k_geometry(const double *__restrict__ gcur, const double *__rest
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nc * ns) return;
+    const double *gcur = sg.at(0, 3 * voff), *gprev = sg.at(-1, 3 * voff);
     const int i = t / ns, j = t % ns;
     const int64_t p = poff + t;
     double Br[3], Fr[3], Fl[3], Bl[3], col[3], nrm[3], ar;
@@ -240,13 +294,15 @@ __global__ __launch_bounds__(kLoadBlock) void k_loads(
     int nsurf, const int *__restrict__ surf_nc, const int *__restrict__ surf_ns,
     const int64_t *__restrict__ surf_poff, const int64_t *__restrict__ surf_woff,
     const double *__restrict__ wake_g, int64_t nrow, int64_t P, const double *__restrict__ gam,
-    const double *__restrict__ gam_prev, int first_step, const double *__restrict__ leg_v,
+    const double *__restrict__ gam_prev, const double *__restrict__ leg_v,
     const double *__restrict__ vm_leg, const double *__restrict__ vl,
     const double *__restrict__ normal, const double *__restrict__ area, double vx, double vy,
-    double vz, double rho, double inv_dt, double sref, int64_t step,
+    double vz, double rho, double inv_dt, double sref, const int64_t *__restrict__ step_dev,
     double *__restrict__ panel_force, double *__restrict__ cl, double *__restrict__ cdi)
 {
     __shared__ double red[kLoadBlock / 64][3];
+    const int64_t step = *step_dev;
+    const bool first_step = step == 0;
     double acc[3] = {0.0, 0.0, 0.0};
     for (int64_t p = threadIdx.x; p < P; p += kLoadBlock) {
         int k = 0;
@@ -311,12 +367,13 @@ __global__ void k_copy(const double *__restrict__ a, double *__restrict__ b, int
 
 // Old wake vertex rows 0 .. nr-1 of one surface as points: row 0 = trailing
 // edge of the current geometry, row r = old row r.
-__global__ void k_wake_points(const double *__restrict__ gcur, int nc, int ns,
+__global__ void k_wake_points(StepGrid sg, int64_t voff, int nc, int ns,
                               const double *__restrict__ wold, int64_t nr,
                               double *__restrict__ pts)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nr * (ns + 1)) return;
+    const double *gcur = sg.at(0, 3 * voff);
     const int64_t r = t / (ns + 1);
     const int j = (int)(t % (ns + 1));
     double v[3];
@@ -330,13 +387,14 @@ __global__ void k_wake_points(const double *__restrict__ gcur, int nc, int ns,
 // geometry, row r+1 = (r == 0 ? trailing edge of the current geometry : old
 // row r) + (V + u_r) dt for r < nrow_new, with u_r the induced velocity at
 // that old vertex (vel; nullptr for the prescribed wake, u = 0).
-__global__ void k_wake_vertices(const double *__restrict__ gnext, const double *__restrict__ gcur,
-                                int nc, int ns, const double *__restrict__ wold,
+__global__ void k_wake_vertices(StepGrid sg, int64_t voff, int nc, int ns,
+                                const double *__restrict__ wold,
                                 const double *__restrict__ vel, double *__restrict__ wnew,
                                 int64_t nrow_new, double vx, double vy, double vz, double dt)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (nrow_new + 1) * (ns + 1)) return;
+    const double *gnext = sg.at(1, 3 * voff), *gcur = sg.at(0, 3 * voff);
     const int64_t r = t / (ns + 1);
     const int j = (int)(t % (ns + 1));
     double v[3];
@@ -417,6 +475,9 @@ __global__ void k_wake_corners(const double *__restrict__ w, const double *__res
     if (oa) oa[o] = a ? a[r * ns + j] : 0.0;
 }
 
+// The device step counter moves to the next step (the step's last launch).
+__global__ void k_step_advance(int64_t *__restrict__ step) { *step += 1; }
+
 // Writes converted strengths back into one surface's wake row.
 __global__ void k_scatter_row(const double *__restrict__ src, int ns, int64_t row,
                               double *__restrict__ g)
@@ -432,15 +493,13 @@ static int hybrid_step_one(fv_hybrid *h)
     const int64_t s = h->step;
     const int64_t P = h->P;
     const double dt = d.dt;
-    const double *gcur = h->grids + (size_t)s * h->TV * 3;
-    const double *gprev = s > 0 ? h->grids + (size_t)(s - 1) * h->TV * 3 : nullptr;
+    const StepGrid sg{h->grids, h->TV * 3, h->step_dev};
     int rc;
     // geometry
     for (auto &sf : h->surf) {
         const int np = sf.nc * sf.ns;
         hipLaunchKernelGGL(k_geometry, dim3((unsigned)ceil_div(np, 128)), dim3(128), 0,
-                           ctx->stream, gcur + 3 * sf.vert_off,
-                           gprev ? gprev + 3 * sf.vert_off : nullptr, sf.nc, sf.ns, sf.panel_off,
+                           ctx->stream, sg, sf.vert_off, sf.nc, sf.ns, sf.panel_off,
                            1.0 / dt, h->br, h->fr, h->fl, h->bl, h->colloc, h->normal, h->area,
                            h->vm_col, h->leg_c, h->leg_v, h->vm_leg);
         FV_LAUNCH_CHECK(ctx);
@@ -514,9 +573,9 @@ static int hybrid_step_one(fv_hybrid *h)
         h->seg_pairs += 4.0 * P * (4.0 * P + 4.0 * W);
         hipLaunchKernelGGL(k_loads, dim3(1), dim3(kLoadBlock), 0, ctx->stream,
                            (int)h->surf.size(), h->d_nc, h->d_ns, h->d_poff, h->d_woff, h->wg[cur],
-                           nrow_old, P, h->gam, h->gam_prev, s == 0 ? 1 : 0, h->leg_v, h->vm_leg,
+                           nrow_old, P, h->gam, h->gam_prev, h->leg_v, h->vm_leg,
                            h->vl, h->normal, h->area, d.vinf[0], d.vinf[1], d.vinf[2], d.rho,
-                           1.0 / dt, d.sref, s, h->force, h->cl, h->cdi);
+                           1.0 / dt, d.sref, h->step_dev, h->force, h->cl, h->cdi);
         FV_LAUNCH_CHECK(ctx);
         hipLaunchKernelGGL(k_copy, dim3((unsigned)ceil_div(P, 128)), dim3(128), 0, ctx->stream,
                            h->gam, h->gam_prev, P);
@@ -562,7 +621,6 @@ static int hybrid_step_one(fv_hybrid *h)
     // wake shedding + conversion (not after the last step)
     int nxt = cur;
     if (shed) {
-        const double *gnext = h->grids + (size_t)(s + 1) * h->TV * 3;
         nxt = 1 - cur;
         const int64_t nrow_new = (h->nrow + 1 < h->R) ? h->nrow + 1 : h->R;
         if (h->ring_free) {
@@ -571,7 +629,7 @@ static int hybrid_step_one(fv_hybrid *h)
             for (auto &sf : h->surf) {
                 const int64_t np_ = nrow_new * (sf.ns + 1);
                 hipLaunchKernelGGL(k_wake_points, dim3((unsigned)ceil_div(np_, 128)), dim3(128), 0,
-                                   ctx->stream, gcur + 3 * sf.vert_off, sf.nc, sf.ns,
+                                   ctx->stream, sg, sf.vert_off, sf.nc, sf.ns,
                                    h->wv[cur] + 3 * sf.wvert_off, nrow_new,
                                    h->wpts + 3 * sf.wvert_off);
                 FV_LAUNCH_CHECK(ctx);
@@ -584,8 +642,7 @@ static int hybrid_step_one(fv_hybrid *h)
         for (auto &sf : h->surf) {
             hipLaunchKernelGGL(k_wake_vertices,
                                dim3((unsigned)ceil_div((nrow_new + 1) * (sf.ns + 1), 128)),
-                               dim3(128), 0, ctx->stream, gnext + 3 * sf.vert_off,
-                               gcur + 3 * sf.vert_off, sf.nc, sf.ns,
+                               dim3(128), 0, ctx->stream, sg, sf.vert_off, sf.nc, sf.ns,
                                h->wv[cur] + 3 * sf.wvert_off,
                                h->ring_free ? h->wvel + 3 * sf.wvert_off : nullptr,
                                h->wv[nxt] + 3 * sf.wvert_off, nrow_new, d.vinf[0], d.vinf[1],
@@ -646,7 +703,239 @@ static int hybrid_step_one(fv_hybrid *h)
         rc = wake_rows(nxt, 0, 1);
         if (rc) return rc;
     }
+    hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(1), 0, ctx->stream, h->step_dev);
+    FV_LAUNCH_CHECK(ctx);
     h->step = s + 1;
+    return FV_OK;
+}
+
+// ---------------------------------------------------------------- graphs
+// A steady step enqueues exactly the work of the step before it: graph mode
+// (launches sized by capacities, device counts), the ring wake at its full
+// n_keep + 1 rows, a conversion every step, particles present, a step that
+// sheds (not the last), and no closure or population control (whose sizes
+// are decided on the host).  Steady steps alternate between the two wake
+// buffers, so two graphs (one per parity) replay them.
+static bool hybrid_steady(const fv_hybrid *h)
+{
+    const fv_hybrid_desc &d = h->d;
+    return h->graph_mode == 1 && h->nrow == h->R && d.n_keep < h->R && h->vpm->np_upper > 0 &&
+           h->step >= 1 && h->step + 1 < d.num_steps && !h->vpm->closure && h->pop_scheme == 0;
+}
+
+static HybridHost host_of(const fv_hybrid *h)
+{
+    const fv_vpm *f = h->vpm;
+    return HybridHost{h->step, h->nrow, h->par, h->seg_pairs, f->np_upper, f->np_host,
+                      f->state_epoch, f->np_exact};
+}
+
+static void set_host(fv_hybrid *h, const HybridHost &x)
+{
+    fv_vpm *f = h->vpm;
+    h->step = x.step;
+    h->nrow = x.nrow;
+    h->par = x.par;
+    h->seg_pairs = x.seg_pairs;
+    f->np_upper = x.np_upper;
+    f->np_host = x.np_host;
+    f->state_epoch = x.state_epoch;
+    f->np_exact = x.np_exact;
+}
+
+static HybridSig sig_of(const fv_hybrid *h)
+{
+    HybridSig g;
+    memset(&g, 0, sizeof g);
+    g.free_wake = h->d.free_wake;
+    g.stretch = h->d.stretch;
+    g.precision = h->d.precision;
+    g.pop_scheme = h->pop_scheme;
+    g.ring_free = h->ring_free;
+    g.reuse_on = h->reuse_on;
+    g.kernel = h->vpm->kernel;
+    g.vprec = h->vpm->precision;
+    g.pair_rule = h->vpm->pair_rule;
+    g.trunc_x = h->trunc_x;
+    g.rlxf = h->vpm->rlxf;
+    g.nu = h->vpm->nu;
+    g.closure = h->vpm->closure;
+    return g;
+}
+
+static void batch_drop_graphs(fv_hybrid_batch *b)
+{
+    for (int p = 0; p < 2; ++p)
+        if (b->exec[p]) {
+            (void)hipGraphExecDestroy(b->exec[p]);
+            b->exec[p] = nullptr;
+        }
+}
+
+// Captures one step of every run into the graph of wake parity par: each
+// run's stream forks from the master stream, enqueues its step and joins
+// back.  The host code of the step runs once; its host-side effects are
+// recorded (after) and undone (before), and every replay applies them again.
+static int batch_capture(fv_hybrid_batch *b, int par)
+{
+    const size_t n = b->runs.size();
+    fv_ctx *ctx0 = b->runs[0]->ctx;
+    b->before[par].resize(n);
+    b->after[par].resize(n);
+    b->sig[par].resize(n);
+    std::vector<char> timing(n);
+    for (size_t i = 0; i < n; ++i) {
+        fv_hybrid *h = b->runs[i];
+        b->before[par][i] = host_of(h);
+        b->sig[par][i] = sig_of(h);
+        timing[i] = h->ctx->timing;
+        h->ctx->timing = false;   // no event records inside the graph
+    }
+    hipGraph_t g = nullptr;
+    int rc = FV_OK;
+    hipError_t e = hipStreamBeginCapture(b->master, hipStreamCaptureModeThreadLocal);
+    const bool began = e == hipSuccess;
+    if (began) e = hipEventRecord(b->ev_fork, b->master);
+    for (size_t i = 0; i < n && e == hipSuccess && rc == FV_OK; ++i) {
+        fv_hybrid *h = b->runs[i];
+        e = hipStreamWaitEvent(h->ctx->stream, b->ev_fork, 0);
+        if (e == hipSuccess) rc = hybrid_step_one(h);
+        if (e == hipSuccess && rc == FV_OK) e = hipEventRecord(b->ev_join[i], h->ctx->stream);
+        if (e == hipSuccess && rc == FV_OK) e = hipStreamWaitEvent(b->master, b->ev_join[i], 0);
+    }
+    const hipError_t e_end = began ? hipStreamEndCapture(b->master, &g) : hipSuccess;
+    for (size_t i = 0; i < n; ++i) {
+        fv_hybrid *h = b->runs[i];
+        b->after[par][i] = host_of(h);
+        set_host(h, b->before[par][i]);
+        h->ctx->timing = timing[i];
+    }
+    if (rc == FV_OK && (e != hipSuccess || e_end != hipSuccess))
+        rc = set_error(ctx0, FV_E_HIP, "capturing the hybrid step failed: %s",
+                       hipGetErrorString(e != hipSuccess ? e : e_end));
+    if (rc == FV_OK) {
+        e = hipGraphInstantiate(&b->exec[par], g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            b->exec[par] = nullptr;
+            rc = set_error(ctx0, FV_E_HIP, "hipGraphInstantiate of the hybrid step failed: %s",
+                           hipGetErrorString(e));
+        }
+    }
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    return rc;
+}
+
+// The master stream's graph launches follow the runs' queued work (join) and
+// precede what the runs enqueue afterwards (release).
+static int batch_join(fv_hybrid_batch *b)
+{
+    if (b->on_master) return FV_OK;
+    fv_ctx *ctx0 = b->runs[0]->ctx;
+    for (size_t i = 0; i < b->runs.size(); ++i) {
+        FV_HIP_CHECK(ctx0, hipEventRecord(b->ev_join[i], b->runs[i]->ctx->stream));
+        FV_HIP_CHECK(ctx0, hipStreamWaitEvent(b->master, b->ev_join[i], 0));
+    }
+    b->on_master = true;
+    return FV_OK;
+}
+
+static int batch_release(fv_hybrid_batch *b)
+{
+    if (!b->on_master) return FV_OK;
+    fv_ctx *ctx0 = b->runs[0]->ctx;
+    FV_HIP_CHECK(ctx0, hipEventRecord(b->ev_fork, b->master));
+    for (fv_hybrid *h : b->runs) FV_HIP_CHECK(ctx0, hipStreamWaitEvent(h->ctx->stream, b->ev_fork, 0));
+    b->on_master = false;
+    return FV_OK;
+}
+
+static int batch_step(fv_hybrid_batch *b, int64_t nsteps)
+{
+    for (int64_t k = 0; k < nsteps; ++k) {
+        const int par = b->runs[0]->par;
+        bool ready = true;
+        for (fv_hybrid *h : b->runs) ready = ready && hybrid_steady(h) && h->warm && h->par == par;
+        if (ready && b->exec[par]) {
+            for (size_t i = 0; i < b->runs.size() && ready; ++i)
+                ready = sig_of(b->runs[i]) == b->sig[par][i];
+            if (!ready) {
+                // a setting changed: the graphs are stale, and the next eager
+                // steady step sizes the buffers again
+                batch_drop_graphs(b);
+                for (fv_hybrid *h : b->runs) h->warm = false;
+            }
+        }
+        int rc = FV_OK;
+        if (ready) {
+            if (!b->exec[par]) {
+                rc = batch_capture(b, par);
+                if (rc) return rc;
+            }
+            rc = batch_join(b);
+            if (rc) return rc;
+            FV_HIP_CHECK(b->runs[0]->ctx, hipGraphLaunch(b->exec[par], b->master));
+            for (size_t i = 0; i < b->runs.size(); ++i) {
+                fv_hybrid *h = b->runs[i];
+                const HybridHost &x0 = b->before[par][i], &x1 = b->after[par][i];
+                HybridHost x = host_of(h);
+                x.step += x1.step - x0.step;
+                x.nrow = x1.nrow;
+                x.par = x1.par;
+                x.seg_pairs += x1.seg_pairs - x0.seg_pairs;
+                x.np_upper += x1.np_upper - x0.np_upper;
+                if (x.np_upper > h->vpm->cap) x.np_upper = h->vpm->cap;
+                x.state_epoch += x1.state_epoch - x0.state_epoch;
+                x.np_exact = x1.np_exact;
+                set_host(h, x);
+            }
+            ++b->graph_launches;
+        } else {
+            rc = batch_release(b);
+            if (rc) return rc;
+            for (fv_hybrid *h : b->runs) {
+                const bool steady = hybrid_steady(h);
+                rc = hybrid_step_one(h);
+                if (rc) return rc;
+                // an eager steady step has sized every buffer a capture uses
+                h->warm = steady;
+            }
+            ++b->eager_steps;
+        }
+    }
+    return batch_release(b);
+}
+
+static void batch_free(fv_hybrid_batch *b)
+{
+    if (!b) return;
+    if (b->master) (void)hipStreamSynchronize(b->master);
+    batch_drop_graphs(b);
+    if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+    for (hipEvent_t e : b->ev_join)
+        if (e) (void)hipEventDestroy(e);
+    if (b->master) (void)hipStreamDestroy(b->master);
+    delete b;
+}
+
+static int batch_make(fv_hybrid *const *runs, int32_t n, fv_hybrid_batch **out)
+{
+    *out = nullptr;
+    fv_ctx *ctx0 = runs[0]->ctx;
+    fv_hybrid_batch *b = new (std::nothrow) fv_hybrid_batch();
+    if (!b) return set_error(ctx0, FV_E_HIP, "out of host memory");
+    b->runs.assign(runs, runs + n);
+    b->ev_join.assign((size_t)n, nullptr);
+    hipError_t e = hipSetDevice(ctx0->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->master, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming);
+    for (int32_t i = 0; i < n && e == hipSuccess; ++i)
+        e = hipEventCreateWithFlags(&b->ev_join[i], hipEventDisableTiming);
+    if (e != hipSuccess) {
+        batch_free(b);
+        return set_error(ctx0, FV_E_HIP, "hybrid batch set-up failed: %s", hipGetErrorString(e));
+    }
+    *out = b;
     return FV_OK;
 }
 
@@ -724,7 +1013,7 @@ int fv_hybrid_create(fv_ctx *ctx, const fv_hybrid_desc *desc, fv_hybrid **out)
     n += 2 * (size_t)d.num_steps;                    // cl, cdi
     const size_t tab_bytes = sizeof(SegRec) * (size_t)(4 * P + 4 * WR + 4 * P);
     const size_t meta_bytes =
-        sizeof(int64_t) * 2 * S + sizeof(int32_t) * (1 + 2 * S + 2 + P) + 64;
+        sizeof(int64_t) * (2 * S + 1) + sizeof(int32_t) * (1 + 2 * S + 2 + P) + 64;
     const size_t bytes = n * sizeof(double) + tab_bytes + meta_bytes + 1024;
     hipError_t e = hipMalloc(&h->block, bytes);
     if (e != hipSuccess) {
@@ -761,19 +1050,20 @@ int fv_hybrid_create(fv_ctx *ctx, const fv_hybrid_desc *desc, fv_hybrid **out)
     h->cbr = take(3 * NS); h->cfr = take(3 * NS); h->cfl = take(3 * NS); h->cbl = take(3 * NS);
     h->cg = take(NS);
     h->cl = take(d.num_steps); h->cdi = take(d.num_steps);
-    // metadata: int64 panel offsets [S], wake ring offsets [S], then int32
-    // status, nc [S], ns [S]
+    // metadata: int64 panel offsets [S], wake ring offsets [S], step counter,
+    // then int32 status, nc [S], ns [S], reuse state, reuse count, row order [P]
     int64_t *m64 = reinterpret_cast<int64_t *>(p);
-    int32_t *m32p = reinterpret_cast<int32_t *>(m64 + 2 * S);
+    int32_t *m32p = reinterpret_cast<int32_t *>(m64 + 2 * S + 1);
     h->d_poff = m64;
     h->d_woff = m64 + S;
+    h->step_dev = m64 + 2 * S;
     h->info = m32p;
     h->d_nc = m32p + 1;
     h->d_ns = m32p + 1 + S;
     h->reuse.state = m32p + 1 + 2 * S;
     h->reuse.hits = m32p + 2 + 2 * S;
     h->reuse.order = m32p + 3 + 2 * S;
-    std::vector<int64_t> po(2 * S);
+    std::vector<int64_t> po(2 * S + 1, 0);   // panel offsets, wake offsets, step 0
     std::vector<int32_t> m32(1 + 2 * S + 2, 0);   // status, nc, ns, reuse state + hits
     for (size_t k = 0; k < S; ++k) {
         po[k] = h->surf[k].panel_off;
@@ -781,7 +1071,7 @@ int fv_hybrid_create(fv_ctx *ctx, const fv_hybrid_desc *desc, fv_hybrid **out)
         m32[1 + k] = h->surf[k].nc;
         m32[1 + S + k] = h->surf[k].ns;
     }
-    e = hipMemcpy(m64, po.data(), sizeof(int64_t) * 2 * S, hipMemcpyHostToDevice);
+    e = hipMemcpy(m64, po.data(), sizeof(int64_t) * po.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(m32p, m32.data(), sizeof(int32_t) * m32.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess)
@@ -834,6 +1124,7 @@ void fv_hybrid_destroy(fv_hybrid *h)
     }
     if (h->ev_rhs) (void)hipEventDestroy(h->ev_rhs);
     if (h->ev_solved) (void)hipEventDestroy(h->ev_solved);
+    batch_free(h->self);
     fv_vpm_destroy(h->vpm);
     (void)hipFree(h->block);
     delete h;
@@ -853,12 +1144,86 @@ int fv_hybrid_step(fv_hybrid *h, int64_t nsteps)
     if (ovf)
         return set_error(ctx, FV_E_CAP, "Particle field would overflow (capacity %lld)",
                          (long long)h->vpm->cap);
+    if (h->graph_mode == 1) {
+        if (!h->self) {
+            int rc = batch_make(&h, 1, &h->self);
+            if (rc) return rc;
+        }
+        return batch_step(h->self, nsteps);
+    }
     for (int64_t k = 0; k < nsteps; ++k) {
         int rc = hybrid_step_one(h);
         if (rc) return rc;
     }
     return FV_OK;
 }
+
+int fv_hybrid_set_graph(fv_hybrid *h, int32_t on)
+{
+    if (!h) return FV_E_ARG;
+    fv_ctx *ctx = h->ctx;
+    FV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    batch_free(h->self);
+    h->self = nullptr;
+    FV_REQUIRE(ctx, on >= 0 && on <= 2, "graph mode %d is not 0, 1 or 2", on);
+    h->graph_mode = on;
+    h->warm = false;
+    ctx->fixed_plans = on != 0;
+    h->vpm->bound_cap = on != 0;
+    return FV_OK;
+}
+
+int fv_hybrid_graph_stats(fv_hybrid *h, int64_t *graph_launches, int64_t *eager_steps)
+{
+    if (!h || !graph_launches || !eager_steps) return FV_E_ARG;
+    *graph_launches = h->self ? h->self->graph_launches : 0;
+    *eager_steps = h->self ? h->self->eager_steps : 0;
+    return FV_OK;
+}
+
+int fv_hybrid_batch_create(fv_hybrid *const *runs, int32_t n, fv_hybrid_batch **out)
+{
+    if (!out) return FV_E_ARG;
+    *out = nullptr;
+    if (!runs || n < 1 || !runs[0]) return FV_E_ARG;
+    fv_ctx *ctx0 = runs[0]->ctx;
+    for (int32_t i = 0; i < n; ++i) {
+        FV_REQUIRE(ctx0, runs[i] != nullptr, "run %d is null", i);
+        FV_REQUIRE(ctx0, runs[i]->ctx->device == ctx0->device, "runs on different devices");
+        for (int32_t j = 0; j < i; ++j)
+            FV_REQUIRE(ctx0, runs[i] != runs[j], "run %d appears twice", i);
+    }
+    return batch_make(runs, n, out);
+}
+
+int fv_hybrid_batch_step(fv_hybrid_batch *b, int64_t nsteps)
+{
+    if (!b) return FV_E_ARG;
+    fv_ctx *ctx0 = b->runs[0]->ctx;
+    FV_REQUIRE(ctx0, nsteps >= 0, "negative step count");
+    for (fv_hybrid *h : b->runs) {
+        FV_REQUIRE(ctx0, h->step + nsteps <= h->d.num_steps, "a run has only %lld steps left",
+                   (long long)(h->d.num_steps - h->step));
+        int32_t ovf = 0;
+        FV_HIP_CHECK(ctx0, hipMemcpyAsync(&ovf, h->vpm->overflow_dev, sizeof ovf,
+                                          hipMemcpyDeviceToHost, h->ctx->stream));
+        FV_HIP_CHECK(ctx0, hipStreamSynchronize(h->ctx->stream));
+        if (ovf)
+            return set_error(ctx0, FV_E_CAP, "Particle field would overflow (capacity %lld)",
+                             (long long)h->vpm->cap);
+    }
+    return batch_step(b, nsteps);
+}
+
+int fv_hybrid_batch_stats(fv_hybrid_batch *b, int64_t *graph_launches, int64_t *eager_steps)
+{
+    if (!b || !graph_launches || !eager_steps) return FV_E_ARG;
+    *graph_launches = b->graph_launches;
+    *eager_steps = b->eager_steps;
+    return FV_OK;
+}
+
+void fv_hybrid_batch_destroy(fv_hybrid_batch *b) { batch_free(b); }
 
 int fv_hybrid_current_step(fv_hybrid *h, int64_t *step)
 {

@@ -636,6 +636,8 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(typename Op::Args a, int64_t n_
     const int64_t n_src = dyn_src ? *dyn_src : n_src_arg;
     const int64_t i0 = (int64_t)blockIdx.x * (BLOCK * TPT);
     if (i0 >= n_tgt) return;   // uniform over the block
+    // chunk <= 0: the slices share the device count evenly (whole tiles)
+    if (chunk <= 0) chunk = ((n_src + gridDim.y - 1) / gridDim.y + BLOCK - 1) / BLOCK * BLOCK;
     typename Op::Tgt t[TPT];
     typename Op::Acc acc[TPT];
 #pragma unroll

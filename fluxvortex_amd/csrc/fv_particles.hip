@@ -290,9 +290,10 @@ int vpm_induce(fv_vpm *f, const double *targets, int64_t n, int precision, doubl
     t.x = targets; t.y = targets + 1; t.z = targets + 2;
     t.ax = t.ay = t.az = nullptr;
     t.stride = 3;
-    int rc = part_sum(f->ctx, f->kernel, precision, 0, field_src(f, f->kernel), t, n,
-                      f->np_upper, nullptr, f->np_dev, out, pslot, "particle_induce");
-    if (rc || n <= 0 || f->np_upper <= 0) return rc;
+    const int64_t nb = launch_count(f);
+    int rc = part_sum(f->ctx, f->kernel, precision, 0, field_src(f, f->kernel), t, n, nb,
+                      nullptr, f->np_dev, out, pslot, "particle_induce");
+    if (rc || n <= 0 || nb <= 0) return rc;
     hipLaunchKernelGGL(k_count_induce, dim3(1), dim3(1), 0, f->ctx->stream, f->np_dev,
                        (double)n, f->pairs_dev);
     FV_LAUNCH_CHECK(f->ctx);
@@ -302,7 +303,7 @@ int vpm_induce(fv_vpm *f, const double *targets, int64_t n, int precision, doubl
 int vpm_advect(fv_vpm *f, double dt, const double *uinf, int stretch, int free_wake)
 {
     fv_ctx *ctx = f->ctx;
-    const int64_t nu = f->np_upper;
+    const int64_t nu = launch_count(f);
     if (nu <= 0) return FV_OK;
     static const double ra[3] = {0.0, -5.0 / 9.0, -153.0 / 128.0};   // particles.py:114
     static const double rb[3] = {1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0};

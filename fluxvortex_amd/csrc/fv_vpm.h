@@ -32,11 +32,21 @@ struct fv_vpm {
     double *pairs_dev = nullptr;   // [0] direct self-sum pairs, [1] induced-target pairs,
                                    // counted on the device from the exact particle count
     int64_t state_epoch = 0;       // bumped by every enqueued change of the particle state
+    // launches sized by the capacity instead of np_upper (the device count
+    // bounds the work): every launch is the same from step to step
+    bool bound_cap = false;
     fv_closure *closure = nullptr; // far-field closure (optional)
     void *pop = nullptr;           // population-control scratch (fv_population.hip), lazily made
 };
 
 namespace fv {
+
+// The particle count that sizes launches (np_upper, or the capacity with
+// bound_cap; 0 while the field is empty).
+inline int64_t launch_count(const fv_vpm *f)
+{
+    return f->np_upper <= 0 ? 0 : (f->bound_cap ? f->cap : f->np_upper);
+}
 
 // Enqueues one advect_rk3 + relaxation step (particles.py:85-246).
 int vpm_advect(fv_vpm *f, double dt, const double *uinf, int stretch, int free_wake);

@@ -99,9 +99,12 @@ class HybridCase:
 class HybridRun:
     """A device-resident hybrid case (fv_hybrid)."""
 
-    def __init__(self, case: HybridCase, context=None):
+    def __init__(self, case: HybridCase, context=None, graph=False):
         self.case = case
-        self._ctx = context or _lib.default_context()
+        # graph mode changes its context's pair-sum plans: it takes a context of its own
+        self._own_ctx = context is None and graph
+        self._ctx = context or (_lib.Context(_lib.default_context().device) if graph
+                                else _lib.default_context())
         d, self._keep = case.desc()
         h = C.c_void_p()
         self._ctx.check(self._ctx.lib.fv_hybrid_create(self._ctx.handle, C.byref(d),
@@ -114,6 +117,8 @@ class HybridRun:
                                              _handle=vh, _owner=self)
         if case.ring_free_wake or math.isfinite(case.truncate_x):
             self.set_ring_wake(case.ring_free_wake, case.truncate_x)
+        if graph:
+            self.set_graph(graph)
 
     def step(self, n: int = 1) -> None:
         rc = self._ctx.lib.fv_hybrid_step(self._h, int(n))
@@ -138,6 +143,18 @@ class HybridRun:
         self._ctx.check(rc)
         return cl, cdi, gam
 
+    def outputs(self, particles: bool = False) -> dict:
+        """CL / CDi histories, circulations, particle count and pair counts
+        (and the particle state with `particles`)."""
+        cl, cdi, gam = self.read()
+        out = {"cl": cl, "cdi": cdi, "gamma": gam, "np": self.particles.np}
+        out["seg_pairs"], out["particle_pairs"] = self.pair_counts()
+        if particles:
+            st = self.particles.state_dict()
+            out.update({"pos": st["pos"], "alpha": st["gamma"], "sigma_p": st["sigma"],
+                        "age": st["age"]})
+        return out
+
     def set_flags(self, free_wake: bool, stretch: bool) -> None:
         self._ctx.check(self._ctx.lib.fv_hybrid_set_flags(self._h, int(free_wake), int(stretch)))
 
@@ -156,6 +173,18 @@ class HybridRun:
         v = C.c_int64()
         self._ctx.check(self._ctx.lib.fv_hybrid_solve_reuse_count(self._h, C.byref(v)))
         return v.value
+
+    def set_graph(self, on) -> None:
+        """Graph mode: launches sized by capacities and device counts, steady
+        steps replayed from hipGraphs (one graph launch per step); 2: the same
+        launches, every step eager; False / 0: off (fv_hybrid_set_graph)."""
+        self._ctx.check(self._ctx.lib.fv_hybrid_set_graph(self._h, int(on)))
+
+    def graph_stats(self) -> tuple[int, int]:
+        """(steps replayed from graphs, steps run eagerly) in graph mode."""
+        a, b = C.c_int64(), C.c_int64()
+        self._ctx.check(self._ctx.lib.fv_hybrid_graph_stats(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def set_population(self, scheme: str = "none", max_particles: int = 60000, eps: float = 1e-3,
                        protect: int = 64, protect_dist: float = 0.0) -> None:
@@ -191,6 +220,9 @@ class HybridRun:
         if getattr(self, "_h", None) is not None and self._h.value:
             self._ctx.lib.fv_hybrid_destroy(self._h)
             self._h = C.c_void_p()
+        if getattr(self, "_own_ctx", False):
+            self._ctx.close()
+            self._own_ctx = False
 
     def __del__(self):
         try:
@@ -199,19 +231,78 @@ class HybridRun:
             pass
 
 
-def run_case(case: HybridCase, steps: int | None = None, particles: bool = False) -> dict:
-    """Runs `steps` (default all) steps on the device and returns the histories."""
-    run = HybridRun(case)
+class HybridBatch:
+    """Independent hybrid runs advanced together (fv_hybrid_batch): once every
+    run is steady, one hipGraph launch per time step advances all of them.
+    Each run gets a context (stream) of its own, in graph mode by default."""
+
+    def __init__(self, cases, graph: bool = True):
+        self.runs = [HybridRun(c, graph=graph) for c in cases]
+        lib = self.runs[0]._ctx.lib
+        arr = (C.c_void_p * len(self.runs))(*[r._h.value for r in self.runs])
+        h = C.c_void_p()
+        self.runs[0]._ctx.check(lib.fv_hybrid_batch_create(arr, len(self.runs), C.byref(h)))
+        self._h = h
+        self._lib = lib
+
+    def step(self, n: int = 1) -> None:
+        rc = self._lib.fv_hybrid_batch_step(self._h, int(n))
+        if rc == _lib.FV_E_CAP:
+            raise RuntimeError(self._lib.fv_last_error(self.runs[0]._ctx.handle).decode())
+        self.runs[0]._ctx.check(rc)
+
+    def stats(self) -> tuple[int, int]:
+        """(graph launches, eager batch steps) so far."""
+        a, b = C.c_int64(), C.c_int64()
+        self.runs[0]._ctx.check(self._lib.fv_hybrid_batch_stats(self._h, C.byref(a),
+                                                                C.byref(b)))
+        return a.value, b.value
+
+    def read(self):
+        return [r.read() for r in self.runs]
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.fv_hybrid_batch_destroy(self._h)
+            self._h = C.c_void_p()
+        for r in getattr(self, "runs", []):
+            r.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def plunging_sweep_cases(ks=(0.5, 0.2, 0.1), n_keeps=(10, 20, 0), **kw) -> list:
+    """The reference's reduced-frequency sweep (tests/experiment_hybrid_panel_
+    particle.py:206-329): the heaving wing at each reduced frequency k with
+    each N_keep (its hybrid runs; N_keep = 0 converts every row), free particle
+    wake, 3 cycles of 50 steps.  Every case has the same step count; the
+    particle counts differ by N_keep."""
+    kw.setdefault("free_wake", True)
+    kw.setdefault("stretch", False)
+    kw.setdefault("kernel", "gaussianerf")
+    kw.setdefault("sigma_factor", 1.0)
+    kw.setdefault("max_particles", 8000)
+    out = []
+    for k in ks:
+        for nk in n_keeps:
+            c = heaving_wing_case(k=k, n_keep=nk, **kw)
+            c.name = f"plunge_k{k}_n{nk}"
+            out.append(c)
+    return out
+
+
+def run_case(case: HybridCase, steps: int | None = None, particles: bool = False,
+             graph=False) -> dict:
+    """Runs `steps` (default all) steps on the device and returns the histories
+    (graph: HybridRun's graph mode)."""
+    run = HybridRun(case, graph=graph)
     try:
         run.step(case.num_steps if steps is None else steps)
-        cl, cdi, gam = run.read()
-        out = {"cl": cl, "cdi": cdi, "gamma": gam, "np": run.particles.np}
-        out["seg_pairs"], out["particle_pairs"] = run.pair_counts()
-        if particles:
-            st = run.particles.state_dict()
-            out.update({"pos": st["pos"], "alpha": st["gamma"], "sigma_p": st["sigma"],
-                        "age": st["age"]})
-        return out
+        return run.outputs(particles)
     finally:
         run.close()
 

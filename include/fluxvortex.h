@@ -58,6 +58,7 @@ extern "C" {
 typedef struct fv_ctx fv_ctx;
 typedef struct fv_vpm fv_vpm;
 typedef struct fv_hybrid fv_hybrid;
+typedef struct fv_hybrid_batch fv_hybrid_batch;
 
 /* ---------------------------------------------------------------- context */
 /* No reference counterpart (Warp keeps an implicit device context,
@@ -321,6 +322,41 @@ int fv_hybrid_set_precision(fv_hybrid *h, int32_t precision);
 int fv_hybrid_set_solve_reuse(fv_hybrid *h, int32_t on);
 /* Steps so far whose circulation solve took the stored factors. */
 int fv_hybrid_solve_reuse_count(fv_hybrid *h, int64_t *reused);
+
+/* Graph mode (no counterpart in the reference, whose step is a Python loop,
+ * tests/experiment_hybrid_panel_particle.py:125-143; for callers that want one
+ * launch per time step).  on: every launch of a step is sized by capacities
+ * (the particle field's max_particles) and the kernels read the particle
+ * count and the step index from device memory, so a steady step -- ring wake
+ * at n_keep + 1 rows, a conversion every step, particles present, not the
+ * last step, no closure and no population control -- enqueues the same work
+ * as the one before; fv_hybrid_step then runs one steady step eagerly (which
+ * sizes every buffer), captures the next one of each wake parity into a
+ * hipGraph and replays it (one graph launch per step).  Other steps run
+ * eagerly.  A change of flags, precision, ring wake, reuse or the particle
+ * field's settings makes the graphs stale; they are captured again.  Graph
+ * mode also sets the context's pair-sum plans to their capacity-sized form
+ * (fixed shapes, source slices sized on the device), which moves particle
+ * sums by round-off against the default plans: graph replays equal eager
+ * graph-mode steps bit for bit (on = 2: graph-mode launches, every step
+ * eager; 0: off).  Use a context of its own per run. */
+int fv_hybrid_set_graph(fv_hybrid *h, int32_t on);
+/* Steps fv_hybrid_step replayed from graphs / ran eagerly in graph mode. */
+int fv_hybrid_graph_stats(fv_hybrid *h, int64_t *graph_launches, int64_t *eager_steps);
+
+/* Batched runs (a reduced-frequency or configuration sweep; no counterpart in
+ * the reference, which runs its cases one after another,
+ * tests/experiment_hybrid_panel_particle.py:206-329): n independent runs with
+ * uneven panel, wake and particle counts advance together.  Steps where every
+ * run is steady and warm (above) replay ONE graph holding a step of every
+ * run (each run's work on its own context stream inside the graph), one graph
+ * launch per time step for the whole batch; other steps run each run's step
+ * eagerly on its own stream.  The runs keep their own state and readers
+ * (fv_hybrid_read ...); destroy the batch before its runs. */
+int fv_hybrid_batch_create(fv_hybrid *const *runs, int32_t n, fv_hybrid_batch **out);
+int fv_hybrid_batch_step(fv_hybrid_batch *b, int64_t nsteps);
+int fv_hybrid_batch_stats(fv_hybrid_batch *b, int64_t *graph_launches, int64_t *eager_steps);
+void fv_hybrid_batch_destroy(fv_hybrid_batch *b);
 
 /* ------------------------------------------ learned far-field closure MLP */
 /* 32 -> 128 -> 128 -> 128 -> 16 SiLU network, BF16 weights / activations on

@@ -40,7 +40,7 @@ def response(ref, runs, key):
     return out
 
 
-def compare(hyb, oracle, case, cl_tol=1e-10, part=True, flat=None):
+def compare(hyb, oracle, case, cl_tol=1e-10, part=True, flat=None, dev=None):
     """Device vs oracle.  Flat (no relaxation, no stretching: the default
     there): the CL/CDi histories agree within cl_tol at every step, the final
     circulations within 1e-10, particle positions within 1e-10 and strengths
@@ -57,7 +57,8 @@ def compare(hyb, oracle, case, cl_tol=1e-10, part=True, flat=None):
     if flat is None:
         flat = (case.rlxf == 0.0 or case.n_keep >= case.num_steps) and not case.stretch
     print(f"[hybrid parity] {case.name} steps={case.num_steps} flat={flat}", flush=True)
-    dev = hyb.run_case(case, particles=part)
+    if dev is None:
+        dev = hyb.run_case(case, particles=part)
     ref = oracle.hybrid_run(case)
     dcl = np.abs(dev["cl"] - ref["cl"])
     dcdi = np.abs(dev["cdi"] - ref["cdi"])
