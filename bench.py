@@ -670,6 +670,39 @@ def flutter_block(n_speeds=64, cpu_sample=4):
     return res
 
 
+def plunging_sweep_block():
+    """The reference's reduced-frequency sweep (k = 0.5 / 0.2 / 0.1 x N_keep =
+    10 / 20 / 0, free particle wake, 150 steps each): the nine runs one after
+    another (eager, the default shapes), then as one batch in graph mode (one
+    graph launch per time step once every run is steady).  Reports both wall
+    times and every run's CL amplitude over Theodorsen's."""
+    from fluxvortex_amd import hybrid
+    cases = hybrid.plunging_sweep_cases(ks=(0.5, 0.2, 0.1), n_keeps=(10, 20, 0))
+    steps = cases[0].num_steps
+    hybrid.run_case(cases[0], steps=12)   # loads the kernels
+    t0 = time.perf_counter()
+    seq = [hybrid.run_case(c) for c in cases]
+    t_seq = time.perf_counter() - t0
+    b = hybrid.HybridBatch(cases)
+    try:
+        t0 = time.perf_counter()
+        b.step(steps)
+        outs = [r.outputs() for r in b.runs]
+        t_bat = time.perf_counter() - t0
+        launches, eager = b.stats()
+    finally:
+        b.close()
+    runs = []
+    for c, o, q in zip(cases, outs, seq):
+        runs.append({"k": c.meta["k"], "n_keep": c.n_keep, "particles": o["np"],
+                     "theodorsen_ratio": hybrid.theodorsen_ratio(c, o["cl"]),
+                     "theodorsen_ratio_eager_default": hybrid.theodorsen_ratio(c, q["cl"])})
+    return {"runs": len(cases), "steps": steps, "sequential_eager_s": t_seq,
+            "batched_graph_s": t_bat, "speedup": t_seq / t_bat,
+            "graph_launches": launches, "eager_batch_steps": eager,
+            "run_steps_per_s_batched": len(cases) * steps / t_bat, "cases": runs}
+
+
 # (cell edge h in m, cutoff kappa); kappa >= 2 keeps every neighbouring cell
 # (centre distance up to sqrt(3) h) in the exact near field
 CLOSURE_CANDIDATES = ((1.25, 2.0), (1.5, 2.0), (2.0, 2.0), (2.5, 2.0), (2.0, 3.0))
@@ -820,6 +853,8 @@ def main(argv=None):
         out["published_sizes"] = published_sizes()
         log("[bench] C1 static wing")
         out["static_wing"] = static_wing_block()
+        log("[bench] batched reduced-frequency sweep (9 runs, graph mode)")
+        out["plunging_sweep"] = plunging_sweep_block()
         if not a.no_mixed:
             log("[bench] C5 flapping airplane at 100k with the closure")
             out["flapping_100k"] = flapping_closure_block(

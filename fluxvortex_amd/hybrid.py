@@ -275,6 +275,29 @@ class HybridBatch:
             pass
 
 
+def theodorsen_cl(k: float, h0c: float, omega: float, t, chord: float = 1.0):
+    """Theodorsen's lift coefficient of an airfoil plunging h = h0 sin(omega t),
+    with C(k) from Hankel functions (tests/experiment_hybrid_panel_particle.py:
+    28-36)."""
+    from scipy.special import hankel2
+    ck = hankel2(1, k) / (hankel2(1, k) + 1j * hankel2(0, k))
+    U = omega * chord / (2 * k)
+    h0 = h0c * chord
+    t = np.asarray(t, dtype=float)
+    return np.real(np.pi * (-h0 * omega ** 2 * np.sin(omega * t)) * chord / (2 * U ** 2)
+                   - 2 * np.pi * ck * h0 * omega * np.cos(omega * t) / U)
+
+
+def theodorsen_ratio(case: HybridCase, cl) -> float:
+    """CL amplitude after two periods over Theodorsen's, the reference's
+    measure (tests/experiment_hybrid_panel_particle.py:258-262)."""
+    cl = np.asarray(cl)
+    t = np.arange(len(cl)) * case.dt
+    m = t > 2 * case.meta["period"]
+    th = theodorsen_cl(case.meta["k"], case.meta["h0"], case.meta["omega"], t)
+    return float((cl[m].max() - cl[m].min()) / (th[m].max() - th[m].min()))
+
+
 def plunging_sweep_cases(ks=(0.5, 0.2, 0.1), n_keeps=(10, 20, 0), **kw) -> list:
     """The reference's reduced-frequency sweep (tests/experiment_hybrid_panel_
     particle.py:206-329): the heaving wing at each reduced frequency k with
