@@ -213,6 +213,24 @@ def test_mixed_precision_cl(hyb, oracle):
     assert np.max(np.abs(mixed["cl"] - ref["cl"])) < 1e-3 * scale
 
 
+def test_mixed_precision_cl_free_wake(hyb):
+    """The hybrid configuration as the reference's HybridSolver advects it
+    (free particle wake, no stretching, relaxation on), 150 steps: mixed
+    precision's CL against FP64 stays within 1e-3 of max |CL|, or within ten
+    times FP64's own response to a 1e-14 change of V where that is larger
+    (printed, with the measured error)."""
+    case = hyb.heaving_wing_case(free_wake=True, stretch=False)
+    ref = hyb.run_case(case)
+    from dataclasses import replace as _replace
+    pert = hyb.run_case(_replace(case, vinf=case.vinf * (1.0 + 1e-14)))
+    mixed = hyb.run_case(hyb.with_precision(case, "mixed"))
+    scale = np.max(np.abs(ref["cl"]))
+    self_sens = np.max(np.abs(pert["cl"] - ref["cl"])) / scale
+    err = np.max(np.abs(mixed["cl"] - ref["cl"])) / scale
+    print(f"free-wake mixed CL error {err:.2e}, FP64 self-sensitivity {self_sens:.2e}", flush=True)
+    assert err < max(1e-3, 10.0 * self_sens)
+
+
 def test_hybrid_solver_api(hyb):
     from fluxvortex_amd.geometry import Airplane, Motion, Surface
     wing = Surface(1.0, 1.0, (0.0, 5.0, 0.0), nc=4, ns=3)
