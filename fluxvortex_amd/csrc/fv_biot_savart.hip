@@ -246,6 +246,9 @@ static int tuned_plan(fv_ctx *ctx, const typename Op::Args &a, int64_t n_tgt, in
     return FV_OK;
 }
 
+// Source slices from which a wavefront per output adds the partial sums.
+constexpr int kWideReduce = 128;
+
 template <class Op>
 static int run_pairs(fv_ctx *ctx, const typename Op::Args &a, int64_t n_tgt, int64_t n_src,
                      const int64_t *dyn_tgt, const int64_t *dyn_src, double *out, int pslot,
@@ -269,9 +272,16 @@ static int run_pairs(fv_ctx *ctx, const typename Op::Args &a, int64_t n_tgt, int
     }
     if (p.split > 1) {
         const int64_t tot = n_tgt * Op::kComp;
-        hipLaunchKernelGGL((k_reduce_partials<Op::kComp>), dim3((unsigned)ceil_div(tot, 256)),
-                           dim3(256), 0, ctx->stream, dst, n_tgt, dyn_tgt, n_tgt, p.split,
-                           out);
+        // few targets over many sources (the particle field at the collocation
+        // points): a wavefront per output instead of one lane walking the slices
+        if (p.split >= kWideReduce)
+            hipLaunchKernelGGL((k_reduce_partials_wide<Op::kComp>),
+                               dim3((unsigned)ceil_div(tot, 4)), dim3(256), 0, ctx->stream, dst,
+                               n_tgt, dyn_tgt, n_tgt, p.split, out);
+        else
+            hipLaunchKernelGGL((k_reduce_partials<Op::kComp>), dim3((unsigned)ceil_div(tot, 256)),
+                               dim3(256), 0, ctx->stream, dst, n_tgt, dyn_tgt, n_tgt, p.split,
+                               out);
         FV_LAUNCH_CHECK(ctx);
     }
     return FV_OK;
@@ -504,9 +514,15 @@ int fv_vortex_collapsed_dev_noalloc(fv_ctx *ctx, const double *points, int64_t n
     }
     FV_LAUNCH_CHECK(ctx);
     if (p.split > 1) {
-        hipLaunchKernelGGL((k_reduce_partials<3>), dim3((unsigned)ceil_div(3 * n, 256)),
-                           dim3(256), 0, ctx->stream, part, n, (const int64_t *)nullptr, n,
-                           p.split, out);
+        // the reduction run_pairs picks for this split
+        if (p.split >= kWideReduce)
+            hipLaunchKernelGGL((k_reduce_partials_wide<3>), dim3((unsigned)ceil_div(3 * n, 4)),
+                               dim3(256), 0, ctx->stream, part, n, (const int64_t *)nullptr, n,
+                               p.split, out);
+        else
+            hipLaunchKernelGGL((k_reduce_partials<3>), dim3((unsigned)ceil_div(3 * n, 256)),
+                               dim3(256), 0, ctx->stream, part, n, (const int64_t *)nullptr, n,
+                               p.split, out);
         FV_LAUNCH_CHECK(ctx);
     }
     return FV_OK;

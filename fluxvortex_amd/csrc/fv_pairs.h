@@ -687,4 +687,26 @@ __global__ void k_reduce_partials(const double *__restrict__ partial, int64_t n_
     out[i * COMP + c] = s;
 }
 
+// Many slices (a few targets over many sources): one wavefront per output
+// element, lane l adding slices l, l + 64, ... in order, then a fixed
+// butterfly over the lanes (deterministic; 4 outputs per 256-lane group).
+template <int COMP>
+__global__ __launch_bounds__(256) void k_reduce_partials_wide(const double *__restrict__ partial,
+                                                              int64_t n_arg,
+                                                              const int64_t *__restrict__ dyn_n,
+                                                              int64_t stride, int split,
+                                                              double *__restrict__ out)
+{
+    const int64_t n = dyn_n ? *dyn_n : n_arg;
+    const int64_t e = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (e >= n * COMP) return;   // uniform over the wavefront
+    const int lane = threadIdx.x & 63;
+    const int64_t i = e / COMP, c = e % COMP;
+    double s = 0.0;
+    for (int y = lane; y < split; y += 64) s += partial[((int64_t)y * stride + i) * COMP + c];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) out[i * COMP + c] = s;
+}
+
 }  // namespace fv
