@@ -194,28 +194,47 @@ __global__ __launch_bounds__(64) void k_mlp_backward(MlpW m, const float *__rest
     }
 }
 
-// Weight gradient dW[n][k] = sum_s E[s][n] H[s][k] and bias gradient
-// db[n] = sum_s E[s][n], in two launches:
-//   k_mlp_wgrad_part: one wavefront per (16 x 16 tile, kWgChunk-sample chunk)
-//     accumulates the tile over its chunk on MFMA (16-sample slices in order)
-//     and, for k-tile 0, the chunk's bias sums (4 sample lanes per column,
-//     combined in a fixed order); partials land in part[chunk][ne*nh + ne];
-//   k_mlp_wgrad_reduce: sums the chunks in order (deterministic, no atomics).
+// Weight gradients dW[n][k] = sum_s E[s][n] H[s][k] and bias gradients
+// db[n] = sum_s E[s][n] of all four layers, in two launches:
+//   k_mlp_wgrad_part: one wavefront per (16 x 16 tile of any layer,
+//     kWgChunk-sample chunk) accumulates the tile over its chunk on MFMA
+//     (16-sample slices in order) and, for k-tile 0, the chunk's bias sums (4
+//     sample lanes per column, combined in a fixed order); each chunk's
+//     partials land in the parameter layout (part[chunk][kParams]);
+//   k_mlp_wgrad_reduce: sums the chunks in order per parameter
+//     (deterministic, no atomics).
 constexpr int kWgChunk = 256;
 
-__global__ __launch_bounds__(64) void k_mlp_wgrad_part(const unsigned short *__restrict__ E,
-                                                       int ne,
-                                                       const unsigned short *__restrict__ H,
-                                                       int nh, const float *__restrict__ X,
-                                                       int64_t nb, float *__restrict__ part)
+struct WgLayer {
+    const unsigned short *E;   // layer errors (bf16, nb x ne)
+    const unsigned short *H;   // layer inputs (bf16, nb x nh), or null: bf16(X)
+    const float *X;            // the network input (fp32, nb x nh) when H is null
+    int ne, nh, off_w, off_b, tile0;
+};
+struct WgPlan {
+    WgLayer L[4];
+};
+
+__global__ __launch_bounds__(64) void k_mlp_wgrad_part(WgPlan plan, int64_t nb,
+                                                       float *__restrict__ part)
 {
     const int l = threadIdx.x;
+    int li = 0;
+#pragma unroll
+    for (int k = 1; k < 4; ++k)
+        if ((int)blockIdx.x >= plan.L[k].tile0) li = k;
+    const WgLayer &L = plan.L[li];
+    const int ne = L.ne, nh = L.nh;
+    const unsigned short *__restrict__ E = L.E;
+    const unsigned short *__restrict__ H = L.H;
+    const float *__restrict__ X = L.X;
+    const int tile = (int)blockIdx.x - L.tile0;
     const int tiles_k = nh / 16;
-    const int n0 = (blockIdx.x / tiles_k) * 16, k0 = (blockIdx.x % tiles_k) * 16;
+    const int n0 = (tile / tiles_k) * 16, k0 = (tile % tiles_k) * 16;
     const int r = l & 15, sq = (l >> 4) * 4;
     const int64_t c0 = (int64_t)blockIdx.y * kWgChunk;
     const int64_t c1 = (c0 + kWgChunk < nb) ? c0 + kWgChunk : nb;
-    float *pw = part + (int64_t)blockIdx.y * (ne * nh + ne);
+    float *pc = part + (int64_t)blockIdx.y * kParams;
     fv_f4 acc = {0.f, 0.f, 0.f, 0.f};
     for (int64_t s0 = c0; s0 < c1; s0 += 16) {
         fv_s4 a, bq;
@@ -231,27 +250,25 @@ __global__ __launch_bounds__(64) void k_mlp_wgrad_part(const unsigned short *__r
         acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bq, acc, 0, 0, 0);
     }
 #pragma unroll
-    for (int v = 0; v < 4; ++v) pw[(n0 + sq + v) * nh + k0 + r] = acc[v];
+    for (int v = 0; v < 4; ++v) pc[L.off_w + (n0 + sq + v) * nh + k0 + r] = acc[v];
     if (k0 == 0) {
         // lane l: column n0 + (l & 15), samples c0 + (l >> 4) + 4 i
         float bs = 0.0f;
         for (int64_t s = c0 + (l >> 4); s < c1; s += 4) bs += bf2f(E[s * ne + n0 + r]);
         bs += __shfl_xor(bs, 16, 64);
         bs += __shfl_xor(bs, 32, 64);
-        if (l < 16) pw[ne * nh + n0 + l] = bs;
+        if (l < 16) pc[L.off_b + n0 + l] = bs;
     }
 }
 
-__global__ void k_mlp_wgrad_reduce(const float *__restrict__ part, int ne, int nh, int nchunk,
-                                   float *__restrict__ dW, float *__restrict__ db)
+__global__ void k_mlp_wgrad_reduce(const float *__restrict__ part, int nchunk,
+                                   float *__restrict__ g)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int tot = ne * nh + ne;
-    if (i >= tot) return;
+    if (i >= kParams) return;
     float s = 0.0f;
-    for (int c = 0; c < nchunk; ++c) s += part[(int64_t)c * tot + i];
-    if (i < ne * nh) dW[i] = s;
-    else db[i - ne * nh] = s;
+    for (int c = 0; c < nchunk; ++c) s += part[(int64_t)c * kParams + i];
+    g[i] = s;
 }
 
 // Adam on the FP32 master parameters; refreshes the bf16 weight copies.
@@ -334,10 +351,10 @@ static int mlp_scratch(fv_mlp *m, int64_t nb)
         m->scratch = nullptr;
     }
     // Y (16 fp32), zs (3x128 fp32), hs (3x128 bf16), dz (3x128 bf16), dy (16 bf16),
-    // then the weight-gradient partials of the largest layer per sample chunk
+    // then the weight-gradient partials of every layer per sample chunk (parameter layout)
     const size_t nchunk = (size_t)ceil_div(nb, kWgChunk);
     const size_t bytes = (size_t)nb * (16 * 4 + 384 * 4 + 384 * 2 + 384 * 2 + 16 * 2) + 1024 +
-                         nchunk * (kHid * kHid + kHid) * sizeof(float) + 256;
+                         nchunk * (size_t)kParams * sizeof(float) + 256;
     FV_HIP_CHECK(ctx, hipMalloc(&m->scratch, bytes));
     m->scratch_nb = nb;
     return FV_OK;
@@ -459,29 +476,27 @@ static int mlp_gradients(fv_mlp *m, const float *X, const float *T, const float 
     hipLaunchKernelGGL(k_mlp_backward, dim3(grid), dim3(64), 0, ctx->stream, w, Y, T, wout, nb,
                        zs, dz, dy);
     FV_LAUNCH_CHECK(ctx);
-    float *g = m->grad;
     const size_t base = (size_t)nb * (16 * 4 + 384 * 4 + 384 * 2 + 384 * 2 + 16 * 2) + 1024;
     float *part = reinterpret_cast<float *>(s + ((base + 255) & ~size_t(255)));
     const int nchunk = (int)ceil_div(nb, kWgChunk);
-    auto wgrad = [&](const unsigned short *E, int ne, const unsigned short *H, int nh,
-                     const float *Xin, float *dW, float *db) -> int {
-        hipLaunchKernelGGL(k_mlp_wgrad_part, dim3((ne / 16) * (nh / 16), nchunk), dim3(64), 0,
-                           ctx->stream, E, ne, H, nh, Xin, nb, part);
-        FV_LAUNCH_CHECK(ctx);
-        const int tot = ne * nh + ne;
-        hipLaunchKernelGGL(k_mlp_wgrad_reduce, dim3((unsigned)ceil_div(tot, 256)), dim3(256), 0,
-                           ctx->stream, part, ne, nh, nchunk, dW, db);
-        FV_LAUNCH_CHECK(ctx);
-        return FV_OK;
-    };
-    // W4 / b4: E = dy (16), H = h3;  W3 / b3: E = dz[2], H = h2;  W2 / b2: E = dz[1],
-    // H = h1;  W1 / b1: E = dz[0], H = bf16(X)
-    int rc2 = wgrad(dy, kOut, hs + 2 * nb * kHid, kHid, nullptr, g + kOffW4, g + kOffB4);
-    if (!rc2) rc2 = wgrad(dz + 2 * nb * kHid, kHid, hs + nb * kHid, kHid, nullptr, g + kOffW3,
-                          g + kOffB3);
-    if (!rc2) rc2 = wgrad(dz + nb * kHid, kHid, hs, kHid, nullptr, g + kOffW2, g + kOffB2);
-    if (!rc2) rc2 = wgrad(dz, kHid, nullptr, kIn, X, g + kOffW1, g + kOffB1);
-    if (rc2) return rc2;
+    // W1 / b1: E = dz[0], H = bf16(X);  W2 / b2: E = dz[1], H = h1;
+    // W3 / b3: E = dz[2], H = h2;  W4 / b4: E = dy (16), H = h3
+    WgPlan plan;
+    plan.L[0] = WgLayer{dz, nullptr, X, kHid, kIn, kOffW1, kOffB1, 0};
+    plan.L[1] = WgLayer{dz + nb * kHid, hs, nullptr, kHid, kHid, kOffW2, kOffB2, 0};
+    plan.L[2] = WgLayer{dz + 2 * nb * kHid, hs + nb * kHid, nullptr, kHid, kHid, kOffW3, kOffB3, 0};
+    plan.L[3] = WgLayer{dy, hs + 2 * nb * kHid, nullptr, kOut, kHid, kOffW4, kOffB4, 0};
+    int tiles = 0;
+    for (int k = 0; k < 4; ++k) {
+        plan.L[k].tile0 = tiles;
+        tiles += (plan.L[k].ne / 16) * (plan.L[k].nh / 16);
+    }
+    hipLaunchKernelGGL(k_mlp_wgrad_part, dim3((unsigned)tiles, (unsigned)nchunk), dim3(64), 0,
+                       ctx->stream, plan, nb, part);
+    FV_LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_mlp_wgrad_reduce, dim3((unsigned)ceil_div(kParams, 256)), dim3(256), 0,
+                       ctx->stream, part, nchunk, m->grad);
+    FV_LAUNCH_CHECK(ctx);
     tl.done();
     return FV_OK;
 }
