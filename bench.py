@@ -713,7 +713,7 @@ def closure_params(cell, kappa):
                 train_iters=2, lr=2e-3, kappa_local=2.0 * kappa)
 
 
-def big_block(hybrid, n, skip_mixed):
+def big_block(hybrid, n, skip_mixed, candidates=CLOSURE_CANDIDATES):
     """The 100k block: FP64, mixed, then the closure candidates cheapest first
     with the network and without it (expansion only), each kept only if every
     one of its own checks passed; accuracy windows as for the 20k line."""
@@ -734,7 +734,7 @@ def big_block(hybrid, n, skip_mixed):
     chosen = {}
     for mlp in (True, False):
         tried = []
-        for cell, kappa in CLOSURE_CANDIDATES:
+        for cell, kappa in candidates:
             clo = closure_params(cell, kappa)
             log(f"[bench] {n} particles, mixed + closure (network {mlp}), cell {cell} "
                 f"kappa {kappa}")
@@ -778,6 +778,8 @@ def main(argv=None):
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-flutter", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--closure", default=None,
+                    help="CELL,KAPPA: the 100k block's only closure candidate (default: all)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip C1 / C3 / C5, the published sizes and the MLP roofline")
     a = ap.parse_args(argv)
@@ -832,7 +834,10 @@ def main(argv=None):
         out["mixed"]["cl_max_rel_err_vs_fp64"] = acc["unstretched"]["mixed_cl_max_rel_err_vs_fp64"]
         out["mixed"]["cl_error_window"] = "unstretched free wake (HybridSolver's advect)"
     if a.big:
-        big, b64, chosen = big_block(hybrid, a.big, a.no_mixed)
+        cands = CLOSURE_CANDIDATES
+        if a.closure:
+            cands = (tuple(float(v) for v in a.closure.split(",")),)
+        big, b64, chosen = big_block(hybrid, a.big, a.no_mixed, cands)
         if chosen is not None:
             clo, bml = chosen
             if "families" in bml:
