@@ -78,19 +78,84 @@ __device__ __forceinline__ void wave_gemm_nt(const unsigned short *A, int lda,
     }
 }
 
-// Forward pass of nb samples; one wavefront per 16-sample group.
+// LDS plan of the forward and backward kernels: the bf16 weight matrices of
+// the pass, staged once per workgroup with padded row strides (K + 8, so the
+// 16 rows a wavefront reads per MFMA step fall in distinct banks), then per
+// wavefront two 16 x kLda activation buffers.  Up to kMlpWaves wavefronts
+// share the staged weights, each owning 16 samples at a time.
+constexpr int kMlpWaves = 8;
+constexpr int kLdw1 = kIn + 8;    // W1 [128][32]
+constexpr int kLdwH = kHid + 8;   // W2, W3 [128][128]; W4 [16][128]
+constexpr int kLdwT = kOut + 8;   // W4^T [128][16]
+// forward: W1, W2, W3, W4
+constexpr int kFw1 = 0, kFw2 = kFw1 + kHid * kLdw1, kFw3 = kFw2 + kHid * kLdwH,
+              kFw4 = kFw3 + kHid * kLdwH, kFwEnd = kFw4 + kOut * kLdwH;
+// backward: W2^T, W3^T, W4^T
+constexpr int kBw1 = 0, kBw2 = kBw1 + kHid * kLdwH, kBw3 = kBw2 + kHid * kLdwH,
+              kBwEnd = kBw3 + kHid * kLdwT;
+constexpr int kActPerWave = 2 * 16 * kLda;
+__host__ __device__ constexpr size_t fwd_lds(int waves)
+{
+    return sizeof(unsigned short) * ((size_t)kFwEnd + (size_t)waves * kActPerWave);
+}
+__host__ __device__ constexpr size_t bwd_lds(int waves)
+{
+    return sizeof(unsigned short) * ((size_t)kBwEnd + (size_t)waves * kActPerWave);
+}
+
+__device__ __forceinline__ void mlp_wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// rows x K bf16 (dense) -> LDS with row stride ld, 4 values per lane and step
+__device__ __forceinline__ void stage_rows(const unsigned short *__restrict__ src, int rows,
+                                           int K, unsigned short *dst, int ld)
+{
+    const int q = K / 4;
+    for (int e = threadIdx.x; e < rows * q; e += blockDim.x) {
+        const int r = e / q, c = (e - r * q) * 4;
+        *reinterpret_cast<fv_s4 *>(dst + r * ld + c) =
+            *reinterpret_cast<const fv_s4 *>(src + r * K + c);
+    }
+}
+
+// Forward pass of nb samples; each wavefront takes 16-sample groups in turn.
 //   X (nb x 16 fp32) -> Y (nb x 16 fp32).  When zs / hs are non-null the
 //   pre-activations (fp32) and activations (bf16) of the 3 hidden layers are
 //   kept for the backward pass: zs[(layer * nb + s) * 128 + n].
-__global__ __launch_bounds__(64) void k_mlp_forward(MlpW m, const float *__restrict__ X,
-                                                    int64_t nb, float *__restrict__ Y,
-                                                    float *__restrict__ zs,
-                                                    unsigned short *__restrict__ hs)
+// STAGE: the weights are staged in LDS (large batches); else every wavefront
+// reads them from global memory (L2), which small batches pay less for.
+template <bool STAGE>
+__global__ __launch_bounds__(STAGE ? 64 * kMlpWaves : 64) void k_mlp_forward(MlpW m,
+                                                                const float *__restrict__ X,
+                                                                int64_t nb, float *__restrict__ Y,
+                                                                float *__restrict__ zs,
+                                                                unsigned short *__restrict__ hs)
 {
-    __shared__ __attribute__((aligned(16))) unsigned short act[2][16 * kLda];
-    const int l = threadIdx.x;
+    extern __shared__ __attribute__((aligned(16))) unsigned short smem[];
+    const unsigned short *wl[4] = {m.w[0], m.w[1], m.w[2], m.w[3]};
+    const int ld1 = STAGE ? kLdw1 : kIn, ldh = STAGE ? kLdwH : kHid;
+    if (STAGE) {
+        wl[0] = smem + kFw1;
+        wl[1] = smem + kFw2;
+        wl[2] = smem + kFw3;
+        wl[3] = smem + kFw4;
+        stage_rows(m.w[0], kHid, kIn, smem + kFw1, kLdw1);
+        stage_rows(m.w[1], kHid, kHid, smem + kFw2, kLdwH);
+        stage_rows(m.w[2], kHid, kHid, smem + kFw3, kLdwH);
+        stage_rows(m.w[3], kOut, kHid, smem + kFw4, kLdwH);
+        __syncthreads();
+    }
+    const int wv = threadIdx.x >> 6;
+    unsigned short(*act)[16 * kLda] = reinterpret_cast<unsigned short(*)[16 * kLda]>(
+        smem + (STAGE ? kFwEnd : 0) + wv * kActPerWave);
+    const int l = threadIdx.x & 63;
     const int col = l & 15, rq = (l >> 4) * 4;
-    for (int64_t g = blockIdx.x; g * 16 < nb; g += gridDim.x) {
+    const int nw = blockDim.x >> 6;
+    for (int64_t g = (int64_t)blockIdx.x * nw + wv; g * 16 < nb; g += (int64_t)gridDim.x * nw) {
         const int64_t s0 = g * 16;
         // inputs -> bf16 LDS (16 x kIn)
         for (int e = l; e < 16 * kIn; e += 64) {
@@ -98,15 +163,15 @@ __global__ __launch_bounds__(64) void k_mlp_forward(MlpW m, const float *__restr
             const float v = (s0 + i < nb) ? X[(s0 + i) * kIn + k] : 0.0f;
             act[0][i * kLda + k] = f2bf(v);
         }
-        __syncthreads();
+        mlp_wave_sync();
         int cur = 0;
 #pragma unroll
         for (int layer = 0; layer < 3; ++layer) {
             fv_f4 acc[8];
             if (layer == 0)
-                wave_gemm_nt<kIn, 8>(act[cur], kLda, m.w[0], kIn, acc);
+                wave_gemm_nt<kIn, 8>(act[cur], kLda, wl[0], ld1, acc);
             else
-                wave_gemm_nt<kHid, 8>(act[cur], kLda, m.w[layer], kHid, acc);
+                wave_gemm_nt<kHid, 8>(act[cur], kLda, wl[layer], ldh, acc);
             const int nxt = cur ^ 1;
 #pragma unroll
             for (int t = 0; t < 8; ++t) {
@@ -124,37 +189,51 @@ __global__ __launch_bounds__(64) void k_mlp_forward(MlpW m, const float *__restr
                     }
                 }
             }
-            __syncthreads();
+            mlp_wave_sync();
             cur = nxt;
         }
         fv_f4 out[1];
-        wave_gemm_nt<kHid, 1>(act[cur], kLda, m.w[3], kHid, out);
+        wave_gemm_nt<kHid, 1>(act[cur], kLda, wl[3], ldh, out);
         const float bias = m.b[3][col];
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int64_t s = s0 + rq + v;
             if (s < nb) Y[s * kOut + col] = out[0][v] + bias;
         }
-        __syncthreads();
+        mlp_wave_sync();
     }
 }
 
 // Backward of the weighted MSE  L = (1/nb) sum_s sum_j w_j (Y_sj - T_sj)^2
 // down to the layer errors, kept as bf16 (the MFMA inputs of the weight
 // gradients):  dz[(L * nb + s) * 128 + n] for L = 0..2 (hidden layers), and
-// dy[s * 16 + j] for the output layer.
-__global__ __launch_bounds__(64) void k_mlp_backward(MlpW m, const float *__restrict__ Y,
-                                                     const float *__restrict__ T,
-                                                     const float *__restrict__ wout, int64_t nb,
-                                                     const float *__restrict__ zs,
-                                                     unsigned short *__restrict__ dz,
-                                                     unsigned short *__restrict__ dy)
+// dy[s * 16 + j] for the output layer.  STAGE as for the forward pass.
+template <bool STAGE>
+__global__ __launch_bounds__(STAGE ? 64 * kMlpWaves : 64) void k_mlp_backward(
+    MlpW m, const float *__restrict__ Y, const float *__restrict__ T,
+    const float *__restrict__ wout, int64_t nb, const float *__restrict__ zs,
+    unsigned short *__restrict__ dz, unsigned short *__restrict__ dy)
 {
-    __shared__ __attribute__((aligned(16))) unsigned short buf[2][16 * kLda];
-    const int l = threadIdx.x;
+    extern __shared__ __attribute__((aligned(16))) unsigned short smem[];
+    const unsigned short *wt1 = m.wt[1], *wt2 = m.wt[2], *wt3 = m.wt[3];
+    const int ldh = STAGE ? kLdwH : kHid, ldt = STAGE ? kLdwT : kOut;
+    if (STAGE) {
+        wt1 = smem + kBw1;
+        wt2 = smem + kBw2;
+        wt3 = smem + kBw3;
+        stage_rows(m.wt[1], kHid, kHid, smem + kBw1, kLdwH);
+        stage_rows(m.wt[2], kHid, kHid, smem + kBw2, kLdwH);
+        stage_rows(m.wt[3], kHid, kOut, smem + kBw3, kLdwT);
+        __syncthreads();
+    }
+    const int wv = threadIdx.x >> 6;
+    unsigned short(*buf)[16 * kLda] = reinterpret_cast<unsigned short(*)[16 * kLda]>(
+        smem + (STAGE ? kBwEnd : 0) + wv * kActPerWave);
+    const int l = threadIdx.x & 63;
     const int col = l & 15, rq = (l >> 4) * 4;
     const float inv = 2.0f / (float)nb;
-    for (int64_t g = blockIdx.x; g * 16 < nb; g += gridDim.x) {
+    const int nw = blockDim.x >> 6;
+    for (int64_t g = (int64_t)blockIdx.x * nw + wv; g * 16 < nb; g += (int64_t)gridDim.x * nw) {
         const int64_t s0 = g * 16;
         for (int e = l; e < 256; e += 64) {
             const int i = e >> 4, j = e & 15;
@@ -165,14 +244,14 @@ __global__ __launch_bounds__(64) void k_mlp_backward(MlpW m, const float *__rest
             buf[0][i * kLda + j] = h;
             if (s < nb) dy[s * kOut + j] = h;
         }
-        __syncthreads();
+        mlp_wave_sync();
         int cur = 0;
         for (int layer = 2; layer >= 0; --layer) {
             fv_f4 acc[8];
             if (layer == 2)
-                wave_gemm_nt<kOut, 8>(buf[cur], kLda, m.wt[3], kOut, acc);
+                wave_gemm_nt<kOut, 8>(buf[cur], kLda, wt3, ldt, acc);
             else
-                wave_gemm_nt<kHid, 8>(buf[cur], kLda, m.wt[layer + 1], kHid, acc);
+                wave_gemm_nt<kHid, 8>(buf[cur], kLda, layer == 1 ? wt2 : wt1, ldh, acc);
             const int nxt = cur ^ 1;
 #pragma unroll
             for (int t = 0; t < 8; ++t) {
@@ -188,7 +267,7 @@ __global__ __launch_bounds__(64) void k_mlp_backward(MlpW m, const float *__rest
                     if (s < nb) dz[((int64_t)layer * nb + s) * kHid + n] = h;
                 }
             }
-            __syncthreads();
+            mlp_wave_sync();
             cur = nxt;
         }
     }
@@ -320,6 +399,30 @@ struct fv_mlp {
 
 static constexpr int kWeightsBf = kHid * kIn + 2 * kHid * kHid + kOut * kHid;
 
+// Shape of a forward / backward launch.  Large batches (at least four
+// 16-sample groups per CU): one workgroup per CU, kMlpWaves wavefronts
+// sharing weights staged in LDS (~150 KB).  Small batches (the online
+// training's few thousand samples): one wavefront per workgroup and group,
+// the weights read from L2, since staging them per workgroup would cost more
+// than the few groups each workgroup holds.
+struct MlpShape {
+    unsigned grid;
+    int waves;
+    bool stage;
+    size_t lds_fwd, lds_bwd;
+};
+static MlpShape mlp_shape(fv_ctx *ctx, int64_t nb)
+{
+    const int64_t groups = ceil_div(nb, 16), cus = num_cus(ctx);
+    if (groups >= 4 * cus) {
+        const int64_t wg = ceil_div(groups, kMlpWaves);
+        return MlpShape{(unsigned)(wg < cus ? wg : cus), kMlpWaves, true, fwd_lds(kMlpWaves),
+                        bwd_lds(kMlpWaves)};
+    }
+    const size_t act = sizeof(unsigned short) * kActPerWave;
+    return MlpShape{(unsigned)(groups < 8192 ? groups : 8192), 1, false, act, act};
+}
+
 static MlpW mlp_views(const fv_mlp *m)
 {
     MlpW w;
@@ -444,11 +547,16 @@ int fv_mlp_forward_dev(fv_mlp *m, const float *X, int64_t nb, float *Y)
     FV_REQUIRE(ctx, nb >= 0, "negative batch");
     if (nb == 0) return FV_OK;
     FV_REQUIRE(ctx, X && Y, "null X/Y");
-    const int64_t groups = ceil_div(nb, 16);
-    const unsigned grid = (unsigned)(groups < 8192 ? groups : 8192);
+    const MlpShape sh = mlp_shape(ctx, nb);
     TimedLaunch tl(ctx, "mlp_forward");
-    hipLaunchKernelGGL(k_mlp_forward, dim3(grid), dim3(64), 0, ctx->stream, mlp_views(m), X, nb,
-                       Y, (float *)nullptr, (unsigned short *)nullptr);
+    if (sh.stage)
+        hipLaunchKernelGGL(k_mlp_forward<true>, dim3(sh.grid), dim3(64 * sh.waves),
+                           (unsigned)sh.lds_fwd, ctx->stream, mlp_views(m), X, nb, Y,
+                           (float *)nullptr, (unsigned short *)nullptr);
+    else
+        hipLaunchKernelGGL(k_mlp_forward<false>, dim3(sh.grid), dim3(64 * sh.waves),
+                           (unsigned)sh.lds_fwd, ctx->stream, mlp_views(m), X, nb, Y,
+                           (float *)nullptr, (unsigned short *)nullptr);
     FV_LAUNCH_CHECK(ctx);
     tl.done();
     return FV_OK;
@@ -467,14 +575,22 @@ static int mlp_gradients(fv_mlp *m, const float *X, const float *T, const float 
     unsigned short *hs = reinterpret_cast<unsigned short *>(zs + nb * 384);
     unsigned short *dz = hs + nb * 384;
     unsigned short *dy = dz + nb * 384;
-    const int64_t groups = ceil_div(nb, 16);
-    const unsigned grid = (unsigned)(groups < 8192 ? groups : 8192);
+    const MlpShape sh = mlp_shape(ctx, nb);
     const MlpW w = mlp_views(m);
     TimedLaunch tl(ctx, "mlp_train");
-    hipLaunchKernelGGL(k_mlp_forward, dim3(grid), dim3(64), 0, ctx->stream, w, X, nb, Y, zs, hs);
-    FV_LAUNCH_CHECK(ctx);
-    hipLaunchKernelGGL(k_mlp_backward, dim3(grid), dim3(64), 0, ctx->stream, w, Y, T, wout, nb,
-                       zs, dz, dy);
+    if (sh.stage) {
+        hipLaunchKernelGGL(k_mlp_forward<true>, dim3(sh.grid), dim3(64 * sh.waves),
+                           (unsigned)sh.lds_fwd, ctx->stream, w, X, nb, Y, zs, hs);
+        FV_LAUNCH_CHECK(ctx);
+        hipLaunchKernelGGL(k_mlp_backward<true>, dim3(sh.grid), dim3(64 * sh.waves),
+                           (unsigned)sh.lds_bwd, ctx->stream, w, Y, T, wout, nb, zs, dz, dy);
+    } else {
+        hipLaunchKernelGGL(k_mlp_forward<false>, dim3(sh.grid), dim3(64 * sh.waves),
+                           (unsigned)sh.lds_fwd, ctx->stream, w, X, nb, Y, zs, hs);
+        FV_LAUNCH_CHECK(ctx);
+        hipLaunchKernelGGL(k_mlp_backward<false>, dim3(sh.grid), dim3(64 * sh.waves),
+                           (unsigned)sh.lds_bwd, ctx->stream, w, Y, T, wout, nb, zs, dz, dy);
+    }
     FV_LAUNCH_CHECK(ctx);
     const size_t base = (size_t)nb * (16 * 4 + 384 * 4 + 384 * 2 + 384 * 2 + 16 * 2) + 1024;
     float *part = reinterpret_cast<float *>(s + ((base + 255) & ~size_t(255)));
