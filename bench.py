@@ -381,7 +381,7 @@ def published_sizes():
     (N points, M rings) = (500, 2000), (1000, 5000) and 10 000 x 10 000
     (README.md:178-180), plus its benchmark's third size (500, 10 000)
     (tests/test_benchmark.py:20), through the patch surface with the host
-    copies included, seeded as that benchmark; mean of 3 after a warm-up."""
+    copies included, seeded as that benchmark; median of 7 after a warm-up."""
     from fluxvortex_amd import ptera_patch
     ns = ptera_patch.default_target()
     ptera_patch.patch(verbose=False)
@@ -401,11 +401,11 @@ def published_sizes():
             args = (points, br, fr, fl, bl, g, rc, sing, ages, 1.5e-5)
             fn(*args)
             ts = []
-            for _ in range(3):
+            for _ in range(7):
                 t0 = time.perf_counter()
                 fn(*args)
                 ts.append(time.perf_counter() - t0)
-            ms = float(np.mean(ts)) * 1e3
+            ms = float(np.median(ts)) * 1e3
             e = {"N": N, "M": M, "ms_host_copies_included": ms,
                  "gpairs_per_s": 4.0 * N * M / (ms / 1e3) / 1e9}
             if gpu_ms:
