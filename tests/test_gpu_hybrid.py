@@ -191,6 +191,30 @@ def test_solve_reuse_is_bitwise(hyb, precision):
         run.close()
 
 
+def test_solve_reuse_across_precision_switches(hyb):
+    """Switching precision mid-run drops the stored factors (they are of the
+    other precision's type): FP64 -> mixed -> FP64 on the static wing equals
+    the same switches without reuse bit for bit."""
+    case = hyb.static_wing_case(symmetric=False)
+    outs = []
+    for on in (True, False):
+        run = hyb.HybridRun(case)
+        try:
+            run.set_solve_reuse(on)
+            run.step(10)
+            run.set_precision("mixed")
+            run.step(10)
+            run.set_precision("fp64")
+            run.step(case.num_steps - 20)
+            outs.append(run.read())
+            if on:
+                assert run.solve_reuse_count() == case.num_steps - 3
+        finally:
+            run.close()
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
+
+
 def test_flapping_three_surfaces(hyb, oracle):
     case = hyb.flapping_case(num_steps=20, n_keep=6)
     compare(hyb, oracle, case)
