@@ -134,8 +134,10 @@ def kernel_families(run, steps=2):
     ctx = run._ctx
     ctx.set_timing(True)
     fam = {}
+    clo = run.particles.has_closure()
     for _ in range(steps):
         marks = {f: ctx.kernel_stats(f) for f in FAMILIES}
+        near0 = run.particles.closure_stats()["near_pairs"] if clo else 0.0
         run.step(1)
         n = run.particles.np
         for f in FAMILIES:
@@ -146,6 +148,10 @@ def kernel_families(run, steps=2):
             if f in SELF_FAMILIES and k > marks[f][0]:
                 e["pairs"] += (k - marks[f][0]) * float(n) * float(n)
                 e["n"].append(n)
+        if clo:
+            e = fam["closure_near"]
+            e["pairs"] += run.particles.closure_stats()["near_pairs"] - near0
+            e["n"].append(n)
     ctx.set_timing(False)
     return fam
 
@@ -670,6 +676,24 @@ def flutter_block(n_speeds=64, cpu_sample=4):
     return res
 
 
+def near_roofline(fam):
+    """The closure's exact near field (k_near) over a timed step: the pairs it
+    evaluated (device count) per second, and as FP64 VALU flops against the
+    78.6 TFLOP/s peak with the direct sums' algorithmic flops per pair
+    (velocity 31, stretching 66, Gaussian vorticity 72; 3 + 3 + 1 passes per
+    step, mixed precision: FP32 kernel factors, FP64 geometry and sums)."""
+    e = fam.get("closure_near")
+    if not e or not e["launches"] or not e["pairs"]:
+        return None
+    sec = e["ms"] / 1e3
+    flops_per_pair = (3 * 31 + 3 * 66 + 72) / 7.0
+    ach = e["pairs"] * flops_per_pair / sec
+    return {"bound": "valu", "pairs": e["pairs"], "launches": e["launches"],
+            "avg_launch_us": e["ms"] / e["launches"] * 1e3, "gpairs_per_s": e["pairs"] / sec / 1e9,
+            "achieved_TFLOPs": ach / 1e12, "peak_TFLOPs": PEAK_FP64 / 1e12,
+            "frac": ach / PEAK_FP64}
+
+
 def plunging_sweep_block():
     """The reference's reduced-frequency sweep (k = 0.5 / 0.2 / 0.1 x N_keep =
     10 / 20 / 0, free particle wake, 150 steps each): the nine runs one after
@@ -845,6 +869,7 @@ def main(argv=None):
                     k: bml["families"][k] for k in ("mlp_forward", "mlp_train", "closure_near",
                                                     "closure_far", "closure_residual")
                     if k in bml["families"]}
+                big["closure_near_roofline"] = near_roofline(bml["families"])
             log("[bench] 100k accuracy windows (FP64 self-sensitivity, mixed, mixed + learned)")
             acc = accuracy(hybrid, a.big, BIG_STEPS, 1, closure=clo)
             big["accuracy"] = acc

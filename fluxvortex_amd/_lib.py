@@ -156,6 +156,7 @@ _SIGNATURES = {
     "fv_vpm_set_closure_local": (C.c_int, [_vp, C.c_double]),
     "fv_vpm_clear_closure": (C.c_int, [_vp]),
     "fv_vpm_closure_stats": (C.c_int, [_vp, _dp, _i64p, _i64p, _i32p, _dp]),
+    "fv_vpm_closure_near_pairs": (C.c_int, [_vp, _dp]),
     "fv_vpm_closure_velocity_host": (C.c_int, [_vp, C.c_int32, _dp, _dp]),
     "fv_vpm_closure_sum_host": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32, _dp]),
     "fv_sweep_create": (C.c_int, [_ctx, _vp, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),

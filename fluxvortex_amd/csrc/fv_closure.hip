@@ -109,7 +109,8 @@ struct fv_closure_state {
     int32_t *dstencil = nullptr;       // columns, then slice offsets
     void *tmp = nullptr;
     size_t tmp_bytes = 0;
-    double *bbox = nullptr;            // 6 + per-block partials
+    double *bbox = nullptr;            // 6 + 2 (occupancy) + per-block partials + near-pair counts
+    unsigned long long *near_cnt = nullptr;   // [0] near pairs of the binning, [1] of every pass
     float *tX = nullptr, *tT = nullptr, *tw = nullptr;
     double *vdir = nullptr, *vclo = nullptr, *verr = nullptr;
 };
@@ -782,6 +783,39 @@ __global__ __launch_bounds__(64) void k_near(
     }
 }
 
+__global__ void k_set_u64(unsigned long long *p, unsigned long long v) { *p = v; }
+
+// Near-field pairs of a binning: every occupied target cell's population
+// times the particles of its stencil columns (what k_near evaluates), summed
+// exactly in integers (one atomic add per wavefront).
+__global__ void k_near_pairs(int64_t nocc, const int32_t *__restrict__ occ,
+                             const int32_t *__restrict__ cstart, const int32_t *__restrict__ cend,
+                             const int32_t *__restrict__ pstart, const int32_t *__restrict__ cols,
+                             int ncol, int nx, int ny, int nz, unsigned long long *cnt)
+{
+    const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long v = 0;
+    if (o < nocc) {
+        const int c = occ[o];
+        const int cx = c / (ny * nz), cy = (c / nz) % ny, cz = c % nz;
+        long long src = 0;
+        for (int e = 0; e < ncol; ++e) {
+            const int ix = cx + cols[3 * e], iy = cy + cols[3 * e + 1], dzm = cols[3 * e + 2];
+            if (ix < 0 || ix >= nx || iy < 0 || iy >= ny) continue;
+            const int iz0 = cz - dzm < 0 ? 0 : cz - dzm;
+            const int iz1 = cz + dzm >= nz ? nz - 1 : cz + dzm;
+            const int kb = (ix * ny + iy) * nz;
+            src += pstart[kb + iz1 + 1] - pstart[kb + iz0];
+        }
+        v = (unsigned long long)(cend[c] - cstart[c]) * (unsigned long long)src;
+    }
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(cnt, v);
+}
+
+// The pairs of one near-field pass (the binning's) into the running total.
+__global__ void k_add_near_pairs(unsigned long long *cnt) { cnt[1] += cnt[0]; }
+
 // Population of every cell (empty cells 0) and one trailing zero, for the
 // exclusive scan that gives pstart.
 __global__ void k_cell_counts(int64_t ncell, const int32_t *__restrict__ cstart,
@@ -1166,7 +1200,10 @@ static int closure_build(fv_vpm *f, bool use_mlp, bool with_res = true, bool sta
     rc = grow(ctx, &cl->tmp, &cl->tmp_bytes, 4096);
     if (rc) return rc;
     if (!cl->bbox) {
-        FV_HIP_CHECK(ctx, hipMalloc(&cl->bbox, sizeof(double) * (6 * nbb + 8)));
+        FV_HIP_CHECK(ctx, hipMalloc(&cl->bbox, sizeof(double) * (6 * nbb + 10)));
+        cl->near_cnt = reinterpret_cast<unsigned long long *>(cl->bbox + 8 + 6 * nbb);
+        FV_HIP_CHECK(ctx, hipMemsetAsync(cl->near_cnt, 0, 2 * sizeof(unsigned long long),
+                                         ctx->stream));
     }
     hipLaunchKernelGGL(k_bbox_partial, dim3(nbb), dim3(256), 0, ctx->stream, f->np_dev, f->x,
                        f->y, f->z, cl->bbox + 8);
@@ -1322,6 +1359,14 @@ static int closure_build(fv_vpm *f, bool use_mlp, bool with_res = true, bool sta
     if (rc) return rc;
     rc = update_stencil(cl, ctx);
     if (rc) return rc;
+    // near-field pairs of this binning (every pass of it evaluates them)
+    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, ctx->stream, cl->near_cnt, 0ull);
+    FV_LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_near_pairs, dim3((unsigned)ceil_div(nocc, 256)), dim3(256), 0,
+                       ctx->stream, nocc, cl->occ, cl->cstart, cl->cend, cl->pstart,
+                       cl->dstencil, (int)(cl->stencil.size() / 3), cl->nx, cl->ny, cl->nz,
+                       cl->near_cnt);
+    FV_LAUNCH_CHECK(ctx);
     cl->built_epoch = f->state_epoch;
     return with_res ? closure_residual(f, use_mlp) : FV_OK;
 }
@@ -1502,6 +1547,8 @@ static int closure_pass(fv_vpm *f, int mode, int kernel, double *out, bool use_m
         FV_LAUNCH_CHECK(ctx);
         tl.done();
     }
+    hipLaunchKernelGGL(k_add_near_pairs, dim3(1), dim3(1), 0, ctx->stream, cl->near_cnt);
+    FV_LAUNCH_CHECK(ctx);
     const int nfs = cl->kappa_local > 0.0 ? 1 : far_slices(cl);
     {
         TimedLaunch tl(ctx, "closure_far");
@@ -1676,6 +1723,20 @@ int fv_vpm_clear_closure(fv_vpm *f)
     (void)hipStreamSynchronize(f->ctx->stream);
     closure_destroy(f->closure);
     f->closure = nullptr;
+    return FV_OK;
+}
+
+int fv_vpm_closure_near_pairs(fv_vpm *f, double *pairs)
+{
+    if (!f || !f->closure || !pairs) return FV_E_ARG;
+    fv_ctx *ctx = f->ctx;
+    unsigned long long v[2] = {0, 0};
+    if (f->closure->near_cnt) {
+        FV_HIP_CHECK(ctx, hipMemcpyAsync(v, f->closure->near_cnt, sizeof v, hipMemcpyDeviceToHost,
+                                         ctx->stream));
+        FV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    *pairs = (double)v[1];
     return FV_OK;
 }
 

@@ -168,6 +168,7 @@ class VortexParticleField:
         (a fluxvortex_amd.mlp.ClosureMLP, trained online); checked against the
         exact sum every `check_every` advects and rejected above `tol`."""
         self._closure_mlp = mlp        # keep the network alive with the field
+        self._closure_on = True
         self._ctx.check(self._ctx.lib.fv_vpm_set_closure(
             self._h, mlp.handle if mlp is not None else None, float(cell), float(kappa),
             int(check_every), float(tol), int(train_samples), int(train_iters), float(lr)))
@@ -177,9 +178,13 @@ class VortexParticleField:
         expansion at each target cell (0: off; else >= 2 kappa)."""
         self._ctx.check(self._ctx.lib.fv_vpm_set_closure_local(self._h, float(kappa_local)))
 
+    def has_closure(self) -> bool:
+        return getattr(self, "_closure_on", False)
+
     def disable_closure(self):
         self._ctx.check(self._ctx.lib.fv_vpm_clear_closure(self._h))
         self._closure_mlp = None
+        self._closure_on = False
 
     def closure_stats(self):
         e, fe = C.c_double(), C.c_double()
@@ -188,8 +193,10 @@ class VortexParticleField:
         self._ctx.check(self._ctx.lib.fv_vpm_closure_stats(self._h, C.byref(e), C.byref(ch),
                                                            C.byref(rj), C.byref(act),
                                                            C.byref(fe)))
+        npr = C.c_double()
+        self._ctx.check(self._ctx.lib.fv_vpm_closure_near_pairs(self._h, C.byref(npr)))
         return {"last_err": e.value, "checks": ch.value, "rejections": rj.value,
-                "active": bool(act.value), "far_evals": fe.value}
+                "active": bool(act.value), "far_evals": fe.value, "near_pairs": npr.value}
 
     def closure_velocity(self, use_mlp=True):
         """(closure velocity at the particles, its normwise error vs the exact sum)."""

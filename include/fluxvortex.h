@@ -409,6 +409,9 @@ int fv_vpm_set_closure_local(fv_vpm *f, double kappa_local);
 int fv_vpm_clear_closure(fv_vpm *f);
 int fv_vpm_closure_stats(fv_vpm *f, double *last_err, int64_t *checks, int64_t *rejections,
                          int32_t *active, double *far_evals);
+/* Near-field pairs the closure has evaluated so far (every pass: each target
+ * particle against the particles of its cell's stencil columns; exact). */
+int fv_vpm_closure_near_pairs(fv_vpm *f, double *pairs);
 /* Closure velocity at the particles (host, n x 3; nullable) with or without
  * the learned residual and its normwise error against the exact sum. */
 int fv_vpm_closure_velocity_host(fv_vpm *f, int32_t use_mlp, double *out, double *err);

@@ -49,6 +49,13 @@ def test_near_only_is_exact(field, oracle):
     sub = np.arange(0, len(sg), 97)
     ref = oracle.particle_vel(pos[sub], pos, al, sg, "winckelmans")
     assert relerr(u[sub], ref) < 1e-12
+    # the near-pair count: every target against every particle, per pass
+    n2 = float(len(sg)) ** 2
+    npairs = f.closure_stats()["near_pairs"]
+    assert npairs > 0 and npairs % n2 == 0
+    f.enable_closure(None, cell=2.0, kappa=2.0)
+    f.closure_velocity(use_mlp=False)
+    assert 0 < f.closure_stats()["near_pairs"] < n2
 
 
 def test_near_only_clamped_vorticity_matches_oracle(ctx, oracle):
