@@ -120,6 +120,66 @@ __global__ __launch_bounds__(kLuBlock) void k_lu_solve_f64(int n, double *__rest
     if (threadIdx.x == 0 && info) info[blockIdx.x] = inf;
 }
 
+// n <= 128: the same solve by 16 wavefronts with the matrix factored in
+// registers (lu_regs); the matrix is staged through LDS (coalesced loads and
+// the bitwise comparison with the stored matrix on the way in), the factors
+// come back to LDS in LAPACK's row order, and the substitutions and the
+// stored factors are those of the kernel above.
+__global__ __launch_bounds__(kLuRegLanes) void k_lu_solve_reg(int n, double *__restrict__ A,
+                                                              double *__restrict__ B,
+                                                              int32_t *__restrict__ info, int lda,
+                                                              unsigned mat_bytes, LuReuse ru,
+                                                              int store_factors)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+    const int64_t nn = (int64_t)n * n;
+    double *a = A + (int64_t)blockIdx.x * nn;
+    double *bg = B + (int64_t)blockIdx.x * n;
+    double *w = reinterpret_cast<double *>(lds_raw);
+    double *b = reinterpret_cast<double *>(lds_raw + mat_bytes);
+    double *rdiag = b + kLuRegMax;
+    int *order = reinterpret_cast<int *>(rdiag + kLuRegMax);
+    auto *sh = reinterpret_cast<LuRegShared<double> *>(order + kLuRegMax);
+    double *key = ru.key ? ru.key + (int64_t)blockIdx.x * nn : nullptr;
+    double *fac = ru.key ? static_cast<double *>(ru.fac) + (int64_t)blockIdx.x * nn : nullptr;
+    int32_t *ord = ru.key ? ru.order + (int64_t)blockIdx.x * n : nullptr;
+    const bool check = key && ru.state[blockIdx.x] == 1;
+    const bool differ =
+        mat_move<kLuRegLanes>(a, n, w, lda, n, check ? key : (const double *)nullptr);
+    const bool reuse = !__syncthreads_or((!check || differ) ? 1 : 0);
+    int inf = 0;
+    if (reuse) {
+        mat_move<kLuRegLanes>(fac, n, w, lda, n, (const double *)nullptr);
+        for (int i = threadIdx.x; i < n; i += kLuRegLanes) order[i] = ord[i];
+        if (threadIdx.x == 0 && ru.hits) ru.hits[blockIdx.x] += 1;
+    } else {
+        if (key) {
+            mat_move<kLuRegLanes>(w, lda, key, n, n, (const double *)nullptr);
+            if (threadIdx.x == 0) ru.state[blockIdx.x] = 0;
+        }
+        inf = lu_regs<double>(w, lda, n, order, sh);
+        if (key && inf == 0) {
+            mat_move<kLuRegLanes>(w, lda, fac, n, n, (const double *)nullptr);
+            for (int i = threadIdx.x; i < n; i += kLuRegLanes) ord[i] = order[i];
+            if (threadIdx.x == 0) ru.state[blockIdx.x] = 1;
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += kLuRegLanes) {
+        b[i] = bg[order[i]];
+        rdiag[i] = 1.0 / w[i * lda + i];
+    }
+    __syncthreads();
+    if (inf == 0) {
+        wave_fwd_subst128<double, double>(w, lda, n, b);
+        wave_back_subst128<double, double>(w, lda, n, b, rdiag);
+    }
+    __syncthreads();
+    if (store_factors) mat_move<kLuRegLanes>(w, lda, a, n, n, (const double *)nullptr);
+    for (int i = threadIdx.x; i < n; i += kLuRegLanes) bg[i] = b[i];
+    if (threadIdx.x == 0 && info) info[blockIdx.x] = inf;
+}
+
 // Mixed: FP32 factors (LDS or the global scratch f32), FP64 refinement with
 // the vectors in LDS.
 template <int IN_LDS>
@@ -262,6 +322,21 @@ int dense_solve(fv_ctx *ctx, int64_t batch, int64_t n, double *A, double *rhs, i
 {
     const LuReuse ru = reuse ? *reuse : LuReuse{nullptr, nullptr, nullptr, nullptr, nullptr};
     if (batch <= 0 || n <= 0) return FV_OK;
+    if (n <= kLuRegMax) {
+        const int lda = (int)(n | 1);
+        const size_t mat = lu_align((size_t)n * lda * sizeof(double));
+        const size_t total = mat + 2 * kLuRegMax * sizeof(double) + kLuRegMax * sizeof(int) +
+                             sizeof(LuRegShared<double>);
+        FV_REQUIRE(ctx, total <= max_lds(ctx), "dense solve: %zu bytes of LDS for n = %lld",
+                   total, (long long)n);
+        TimedLaunch tl(ctx, "dense_solve");
+        hipLaunchKernelGGL(k_lu_solve_reg, dim3((unsigned)batch), dim3(kLuRegLanes),
+                           (unsigned)total, ctx->stream, (int)n, A, rhs, info, lda, (unsigned)mat,
+                           ru, store_factors ? 1 : 0);
+        FV_LAUNCH_CHECK(ctx);
+        tl.done();
+        return FV_OK;
+    }
     const size_t vec = 2 * lu_align((size_t)n * sizeof(double)) + lu_align((size_t)n * sizeof(int));
     const LuLds p = lu_lds_plan<double>(n, vec + lu_align((size_t)n * sizeof(int)), vec,
                                         max_lds(ctx));

@@ -468,6 +468,263 @@ __device__ __forceinline__ int lu_blocked(T *a, int lda, int n, int *perm, TB *b
     return info_s;
 }
 
+// ------------------------------------------------ register-resident whole LU
+// LU of an n x n matrix, n <= 128, by a workgroup of 16 wavefronts with the
+// whole matrix in registers: wavefront w holds columns w + 16 mj (mj = 0..7),
+// lane l rows l and l + 64.  Each column thus lives in one wavefront (its
+// pivot search is a wavefront reduction, no barrier) and each row in lane
+// l % 64 of every wavefront (the pivot row's entries reach each wavefront's
+// lanes with v_readlane).  As in the register panels above, rows are not
+// moved: every row carries its LAPACK row position (the same copy in every
+// wavefront), a pivot exchanges two positions, and the rows are stored at
+// their final positions at the end, so L and U come out in LAPACK's row
+// order with LAPACK's pivot choices (first maximum in position order).
+// One workgroup barrier per column: during step k the owner of column k+1
+// updates that column first, finds its pivot and publishes the pivot row's
+// lane / slot / position and every row's multiplier (double-buffered in LDS)
+// while the other wavefronts finish the update of step k.
+constexpr int kLuRegLanes = 1024;
+constexpr int kLuRegMax = 128;
+
+template <typename T>
+struct LuRegShared {
+    T l[2][2][64];   // multipliers per buffer, register slot, lane
+    int rec[2][4];   // per buffer: pivot lane, slot, position before the swap
+    int info;        // 1-based column of the first exactly zero pivot, or 0
+};
+
+// max(a, b) of doubles without the NaN canonicalisation fmax() adds (the
+// operands here are magnitudes, the sentinel -1 and zeros, never NaN).
+__device__ __forceinline__ double dmax_raw(double a, double b)
+{
+    double r;
+    asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
+// DPP move of a double whose lanes without a source (or outside ROWMASK)
+// read 0, the identity of a maximum over values >= 0.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_d0(double v)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWMASK, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWMASK, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
+
+// Largest value over the wavefront when at least one lane holds a value
+// >= 0 (lanes without a candidate pass -1).
+__device__ __forceinline__ double wave_max_nonneg(double m)
+{
+    m = dmax_raw(m, dpp_d0<0x111, 0xf>(m));
+    m = dmax_raw(m, dpp_d0<0x112, 0xf>(m));
+    m = dmax_raw(m, dpp_d0<0x114, 0xf>(m));
+    m = dmax_raw(m, dpp_d0<0x118, 0xf>(m));
+    m = dmax_raw(m, dpp_d0<0x142, 0xa>(m));
+    m = dmax_raw(m, dpp_d0<0x143, 0xc>(m));
+    return readlane_d(m, 63);
+}
+
+__device__ __forceinline__ int wave_min_i(int c)
+{
+    const int big = 0x7fffffff;
+    c = min(c, dpp_i<0x111, 0xf>(big, c));
+    c = min(c, dpp_i<0x112, 0xf>(big, c));
+    c = min(c, dpp_i<0x114, 0xf>(big, c));
+    c = min(c, dpp_i<0x118, 0xf>(big, c));
+    c = min(c, dpp_i<0x142, 0xa>(big, c));
+    c = min(c, dpp_i<0x143, 0xc>(big, c));
+    return __builtin_amdgcn_readlane(c, 63);
+}
+
+// Pivot of column jj (register column MJ of the calling wavefront, its
+// owner) among the rows at positions >= jj: the largest |value|, ties to
+// the smallest position (idamax).  One maximum reduction; the lanes holding
+// that maximum are found by a ballot, and only an exact tie between lanes
+// takes a second (position) reduction.  Stores the multipliers into the
+// column and publishes the step record into buffer buf.
+template <int MJ, typename T>
+__device__ __forceinline__ void lureg_pivot(T (&v)[2][8], const int (&pos)[2], int jj, int n,
+                                            LuRegShared<T> *sh, int buf)
+{
+    const int lane = threadIdx.x & 63;
+    // the lane's candidate: the larger |value| of its two rows, ties to the
+    // smaller position (branch-free; non-candidates enter as -1)
+    const double b0 = (pos[0] >= jj && pos[0] < n) ? fabs((double)v[0][MJ]) : -1.0;
+    const double b1 = (pos[1] >= jj && pos[1] < n) ? fabs((double)v[1][MJ]) : -1.0;
+    const bool t1 = b1 > b0 || (b1 == b0 && pos[1] < pos[0]);
+    const double best = t1 ? b1 : b0;
+    const int bq = t1 ? 1 : 0;
+    const int bp = t1 ? pos[1] : pos[0];
+    const double M = wave_max_nonneg(best);
+    const uint64_t hit = __ballot(best == M);
+    int lp;
+    if (__popcll(hit) == 1) {
+        lp = __builtin_ctzll(hit);
+    } else {
+        const int pmin = wave_min_i(best == M ? bp : 0x7fffffff);
+        lp = __builtin_ctzll(__ballot(best == M && bp == pmin));
+    }
+    const int qp = __builtin_amdgcn_readlane(bq, lp);
+    const int pl = __builtin_amdgcn_readlane(bp, lp);
+    const T pv = readlane_t(qp ? v[1][MJ] : v[0][MJ], lp);
+    const bool zero = pv == (T)0;
+    const T inv = zero ? (T)0 : (T)1 / pv;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        // rows below the pivot after its exchange take l = v / pivot, the
+        // others l = 0 (an exact no-op update)
+        const bool below = !zero && pos[q] >= jj && pos[q] < n && pos[q] != pl;
+        const T l = below ? v[q][MJ] * inv : (T)0;
+        v[q][MJ] = below ? l : v[q][MJ];
+        sh->l[buf][q][lane] = l;
+    }
+    if (lane == 0) {
+        sh->rec[buf][0] = lp;
+        sh->rec[buf][1] = qp;
+        sh->rec[buf][2] = pl;
+        if (zero && sh->info == 0) sh->info = jj + 1;
+    }
+}
+
+// v[q][mj] -= l[q] * (pivot row entry of column wv + 16 mj), QP the pivot
+// row's register slot: for every mj > MJ0 (those columns lie beyond step k
+// of block MJ0; padding columns >= n take the update too, harmlessly, so the
+// loop has no branches) and, when first, for mj == MJ0.
+template <int MJ0, int QP, typename T>
+__device__ __forceinline__ void lureg_update(T (&v)[2][8], const T (&l)[2], int lp, bool first)
+{
+    if (first) {
+        const T u = readlane_t(v[QP][MJ0], lp);
+#pragma unroll
+        for (int q = 0; q < 2; ++q) v[q][MJ0] = fma(-l[q], u, v[q][MJ0]);
+    }
+#pragma unroll
+    for (int mj = MJ0 + 1; mj < 8; ++mj) {
+        const T u = readlane_t(v[QP][mj], lp);
+#pragma unroll
+        for (int q = 0; q < 2; ++q) v[q][mj] = fma(-l[q], u, v[q][mj]);
+    }
+}
+
+// The previous step's multipliers and pivot row, kept by every wavefront:
+// the owner of the look-ahead column applies the rest of that step's update
+// one step late (after publishing its pivot), so the barrier does not wait
+// for it.
+template <typename T>
+struct LuRegPrev {
+    T l[2];
+    int lp, qp;
+};
+
+// Step k of the elimination (k / 16 == MJ0) by every wavefront (wv uniform).
+template <int MJ0, typename T>
+__device__ __forceinline__ void lureg_step(T (&v)[2][8], int (&pos)[2], int k, int n, int wv,
+                                           LuRegShared<T> *sh, LuRegPrev<T> &prev)
+{
+    const int lane = threadIdx.x & 63;
+    const int buf = k & 1;
+    const int lp = __builtin_amdgcn_readfirstlane(sh->rec[buf][0]);
+    const int qp = __builtin_amdgcn_readfirstlane(sh->rec[buf][1]);
+    const int pl = __builtin_amdgcn_readfirstlane(sh->rec[buf][2]);
+    const T l[2] = {sh->l[buf][0][lane], sh->l[buf][1][lane]};
+#pragma unroll
+    for (int q = 0; q < 2; ++q) pos[q] = pos[q] == pl ? k : (pos[q] == k ? pl : pos[q]);
+    const int kl = k & 15;
+    // the owner of column k (the previous step's look-ahead) catches up on
+    // the previous step's update of its later columns
+    if (k > 0 && kl == wv) {
+        if (prev.qp)
+            lureg_update<MJ0, 1>(v, prev.l, prev.lp, false);
+        else
+            lureg_update<MJ0, 0>(v, prev.l, prev.lp, false);
+    }
+    const int k1 = k + 1;
+    if (k1 < n && (k1 & 15) == wv) {
+        // look-ahead: column k+1 first, then its pivot; the rest of this
+        // step's update follows in the next step
+        if (kl != 15) {
+            const T u = readlane_t(qp ? v[1][MJ0] : v[0][MJ0], lp);
+#pragma unroll
+            for (int q = 0; q < 2; ++q) v[q][MJ0] = fma(-l[q], u, v[q][MJ0]);
+            lureg_pivot<MJ0>(v, pos, k1, n, sh, buf ^ 1);
+        } else if (MJ0 < 7) {
+            constexpr int M1 = MJ0 < 7 ? MJ0 + 1 : 7;
+            const T u = readlane_t(qp ? v[1][M1] : v[0][M1], lp);
+#pragma unroll
+            for (int q = 0; q < 2; ++q) v[q][M1] = fma(-l[q], u, v[q][M1]);
+            lureg_pivot<M1>(v, pos, k1, n, sh, buf ^ 1);
+        }
+    } else if (qp) {
+        lureg_update<MJ0, 1>(v, l, lp, wv > kl);
+    } else {
+        lureg_update<MJ0, 0>(v, l, lp, wv > kl);
+    }
+    prev.l[0] = l[0];
+    prev.l[1] = l[1];
+    prev.lp = lp;
+    prev.qp = qp;
+    __syncthreads();
+}
+
+// Steps 16 MJ0 .. min(n, 16 MJ0 + 16) - 1, then the next block: one loop per
+// block, so every register column index is a constant.
+template <int MJ0, typename T>
+__device__ __forceinline__ void lureg_block(T (&v)[2][8], int (&pos)[2], int n, int wv,
+                                            LuRegShared<T> *sh, LuRegPrev<T> &prev)
+{
+    const int kend = n < 16 * MJ0 + 16 ? n : 16 * MJ0 + 16;
+    for (int k = 16 * MJ0; k < kend; ++k) lureg_step<MJ0>(v, pos, k, n, wv, sh, prev);
+    if constexpr (MJ0 < 7) {
+        if (n > 16 * MJ0 + 16) lureg_block<MJ0 + 1>(v, pos, n, wv, sh, prev);
+    }
+}
+
+// Factors the n x n matrix a (row stride lda, n <= 128; LDS or global) by a
+// workgroup of kLuRegLanes lanes, in place: on return a holds L and U in
+// LAPACK's row order and order[i] the original row of factor row i
+// ((P v)[i] = v[order[i]]).  Returns the 1-based column of the first exactly
+// zero pivot (the factorization continues past it, as LAPACK's), or 0.
+template <typename T>
+__device__ __forceinline__ int lu_regs(T *a, int lda, int n, int *order, LuRegShared<T> *sh)
+{
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    constexpr int kNone = 1 << 30;
+    T v[2][8];
+    int pos[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int r = lane + 64 * q;
+        pos[q] = r < n ? r : kNone;
+#pragma unroll
+        for (int mj = 0; mj < 8; ++mj) {
+            const int j = wv + 16 * mj;
+            v[q][mj] = (r < n && j < n) ? a[r * lda + j] : (T)0;
+        }
+    }
+    if (threadIdx.x == 0) sh->info = 0;
+    if (wv == 0) lureg_pivot<0>(v, pos, 0, n, sh, 0);
+    __syncthreads();
+    LuRegPrev<T> prev{{(T)0, (T)0}, 0, 0};
+    lureg_block<0>(v, pos, n, wv, sh, prev);
+    // every row to its final position (a was read before the first barrier)
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        if (pos[q] < n) {
+#pragma unroll
+            for (int mj = 0; mj < 8; ++mj) {
+                const int j = wv + 16 * mj;
+                if (j < n) a[pos[q] * lda + j] = v[q][mj];
+            }
+            if (wv == 0) order[pos[q]] = lane + 64 * q;
+        }
+    }
+    const int inf = sh->info;
+    __syncthreads();
+    return inf;
+}
+
 // Register form of the substitutions (n <= 256): lane l holds x[l + 64 q].
 // Blocks of 16 columns: the block's own rows are solved sequentially from
 // the 16 x 16 diagonal block held in registers (x broadcast with
@@ -600,6 +857,116 @@ __device__ __forceinline__ void wave_fwd_subst(const T *a, int lda, int n, TB *x
         for (int i = k + 1 + lane; i < n; i += 64) x[i] -= (TB)a[(int64_t)i * lda + k] * xk;
         wave_sync();
     }
+}
+
+// Substitutions for n <= 128 by wave 0, x[i] in lane i % 64, slot i / 64 of
+// registers: column-oriented, so the dependency chain per column is one
+// broadcast (v_readlane of the finished entry) and one FMA; the column's
+// coefficients are loaded from LDS in groups of 8 columns ahead of the chain
+// and masked off it (rows the column must not change take 0, an exact no-op
+// apart from the sign of a zero).  Loads of padding rows (>= n) read row
+// n - 1; their results are never stored.
+//   forward:  y = L^{-1} x (unit lower triangle)
+//   backward: x = U^{-1} y with y_k scaled by rdiag[k] = 1 / U_kk; the
+//             update of row i by column k takes (U_ik rdiag[k]) y_k, and
+//             x_i = y_i rdiag[i] at the end.
+template <typename T, typename TB>
+__device__ __forceinline__ void wave_fwd_subst128(const T *a, int lda, int n, TB *x)
+{
+    if ((threadIdx.x >> 6) != 0) return;
+    const int lane = threadIdx.x & 63;
+    const T *c0 = a + (lane < n ? lane : n - 1) * lda;
+    const T *c1 = a + (lane + 64 < n ? lane + 64 : n - 1) * lda;
+    TB x0 = lane < n ? x[lane] : (TB)0;
+    TB x1 = lane + 64 < n ? x[lane + 64] : (TB)0;
+    const int last = n - 1;                      // columns 0 .. n-2 update
+    const int kA = last < 64 ? last : 64;        // columns whose entry sits in slot 0
+    int k = 0;
+    for (; k + 8 <= kA; k += 8) {
+        TB l0[8], l1[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            l0[j] = lane > k + j ? (TB)c0[k + j] : (TB)0;
+            l1[j] = (TB)c1[k + j];
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const TB xk = readlane_t(x0, k + j);
+            x0 = fma(-l0[j], xk, x0);
+            x1 = fma(-l1[j], xk, x1);
+        }
+    }
+    for (; k < kA; ++k) {
+        const TB l0 = lane > k ? (TB)c0[k] : (TB)0, l1 = (TB)c1[k];
+        const TB xk = readlane_t(x0, k);
+        x0 = fma(-l0, xk, x0);
+        x1 = fma(-l1, xk, x1);
+    }
+    for (; k + 8 <= last; k += 8) {
+        TB l1[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) l1[j] = lane + 64 > k + j ? (TB)c1[k + j] : (TB)0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) x1 = fma(-l1[j], readlane_t(x1, k + j - 64), x1);
+    }
+    for (; k < last; ++k) {
+        const TB l1 = lane + 64 > k ? (TB)c1[k] : (TB)0;
+        x1 = fma(-l1, readlane_t(x1, k - 64), x1);
+    }
+    if (lane < n) x[lane] = x0;
+    if (lane + 64 < n) x[lane + 64] = x1;
+    wave_sync();
+}
+
+template <typename T, typename TB>
+__device__ __forceinline__ void wave_back_subst128(const T *a, int lda, int n, TB *x,
+                                                   const TB *rdiag)
+{
+    if ((threadIdx.x >> 6) != 0) return;
+    const int lane = threadIdx.x & 63;
+    const T *c0 = a + (lane < n ? lane : n - 1) * lda;
+    const T *c1 = a + (lane + 64 < n ? lane + 64 : n - 1) * lda;
+    TB x0 = lane < n ? x[lane] : (TB)0;
+    TB x1 = lane + 64 < n ? x[lane + 64] : (TB)0;
+    // columns n-1 .. 64 (entry in slot 1): rows above in both slots
+    int k = n - 1;
+    for (; k - 8 >= 63; k -= 8) {
+        TB u0[8], u1[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const TB rd = rdiag[k - j];
+            u0[j] = (TB)c0[k - j] * rd;
+            u1[j] = lane + 64 < k - j ? (TB)c1[k - j] * rd : (TB)0;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const TB yk = readlane_t(x1, k - j - 64);
+            x0 = fma(-u0[j], yk, x0);
+            x1 = fma(-u1[j], yk, x1);
+        }
+    }
+    for (; k >= 64; --k) {
+        const TB rd = rdiag[k];
+        const TB u0 = (TB)c0[k] * rd, u1 = lane + 64 < k ? (TB)c1[k] * rd : (TB)0;
+        const TB yk = readlane_t(x1, k - 64);
+        x0 = fma(-u0, yk, x0);
+        x1 = fma(-u1, yk, x1);
+    }
+    // columns min(n-1, 63) .. 1 (entry in slot 0): rows above in slot 0
+    for (; k - 8 >= 0; k -= 8) {
+        TB u0[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) u0[j] = lane < k - j ? (TB)c0[k - j] * rdiag[k - j] : (TB)0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) x0 = fma(-u0[j], readlane_t(x0, k - j), x0);
+    }
+    for (; k >= 1; --k) {
+        const TB u0 = lane < k ? (TB)c0[k] * rdiag[k] : (TB)0;
+        x0 = fma(-u0, readlane_t(x0, k), x0);
+    }
+    if (lane < n) x[lane] = x0 * rdiag[lane];
+    if (lane + 64 < n) x[lane + 64] = x1 * rdiag[lane + 64];
+    wave_sync();
 }
 
 }  // namespace fv
