@@ -301,6 +301,134 @@ __global__ __launch_bounds__(kLuBlock) void k_lu_solve_mixed(
     }
 }
 
+// Mixed, n <= 128: FP32 factors from the register factorization (lu_regs),
+// the substitutions in FP64 on them, and the FP64 residual r = b - A x from
+// the matrix held in the same register layout (lane: rows l, l + 64;
+// wavefront: columns w + 16 mj), the 16 wavefronts' partial row sums added in
+// LDS in wavefront order.  Refinement as in the kernel above.
+__global__ __launch_bounds__(kLuRegLanes) void k_lu_solve_mixed_reg(
+    int n, const double *__restrict__ A, double *__restrict__ B, int max_iter, double tol,
+    int32_t *__restrict__ info, int32_t *__restrict__ iters, int lda, unsigned mat_bytes,
+    LuReuse ru)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t nn = (int64_t)n * n;
+    const double *a = A + (int64_t)blockIdx.x * nn;
+    double *bg = B + (int64_t)blockIdx.x * n;
+    float *lu = reinterpret_cast<float *>(lds_raw);
+    double *x = reinterpret_cast<double *>(lds_raw + mat_bytes);
+    double *r = x + kLuRegMax;
+    double *d = r + kLuRegMax;
+    double *rdiag = d + kLuRegMax;
+    double *part = rdiag + kLuRegMax;                        // [16][128] partial row sums
+    int *order = reinterpret_cast<int *>(part + 16 * kLuRegMax);
+    auto *sh = reinterpret_cast<LuRegShared<float> *>(order + kLuRegMax);
+    __shared__ double stat[2];
+    double *key = ru.key ? ru.key + (int64_t)blockIdx.x * nn : nullptr;
+    float *fac = ru.key ? static_cast<float *>(ru.fac) + (int64_t)blockIdx.x * nn : nullptr;
+    int32_t *ord = ru.key ? ru.order + (int64_t)blockIdx.x * n : nullptr;
+    const bool check = key && ru.state[blockIdx.x] == 1;
+    const bool differ =
+        mat_move<kLuRegLanes>(a, n, lu, lda, n, check ? key : (const double *)nullptr);
+    const bool reuse = !__syncthreads_or((!check || differ) ? 1 : 0);
+    if (reuse) {
+        mat_move<kLuRegLanes>(fac, n, lu, lda, n, (const double *)nullptr);
+        for (int i = threadIdx.x; i < n; i += kLuRegLanes) order[i] = ord[i];
+        if (threadIdx.x == 0 && ru.hits) ru.hits[blockIdx.x] += 1;
+    } else {
+        if (key) {
+            mat_move<kLuRegLanes>(a, n, key, n, n, (const double *)nullptr);
+            if (threadIdx.x == 0) ru.state[blockIdx.x] = 0;
+        }
+        const int inf = lu_regs<float>(lu, lda, n, order, sh);
+        if (inf != 0) {
+            if (threadIdx.x == 0) {
+                if (info) info[blockIdx.x] = inf;
+                if (iters) iters[blockIdx.x] = 0;
+            }
+            return;
+        }
+        if (key) {
+            mat_move<kLuRegLanes>(lu, lda, fac, n, n, (const double *)nullptr);
+            for (int i = threadIdx.x; i < n; i += kLuRegLanes) ord[i] = order[i];
+            if (threadIdx.x == 0) ru.state[blockIdx.x] = 1;
+        }
+    }
+    // the FP64 matrix in the register layout, for the residuals
+    double av[2][8];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int row = lane + 64 * q;
+#pragma unroll
+        for (int mj = 0; mj < 8; ++mj) {
+            const int j = wv + 16 * mj;
+            av[q][mj] = (row < n && j < n) ? a[row * n + j] : 0.0;
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += kLuRegLanes) {
+        x[i] = 0.0;
+        r[i] = bg[i];
+        rdiag[i] = 1.0 / (double)lu[i * lda + i];
+    }
+    __syncthreads();
+    int it = 0;
+    for (; it <= max_iter; ++it) {
+        // correction d = U32^{-1} L32^{-1} P r (carried in FP64 storage)
+        for (int i = threadIdx.x; i < n; i += kLuRegLanes) d[i] = r[order[i]];
+        __syncthreads();
+        wave_fwd_subst128<float, double>(lu, lda, n, d);
+        wave_back_subst128<float, double>(lu, lda, n, d, rdiag);
+        __syncthreads();
+        if (wv == 0) {
+            double dmax = 0.0, xmax = 0.0;
+            for (int i = lane; i < n; i += 64) {
+                x[i] += d[i];
+                dmax = fmax(dmax, fabs(d[i]));
+                xmax = fmax(xmax, fabs(x[i]));
+            }
+            for (int off = 32; off > 0; off >>= 1) {
+                dmax = fmax(dmax, __shfl_xor(dmax, off, 64));
+                xmax = fmax(xmax, __shfl_xor(xmax, off, 64));
+            }
+            if (lane == 0) {
+                stat[0] = dmax;
+                stat[1] = xmax;
+            }
+        }
+        __syncthreads();
+        const bool done = stat[0] <= tol * stat[1];
+        if (done || it == max_iter) break;
+        // FP64 residual r = b - A x
+        double p0 = 0.0, p1 = 0.0;
+#pragma unroll
+        for (int mj = 0; mj < 8; ++mj) {
+            const int j = wv + 16 * mj;
+            const double xj = j < n ? x[j] : 0.0;
+            p0 = fma(av[0][mj], xj, p0);
+            p1 = fma(av[1][mj], xj, p1);
+        }
+        part[wv * kLuRegMax + lane] = p0;
+        part[wv * kLuRegMax + lane + 64] = p1;
+        __syncthreads();
+        for (int i = threadIdx.x; i < n; i += kLuRegLanes) {
+            double s = 0.0;
+#pragma unroll
+            for (int w = 0; w < 16; ++w) s += part[w * kLuRegMax + i];
+            r[i] = bg[i] - s;
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += kLuRegLanes) bg[i] = x[i];
+    if (threadIdx.x == 0) {
+        if (info) info[blockIdx.x] = 0;
+        if (iters) iters[blockIdx.x] = it;
+    }
+}
+
 // Dynamic LDS a solve may take: the device's per-workgroup limit less a
 // margin for the kernels' static LDS (a few hundred bytes).
 constexpr size_t kLuStaticLds = 1024;
@@ -362,6 +490,21 @@ int dense_solve_mixed(fv_ctx *ctx, int64_t batch, int64_t n, const double *A, do
 {
     const LuReuse ru = reuse ? *reuse : LuReuse{nullptr, nullptr, nullptr, nullptr, nullptr};
     if (batch <= 0 || n <= 0) return FV_OK;
+    if (n <= kLuRegMax) {
+        const int lda = (int)(n | 1);
+        const size_t mat = lu_align((size_t)n * lda * sizeof(float));
+        const size_t total = mat + (4 + 16) * kLuRegMax * sizeof(double) +
+                             kLuRegMax * sizeof(int) + sizeof(LuRegShared<float>);
+        FV_REQUIRE(ctx, total <= max_lds(ctx), "dense solve: %zu bytes of LDS for n = %lld",
+                   total, (long long)n);
+        TimedLaunch tl(ctx, "dense_solve");
+        hipLaunchKernelGGL(k_lu_solve_mixed_reg, dim3((unsigned)batch), dim3(kLuRegLanes),
+                           (unsigned)total, ctx->stream, (int)n, A, rhs, max_iter, tol, info,
+                           iters, lda, (unsigned)mat, ru);
+        FV_LAUNCH_CHECK(ctx);
+        tl.done();
+        return FV_OK;
+    }
     const size_t vec = 4 * lu_align((size_t)n * sizeof(double)) +
                        2 * lu_align((size_t)n * sizeof(int));
     const LuLds p = lu_lds_plan<float>(n, vec, vec, max_lds(ctx));
