@@ -5,22 +5,25 @@
 // PteraSoftware UVLM run coupled to BeamFE by AeroelasticSolver
 // (src/fluxvortex/aeroelastic_solver.py:63-221, beam_fe.py:231-351).  Here
 // every airspeed is an instance of the hybrid N_keep step and all instances
-// advance together, two launches per step:
+// advance together, four launches per step.  The first three spread the
+// pair work of every instance over the whole device:
+//   k_sw_geom    AIC (unit rings . normals at the collocation points) and the
+//                unit-strength bound-ring velocities at the leg centres of
+//                the step's lattice, one thread per entry;
+//   k_sw_wake    wake-ring velocities at the collocation points, the leg
+//                centres (and free wake vertices), one wavefront per
+//                (instance, segment chunk, 64 targets), partial sums per chunk;
 //   k_sw_induce  particle -> collocation-point velocities, one workgroup per
-//                (instance, 1024-particle chunk), partial sums per chunk;
-//   k_sw_step    one workgroup per instance for the rest of the step, every
-//                array in LDS: lattice deformation, rings / frames / legs,
-//                AIC, wake-ring velocities, right-hand side, LU solve, leg
-//                velocities and Kutta-Joukowski loads, CL / CDi, station loads
-//                -> beam nodes, one Newmark step, the next lattice's
-//                deformation, wake shedding, N_keep conversion, particle
-//                convection and the next step's collocation points.
-// Instances keeping more than kSwMaxR - 1 ring rows (up to the whole ring
-// wake, the reference benchmark's configuration) keep the wake in HBM and add
-// a third launch ahead of the step:
-//   k_sw_wake    wake-ring velocities at the collocation points and leg
-//                centres of the step's lattice, one wavefront per (instance,
-//                512-segment chunk, 64 targets), partial sums per chunk.
+//                (instance, 1024-particle chunk), partial sums per chunk; it
+//                also applies the previous step's prescribed-wake convection
+//                (x += V dt) to the particles it stages;
+// and one workgroup per instance then runs the serial rest of the step with
+// its arrays in LDS:
+//   k_sw_step    rings / frames / legs, right-hand side, LU solve (factor
+//                reuse when the AIC is unchanged), leg velocities, Kutta-
+//                Joukowski loads, CL / CDi, station loads -> beam nodes, one
+//                Newmark step, the next lattice, wake shedding, N_keep
+//                conversion.
 // The launches are captured once in a hipGraph and replayed per step (every
 // per-step quantity lives in device memory).
 //
@@ -44,14 +47,16 @@
 
 namespace fv {
 
-constexpr int kSwMaxNc = 16, kSwMaxNs = 32, kSwMaxP = 64, kSwMaxR = 16, kSwMaxNodes = 17;
+constexpr int kSwMaxNc = 16, kSwMaxNs = 32, kSwMaxP = 64, kSwMaxNodes = 17;
 constexpr int kSwMaxFree = 3 * (kSwMaxNodes - 1);
 constexpr int kSwMaxL = 3 * kSwMaxP + kSwMaxNs;   // right / front / left legs + trailing-edge back legs
 constexpr int kSwBlock = 256;       // induction workgroup
 constexpr int kSwStepBlock = 1024;  // step workgroup: 16 waves, one instance per CU
+constexpr int kSwGeomBlock = 256;   // geometry workgroup: one AIC / leg-influence entry per lane
 constexpr int kSwChunk = 1024;   // particles per induction partial
-constexpr int kSwParts = 8;      // segment partial sums per target
-constexpr int kSwWChunk = 256;   // wake segments per partial sum (HBM wake)
+constexpr int kSwStreams = 4;    // source streams per induction wavefront (targets <= 64 / streams)
+constexpr int kSwWChunkSmall = 16;    // wake segments per partial sum (wakes up to 64 chunks)
+constexpr int kSwWChunkLarge = 256;   // wake segments per partial sum (longer wakes)
 constexpr int kSwWBlock = 64;    // wake induction: one wavefront,
 constexpr int kSwWTpt = 1;       // one target per lane (many small workgroups)
 constexpr int kSwWTile = 64;     // wake segments staged in LDS at a time
@@ -60,7 +65,8 @@ struct SwInst {
     int nc, ns, P, nv, R, n_keep, kernel, couple, n_nodes, nfree;
     int status;                  // 0; k + 1: zero pivot at step k; -2: particle overflow
     int has_prev, par, nrow, has_wprev, prec, feedback;
-    int hbm;                     // wake rings in HBM, velocities from k_sw_wake partials
+    int wchunk;                  // wake segments per k_sw_wake partial sum
+    int shift;                   // the particles still owe this step's x += V dt
     int free_wake;               // free ring wake: vertex rows move with V + u
     int truncate;                // rings whose corner-mean x exceeds trunc_x lose Gamma
     int64_t num_steps, step, cap, np, max_chunks, max_wchunks, max_targets;
@@ -70,9 +76,11 @@ struct SwInst {
     int64_t o_grid0, o_coln, o_colp, o_legp, o_gamp, o_gam, o_wv[2], o_wg[2], o_wa[2];
     int64_t o_pos, o_al, o_sg, o_part, o_M, o_C, o_K, o_d, o_v, o_a;
     int64_t o_wdef, o_tdef, o_wprev, o_tprev, o_yn, o_cl, o_cdi, o_tw, o_tt;
-    int64_t o_legn, o_wpart;     // the step's leg centres; wake partial sums (HBM wake)
+    int64_t o_legn, o_wpart;     // the step's leg centres; wake partial sums
     int64_t o_ten, o_vvel;       // the step's trailing-edge points; free-vertex velocities
     int64_t o_wsum;              // the wake partial sums added up, 3 per target
+    int64_t o_gcur;              // the step's lattice (3 nv)
+    int64_t o_aic, o_bleg;       // k_sw_geom: AIC (P x P), leg influences (NL x P x 3)
     // factor reuse of the circulation solve: the AIC last factored, its
     // factors, the composed row order (as doubles) and a state word (1.0:
     // valid); a step whose AIC equals it bit for bit (a lattice that does
@@ -213,45 +221,63 @@ __device__ void sw_leg_centres(const double *g, int nc, int ns, int P, int p, do
 }
 
 // ----------------------------------------------------------- particle induction
+// Lane l of a wavefront takes target l % P of source stream l / P (streams =
+// min(kSwStreams, 64 / P) per wavefront, so P <= 32 keeps every lane busy and
+// streams x P <= 256); the streams' sums are added in stream order.  The particles of the chunk are
+// staged once, with the previous step's convection x += V dt applied and
+// written back when the instance owes it.
 template <int KERNEL, int PREC>
-__device__ void sw_induce_body(const SwInst &I, const double *__restrict__ S, double *part,
+__device__ void sw_induce_body(const SwInst &I, double *__restrict__ S, double *part,
                                int64_t chunk, PartRec *tile, double *tgt, double *red)
 {
     const int t = threadIdx.x, P = I.P;
+    const int G = (64 / P < kSwStreams) ? 64 / P : kSwStreams;
+    const int lane = t & 63, tp = lane % P, gs = lane / P;
+    const bool act = gs < G;
+    const int stream = (t >> 6) * G + gs, nstream = (kSwBlock / 64) * G;
     const int64_t c0 = chunk * kSwChunk;
     const int64_t c1 = (c0 + kSwChunk < I.np) ? c0 + kSwChunk : I.np;
     for (int e = t; e < 3 * P; e += kSwBlock) tgt[e] = S[I.o_coln + e];
-    const int tp = t & 63, grp = t >> 6;
     using Op = PartOp<KERNEL, PREC, 0>;
     typename Op::Tgt tg;
     typename Op::Acc acc;
     Op::init(acc);
     __syncthreads();
-    if (tp < P) {
-        tg.x = tgt[3 * tp]; tg.y = tgt[3 * tp + 1]; tg.z = tgt[3 * tp + 2];
-        tg.ax = tg.ay = tg.az = 0.0;
-    }
-    const double *pos = S + I.o_pos, *al = S + I.o_al, *sg = S + I.o_sg;
+    tg.x = tgt[3 * tp]; tg.y = tgt[3 * tp + 1]; tg.z = tgt[3 * tp + 2];
+    tg.ax = tg.ay = tg.az = 0.0;
+    double *pos = S + I.o_pos;
+    const double *al = S + I.o_al, *sg = S + I.o_sg;
+    const bool shift = I.shift != 0;
+    const double ox = I.V[0] * I.dt, oy = I.V[1] * I.dt, oz = I.V[2] * I.dt;
     for (int64_t b0 = c0; b0 < c1; b0 += kSwBlock) {
         const int cnt = (int)((c1 - b0) < kSwBlock ? (c1 - b0) : kSwBlock);
         __syncthreads();
         if (t < cnt) {
             const int64_t j = b0 + t;
-            tile[t] = make_part_rec<PREC>(KERNEL, pos[3 * j], pos[3 * j + 1], pos[3 * j + 2],
-                                          al[3 * j], al[3 * j + 1], al[3 * j + 2], sg[j]);
+            double x = pos[3 * j], y = pos[3 * j + 1], z = pos[3 * j + 2];
+            if (shift) {
+                x += ox;
+                y += oy;
+                z += oz;
+                pos[3 * j] = x;
+                pos[3 * j + 1] = y;
+                pos[3 * j + 2] = z;
+            }
+            tile[t] = make_part_rec<PREC>(KERNEL, x, y, z, al[3 * j], al[3 * j + 1], al[3 * j + 2],
+                                          sg[j]);
         }
         __syncthreads();
-        if (tp < P)
-            for (int k = grp; k < cnt; k += kSwBlock / 64) Op::pair(tg, tile[k], acc);
+        if (act)
+            for (int k = stream; k < cnt; k += nstream) Op::pair(tg, tile[k], acc);
     }
     Op::finish(tg, acc);
-    if (tp < P)
-        for (int c = 0; c < 3; ++c) red[(grp * kSwMaxP + tp) * 3 + c] = acc.v[c];
+    if (act)
+        for (int c = 0; c < 3; ++c) red[(stream * P + tp) * 3 + c] = acc.v[c];
     __syncthreads();
     if (t < P)
         for (int c = 0; c < 3; ++c) {
             double s = 0.0;
-            for (int g = 0; g < kSwBlock / 64; ++g) s += red[(g * kSwMaxP + t) * 3 + c];
+            for (int g = 0; g < nstream; ++g) s += red[(g * P + t) * 3 + c];
             part[(chunk * P + t) * 3 + c] = s;
         }
 }
@@ -261,7 +287,7 @@ __global__ __launch_bounds__(kSwBlock) void k_sw_induce(const SwInst *__restrict
 {
     __shared__ PartRec tile[kSwBlock];
     __shared__ double tgt[3 * kSwMaxP];
-    __shared__ double red[(kSwBlock / 64) * kSwMaxP * 3];
+    __shared__ double red[(kSwBlock / 64) * kSwMaxP * 3];   // streams x P <= 4 x 64
     const SwInst &I = insts[blockIdx.y];
     if (I.status != 0 || I.step >= I.num_steps) return;
     const int64_t chunk = blockIdx.x;
@@ -277,7 +303,7 @@ __global__ __launch_bounds__(kSwBlock) void k_sw_induce(const SwInst *__restrict
     }
 }
 
-// ------------------------------------------------------------ HBM ring wake
+// ------------------------------------------------------------------ ring wake
 // Wake-ring velocities at the T = P + NL + NV targets of the step (collocation
 // points, the leg centres in slot order, then with a free ring wake the NV =
 // nfree (ns + 1) vertices of rows 0 .. nfree - 1) from the segments [c0, c1)
@@ -288,14 +314,14 @@ __global__ __launch_bounds__(kSwWBlock) void k_sw_wake(const SwInst *__restrict_
 {
     __shared__ SegRec tile[kSwWTile];
     const SwInst &I = insts[blockIdx.y];
-    if (!I.hbm || I.status != 0 || I.step >= I.num_steps) return;
+    if (I.status != 0 || I.step >= I.num_steps) return;
     const int ns = I.ns, P = I.P, NL = 3 * P + ns;
     const int T = P + NL + sw_nfree(I) * (ns + 1);
     const int64_t nseg = 4 * (int64_t)I.nrow * ns;
-    const int64_t c0 = (int64_t)blockIdx.x * kSwWChunk;
+    const int64_t c0 = (int64_t)blockIdx.x * I.wchunk;
     const int tz = (int)blockIdx.z * (kSwWBlock * kSwWTpt);
     if (c0 >= nseg || tz >= T) return;   // uniform over the workgroup
-    const int64_t c1 = (c0 + kSwWChunk < nseg) ? c0 + kSwWChunk : nseg;
+    const int64_t c1 = (c0 + I.wchunk < nseg) ? c0 + I.wchunk : nseg;
     const double *wv = S + I.o_wv[I.par], *wg = S + I.o_wg[I.par], *wa = S + I.o_wa[I.par];
     constexpr int kTpt = kSwWTpt;
     double u[kTpt][3];
@@ -354,7 +380,6 @@ __global__ __launch_bounds__(kSwWBlock) void k_sw_wake(const SwInst *__restrict_
 // LDS plan (doubles) of k_sw_step at the limits above.
 constexpr int kSwNv = (kSwMaxNc + 1) * (kSwMaxNs + 1);
 constexpr int kSwLdaMax = kSwMaxP | 1;
-constexpr int kSwWv = (kSwMaxR + 1) * (kSwMaxNs + 1);
 constexpr int kL_g = 0;                                   // grid 3 nv
 constexpr int kL_ring = kL_g + 3 * kSwNv;                 // Br Fr Fl Bl 12 P
 constexpr int kL_col = kL_ring + 12 * kSwMaxP;            // collocation 3 P
@@ -368,17 +393,67 @@ constexpr int kL_aic = kL_vml + 3 * kSwMaxL;              // AIC P x lda
 constexpr int kL_rhs = kL_aic + kSwMaxP * kSwLdaMax;      // rhs / gamma P
 constexpr int kL_rdg = kL_rhs + kSwMaxP;                  // 1 / diag P
 constexpr int kL_perm = kL_rdg + kSwMaxP;                 // pivots, row order (int) 2 P
-constexpr int kL_red = kL_perm + kSwMaxP;                 // partial sums NL x parts x 3
-constexpr int kL_vw = kL_red + 3 * kSwMaxL * kSwParts;    // wake / particle velocity 3 P
-constexpr int kL_vl = kL_vw + 3 * kSwMaxP;                // leg velocities 3 NL
+constexpr int kL_vl = kL_perm + kSwMaxP;                  // leg velocities 3 NL
 constexpr int kL_F = kL_vl + 3 * kSwMaxL;                 // panel forces 3 P
 constexpr int kL_PF = kL_F + 3 * kSwMaxP;                 // panel shares, front-leg forces 6 P
-constexpr int kL_wv = kL_PF + 6 * kSwMaxP;                // wake vertices 3 wv
-constexpr int kL_wg = kL_wv + 3 * kSwWv;                  // wake strengths R ns
-constexpr int kL_wa = kL_wg + kSwMaxR * kSwMaxNs;         // wake ages R ns
-constexpr int kL_beam = kL_wa + kSwMaxR * kSwMaxNs;       // nodal forces / vectors
+constexpr int kL_st = kL_PF + 6 * kSwMaxP;                // station lift, moment, node, xi 4 ns
+constexpr int kL_mck = kL_st + 4 * kSwMaxNs;              // Newmark M, C, K^-1 3 nf^2
+constexpr int kL_dva = kL_mck + 3 * kSwMaxFree * kSwMaxFree;   // Newmark d, v, a 3 nf
+constexpr int kL_beam = kL_dva + 3 * kSwMaxFree;          // nodal forces / vectors
 constexpr int kL_end = kL_beam + 3 * kSwMaxNodes + 4 * kSwMaxFree + 4 * kSwMaxNodes + 8;
 constexpr size_t kSwLdsBytes = sizeof(double) * kL_end;
+static_assert(kSwLdsBytes + 2048 <= 160 * 1024, "k_sw_step LDS plan exceeds the 160 KB of a CU");
+
+// ------------------------------------------------------------- step geometry
+// The step's lattice (S[o_gcur]) -> entry e < P^2: the AIC, A[i][j] = (ring
+// j's four unit-strength legs at collocation point i) . n_i, in the instance's
+// precision; entry P^2 + q P + j: the velocity of ring j's unit legs at leg
+// centre q, stored at o_bleg + (j NL + q) 3 (k_sw_step contracts it with the
+// step's circulations; the bound rings have age 0, so their cores do not
+// depend on the circulation and the velocity is linear in it).
+__global__ __launch_bounds__(kSwGeomBlock) void k_sw_geom(const SwInst *__restrict__ insts,
+                                                         double *__restrict__ S)
+{
+    __shared__ double g[3 * kSwNv];
+    __shared__ double ring[12 * kSwMaxP];
+    __shared__ double col[3 * kSwMaxP], nrm[3 * kSwMaxP];
+    __shared__ double lc[3 * kSwMaxL];
+    const SwInst &I = insts[blockIdx.y];
+    if (I.status != 0 || I.step >= I.num_steps) return;
+    const int nc = I.nc, ns = I.ns, P = I.P, nv = I.nv, NL = 3 * P + ns;
+    const int E = P * P + NL * P;
+    if ((int)blockIdx.x * kSwGeomBlock >= E) return;   // uniform over the workgroup
+    const int t = threadIdx.x;
+    for (int v = t; v < 3 * nv; v += kSwGeomBlock) g[v] = S[I.o_gcur + v];
+    __syncthreads();
+    for (int p = t; p < P; p += kSwGeomBlock) {
+        const int i = p / ns, j = p % ns;
+        double *Br = ring + 12 * p;
+        sw_ring(g, nc, ns, i, j, Br, Br + 3, Br + 6, Br + 9);
+        double ar;
+        sw_frame(g, ns, i, j, col + 3 * p, nrm + 3 * p, &ar);
+        sw_leg_centres(g, nc, ns, P, p, lc);
+    }
+    __syncthreads();
+    const int e = (int)blockIdx.x * kSwGeomBlock + t;
+    if (e >= E) return;
+    const bool aic = e < P * P;
+    const int f = aic ? e : e - P * P;
+    const int q = f / P, j = f % P;   // AIC: row (collocation point) q; else leg centre q
+    const double *x = aic ? col + 3 * q : lc + 3 * q;
+    const double *R = ring + 12 * j;
+    double u[3] = {0.0, 0.0, 0.0};
+    for (int k = 0; k < 4; ++k) {
+        const SegRec sr = sw_seg(R + 3 * k, R + 3 * ((k + 1) & 3), 1.0, I.rc0b, 0.0, I.nu);
+        sw_pair(I.prec, x, sr, u);
+    }
+    if (aic) {
+        S[I.o_aic + e] = u[0] * nrm[3 * q] + u[1] * nrm[3 * q + 1] + u[2] * nrm[3 * q + 2];
+    } else {
+        double *b = S + I.o_bleg + ((int64_t)j * NL + q) * 3;
+        for (int c = 0; c < 3; ++c) b[c] = u[c];
+    }
+}
 
 __global__ __launch_bounds__(kSwStepBlock) void k_sw_step(SwInst *__restrict__ insts,
                                                       double *__restrict__ S)
@@ -400,20 +475,34 @@ __global__ __launch_bounds__(kSwStepBlock) void k_sw_step(SwInst *__restrict__ i
     double *area = L + kL_area, *vmc = L + kL_vmc, *lc = L + kL_lc, *lv = L + kL_lv;
     double *vml = L + kL_vml, *A = L + kL_aic, *rhs = L + kL_rhs, *rdg = L + kL_rdg;
     int *perm = reinterpret_cast<int *>(L + kL_perm);
-    double *red = L + kL_red, *vw = L + kL_vw, *vl = L + kL_vl, *F = L + kL_F;
+    double *vl = L + kL_vl, *F = L + kL_F;
     double *PF = L + kL_PF, *Ff = PF + 3 * kSwMaxP;
-    const bool hbm = I.hbm != 0;
-    // the wake of this step: staged in LDS, or read in place (HBM wake)
-    const double *wv = hbm ? S + I.o_wv[I.par] : L + kL_wv;
-    const double *wg = hbm ? S + I.o_wg[I.par] : L + kL_wg;
-    const double *wa = hbm ? S + I.o_wa[I.par] : L + kL_wa;
+    double *Mb = L + kL_mck, *Cb = Mb + nf * nf, *Kb = Cb + nf * nf;
+    double *db = L + kL_dva, *vb = db + nf, *ab = vb + nf;
+    // the wake of this step, read in place
+    const int nrow = I.nrow, cur = I.par;
+    const double *wv = S + I.o_wv[cur], *wg = S + I.o_wg[cur], *wa = S + I.o_wa[cur];
     double *bm = L + kL_beam;
     const double *grid0 = S + I.o_grid0, *yn = S + I.o_yn;
     double *wdef = S + I.o_wdef, *tdef = S + I.o_tdef;
     const int lda = P | 1;
     const int NL = 3 * P + ns;   // leg slots: 3 p + l, then 3 P + j for the back leg of (nc-1, j)
-    // ---- lattice of this step
-    for (int v = t; v < nv; v += kSwStepBlock) sw_deform(grid0 + 3 * v, yn, wdef, tdef, nn, I.x_ea, g + 3 * v);
+    const bool couple = I.couple && s >= 1 && s <= I.num_steps - 2;
+    // ---- lattice of this step (written by the previous step or k_sw_init)
+    // and the Newmark matrices / state, staged in LDS
+    for (int v = t; v < 3 * nv; v += kSwStepBlock) g[v] = S[I.o_gcur + v];
+    if (couple) {
+        for (int e = t; e < nf * nf; e += kSwStepBlock) {
+            Mb[e] = S[I.o_M + e];
+            Cb[e] = S[I.o_C + e];
+            Kb[e] = S[I.o_K + e];
+        }
+        for (int r = t; r < nf; r += kSwStepBlock) {
+            db[r] = S[I.o_d + r];
+            vb[r] = S[I.o_v + r];
+            ab[r] = S[I.o_a + r];
+        }
+    }
     __syncthreads();
     for (int p = t; p < P; p += kSwStepBlock) {
         const int i = p / ns, j = p % ns;
@@ -436,78 +525,32 @@ __global__ __launch_bounds__(kSwStepBlock) void k_sw_step(SwInst *__restrict__ i
             }
         }
     }
-    // wake of the current step (rows 0 .. nrow-1, vertices 0 .. nrow)
-    const int nrow = I.nrow, cur = I.par;
-    if (!hbm) {
-        double *lwv = L + kL_wv, *lwg = L + kL_wg, *lwa = L + kL_wa;
-        for (int e = t; e < 3 * (nrow + 1) * (ns + 1); e += kSwStepBlock) lwv[e] = S[I.o_wv[cur] + e];
-        for (int e = t; e < nrow * ns; e += kSwStepBlock) {
-            lwg[e] = S[I.o_wg[cur] + e];
-            lwa[e] = S[I.o_wa[cur] + e];
-        }
-    }
-    __syncthreads();
-    // ---- AIC: unit-strength rings (bound core, age 0) . normals
-    for (int e = t; e < P * P; e += kSwStepBlock) {
-        const int i = e / P, j = e % P;
-        const double *R = ring + 12 * j;
-        double u[3] = {0.0, 0.0, 0.0};
-        for (int k = 0; k < 4; ++k) {
-            const SegRec sr = sw_seg(R + 3 * k, R + 3 * ((k + 1) & 3), 1.0, I.rc0b, 0.0, I.nu);
-            sw_pair(I.prec, col + 3 * i, sr, u);
-        }
-        A[i * lda + j] = u[0] * nrm[3 * i] + u[1] * nrm[3 * i + 1] + u[2] * nrm[3 * i + 2];
-    }
-    // ---- wake-ring velocity at the collocation points (kSwParts partial sums;
-    // the HBM wake's come from k_sw_wake)
+    // ---- wake-ring velocities at the step's targets: k_sw_wake's chunk
+    // partials added per (target, component), four interleaved running sums
+    // combined in a fixed order
     const int W = nrow * ns;
     const int nfree = sw_nfree(I), NV = nfree * (ns + 1);
     const int T = P + NL + NV;
-    const int nwch = (int)((4 * (int64_t)W + kSwWChunk - 1) / kSwWChunk);
-    for (int e = t; e < P * kSwParts && !hbm; e += kSwStepBlock) {
-        const int p = e / kSwParts, q = e % kSwParts;
-        double u[3] = {0.0, 0.0, 0.0};
-        for (int k = q; k < 4 * W; k += kSwParts) {
-            const int rg = k >> 2, leg = k & 3, r = rg / ns, j = rg % ns;
-            const double *c4[4] = {wv + 3 * ((r + 1) * (ns + 1) + j + 1),   // Br
-                                   wv + 3 * (r * (ns + 1) + j + 1),         // Fr
-                                   wv + 3 * (r * (ns + 1) + j),             // Fl
-                                   wv + 3 * ((r + 1) * (ns + 1) + j)};      // Bl
-            const SegRec sr = sw_seg(c4[leg], c4[(leg + 1) & 3], wg[rg], I.rc0w, wa[rg], I.nu);
-            sw_pair(I.prec, col + 3 * p, sr, u);
+    const int nwch = (int)((4 * (int64_t)W + I.wchunk - 1) / I.wchunk);
+    for (int e = t; e < 3 * T; e += kSwStepBlock) {
+        const double *pp = S + I.o_wpart + e;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        int ch = 0;
+        for (; ch + 4 <= nwch; ch += 4) {
+            a0 += pp[(int64_t)ch * 3 * T];
+            a1 += pp[(int64_t)(ch + 1) * 3 * T];
+            a2 += pp[(int64_t)(ch + 2) * 3 * T];
+            a3 += pp[(int64_t)(ch + 3) * 3 * T];
         }
-        for (int c = 0; c < 3; ++c) red[(p * kSwParts + q) * 3 + c] = u[c];
+        for (; ch < nwch; ++ch) a0 += pp[(int64_t)ch * 3 * T];
+        S[I.o_wsum + e] = (a0 + a1) + (a2 + a3);
     }
     __syncthreads();
-    if (hbm) {
-        // k_sw_wake's chunk partials added per (target, component), four
-        // interleaved running sums combined in a fixed order
-        for (int e = t; e < 3 * T; e += kSwStepBlock) {
-            const double *pp = S + I.o_wpart + e;
-            double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-            int ch = 0;
-            for (; ch + 4 <= nwch; ch += 4) {
-                a0 += pp[(int64_t)ch * 3 * T];
-                a1 += pp[(int64_t)(ch + 1) * 3 * T];
-                a2 += pp[(int64_t)(ch + 2) * 3 * T];
-                a3 += pp[(int64_t)(ch + 3) * 3 * T];
-            }
-            for (; ch < nwch; ++ch) a0 += pp[(int64_t)ch * 3 * T];
-            S[I.o_wsum + e] = (a0 + a1) + (a2 + a3);
-        }
-        __syncthreads();
-    }
+    // ---- right-hand side
     const int nchunk = (int)((I.np + kSwChunk - 1) / kSwChunk);
     for (int p = t; p < P; p += kSwStepBlock) {
         double w3[3], q3[3] = {0.0, 0.0, 0.0};
-        for (int c = 0; c < 3; ++c) {
-            double a = 0.0;
-            if (hbm)
-                a = S[I.o_wsum + 3 * p + c];
-            else
-                for (int q = 0; q < kSwParts; ++q) a += red[(p * kSwParts + q) * 3 + c];
-            w3[c] = a;
-        }
+        for (int c = 0; c < 3; ++c) w3[c] = S[I.o_wsum + 3 * p + c];
         for (int ch = 0; ch < nchunk; ++ch)
             for (int c = 0; c < 3; ++c) q3[c] += S[I.o_part + (ch * P + p) * 3 + c];
         const double *n = nrm + 3 * p;
@@ -518,12 +561,14 @@ __global__ __launch_bounds__(kSwStepBlock) void k_sw_step(SwInst *__restrict__ i
         rhs[p] = -((wn + qn) + fs);
     }
     __syncthreads();
-    // ---- circulation solve (LU with partial pivoting in LDS); the
-    // factorization is skipped when the AIC equals the last factored one
+    // ---- circulation solve (LU with partial pivoting in LDS) of k_sw_geom's
+    // AIC; the factorization is skipped when the AIC equals the last factored
+    // one
     double *lkey = S + I.o_lkey, *lfac = S + I.o_lfac, *lord = S + I.o_lord;
     int *order = perm + P;   // the pivots composed into one gather order (kL_perm holds 2 P ints)
     const bool check = S[I.o_lst] == 1.0;
-    const bool differ = check && mat_move<kSwStepBlock, 2>(A, lda, (double *)nullptr, 0, P, lkey);
+    const bool differ = mat_move<kSwStepBlock, 2>(S + I.o_aic, P, A, lda, P,
+                                                  check ? (const double *)lkey : (const double *)nullptr);
     if (!__syncthreads_or((!check || differ) ? 1 : 0)) {
         mat_move<kSwStepBlock, 2>(lfac, P, A, lda, P, (const double *)nullptr);
         for (int i = t; i < P; i += kSwStepBlock) order[i] = (int)lord[i];
@@ -557,36 +602,17 @@ __global__ __launch_bounds__(kSwStepBlock) void k_sw_step(SwInst *__restrict__ i
     wave_back_subst<double, double>(A, lda, P, rhs, rdg);
     __syncthreads();
     const double *gam = rhs;
-    // ---- velocities at the leg centres from bound (gamma) + wake rings
-    for (int e = t; e < NL * kSwParts; e += kSwStepBlock) {
-        const int tq = e / kSwParts, q = e % kSwParts;
-        const double *x = lc + 3 * tq;
-        double u[3] = {0.0, 0.0, 0.0};
-        for (int k = q; k < 4 * P; k += kSwParts) {
-            const int j = k >> 2, leg = k & 3;
-            const double *R = ring + 12 * j;
-            const SegRec sr = sw_seg(R + 3 * leg, R + 3 * ((leg + 1) & 3), gam[j], I.rc0b, 0.0, I.nu);
-            sw_pair(I.prec, x, sr, u);
-        }
-        for (int k = q; k < 4 * W && !hbm; k += kSwParts) {
-            const int rg = k >> 2, leg = k & 3, r = rg / ns, j = rg % ns;
-            const double *c4[4] = {wv + 3 * ((r + 1) * (ns + 1) + j + 1),
-                                   wv + 3 * (r * (ns + 1) + j + 1),
-                                   wv + 3 * (r * (ns + 1) + j),
-                                   wv + 3 * ((r + 1) * (ns + 1) + j)};
-            const SegRec sr = sw_seg(c4[leg], c4[(leg + 1) & 3], wg[rg], I.rc0w, wa[rg], I.nu);
-            sw_pair(I.prec, x, sr, u);
-        }
-        for (int c = 0; c < 3; ++c) red[(tq * kSwParts + q) * 3 + c] = u[c];
-    }
-    __syncthreads();
-    for (int tq = t; tq < NL; tq += kSwStepBlock)
-        for (int c = 0; c < 3; ++c) {
+    // ---- velocities at the leg centres: the bound rings through k_sw_geom's
+    // unit influences contracted with the circulations (in ring order) + the
+    // wake rings
+    {
+        const double *B = S + I.o_bleg;
+        for (int e = t; e < 3 * NL; e += kSwStepBlock) {
             double a = 0.0;
-            for (int q = 0; q < kSwParts; ++q) a += red[(tq * kSwParts + q) * 3 + c];
-            if (hbm) a += S[I.o_wsum + 3 * (P + tq) + c];
-            vl[3 * tq + c] = a;
+            for (int j = 0; j < P; ++j) a += gam[j] * B[(int64_t)j * 3 * NL + e];
+            vl[e] = a + S[I.o_wsum + 3 * P + e];
         }
+    }
     __syncthreads();
     // ---- free ring wake: velocities at vertex rows 0 .. nfree - 1 from the
     // bound rings (this step's circulation) + the wake rings (k_sw_wake's
@@ -667,86 +693,102 @@ __global__ __launch_bounds__(kSwStepBlock) void k_sw_step(SwInst *__restrict__ i
     // ---- structure: station loads -> beam nodes, Newmark, next deformation
     double *Fn = bm, *Fe = bm + 3 * kSwMaxNodes, *dn = Fe + kSwMaxFree, *an = dn + kSwMaxFree;
     double *tmp = an + kSwMaxFree, *wnew = tmp + kSwMaxFree, *tnew = wnew + kSwMaxNodes;
-    const bool couple = I.couple && s >= 1 && s <= I.num_steps - 2;
     if (couple) {
-        if (t == 0) {
-            for (int k = 0; k < 3 * nn; ++k) Fn[k] = 0.0;
-            for (int j = 0; j < ns; ++j) {
-                double lift = 0.0, mom = 0.0;
-                for (int i = 0; i < nc; ++i) {
-                    const int p = i * ns + j;
-                    const double fz = PF[3 * p + 2] + (i + 1 < nc ? 0.5 * Ff[3 * (p + ns) + 2] : 0.0);
-                    const double lp = I.lift_sign * fz;
-                    lift += lp;
-                    mom += col[3 * p] * lp;
-                }
-                const double y = col[3 * j + 1];   // first chordwise panel of the station
-                if (!(fabs(y) > 1e-6)) continue;
+        // per station (one lane each): lift, moment and the node pair it loads
+        double *st = L + kL_st;
+        for (int j = t; j < ns; j += kSwStepBlock) {
+            double lift = 0.0, mom = 0.0;
+            for (int i = 0; i < nc; ++i) {
+                const int p = i * ns + j;
+                const double fz = PF[3 * p + 2] + (i + 1 < nc ? 0.5 * Ff[3 * (p + ns) + 2] : 0.0);
+                const double lp = I.lift_sign * fz;
+                lift += lp;
+                mom += col[3 * p] * lp;
+            }
+            const double y = col[3 * j + 1];   // first chordwise panel of the station
+            double node = -1.0, xi = 0.0;      // node -1: no load (the root station)
+            if (fabs(y) > 1e-6) {
                 int idx = -1;
                 for (int k = 0; k < nn; ++k)
                     if (yn[k] <= y) idx = k;
                 idx = idx < 0 ? 0 : (idx > nn - 2 ? nn - 2 : idx);
                 const double Le = yn[idx + 1] - yn[idx];
-                double xi = (y - yn[idx]) / Le;
+                xi = (y - yn[idx]) / Le;
                 xi = xi < 0.0 ? 0.0 : (xi > 1.0 ? 1.0 : xi);
-                Fn[3 * idx] += lift * (1 - xi);
-                Fn[3 * (idx + 1)] += lift * xi;
-                Fn[3 * idx + 2] += mom * (1 - xi);
-                Fn[3 * (idx + 1) + 2] += mom * xi;
+                node = (double)idx;
             }
+            st[j] = lift;
+            st[kSwMaxNs + j] = mom;
+            st[2 * kSwMaxNs + j] = node;
+            st[3 * kSwMaxNs + j] = xi;
+        }
+        __syncthreads();
+        // per node (one lane each): its stations' shares added in station order
+        for (int k = t; k < nn; k += kSwStepBlock) {
+            double fw = 0.0, fm = 0.0;
+            for (int j = 0; j < ns; ++j) {
+                const double node = st[2 * kSwMaxNs + j];
+                if (node < 0.0) continue;
+                const int idx = (int)node;
+                const double lift = st[j], mom = st[kSwMaxNs + j], xi = st[3 * kSwMaxNs + j];
+                if (idx == k) {
+                    fw += lift * (1 - xi);
+                    fm += mom * (1 - xi);
+                } else if (idx + 1 == k) {
+                    fw += lift * xi;
+                    fm += mom * xi;
+                }
+            }
+            Fn[3 * k] = fw;
+            Fn[3 * k + 1] = 0.0;
+            Fn[3 * k + 2] = fm;
         }
         __syncthreads();
         const double be = I.beta, ga = I.gamma;
         const double a0 = 1.0 / (be * dt * dt), a1 = 1.0 / (be * dt), a2 = 1.0 / (2.0 * be) - 1.0;
         const double a3 = ga / (be * dt), a4 = ga / be - 1.0, a5 = dt * (ga / (2.0 * be) - 1.0);
-        const double *M = S + I.o_M, *C = S + I.o_C, *K = S + I.o_K;
-        double *d = S + I.o_d, *v = S + I.o_v, *a = S + I.o_a;
         // F_eff = F + M (a0 d + a1 v + a2 a) + C (a3 d + a4 v + a5 a)
         for (int r = t; r < nf; r += kSwStepBlock) {
             double sm = 0.0, sc = 0.0;
             for (int c = 0; c < nf; ++c) {
-                sm += M[r * nf + c] * (a0 * d[c] + a1 * v[c] + a2 * a[c]);
-                sc += C[r * nf + c] * (a3 * d[c] + a4 * v[c] + a5 * a[c]);
+                sm += Mb[r * nf + c] * (a0 * db[c] + a1 * vb[c] + a2 * ab[c]);
+                sc += Cb[r * nf + c] * (a3 * db[c] + a4 * vb[c] + a5 * ab[c]);
             }
             Fe[r] = Fn[3 + r] + sm + sc;
         }
         __syncthreads();
+        double *d = S + I.o_d, *v = S + I.o_v, *a = S + I.o_a;
         for (int r = t; r < nf; r += kSwStepBlock) {
             double x = 0.0;
-            for (int c = 0; c < nf; ++c) x += K[r * nf + c] * Fe[c];
+            for (int c = 0; c < nf; ++c) x += Kb[r * nf + c] * Fe[c];
             dn[r] = x;
-            an[r] = a0 * (dn[r] - d[r]) - a1 * v[r] - a2 * a[r];
-        }
-        __syncthreads();
-        for (int r = t; r < nf; r += kSwStepBlock) {
-            v[r] = v[r] + dt * ((1.0 - ga) * a[r] + ga * an[r]);
+            an[r] = a0 * (dn[r] - db[r]) - a1 * vb[r] - a2 * ab[r];
+            v[r] = vb[r] + dt * ((1.0 - ga) * ab[r] + ga * an[r]);
             a[r] = an[r];
             d[r] = dn[r];
         }
         __syncthreads();
-        if (t == 0) {
-            double *wp = S + I.o_wprev, *tp = S + I.o_tprev;
-            for (int k = 0; k < nn; ++k) {
-                double w = k == 0 ? 0.0 : d[3 * (k - 1)];
-                double th = k == 0 ? 0.0 : d[3 * (k - 1) + 2];
-                if (I.has_wprev) {
-                    w = I.relax * w + (1.0 - I.relax) * wp[k];
-                    th = I.relax * th + (1.0 - I.relax) * tp[k];
-                }
-                wnew[k] = w;
-                tnew[k] = th;
-                wp[k] = w;
-                tp[k] = th;
+        // nodal heave / twist (one lane per node), relaxed against the last
+        double *wp = S + I.o_wprev, *tp = S + I.o_tprev;
+        for (int k = t; k < nn; k += kSwStepBlock) {
+            double w = k == 0 ? 0.0 : dn[3 * (k - 1)];
+            double th = k == 0 ? 0.0 : dn[3 * (k - 1) + 2];
+            if (I.has_wprev) {
+                w = I.relax * w + (1.0 - I.relax) * wp[k];
+                th = I.relax * th + (1.0 - I.relax) * tp[k];
             }
-            I.has_wprev = 1;
-            S[I.o_tw + s] = wnew[nn - 1];
-            S[I.o_tt + s] = tnew[nn - 1];
-            if (!I.feedback)   // the aerodynamics stay on the undeformed lattice
-                for (int k = 0; k < nn; ++k) {
-                    wnew[k] = 0.0;
-                    tnew[k] = 0.0;
-                }
+            wp[k] = w;
+            tp[k] = th;
+            if (k == nn - 1) {
+                S[I.o_tw + s] = w;
+                S[I.o_tt + s] = th;
+            }
+            // without feedback the aerodynamics stay on the undeformed lattice
+            wnew[k] = I.feedback ? w : 0.0;
+            tnew[k] = I.feedback ? th : 0.0;
         }
+        __syncthreads();
+        if (t == 0) I.has_wprev = 1;
     } else if (t == 0) {
         for (int k = 0; k < nn; ++k) {
             wnew[k] = 0.0;
@@ -754,7 +796,7 @@ __global__ __launch_bounds__(kSwStepBlock) void k_sw_step(SwInst *__restrict__ i
         }
     }
     __syncthreads();
-    // ---- wake shedding, conversion, particle convection (not after the last step)
+    // ---- wake shedding, conversion, next lattice (not after the last step)
     const bool more = s + 1 < I.num_steps;
     const int nxt = 1 - cur;
     if (more) {
@@ -818,66 +860,72 @@ __global__ __launch_bounds__(kSwStepBlock) void k_sw_step(SwInst *__restrict__ i
             }
             __syncthreads();
         }
-        // conversion of ring row n_keep (span order, legs Fr->Fl, Fl->Bl, Bl->Br, Br->Fr)
-        if (I.n_keep < nrow_new && t == 0) {
+        // conversion of ring row n_keep (span order, legs Fr->Fl, Fl->Bl, Bl->Br,
+        // Br->Fr): wavefront 0, lane = 4 j + leg of a 64-leg chunk; the legs
+        // that emit a particle take consecutive slots in (j, leg) order from a
+        // ballot prefix count
+        if (I.n_keep < nrow_new && t < 64) {
             const int kr = I.n_keep;
             int64_t np = I.np;
             double *pos = S + I.o_pos, *al = S + I.o_al, *sg = S + I.o_sg;
-            for (int j = 0; j < ns && !fail_s; ++j) {
-                const double G = wgn[kr * ns + j];
-                if (!(fabs(G) >= 1e-15)) continue;
-                const double *FL = wvn + 3 * (kr * (ns + 1) + j), *FR = FL + 3;
-                const double *BL = wvn + 3 * ((kr + 1) * (ns + 1) + j), *BR = BL + 3;
-                const double *ends[4][2] = {{FR, FL}, {FL, BL}, {BL, BR}, {BR, FR}};
-                for (int l = 0; l < 4; ++l) {
-                    double pp[3], aa[3];
+            bool over = false;
+            for (int e0 = 0; e0 < 4 * ns; e0 += 64) {
+                const int e = e0 + t, j = e >> 2, l = e & 3;
+                const bool lane_ok = e < 4 * ns;
+                const double G = lane_ok ? wgn[kr * ns + j] : 0.0;
+                const bool ring_ok = lane_ok && fabs(G) >= 1e-15;
+                double pp[3] = {0.0, 0.0, 0.0}, aa[3] = {0.0, 0.0, 0.0};
+                if (ring_ok) {
+                    const double *FL = wvn + 3 * (kr * (ns + 1) + j), *FR = FL + 3;
+                    const double *BL = wvn + 3 * ((kr + 1) * (ns + 1) + j), *BR = BL + 3;
+                    const double *ends[4][2] = {{FR, FL}, {FL, BL}, {BL, BR}, {BR, FR}};
                     for (int c = 0; c < 3; ++c) {
                         pp[c] = 0.5 * (ends[l][0][c] + ends[l][1][c]);
                         aa[c] = G * (ends[l][1][c] - ends[l][0][c]);
                     }
-                    if (!(aa[0] * aa[0] + aa[1] * aa[1] + aa[2] * aa[2] > 1e-30)) continue;
-                    if (np >= I.cap) {
-                        fail_s = 1;
-                        break;
-                    }
-                    for (int c = 0; c < 3; ++c) {
-                        pos[3 * np + c] = pp[c];
-                        al[3 * np + c] = aa[c];
-                    }
-                    sg[np] = I.sigma;
-                    ++np;
                 }
-                wgn[kr * ns + j] = 0.0;
+                const bool emit = ring_ok && aa[0] * aa[0] + aa[1] * aa[1] + aa[2] * aa[2] > 1e-30;
+                const unsigned long long mask = __ballot(emit);
+                const int64_t slot = np + __popcll(mask & ((1ull << t) - 1ull));
+                if (emit && slot < I.cap) {
+                    for (int c = 0; c < 3; ++c) {
+                        pos[3 * slot + c] = pp[c];
+                        al[3 * slot + c] = aa[c];
+                    }
+                    sg[slot] = I.sigma;
+                }
+                over = over || (emit && slot >= I.cap);
+                np += __popcll(mask);
+                if (ring_ok && l == 0) wgn[kr * ns + j] = 0.0;
             }
-            I.np = np;
+            if (__ballot(over) != 0ull && t == 0) fail_s = 1;
+            if (t == 0) I.np = np;
         }
         __syncthreads();
         if (fail_s) {
             if (t == 0) insts[blockIdx.x].status = -2;
             return;
         }
-        // particles: prescribed wake, x += V dt
-        double *pos = S + I.o_pos;
-        for (int64_t k = t; k < I.np; k += kSwStepBlock) {
-            pos[3 * k] += ox;
-            pos[3 * k + 1] += oy;
-            pos[3 * k + 2] += oz;
-        }
-        // next lattice: deformation and collocation points for the next induction
-        __syncthreads();
+        // the particles owe x += V dt (prescribed wake): k_sw_induce applies it
+        // when it stages them for the next step
+        // next lattice: deformation, collocation points and leg centres for the
+        // next step's launches
         for (int k = t; k < nn; k += kSwStepBlock) {
             wdef[k] = wnew[k];
             tdef[k] = tnew[k];
         }
         double *gn = g;   // the current lattice is no longer needed
         __syncthreads();
-        for (int v = t; v < nv; v += kSwStepBlock) sw_deform(grid0 + 3 * v, yn, wnew, tnew, nn, I.x_ea, gn + 3 * v);
+        for (int v = t; v < nv; v += kSwStepBlock) {
+            sw_deform(grid0 + 3 * v, yn, wnew, tnew, nn, I.x_ea, gn + 3 * v);
+            for (int c = 0; c < 3; ++c) S[I.o_gcur + 3 * v + c] = gn[3 * v + c];
+        }
         __syncthreads();
         for (int p = t; p < P; p += kSwStepBlock) {
             double cn[3], nn3[3], ar;
             sw_frame(gn, ns, p / ns, p % ns, cn, nn3, &ar);
             for (int c = 0; c < 3; ++c) S[I.o_coln + 3 * p + c] = cn[c];
-            if (hbm) sw_leg_centres(gn, nc, ns, P, p, S + I.o_legn);
+            sw_leg_centres(gn, nc, ns, P, p, S + I.o_legn);
         }
         if (I.free_wake)
             for (int j = t; j <= ns; j += kSwStepBlock) {
@@ -907,28 +955,33 @@ __global__ __launch_bounds__(kSwStepBlock) void k_sw_step(SwInst *__restrict__ i
         o.par = I.par;
         o.has_wprev = I.has_wprev;
         o.has_prev = 1;
+        o.shift = more ? 1 : 0;
         o.step = s + 1;
     }
 }
 
-// Collocation points (and, for an HBM wake, leg centres) of the undeformed
-// lattice: the first step's induction targets.
+// The undeformed lattice (the first step's, deformed by the zero initial
+// heave / twist exactly as every later step's), its collocation points, leg
+// centres and trailing-edge points: the first step's launch inputs.
 __global__ __launch_bounds__(64) void k_sw_init(const SwInst *__restrict__ insts,
                                                 double *__restrict__ S)
 {
     const SwInst &I = insts[blockIdx.x];
+    double *g = S + I.o_gcur;
+    for (int v = threadIdx.x; v < I.nv; v += 64)
+        sw_deform(S + I.o_grid0 + 3 * v, S + I.o_yn, S + I.o_wdef, S + I.o_tdef, I.n_nodes, I.x_ea,
+                  g + 3 * v);
+    __syncthreads();
     for (int p = threadIdx.x; p < I.P; p += 64) {
-        const double *g = S + I.o_grid0;
         double cn[3], nn3[3], ar;
-        // an undeformed lattice: sw_deform with zero heave / twist is the identity
         sw_frame(g, I.ns, p / I.ns, p % I.ns, cn, nn3, &ar);
         for (int c = 0; c < 3; ++c) S[I.o_coln + 3 * p + c] = cn[c];
-        if (I.hbm) sw_leg_centres(g, I.nc, I.ns, I.P, p, S + I.o_legn);
+        sw_leg_centres(g, I.nc, I.ns, I.P, p, S + I.o_legn);
     }
     if (I.free_wake)
         for (int j = threadIdx.x; j <= I.ns; j += 64) {
-            const double *va = sw_v(S + I.o_grid0, I.ns, I.nc - 1, j);
-            const double *vb = sw_v(S + I.o_grid0, I.ns, I.nc, j);
+            const double *va = sw_v(g, I.ns, I.nc - 1, j);
+            const double *vb = sw_v(g, I.ns, I.nc, j);
             for (int c = 0; c < 3; ++c) S[I.o_ten + 3 * j + c] = vb[c] + 0.25 * (vb[c] - va[c]);
         }
 }
@@ -943,7 +996,8 @@ struct fv_sweep {
     std::vector<SwInst> host;
     SwInst *dinst = nullptr;
     double *block = nullptr;
-    int64_t max_chunks = 1, max_wchunks = 0, max_tchunks = 1, max_steps = 0, done_steps = 0;
+    int64_t max_chunks = 1, max_wchunks = 1, max_tchunks = 1, max_gchunks = 1, max_steps = 0;
+    int64_t done_steps = 0;
     int use_graph = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -952,12 +1006,13 @@ struct fv_sweep {
 static int sweep_launch(fv_sweep *sw)
 {
     fv_ctx *ctx = sw->ctx;
-    if (sw->max_wchunks > 0) {
-        hipLaunchKernelGGL(k_sw_wake,
-                           dim3((unsigned)sw->max_wchunks, (unsigned)sw->n, (unsigned)sw->max_tchunks),
-                           dim3(kSwWBlock), 0, ctx->stream, sw->dinst, sw->block);
-        FV_LAUNCH_CHECK(ctx);
-    }
+    hipLaunchKernelGGL(k_sw_geom, dim3((unsigned)sw->max_gchunks, (unsigned)sw->n),
+                       dim3(kSwGeomBlock), 0, ctx->stream, sw->dinst, sw->block);
+    FV_LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_sw_wake,
+                       dim3((unsigned)sw->max_wchunks, (unsigned)sw->n, (unsigned)sw->max_tchunks),
+                       dim3(kSwWBlock), 0, ctx->stream, sw->dinst, sw->block);
+    FV_LAUNCH_CHECK(ctx);
     hipLaunchKernelGGL(k_sw_induce, dim3((unsigned)sw->max_chunks, (unsigned)sw->n),
                        dim3(kSwBlock), 0, ctx->stream, sw->dinst, sw->block);
     FV_LAUNCH_CHECK(ctx);
@@ -1012,7 +1067,6 @@ int fv_sweep_create(fv_ctx *ctx, const fv_sweep_desc *descs, int32_t n, int32_t 
         I.free_wake = d.free_ring_wake ? 1 : 0;
         I.truncate = d.truncate_wake ? 1 : 0;
         I.trunc_x = d.truncate_x;
-        I.hbm = (I.R > kSwMaxR || I.free_wake) ? 1 : 0;
         I.n_keep = d.n_keep;
         I.kernel = d.kernel;
         I.prec = d.precision;
@@ -1053,10 +1107,13 @@ int fv_sweep_create(fv_ctx *ctx, const fv_sweep_desc *descs, int32_t n, int32_t 
         I.o_al = take(3 * I.cap);
         I.o_sg = take(I.cap);
         I.o_part = take(3 * I.P * I.max_chunks);
-        if (I.hbm) {
+        {
             // the wake holds at most R rows when a step starts; the free vertex
-            // rows are at most min(R, n_keep) + 1
-            I.max_wchunks = ceil_div(4 * (int64_t)I.R * d.ns, kSwWChunk);
+            // rows are at most min(R, n_keep) + 1.  Short wakes take small
+            // segment chunks (more workgroups, shorter per-lane loops).
+            const int64_t nseg = 4 * (int64_t)I.R * d.ns;
+            I.wchunk = ceil_div(nseg, kSwWChunkSmall) <= 64 ? kSwWChunkSmall : kSwWChunkLarge;
+            I.max_wchunks = ceil_div(nseg, I.wchunk);
             if (I.max_wchunks < 1) I.max_wchunks = 1;
             if (I.max_wchunks > sw->max_wchunks) sw->max_wchunks = I.max_wchunks;
             const int64_t nv = I.free_wake
@@ -1070,6 +1127,14 @@ int fv_sweep_create(fv_ctx *ctx, const fv_sweep_desc *descs, int32_t n, int32_t 
             I.o_wsum = take(3 * I.max_targets);
             I.o_ten = take(3 * (d.ns + 1));
             I.o_vvel = take(3 * (nv > 0 ? nv : 1));
+        }
+        {
+            const int64_t NL = 3 * (int64_t)I.P + d.ns;
+            I.o_gcur = take(3 * I.nv);
+            I.o_aic = take((int64_t)I.P * I.P);
+            I.o_bleg = take(3 * NL * I.P);
+            const int64_t gch = ceil_div((int64_t)I.P * I.P + NL * I.P, kSwGeomBlock);
+            if (gch > sw->max_gchunks) sw->max_gchunks = gch;
         }
         I.o_M = take((int64_t)nfree * nfree);
         I.o_C = take((int64_t)nfree * nfree);
