@@ -5,15 +5,16 @@
 // PteraSoftware UVLM run coupled to BeamFE by AeroelasticSolver
 // (src/fluxvortex/aeroelastic_solver.py:63-221, beam_fe.py:231-351).  Here
 // every airspeed is an instance of the hybrid N_keep step and all instances
-// advance together, four launches per step.  The first three spread the
-// pair work of every instance over the whole device:
-//   k_sw_geom    AIC (unit rings . normals at the collocation points) and the
+// advance together, two launches per step.  The first spreads the pair work
+// of every instance over the whole device, one workgroup per unit of three
+// roles:
+//   sw_geom      AIC (unit rings . normals at the collocation points) and the
 //                unit-strength bound-ring velocities at the leg centres of
-//                the step's lattice, one thread per entry;
-//   k_sw_wake    wake-ring velocities at the collocation points, the leg
-//                centres (and free wake vertices), one wavefront per
-//                (instance, segment chunk, 64 targets), partial sums per chunk;
-//   k_sw_induce  particle -> collocation-point velocities, one workgroup per
+//                the step's lattice, one lane per entry;
+//   sw_wake      wake-ring velocities at the collocation points, the leg
+//                centres (and free wake vertices), one lane per target of a
+//                segment chunk, partial sums per chunk;
+//   sw_induce    particle -> collocation-point velocities, one workgroup per
 //                (instance, 1024-particle chunk), partial sums per chunk; it
 //                also applies the previous step's prescribed-wake convection
 //                (x += V dt) to the particles it stages;
@@ -57,15 +58,12 @@ constexpr int kSwChunk = 1024;   // particles per induction partial
 constexpr int kSwStreams = 4;    // source streams per induction wavefront (targets <= 64 / streams)
 constexpr int kSwWChunkSmall = 16;    // wake segments per partial sum (wakes up to 64 chunks)
 constexpr int kSwWChunkLarge = 256;   // wake segments per partial sum (longer wakes)
-constexpr int kSwWBlock = 64;    // wake induction: one wavefront,
-constexpr int kSwWTpt = 1;       // one target per lane (many small workgroups)
-constexpr int kSwWTile = 64;     // wake segments staged in LDS at a time
 
 struct SwInst {
     int nc, ns, P, nv, R, n_keep, kernel, couple, n_nodes, nfree;
     int status;                  // 0; k + 1: zero pivot at step k; -2: particle overflow
     int has_prev, par, nrow, has_wprev, prec, feedback;
-    int wchunk;                  // wake segments per k_sw_wake partial sum
+    int wchunk;                  // wake segments per sw_wake partial sum (<= kSwBlock)
     int shift;                   // the particles still owe this step's x += V dt
     int free_wake;               // free ring wake: vertex rows move with V + u
     int truncate;                // rings whose corner-mean x exceeds trunc_x lose Gamma
@@ -80,7 +78,7 @@ struct SwInst {
     int64_t o_ten, o_vvel;       // the step's trailing-edge points; free-vertex velocities
     int64_t o_wsum;              // the wake partial sums added up, 3 per target
     int64_t o_gcur;              // the step's lattice (3 nv)
-    int64_t o_aic, o_bleg;       // k_sw_geom: AIC (P x P), leg influences (NL x P x 3)
+    int64_t o_aic, o_bleg;       // sw_geom: AIC (P x P), leg influences (NL x P x 3)
     // factor reuse of the circulation solve: the AIC last factored, its
     // factors, the composed row order (as doubles) and a state word (1.0:
     // valid); a step whose AIC equals it bit for bit (a lattice that does
@@ -282,16 +280,17 @@ __device__ void sw_induce_body(const SwInst &I, double *__restrict__ S, double *
         }
 }
 
-__global__ __launch_bounds__(kSwBlock) void k_sw_induce(const SwInst *__restrict__ insts,
-                                                        double *__restrict__ S)
+// LDS of the induction role: a source tile, the targets, the stream sums.
+constexpr size_t kSwIndLds = sizeof(PartRec) * kSwBlock + sizeof(double) * (3 * kSwMaxP +
+                                                                           (kSwBlock / 64) * kSwMaxP * 3);
+
+__device__ void sw_induce(const SwInst &I, double *__restrict__ S, int64_t chunk, unsigned char *lds)
 {
-    __shared__ PartRec tile[kSwBlock];
-    __shared__ double tgt[3 * kSwMaxP];
-    __shared__ double red[(kSwBlock / 64) * kSwMaxP * 3];   // streams x P <= 4 x 64
-    const SwInst &I = insts[blockIdx.y];
     if (I.status != 0 || I.step >= I.num_steps) return;
-    const int64_t chunk = blockIdx.x;
-    if (chunk * kSwChunk >= I.np) return;
+    if (chunk * kSwChunk >= I.np) return;   // uniform over the workgroup
+    PartRec *tile = reinterpret_cast<PartRec *>(lds);
+    double *tgt = reinterpret_cast<double *>(lds + sizeof(PartRec) * kSwBlock);
+    double *red = tgt + 3 * kSwMaxP;   // streams x P <= 4 x 64
     double *part = S + I.o_part;
     const bool mixed = I.prec == FV_PREC_MIXED;
     if (I.kernel == FV_KERNEL_WINCKELMANS) {
@@ -307,73 +306,52 @@ __global__ __launch_bounds__(kSwBlock) void k_sw_induce(const SwInst *__restrict
 // Wake-ring velocities at the T = P + NL + NV targets of the step (collocation
 // points, the leg centres in slot order, then with a free ring wake the NV =
 // nfree (ns + 1) vertices of rows 0 .. nfree - 1) from the segments [c0, c1)
-// of this chunk (segment k: leg k & 3 of ring k >> 2), 64 targets per
-// workgroup (blockIdx.z), partial sums wpart[(chunk T + target) 3 + c].
-__global__ __launch_bounds__(kSwWBlock) void k_sw_wake(const SwInst *__restrict__ insts,
-                                                       double *__restrict__ S)
+// of one chunk (segment k: leg k & 3 of ring k >> 2; at most kSwBlock, staged
+// in LDS at once), one target per lane of a group of kSwBlock targets, partial
+// sums wpart[(chunk T + target) 3 + c].
+constexpr size_t kSwWakeLds = sizeof(SegRec) * kSwBlock;
+
+__device__ void sw_wake(const SwInst &I, double *__restrict__ S, int64_t chunk, int tgroup,
+                        unsigned char *lds)
 {
-    __shared__ SegRec tile[kSwWTile];
-    const SwInst &I = insts[blockIdx.y];
     if (I.status != 0 || I.step >= I.num_steps) return;
+    SegRec *tile = reinterpret_cast<SegRec *>(lds);
     const int ns = I.ns, P = I.P, NL = 3 * P + ns;
     const int T = P + NL + sw_nfree(I) * (ns + 1);
     const int64_t nseg = 4 * (int64_t)I.nrow * ns;
-    const int64_t c0 = (int64_t)blockIdx.x * I.wchunk;
-    const int tz = (int)blockIdx.z * (kSwWBlock * kSwWTpt);
+    const int64_t c0 = chunk * I.wchunk;
+    const int tz = tgroup * kSwBlock;
     if (c0 >= nseg || tz >= T) return;   // uniform over the workgroup
-    const int64_t c1 = (c0 + I.wchunk < nseg) ? c0 + I.wchunk : nseg;
+    const int cnt = (int)((nseg - c0 < I.wchunk) ? nseg - c0 : I.wchunk);   // <= kSwBlock
     const double *wv = S + I.o_wv[I.par], *wg = S + I.o_wg[I.par], *wa = S + I.o_wa[I.par];
-    constexpr int kTpt = kSwWTpt;
-    double u[kTpt][3];
-    const double *x[kTpt];
-#pragma unroll
-    for (int q = 0; q < kTpt; ++q) {
-        const int tq = tz + threadIdx.x + q * kSwWBlock;
-        const int tc = tq < T ? tq : 0;
-        const int tv = tc - P - NL;   // free vertex: row tv / (ns + 1), column tv % (ns + 1)
-        x[q] = tc < P ? S + I.o_coln + 3 * tc
-             : tv < 0 ? S + I.o_legn + 3 * (tc - P)
-             : tv < ns + 1 ? S + I.o_ten + 3 * tv
-                           : wv + 3 * tv;
-        u[q][0] = u[q][1] = u[q][2] = 0.0;
+    const int t = threadIdx.x;
+    if (t < cnt) {
+        const int k = (int)(c0 + t);   // < 4 nrow ns < 2^31
+        const int rg = k >> 2;
+        const int leg = k & 3, r = rg / ns, j = rg % ns;
+        const double *c4[4] = {wv + 3 * ((r + 1) * (ns + 1) + j + 1),   // Br
+                               wv + 3 * (r * (ns + 1) + j + 1),         // Fr
+                               wv + 3 * (r * (ns + 1) + j),             // Fl
+                               wv + 3 * ((r + 1) * (ns + 1) + j)};      // Bl
+        tile[t] = sw_seg(c4[leg], c4[(leg + 1) & 3], wg[rg], I.rc0w, wa[rg], I.nu);
     }
+    const int tq = tz + t;
+    const bool act = tq < T;
+    const int tc = act ? tq : 0;
+    const int tv = tc - P - NL;   // free vertex: row tv / (ns + 1), column tv % (ns + 1)
+    const double *x = tc < P ? S + I.o_coln + 3 * tc
+                    : tv < 0 ? S + I.o_legn + 3 * (tc - P)
+                    : tv < ns + 1 ? S + I.o_ten + 3 * tv
+                                  : wv + 3 * tv;
+    const double xp[3] = {x[0], x[1], x[2]};
+    double u[3] = {0.0, 0.0, 0.0};
+    __syncthreads();
+    if (!act) return;
     const int prec = I.prec;
-    bool act[kTpt];
-    double xp[kTpt][3];
-#pragma unroll
-    for (int q = 0; q < kTpt; ++q) {
-        act[q] = tz + (int)threadIdx.x + q * kSwWBlock < T;
-        for (int c = 0; c < 3; ++c) xp[q][c] = x[q][c];
-    }
-    for (int64_t base = c0; base < c1; base += kSwWTile) {
-        const int cnt = (int)((c1 - base < kSwWTile) ? c1 - base : kSwWTile);
-        __syncthreads();
-        if ((int)threadIdx.x < cnt) {
-            const int k = (int)(base + threadIdx.x);   // < 4 nrow ns < 2^31
-            const int rg = k >> 2;
-            const int leg = k & 3, r = rg / ns, j = rg % ns;
-            const double *c4[4] = {wv + 3 * ((r + 1) * (ns + 1) + j + 1),   // Br
-                                   wv + 3 * (r * (ns + 1) + j + 1),         // Fr
-                                   wv + 3 * (r * (ns + 1) + j),             // Fl
-                                   wv + 3 * ((r + 1) * (ns + 1) + j)};      // Bl
-            tile[threadIdx.x] = sw_seg(c4[leg], c4[(leg + 1) & 3], wg[rg], I.rc0w, wa[rg], I.nu);
-        }
-        __syncthreads();
 #pragma unroll 2
-        for (int k = 0; k < cnt; ++k) {
-            const SegRec sr = tile[k];
-#pragma unroll
-            for (int q = 0; q < kTpt; ++q)
-                if (act[q]) sw_pair(prec, xp[q], sr, u[q]);
-        }
-    }
-    double *part = S + I.o_wpart + (int64_t)blockIdx.x * T * 3;
-#pragma unroll
-    for (int q = 0; q < kTpt; ++q) {
-        const int tq = tz + threadIdx.x + q * kSwWBlock;
-        if (tq < T)
-            for (int c = 0; c < 3; ++c) part[3 * tq + c] = u[q][c];
-    }
+    for (int k = 0; k < cnt; ++k) sw_pair(prec, xp, tile[k], u);
+    double *part = S + I.o_wpart + chunk * T * 3;
+    for (int c = 0; c < 3; ++c) part[3 * tq + c] = u[c];
 }
 
 // ------------------------------------------------------------------ the step
@@ -411,18 +389,16 @@ static_assert(kSwLdsBytes + 2048 <= 160 * 1024, "k_sw_step LDS plan exceeds the 
 // centre q, stored at o_bleg + (j NL + q) 3 (k_sw_step contracts it with the
 // step's circulations; the bound rings have age 0, so their cores do not
 // depend on the circulation and the velocity is linear in it).
-__global__ __launch_bounds__(kSwGeomBlock) void k_sw_geom(const SwInst *__restrict__ insts,
-                                                         double *__restrict__ S)
+constexpr size_t kSwGeomLds = sizeof(double) * (3 * kSwNv + 12 * kSwMaxP + 6 * kSwMaxP + 3 * kSwMaxL);
+
+__device__ void sw_geom(const SwInst &I, double *__restrict__ S, int chunk, unsigned char *lds)
 {
-    __shared__ double g[3 * kSwNv];
-    __shared__ double ring[12 * kSwMaxP];
-    __shared__ double col[3 * kSwMaxP], nrm[3 * kSwMaxP];
-    __shared__ double lc[3 * kSwMaxL];
-    const SwInst &I = insts[blockIdx.y];
     if (I.status != 0 || I.step >= I.num_steps) return;
     const int nc = I.nc, ns = I.ns, P = I.P, nv = I.nv, NL = 3 * P + ns;
     const int E = P * P + NL * P;
-    if ((int)blockIdx.x * kSwGeomBlock >= E) return;   // uniform over the workgroup
+    if (chunk * kSwGeomBlock >= E) return;   // uniform over the workgroup
+    double *g = reinterpret_cast<double *>(lds), *ring = g + 3 * kSwNv;
+    double *col = ring + 12 * kSwMaxP, *nrm = col + 3 * kSwMaxP, *lc = nrm + 3 * kSwMaxP;
     const int t = threadIdx.x;
     for (int v = t; v < 3 * nv; v += kSwGeomBlock) g[v] = S[I.o_gcur + v];
     __syncthreads();
@@ -435,7 +411,7 @@ __global__ __launch_bounds__(kSwGeomBlock) void k_sw_geom(const SwInst *__restri
         sw_leg_centres(g, nc, ns, P, p, lc);
     }
     __syncthreads();
-    const int e = (int)blockIdx.x * kSwGeomBlock + t;
+    const int e = chunk * kSwGeomBlock + t;
     if (e >= E) return;
     const bool aic = e < P * P;
     const int f = aic ? e : e - P * P;
@@ -453,6 +429,36 @@ __global__ __launch_bounds__(kSwGeomBlock) void k_sw_geom(const SwInst *__restri
         double *b = S + I.o_bleg + ((int64_t)j * NL + q) * 3;
         for (int c = 0; c < 3; ++c) b[c] = u[c];
     }
+}
+
+// The three device-wide roles of a step in one launch (one workgroup = one
+// unit of one role, heaviest first): particle induction (n x gi units: the
+// instance's 1024-particle chunks), wake induction (n x gw x tg units: segment
+// chunk x group of kSwBlock targets), geometry (n x gg units of kSwGeomBlock
+// entries).  The roles write disjoint outputs.
+constexpr size_t kSwPreLds = kSwIndLds > kSwGeomLds ? (kSwIndLds > kSwWakeLds ? kSwIndLds : kSwWakeLds)
+                                                    : (kSwGeomLds > kSwWakeLds ? kSwGeomLds : kSwWakeLds);
+static_assert(kSwGeomBlock == kSwBlock, "the roles share one workgroup size");
+
+__global__ __launch_bounds__(kSwBlock) void k_sw_pre(const SwInst *__restrict__ insts,
+                                                     double *__restrict__ S, int n, int gi, int gw,
+                                                     int tg, int gg)
+{
+    __shared__ __attribute__((aligned(16))) unsigned char lds[kSwPreLds];
+    int64_t b = blockIdx.x;
+    const int64_t ni = (int64_t)n * gi, nw = (int64_t)n * gw * tg;
+    if (b < ni) {
+        sw_induce(insts[b / gi], S, b % gi, lds);
+        return;
+    }
+    b -= ni;
+    if (b < nw) {
+        const int64_t per = (int64_t)gw * tg, r = b % per;
+        sw_wake(insts[b / per], S, r / tg, (int)(r % tg), lds);
+        return;
+    }
+    b -= nw;
+    sw_geom(insts[b / gg], S, (int)(b % gg), lds);
 }
 
 __global__ __launch_bounds__(kSwStepBlock) void k_sw_step(SwInst *__restrict__ insts,
@@ -525,7 +531,7 @@ __global__ __launch_bounds__(kSwStepBlock) void k_sw_step(SwInst *__restrict__ i
             }
         }
     }
-    // ---- wake-ring velocities at the step's targets: k_sw_wake's chunk
+    // ---- wake-ring velocities at the step's targets: sw_wake's chunk
     // partials added per (target, component), four interleaved running sums
     // combined in a fixed order
     const int W = nrow * ns;
@@ -561,7 +567,7 @@ __global__ __launch_bounds__(kSwStepBlock) void k_sw_step(SwInst *__restrict__ i
         rhs[p] = -((wn + qn) + fs);
     }
     __syncthreads();
-    // ---- circulation solve (LU with partial pivoting in LDS) of k_sw_geom's
+    // ---- circulation solve (LU with partial pivoting in LDS) of sw_geom's
     // AIC; the factorization is skipped when the AIC equals the last factored
     // one
     double *lkey = S + I.o_lkey, *lfac = S + I.o_lfac, *lord = S + I.o_lord;
@@ -602,7 +608,7 @@ __global__ __launch_bounds__(kSwStepBlock) void k_sw_step(SwInst *__restrict__ i
     wave_back_subst<double, double>(A, lda, P, rhs, rdg);
     __syncthreads();
     const double *gam = rhs;
-    // ---- velocities at the leg centres: the bound rings through k_sw_geom's
+    // ---- velocities at the leg centres: the bound rings through sw_geom's
     // unit influences contracted with the circulations (in ring order) + the
     // wake rings
     {
@@ -615,7 +621,7 @@ __global__ __launch_bounds__(kSwStepBlock) void k_sw_step(SwInst *__restrict__ i
     }
     __syncthreads();
     // ---- free ring wake: velocities at vertex rows 0 .. nfree - 1 from the
-    // bound rings (this step's circulation) + the wake rings (k_sw_wake's
+    // bound rings (this step's circulation) + the wake rings (sw_wake's
     // partial sums), stored for the shedding below
     if (NV > 0) {
         double *vv = S + I.o_vvel;
@@ -906,7 +912,7 @@ __global__ __launch_bounds__(kSwStepBlock) void k_sw_step(SwInst *__restrict__ i
             if (t == 0) insts[blockIdx.x].status = -2;
             return;
         }
-        // the particles owe x += V dt (prescribed wake): k_sw_induce applies it
+        // the particles owe x += V dt (prescribed wake): sw_induce applies it
         // when it stages them for the next step
         // next lattice: deformation, collocation points and leg centres for the
         // next step's launches
@@ -1006,15 +1012,11 @@ struct fv_sweep {
 static int sweep_launch(fv_sweep *sw)
 {
     fv_ctx *ctx = sw->ctx;
-    hipLaunchKernelGGL(k_sw_geom, dim3((unsigned)sw->max_gchunks, (unsigned)sw->n),
-                       dim3(kSwGeomBlock), 0, ctx->stream, sw->dinst, sw->block);
-    FV_LAUNCH_CHECK(ctx);
-    hipLaunchKernelGGL(k_sw_wake,
-                       dim3((unsigned)sw->max_wchunks, (unsigned)sw->n, (unsigned)sw->max_tchunks),
-                       dim3(kSwWBlock), 0, ctx->stream, sw->dinst, sw->block);
-    FV_LAUNCH_CHECK(ctx);
-    hipLaunchKernelGGL(k_sw_induce, dim3((unsigned)sw->max_chunks, (unsigned)sw->n),
-                       dim3(kSwBlock), 0, ctx->stream, sw->dinst, sw->block);
+    const int64_t blocks = (int64_t)sw->n * (sw->max_chunks + sw->max_wchunks * sw->max_tchunks +
+                                             sw->max_gchunks);
+    hipLaunchKernelGGL(k_sw_pre, dim3((unsigned)blocks), dim3(kSwBlock), 0, ctx->stream, sw->dinst,
+                       sw->block, sw->n, (int)sw->max_chunks, (int)sw->max_wchunks,
+                       (int)sw->max_tchunks, (int)sw->max_gchunks);
     FV_LAUNCH_CHECK(ctx);
     hipLaunchKernelGGL(k_sw_step, dim3((unsigned)sw->n), dim3(kSwStepBlock), kSwLdsBytes, ctx->stream,
                        sw->dinst, sw->block);
@@ -1120,7 +1122,7 @@ int fv_sweep_create(fv_ctx *ctx, const fv_sweep_desc *descs, int32_t n, int32_t 
                                    ? ((int64_t)(I.R < d.n_keep ? I.R : d.n_keep) + 1) * (d.ns + 1)
                                    : 0;
             I.max_targets = 4 * (int64_t)I.P + d.ns + nv;
-            const int64_t tch = ceil_div(I.max_targets, kSwWBlock * kSwWTpt);
+            const int64_t tch = ceil_div(I.max_targets, kSwBlock);
             if (tch > sw->max_tchunks) sw->max_tchunks = tch;
             I.o_legn = take(3 * (3 * I.P + d.ns));
             I.o_wpart = take(3 * I.max_targets * I.max_wchunks);

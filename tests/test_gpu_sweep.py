@@ -107,10 +107,10 @@ def test_small_flutter_sweep_matches_oracle(oracle):
 
 
 def test_sweep_hbm_ring_wake_matches_oracle(oracle):
-    """Instances keeping more than 15 ring rows hold their wake in device
-    memory (k_sw_wake): the whole ring wake (no particles) at two speeds and
-    n_keep 20 (rows 20+ converted) in FP64 against the oracle to 1e-10, and
-    mixed within 1e-5 of FP64."""
+    """Long ring wakes (hundreds of rows, read in place from device memory by
+    the wake role of k_sw_pre in 256-segment chunks): the whole ring wake (no
+    particles) at two speeds and n_keep 20 (rows 20+ converted) in FP64
+    against the oracle to 1e-10, and mixed within 1e-5 of FP64."""
     insts = [ae.goland_instance(100.0, num_chords=10, n_keep=100000),
              ae.goland_instance(160.0, num_chords=12, n_keep=100000, kernel="winckelmans"),
              ae.goland_instance(130.0, num_chords=10, n_keep=20),
