@@ -204,7 +204,7 @@ def test_sweep_wake_truncation_matches_oracle(oracle):
 
 def test_sweep_hbm_wake_at_the_panel_limit(oracle):
     """64 panels (4 x 16, the largest lattice): 272 wake-induction targets per
-    instance (five 64-target groups), whole ring wake, prescribed and free,
+    instance (two 256-target groups), whole ring wake, prescribed and free,
     FP64 against the oracle to 1e-10."""
     insts = [ae.goland_instance(150.0, num_chords=6, nc=4, ns=16, n_keep=100000),
              ae.goland_instance(150.0, num_chords=6, nc=4, ns=16, n_keep=100000,
