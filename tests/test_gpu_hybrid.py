@@ -17,16 +17,56 @@ def hyb(ctx):
 
 
 def oracle_perturbed(oracle, case):
-    """The oracle under two 1e-14 input changes: a uniform one (V * (1 + 1e-14))
-    and a mirror-antisymmetric one (x of every grid point with y > 0 shifted by
-    1e-14 of the largest |x|), which excites what a symmetric change cannot
-    (relaxation turns particles towards their local vorticity direction, which
-    round-off fixes where that vorticity is tiny)."""
+    """The oracle under three 1e-14 input changes: a uniform one (V * (1 +
+    1e-14)) and a mirror-antisymmetric one of either sign (x of every grid
+    point with y > 0 shifted by +-1e-14 of the largest |x|), which excite what
+    a symmetric change cannot (relaxation turns particles towards their local
+    vorticity direction, which round-off fixes where that vorticity is tiny;
+    on the mirrored lattice the round-off's sign can pick either of two mirror
+    images of the wake, and the two signs of the change show both)."""
     a = oracle.hybrid_run(replace(case, vinf=case.vinf * (1.0 + 1e-14)))
-    g = case.grids.copy()
-    g[..., 0] += np.where(g[..., 1] > 0, 1e-14 * np.abs(g[..., 0]).max(), 0.0)
-    b = oracle.hybrid_run(replace(case, grids=g))
-    return a, b
+    out = [a]
+    for sgn in (1.0, -1.0):
+        g = case.grids.copy()
+        g[..., 0] += np.where(g[..., 1] > 0, sgn * 1e-14 * np.abs(g[..., 0]).max(), 0.0)
+        out.append(oracle.hybrid_run(replace(case, grids=g)))
+    return tuple(out)
+
+
+def mirror_image(case, res):
+    """The mirror image (y -> -y) of a run's state on a single lattice
+    symmetric about y = 0, or None: circulations with the span order reversed
+    in every chordwise row, particles reflected (positions y -> -y, strengths
+    as axial vectors (-ax, ay, -az)).  The algorithm is mirror-equivariant, so
+    on such a lattice both images are solutions; where the relaxation breaks
+    the symmetry, round-off picks one of them."""
+    if len(case.nc) != 1:
+        return None
+    nc, ns = case.nc[0], case.ns[0]
+    y = case.grids[0][:, 1]   # the first step's lattice vertices
+    if not np.allclose(np.sort(y), np.sort(-y), rtol=0.0, atol=1e-12 * np.abs(y).max()):
+        return None
+    out = dict(res)
+    out["gamma"] = res["gamma"].reshape(nc, ns)[:, ::-1].reshape(-1).copy()
+    if "pos" in res and res["pos"] is not None and len(res["pos"]):
+        out["pos"] = res["pos"] * np.array([1.0, -1.0, 1.0])
+        out["gamma_p"] = res["gamma_p"] * np.array([-1.0, 1.0, -1.0])
+    return out
+
+
+def matched(dev, ref):
+    """ref's particles reordered to dev's: nearest neighbour in (position,
+    strength), which tells apart the coincident particles of opposite strength
+    that the conversion makes; the match must be one to one."""
+    from scipy.spatial import cKDTree
+    L = max(np.abs(ref["pos"]).max(), 1e-300)
+    A = max(np.abs(ref["gamma_p"]).max(), 1e-300)
+    key = lambda r, a: np.concatenate([r / L, a / A], axis=1)
+    _, idx = cKDTree(key(ref["pos"], ref["gamma_p"])).query(key(dev["pos"], dev["alpha"]))
+    assert np.unique(idx).size == idx.size, "particles do not match one to one"
+    out = dict(ref)
+    out["pos"], out["gamma_p"] = ref["pos"][idx], ref["gamma_p"][idx]
+    return out
 
 
 def response(ref, runs, key):
@@ -83,12 +123,24 @@ def compare(hyb, oracle, case, cl_tol=1e-10, part=True, flat=None, dev=None):
     assert bad.size == 0, (bad[:5], dcl[bad[:5]], bound[bad[:5]])
     well = int(np.sum(sens < 1e-12))
     assert well >= min(case.num_steps, case.n_keep + 1), f"only {well} well-conditioned steps"
-    assert relerr(dev["gamma"], ref["gamma"]) < max(1e-10, 30.0 * response(ref, runs, "gamma"))
+    g_bound = max(1e-10, 30.0 * response(ref, runs, "gamma"))
+    cmp_ref = ref
+    if relerr(dev["gamma"], ref["gamma"]) >= g_bound:
+        # CL / CDi (mirror-invariant) agreed above; the state may be the
+        # other mirror image of a symmetry-broken wake
+        m = mirror_image(case, ref)
+        if m is not None and relerr(dev["gamma"], m["gamma"]) < g_bound:
+            print("[hybrid parity] the device's wake is the oracle's mirror image", flush=True)
+            cmp_ref = m
+    assert relerr(dev["gamma"], cmp_ref["gamma"]) < g_bound
     if part and all(r["np"] == ref["np"] for r in runs):
         assert dev["np"] == ref["np"]
         if ref["np"]:
-            assert relerr(dev["pos"], ref["pos"]) < max(1e-10, 30.0 * response(ref, runs, "pos"))
-            assert relerr(dev["alpha"], ref["gamma_p"]) < max(
+            if cmp_ref is not ref:
+                cmp_ref = matched(dev, cmp_ref)
+            assert relerr(dev["pos"], cmp_ref["pos"]) < max(1e-10,
+                                                            30.0 * response(ref, runs, "pos"))
+            assert relerr(dev["alpha"], cmp_ref["gamma_p"]) < max(
                 1e-9, 30.0 * response(ref, runs, "gamma_p"))
     return dev, ref, sens
 
