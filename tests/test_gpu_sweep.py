@@ -220,3 +220,33 @@ def test_sweep_hbm_wake_at_the_panel_limit(oracle):
         ok = ~np.isnan(ref["tip_w"])
         assert _rel(got["tip_w"][ok], ref["tip_w"][ok]) < 1e-10, i
     sw.close()
+
+
+def test_sweep_particle_capacity_overflow_stops_only_that_instance():
+    """An instance whose particle capacity is too small for its first
+    conversion (32 particles into 10 slots) stops with status -2 and takes no
+    further steps; the other instances of the same launches run on, equal bit
+    for bit to a sweep without the failing instance (instances are
+    independent)."""
+    from dataclasses import replace
+    good = [ae.goland_instance(V, num_chords=4) for V in (100.0, 150.0)]
+    tiny = replace(ae.goland_instance(120.0, num_chords=4), max_particles=10)
+    both = ae.Sweep([good[0], tiny, good[1]], use_graph=True)
+    both.step()
+    alone = ae.Sweep(good, use_graph=True)
+    alone.step()
+    bad = both.read(1)
+    assert bad["status"] == -2
+    assert bad["np"] == 0
+    # row n_keep converts at the end of step n_keep, the step that fails: its
+    # CL is written, no later step runs
+    k = tiny.n_keep
+    assert bad["cl"][k] != 0.0 and np.all(bad["cl"][k + 1:] == 0.0)
+    for i, j in ((0, 0), (2, 1)):
+        a, b = both.read(i), alone.read(j)
+        assert a["status"] == 0 and b["status"] == 0
+        for key in ("cl", "cdi", "tip_w", "tip_theta", "gamma"):
+            assert np.array_equal(a[key], b[key], equal_nan=True), (i, key)
+        assert a["np"] == b["np"] > 0
+    both.close()
+    alone.close()
