@@ -32,6 +32,11 @@ PRECISIONS = {"fp64": PREC_FP64, "mixed": PREC_MIXED}
 # as the reference's CPU field (kernel.py:105,156), which HybridSolver runs
 PAIR_RULES = {"skip": 0, "clamp": 1}
 
+# fv_stage_velocity_fn: int (*)(void *user, int64_t n, const double *pos,
+#                               double *u_bound, double *u_inf)
+STAGE_VELOCITY_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int64, C.POINTER(C.c_double),
+                                C.POINTER(C.c_double), C.POINTER(C.c_double))
+
 LEGS_RING = 0xF
 LEGS_CHORDWISE = 0x5
 LEGS_HORSESHOE = 0x7
@@ -123,6 +128,8 @@ _SIGNATURES = {
     "fv_vpm_write_host": (C.c_int, [_vp, C.c_int64, _dp, _dp, _dp, _dp]),
     "fv_vpm_induce_host": (C.c_int, [_vp, _dp, C.c_int64, _dp]),
     "fv_vpm_advect_rk3": (C.c_int, [_vp, C.c_double, _dp, C.c_int32, C.c_int32]),
+    "fv_vpm_advect_rk3_ext": (C.c_int, [_vp, C.c_double, C.c_void_p, C.c_void_p, C.c_int32,
+                                        C.c_int32]),
     "fv_convert_rings_host": (C.c_int, [_vp, C.c_int64, _dp, _dp, _dp, _dp, _dp, C.c_double,
                                         _i64p]),
     "fv_vpm_drop_oldest": (C.c_int, [_vp, C.c_int64]),

@@ -8,8 +8,13 @@
 #include <cmath>
 #include <cstring>
 #include <new>
+#include <vector>
 
 #include "fv_vpm.h"
+
+// (n, 3) from three columns, defined with the host-API staging below
+extern "C" __global__ void k_soa_to_aos(const double *__restrict__ a, const double *__restrict__ b,
+                             const double *__restrict__ c, int64_t n, double *__restrict__ out);
 
 namespace fv {
 
@@ -19,13 +24,21 @@ namespace fv {
 __global__ void k_vpm_stage_pos(const int64_t *__restrict__ np_dev, double *__restrict__ x,
                                 double *__restrict__ y, double *__restrict__ z,
                                 const double *__restrict__ upart, double ux, double uy,
-                                double uz, double a, double b, double dt,
+                                double uz, const double *__restrict__ ub,
+                                const double *__restrict__ ui, double a, double b, double dt,
                                 double *__restrict__ rkX)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= *np_dev) return;
     double u0 = ux, u1 = uy, u2 = uz;
-    if (upart) {
+    if (ub) {
+        // the callables' fields: (U_part + U_bound) + U_inf, :133
+        const double p0 = upart ? upart[3 * i] : 0.0, p1 = upart ? upart[3 * i + 1] : 0.0,
+                     p2 = upart ? upart[3 * i + 2] : 0.0;
+        u0 = (p0 + ub[3 * i]) + ui[3 * i];
+        u1 = (p1 + ub[3 * i + 1]) + ui[3 * i + 1];
+        u2 = (p2 + ub[3 * i + 2]) + ui[3 * i + 2];
+    } else if (upart) {
         u0 = (upart[3 * i] + 0.0) + ux;
         u1 = (upart[3 * i + 1] + 0.0) + uy;
         u2 = (upart[3 * i + 2] + 0.0) + uz;
@@ -300,11 +313,41 @@ int vpm_induce(fv_vpm *f, const double *targets, int64_t n, int precision, doubl
     return FV_OK;
 }
 
-int vpm_advect(fv_vpm *f, double dt, const double *uinf, int stretch, int free_wake)
+// The callables' fields at the current (stage) positions into the staging
+// buffer: positions to the host, the callback, u_bound then u_inf back
+// (f->stage[0, 3n) and [3n, 6n)); synchronous.
+static int stage_fields(fv_vpm *f, fv_stage_velocity_fn ext, void *user, std::vector<double> &h)
 {
     fv_ctx *ctx = f->ctx;
+    const int64_t n = f->np_host;
+    h.assign((size_t)9 * n, 0.0);
+    hipLaunchKernelGGL(k_soa_to_aos, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, ctx->stream,
+                       f->x, f->y, f->z, n, f->stage);
+    FV_LAUNCH_CHECK(ctx);
+    FV_HIP_CHECK(ctx, hipMemcpyAsync(h.data(), f->stage, sizeof(double) * 3 * n,
+                                     hipMemcpyDeviceToHost, ctx->stream));
+    FV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    const int r = ext(user, n, h.data(), h.data() + 3 * n, h.data() + 6 * n);
+    if (r != 0) return set_error(ctx, FV_E_ARG, "stage velocity callback returned %d", r);
+    FV_HIP_CHECK(ctx, hipMemcpyAsync(f->stage, h.data() + 3 * n, sizeof(double) * 6 * n,
+                                     hipMemcpyHostToDevice, ctx->stream));
+    FV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return FV_OK;
+}
+
+int vpm_advect(fv_vpm *f, double dt, const double *uinf, int stretch, int free_wake,
+               fv_stage_velocity_fn ext, void *user)
+{
+    fv_ctx *ctx = f->ctx;
+    if (ext && !f->np_exact) {
+        const int rc = vpm_sync_count(f);
+        if (rc) return rc;
+    }
     const int64_t nu = launch_count(f);
     if (nu <= 0) return FV_OK;
+    static const double kZero[3] = {0.0, 0.0, 0.0};
+    if (ext) uinf = kZero;
+    std::vector<double> hbuf;
     static const double ra[3] = {0.0, -5.0 / 9.0, -153.0 / 128.0};   // particles.py:114
     static const double rb[3] = {1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0};
     int rc = FV_OK;
@@ -338,9 +381,16 @@ int vpm_advect(fv_vpm *f, double dt, const double *uinf, int stretch, int free_w
             if (rc) return rc;
             upart = f->sum;
         }
+        const double *ub = nullptr, *ui = nullptr;
+        if (ext) {
+            rc = stage_fields(f, ext, user, hbuf);
+            if (rc) return rc;
+            ub = f->stage;
+            ui = f->stage + 3 * f->np_host;
+        }
         hipLaunchKernelGGL(k_vpm_stage_pos, dim3(gb), dim3(256), 0, ctx->stream, f->np_dev, f->x,
-                           f->y, f->z, upart, uinf[0], uinf[1], uinf[2], ra[st], rb[st], dt,
-                           f->rkX);
+                           f->y, f->z, upart, uinf[0], uinf[1], uinf[2], ub, ui, ra[st], rb[st],
+                           dt, f->rkX);
         FV_LAUNCH_CHECK(ctx);
         f->state_epoch++;
         if (stretch) {
@@ -686,6 +736,18 @@ int fv_vpm_advect_rk3(fv_vpm *f, double dt, const double *uinf, int32_t stretch,
     fv_ctx *ctx = f->ctx;
     FV_REQUIRE(ctx, uinf, "null uinf");
     int rc = vpm_advect(f, dt, uinf, stretch, free_wake);
+    if (rc) return rc;
+    FV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return FV_OK;
+}
+
+int fv_vpm_advect_rk3_ext(fv_vpm *f, double dt, fv_stage_velocity_fn fn, void *user,
+                          int32_t stretch, int32_t free_wake)
+{
+    if (!f) return FV_E_ARG;
+    fv_ctx *ctx = f->ctx;
+    FV_REQUIRE(ctx, fn, "null stage velocity callback");
+    int rc = vpm_advect(f, dt, nullptr, stretch, free_wake, fn, user);
     if (rc) return rc;
     FV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     return FV_OK;

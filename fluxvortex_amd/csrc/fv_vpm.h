@@ -49,7 +49,10 @@ inline int64_t launch_count(const fv_vpm *f)
 }
 
 // Enqueues one advect_rk3 + relaxation step (particles.py:85-246).
-int vpm_advect(fv_vpm *f, double dt, const double *uinf, int stretch, int free_wake);
+// With ext (nullable) the freestream and the bound-vortex velocity come from
+// the host callback at every stage (uinf unused).
+int vpm_advect(fv_vpm *f, double dt, const double *uinf, int stretch, int free_wake,
+               fv_stage_velocity_fn ext = nullptr, void *user = nullptr);
 
 // Enqueues the conversion of nr rings (AoS corners on the device) into
 // particles appended to the field; converted rings get gamma = 0.

@@ -140,25 +140,49 @@ class VortexParticleField:
                    free_wake=True):
         """One RK3 step followed by Pedrizzetti relaxation, on the device.
 
-        ``U_inf_func`` may be a (3,) vector or a callable returning a uniform
-        (N, 3) field (the only form the reference's callers use,
-        tests/experiment_hybrid_panel_particle.py:140-143).  Non-uniform
-        freestreams and ``bound_velocity_func`` are not supported."""
-        if bound_velocity_func is not None:
-            raise NotImplementedError("bound_velocity_func is not supported on the device path")
-        if callable(U_inf_func):
+        ``U_inf_func`` is a (3,) vector or a callable X (N, 3) -> (N, 3);
+        ``bound_velocity_func`` (optional) a callable X -> (N, 3), the velocity
+        the bound wing vortices induce.  A uniform freestream without a bound
+        velocity runs entirely on the device; otherwise the callables are
+        evaluated at every RK stage's positions, as the reference does
+        (particles.py:126-133: U_total = U_part + U_bound + U_inf), each stage
+        reading the positions back to the host."""
+        uniform = None
+        if not callable(U_inf_func):
+            uniform = _lib.f64(U_inf_func).reshape(3)
+        elif bound_velocity_func is None:
             pos = self.positions
             if pos.shape[0] == 0:
                 return
-            U = np.asarray(U_inf_func(pos), dtype=np.float64).reshape(-1, 3)
-            if not np.all(U == U[0]):
-                raise NotImplementedError("only uniform freestreams are supported")
-            uinf = np.ascontiguousarray(U[0])
-        else:
-            uinf = _lib.f64(U_inf_func).reshape(3)
-        self._ctx.check(self._ctx.lib.fv_vpm_advect_rk3(self._h, float(dt), _lib.ptr(uinf),
-                                                        1 if stretch else 0,
-                                                        1 if free_wake else 0))
+            U = np.broadcast_to(np.asarray(U_inf_func(pos), dtype=np.float64), pos.shape)
+            if np.all(U == U[0]):
+                uniform = np.ascontiguousarray(U[0])
+        if uniform is not None and bound_velocity_func is None:
+            self._ctx.check(self._ctx.lib.fv_vpm_advect_rk3(self._h, float(dt), _lib.ptr(uniform),
+                                                            1 if stretch else 0,
+                                                            1 if free_wake else 0))
+            return
+        u_inf = (lambda X: uniform) if uniform is not None else U_inf_func
+        errors = []
+
+        def stage(_user, n, p, ub, ui):
+            try:
+                X = np.ctypeslib.as_array(p, shape=(n, 3)).copy()
+                np.ctypeslib.as_array(ub, shape=(n, 3))[...] = (
+                    0.0 if bound_velocity_func is None else bound_velocity_func(X))
+                np.ctypeslib.as_array(ui, shape=(n, 3))[...] = u_inf(X)
+                return 0
+            except Exception as e:   # raised below, after the library returns
+                errors.append(e)
+                return 1
+
+        cb = _lib.STAGE_VELOCITY_FN(stage)
+        rc = self._ctx.lib.fv_vpm_advect_rk3_ext(self._h, float(dt),
+                                                 C.cast(cb, C.c_void_p), None,
+                                                 1 if stretch else 0, 1 if free_wake else 0)
+        if errors:
+            raise errors[0]
+        self._ctx.check(rc)
 
     # -- learned far-field closure -------------------------------------------
     def enable_closure(self, mlp=None, cell=2.0, kappa=4.0, check_every=10, tol=1e-3,

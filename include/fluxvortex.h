@@ -217,6 +217,18 @@ int fv_vpm_set_pair_rule(fv_vpm *f, int32_t rule);
  * constant freestream uinf[3]. */
 int fv_vpm_advect_rk3(fv_vpm *f, double dt, const double *uinf, int32_t stretch,
                       int32_t free_wake);
+/* The stage velocity of advect_rk3's callables U_inf_func and
+ * bound_velocity_func (particles.py:126-133): called at every RK stage with
+ * the stage positions (n, 3, host memory); fills u_bound and u_inf (n, 3)
+ * and returns 0, or nonzero to abort the step. */
+typedef int (*fv_stage_velocity_fn)(void *user, int64_t n, const double *pos, double *u_bound,
+                                    double *u_inf);
+/* particles.py:85-199 with the callables (U_total = U_part + U_bound + U_inf
+ * at every stage) followed by _relax_pedrizzetti (:202-246).  Every stage
+ * reads the positions back and uploads the two fields (synchronous); a
+ * nonzero return of fn aborts the step with FV_E_ARG. */
+int fv_vpm_advect_rk3_ext(fv_vpm *f, double dt, fv_stage_velocity_fn fn, void *user,
+                          int32_t stretch, int32_t free_wake);
 
 /* ----------------------------------------- panel -> particle conversion (a11) */
 /* Each ring with |gamma| >= 1e-15 yields its leg particles (front, left,

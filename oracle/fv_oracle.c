@@ -388,9 +388,34 @@ int orc_vpm_advect_rk3(int64_t n, double *pos, double *gamma, double *sigma, dou
                                    kernel, nu, 0, nthreads);
 }
 
+static int advect_impl(int64_t n, double *pos, double *gamma, double *sigma, double *age,
+                       double dt, const double *Uinf, orc_stage_velocity_fn ext, void *user,
+                       int stretch, int free_wake, int kernel, double nu, int rule,
+                       int nthreads);
+
 int orc_vpm_advect_rk3_rule(int64_t n, double *pos, double *gamma, double *sigma, double *age,
                             double dt, const double *Uinf, int stretch, int free_wake,
                             int kernel, double nu, int rule, int nthreads)
+{
+    return advect_impl(n, pos, gamma, sigma, age, dt, Uinf, NULL, NULL, stretch, free_wake,
+                       kernel, nu, rule, nthreads);
+}
+
+/* advect_rk3 with U_inf_func and bound_velocity_func evaluated at every stage's
+ * positions (particles.py:126-133): ext fills U_bound and U_inf (n, 3). */
+int orc_vpm_advect_rk3_ext(int64_t n, double *pos, double *gamma, double *sigma, double *age,
+                           double dt, orc_stage_velocity_fn ext, void *user, int stretch,
+                           int free_wake, int kernel, double nu, int rule, int nthreads)
+{
+    if (!ext) return -1;
+    return advect_impl(n, pos, gamma, sigma, age, dt, NULL, ext, user, stretch, free_wake,
+                       kernel, nu, rule, nthreads);
+}
+
+static int advect_impl(int64_t n, double *pos, double *gamma, double *sigma, double *age,
+                       double dt, const double *Uinf, orc_stage_velocity_fn ext, void *user,
+                       int stretch, int free_wake, int kernel, double nu, int rule,
+                       int nthreads)
 {
     if (n == 0) return 0;
     static const double ra[3] = {0.0, -5.0 / 9.0, -153.0 / 128.0};   /* :114 */
@@ -402,8 +427,10 @@ int orc_vpm_advect_rk3_rule(int64_t n, double *pos, double *gamma, double *sigma
     double *rks = calloc((size_t)n, sizeof(double));
     double *U = calloc(n3, sizeof(double));
     double *J = stretch ? calloc(9 * (size_t)n, sizeof(double)) : NULL;
-    if (!rkX || !rkG || !rks || !U || (stretch && !J)) {
-        free(rkX); free(rkG); free(rks); free(U); free(J);
+    double *Ub = ext ? calloc(n3, sizeof(double)) : NULL;
+    double *Ui = ext ? calloc(n3, sizeof(double)) : NULL;
+    if (!rkX || !rkG || !rks || !U || (stretch && !J) || (ext && (!Ub || !Ui))) {
+        free(rkX); free(rkG); free(rks); free(U); free(J); free(Ub); free(Ui);
         return -1;
     }
     for (int st = 0; st < 3; ++st) {
@@ -412,10 +439,16 @@ int orc_vpm_advect_rk3_rule(int64_t n, double *pos, double *gamma, double *sigma
             orc_particle_vel_rule(pos, n, pos, gamma, sigma, n, kernel, rule, U, nthreads);
         else
             memset(U, 0, n3 * sizeof(double));
+        if (ext && ext(user, n, pos, Ub, Ui) != 0) {                      /* :126-131 */
+            free(rkX); free(rkG); free(rks); free(U); free(J); free(Ub); free(Ui);
+            return -2;
+        }
         for (int64_t i = 0; i < n; ++i) {
-            /* U_total = U_part + U_bound(=0) + U_inf, :133 */
+            /* U_total = U_part + U_bound + U_inf, :133 */
             double u[3];
-            for (int c = 0; c < 3; ++c) u[c] = U[3 * i + c] + 0.0 + Uinf[c];
+            for (int c = 0; c < 3; ++c)
+                u[c] = ext ? (U[3 * i + c] + Ub[3 * i + c]) + Ui[3 * i + c]
+                           : U[3 * i + c] + 0.0 + Uinf[c];
             double sp = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);   /* :136-140 */
             if (sp > 50.0) {
                 double f = 50.0 / sp;
@@ -468,7 +501,7 @@ int orc_vpm_advect_rk3_rule(int64_t n, double *pos, double *gamma, double *sigma
         if (bad) gamma[3 * i] = gamma[3 * i + 1] = gamma[3 * i + 2] = 0.0;
         age[i] += dt;
     }
-    free(rkX); free(rkG); free(rks); free(U); free(J);
+    free(rkX); free(rkG); free(rks); free(U); free(J); free(Ub); free(Ui);
     return 0;
 }
 

@@ -38,6 +38,13 @@ int orc_vpm_advect_rk3_rule(int64_t n, double *pos, double *gamma, double *sigma
                             int kernel, double nu, int rule, int nthreads);
 int orc_vpm_relax_rule(int64_t n, const double *pos, double *gamma, const double *sigma,
                        double rlxf, int rule, int nthreads);
+/* Stage velocity of advect_rk3's callables (particles.py:126-131): the stage
+ * positions (n, 3) in, U_bound and U_inf (n, 3) out; nonzero aborts. */
+typedef int (*orc_stage_velocity_fn)(void *user, int64_t n, const double *pos, double *u_bound,
+                                     double *u_inf);
+int orc_vpm_advect_rk3_ext(int64_t n, double *pos, double *gamma, double *sigma, double *age,
+                           double dt, orc_stage_velocity_fn ext, void *user, int stretch,
+                           int free_wake, int kernel, double nu, int rule, int nthreads);
 double orc_erf(double x);
 int64_t orc_convert_rings(int64_t nr, const double *Br, const double *Fr, const double *Fl,
                           const double *Bl, double *gamma, double sigma, double *pos_out,

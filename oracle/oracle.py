@@ -54,6 +54,9 @@ def lib() -> C.CDLL:
         L.orc_vpm_advect_rk3_rule.argtypes = [i64, _dp, _dp, _dp, _dp, C.c_double, _dp, C.c_int,
                                               C.c_int, C.c_int, C.c_double, C.c_int, C.c_int]
         L.orc_vpm_relax_rule.argtypes = [i64, _dp, _dp, _dp, C.c_double, C.c_int, C.c_int]
+        L.orc_vpm_advect_rk3_ext.argtypes = [i64, _dp, _dp, _dp, _dp, C.c_double, STAGE_VEL_FN,
+                                             C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double,
+                                             C.c_int, C.c_int]
         L.orc_erf.argtypes = [C.c_double]
         L.orc_erf.restype = C.c_double
         L.orc_convert_rings.argtypes = [i64, _dp, _dp, _dp, _dp, _dp, C.c_double, _dp, _dp, _dp]
@@ -148,6 +151,45 @@ def vpm_advect_rk3(pos, gamma, sigma, age, dt, uinf, stretch, free_wake, kernel,
     rc = L.orc_vpm_advect_rk3_rule(n, _p(pos), _p(gamma), _p(sigma), _p(age), dt, _p(u),
                                    int(stretch), int(free_wake), KERNELS[kernel], nu, int(rule),
                                    threads)
+    assert rc == 0
+    rc = L.orc_vpm_relax_rule(n, _p(pos), _p(gamma), _p(sigma), rlxf, int(rule), threads)
+    assert rc == 0
+    return pos, gamma, sigma, age
+
+
+# int (*)(void *user, int64_t n, const double *pos, double *u_bound, double *u_inf)
+STAGE_VEL_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int64, C.POINTER(C.c_double),
+                           C.POINTER(C.c_double), C.POINTER(C.c_double))
+
+
+def vpm_advect_rk3_ext(pos, gamma, sigma, age, dt, u_inf_func, bound_velocity_func, stretch,
+                       free_wake, kernel, nu=0.0, rlxf=0.3, threads=0, rule=0):
+    """advect_rk3 with the reference's callables U_inf_func(X) and
+    bound_velocity_func(X) (None: zeros) evaluated at every stage's positions
+    (particles.py:126-133), then _relax_pedrizzetti; returns new arrays."""
+    pos, gamma = _f(pos).copy(), _f(gamma).copy()
+    sigma, age = _f(sigma).copy(), _f(age).copy()
+    n = pos.shape[0]
+    errors = []
+
+    def stage(_user, m, p, ub, ui):
+        try:
+            X = np.ctypeslib.as_array(p, shape=(m, 3)).copy()
+            np.ctypeslib.as_array(ub, shape=(m, 3))[...] = (
+                0.0 if bound_velocity_func is None else bound_velocity_func(X))
+            np.ctypeslib.as_array(ui, shape=(m, 3))[...] = u_inf_func(X)
+            return 0
+        except Exception as e:   # reported after the C call returns
+            errors.append(e)
+            return 1
+
+    cb = STAGE_VEL_FN(stage)
+    L = lib()
+    rc = L.orc_vpm_advect_rk3_ext(n, _p(pos), _p(gamma), _p(sigma), _p(age), dt, cb, None,
+                                  int(stretch), int(free_wake), KERNELS[kernel], nu, int(rule),
+                                  threads)
+    if errors:
+        raise errors[0]
     assert rc == 0
     rc = L.orc_vpm_relax_rule(n, _p(pos), _p(gamma), _p(sigma), rlxf, int(rule), threads)
     assert rc == 0

@@ -23,7 +23,13 @@ reference checkout (pure NumPy + SciPy) and records, for seeded inputs,
                      HybridSolver converts rows (four per ring at the leg
                      midpoints, shared legs coincide exactly), Gaussian,
                      sigma = V dt, free wake, no stretching, rlxf = 0.3
-                     (experiment_hybrid_panel_particle.py:111,140-143,173-203).
+                     (experiment_hybrid_panel_particle.py:111,140-143,173-203);
+      - cloud_w_ext  Winckelmans, stretching + free wake, rlxf = 0, with a
+                     non-uniform U_inf_func and a bound_velocity_func
+                     (tests/helpers.py: shear_freestream, swirl_bound), both
+                     evaluated at every stage's positions (:126-133);
+      - cloud_g_ext  Gaussian, prescribed wake (free_wake False: zeros +
+                     U_bound + U_inf), rlxf = 0.3, the same callables.
 The fixture holds inputs and outputs only.
 
     python tests/golden/make_ref_vpm.py [/path/to/reference]
@@ -35,6 +41,8 @@ import numpy as np
 
 HERE = Path(__file__).resolve().parent
 REF = Path(sys.argv[1]) if len(sys.argv) > 1 else Path("/root/reference")
+sys.path.insert(0, str(HERE.parent))
+from helpers import shear_freestream, swirl_bound  # noqa: E402
 
 STEPS = 5
 
@@ -97,16 +105,23 @@ def main():
                             stretch=False, free=True, dt=K05_DT, uinf=(10.0, 0.0, 0.0)),
         "sheet_rlx": dict(inp=sheet(), kernel="gaussianerf", rlxf=0.3, stretch=False,
                           free=True, dt=K05_DT, uinf=(10.0, 0.0, 0.0)),
+        "cloud_w_ext": dict(inp=cloud(200, 15, 0.1), kernel="winckelmans", rlxf=0.0,
+                            stretch=True, free=True, dt=0.01, uinf=None),
+        "cloud_g_ext": dict(inp=cloud(200, 16, 10.0 * K05_DT), kernel="gaussianerf", rlxf=0.3,
+                            stretch=False, free=False, dt=K05_DT, uinf=None),
     }
     for name, c in cases.items():
         p0, a0, s0 = c["inp"]
         f = VortexParticleField(max_particles=len(p0), nu=0.0, rlxf=c["rlxf"],
                                 kernel=c["kernel"])
         f.add_particles_batch(p0, a0, s0)
-        U = np.array(c["uinf"], dtype=np.float64)
+        ext = c["uinf"] is None
+        U = np.full(3, np.nan) if ext else np.array(c["uinf"], dtype=np.float64)
+        uinf_func = shear_freestream if ext else (lambda X: np.broadcast_to(U, X.shape).copy())
+        bound = swirl_bound if ext else None
         traj_p, traj_g, traj_s = [], [], []
         for _ in range(STEPS):
-            f.advect_rk3(c["dt"], lambda X: np.broadcast_to(U, X.shape).copy(),
+            f.advect_rk3(c["dt"], uinf_func, bound_velocity_func=bound,
                          stretch=c["stretch"], free_wake=c["free"])
             traj_p.append(f.positions)
             traj_g.append(f.gammas)
@@ -115,7 +130,7 @@ def main():
         out[f"{name}_pos"] = np.array(traj_p)
         out[f"{name}_alpha"] = np.array(traj_g)
         out[f"{name}_sigma"] = np.array(traj_s)
-        out[f"{name}_params"] = np.array([c["dt"], *c["uinf"], c["rlxf"], float(c["stretch"]),
+        out[f"{name}_params"] = np.array([c["dt"], *U, c["rlxf"], float(c["stretch"]),
                                           float(c["free"]),
                                           1.0 if c["kernel"] == "winckelmans" else 0.0])
     np.savez_compressed(HERE / "ref_vpm.npz", **out)

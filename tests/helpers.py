@@ -39,3 +39,18 @@ def sheet_rings(nx, ny, seed=42, lx=20.0, ly=10.0):
     rc0 = np.full(m, 0.03)
     age = rng.uniform(0.0, 1.0, m)
     return br.copy(), fr.copy(), fl.copy(), bl.copy(), gamma, rc0, age
+
+
+def shear_freestream(X):
+    """A non-uniform freestream (N, 3) at points X (N, 3) for advect_rk3's
+    callable U_inf_func (the reference-written vectors and the tests use it)."""
+    X = np.asarray(X, dtype=np.float64)
+    return np.stack([10.0 + 0.3 * X[:, 2], 0.1 * X[:, 0], 0.5 - 0.2 * X[:, 1]], axis=1)
+
+
+def swirl_bound(X):
+    """A smooth stand-in for the bound vortices' velocity (N, 3) at points X,
+    for advect_rk3's bound_velocity_func."""
+    X = np.asarray(X, dtype=np.float64)
+    w = 0.05 / (1.0 + X[:, 0] ** 2 + X[:, 1] ** 2)
+    return np.stack([-w * X[:, 1], w * X[:, 0], np.zeros(X.shape[0])], axis=1)

@@ -5,7 +5,7 @@ from pathlib import Path
 import numpy as np
 import pytest
 
-from helpers import particle_cloud, relerr
+from helpers import particle_cloud, relerr, shear_freestream, swirl_bound
 
 pytestmark = pytest.mark.gpu
 
@@ -74,6 +74,8 @@ REF_CASES = {
     "cloud_w_rlx": (1e-12, 1e-12, 1e-12, 2),
     "cloud_g_rlx": (1e-12, 1e-12, 1e-12, 5),
     "sheet_rlx": (1e-12, 2e-10, 1e-12, 5),
+    "cloud_w_ext": (1e-12, 1e-12, 1e-12, 5),
+    "cloud_g_ext": (1e-12, 1e-12, 1e-12, 5),
 }
 
 
@@ -90,12 +92,61 @@ def test_advect_matches_reference_vectors(ctx, name, precision):
     f = make_field(z[f"{name}_pos0"], z[f"{name}_alpha0"], z[f"{name}_sigma0"], kernel,
                    rlxf=prm[4], precision=precision, rule="clamp")
     print(f"[ref vectors] {name} {kernel} rlxf={prm[4]}", flush=True)
+    ext = bool(np.isnan(prm[1]))   # the callables of tests/helpers.py
     for i in range(nsteps):
-        f.advect_rk3(prm[0], prm[1:4], stretch=bool(prm[5]), free_wake=bool(prm[6]))
+        if ext:
+            f.advect_rk3(prm[0], shear_freestream, bound_velocity_func=swirl_bound,
+                         stretch=bool(prm[5]), free_wake=bool(prm[6]))
+        else:
+            f.advect_rk3(prm[0], prm[1:4], stretch=bool(prm[5]), free_wake=bool(prm[6]))
         st = f.state_dict()
         e = (relerr(st["pos"], z[f"{name}_pos"][i]), relerr(st["gamma"], z[f"{name}_alpha"][i]),
              relerr(st["sigma"], z[f"{name}_sigma"][i]))
         assert e[0] < tp and e[1] < ta and e[2] < ts, (name, i, e)
+
+
+@pytest.mark.parametrize("kernel", ["gaussianerf", "winckelmans"])
+def test_advect_stage_callables_match_oracle(oracle, ctx, kernel):
+    """advect_rk3 with a non-uniform U_inf_func and a bound_velocity_func
+    (evaluated at every stage's positions, particles.py:126-133) against the
+    oracle's same path, three steps, free wake + stretching + relaxation (the
+    skip pair rule: under the clamp rule the stretched cores' last bits steer
+    the relaxation); and
+    a bound velocity of zeros beside a uniform freestream (the callback path)
+    equals the all-device uniform path bit for bit."""
+    pos, al, sg = particle_cloud(600, seed=21, sigma=0.1)
+    f = make_field(pos, al, sg, kernel, rlxf=0.3)
+    p, g, s, a = pos.copy(), al.copy(), sg.copy(), np.zeros(len(sg))
+    for _ in range(3):
+        f.advect_rk3(0.01, shear_freestream, bound_velocity_func=swirl_bound, stretch=True,
+                     free_wake=True)
+        p, g, s, a = oracle.vpm_advect_rk3_ext(p, g, s, a, 0.01, shear_freestream, swirl_bound,
+                                               True, True, kernel, rlxf=0.3, rule=0)
+    st = f.state_dict()
+    e = (relerr(st["pos"], p), relerr(st["gamma"], g), relerr(st["sigma"], s))
+    print(f"stage callables {kernel}: {e}", flush=True)
+    assert e[0] < 1e-12 and e[1] < 1e-10 and e[2] < 1e-12
+    U = np.array([10.0, 0.0, 0.5])
+    f1 = make_field(pos, al, sg, kernel, rlxf=0.0)
+    f2 = make_field(pos, al, sg, kernel, rlxf=0.0)
+    for _ in range(2):
+        f1.advect_rk3(0.01, U, stretch=True, free_wake=True)
+        f2.advect_rk3(0.01, U, bound_velocity_func=lambda X: np.zeros_like(X), stretch=True,
+                      free_wake=True)
+    a1, a2 = f1.state_dict(), f2.state_dict()
+    for k in ("pos", "gamma", "sigma"):
+        assert np.array_equal(a1[k], a2[k]), k
+
+
+def test_advect_stage_callable_errors_propagate(ctx):
+    pos, al, sg = particle_cloud(50, seed=22)
+    f = make_field(pos, al, sg, "winckelmans")
+
+    def bad(X):
+        raise ValueError("no bound velocity here")
+
+    with pytest.raises(ValueError, match="no bound velocity"):
+        f.advect_rk3(0.01, np.array([10.0, 0.0, 0.0]), bound_velocity_func=bad)
 
 
 def test_golden_rk3(ctx):

@@ -11,7 +11,7 @@ from pathlib import Path
 import numpy as np
 import pytest
 
-from helpers import relerr
+from helpers import relerr, shear_freestream, swirl_bound
 
 GOLD = Path(__file__).parent / "golden" / "ref_vpm.npz"
 
@@ -31,13 +31,28 @@ CASES = {
     "cloud_w_rlx": (1, 1e-13, 1e-13, 1e-13, 2),
     "cloud_g_rlx": (1, 1e-13, 1e-13, 1e-13, 5),
     "sheet_rlx": (1, 1e-12, 1e-10, 1e-13, 5),
+    "cloud_w_ext": (1, 1e-13, 1e-13, 1e-13, 5),
+    "cloud_g_ext": (1, 1e-13, 1e-13, 1e-13, 5),
 }
 
 
 def case_args(z, name):
+    """The trajectory's parameters; ext: the reference ran advect_rk3 with the
+    callables U_inf_func = shear_freestream and bound_velocity_func =
+    swirl_bound (tests/helpers.py), stored as a NaN freestream."""
     prm = z[f"{name}_params"]
     return dict(dt=prm[0], uinf=prm[1:4], rlxf=prm[4], stretch=bool(prm[5]), free=bool(prm[6]),
-                kernel="winckelmans" if prm[7] == 1 else "gaussianerf")
+                kernel="winckelmans" if prm[7] == 1 else "gaussianerf",
+                ext=bool(np.isnan(prm[1])))
+
+
+def oracle_step(oracle, c, pos, al, sg, age, rule):
+    if c["ext"]:
+        return oracle.vpm_advect_rk3_ext(pos, al, sg, age, c["dt"], shear_freestream, swirl_bound,
+                                         c["stretch"], c["free"], c["kernel"], rlxf=c["rlxf"],
+                                         rule=rule)
+    return oracle.vpm_advect_rk3(pos, al, sg, age, c["dt"], c["uinf"], c["stretch"], c["free"],
+                                 c["kernel"], rlxf=c["rlxf"], rule=rule)
 
 
 @pytest.fixture(scope="module")
@@ -86,9 +101,7 @@ def test_rk3_trajectories_match_reference(oracle, gold, name):
     pos, al, sg = gold[f"{name}_pos0"], gold[f"{name}_alpha0"], gold[f"{name}_sigma0"]
     age = np.zeros(len(sg))
     for i in range(nsteps):
-        pos, al, sg, age = oracle.vpm_advect_rk3(pos, al, sg, age, c["dt"], c["uinf"],
-                                                 c["stretch"], c["free"], c["kernel"],
-                                                 rlxf=c["rlxf"], rule=rule)
+        pos, al, sg, age = oracle_step(oracle, c, pos, al, sg, age, rule)
         e = (relerr(pos, gold[f"{name}_pos"][i]), relerr(al, gold[f"{name}_alpha"][i]),
              relerr(sg, gold[f"{name}_sigma"][i]))
         assert e[0] < tp and e[1] < ta and e[2] < ts, (name, i, e)
@@ -114,3 +127,17 @@ def test_stage_composition_is_what_pins_cloud_w(oracle, gold):
         g = g + b * qg
         s = s + b * qs
     assert relerr(g, gold["cloud_w_alpha"][0]) > 1e-8
+
+
+def test_stage_callables_are_what_pins_the_ext_cases(oracle, gold):
+    """The per-stage callables matter: the same trajectory with the freestream
+    frozen at the step's starting positions (evaluated once per step instead
+    of at every stage) misses the reference's vectors."""
+    c = case_args(gold, "cloud_w_ext")
+    pos, al, sg = gold["cloud_w_ext_pos0"], gold["cloud_w_ext_alpha0"], gold["cloud_w_ext_sigma0"]
+    age = np.zeros(len(sg))
+    frozen_u, frozen_b = shear_freestream(pos), swirl_bound(pos)
+    p1 = oracle.vpm_advect_rk3_ext(pos, al, sg, age, c["dt"], lambda X: frozen_u,
+                                   lambda X: frozen_b, c["stretch"], c["free"], c["kernel"],
+                                   rlxf=c["rlxf"], rule=1)[0]
+    assert relerr(p1, gold["cloud_w_ext_pos"][0]) > 1e-9
