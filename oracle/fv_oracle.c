@@ -27,11 +27,14 @@
  *       point-vortex limit, Jacobian vs central differences of the velocity,
  *       trace J = 0 at sigma -> 0, NumPy restatement.
  *   orc_lu_solve : residual vs numpy.linalg.solve.
- *   orc_vpm_advect_rk3 : constant field moves particles by exactly U dt, each
- *       clamp forced one at a time.  The combined clamp/relaxation branch order
- *       against the reference's own trajectory is parity unpinned (no golden
- *       trajectory exists in the reference; the reference module was not run).
- *   orc_vpm_relax : fixed points (Gamma parallel to omega), unit-norm limit.
+ *   orc_vpm_advect_rk3 / orc_vpm_advect_rk3_ext : constant field moves
+ *       particles by exactly U dt, each clamp forced one at a time; whole
+ *       trajectories written by the reference's own particles.py
+ *       (tests/golden/ref_vpm.npz, tests/test_ref_golden.py: the stage
+ *       composition, the clamped self pair, the per-stage callables), which a
+ *       planted stage-order mistake or a freestream frozen per step fails.
+ *   orc_vpm_relax : fixed points (Gamma parallel to omega), unit-norm limit,
+ *       and the same reference trajectories with rlxf = 0.3.
  *   orc_convert_rings : sum of alpha over a closed ring = 0, leg midpoints,
  *       far-field dipole equivalence.
  */
