@@ -48,7 +48,9 @@ PEAK_BF16 = 2.5e15           # dense BF16 MFMA
 PEAK_HBM = 8.0e12
 MLP_FLOPS_PER_SAMPLE = 2 * (32 * 128 + 2 * 128 * 128 + 128 * 16)   # 77 824
 BIG_STEPS = 10   # timed steps of the 100k block: one closure check period
-PREFILL = 150
+# heave cycles of free, unstretched wake before a timed window: nine reach
+# ~21k particles by themselves (the 20k line); larger states copy periods
+NATURAL_CYCLES = 9
 # HBM fetch of the stretching pair sum per target particle: FETCH_SIZE x 2.00
 # (the counter's scale on a plain copy) over the launches of the profiled
 # 20k bench, divided by their particle count (profiles/
@@ -61,22 +63,25 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def extend_wake(state, target, period_shift, period_age, block):
-    """Appends downstream copies of the oldest `block` particles (one heave
-    period of wake) shifted by k * V * T until `target` particles exist."""
+def extend_wake(state, target, period_shift, period_age, block, period=0):
+    """Appends downstream copies of one heave period of wake (the `block`
+    particles of the `period`-th oldest period) shifted by (k + period) V T,
+    k = 1, 2, ..., until `target` particles exist."""
     pos, gam, sig, age = state["pos"], state["gamma"], state["sigma"], state["age"]
     n0 = pos.shape[0]
     if target <= n0:
         return {k: v[:target].copy() for k, v in state.items()}
-    base = min(block, n0)
+    p0 = min(period * block, max(n0 - block, 0))
+    base = min(block, n0 - p0)
+    src = slice(p0, p0 + base)
     need = target - n0
     reps = []
     k = 1
     while need > 0:
         take = min(need, base)
-        p = pos[:take].copy()
-        p[:, 0] += k * period_shift
-        reps.append((p, gam[:take], sig[:take], age[:take] + k * period_age))
+        p = pos[src][:take].copy()
+        p[:, 0] += (k + period) * period_shift
+        reps.append((p, gam[src][:take], sig[src][:take], age[src][:take] + (k + period) * period_age))
         need -= take
         k += 1
     return {
@@ -88,18 +93,27 @@ def extend_wake(state, target, period_shift, period_age, block):
 
 
 def prepare(hybrid, particles, steps_after, perturb=0.0):
-    """A heaving-wing run advanced 3 cycles with the particle wake free but
-    unstretched, its particle wake then extended to `particles` (alpha scaled
-    by 1 + perturb).  Deterministic: two calls give bitwise-identical states."""
-    total = PREFILL + steps_after + 2
+    """A heaving-wing run advanced with the particle wake free but unstretched
+    (the reference HybridSolver's advect) for as many steps as it takes the
+    wake to reach `particles` by itself (no particles for the first N_keep +
+    1 = 11 steps, then 48 a step: 428 steps = 8.6 heave cycles for 20 000; at
+    most NATURAL_CYCLES cycles, at least 3), its particle wake then extended
+    to `particles`, if it is still short, by downstream copies of its
+    second-oldest heave period; alpha scaled by 1 + perturb.
+    Deterministic: two calls give bitwise-identical states."""
+    prefill = min(50 * NATURAL_CYCLES, max(150, 11 + -(-particles // 48)))
+    total = prefill + steps_after + 2
     case = hybrid.heaving_wing_case(num_steps=total, precision="fp64",
                                     max_particles=particles + 64 * total)
     run = hybrid.HybridRun(case)
     run.set_flags(True, False)
-    run.step(PREFILL)
+    run.step(prefill)
     T = case.meta["period"]
-    st = extend_wake(run.particles.state_dict(), particles, case.meta["V"] * T, T,
-                     50 * 2 * 6 * 4)
+    st = run.particles.state_dict()
+    if st["pos"].shape[0] < particles:
+        # copies of the second-oldest period (the oldest holds the start-up
+        # transient), placed downstream of the oldest
+        st = extend_wake(st, particles, case.meta["V"] * T, T, 50 * 2 * 6 * 4, period=1)
     if perturb:
         st["gamma"] = st["gamma"] * (1.0 + perturb)
     run.particles.load_state_dict(st)
