@@ -455,8 +455,9 @@ int fv_vpm_closure_train(fv_vpm *f, int32_t rounds);
  * and its heave / twist deform the lattice of step s + 1 (else step s + 1 is
  * undeformed).  Particles are a prescribed wake convected with V dt.
  * n_keep >= num_steps keeps the whole ring wake (no particles, the reference
- * benchmark's wake); instances keeping more than 15 rows hold their wake in
- * device memory and add one wake-induction launch per step.
+ * benchmark's wake).  Every instance holds its ring wake in device memory;
+ * a step is two launches (the device-wide pair work of all instances, then
+ * the per-instance remainder).
  * Limits: nc <= 16, ns <= 32, nc * ns <= 64, beam nodes <= 17. */
 typedef struct fv_sweep_desc {
     int32_t nc, ns;              /* one lattice, (nc+1) x (ns+1) corner grid */

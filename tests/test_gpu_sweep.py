@@ -114,8 +114,8 @@ def test_sweep_hbm_ring_wake_matches_oracle(oracle):
     insts = [ae.goland_instance(100.0, num_chords=10, n_keep=100000),
              ae.goland_instance(160.0, num_chords=12, n_keep=100000, kernel="winckelmans"),
              ae.goland_instance(130.0, num_chords=10, n_keep=20),
-             ae.goland_instance(90.0, num_chords=6)]          # LDS wake beside them
-    print("hbm sweep:", [(it.speed, it.num_steps, it.n_keep) for it in insts], flush=True)
+             ae.goland_instance(90.0, num_chords=6)]          # a short wake beside them
+    print("long-wake sweep:", [(it.speed, it.num_steps, it.n_keep) for it in insts], flush=True)
     sw = ae.Sweep(insts, use_graph=True)
     sw.step()
     for i, it in enumerate(insts):
