@@ -48,6 +48,52 @@ constexpr double kShell = 2.0;      // residual shell: kappa h < R <= kShell kap
 constexpr int kMaxSlices = 16;      // near (dx ranges) and far (source-cell ranges) slices
 constexpr int kFarTile = 64;        // source cells staged per far-field tile
 constexpr int kLocStride = 32;      // local expansion per cell: u0 (3), J0 (9), H (18), pad
+constexpr int32_t kKeyEnd = 1 << 18;   // sort key of the slots past the particle count: above
+                                       // every cell index (a grid has at most 262 144 cells)
+constexpr int kKeyBits = 19;           // radix-sort bits covering kKeyEnd
+
+// Binning parameters, computed on the device from the particles' bounding box
+// and the counts that size the pair lists; the host reads them once per
+// binning, after the last of those counts.
+struct ClGrid {
+    double ox, oy, oz, h;              // grid origin (bounding-box minimum), cell edge
+    int64_t n;                         // particles binned (the device count)
+    int64_t ncell, nocc, maxpop;       // cells, occupied cells, largest population
+    int64_t nl, nb;                    // shell-pair and far-band list lengths
+    int32_t nx, ny, nz, bad;           // bad: a non-finite position
+};
+
+// Cell count bound of a grid: cells <= min(8 max_cells, 262 144).
+__host__ __device__ inline bool grid_fits(double cells, double max_cells)
+{
+    return cells <= max_cells * 8.0 && cells <= 262144.0;
+}
+
+// Cell edge h (from h0, grown by 1.25 until the grid fits) and grid size for
+// the bounding box bb = (min xyz, max xyz), finite.
+__host__ __device__ inline void grid_fit(const double *bb, double h, double max_cells, double *h_out,
+                                         int *nx, int *ny, int *nz)
+{
+    for (;;) {
+        const double gx = floor((bb[3] - bb[0]) / h) + 1, gy = floor((bb[4] - bb[1]) / h) + 1,
+                     gz = floor((bb[5] - bb[2]) / h) + 1;
+        if (grid_fits(gx * gy * gz, max_cells)) {
+            *nx = (int)gx;
+            *ny = (int)gy;
+            *nz = (int)gz;
+            *h_out = h;
+            return;
+        }
+        h *= 1.25;
+    }
+}
+
+// Squared cutoff (kappa h)^2 (1 + 1e-12) of the near field, the shell and the
+// local expansions (the host and the device take the same expression).
+__host__ __device__ inline double cut2(double kappa, double h)
+{
+    return (kappa * h) * (kappa * h) * (1.0 + 1e-12);
+}
 
 struct fv_closure_state {
     fv_mlp *mlp = nullptr;
@@ -109,8 +155,9 @@ struct fv_closure_state {
     int32_t *dstencil = nullptr;       // columns, then slice offsets
     void *tmp = nullptr;
     size_t tmp_bytes = 0;
-    double *bbox = nullptr;            // 6 + 2 (occupancy) + per-block partials + near-pair counts
+    double *bbox = nullptr;            // per-block bounding-box partials, near-pair counts, grid
     unsigned long long *near_cnt = nullptr;   // [0] near pairs of the binning, [1] of every pass
+    ClGrid *grid = nullptr;            // device grid record (in the bbox allocation)
     float *tX = nullptr, *tT = nullptr, *tw = nullptr;
     double *vdir = nullptr, *vclo = nullptr, *verr = nullptr;
 };
@@ -153,42 +200,82 @@ __global__ void k_bbox_partial(const int64_t *__restrict__ np_dev, const double 
     if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = s[threadIdx.x][0];
 }
 
-__global__ void k_bbox_final(const double *__restrict__ part, int nb, double *__restrict__ out)
+// The bounding box (the blocks' partials in order) and the grid of a binning
+// (one lane).  With no particles, or a non-finite position (bad = 1), a
+// one-cell grid that bins nothing.
+__global__ void k_grid(const double *__restrict__ part, int nbb, const int64_t *__restrict__ np_dev,
+                       double h0, double max_cells, ClGrid *__restrict__ g)
 {
-    if (threadIdx.x >= 6) return;
-    double v = threadIdx.x < 3 ? INFINITY : -INFINITY;
-    for (int b = 0; b < nb; ++b)
-        v = threadIdx.x < 3 ? fmin(v, part[b * 6 + threadIdx.x]) : fmax(v, part[b * 6 + threadIdx.x]);
-    out[threadIdx.x] = v;
+    if (threadIdx.x != 0) return;
+    double bb[6];
+    for (int c = 0; c < 6; ++c) {
+        double v = c < 3 ? INFINITY : -INFINITY;
+        for (int b = 0; b < nbb; ++b) v = c < 3 ? fmin(v, part[b * 6 + c]) : fmax(v, part[b * 6 + c]);
+        bb[c] = v;
+    }
+    const int64_t n = *np_dev;
+    bool fin = true;
+    for (int c = 0; c < 6; ++c) fin = fin && isfinite(bb[c]);
+    g->n = n;
+    g->nocc = g->maxpop = g->nl = g->nb = 0;
+    g->bad = (n > 0 && !fin) ? 1 : 0;
+    if (n == 0 || !fin) {
+        g->ox = g->oy = g->oz = 0.0;
+        g->h = h0;
+        g->nx = g->ny = g->nz = 1;
+        g->ncell = 1;
+        return;
+    }
+    double h;
+    int nx, ny, nz;
+    grid_fit(bb, h0, max_cells, &h, &nx, &ny, &nz);
+    g->ox = bb[0];
+    g->oy = bb[1];
+    g->oz = bb[2];
+    g->h = h;
+    g->nx = nx;
+    g->ny = ny;
+    g->nz = nz;
+    g->ncell = (int64_t)nx * ny * nz;
 }
 
-__global__ void k_cell_keys(int64_t n, const double *__restrict__ x, const double *__restrict__ y,
-                            const double *__restrict__ z, double ox, double oy, double oz,
-                            double ih, int nx, int ny, int nz, int32_t *__restrict__ key,
-                            int32_t *__restrict__ idx)
+// Cell key of every particle slot i < nu (launch bound); slots past the
+// particle count take kKeyEnd and sort after every particle.
+__global__ void k_cell_keys(int64_t nu, const ClGrid *__restrict__ g, const double *__restrict__ x,
+                            const double *__restrict__ y, const double *__restrict__ z,
+                            int32_t *__restrict__ key, int32_t *__restrict__ idx)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int ix = (int)floor((x[i] - ox) * ih), iy = (int)floor((y[i] - oy) * ih),
-        iz = (int)floor((z[i] - oz) * ih);
+    if (i >= nu) return;
+    idx[i] = (int32_t)i;
+    if (i >= g->n) {
+        key[i] = kKeyEnd;
+        return;
+    }
+    const double ih = 1.0 / g->h;
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+    int ix = (int)floor((x[i] - g->ox) * ih), iy = (int)floor((y[i] - g->oy) * ih),
+        iz = (int)floor((z[i] - g->oz) * ih);
     ix = ix < 0 ? 0 : (ix >= nx ? nx - 1 : ix);
     iy = iy < 0 ? 0 : (iy >= ny ? ny - 1 : iy);
     iz = iz < 0 ? 0 : (iz >= nz ? nz - 1 : iz);
     key[i] = (ix * ny + iy) * nz + iz;
-    idx[i] = (int32_t)i;
 }
 
-__global__ void k_cell_ranges(int64_t n, const int32_t *__restrict__ key_s,
-                              int32_t *__restrict__ cstart, int32_t *__restrict__ cend)
+__global__ void k_cell_ranges(int64_t nu, const ClGrid *__restrict__ g,
+                              const int32_t *__restrict__ key_s, int32_t *__restrict__ cstart,
+                              int32_t *__restrict__ cend)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const int64_t n = g->n;
+    if (i >= nu || i >= n) return;
     const int32_t k = key_s[i];
     if (i == 0 || key_s[i - 1] != k) cstart[k] = (int32_t)i;
     if (i == n - 1 || key_s[i + 1] != k) cend[k] = (int32_t)(i + 1);
 }
 
-__global__ void k_gather(int64_t n, const int32_t *__restrict__ perm, const double *__restrict__ x,
+__global__ void k_gather(int64_t nu, const ClGrid *__restrict__ g, const int32_t *__restrict__ perm,
+                         const double *__restrict__ x,
                          const double *__restrict__ y, const double *__restrict__ z,
                          const double *__restrict__ ax, const double *__restrict__ ay,
                          const double *__restrict__ az, const double *__restrict__ sg,
@@ -197,20 +284,22 @@ __global__ void k_gather(int64_t n, const int32_t *__restrict__ perm, const doub
                          double *__restrict__ saz, double *__restrict__ ssg)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= nu || i >= g->n) return;
     const int32_t j = perm[i];
     sx[i] = x[j]; sy[i] = y[j]; sz[i] = z[j];
     sax[i] = ax[j]; say[i] = ay[j]; saz[i] = az[j];
     ssg[i] = sg[j];
 }
 
-// Ordered compaction of the occupied cells (one workgroup).
-__global__ __launch_bounds__(1024) void k_occupied(int64_t ncell, const int32_t *__restrict__ cstart,
+// Ordered compaction of the occupied cells (one workgroup); the count and the
+// largest population into the grid record.
+__global__ __launch_bounds__(1024) void k_occupied(ClGrid *__restrict__ g,
+                                                   const int32_t *__restrict__ cstart,
                                                    const int32_t *__restrict__ cend,
                                                    int32_t *__restrict__ occ,
-                                                   int32_t *__restrict__ occ_of,
-                                                   int64_t *__restrict__ nocc_out)
+                                                   int32_t *__restrict__ occ_of)
 {
+    const int64_t ncell = g->ncell;
     __shared__ int32_t cnt[1024];
     __shared__ int64_t base;
     __shared__ int32_t maxpop;
@@ -247,21 +336,23 @@ __global__ __launch_bounds__(1024) void k_occupied(int64_t ncell, const int32_t 
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        nocc_out[0] = base;
-        nocc_out[1] = maxpop;
+        g->nocc = base;
+        g->maxpop = maxpop;
     }
 }
 
-// Moments of each occupied cell (one wavefront per cell, fixed order).
+// Moments of each occupied cell (one wavefront per cell, fixed order; the
+// grid is launched for a bound on the occupied count).
 __global__ __launch_bounds__(64) void k_moments(
-    int64_t nocc, const int32_t *__restrict__ occ, const int32_t *__restrict__ cstart,
+    const ClGrid *__restrict__ g, const int32_t *__restrict__ occ, const int32_t *__restrict__ cstart,
     const int32_t *__restrict__ cend, const double *__restrict__ sx, const double *__restrict__ sy,
     const double *__restrict__ sz, const double *__restrict__ sax, const double *__restrict__ say,
-    const double *__restrict__ saz, double ox, double oy, double oz, double h, int ny, int nz,
-    double *__restrict__ mom)
+    const double *__restrict__ saz, double *__restrict__ mom)
 {
     const int64_t o = blockIdx.x;
-    if (o >= nocc) return;
+    if (o >= g->nocc) return;
+    const double ox = g->ox, oy = g->oy, oz = g->oz, h = g->h;
+    const int ny = g->ny, nz = g->nz;
     const int c = occ[o];
     const int b = cstart[c], e = cend[c];
     const int l = threadIdx.x;
@@ -595,13 +686,24 @@ __device__ __forceinline__ double centre_dist2(const double *mt, const double *m
     return dx * dx + dy * dy + dz * dz;
 }
 
-// Shell pairs per target cell (one wavefront per occupied target cell).
-__global__ __launch_bounds__(64) void k_shell_count(int64_t nocc, const double *__restrict__ mom,
-                                                    double rc2, double rm2,
+// Pairs per target cell of the shell kappa h < D <= kShell kappa h (band 0)
+// or of the band kappa h < D <= kappa_out h (band 1), D the distance of the
+// cell centres: one wavefront per target cell, the grid launched for a bound
+// nb - 1 on the occupied count; cnt[t] = 0 for nocc <= t < nb (the trailing
+// entries of the exclusive scan that follows).
+__global__ __launch_bounds__(64) void k_shell_count(const ClGrid *__restrict__ g,
+                                                    const double *__restrict__ mom, double kappa,
+                                                    double kappa_out, int band,
                                                     int32_t *__restrict__ cnt)
 {
     const int64_t t = blockIdx.x;
-    if (t >= nocc) return;
+    const int64_t nocc = g->nocc;
+    if (t >= nocc) {
+        if (threadIdx.x == 0) cnt[t] = 0;
+        return;
+    }
+    const double rc2 = cut2(kappa, g->h);
+    const double rm2 = band ? cut2(kappa_out, g->h) : rc2 * kShell * kShell;
     const double *mt = mom + t * kMomStride;
     int c = 0;
     for (int64_t s = threadIdx.x; s < nocc; s += 64) {
@@ -610,6 +712,15 @@ __global__ __launch_bounds__(64) void k_shell_count(int64_t nocc, const double *
     }
     for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
     if (threadIdx.x == 0) cnt[t] = c;
+}
+
+// The list lengths into the grid record (the totals of the two scans).
+__global__ void k_list_sizes(ClGrid *__restrict__ g, const int32_t *__restrict__ lend,
+                             const int32_t *__restrict__ bend)
+{
+    if (threadIdx.x != 0) return;
+    g->nl = *lend;
+    g->nb = bend ? *bend : 0;
 }
 
 // Ordered fill of the shell pair list (source order within each target cell).
@@ -1101,246 +1212,215 @@ static int update_stencil(fv_closure *cl, fv_ctx *ctx)
 
 static int closure_residual(fv_vpm *f, bool use_mlp);
 
+static int build_local(fv_closure *cl, fv_ctx *ctx);
+
+// Scan of n int32 counts into offsets (exclusive), temporary storage grown as
+// needed.
+static int scan_counts(fv_closure *cl, fv_ctx *ctx, const int32_t *cnt, int32_t *off, int64_t n)
+{
+    size_t sb = 0;
+    FV_HIP_CHECK(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, sb, cnt, off, (int)n, ctx->stream));
+    int rc = grow(ctx, &cl->tmp, &cl->tmp_bytes, sb);
+    if (rc) return rc;
+    FV_HIP_CHECK(ctx, hipcub::DeviceScan::ExclusiveSum(cl->tmp, sb, cnt, off, (int)n, ctx->stream));
+    return FV_OK;
+}
+
 // Bins the particles (cells, moments, shell pair list) for the current state
 // and, when `with_res`, computes the learned residual.  With `stage` (the RK
 // stages of one advect step, whose particles move a small fraction of a cell)
 // the step's binning is kept: cell membership, the shell list and the learned
 // residual stay, positions, strengths and moments are refreshed.
-static int build_local(fv_closure *cl, fv_ctx *ctx);
-
-// Band list of every target cell: source cells with kappa h < distance <=
-// kappa_local h (geometry only, once per binning).
-static int build_band(fv_closure *cl, fv_ctx *ctx, double rc2)
-{
-    cl->nband = 0;
-    if (!(cl->kappa_local > 0.0) || cl->nocc == 0) return FV_OK;
-    const int64_t nocc = cl->nocc;
-    const double rl = cl->kappa_local * cl->hh;
-    const double rl2 = rl * rl * (1.0 + 1e-12);
-    int rc = grow(ctx, &cl->band, &cl->band_bytes, (size_t)(2 * nocc + 4) * sizeof(int32_t) + 256);
-    if (rc) return rc;
-    cl->bcnt = static_cast<int32_t *>(cl->band);
-    cl->boff = cl->bcnt + (nocc + 1);
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, ctx->stream, cl->bcnt + nocc, (int64_t)1, 0);
-    FV_LAUNCH_CHECK(ctx);
-    hipLaunchKernelGGL(k_shell_count, dim3((unsigned)nocc), dim3(64), 0, ctx->stream, nocc, cl->mom,
-                       rc2, rl2, cl->bcnt);
-    FV_LAUNCH_CHECK(ctx);
-    size_t sb = 0;
-    FV_HIP_CHECK(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, sb, cl->bcnt, cl->boff,
-                                                      (int)(nocc + 1), ctx->stream));
-    rc = grow(ctx, &cl->tmp, &cl->tmp_bytes, sb);
-    if (rc) return rc;
-    FV_HIP_CHECK(ctx, hipcub::DeviceScan::ExclusiveSum(cl->tmp, sb, cl->bcnt, cl->boff,
-                                                      (int)(nocc + 1), ctx->stream));
-    int32_t nb = 0;
-    FV_HIP_CHECK(ctx, hipMemcpyAsync(&nb, cl->boff + nocc, sizeof nb, hipMemcpyDeviceToHost,
-                                     ctx->stream));
-    FV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    cl->nband = nb;
-    // the list follows the counts in the same buffer (which may move)
-    const size_t need = (size_t)(2 * nocc + 4) * sizeof(int32_t) + (size_t)2 * (nb + 1) * sizeof(int32_t) + 256;
-    if (need > cl->band_bytes) {
-        void *nbuf = nullptr;
-        if (hipMalloc(&nbuf, need) != hipSuccess)
-            return set_error(ctx, FV_E_HIP, "hipMalloc(%zu) for the closure band list failed", need);
-        FV_HIP_CHECK(ctx, hipMemcpyAsync(nbuf, cl->band, (size_t)(2 * nocc + 2) * sizeof(int32_t),
-                                         hipMemcpyDeviceToDevice, ctx->stream));
-        FV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-        (void)hipFree(cl->band);
-        cl->band = nbuf;
-        cl->band_bytes = need;
-        cl->bcnt = static_cast<int32_t *>(cl->band);
-        cl->boff = cl->bcnt + (nocc + 1);
-    }
-    cl->bt = cl->boff + (nocc + 1);
-    cl->bs = cl->bt + (nb + 1);
-    if (nb > 0) {
-        hipLaunchKernelGGL(k_shell_fill, dim3((unsigned)nocc), dim3(64), 0, ctx->stream, nocc,
-                           cl->mom, rc2, rl2, cl->boff, cl->bt, cl->bs);
-        FV_LAUNCH_CHECK(ctx);
-    }
-    return FV_OK;
-}
-
+//
+// A full binning is sized on the device: the bounding box, the grid, the cell
+// keys, the stable sort, the cell ranges, the occupied cells, their moments
+// and the shell / band counts run with launches and buffers sized by bounds
+// (the particle launch count, the grid's cell limit), every kernel taking the
+// exact sizes from the device grid record; the host reads that record once,
+// after the last count, and sizes the pair lists and the later passes by it.
 static int closure_build(fv_vpm *f, bool use_mlp, bool with_res = true, bool stage = false)
 {
     fv_closure *cl = f->closure;
     fv_ctx *ctx = f->ctx;
     if (cl->built_epoch == f->state_epoch) return with_res ? closure_residual(f, use_mlp) : FV_OK;
-    int rc = f->np_exact ? FV_OK : vpm_sync_count(f);
-    if (rc) return rc;
-    const int64_t n = f->np_host;
-    if (stage && cl->binned_step == cl->steps && cl->n == n && n > 0 && cl->nocc > 0) {
-        const unsigned gb = (unsigned)ceil_div(n, 256);
-        hipLaunchKernelGGL(k_gather, dim3(gb), dim3(256), 0, ctx->stream, n, cl->perm, f->x, f->y,
-                           f->z, f->gx, f->gy, f->gz, f->sig, cl->sx, cl->sy, cl->sz, cl->sax,
-                           cl->say, cl->saz, cl->ssg);
-        FV_LAUNCH_CHECK(ctx);
-        hipLaunchKernelGGL(k_moments, dim3((unsigned)cl->nocc), dim3(64), 0, ctx->stream, cl->nocc,
-                           cl->occ, cl->cstart, cl->cend, cl->sx, cl->sy, cl->sz, cl->sax, cl->say,
-                           cl->saz, cl->ox, cl->oy, cl->oz, cl->hh, cl->ny, cl->nz, cl->mom);
-        FV_LAUNCH_CHECK(ctx);
-        rc = build_local(cl, ctx);
+    int rc = FV_OK;
+    if (stage && cl->binned_step == cl->steps && cl->nocc > 0) {
+        rc = f->np_exact ? FV_OK : vpm_sync_count(f);
         if (rc) return rc;
-        cl->built_epoch = f->state_epoch;
-        return with_res ? closure_residual(f, use_mlp) : FV_OK;
+        const int64_t n = f->np_host;
+        if (cl->n == n && n > 0) {
+            const unsigned gb = (unsigned)ceil_div(n, 256);
+            hipLaunchKernelGGL(k_gather, dim3(gb), dim3(256), 0, ctx->stream, n, cl->grid, cl->perm,
+                               f->x, f->y, f->z, f->gx, f->gy, f->gz, f->sig, cl->sx, cl->sy,
+                               cl->sz, cl->sax, cl->say, cl->saz, cl->ssg);
+            FV_LAUNCH_CHECK(ctx);
+            hipLaunchKernelGGL(k_moments, dim3((unsigned)cl->nocc), dim3(64), 0, ctx->stream,
+                               cl->grid, cl->occ, cl->cstart, cl->cend, cl->sx, cl->sy, cl->sz,
+                               cl->sax, cl->say, cl->saz, cl->mom);
+            FV_LAUNCH_CHECK(ctx);
+            rc = build_local(cl, ctx);
+            if (rc) return rc;
+            cl->built_epoch = f->state_epoch;
+            return with_res ? closure_residual(f, use_mlp) : FV_OK;
+        }
     }
-    cl->n = n;
+    const int64_t nu = launch_count(f);   // >= the device particle count
     cl->bin_id++;
     cl->binned_step = cl->steps;
-    if (n == 0) {
+    if (nu == 0) {
+        cl->n = 0;
         cl->nocc = 0;
         cl->nlist = 0;
+        cl->nband = 0;
         cl->built_epoch = f->state_epoch;
         return FV_OK;
     }
-    // bounding box
+    // bounding box and grid
     const int nbb = 64;
     rc = grow(ctx, &cl->tmp, &cl->tmp_bytes, 4096);
     if (rc) return rc;
     if (!cl->bbox) {
-        FV_HIP_CHECK(ctx, hipMalloc(&cl->bbox, sizeof(double) * (6 * nbb + 10)));
-        cl->near_cnt = reinterpret_cast<unsigned long long *>(cl->bbox + 8 + 6 * nbb);
+        FV_HIP_CHECK(ctx, hipMalloc(&cl->bbox, sizeof(double) * (6 * nbb + 8) + sizeof(ClGrid)));
+        cl->near_cnt = reinterpret_cast<unsigned long long *>(cl->bbox + 6 * nbb);
+        cl->grid = reinterpret_cast<ClGrid *>(cl->bbox + 6 * nbb + 8);
         FV_HIP_CHECK(ctx, hipMemsetAsync(cl->near_cnt, 0, 2 * sizeof(unsigned long long),
                                          ctx->stream));
     }
     hipLaunchKernelGGL(k_bbox_partial, dim3(nbb), dim3(256), 0, ctx->stream, f->np_dev, f->x,
-                       f->y, f->z, cl->bbox + 8);
+                       f->y, f->z, cl->bbox);
     FV_LAUNCH_CHECK(ctx);
-    hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, ctx->stream, cl->bbox + 8, nbb, cl->bbox);
+    hipLaunchKernelGGL(k_grid, dim3(1), dim3(64), 0, ctx->stream, cl->bbox, nbb, f->np_dev, cl->h,
+                       (double)cl->max_cells, cl->grid);
     FV_LAUNCH_CHECK(ctx);
-    double bb[6];
-    FV_HIP_CHECK(ctx, hipMemcpyAsync(bb, cl->bbox, sizeof bb, hipMemcpyDeviceToHost, ctx->stream));
-    FV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    for (int c = 0; c < 6; ++c)
-        if (!std::isfinite(bb[c]))
-            return set_error(ctx, FV_E_ARG, "closure: non-finite particle positions");
-    // grid: cell edge h, grown until the grid and the occupied set stay bounded
-    double h = cl->h;
-    for (;;) {
-        const double gx = floor((bb[3] - bb[0]) / h) + 1, gy = floor((bb[4] - bb[1]) / h) + 1,
-                     gz = floor((bb[5] - bb[2]) / h) + 1;
-        const double cells = gx * gy * gz;
-        if (cells <= (double)cl->max_cells * 8.0 && cells <= 262144.0) {
-            cl->nx = (int)gx;
-            cl->ny = (int)gy;
-            cl->nz = (int)gz;
-            break;
-        }
-        h *= 1.25;
-    }
-    cl->hh = h;
-    cl->ox = bb[0];
-    cl->oy = bb[1];
-    cl->oz = bb[2];
-    cl->ncell = (int64_t)cl->nx * cl->ny * cl->nz;
+    // bounds: cells of a grid, occupied cells
+    const int64_t ncb = 8 * cl->max_cells < 262144 ? 8 * cl->max_cells : 262144;
+    const int64_t nob = nu < ncb ? nu : ncb;
     // particle-side buffers
-    const size_t need = (size_t)n * (4 * sizeof(int32_t) + (7 + 6 * kMaxSlices) * sizeof(double)) +
+    const size_t need = (size_t)nu * (4 * sizeof(int32_t) + (7 + 6 * kMaxSlices) * sizeof(double)) +
                         1024;
     rc = grow(ctx, &cl->block, &cl->block_bytes, need);
     if (rc) return rc;
     char *p = static_cast<char *>(cl->block);
     auto takeD = [&](size_t k) { double *r = reinterpret_cast<double *>(p); p += k * sizeof(double); return r; };
-    cl->sx = takeD(n); cl->sy = takeD(n); cl->sz = takeD(n);
-    cl->sax = takeD(n); cl->say = takeD(n); cl->saz = takeD(n); cl->ssg = takeD(n);
-    cl->npart = takeD(3 * kMaxSlices * n);
-    cl->fpart = takeD(3 * kMaxSlices * n);
-    cl->key = reinterpret_cast<int32_t *>(p); p += n * sizeof(int32_t);
-    cl->key_s = reinterpret_cast<int32_t *>(p); p += n * sizeof(int32_t);
-    cl->idx = reinterpret_cast<int32_t *>(p); p += n * sizeof(int32_t);
-    cl->perm = reinterpret_cast<int32_t *>(p); p += n * sizeof(int32_t);
-    const unsigned gb = (unsigned)ceil_div(n, 256);
-    hipLaunchKernelGGL(k_cell_keys, dim3(gb), dim3(256), 0, ctx->stream, n, f->x, f->y, f->z,
-                       cl->ox, cl->oy, cl->oz, 1.0 / h, cl->nx, cl->ny, cl->nz, cl->key, cl->idx);
+    cl->sx = takeD(nu); cl->sy = takeD(nu); cl->sz = takeD(nu);
+    cl->sax = takeD(nu); cl->say = takeD(nu); cl->saz = takeD(nu); cl->ssg = takeD(nu);
+    cl->npart = takeD(3 * kMaxSlices * nu);
+    cl->fpart = takeD(3 * kMaxSlices * nu);
+    cl->key = reinterpret_cast<int32_t *>(p); p += nu * sizeof(int32_t);
+    cl->key_s = reinterpret_cast<int32_t *>(p); p += nu * sizeof(int32_t);
+    cl->idx = reinterpret_cast<int32_t *>(p); p += nu * sizeof(int32_t);
+    cl->perm = reinterpret_cast<int32_t *>(p); p += nu * sizeof(int32_t);
+    const unsigned gb = (unsigned)ceil_div(nu, 256);
+    hipLaunchKernelGGL(k_cell_keys, dim3(gb), dim3(256), 0, ctx->stream, nu, cl->grid, f->x, f->y,
+                       f->z, cl->key, cl->idx);
     FV_LAUNCH_CHECK(ctx);
-    // stable radix sort of (cell, index)
+    // stable radix sort of (cell, index); the slots past the count sort last
     size_t tb = 0;
     FV_HIP_CHECK(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, tb, cl->key, cl->key_s, cl->idx,
-                                                        cl->perm, (int)n, 0, 32, ctx->stream));
+                                                        cl->perm, (int)nu, 0, kKeyBits,
+                                                        ctx->stream));
     rc = grow(ctx, &cl->tmp, &cl->tmp_bytes, tb);
     if (rc) return rc;
     FV_HIP_CHECK(ctx, hipcub::DeviceRadixSort::SortPairs(cl->tmp, tb, cl->key, cl->key_s, cl->idx,
-                                                        cl->perm, (int)n, 0, 32, ctx->stream));
-    // per-cell arrays
-    const size_t cneed = ((size_t)cl->ncell * 6 + 2) * sizeof(int32_t) + 1024;
+                                                        cl->perm, (int)nu, 0, kKeyBits,
+                                                        ctx->stream));
+    // per-cell arrays (for the grid's cell limit)
+    const size_t cneed = ((size_t)ncb * 6 + 2) * sizeof(int32_t) + 1024;
     rc = grow(ctx, &cl->cells, &cl->cells_bytes, cneed);
     if (rc) return rc;
     cl->cstart = static_cast<int32_t *>(cl->cells);
-    cl->cend = cl->cstart + cl->ncell;
-    cl->occ = cl->cend + cl->ncell;
-    cl->occ_of = cl->occ + cl->ncell;
-    cl->ccount = cl->occ_of + cl->ncell;
-    cl->pstart = cl->ccount + cl->ncell + 1;
-    hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)ceil_div(2 * cl->ncell, 256)), dim3(256), 0,
-                       ctx->stream, cl->cstart, 2 * cl->ncell, 0);
+    cl->cend = cl->cstart + ncb;
+    cl->occ = cl->cend + ncb;
+    cl->occ_of = cl->occ + ncb;
+    cl->ccount = cl->occ_of + ncb;
+    cl->pstart = cl->ccount + ncb + 1;
+    hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)ceil_div(2 * ncb, 256)), dim3(256), 0,
+                       ctx->stream, cl->cstart, 2 * ncb, 0);
     FV_LAUNCH_CHECK(ctx);
-    hipLaunchKernelGGL(k_cell_ranges, dim3(gb), dim3(256), 0, ctx->stream, n, cl->key_s, cl->cstart,
-                       cl->cend);
+    hipLaunchKernelGGL(k_cell_ranges, dim3(gb), dim3(256), 0, ctx->stream, nu, cl->grid, cl->key_s,
+                       cl->cstart, cl->cend);
     FV_LAUNCH_CHECK(ctx);
-    hipLaunchKernelGGL(k_cell_counts, dim3((unsigned)ceil_div(cl->ncell + 1, 256)), dim3(256), 0,
-                       ctx->stream, cl->ncell, cl->cstart, cl->cend, cl->ccount);
+    hipLaunchKernelGGL(k_cell_counts, dim3((unsigned)ceil_div(ncb + 1, 256)), dim3(256), 0,
+                       ctx->stream, ncb, cl->cstart, cl->cend, cl->ccount);
     FV_LAUNCH_CHECK(ctx);
-    {
-        size_t pb = 0;
-        FV_HIP_CHECK(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, pb, cl->ccount, cl->pstart,
-                                                          (int)(cl->ncell + 1), ctx->stream));
-        rc = grow(ctx, &cl->tmp, &cl->tmp_bytes, pb);
-        if (rc) return rc;
-        FV_HIP_CHECK(ctx, hipcub::DeviceScan::ExclusiveSum(cl->tmp, pb, cl->ccount, cl->pstart,
-                                                          (int)(cl->ncell + 1), ctx->stream));
-    }
-    hipLaunchKernelGGL(k_gather, dim3(gb), dim3(256), 0, ctx->stream, n, cl->perm, f->x, f->y, f->z,
-                       f->gx, f->gy, f->gz, f->sig, cl->sx, cl->sy, cl->sz, cl->sax, cl->say,
-                       cl->saz, cl->ssg);
+    rc = scan_counts(cl, ctx, cl->ccount, cl->pstart, ncb + 1);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_gather, dim3(gb), dim3(256), 0, ctx->stream, nu, cl->grid, cl->perm, f->x,
+                       f->y, f->z, f->gx, f->gy, f->gz, f->sig, cl->sx, cl->sy, cl->sz, cl->sax,
+                       cl->say, cl->saz, cl->ssg);
     FV_LAUNCH_CHECK(ctx);
-    int64_t *nocc_d = reinterpret_cast<int64_t *>(cl->bbox + 6);
-    hipLaunchKernelGGL(k_occupied, dim3(1), dim3(1024), 0, ctx->stream, cl->ncell, cl->cstart,
-                       cl->cend, cl->occ, cl->occ_of, nocc_d);
+    hipLaunchKernelGGL(k_occupied, dim3(1), dim3(1024), 0, ctx->stream, cl->grid, cl->cstart,
+                       cl->cend, cl->occ, cl->occ_of);
     FV_LAUNCH_CHECK(ctx);
-    int64_t occinfo[2] = {0, 0};
-    FV_HIP_CHECK(ctx, hipMemcpyAsync(occinfo, nocc_d, sizeof occinfo, hipMemcpyDeviceToHost,
-                                     ctx->stream));
-    FV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    const int64_t nocc = occinfo[0];
-    cl->nocc = nocc;
-    cl->maxpop = occinfo[1];
-    // moments, residual per cell, shell counts / offsets
-    const size_t fneed = (size_t)nocc * (kMomStride + 3 + kLocStride) * sizeof(double) +
-                         (size_t)(2 * nocc + 2) * sizeof(int32_t) + 1024;
+    // moments, residual and local expansions per cell, shell and band counts /
+    // offsets (for the occupied-cell bound)
+    const size_t fneed = (size_t)nob * (kMomStride + 3 + kLocStride) * sizeof(double) +
+                         (size_t)4 * (nob + 1) * sizeof(int32_t) + 1024;
     rc = grow(ctx, &cl->far, &cl->far_bytes, fneed);
     if (rc) return rc;
     cl->mom = static_cast<double *>(cl->far);
-    cl->res = cl->mom + nocc * kMomStride;
-    cl->loc = cl->res + 3 * nocc;
-    cl->lcnt = reinterpret_cast<int32_t *>(cl->loc + kLocStride * nocc);
-    cl->loff = cl->lcnt + (nocc + 1);
-    hipLaunchKernelGGL(k_moments, dim3((unsigned)nocc), dim3(64), 0, ctx->stream, nocc, cl->occ,
+    cl->res = cl->mom + nob * kMomStride;
+    cl->loc = cl->res + 3 * nob;
+    cl->lcnt = reinterpret_cast<int32_t *>(cl->loc + kLocStride * nob);
+    cl->loff = cl->lcnt + (nob + 1);
+    cl->bcnt = cl->loff + (nob + 1);
+    cl->boff = cl->bcnt + (nob + 1);
+    hipLaunchKernelGGL(k_moments, dim3((unsigned)nob), dim3(64), 0, ctx->stream, cl->grid, cl->occ,
                        cl->cstart, cl->cend, cl->sx, cl->sy, cl->sz, cl->sax, cl->say, cl->saz,
-                       cl->ox, cl->oy, cl->oz, h, cl->ny, cl->nz, cl->mom);
+                       cl->mom);
     FV_LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_shell_count, dim3((unsigned)(nob + 1)), dim3(64), 0, ctx->stream,
+                       cl->grid, cl->mom, cl->kappa, 0.0, 0, cl->lcnt);
+    FV_LAUNCH_CHECK(ctx);
+    rc = scan_counts(cl, ctx, cl->lcnt, cl->loff, nob + 1);
+    if (rc) return rc;
+    const bool band = cl->kappa_local > 0.0;
+    if (band) {
+        hipLaunchKernelGGL(k_shell_count, dim3((unsigned)(nob + 1)), dim3(64), 0, ctx->stream,
+                           cl->grid, cl->mom, cl->kappa, cl->kappa_local, 1, cl->bcnt);
+        FV_LAUNCH_CHECK(ctx);
+        rc = scan_counts(cl, ctx, cl->bcnt, cl->boff, nob + 1);
+        if (rc) return rc;
+    }
+    hipLaunchKernelGGL(k_list_sizes, dim3(1), dim3(64), 0, ctx->stream, cl->grid, cl->loff + nob,
+                       band ? cl->boff + nob : (const int32_t *)nullptr);
+    FV_LAUNCH_CHECK(ctx);
+    // the binning's one host read
+    ClGrid G;
+    FV_HIP_CHECK(ctx, hipMemcpyAsync(&G, cl->grid, sizeof G, hipMemcpyDeviceToHost, ctx->stream));
+    FV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if (G.bad) return set_error(ctx, FV_E_ARG, "closure: non-finite particle positions");
+    f->np_host = G.n;
+    f->np_upper = G.n;
+    f->np_exact = true;
+    const int64_t n = G.n;
+    cl->n = n;
+    cl->hh = G.h;
+    cl->ox = G.ox;
+    cl->oy = G.oy;
+    cl->oz = G.oz;
+    cl->nx = G.nx;
+    cl->ny = G.ny;
+    cl->nz = G.nz;
+    cl->ncell = G.ncell;
+    cl->nocc = G.nocc;
+    cl->maxpop = G.maxpop;
+    cl->nlist = G.nl;
+    cl->nband = band ? G.nb : 0;
+    if (n == 0) {
+        cl->nocc = 0;
+        cl->nlist = 0;
+        cl->nband = 0;
+        cl->built_epoch = f->state_epoch;
+        return FV_OK;
+    }
+    const int64_t nocc = cl->nocc;
+    const double h = cl->hh;
     rc = build_local(cl, ctx);
     if (rc) return rc;
     // shell pair list (geometry only)
-    const double rc2 = (cl->kappa * h) * (cl->kappa * h) * (1.0 + 1e-12);
-    const double rm2 = rc2 * kShell * kShell;
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, ctx->stream, cl->lcnt + nocc, (int64_t)1, 0);
-    FV_LAUNCH_CHECK(ctx);
-    hipLaunchKernelGGL(k_shell_count, dim3((unsigned)nocc), dim3(64), 0, ctx->stream, nocc, cl->mom,
-                       rc2, rm2, cl->lcnt);
-    FV_LAUNCH_CHECK(ctx);
-    size_t sb = 0;
-    FV_HIP_CHECK(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, sb, cl->lcnt, cl->loff,
-                                                      (int)(nocc + 1), ctx->stream));
-    rc = grow(ctx, &cl->tmp, &cl->tmp_bytes, sb);
-    if (rc) return rc;
-    FV_HIP_CHECK(ctx, hipcub::DeviceScan::ExclusiveSum(cl->tmp, sb, cl->lcnt, cl->loff,
-                                                      (int)(nocc + 1), ctx->stream));
-    int32_t nl = 0;
-    FV_HIP_CHECK(ctx, hipMemcpyAsync(&nl, cl->loff + nocc, sizeof nl, hipMemcpyDeviceToHost,
-                                     ctx->stream));
-    FV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    cl->nlist = nl;
+    const double rc2 = cut2(cl->kappa, h);
+    const int64_t nl = cl->nlist;
     if (nl > 0) {
         const size_t lneed = (size_t)nl * (2 * sizeof(int32_t) + sizeof(double) +
                                            (kFeat + kOutW) * sizeof(float)) + 1024;
@@ -1352,11 +1432,20 @@ static int closure_build(fv_vpm *f, bool use_mlp, bool with_res = true, bool sta
         cl->lt = reinterpret_cast<int32_t *>(cl->Y + (size_t)nl * kOutW);
         cl->ls = cl->lt + nl;
         hipLaunchKernelGGL(k_shell_fill, dim3((unsigned)nocc), dim3(64), 0, ctx->stream, nocc,
-                           cl->mom, rc2, rm2, cl->loff, cl->lt, cl->ls);
+                           cl->mom, rc2, rc2 * kShell * kShell, cl->loff, cl->lt, cl->ls);
         FV_LAUNCH_CHECK(ctx);
     }
-    rc = build_band(cl, ctx, rc2);
-    if (rc) return rc;
+    // far band list: source cells with kappa h < distance <= kappa_local h
+    const int64_t nb = cl->nband;
+    if (nb > 0) {
+        rc = grow(ctx, &cl->band, &cl->band_bytes, (size_t)2 * (nb + 1) * sizeof(int32_t));
+        if (rc) return rc;
+        cl->bt = static_cast<int32_t *>(cl->band);
+        cl->bs = cl->bt + (nb + 1);
+        hipLaunchKernelGGL(k_shell_fill, dim3((unsigned)nocc), dim3(64), 0, ctx->stream, nocc,
+                           cl->mom, rc2, cut2(cl->kappa_local, h), cl->boff, cl->bt, cl->bs);
+        FV_LAUNCH_CHECK(ctx);
+    }
     rc = update_stencil(cl, ctx);
     if (rc) return rc;
     // near-field pairs of this binning (every pass of it evaluates them)

@@ -120,17 +120,19 @@ __global__ __launch_bounds__(kLuBlock) void k_lu_solve_f64(int n, double *__rest
     if (threadIdx.x == 0 && info) info[blockIdx.x] = inf;
 }
 
-// n <= 128: the same solve by 16 wavefronts with the matrix factored in
+// n <= 128: the same solve by NW wavefronts with the matrix factored in
 // registers (lu_regs); the matrix is staged through LDS (coalesced loads and
 // the bitwise comparison with the stored matrix on the way in), the factors
 // come back to LDS in LAPACK's row order, and the substitutions and the
 // stored factors are those of the kernel above.
-__global__ __launch_bounds__(kLuRegLanes) void k_lu_solve_reg(int n, double *__restrict__ A,
-                                                              double *__restrict__ B,
-                                                              int32_t *__restrict__ info, int lda,
-                                                              unsigned mat_bytes, LuReuse ru,
-                                                              int store_factors)
+template <int NW>
+__global__ __launch_bounds__(64 * NW) void k_lu_solve_reg(int n, double *__restrict__ A,
+                                                          double *__restrict__ B,
+                                                          int32_t *__restrict__ info, int lda,
+                                                          unsigned mat_bytes, LuReuse ru,
+                                                          int store_factors)
 {
+    constexpr int kLuRegLanes = 64 * NW;
     extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
     const int64_t nn = (int64_t)n * n;
     double *a = A + (int64_t)blockIdx.x * nn;
@@ -157,7 +159,7 @@ __global__ __launch_bounds__(kLuRegLanes) void k_lu_solve_reg(int n, double *__r
             mat_move<kLuRegLanes>(w, lda, key, n, n, (const double *)nullptr);
             if (threadIdx.x == 0) ru.state[blockIdx.x] = 0;
         }
-        inf = lu_regs<double>(w, lda, n, order, sh);
+        inf = lu_regs<NW, double>(w, lda, n, order, sh);
         if (key && inf == 0) {
             mat_move<kLuRegLanes>(w, lda, fac, n, n, (const double *)nullptr);
             for (int i = threadIdx.x; i < n; i += kLuRegLanes) ord[i] = order[i];
@@ -304,13 +306,16 @@ __global__ __launch_bounds__(kLuBlock) void k_lu_solve_mixed(
 // Mixed, n <= 128: FP32 factors from the register factorization (lu_regs),
 // the substitutions in FP64 on them, and the FP64 residual r = b - A x from
 // the matrix held in the same register layout (lane: rows l, l + 64;
-// wavefront: columns w + 16 mj), the 16 wavefronts' partial row sums added in
-// LDS in wavefront order.  Refinement as in the kernel above.
-__global__ __launch_bounds__(kLuRegLanes) void k_lu_solve_mixed_reg(
+// wavefront: columns w + NW mj), the NW wavefronts' partial row sums added
+// in LDS in wavefront order.  Refinement as in the kernel above.
+template <int NW>
+__global__ __launch_bounds__(64 * NW) void k_lu_solve_mixed_reg(
     int n, const double *__restrict__ A, double *__restrict__ B, int max_iter, double tol,
     int32_t *__restrict__ info, int32_t *__restrict__ iters, int lda, unsigned mat_bytes,
     LuReuse ru)
 {
+    constexpr int kLuRegLanes = 64 * NW;
+    constexpr int CW = kLuRegMax / NW;
     extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -322,8 +327,8 @@ __global__ __launch_bounds__(kLuRegLanes) void k_lu_solve_mixed_reg(
     double *r = x + kLuRegMax;
     double *d = r + kLuRegMax;
     double *rdiag = d + kLuRegMax;
-    double *part = rdiag + kLuRegMax;                        // [16][128] partial row sums
-    int *order = reinterpret_cast<int *>(part + 16 * kLuRegMax);
+    double *part = rdiag + kLuRegMax;                        // [NW][128] partial row sums
+    int *order = reinterpret_cast<int *>(part + NW * kLuRegMax);
     auto *sh = reinterpret_cast<LuRegShared<float> *>(order + kLuRegMax);
     __shared__ double stat[2];
     double *key = ru.key ? ru.key + (int64_t)blockIdx.x * nn : nullptr;
@@ -342,7 +347,7 @@ __global__ __launch_bounds__(kLuRegLanes) void k_lu_solve_mixed_reg(
             mat_move<kLuRegLanes>(a, n, key, n, n, (const double *)nullptr);
             if (threadIdx.x == 0) ru.state[blockIdx.x] = 0;
         }
-        const int inf = lu_regs<float>(lu, lda, n, order, sh);
+        const int inf = lu_regs<NW, float>(lu, lda, n, order, sh);
         if (inf != 0) {
             if (threadIdx.x == 0) {
                 if (info) info[blockIdx.x] = inf;
@@ -357,13 +362,13 @@ __global__ __launch_bounds__(kLuRegLanes) void k_lu_solve_mixed_reg(
         }
     }
     // the FP64 matrix in the register layout, for the residuals
-    double av[2][8];
+    double av[2][CW];
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
         const int row = lane + 64 * q;
 #pragma unroll
-        for (int mj = 0; mj < 8; ++mj) {
-            const int j = wv + 16 * mj;
+        for (int mj = 0; mj < CW; ++mj) {
+            const int j = wv + NW * mj;
             av[q][mj] = (row < n && j < n) ? a[row * n + j] : 0.0;
         }
     }
@@ -404,8 +409,8 @@ __global__ __launch_bounds__(kLuRegLanes) void k_lu_solve_mixed_reg(
         // FP64 residual r = b - A x
         double p0 = 0.0, p1 = 0.0;
 #pragma unroll
-        for (int mj = 0; mj < 8; ++mj) {
-            const int j = wv + 16 * mj;
+        for (int mj = 0; mj < CW; ++mj) {
+            const int j = wv + NW * mj;
             const double xj = j < n ? x[j] : 0.0;
             p0 = fma(av[0][mj], xj, p0);
             p1 = fma(av[1][mj], xj, p1);
@@ -416,7 +421,7 @@ __global__ __launch_bounds__(kLuRegLanes) void k_lu_solve_mixed_reg(
         for (int i = threadIdx.x; i < n; i += kLuRegLanes) {
             double s = 0.0;
 #pragma unroll
-            for (int w = 0; w < 16; ++w) s += part[w * kLuRegMax + i];
+            for (int w = 0; w < NW; ++w) s += part[w * kLuRegMax + i];
             r[i] = bg[i] - s;
         }
         __syncthreads();
@@ -443,6 +448,9 @@ static size_t max_lds(fv_ctx *ctx)
     return (size_t)v - kLuStaticLds;
 }
 
+// Wavefronts of the register solve (n <= 128).
+constexpr int kLuRegWaves = 16;
+
 // store_factors: A takes its LU factors (else A is left as it was when the
 // matrix is factored in LDS).
 int dense_solve(fv_ctx *ctx, int64_t batch, int64_t n, double *A, double *rhs, int32_t *info,
@@ -458,9 +466,9 @@ int dense_solve(fv_ctx *ctx, int64_t batch, int64_t n, double *A, double *rhs, i
         FV_REQUIRE(ctx, total <= max_lds(ctx), "dense solve: %zu bytes of LDS for n = %lld",
                    total, (long long)n);
         TimedLaunch tl(ctx, "dense_solve");
-        hipLaunchKernelGGL(k_lu_solve_reg, dim3((unsigned)batch), dim3(kLuRegLanes),
-                           (unsigned)total, ctx->stream, (int)n, A, rhs, info, lda, (unsigned)mat,
-                           ru, store_factors ? 1 : 0);
+        hipLaunchKernelGGL(k_lu_solve_reg<kLuRegWaves>, dim3((unsigned)batch),
+                           dim3(64 * kLuRegWaves), (unsigned)total, ctx->stream, (int)n, A, rhs,
+                           info, lda, (unsigned)mat, ru, store_factors ? 1 : 0);
         FV_LAUNCH_CHECK(ctx);
         tl.done();
         return FV_OK;
@@ -493,14 +501,14 @@ int dense_solve_mixed(fv_ctx *ctx, int64_t batch, int64_t n, const double *A, do
     if (n <= kLuRegMax) {
         const int lda = (int)(n | 1);
         const size_t mat = lu_align((size_t)n * lda * sizeof(float));
-        const size_t total = mat + (4 + 16) * kLuRegMax * sizeof(double) +
+        const size_t total = mat + (4 + kLuRegWaves) * kLuRegMax * sizeof(double) +
                              kLuRegMax * sizeof(int) + sizeof(LuRegShared<float>);
         FV_REQUIRE(ctx, total <= max_lds(ctx), "dense solve: %zu bytes of LDS for n = %lld",
                    total, (long long)n);
         TimedLaunch tl(ctx, "dense_solve");
-        hipLaunchKernelGGL(k_lu_solve_mixed_reg, dim3((unsigned)batch), dim3(kLuRegLanes),
-                           (unsigned)total, ctx->stream, (int)n, A, rhs, max_iter, tol, info,
-                           iters, lda, (unsigned)mat, ru);
+        hipLaunchKernelGGL(k_lu_solve_mixed_reg<kLuRegWaves>, dim3((unsigned)batch),
+                           dim3(64 * kLuRegWaves), (unsigned)total, ctx->stream, (int)n, A, rhs,
+                           max_iter, tol, info, iters, lda, (unsigned)mat, ru);
         FV_LAUNCH_CHECK(ctx);
         tl.done();
         return FV_OK;

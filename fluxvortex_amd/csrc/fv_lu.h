@@ -469,9 +469,10 @@ __device__ __forceinline__ int lu_blocked(T *a, int lda, int n, int *perm, TB *b
 }
 
 // ------------------------------------------------ register-resident whole LU
-// LU of an n x n matrix, n <= 128, by a workgroup of 16 wavefronts with the
-// whole matrix in registers: wavefront w holds columns w + 16 mj (mj = 0..7),
-// lane l rows l and l + 64.  Each column thus lives in one wavefront (its
+// LU of an n x n matrix, n <= 128, by a workgroup of NW wavefronts (4, 8 or
+// 16) with the whole matrix in registers: wavefront w holds columns w + NW mj
+// (mj = 0 .. 128 / NW - 1), lane l rows l and l + 64.  Each column thus lives
+// in one wavefront (its
 // pivot search is a wavefront reduction, no barrier) and each row in lane
 // l % 64 of every wavefront (the pivot row's entries reach each wavefront's
 // lanes with v_readlane).  As in the register panels above, rows are not
@@ -483,7 +484,6 @@ __device__ __forceinline__ int lu_blocked(T *a, int lda, int n, int *perm, TB *b
 // updates that column first, finds its pivot and publishes the pivot row's
 // lane / slot / position and every row's multiplier (double-buffered in LDS)
 // while the other wavefronts finish the update of step k.
-constexpr int kLuRegLanes = 1024;
 constexpr int kLuRegMax = 128;
 
 template <typename T>
@@ -554,8 +554,8 @@ __device__ __forceinline__ float lureg_recip(float p)
 // that maximum are found by a ballot, and only an exact tie between lanes
 // takes a second (position) reduction.  Stores the multipliers into the
 // column and publishes the step record into buffer buf.
-template <int MJ, typename T>
-__device__ __forceinline__ void lureg_pivot(T (&v)[2][8], const int (&pos)[2], int jj, int n,
+template <int MJ, int CW, typename T>
+__device__ __forceinline__ void lureg_pivot(T (&v)[2][CW], const int (&pos)[2], int jj, int n,
                                             LuRegShared<T> *sh, int buf)
 {
     const int lane = threadIdx.x & 63;
@@ -598,12 +598,12 @@ __device__ __forceinline__ void lureg_pivot(T (&v)[2][8], const int (&pos)[2], i
     }
 }
 
-// v[q][mj] -= l[q] * (pivot row entry of column wv + 16 mj), QP the pivot
+// v[q][mj] -= l[q] * (pivot row entry of column wv + NW mj), QP the pivot
 // row's register slot: for every mj > MJ0 (those columns lie beyond step k
 // of block MJ0; padding columns >= n take the update too, harmlessly, so the
 // loop has no branches) and, when first, for mj == MJ0.
-template <int MJ0, int QP, typename T>
-__device__ __forceinline__ void lureg_update(T (&v)[2][8], const T (&l)[2], int lp, bool first)
+template <int MJ0, int QP, int CW, typename T>
+__device__ __forceinline__ void lureg_update(T (&v)[2][CW], const T (&l)[2], int lp, bool first)
 {
     if (first) {
         const T u = readlane_t(v[QP][MJ0], lp);
@@ -611,7 +611,7 @@ __device__ __forceinline__ void lureg_update(T (&v)[2][8], const T (&l)[2], int 
         for (int q = 0; q < 2; ++q) v[q][MJ0] = fma(-l[q], u, v[q][MJ0]);
     }
 #pragma unroll
-    for (int mj = MJ0 + 1; mj < 8; ++mj) {
+    for (int mj = MJ0 + 1; mj < CW; ++mj) {
         const T u = readlane_t(v[QP][mj], lp);
 #pragma unroll
         for (int q = 0; q < 2; ++q) v[q][mj] = fma(-l[q], u, v[q][mj]);
@@ -628,11 +628,12 @@ struct LuRegPrev {
     int lp, qp;
 };
 
-// Step k of the elimination (k / 16 == MJ0) by every wavefront (wv uniform).
-template <int MJ0, typename T>
-__device__ __forceinline__ void lureg_step(T (&v)[2][8], int (&pos)[2], int k, int n, int wv,
-                                           LuRegShared<T> *sh, LuRegPrev<T> &prev)
+// Step k of the elimination (k / NW == MJ0) by every wavefront (wv uniform).
+template <int MJ0, int NW, typename T>
+__device__ __forceinline__ void lureg_step(T (&v)[2][kLuRegMax / NW], int (&pos)[2], int k,
+                                           int n, int wv, LuRegShared<T> *sh, LuRegPrev<T> &prev)
 {
+    constexpr int CW = kLuRegMax / NW;
     const int lane = threadIdx.x & 63;
     const int buf = k & 1;
     const int lp = __builtin_amdgcn_readfirstlane(sh->rec[buf][0]);
@@ -641,7 +642,7 @@ __device__ __forceinline__ void lureg_step(T (&v)[2][8], int (&pos)[2], int k, i
     const T l[2] = {sh->l[buf][0][lane], sh->l[buf][1][lane]};
 #pragma unroll
     for (int q = 0; q < 2; ++q) pos[q] = pos[q] == pl ? k : (pos[q] == k ? pl : pos[q]);
-    const int kl = k & 15;
+    const int kl = k & (NW - 1);
     // the owner of column k (the previous step's look-ahead) catches up on
     // the previous step's update of its later columns
     if (k > 0 && kl == wv) {
@@ -651,19 +652,19 @@ __device__ __forceinline__ void lureg_step(T (&v)[2][8], int (&pos)[2], int k, i
             lureg_update<MJ0, 0>(v, prev.l, prev.lp, false);
     }
     const int k1 = k + 1;
-    if (k1 < n && (k1 & 15) == wv) {
+    if (k1 < n && (k1 & (NW - 1)) == wv) {
         // look-ahead: column k+1 first, then its pivot; the rest of this
         // step's update follows in the next step.  This wavefront is the
-        // step's critical path: it issues ahead of the three others on its
-        // SIMD (wave priority) until the pivot is published.
+        // step's critical path: it issues ahead of the others on its SIMD
+        // (wave priority) until the pivot is published.
         __builtin_amdgcn_s_setprio(3);
-        if (kl != 15) {
+        if (kl != NW - 1) {
             const T u = readlane_t(qp ? v[1][MJ0] : v[0][MJ0], lp);
 #pragma unroll
             for (int q = 0; q < 2; ++q) v[q][MJ0] = fma(-l[q], u, v[q][MJ0]);
             lureg_pivot<MJ0>(v, pos, k1, n, sh, buf ^ 1);
-        } else if (MJ0 < 7) {
-            constexpr int M1 = MJ0 < 7 ? MJ0 + 1 : 7;
+        } else if (MJ0 < CW - 1) {
+            constexpr int M1 = MJ0 < CW - 1 ? MJ0 + 1 : CW - 1;
             const T u = readlane_t(qp ? v[1][M1] : v[0][M1], lp);
 #pragma unroll
             for (int q = 0; q < 2; ++q) v[q][M1] = fma(-l[q], u, v[q][M1]);
@@ -682,39 +683,41 @@ __device__ __forceinline__ void lureg_step(T (&v)[2][8], int (&pos)[2], int k, i
     __syncthreads();
 }
 
-// Steps 16 MJ0 .. min(n, 16 MJ0 + 16) - 1, then the next block: one loop per
+// Steps NW MJ0 .. min(n, NW MJ0 + NW) - 1, then the next block: one loop per
 // block, so every register column index is a constant.
-template <int MJ0, typename T>
-__device__ __forceinline__ void lureg_block(T (&v)[2][8], int (&pos)[2], int n, int wv,
-                                            LuRegShared<T> *sh, LuRegPrev<T> &prev)
+template <int MJ0, int NW, typename T>
+__device__ __forceinline__ void lureg_block(T (&v)[2][kLuRegMax / NW], int (&pos)[2], int n,
+                                            int wv, LuRegShared<T> *sh, LuRegPrev<T> &prev)
 {
-    const int kend = n < 16 * MJ0 + 16 ? n : 16 * MJ0 + 16;
-    for (int k = 16 * MJ0; k < kend; ++k) lureg_step<MJ0>(v, pos, k, n, wv, sh, prev);
-    if constexpr (MJ0 < 7) {
-        if (n > 16 * MJ0 + 16) lureg_block<MJ0 + 1>(v, pos, n, wv, sh, prev);
+    const int kend = n < NW * MJ0 + NW ? n : NW * MJ0 + NW;
+    for (int k = NW * MJ0; k < kend; ++k) lureg_step<MJ0, NW>(v, pos, k, n, wv, sh, prev);
+    if constexpr (MJ0 < kLuRegMax / NW - 1) {
+        if (n > NW * MJ0 + NW) lureg_block<MJ0 + 1, NW>(v, pos, n, wv, sh, prev);
     }
 }
 
 // Factors the n x n matrix a (row stride lda, n <= 128; LDS or global) by a
-// workgroup of kLuRegLanes lanes, in place: on return a holds L and U in
+// workgroup of 64 NW lanes, in place: on return a holds L and U in
 // LAPACK's row order and order[i] the original row of factor row i
 // ((P v)[i] = v[order[i]]).  Returns the 1-based column of the first exactly
 // zero pivot (the factorization continues past it, as LAPACK's), or 0.
-template <typename T>
+template <int NW, typename T>
 __device__ __forceinline__ int lu_regs(T *a, int lda, int n, int *order, LuRegShared<T> *sh)
 {
+    static_assert(NW == 4 || NW == 8 || NW == 16, "4, 8 or 16 wavefronts");
+    constexpr int CW = kLuRegMax / NW;
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     constexpr int kNone = 1 << 30;
-    T v[2][8];
+    T v[2][CW];
     int pos[2];
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
         const int r = lane + 64 * q;
         pos[q] = r < n ? r : kNone;
 #pragma unroll
-        for (int mj = 0; mj < 8; ++mj) {
-            const int j = wv + 16 * mj;
+        for (int mj = 0; mj < CW; ++mj) {
+            const int j = wv + NW * mj;
             v[q][mj] = (r < n && j < n) ? a[r * lda + j] : (T)0;
         }
     }
@@ -722,14 +725,14 @@ __device__ __forceinline__ int lu_regs(T *a, int lda, int n, int *order, LuRegSh
     if (wv == 0) lureg_pivot<0>(v, pos, 0, n, sh, 0);
     __syncthreads();
     LuRegPrev<T> prev{{(T)0, (T)0}, 0, 0};
-    lureg_block<0>(v, pos, n, wv, sh, prev);
+    lureg_block<0, NW>(v, pos, n, wv, sh, prev);
     // every row to its final position (a was read before the first barrier)
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
         if (pos[q] < n) {
 #pragma unroll
-            for (int mj = 0; mj < 8; ++mj) {
-                const int j = wv + 16 * mj;
+            for (int mj = 0; mj < CW; ++mj) {
+                const int j = wv + NW * mj;
                 if (j < n) a[pos[q] * lda + j] = v[q][mj];
             }
             if (wv == 0) order[pos[q]] = lane + 64 * q;

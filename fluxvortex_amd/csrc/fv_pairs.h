@@ -527,7 +527,9 @@ struct PartOp {
             if (KERNEL == 0 && (MODE == 3 || MODE == 4)) {
                 double B;
                 gauss_curl_factors_f64(r2d, keep, s, CH, B);
-                if (MODE == 4 && !keep) clamped_pair(dxd, dyd, dzd, r2d, s, c);
+                if constexpr (MODE == 4) {
+                    if (!keep) clamped_pair(dxd, dyd, dzd, r2d, s, c);
+                }
                 const double a = CH * (dxd * s.ax + dyd * s.ay + dzd * s.az);
                 c.v[0] = fma(-B, s.ax, fma(a, dxd, c.v[0]));
                 c.v[1] = fma(-B, s.ay, fma(a, dyd, c.v[1]));
@@ -546,7 +548,9 @@ struct PartOp {
             const bool keep = r2d >= 1.0e-24;
             float CGf, CHf = 0.0f;
             part_factors_mixed<KERNEL, MODE != 0>((float)r2d, keep, dxd, dyd, dzd, s, CGf, CHf);
-            if (MODE == 4 && !keep) clamped_pair(dxd, dyd, dzd, r2d, s, c);
+            if constexpr (MODE == 4) {
+                if (!keep) clamped_pair(dxd, dyd, dzd, r2d, s, c);
+            }
             accumulate(dxd, dyd, dzd, r2d, (double)CGf, (double)CHf, s, t, c);
         }
     }
