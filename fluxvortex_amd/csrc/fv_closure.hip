@@ -133,8 +133,9 @@ struct fv_closure_state {
     double *fpart = nullptr;           // far partial sums, kMaxSlices x 3 per sorted particle
     void *cells = nullptr;
     size_t cells_bytes = 0;
-    int32_t *cstart = nullptr, *cend = nullptr, *occ = nullptr, *occ_of = nullptr;
-    int32_t *ccount = nullptr, *pstart = nullptr;   // ncell + 1 each: population, prefix
+    int32_t *occ = nullptr, *occ_of = nullptr;
+    int32_t *pstart = nullptr;         // cell starts in sorted order, ncell + 1
+    int32_t *cstart = nullptr, *cend = nullptr;     // pstart and pstart + 1: cell c's range
     void *far = nullptr;
     size_t far_bytes = 0;
     double *mom = nullptr;             // kMomStride per occupied cell
@@ -204,9 +205,11 @@ __global__ void k_bbox_partial(const int64_t *__restrict__ np_dev, const double 
 // (one lane).  With no particles, or a non-finite position (bad = 1), a
 // one-cell grid that bins nothing.
 __global__ void k_grid(const double *__restrict__ part, int nbb, const int64_t *__restrict__ np_dev,
-                       double h0, double max_cells, ClGrid *__restrict__ g)
+                       double h0, double max_cells, ClGrid *__restrict__ g,
+                       unsigned long long *__restrict__ near_pairs)
 {
     if (threadIdx.x != 0) return;
+    *near_pairs = 0;   // the near-field pairs of this binning, counted after the read
     double bb[6];
     for (int c = 0; c < 6; ++c) {
         double v = c < 3 ? INFINITY : -INFINITY;
@@ -262,16 +265,24 @@ __global__ void k_cell_keys(int64_t nu, const ClGrid *__restrict__ g, const doub
     key[i] = (ix * ny + iy) * nz + iz;
 }
 
-__global__ void k_cell_ranges(int64_t nu, const ClGrid *__restrict__ g,
-                              const int32_t *__restrict__ key_s, int32_t *__restrict__ cstart,
-                              int32_t *__restrict__ cend)
+// Start of every cell's particle range in the sorted keys, p[c] = the lower
+// bound of c among the first n keys, for c = 0 .. ncb (ncb >= the grid's
+// cell count, so p[ncb] = n): cell c holds the sorted particles
+// [p[c], p[c + 1]).
+__global__ void k_cell_bounds(int64_t ncb, const ClGrid *__restrict__ g,
+                              const int32_t *__restrict__ key_s, int32_t *__restrict__ p)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t n = g->n;
-    if (i >= nu || i >= n) return;
-    const int32_t k = key_s[i];
-    if (i == 0 || key_s[i - 1] != k) cstart[k] = (int32_t)i;
-    if (i == n - 1 || key_s[i + 1] != k) cend[k] = (int32_t)(i + 1);
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > ncb) return;
+    int64_t lo = 0, hi = g->n;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if ((int64_t)key_s[mid] < c)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    p[c] = (int32_t)lo;
 }
 
 __global__ void k_gather(int64_t nu, const ClGrid *__restrict__ g, const int32_t *__restrict__ perm,
@@ -291,50 +302,50 @@ __global__ void k_gather(int64_t nu, const ClGrid *__restrict__ g, const int32_t
     ssg[i] = sg[j];
 }
 
-// Ordered compaction of the occupied cells (one workgroup); the count and the
-// largest population into the grid record.
+// Ordered compaction of the occupied cells (one workgroup of 16 wavefronts,
+// 1024 cells a round: ballots, the wavefronts' counts added in order); the
+// count and the largest population into the grid record.
 __global__ __launch_bounds__(1024) void k_occupied(ClGrid *__restrict__ g,
                                                    const int32_t *__restrict__ cstart,
                                                    const int32_t *__restrict__ cend,
                                                    int32_t *__restrict__ occ,
                                                    int32_t *__restrict__ occ_of)
 {
-    const int64_t ncell = g->ncell;
-    __shared__ int32_t cnt[1024];
-    __shared__ int64_t base;
+    __shared__ int32_t wcnt[16];
     __shared__ int32_t maxpop;
-    if (threadIdx.x == 0) {
-        base = 0;
-        maxpop = 0;
-    }
-    __syncthreads();
+    const int64_t ncell = g->ncell;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    if (threadIdx.x == 0) maxpop = 0;
+    int64_t base = 0;
+    int32_t mp = 0;
     for (int64_t c0 = 0; c0 < ncell; c0 += 1024) {
         const int64_t c = c0 + threadIdx.x;
-        const int f = (c < ncell && cend[c] > cstart[c]) ? 1 : 0;
-        if (f) atomicMax(&maxpop, cend[c] - cstart[c]);
-        cnt[threadIdx.x] = f;
+        const int pop = c < ncell ? cend[c] - cstart[c] : 0;
+        mp = pop > mp ? pop : mp;
+        const unsigned long long m = __ballot(pop > 0);
+        if (lane == 0) wcnt[w] = __popcll(m);
         __syncthreads();
-        if (threadIdx.x == 0) {
-            int run = 0;
-            for (int k = 0; k < 1024; ++k) {
-                const int v = cnt[k];
-                cnt[k] = run;
-                run += v;
-            }
+        int64_t at = base;
+        int32_t tot = 0;
+        for (int k = 0; k < 16; ++k) {
+            at += k < w ? wcnt[k] : 0;
+            tot += wcnt[k];
         }
-        __syncthreads();
         if (c < ncell) {
-            if (f) {
-                occ[base + cnt[threadIdx.x]] = (int32_t)c;
-                occ_of[c] = (int32_t)(base + cnt[threadIdx.x]);
+            if (pop > 0) {
+                const int64_t o = at + __popcll(m & below);
+                occ[o] = (int32_t)c;
+                occ_of[c] = (int32_t)o;
             } else {
                 occ_of[c] = -1;
             }
         }
-        __syncthreads();
-        if (threadIdx.x == 1023) base += cnt[1023] + f;
+        base += tot;
         __syncthreads();
     }
+    atomicMax(&maxpop, mp);
+    __syncthreads();
     if (threadIdx.x == 0) {
         g->nocc = base;
         g->maxpop = maxpop;
@@ -686,70 +697,127 @@ __device__ __forceinline__ double centre_dist2(const double *mt, const double *m
     return dx * dx + dy * dy + dz * dz;
 }
 
-// Pairs per target cell of the shell kappa h < D <= kShell kappa h (band 0)
-// or of the band kappa h < D <= kappa_out h (band 1), D the distance of the
-// cell centres: one wavefront per target cell, the grid launched for a bound
-// nb - 1 on the occupied count; cnt[t] = 0 for nocc <= t < nb (the trailing
-// entries of the exclusive scan that follows).
+// Pairs per target cell of the shell kappa h < D <= kShell kappa h and, when
+// bcnt is non-null, of the band kappa h < D <= kappa_local h, D the distance
+// of the cell centres: one wavefront per target cell, the grid launched for a
+// bound nb - 1 on the occupied count; the counts are 0 for nocc <= t < nb (the
+// trailing entries of the exclusive scans that follow).
 __global__ __launch_bounds__(64) void k_shell_count(const ClGrid *__restrict__ g,
                                                     const double *__restrict__ mom, double kappa,
-                                                    double kappa_out, int band,
-                                                    int32_t *__restrict__ cnt)
+                                                    double kappa_local, int32_t *__restrict__ lcnt,
+                                                    int32_t *__restrict__ bcnt)
 {
     const int64_t t = blockIdx.x;
     const int64_t nocc = g->nocc;
     if (t >= nocc) {
-        if (threadIdx.x == 0) cnt[t] = 0;
+        if (threadIdx.x == 0) {
+            lcnt[t] = 0;
+            if (bcnt) bcnt[t] = 0;
+        }
         return;
     }
     const double rc2 = cut2(kappa, g->h);
-    const double rm2 = band ? cut2(kappa_out, g->h) : rc2 * kShell * kShell;
+    const double rm2 = rc2 * kShell * kShell;
+    const double rl2 = bcnt ? cut2(kappa_local, g->h) : 0.0;
     const double *mt = mom + t * kMomStride;
-    int c = 0;
+    int c = 0, cb = 0;
     for (int64_t s = threadIdx.x; s < nocc; s += 64) {
         const double d2 = centre_dist2(mt, mom + s * kMomStride);
         c += (d2 > rc2 && d2 <= rm2) ? 1 : 0;
+        cb += (d2 > rc2 && d2 <= rl2) ? 1 : 0;
     }
-    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
-    if (threadIdx.x == 0) cnt[t] = c;
+    for (int off = 32; off > 0; off >>= 1) {
+        c += __shfl_xor(c, off, 64);
+        cb += __shfl_xor(cb, off, 64);
+    }
+    if (threadIdx.x == 0) {
+        lcnt[t] = c;
+        if (bcnt) bcnt[t] = cb;
+    }
 }
 
-// The list lengths into the grid record (the totals of the two scans).
-__global__ void k_list_sizes(ClGrid *__restrict__ g, const int32_t *__restrict__ lend,
-                             const int32_t *__restrict__ bend)
+// Exclusive scans of the shell counts and (bc non-null) the band counts, nb
+// entries each, by one workgroup of 16 wavefronts (1024 entries a round); the
+// totals, the list lengths, into the grid record.
+__global__ __launch_bounds__(1024) void k_scan_lists(ClGrid *__restrict__ g, int64_t nb,
+                                                     const int32_t *__restrict__ lc,
+                                                     int32_t *__restrict__ lo,
+                                                     const int32_t *__restrict__ bc,
+                                                     int32_t *__restrict__ bo)
 {
-    if (threadIdx.x != 0) return;
-    g->nl = *lend;
-    g->nb = bend ? *bend : 0;
+    __shared__ int32_t wsum[16];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int a = 0; a < (bc ? 2 : 1); ++a) {
+        const int32_t *cnt = a ? bc : lc;
+        int32_t *off = a ? bo : lo;
+        int64_t carry = 0;
+        for (int64_t base = 0; base < nb; base += 1024) {
+            const int64_t i = base + threadIdx.x;
+            const int32_t v = i < nb ? cnt[i] : 0;
+            int32_t x = v;   // inclusive scan over the wavefront
+            for (int d = 1; d < 64; d <<= 1) {
+                const int32_t y = __shfl_up(x, d, 64);
+                if (lane >= d) x += y;
+            }
+            if (lane == 63) wsum[w] = x;
+            __syncthreads();
+            int64_t pre = carry, tot = 0;
+            for (int k = 0; k < 16; ++k) {
+                pre += k < w ? wsum[k] : 0;
+                tot += wsum[k];
+            }
+            if (i < nb) off[i] = (int32_t)(pre + x - v);
+            carry += tot;
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            if (a)
+                g->nb = carry;
+            else
+                g->nl = carry;
+        }
+    }
 }
 
-// Ordered fill of the shell pair list (source order within each target cell).
+// Ordered fill of the shell pair list (lt non-null: rc2 < D^2 <= rm2) and of
+// the far band list (bt non-null: rc2 < D^2 <= rl2), source order within each
+// target cell, at the offsets of the scans.
 __global__ __launch_bounds__(64) void k_shell_fill(int64_t nocc, const double *__restrict__ mom,
-                                                   double rc2, double rm2,
-                                                   const int32_t *__restrict__ off,
+                                                   double rc2, double rm2, double rl2,
+                                                   const int32_t *__restrict__ loff,
                                                    int32_t *__restrict__ lt,
-                                                   int32_t *__restrict__ ls)
+                                                   int32_t *__restrict__ ls,
+                                                   const int32_t *__restrict__ boff,
+                                                   int32_t *__restrict__ bt,
+                                                   int32_t *__restrict__ bs)
 {
     const int64_t t = blockIdx.x;
     if (t >= nocc) return;
     const double *mt = mom + t * kMomStride;
     const int lane = threadIdx.x;
     const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
-    int64_t base = off[t];
+    int64_t lb = lt ? loff[t] : 0, bb = bt ? boff[t] : 0;
     for (int64_t s0 = 0; s0 < nocc; s0 += 64) {
         const int64_t s = s0 + lane;
-        bool in = false;
+        bool inl = false, inb = false;
         if (s < nocc) {
             const double d2 = centre_dist2(mt, mom + s * kMomStride);
-            in = d2 > rc2 && d2 <= rm2;
+            inl = lt && d2 > rc2 && d2 <= rm2;
+            inb = bt && d2 > rc2 && d2 <= rl2;
         }
-        const unsigned long long mask = __ballot(in);
-        if (in) {
-            const int64_t p = base + __popcll(mask & below);
+        const unsigned long long ml = __ballot(inl), mb = __ballot(inb);
+        if (inl) {
+            const int64_t p = lb + __popcll(ml & below);
             lt[p] = (int32_t)t;
             ls[p] = (int32_t)s;
         }
-        base += __popcll(mask);
+        if (inb) {
+            const int64_t p = bb + __popcll(mb & below);
+            bt[p] = (int32_t)t;
+            bs[p] = (int32_t)s;
+        }
+        lb += __popcll(ml);
+        bb += __popcll(mb);
     }
 }
 
@@ -894,7 +962,6 @@ __global__ __launch_bounds__(64) void k_near(
     }
 }
 
-__global__ void k_set_u64(unsigned long long *p, unsigned long long v) { *p = v; }
 
 // Near-field pairs of a binning: every occupied target cell's population
 // times the particles of its stencil columns (what k_near evaluates), summed
@@ -926,16 +993,6 @@ __global__ void k_near_pairs(int64_t nocc, const int32_t *__restrict__ occ,
 
 // The pairs of one near-field pass (the binning's) into the running total.
 __global__ void k_add_near_pairs(unsigned long long *cnt) { cnt[1] += cnt[0]; }
-
-// Population of every cell (empty cells 0) and one trailing zero, for the
-// exclusive scan that gives pstart.
-__global__ void k_cell_counts(int64_t ncell, const int32_t *__restrict__ cstart,
-                              const int32_t *__restrict__ cend, int32_t *__restrict__ cnt)
-{
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k > ncell) return;
-    cnt[k] = k < ncell ? cend[k] - cstart[k] : 0;
-}
 
 // Far field at each particle (sorted order) from the far cells of one slice of
 // the occupied cells (grid.y), as that slice's partial sums: mode 0 velocity;
@@ -1214,18 +1271,6 @@ static int closure_residual(fv_vpm *f, bool use_mlp);
 
 static int build_local(fv_closure *cl, fv_ctx *ctx);
 
-// Scan of n int32 counts into offsets (exclusive), temporary storage grown as
-// needed.
-static int scan_counts(fv_closure *cl, fv_ctx *ctx, const int32_t *cnt, int32_t *off, int64_t n)
-{
-    size_t sb = 0;
-    FV_HIP_CHECK(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, sb, cnt, off, (int)n, ctx->stream));
-    int rc = grow(ctx, &cl->tmp, &cl->tmp_bytes, sb);
-    if (rc) return rc;
-    FV_HIP_CHECK(ctx, hipcub::DeviceScan::ExclusiveSum(cl->tmp, sb, cnt, off, (int)n, ctx->stream));
-    return FV_OK;
-}
-
 // Bins the particles (cells, moments, shell pair list) for the current state
 // and, when `with_res`, computes the learned residual.  With `stage` (the RK
 // stages of one advect step, whose particles move a small fraction of a cell)
@@ -1290,7 +1335,7 @@ static int closure_build(fv_vpm *f, bool use_mlp, bool with_res = true, bool sta
                        f->y, f->z, cl->bbox);
     FV_LAUNCH_CHECK(ctx);
     hipLaunchKernelGGL(k_grid, dim3(1), dim3(64), 0, ctx->stream, cl->bbox, nbb, f->np_dev, cl->h,
-                       (double)cl->max_cells, cl->grid);
+                       (double)cl->max_cells, cl->grid, cl->near_cnt);
     FV_LAUNCH_CHECK(ctx);
     // bounds: cells of a grid, occupied cells
     const int64_t ncb = 8 * cl->max_cells < 262144 ? 8 * cl->max_cells : 262144;
@@ -1324,27 +1369,18 @@ static int closure_build(fv_vpm *f, bool use_mlp, bool with_res = true, bool sta
     FV_HIP_CHECK(ctx, hipcub::DeviceRadixSort::SortPairs(cl->tmp, tb, cl->key, cl->key_s, cl->idx,
                                                         cl->perm, (int)nu, 0, kKeyBits,
                                                         ctx->stream));
-    // per-cell arrays (for the grid's cell limit)
-    const size_t cneed = ((size_t)ncb * 6 + 2) * sizeof(int32_t) + 1024;
+    // per-cell arrays (for the grid's cell limit): cell starts, occupied cells
+    const size_t cneed = ((size_t)ncb * 3 + 2) * sizeof(int32_t) + 1024;
     rc = grow(ctx, &cl->cells, &cl->cells_bytes, cneed);
     if (rc) return rc;
-    cl->cstart = static_cast<int32_t *>(cl->cells);
-    cl->cend = cl->cstart + ncb;
-    cl->occ = cl->cend + ncb;
+    cl->pstart = static_cast<int32_t *>(cl->cells);
+    cl->cstart = cl->pstart;
+    cl->cend = cl->pstart + 1;
+    cl->occ = cl->pstart + ncb + 1;
     cl->occ_of = cl->occ + ncb;
-    cl->ccount = cl->occ_of + ncb;
-    cl->pstart = cl->ccount + ncb + 1;
-    hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)ceil_div(2 * ncb, 256)), dim3(256), 0,
-                       ctx->stream, cl->cstart, 2 * ncb, 0);
+    hipLaunchKernelGGL(k_cell_bounds, dim3((unsigned)ceil_div(ncb + 1, 256)), dim3(256), 0,
+                       ctx->stream, ncb, cl->grid, cl->key_s, cl->pstart);
     FV_LAUNCH_CHECK(ctx);
-    hipLaunchKernelGGL(k_cell_ranges, dim3(gb), dim3(256), 0, ctx->stream, nu, cl->grid, cl->key_s,
-                       cl->cstart, cl->cend);
-    FV_LAUNCH_CHECK(ctx);
-    hipLaunchKernelGGL(k_cell_counts, dim3((unsigned)ceil_div(ncb + 1, 256)), dim3(256), 0,
-                       ctx->stream, ncb, cl->cstart, cl->cend, cl->ccount);
-    FV_LAUNCH_CHECK(ctx);
-    rc = scan_counts(cl, ctx, cl->ccount, cl->pstart, ncb + 1);
-    if (rc) return rc;
     hipLaunchKernelGGL(k_gather, dim3(gb), dim3(256), 0, ctx->stream, nu, cl->grid, cl->perm, f->x,
                        f->y, f->z, f->gx, f->gy, f->gz, f->sig, cl->sx, cl->sy, cl->sz, cl->sax,
                        cl->say, cl->saz, cl->ssg);
@@ -1369,21 +1405,13 @@ static int closure_build(fv_vpm *f, bool use_mlp, bool with_res = true, bool sta
                        cl->cstart, cl->cend, cl->sx, cl->sy, cl->sz, cl->sax, cl->say, cl->saz,
                        cl->mom);
     FV_LAUNCH_CHECK(ctx);
-    hipLaunchKernelGGL(k_shell_count, dim3((unsigned)(nob + 1)), dim3(64), 0, ctx->stream,
-                       cl->grid, cl->mom, cl->kappa, 0.0, 0, cl->lcnt);
-    FV_LAUNCH_CHECK(ctx);
-    rc = scan_counts(cl, ctx, cl->lcnt, cl->loff, nob + 1);
-    if (rc) return rc;
     const bool band = cl->kappa_local > 0.0;
-    if (band) {
-        hipLaunchKernelGGL(k_shell_count, dim3((unsigned)(nob + 1)), dim3(64), 0, ctx->stream,
-                           cl->grid, cl->mom, cl->kappa, cl->kappa_local, 1, cl->bcnt);
-        FV_LAUNCH_CHECK(ctx);
-        rc = scan_counts(cl, ctx, cl->bcnt, cl->boff, nob + 1);
-        if (rc) return rc;
-    }
-    hipLaunchKernelGGL(k_list_sizes, dim3(1), dim3(64), 0, ctx->stream, cl->grid, cl->loff + nob,
-                       band ? cl->boff + nob : (const int32_t *)nullptr);
+    hipLaunchKernelGGL(k_shell_count, dim3((unsigned)(nob + 1)), dim3(64), 0, ctx->stream,
+                       cl->grid, cl->mom, cl->kappa, cl->kappa_local, cl->lcnt,
+                       band ? cl->bcnt : (int32_t *)nullptr);
+    FV_LAUNCH_CHECK(ctx);
+    hipLaunchKernelGGL(k_scan_lists, dim3(1), dim3(1024), 0, ctx->stream, cl->grid, nob + 1,
+                       cl->lcnt, cl->loff, band ? cl->bcnt : (const int32_t *)nullptr, cl->boff);
     FV_LAUNCH_CHECK(ctx);
     // the binning's one host read
     ClGrid G;
@@ -1418,7 +1446,8 @@ static int closure_build(fv_vpm *f, bool use_mlp, bool with_res = true, bool sta
     const double h = cl->hh;
     rc = build_local(cl, ctx);
     if (rc) return rc;
-    // shell pair list (geometry only)
+    // shell pair list (geometry only) and far band list (source cells with
+    // kappa h < distance <= kappa_local h), in one pass
     const double rc2 = cut2(cl->kappa, h);
     const int64_t nl = cl->nlist;
     if (nl > 0) {
@@ -1431,26 +1460,26 @@ static int closure_build(fv_vpm *f, bool use_mlp, bool with_res = true, bool sta
         cl->Y = cl->X + (size_t)nl * kFeat;
         cl->lt = reinterpret_cast<int32_t *>(cl->Y + (size_t)nl * kOutW);
         cl->ls = cl->lt + nl;
-        hipLaunchKernelGGL(k_shell_fill, dim3((unsigned)nocc), dim3(64), 0, ctx->stream, nocc,
-                           cl->mom, rc2, rc2 * kShell * kShell, cl->loff, cl->lt, cl->ls);
-        FV_LAUNCH_CHECK(ctx);
     }
-    // far band list: source cells with kappa h < distance <= kappa_local h
     const int64_t nb = cl->nband;
     if (nb > 0) {
         rc = grow(ctx, &cl->band, &cl->band_bytes, (size_t)2 * (nb + 1) * sizeof(int32_t));
         if (rc) return rc;
         cl->bt = static_cast<int32_t *>(cl->band);
         cl->bs = cl->bt + (nb + 1);
+    }
+    if (nl > 0 || nb > 0) {
         hipLaunchKernelGGL(k_shell_fill, dim3((unsigned)nocc), dim3(64), 0, ctx->stream, nocc,
-                           cl->mom, rc2, cut2(cl->kappa_local, h), cl->boff, cl->bt, cl->bs);
+                           cl->mom, rc2, rc2 * kShell * kShell,
+                           nb > 0 ? cut2(cl->kappa_local, h) : 0.0, cl->loff,
+                           nl > 0 ? cl->lt : (int32_t *)nullptr, cl->ls, cl->boff,
+                           nb > 0 ? cl->bt : (int32_t *)nullptr, cl->bs);
         FV_LAUNCH_CHECK(ctx);
     }
     rc = update_stencil(cl, ctx);
     if (rc) return rc;
-    // near-field pairs of this binning (every pass of it evaluates them)
-    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, ctx->stream, cl->near_cnt, 0ull);
-    FV_LAUNCH_CHECK(ctx);
+    // near-field pairs of this binning (every pass of it evaluates them; the
+    // count was cleared by k_grid)
     hipLaunchKernelGGL(k_near_pairs, dim3((unsigned)ceil_div(nocc, 256)), dim3(256), 0,
                        ctx->stream, nocc, cl->occ, cl->cstart, cl->cend, cl->pstart,
                        cl->dstencil, (int)(cl->stencil.size() / 3), cl->nx, cl->ny, cl->nz,
