@@ -900,6 +900,7 @@ __global__ __launch_bounds__(64) void k_near(
         tg[u].ax = sax[tl]; tg[u].ay = say[tl]; tg[u].az = saz[tl];
         Op::init(acc[u]);
     }
+    if constexpr (KERNEL == 0) gauss_stage();   // read after the first tile barrier
     const int cx = c / (ny * nz), cy = (c / nz) % ny, cz = c % nz;
     const int sl = blockIdx.z;
     for (int e = slice_off[sl]; e < slice_off[sl + 1]; ++e) {

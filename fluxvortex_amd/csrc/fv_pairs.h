@@ -13,6 +13,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
+
+#include "fv_gauss_table.h"
 
 namespace fv {
 
@@ -161,6 +164,46 @@ __device__ __forceinline__ void seg_pair_mixed(double px, double py, double pz, 
 }
 
 // ----------------------------------------------------- particle kernels
+// Gaussian-erf factors G1 = g / rho^3, G2 = g' / rho^4 - 3 g / rho^5 and
+// G3 = g' / rho^2 - g / rho^3 for rho = r / sigma in [0, 10], from the
+// piecewise polynomials of fv_gauss_table.h (fitted at 50 digits; about one
+// unit in the last place of the terms' scale).  The closed form g = erf -
+// sqrt(2/pi) rho e^(-rho^2/2) cancels as rho -> 0 (g ~ rho^3), and the erf and
+// exp it needs are long, branchy library routines that a wavefront runs for
+// every interval its lanes fall in; the table costs one row of LDS reads and
+// a degree-12 Horner chain per factor.  Every kernel that evaluates the
+// Gaussian factors stages the table in LDS first (gauss_stage(), before a
+// barrier).  rho beyond the table or NaN selects the last piece (callers
+// discard such pairs).
+static __shared__ __attribute__((aligned(16))) double g_gauss_lds[kGaussPieces * 3 * kGaussRow];
+
+__device__ __forceinline__ void gauss_stage()
+{
+    const double *src = &kGaussTab[0][0][0];
+    for (int i = threadIdx.x; i < kGaussPieces * 3 * kGaussRow; i += blockDim.x)
+        g_gauss_lds[i] = src[i];
+}
+
+__device__ __forceinline__ double gauss_poly(const double *c, double t)
+{
+    double v = c[kGaussDeg];
+#pragma unroll
+    for (int j = kGaussDeg - 1; j >= 0; --j) v = fma(v, t, c[j]);
+    return v;
+}
+
+// Table factors FA and, when FB >= 0, FB (0: G1, 1: G2, 2: G3) at rho.
+template <int FA, int FB>
+__device__ __forceinline__ void gauss_factors(double rho, double &a, double &b)
+{
+    int k = rho < 10.0 ? (int)(rho * 2.0) : kGaussPieces - 1;
+    k = k < 0 ? 0 : (k > kGaussPieces - 1 ? kGaussPieces - 1 : k);
+    const double t = fma(4.0, rho, -(double)(2 * k + 1));
+    const double *c = g_gauss_lds + k * (3 * kGaussRow);
+    a = gauss_poly(c + FA * kGaussRow, t);
+    b = FB >= 0 ? gauss_poly(c + (FB >= 0 ? FB : 0) * kGaussRow, t) : 0.0;
+}
+
 // Smoothing-kernel factors for one pair at squared distance r2:
 //   CG = -(1/4pi) g(rho)/r^3                       (velocity factor)
 //   CH = -(1/4pi) (g'(rho)/(sigma r) - 3 g/r^2)/r^3  (gradient factor)
@@ -173,7 +216,8 @@ __device__ __forceinline__ void seg_pair_mixed(double px, double py, double pz, 
 // evaluated with Z = 2t from rsqrt2_refined, Z^5 = 32 t^5 and Z^7 = 128 t^7
 // folded into k1..k4: CG = (k1 q + k2) Z^5, CH = (k3 q + k4) Z^7.
 // Gaussian-erf (kernel.py:26-31): g = erf(rho/sqrt2) - sqrt(2/pi) rho e^(-rho^2/2),
-//   g' = sqrt(2/pi) rho^2 e^(-rho^2/2).
+//   g' = sqrt(2/pi) rho^2 e^(-rho^2/2); for rho <= 10 CG = -G1 / (4 pi sigma^3) and
+//   CH = -G2 / (4 pi sigma^5) from the table above.
 template <int KERNEL, bool GRAD>
 __device__ __forceinline__ void part_factors_f64(double r2, bool keep, const PartRec &s,
                                                  double &CG, double &CH)
@@ -202,17 +246,12 @@ __device__ __forceinline__ void part_factors_f64(double r2, bool keep, const Par
             if (GRAD) CH = (3.0 * kInv4Pi / 32.0) * (Z3 * Z2);
         } else {
             const double ir = 0.5 * rsqrt2_refined(r2);
-            const double r = r2 * ir;
-            const double rho = r * s.k0;
-            const double ev = exp(-0.5 * rho * rho);
-            const double g = erf(rho * kInvSqrt2) - kSqrt2OverPi * rho * ev;
-            const double ir3 = ir * ir * ir;
-            CG = -kInv4Pi * g * ir3;
-            if (GRAD) {
-                const double dgdr = kSqrt2OverPi * rho * rho * ev;
-                const double aux = dgdr * (s.k0 * ir) - 3.0 * g * (ir * ir);
-                CH = -kInv4Pi * aux * ir3;
-            }
+            const double rho = (r2 * ir) * s.k0;
+            double G1, G2;
+            gauss_factors<0, GRAD ? 1 : -1>(rho, G1, G2);
+            const double is3 = s.k0 * s.k2;   // 1 / sigma^3
+            CG = (-kInv4Pi * G1) * is3;
+            if (GRAD) CH = (-kInv4Pi * G2) * (is3 * s.k2);
         }
         CG = keep ? CG : 0.0;
         if (GRAD) CH = keep ? CH : 0.0;
@@ -286,9 +325,14 @@ __device__ __forceinline__ void gauss_curl_factors_f64(double r2, bool keep, con
         CH = (3.0 * kInv4Pi / 32.0) * (Z3 * Z2);
         B = (kInv4Pi / 8.0) * Z3;
     } else {
-        double CG;
-        part_factors_f64<0, true>(r2, true, s, CG, CH);
-        B = fma(CH, r2, CG + CG);
+        // CH = -G2 / (4 pi sigma^5), B = -G3 / (4 pi sigma^3) (table)
+        const double ir = 0.5 * rsqrt2_refined(r2);
+        const double rho = (r2 * ir) * s.k0;
+        double G2, G3;
+        gauss_factors<1, 2>(rho, G2, G3);
+        const double is3 = s.k0 * s.k2;
+        CH = (-kInv4Pi * G2) * (is3 * s.k2);
+        B = (-kInv4Pi * G3) * is3;
     }
     CH = keep ? CH : 0.0;
     B = keep ? B : 0.0;
@@ -446,6 +490,7 @@ struct PartOp {
     // source-loop unroll at two targets per lane: the Gaussian's erf / exp
     // branch doubled would exceed 128 VGPRs (three wavefronts per SIMD)
     static constexpr int kUnroll2 = (KERNEL == 0) ? 1 : 2;
+    static constexpr bool kGaussTab = KERNEL == 0;   // gauss_stage() before the pairs
     struct Args {
         PartSrcArgs src;
         PartTgtArgs tgt;
@@ -622,6 +667,16 @@ template <class Op, class = void>
 struct PairUnroll2 {
     static constexpr int value = 2;
 };
+
+// Ops whose pairs evaluate the Gaussian factor table (staged in LDS).
+template <class Op, class = void>
+struct UsesGaussTab {
+    static constexpr bool value = false;
+};
+template <class Op>
+struct UsesGaussTab<Op, std::void_t<decltype(Op::kGaussTab)>> {
+    static constexpr bool value = Op::kGaussTab;
+};
 template <class Op>
 struct PairUnroll2<Op, decltype((void)Op::kUnroll2)> {
     static constexpr int value = Op::kUnroll2;
@@ -642,6 +697,7 @@ __global__ __launch_bounds__(BLOCK) void k_pairs(typename Op::Args a, int64_t n_
     if (i0 >= n_tgt) return;   // uniform over the block
     // chunk <= 0: the slices share the device count evenly (whole tiles)
     if (chunk <= 0) chunk = ((n_src + gridDim.y - 1) / gridDim.y + BLOCK - 1) / BLOCK * BLOCK;
+    if constexpr (UsesGaussTab<Op>::value) gauss_stage();   // read after the tile barrier
     typename Op::Tgt t[TPT];
     typename Op::Acc acc[TPT];
 #pragma unroll

@@ -237,6 +237,7 @@ __device__ void sw_induce_body(const SwInst &I, double *__restrict__ S, double *
     const int64_t c1 = (c0 + kSwChunk < I.np) ? c0 + kSwChunk : I.np;
     for (int e = t; e < 3 * P; e += kSwBlock) tgt[e] = S[I.o_coln + e];
     using Op = PartOp<KERNEL, PREC, 0>;
+    if constexpr (KERNEL == 0) gauss_stage();
     typename Op::Tgt tg;
     typename Op::Acc acc;
     Op::init(acc);
