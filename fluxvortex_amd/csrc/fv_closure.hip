@@ -201,21 +201,26 @@ __global__ void k_bbox_partial(const int64_t *__restrict__ np_dev, const double 
     if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = s[threadIdx.x][0];
 }
 
-// The bounding box (the blocks' partials in order) and the grid of a binning
-// (one lane).  With no particles, or a non-finite position (bad = 1), a
-// one-cell grid that bins nothing.
-__global__ void k_grid(const double *__restrict__ part, int nbb, const int64_t *__restrict__ np_dev,
-                       double h0, double max_cells, ClGrid *__restrict__ g,
-                       unsigned long long *__restrict__ near_pairs)
+// The bounding box (the blocks' partials, one per lane of one wavefront,
+// nbb <= 64) and the grid of a binning.  With no particles, or a non-finite
+// position (bad = 1), a one-cell grid that bins nothing.
+__global__ __launch_bounds__(64) void k_grid(const double *__restrict__ part, int nbb,
+                                             const int64_t *__restrict__ np_dev, double h0,
+                                             double max_cells, ClGrid *__restrict__ g,
+                                             unsigned long long *__restrict__ near_pairs)
 {
-    if (threadIdx.x != 0) return;
-    *near_pairs = 0;   // the near-field pairs of this binning, counted after the read
+    const int l = threadIdx.x;
     double bb[6];
     for (int c = 0; c < 6; ++c) {
-        double v = c < 3 ? INFINITY : -INFINITY;
-        for (int b = 0; b < nbb; ++b) v = c < 3 ? fmin(v, part[b * 6 + c]) : fmax(v, part[b * 6 + c]);
+        double v = l < nbb ? part[l * 6 + c] : (c < 3 ? INFINITY : -INFINITY);
+        for (int off = 32; off > 0; off >>= 1) {
+            const double o = __shfl_xor(v, off, 64);
+            v = c < 3 ? fmin(v, o) : fmax(v, o);
+        }
         bb[c] = v;
     }
+    if (l != 0) return;
+    *near_pairs = 0;   // the near-field pairs of this binning, counted after the read
     const int64_t n = *np_dev;
     bool fin = true;
     for (int c = 0; c < 6; ++c) fin = fin && isfinite(bb[c]);
@@ -736,10 +741,11 @@ __global__ __launch_bounds__(64) void k_shell_count(const ClGrid *__restrict__ g
     }
 }
 
-// Exclusive scans of the shell counts and (bc non-null) the band counts, nb
-// entries each, by one workgroup of 16 wavefronts (1024 entries a round); the
-// totals, the list lengths, into the grid record.
-__global__ __launch_bounds__(1024) void k_scan_lists(ClGrid *__restrict__ g, int64_t nb,
+// Exclusive scans of the shell counts and (bc non-null) the band counts by
+// one workgroup of 16 wavefronts (4 consecutive entries a lane, 4096 a
+// round) over the entries 0 .. nocc (the offsets the lists are read at; the
+// counts past nocc are 0); the totals, the list lengths, into the grid record.
+__global__ __launch_bounds__(1024) void k_scan_lists(ClGrid *__restrict__ g,
                                                      const int32_t *__restrict__ lc,
                                                      int32_t *__restrict__ lo,
                                                      const int32_t *__restrict__ bc,
@@ -747,14 +753,19 @@ __global__ __launch_bounds__(1024) void k_scan_lists(ClGrid *__restrict__ g, int
 {
     __shared__ int32_t wsum[16];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t nb = g->nocc + 1;
     for (int a = 0; a < (bc ? 2 : 1); ++a) {
         const int32_t *cnt = a ? bc : lc;
         int32_t *off = a ? bo : lo;
         int64_t carry = 0;
-        for (int64_t base = 0; base < nb; base += 1024) {
-            const int64_t i = base + threadIdx.x;
-            const int32_t v = i < nb ? cnt[i] : 0;
-            int32_t x = v;   // inclusive scan over the wavefront
+        for (int64_t base = 0; base < nb; base += 4096) {
+            const int64_t i0 = base + 4 * (int64_t)threadIdx.x;
+            int32_t v[4], run = 0;
+            for (int q = 0; q < 4; ++q) {
+                v[q] = i0 + q < nb ? cnt[i0 + q] : 0;
+                run += v[q];
+            }
+            int32_t x = run;   // inclusive scan of the lanes' totals over the wavefront
             for (int d = 1; d < 64; d <<= 1) {
                 const int32_t y = __shfl_up(x, d, 64);
                 if (lane >= d) x += y;
@@ -766,7 +777,11 @@ __global__ __launch_bounds__(1024) void k_scan_lists(ClGrid *__restrict__ g, int
                 pre += k < w ? wsum[k] : 0;
                 tot += wsum[k];
             }
-            if (i < nb) off[i] = (int32_t)(pre + x - v);
+            int64_t o = pre + x - run;
+            for (int q = 0; q < 4; ++q) {
+                if (i0 + q < nb) off[i0 + q] = (int32_t)o;
+                o += v[q];
+            }
             carry += tot;
             __syncthreads();
         }
@@ -1411,8 +1426,8 @@ static int closure_build(fv_vpm *f, bool use_mlp, bool with_res = true, bool sta
                        cl->grid, cl->mom, cl->kappa, cl->kappa_local, cl->lcnt,
                        band ? cl->bcnt : (int32_t *)nullptr);
     FV_LAUNCH_CHECK(ctx);
-    hipLaunchKernelGGL(k_scan_lists, dim3(1), dim3(1024), 0, ctx->stream, cl->grid, nob + 1,
-                       cl->lcnt, cl->loff, band ? cl->bcnt : (const int32_t *)nullptr, cl->boff);
+    hipLaunchKernelGGL(k_scan_lists, dim3(1), dim3(1024), 0, ctx->stream, cl->grid, cl->lcnt,
+                       cl->loff, band ? cl->bcnt : (const int32_t *)nullptr, cl->boff);
     FV_LAUNCH_CHECK(ctx);
     // the binning's one host read
     ClGrid G;
