@@ -310,3 +310,50 @@ def test_flapping_airplane_closure_at_100k(ctx):
     err = float(np.max(np.abs(cl - cl_ref)) / np.max(np.abs(cl_ref)))
     print(f"flapping 100k: CL error vs FP64 {err:.2e} (cell {cell}, kappa {kappa})", flush=True)
     assert np.all(np.isfinite(cl)) and err < 1e-3
+
+
+def test_closure_binning_sized_on_device(ctx):
+    """The binning takes its sizes from the device (launches and buffers sized
+    by bounds, one host read per binning): after population control drops
+    particles on the device (the host's launch bound then stays above the
+    count) and after the field grows, an advect step with the closure equals,
+    bit for bit, the same step of a field whose count the host knows exactly."""
+    from fluxvortex_amd.mlp import ClosureMLP, closure_params
+    from fluxvortex_amd.particles import VortexParticleField
+    pos, al, sg = wake_sheet(nx=500)
+    n0 = len(sg)
+    net = ClosureMLP(closure_params(seed=4))
+    U = np.array([10.0, 0.0, 0.0])
+
+    def make(st):
+        f = VortexParticleField(max_particles=n0 + 4096, rlxf=0.3, kernel="winckelmans")
+        f.load_state_dict(st)
+        # no training (the two fields share the network), a check that always
+        # passes, the far band and local expansions on
+        f.enable_closure(net, cell=2.0, kappa=3.0, check_every=1000, tol=1.0, train_iters=0)
+        f.set_closure_local(6.0)
+        return f
+
+    a = VortexParticleField(max_particles=n0 + 4096, rlxf=0.3, kernel="winckelmans")
+    a.add_particles_batch(pos, al, sg)
+    a.enable_closure(net, cell=2.0, kappa=3.0, check_every=1000, tol=1.0, train_iters=0)
+    a.set_closure_local(6.0)
+    extra = wake_sheet(nx=120, seed=9)
+    for change in ("drop", "grow", "drop"):
+        st = a.state_dict()
+        keep = len(st["sigma"]) - 700
+        if change == "drop":
+            a.drop_oldest(keep)            # the count changes on the device only
+            c = make(st)
+            c.drop_oldest(keep)
+            c.load_state_dict(c.state_dict())   # c: the exact count on the host
+        else:
+            shift = np.array([a.positions[:, 0].max() + 0.5, 0.0, 0.0])
+            a.add_particles_batch(extra[0] + shift, extra[1], extra[2])
+            c = make(a.state_dict())
+        a.advect_rk3(0.01, U, stretch=True, free_wake=True)
+        c.advect_rk3(0.01, U, stretch=True, free_wake=True)
+        sa, sc = a.state_dict(), c.state_dict()
+        for k in ("pos", "gamma", "sigma"):
+            assert np.array_equal(sa[k], sc[k]), (change, k)
+        assert a.closure_stats()["active"] and c.closure_stats()["active"]
