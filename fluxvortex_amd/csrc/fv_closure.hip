@@ -37,6 +37,22 @@ extern "C" int fv_mlp_train_step_dev(fv_mlp *m, const float *X, const float *T,
 
 namespace fv {
 
+hipStream_t mlp_set_stream(fv_mlp *m, hipStream_t s);   // fv_mlp.hip
+
+// The network's launches on stream s for the scope (the closure orders them
+// against the field's stream itself).
+struct MlpOn {
+    fv_mlp *m;
+    hipStream_t prev;
+    MlpOn(fv_mlp *mm, hipStream_t s) : m(mm), prev(mm ? mlp_set_stream(mm, s) : nullptr) {}
+    ~MlpOn()
+    {
+        if (m) mlp_set_stream(m, prev);
+    }
+    MlpOn(const MlpOn &) = delete;
+    MlpOn &operator=(const MlpOn &) = delete;
+};
+
 // Per occupied cell: n, xbar(3), A(3), M(9, row-major M[r][k] = Sum a_r d_k),
 // centre(3), S = Sum |alpha|, L2 = Sum |alpha| |d|^2 / S, Q(18: per component r
 // the symmetric Sum a_r d_k d_l for kl = 00 01 02 11 12 22), pad.
@@ -161,6 +177,12 @@ struct fv_closure_state {
     ClGrid *grid = nullptr;            // device grid record (in the bbox allocation)
     float *tX = nullptr, *tT = nullptr, *tw = nullptr;
     double *vdir = nullptr, *vclo = nullptr, *verr = nullptr;
+    // online training on a side stream (default priority), overlapping the
+    // step's first near- and far-field passes; the field's stream waits for
+    // ev_train where the weights are next read or the binning is rewritten
+    hipStream_t side = nullptr;
+    hipEvent_t ev_bin = nullptr, ev_train = nullptr;
+    bool train_pending = false;
 };
 
 }  // namespace fv
@@ -1287,6 +1309,16 @@ static int closure_residual(fv_vpm *f, bool use_mlp);
 
 static int build_local(fv_closure *cl, fv_ctx *ctx);
 
+// The field's stream waits for the pending online training (closure_train)
+// before anything reads the weights it writes or rewrites the binning it reads.
+static int join_training(fv_closure *cl, fv_ctx *ctx)
+{
+    if (!cl->train_pending) return FV_OK;
+    FV_HIP_CHECK(ctx, hipStreamWaitEvent(ctx->stream, cl->ev_train, 0));
+    cl->train_pending = false;
+    return FV_OK;
+}
+
 // Bins the particles (cells, moments, shell pair list) for the current state
 // and, when `with_res`, computes the learned residual.  With `stage` (the RK
 // stages of one advect step, whose particles move a small fraction of a cell)
@@ -1304,7 +1336,8 @@ static int closure_build(fv_vpm *f, bool use_mlp, bool with_res = true, bool sta
     fv_closure *cl = f->closure;
     fv_ctx *ctx = f->ctx;
     if (cl->built_epoch == f->state_epoch) return with_res ? closure_residual(f, use_mlp) : FV_OK;
-    int rc = FV_OK;
+    int rc = join_training(cl, ctx);
+    if (rc) return rc;
     if (stage && cl->binned_step == cl->steps && cl->nocc > 0) {
         rc = f->np_exact ? FV_OK : vpm_sync_count(f);
         if (rc) return rc;
@@ -1517,13 +1550,18 @@ static int closure_residual(fv_vpm *f, bool use_mlp)
         return FV_OK;
     const int64_t nocc = cl->nocc;
     if (nocc == 0) return FV_OK;
+    int rc = join_training(cl, ctx);
+    if (rc) return rc;
     if (use_mlp) {
         TimedLaunch tl(ctx, "closure_residual");
         const int64_t nl = cl->nlist;
         hipLaunchKernelGGL(k_shell_inputs, dim3((unsigned)ceil_div(nl, 256)), dim3(256), 0,
                            ctx->stream, nl, cl->lt, cl->ls, cl->mom, cl->hh, cl->X, cl->lk);
         FV_LAUNCH_CHECK(ctx);
-        int rc = fv_mlp_forward_dev(cl->mlp, cl->X, nl, cl->Y);
+        {
+            MlpOn on(cl->mlp, ctx->stream);
+            rc = fv_mlp_forward_dev(cl->mlp, cl->X, nl, cl->Y);
+        }
         if (rc) return rc;
         hipLaunchKernelGGL(k_shell_reduce, dim3((unsigned)nocc), dim3(64), 0, ctx->stream, nocc,
                            cl->loff, cl->lk, cl->Y, cl->res);
@@ -1531,7 +1569,7 @@ static int closure_residual(fv_vpm *f, bool use_mlp)
         tl.done();
         cl->far_evals += (double)nl;
     } else {
-        int rc = fill(ctx, cl->res, 3 * nocc, 0.0);
+        rc = fill(ctx, cl->res, 3 * nocc, 0.0);
         if (rc) return rc;
     }
     cl->res_bin = cl->bin_id;
@@ -1621,7 +1659,9 @@ static int closure_pass(fv_vpm *f, int mode, int kernel, double *out, bool use_m
     fv_ctx *ctx = f->ctx;
     FV_REQUIRE(ctx, mode != 4 || kernel == FV_KERNEL_GAUSSIAN_ERF,
                "the clamped-pair vorticity (mode 4) is the Gaussian field's");
-    int rc = closure_build(f, use_mlp, mode == 0, stage);
+    // the learned residual (velocity only) is formed after the near and far
+    // passes, so the online training of this binning overlaps them
+    int rc = closure_build(f, use_mlp, false, stage);
     if (rc) return rc;
     const int64_t n = cl->n;
     if (n == 0) return FV_OK;
@@ -1693,6 +1733,10 @@ static int closure_pass(fv_vpm *f, int mode, int kernel, double *out, bool use_m
         FV_LAUNCH_CHECK(ctx);
         tl.done();
     }
+    if (mode == 0) {
+        rc = closure_residual(f, use_mlp);
+        if (rc) return rc;
+    }
     hipLaunchKernelGGL(k_combine, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, ctx->stream, n,
                        cl->nsl, nfs, cl->key_s, cl->occ_of, cl->perm, cl->npart, cl->fpart,
                        mode == 0 ? cl->res : (const double *)nullptr, out);
@@ -1714,17 +1758,31 @@ static int closure_train(fv_vpm *f)
         FV_HIP_CHECK(ctx, hipMemcpyAsync(cl->tw, w, sizeof w, hipMemcpyHostToDevice, ctx->stream));
         FV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     }
-    TimedLaunch tl(ctx, "closure_train");
+    if (!cl->side) {
+        FV_HIP_CHECK(ctx, hipStreamCreateWithFlags(&cl->side, hipStreamNonBlocking));
+        FV_HIP_CHECK(ctx, hipEventCreateWithFlags(&cl->ev_bin, hipEventDisableTiming));
+        FV_HIP_CHECK(ctx, hipEventCreateWithFlags(&cl->ev_train, hipEventDisableTiming));
+    }
+    // after the binning, on the side stream; the weights are next read by the
+    // residual of the step's first velocity pass (join_training)
+    FV_HIP_CHECK(ctx, hipEventRecord(cl->ev_bin, ctx->stream));
+    FV_HIP_CHECK(ctx, hipStreamWaitEvent(cl->side, cl->ev_bin, 0));
+    TimedLaunch tl(ctx, "closure_train", cl->side);
     cl->seed = cl->seed * 6364136223846793005ull + 1442695040888963407ull;
-    hipLaunchKernelGGL(k_train_samples, dim3((unsigned)S), dim3(64), 0, ctx->stream, cl->nlist,
+    hipLaunchKernelGGL(k_train_samples, dim3((unsigned)S), dim3(64), 0, cl->side, cl->nlist,
                        cl->lt, cl->ls, cl->occ, cl->cstart, cl->cend, cl->sx, cl->sy, cl->sz,
                        cl->sax, cl->say, cl->saz, cl->mom, cl->hh, cl->seed, cl->tX, cl->tT);
     FV_LAUNCH_CHECK(ctx);
-    for (int it = 0; it < cl->train_iters; ++it) {
-        int rc = fv_mlp_train_step_dev(cl->mlp, cl->tX, cl->tT, cl->tw, S, (float)cl->lr);
-        if (rc) return rc;
+    {
+        MlpOn on(cl->mlp, cl->side);
+        for (int it = 0; it < cl->train_iters; ++it) {
+            int rc = fv_mlp_train_step_dev(cl->mlp, cl->tX, cl->tT, cl->tw, S, (float)cl->lr);
+            if (rc) return rc;
+        }
     }
     tl.done();
+    FV_HIP_CHECK(ctx, hipEventRecord(cl->ev_train, cl->side));
+    cl->train_pending = true;
     cl->mlp_version++;
     return FV_OK;
 }
@@ -1800,6 +1858,12 @@ int closure_self_sum(fv_vpm *f, int mode, int kernel, double *out)
 void closure_destroy(fv_closure *cl)
 {
     if (!cl) return;
+    if (cl->side) {
+        (void)hipStreamSynchronize(cl->side);
+        (void)hipEventDestroy(cl->ev_bin);
+        (void)hipEventDestroy(cl->ev_train);
+        (void)hipStreamDestroy(cl->side);
+    }
     void *bufs[] = {cl->block, cl->cells, cl->far, cl->band, cl->list, cl->tmp, cl->bbox, cl->dstencil,
                     cl->tX, cl->vdir, cl->lpart};
     for (void *b : bufs)
@@ -1937,6 +2001,7 @@ int fv_vpm_closure_train(fv_vpm *f, int32_t rounds)
     if (!f || !f->closure) return FV_E_ARG;
     int rc = closure_build(f, false, false);
     for (int r = 0; !rc && r < rounds; ++r) rc = closure_train(f);
+    if (!rc) rc = join_training(f->closure, f->ctx);
     if (!rc) rc = fv_ctx_synchronize(f->ctx);
     return rc;
 }

@@ -95,12 +95,14 @@ void *workspace(fv_ctx *ctx, int slot, size_t bytes);
 double *upload(fv_ctx *ctx, int slot, const double *host, size_t count);
 
 // Brackets one launch of a timed kernel family with events when timing is
-// on:  TimedLaunch t(ctx, "seg_pairs"); launch...; t.done();
+// on:  TimedLaunch t(ctx, "seg_pairs"); launch...; t.done();  the events go
+// on `stream` (default: the context's stream), where the launches go.
 struct TimedLaunch {
     fv_ctx *ctx;
     const char *family;
+    hipStream_t stream;
     hipEvent_t start = nullptr;
-    TimedLaunch(fv_ctx *c, const char *fam);
+    TimedLaunch(fv_ctx *c, const char *fam, hipStream_t s = nullptr);
     void done();
 };
 

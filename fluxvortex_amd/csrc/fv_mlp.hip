@@ -395,7 +395,23 @@ struct fv_mlp {
     // training scratch (grown on demand)
     void *scratch = nullptr;
     int64_t scratch_nb = 0;
+    // stream of the network's launches when set (a caller that orders them
+    // against its own streams, mlp_set_stream), else the context's
+    hipStream_t on = nullptr;
 };
+
+namespace fv {
+static hipStream_t mlp_stream(const fv_mlp *m) { return m->on ? m->on : m->ctx->stream; }
+
+// Sets the stream of the network's launches (nullptr: its context's);
+// returns the previous setting.
+hipStream_t mlp_set_stream(fv_mlp *m, hipStream_t s)
+{
+    const hipStream_t prev = m->on;
+    m->on = s;
+    return prev;
+}
+}  // namespace fv
 
 static constexpr int kWeightsBf = kHid * kIn + 2 * kHid * kHid + kOut * kHid;
 
@@ -449,7 +465,7 @@ static int mlp_scratch(fv_mlp *m, int64_t nb)
     if (m->scratch_nb >= nb) return FV_OK;
     fv_ctx *ctx = m->ctx;
     if (m->scratch) {
-        FV_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        FV_HIP_CHECK(ctx, hipStreamSynchronize(mlp_stream(m)));
         FV_HIP_CHECK(ctx, hipFree(m->scratch));
         m->scratch = nullptr;
     }
@@ -548,14 +564,14 @@ int fv_mlp_forward_dev(fv_mlp *m, const float *X, int64_t nb, float *Y)
     if (nb == 0) return FV_OK;
     FV_REQUIRE(ctx, X && Y, "null X/Y");
     const MlpShape sh = mlp_shape(ctx, nb);
-    TimedLaunch tl(ctx, "mlp_forward");
+    TimedLaunch tl(ctx, "mlp_forward", mlp_stream(m));
     if (sh.stage)
         hipLaunchKernelGGL(k_mlp_forward<true>, dim3(sh.grid), dim3(64 * sh.waves),
-                           (unsigned)sh.lds_fwd, ctx->stream, mlp_views(m), X, nb, Y,
+                           (unsigned)sh.lds_fwd, mlp_stream(m), mlp_views(m), X, nb, Y,
                            (float *)nullptr, (unsigned short *)nullptr);
     else
         hipLaunchKernelGGL(k_mlp_forward<false>, dim3(sh.grid), dim3(64 * sh.waves),
-                           (unsigned)sh.lds_fwd, ctx->stream, mlp_views(m), X, nb, Y,
+                           (unsigned)sh.lds_fwd, mlp_stream(m), mlp_views(m), X, nb, Y,
                            (float *)nullptr, (unsigned short *)nullptr);
     FV_LAUNCH_CHECK(ctx);
     tl.done();
@@ -577,19 +593,19 @@ static int mlp_gradients(fv_mlp *m, const float *X, const float *T, const float 
     unsigned short *dy = dz + nb * 384;
     const MlpShape sh = mlp_shape(ctx, nb);
     const MlpW w = mlp_views(m);
-    TimedLaunch tl(ctx, "mlp_train");
+    TimedLaunch tl(ctx, "mlp_train", mlp_stream(m));
     if (sh.stage) {
         hipLaunchKernelGGL(k_mlp_forward<true>, dim3(sh.grid), dim3(64 * sh.waves),
-                           (unsigned)sh.lds_fwd, ctx->stream, w, X, nb, Y, zs, hs);
+                           (unsigned)sh.lds_fwd, mlp_stream(m), w, X, nb, Y, zs, hs);
         FV_LAUNCH_CHECK(ctx);
         hipLaunchKernelGGL(k_mlp_backward<true>, dim3(sh.grid), dim3(64 * sh.waves),
-                           (unsigned)sh.lds_bwd, ctx->stream, w, Y, T, wout, nb, zs, dz, dy);
+                           (unsigned)sh.lds_bwd, mlp_stream(m), w, Y, T, wout, nb, zs, dz, dy);
     } else {
         hipLaunchKernelGGL(k_mlp_forward<false>, dim3(sh.grid), dim3(64 * sh.waves),
-                           (unsigned)sh.lds_fwd, ctx->stream, w, X, nb, Y, zs, hs);
+                           (unsigned)sh.lds_fwd, mlp_stream(m), w, X, nb, Y, zs, hs);
         FV_LAUNCH_CHECK(ctx);
         hipLaunchKernelGGL(k_mlp_backward<false>, dim3(sh.grid), dim3(64 * sh.waves),
-                           (unsigned)sh.lds_bwd, ctx->stream, w, Y, T, wout, nb, zs, dz, dy);
+                           (unsigned)sh.lds_bwd, mlp_stream(m), w, Y, T, wout, nb, zs, dz, dy);
     }
     FV_LAUNCH_CHECK(ctx);
     const size_t base = (size_t)nb * (16 * 4 + 384 * 4 + 384 * 2 + 384 * 2 + 16 * 2) + 1024;
@@ -608,10 +624,10 @@ static int mlp_gradients(fv_mlp *m, const float *X, const float *T, const float 
         tiles += (plan.L[k].ne / 16) * (plan.L[k].nh / 16);
     }
     hipLaunchKernelGGL(k_mlp_wgrad_part, dim3((unsigned)tiles, (unsigned)nchunk), dim3(64), 0,
-                       ctx->stream, plan, nb, part);
+                       mlp_stream(m), plan, nb, part);
     FV_LAUNCH_CHECK(ctx);
     hipLaunchKernelGGL(k_mlp_wgrad_reduce, dim3((unsigned)ceil_div(kParams, 256)), dim3(256), 0,
-                       ctx->stream, part, nchunk, m->grad);
+                       mlp_stream(m), part, nchunk, m->grad);
     FV_LAUNCH_CHECK(ctx);
     tl.done();
     return FV_OK;
@@ -627,7 +643,7 @@ int fv_mlp_gradients_dev(fv_mlp *m, const float *X, const float *T, const float 
     if (rc) return rc;
     if (grad)
         FV_HIP_CHECK(ctx, hipMemcpyAsync(grad, m->grad, sizeof(float) * kParams,
-                                         hipMemcpyDeviceToDevice, ctx->stream));
+                                         hipMemcpyDeviceToDevice, mlp_stream(m)));
     return FV_OK;
 }
 
@@ -639,9 +655,9 @@ int fv_mlp_adam_dev(fv_mlp *m, const float *grad, float lr)
     m->step += 1;
     const float c1 = 1.0f - powf(b1, (float)m->step);
     const float c2 = 1.0f - powf(b2, (float)m->step);
-    TimedLaunch tl(ctx, "mlp_adam");
+    TimedLaunch tl(ctx, "mlp_adam", mlp_stream(m));
     hipLaunchKernelGGL(k_mlp_adam, dim3((unsigned)ceil_div(kParams, 256)), dim3(256), 0,
-                       ctx->stream, m->params, grad ? grad : m->grad, m->m1, m->m2, kParams, lr,
+                       mlp_stream(m), m->params, grad ? grad : m->grad, m->m1, m->m2, kParams, lr,
                        b1, b2, eps, c1, c2, m->wbf, m->wtbf);
     FV_LAUNCH_CHECK(ctx);
     tl.done();

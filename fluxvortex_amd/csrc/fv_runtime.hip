@@ -76,11 +76,12 @@ static hipEvent_t pooled_event(fv_ctx *ctx)
     return e;
 }
 
-TimedLaunch::TimedLaunch(fv_ctx *c, const char *fam) : ctx(c), family(fam)
+TimedLaunch::TimedLaunch(fv_ctx *c, const char *fam, hipStream_t s)
+    : ctx(c), family(fam), stream(s ? s : c->stream)
 {
     if (ctx->timing) {
         start = pooled_event(ctx);
-        if (start) hipEventRecord(start, ctx->stream);
+        if (start) (void)hipEventRecord(start, stream);
     }
 }
 
@@ -89,7 +90,7 @@ void TimedLaunch::done()
     if (!ctx->timing || !start) return;
     hipEvent_t stop = pooled_event(ctx);
     if (!stop) return;
-    (void)hipEventRecord(stop, ctx->stream);
+    (void)hipEventRecord(stop, stream);
     ctx->timed.push_back(fv_timed_launch{family, start, stop});
     start = nullptr;
 }
@@ -142,17 +143,17 @@ void fv_ctx_destroy(fv_ctx *ctx)
 {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
-    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto &w : ctx->ws)
-        if (w.ptr) hipFree(w.ptr);
+        if (w.ptr) (void)hipFree(w.ptr);
     for (auto &t : ctx->timed) {
         (void)hipEventDestroy(t.start);
         (void)hipEventDestroy(t.stop);
     }
-    for (auto e : ctx->event_pool) hipEventDestroy(e);
+    for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
     for (auto e : ctx->user_events)
-        if (e) hipEventDestroy(e);
-    if (ctx->stream) hipStreamDestroy(ctx->stream);
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
 
